@@ -1,0 +1,257 @@
+"""agama_amd: MI355X-native Multipole potential evaluation and batched DOP853 orbit integration.
+
+Python mirror of the reference's interface for this path (Agama's Python extension,
+src/interface_python.cpp): `Potential` (type='Multipole' from a coefficient file or from a
+density), its `potential`, `force` and `eval` methods, `Density` descriptions for the
+construction, and `orbit(potential, ic, time, timestart, trajsize, accuracy, maxNumSteps,
+Omega, dtype, separateTime)`.  All evaluation and integration runs in libagama_amd.so on
+the GPU; there is no CPU fallback.
+"""
+import numpy as np
+
+from . import _native
+from ._native import AgamaAmdError, check, lib, ptr
+
+__all__ = ["Potential", "Density", "orbit", "AgamaAmdError", "PRECISION_FP64", "PRECISION_FP32"]
+
+PRECISION_FP64 = 0
+PRECISION_FP32 = 1
+
+# keyword spellings accepted by the reference (case-insensitive) and used by the library
+_KEYS = {k.lower(): k for k in ["type", "file", "density", "lmax", "mmax", "gridSizeR", "rmin", "rmax",
+                                "symmetry", "mass", "scaleRadius", "gamma", "axisRatioY", "axisRatioZ",
+                                "scaleHeight", "surfaceDensity"]}
+
+
+def _fmt(v):
+    if isinstance(v, (float, np.floating)):
+        return repr(float(v))
+    return str(v)
+
+
+class Density:
+    """Description of a density profile used to construct a Multipole expansion, mirroring
+    agama.Density(type=..., **params) and the composite form agama.Density(d1, d2, ...).
+    Supported types: Dehnen, Hernquist, Ferrers, Disk (exponential, scaleHeight>0), NFW,
+    Plummer."""
+
+    def __init__(self, *components, **params):
+        if components:
+            if params:
+                raise TypeError("either components or parameters, not both")
+            self.components = []
+            for c in components:
+                if isinstance(c, dict):
+                    c = Density(**c)
+                if not isinstance(c, Density):
+                    raise TypeError("components must be Density objects or dicts")
+                self.components.extend(c.components)
+        else:
+            self.components = [{k.lower(): v for k, v in params.items()}]
+            if "type" not in self.components[0]:
+                raise TypeError("Density requires a type")
+
+    def _params(self):
+        if len(self.components) == 1:
+            c = dict(self.components[0])
+            items = [("density", c.pop("type"))] + list(c.items())
+            return items
+        items = [("density", "+".join(str(c["type"]) for c in self.components))]
+        for i, c in enumerate(self.components):
+            items += [(f"{i}.{k}", v) for k, v in c.items() if k != "type"]
+        return items
+
+
+class Potential:
+    """Multipole potential on the GPU.
+
+    Potential(type='Multipole', file='coefs.ini')
+    Potential(type='Multipole', density='Dehnen', gamma=1, axisRatioY=0.8, axisRatioZ=0.6,
+              lmax=8, gridSizeR=50, rmin=1e-3, rmax=1e3)
+    Potential(type='Multipole', density=Density(...), lmax=..., ...)
+    Potential(coefs=(radii, Phi, dPhi))   # arrays of shape [nr], [(lmax+1)^2, nr] x 2
+
+    A string density name is expanded from the density profile by the Poisson solver
+    (the reference instead expands the analytic potential of a named model).
+    """
+
+    def __init__(self, *args, **kwargs):
+        self._h = None
+        if len(args) == 1 and isinstance(args[0], str) and not kwargs:
+            kwargs = {"file": args[0]}
+        elif args:
+            raise TypeError("Potential accepts only named arguments (or a single file name)")
+        coefs = kwargs.pop("coefs", None)
+        if coefs is not None:
+            radii, Phi, dPhi = (np.ascontiguousarray(a, dtype=np.float64) for a in coefs)
+            nc = Phi.shape[0]
+            lmax = int(round(np.sqrt(nc))) - 1
+            h = lib.agx_potential_from_coefs(len(radii), lmax, ptr(radii), ptr(Phi), ptr(dPhi))
+        else:
+            items = []
+            for k, v in kwargs.items():
+                kl = k.lower()
+                if kl == "density" and isinstance(v, Density):
+                    items += v._params()
+                elif kl in _KEYS:
+                    items.append((_KEYS[kl], v))
+                else:
+                    raise TypeError(f"unknown argument '{k}'")
+            if not any(k == "type" for k, _ in items):
+                items.insert(0, ("type", "Multipole"))
+            text = ", ".join(f"{k}={_fmt(v)}" for k, v in items)
+            h = lib.agx_potential_create(text.encode())
+        if not h:
+            raise AgamaAmdError("Potential construction failed: " + _native.last_error())
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and lib is not None:
+            lib.agx_potential_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def coefs(self):
+        """(radii[nr], Phi[(lmax+1)^2, nr], dPhi[...]) of the expansion"""
+        nr, lmax = np.zeros(1, np.int32), np.zeros(1, np.int32)
+        check(lib.agx_potential_coefs(self._h, nr.ctypes.data_as(_native._ip), lmax.ctypes.data_as(_native._ip),
+                                      None, None, None), "coefs")
+        nr, lmax = int(nr[0]), int(lmax[0])
+        r = np.empty(nr)
+        P = np.empty(((lmax + 1) ** 2, nr))
+        dP = np.empty_like(P)
+        check(lib.agx_potential_coefs(self._h, None, None, ptr(r), ptr(P), ptr(dP)), "coefs")
+        return r, P, dP
+
+    def export(self, filename):
+        """write the coefficients in the reference's text format"""
+        check(lib.agx_potential_write(self._h, str(filename).encode()), "export")
+
+    def _eval(self, points, want_pot, want_grad, t=0.0, precision=PRECISION_FP64):
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        single = pts.ndim == 1
+        pts2 = pts.reshape(-1, 3)
+        n = pts2.shape[0]
+        phi = np.empty(n) if want_pot else None
+        grad = np.empty((n, 3)) if want_grad else None
+        check(lib.agx_eval_many(self._h, n, ptr(pts2), float(t), ptr(phi), ptr(grad), int(precision)),
+              "eval")
+        if single:
+            phi = phi[0] if phi is not None else None
+            grad = grad[0] if grad is not None else None
+        return phi, grad
+
+    def potential(self, points, t=0.0, precision=PRECISION_FP64):
+        """Phi at points (N x 3 or 3) - reference: Potential.potential"""
+        return self._eval(points, True, False, t, precision)[0]
+
+    def force(self, points, t=0.0, precision=PRECISION_FP64):
+        """acceleration -grad(Phi) - reference: Potential.force"""
+        return -self._eval(points, False, True, t, precision)[1]
+
+    def eval(self, points, pot=False, acc=False, der=False, t=0.0, precision=PRECISION_FP64):
+        """reference: Potential.eval(points, pot=, acc=, der=); der (Hessian) is not provided"""
+        if der:
+            raise AgamaAmdError("second derivatives (der=True) are not provided by this version")
+        phi, grad = self._eval(points, pot, acc, t, precision)
+        out = []
+        if pot:
+            out.append(phi)
+        if acc:
+            out.append(-grad)
+        return out[0] if len(out) == 1 else tuple(out)
+
+
+def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, accuracy=1e-8,
+          maxNumSteps=None, Omega=0.0, dtype="float32", separateTime=False, method="dop853",
+          verbose=False, info=None, precision=PRECISION_FP64):
+    """Integrate orbits with DOP853 on the GPU - reference: agama.orbit
+    (src/interface_python.cpp:7689-8281, output conventions :7941-7961).
+
+    Returns the trajectory output as in the reference: with separateTime=True and equal
+    trajsize a tuple (times[N,S], traj[N,S,6]); with separateTime=False an (N,2) object array
+    of [times, traj] pairs; a single orbit (ic of length 6) drops the leading dimension.
+    `info`, if a dict, receives per-orbit diagnostics (nsteps, nattempts, nevals, maxde, status,
+    final) that the reference does not return."""
+    if not isinstance(potential, Potential):
+        raise TypeError("Argument 'potential' must be a valid Potential object")
+    if str(method).lower() != "dop853":
+        raise AgamaAmdError("only method='dop853' is provided by this version")
+    icarr = np.ascontiguousarray(ic, dtype=np.float64)
+    if not (icarr.shape == (6,) or (icarr.ndim == 2 and icarr.shape[1] == 6)):
+        raise TypeError("Argument 'ic' does not contain a valid array of length Nx6")
+    single = icarr.ndim == 1
+    ic2 = icarr.reshape(-1, 6)
+    n = ic2.shape[0]
+    if time is None:
+        raise ValueError("Argument 'time' must be provided")
+    tt = np.asarray(time, dtype=np.float64)
+    if tt.ndim == 0:
+        tt = np.full(n, float(tt))
+    elif tt.shape != (n,):
+        raise ValueError("Argument 'time' must either be a scalar or have the same length as the number of points in the initial conditions")
+    ts = None
+    if timestart is not None:
+        ts = np.asarray(timestart, dtype=np.float64)
+        ts = np.full(n, float(ts)) if ts.ndim == 0 else np.ascontiguousarray(ts)
+        if ts.shape != (n,):
+            raise ValueError("Argument 'timestart' must either be a scalar or have the same length as the number of points in the initial conditions")
+    have_traj = trajsize is not None
+    if have_traj:
+        sz = np.asarray(trajsize)
+        sz = np.full(n, int(sz), dtype=np.int32) if sz.ndim == 0 else np.ascontiguousarray(sz, dtype=np.int32)
+        if sz.shape != (n,) or np.any(sz < 0):
+            raise ValueError("Argument 'trajsize', if provided, must either be a nonnegative integer or an array of integers with the same length as the number of points in the initial conditions")
+        if np.any(sz == 0):
+            raise AgamaAmdError("trajsize=0 (every timestep) is not provided by this version")
+    else:
+        sz = np.ones(n, dtype=np.int32)
+    if dtype in ("float32", np.float32, "float", float, "float64", np.float64, "double"):
+        out_dtype = np.float32 if dtype in ("float32", np.float32) else np.float64
+    elif dtype in ("complex64", "complex128", "complex", np.complex64, np.complex128):
+        out_dtype = np.complex64 if dtype in ("complex64", np.complex64) else np.complex128
+    else:
+        raise AgamaAmdError(f"dtype={dtype} is not provided by this version")
+    maxsteps = int(1e8) if maxNumSteps is None else int(maxNumSteps)
+    rows = int(sz.sum())
+    times = np.empty(rows)
+    traj = np.empty((rows, 6))
+    final = np.empty((n, 6))
+    nsteps = np.empty(n, dtype=np.int64)
+    natt = np.empty(n, dtype=np.int64)
+    nev = np.empty(n, dtype=np.int64)
+    maxde = np.empty(n)
+    status = np.empty(n, dtype=np.int32)
+    check(lib.agx_orbit_batch(potential.handle, n, ptr(ic2), ptr(ts), ptr(tt), ptr(sz), float(accuracy),
+                              float(Omega), maxsteps, ptr(times), ptr(traj), ptr(final), ptr(nsteps),
+                              ptr(natt), ptr(nev), ptr(maxde), ptr(status), int(precision)), "orbit")
+    if isinstance(info, dict):
+        info.update(nsteps=nsteps, nattempts=natt, nevals=nev, maxde=maxde, status=status, final=final)
+    if not have_traj:
+        return final[0] if single else final
+
+    def conv(a):
+        if out_dtype in (np.complex64, np.complex128):
+            return (a[..., :3] + 1j * a[..., 3:]).astype(out_dtype)
+        return a.astype(out_dtype)
+
+    offs = np.concatenate([[0], np.cumsum(sz)])
+    if separateTime and np.all(sz == sz[0]):
+        S = int(sz[0])
+        t_out = times.reshape(n, S)
+        x_out = conv(traj.reshape(n, S, 6))
+        return (t_out[0], x_out[0]) if single else (t_out, x_out)
+    parts = [(times[offs[i]:offs[i + 1]], conv(traj[offs[i]:offs[i + 1]])) for i in range(n)]
+    if separateTime:
+        tarr = np.empty(n, dtype=object)
+        xarr = np.empty(n, dtype=object)
+        for i, (a, b) in enumerate(parts):
+            tarr[i], xarr[i] = a, b
+        return (tarr[0], xarr[0]) if single else (tarr, xarr)
+    res = np.empty((n, 2), dtype=object)
+    for i, (a, b) in enumerate(parts):
+        res[i, 0], res[i, 1] = a, b
+    return res[0] if single else res

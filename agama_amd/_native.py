@@ -1,0 +1,82 @@
+"""ctypes binding of libagama_amd.so (the C ABI declared in include/agama_amd.h).
+
+The library is built in-tree by agama_amd.build; importing this module loads it and fails
+loudly when it is missing - there is no CPU fallback for the product path."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libagama_amd.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+_vp = ctypes.c_void_p
+
+
+class AgamaAmdError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"agama_amd native library not found at {LIB_PATH}; build it with "
+            "`python agama_amd/build.py` (hipcc for gfx950)")
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "agx_last_error": (ctypes.c_char_p, []),
+        "agx_potential_create": (_vp, [ctypes.c_char_p]),
+        "agx_potential_from_coefs": (_vp, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
+        "agx_potential_destroy": (None, [_vp]),
+        "agx_potential_write": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+        "agx_build_coef_file": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p]),
+        "agx_debug_tables": (ctypes.c_long, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _vp]),
+        "agx_potential_coefs": (ctypes.c_int, [_vp, _ip, _ip, _vp, _vp, _vp]),
+        "agx_eval_potential": (ctypes.c_double, [_vp, _vp, ctypes.c_double, _vp, _vp]),
+        "agx_eval_many": (ctypes.c_int, [_vp, ctypes.c_long, _vp, ctypes.c_double, _vp, _vp, ctypes.c_int]),
+        "agx_eval_many_device": (ctypes.c_int, [_vp, ctypes.c_long, _vp, ctypes.c_double, _vp, _vp,
+                                                ctypes.c_int, _vp]),
+        "agx_orbit_batch": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp,
+                                           _vp, _vp, ctypes.c_int]),
+        "agx_orbit_batch_device": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                  ctypes.c_double, ctypes.c_double, ctypes.c_long,
+                                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+        "agx_set_orbit_launch": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+        "agama_getError": (ctypes.c_char_p, []),
+        "agama_createPotential": (_vp, [ctypes.c_char_p]),
+        "agama_deletePotential": (None, [_vp]),
+        "agama_evalPotential": (ctypes.c_double, [_vp, _vp, ctypes.c_double, _vp, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+EXPORTED = ["agx_last_error", "agx_potential_create", "agx_potential_from_coefs",
+            "agx_potential_destroy", "agx_potential_write", "agx_build_coef_file",
+            "agx_debug_tables", "agx_potential_coefs",
+            "agx_eval_potential", "agx_eval_many", "agx_eval_many_device", "agx_orbit_batch",
+            "agx_orbit_batch_device", "agx_set_orbit_launch", "iniacceleration",
+            "agama_getError", "agama_createPotential", "agama_deletePotential",
+            "agama_evalPotential"]
+
+
+def last_error():
+    msg = lib.agx_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc, what):
+    if rc != 0:
+        raise AgamaAmdError(f"{what} failed (code {rc}): {last_error()}")
+
+
+def ptr(a):
+    """data pointer of a numpy array (or None)"""
+    return None if a is None else a.ctypes.data

@@ -1,0 +1,43 @@
+"""Build libagama_amd.so in-tree: host construction code with g++, HIP kernels and the C ABI
+with hipcc for gfx950.  Usage: python agama_amd/build.py  (runs without importing the package)."""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libagama_amd.so")
+BUILD = os.path.join(HERE, "_build")
+ARCH = os.environ.get("AGX_OFFLOAD_ARCH", "gfx950")
+
+HOST_FLAGS = ["-std=c++17", "-O2", "-march=x86-64-v3", "-ffp-contract=off", "-fPIC", "-Wall"]
+HIP_FLAGS = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-result"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build(verbose_resources=False):
+    os.makedirs(BUILD, exist_ok=True)
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    objs = []
+    o = os.path.join(BUILD, "host_multipole.o")
+    _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, "host_multipole.cpp"), "-o", o])
+    objs.append(o)
+    for src in ("agx_kernels.hip", "agx_capi.hip"):
+        o = os.path.join(BUILD, src.replace(".hip", ".o"))
+        extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources and src == "agx_kernels.hip" else []
+        _run([hipcc] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
+    tmp = OUT + ".tmp"
+    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose_resources="-v" in sys.argv)

@@ -1,0 +1,475 @@
+// C ABI of libagama_amd (declared in include/agama_amd.h): potential handles, table upload,
+// batched evaluation and orbit integration launchers, the NEMO plugin entry point and the
+// drop-in aliases of the reference's plain C interface.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "../../include/agama_amd.h"
+#include "agx_kernels.h"
+#include "host_multipole.h"
+
+hipError_t agxLaunchEval(const AgxMultipoleDev& P, int precision, long n, const double* pos,
+    double* phi, double* grad, hipStream_t stream);
+hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, int precision, const AgxOrbitArgs& A,
+    int numBlocks, int blockSize, hipStream_t stream);
+int agxOrbitKernelBlocksPerCU(int precision, int blockSize);
+
+struct agx_potential {
+    agx::Coefs coefs;
+    agx::HostTables T;
+    AgxMultipoleDev dev;
+    void* dmem = nullptr;
+    unsigned int* dcounter = nullptr;
+    int device = 0;
+    ~agx_potential() { if(dmem) (void)hipFree(dmem); }
+};
+
+namespace {
+
+thread_local std::string g_err;
+int g_orbitBlocks = 0, g_orbitBlockSize = 0;
+
+int fail(int code, const std::string& msg) { g_err = msg; return code; }
+
+struct HipError : std::runtime_error {
+    explicit HipError(const std::string& s) : std::runtime_error(s) {}
+};
+#define HIPCHECK(expr) do { hipError_t e_ = (expr); if(e_ != hipSuccess) \
+    throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_)); } while(0)
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// upload the host tables into one device allocation and fill the kernel-argument header
+void upload(agx_potential& p)
+{
+    const agx::HostTables& T = p.T;
+    const size_t nc = p.coefs.Phi.size();
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
+    const size_t oR = take(T.gridR.size() * 8), oT = take(T.gridT.size() * 8),
+        oN = take(T.nodes.size() * 8), oF = take(T.nodes.size() * 4),
+        oIS = take(nc * 8), oIU = take(nc * 8), oIW = take(nc * 8),
+        oOS = take(nc * 8), oOU = take(nc * 8), oOW = take(nc * 8), oC = take(64);
+    HIPCHECK(hipGetDevice(&p.device));
+    HIPCHECK(hipMalloc(&p.dmem, off));
+    char* base = static_cast<char*>(p.dmem);
+    std::vector<float> nf(T.nodes.begin(), T.nodes.end());
+    auto put = [&](size_t o, const void* src, size_t bytes) {
+        if(bytes) HIPCHECK(hipMemcpy(base + o, src, bytes, hipMemcpyHostToDevice));
+    };
+    put(oR, T.gridR.data(), T.gridR.size() * 8);
+    put(oT, T.gridT.data(), T.gridT.size() * 8);
+    put(oN, T.nodes.data(), T.nodes.size() * 8);
+    put(oF, nf.data(), nf.size() * 4);
+    put(oIS, T.inner.S.data(), nc * 8); put(oIU, T.inner.U.data(), nc * 8); put(oIW, T.inner.W.data(), nc * 8);
+    put(oOS, T.outer.S.data(), nc * 8); put(oOU, T.outer.U.data(), nc * 8); put(oOW, T.outer.W.data(), nc * 8);
+    HIPCHECK(hipMemset(base + oC, 0, 64));
+    AgxMultipoleDev& D = p.dev;
+    memset(&D, 0, sizeof(D));
+    D.nr = T.nr; D.nt = T.nt; D.ns = T.ns; D.mmax = T.mmax; D.mmin = T.mmin;
+    D.logScaling = T.logScaling; D.invPhi0 = T.invPhi0;
+    D.rsqInner = T.rsqInner; D.rsqOuter = T.rsqOuter;
+    for(int s = 0; s < T.ns; s++) D.slotM[s] = T.slotM[s];
+    D.zeroSlot = T.zeroSlot;
+    D.gridR = reinterpret_cast<const double*>(base + oR);
+    D.gridT = reinterpret_cast<const double*>(base + oT);
+    D.nodes = reinterpret_cast<const double*>(base + oN);
+    D.nodesF = reinterpret_cast<const float*>(base + oF);
+    auto fillPL = [&](AgxPowerLawDev& L, const agx::HostPowerLaw& H, size_t s, size_t u, size_t w) {
+        L.r0sq = H.r0sq; L.Q = H.Q; L.inner = H.inner; L.lmax = H.lmax; L.mmax = H.mmax;
+        L.mmin = H.mmin; L.step = H.step;
+        for(int i = 0; i < AGX_MAX_SLOTS; i++) L.lmin[i] = i < (int)H.lmin.size() ? H.lmin[i] : 1 << 20;
+        L.S = reinterpret_cast<const double*>(base + s);
+        L.U = reinterpret_cast<const double*>(base + u);
+        L.W = reinterpret_cast<const double*>(base + w);
+    };
+    fillPL(D.inner, T.inner, oIS, oIU, oIW);
+    fillPL(D.outer, T.outer, oOS, oOU, oOW);
+    p.dcounter = reinterpret_cast<unsigned int*>(base + oC);
+}
+
+agx_potential* makeFromCoefs(agx::Coefs&& c)
+{
+    std::unique_ptr<agx_potential> p(new agx_potential);
+    p->coefs = std::move(c);
+    p->T = agx::buildTables(p->coefs);
+    upload(*p);
+    return p.release();
+}
+
+// "type=Multipole, file=..." or "type=Multipole, density=..., lmax=..., gridSizeR=..., ..."
+agx_potential* createFromParams(const std::string& text)
+{
+    agx::Params P = agx::Params::parse(text);
+    std::string type = P.str("type", "Multipole");
+    std::string tl = type;
+    for(auto& ch : tl) ch = (char)tolower(ch);
+    if(tl != "multipole")
+        throw std::invalid_argument("only type=Multipole is provided by this library (got " + type + ")");
+    if(P.has("file")) {
+        P.checkOnly({"type", "file"});
+        return makeFromCoefs(agx::readCoefFile(P.str("file")));
+    }
+    if(!P.has("density"))
+        throw std::invalid_argument("Multipole needs either file=... or density=...");
+    P.checkOnly({"type", "density", "lmax", "mmax", "gridsizer", "rmin", "rmax", "symmetry",
+        "mass", "scaleradius", "gamma", "axisratioy", "axisratioz", "scaleheight", "surfacedensity",
+        "0.*", "1.*", "2.*", "3.*", "4.*", "5.*"});
+    auto dens = agx::makeDensity(P);
+    const int lmax = (int)P.num("lmax", 6);
+    const int mmax = (int)P.num("mmax", lmax);
+    const int nr = (int)P.num("gridsizer", 25);
+    const int sym = agx::symmetryFromName(P.str("symmetry"));
+    return makeFromCoefs(agx::multipoleFromDensity(*dens, lmax, mmax, sym, nr,
+        P.num("rmin", 0), P.num("rmax", 0)));
+}
+
+int orbitGeometry(long n, int precision, int& blocks, int& bs)
+{
+    bs = g_orbitBlockSize > 0 ? g_orbitBlockSize : 256;
+    if(g_orbitBlocks > 0) blocks = g_orbitBlocks;
+    else {
+        int dev = 0, cus = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const int perCU = std::max(1, agxOrbitKernelBlocksPerCU(precision, bs));
+        blocks = cus * perCU;
+    }
+    const long need = (n + bs - 1) / bs;
+    if(need < blocks) blocks = (int)std::max<long>(need, 1);
+    return 0;
+}
+
+template<typename T>
+struct DevBuf {
+    T* p = nullptr;
+    explicit DevBuf(size_t n) { if(n) HIPCHECK(hipMalloc(&p, n * sizeof(T))); }
+    ~DevBuf() { if(p) (void)hipFree(p); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* agx_last_error(void) { return g_err.c_str(); }
+
+agx_potential* agx_potential_create(const char* params)
+{
+    g_err.clear();
+    if(!params) { fail(AGX_EINVAL, "params is NULL"); return nullptr; }
+    try { return createFromParams(params); }
+    catch(std::exception& e) { fail(AGX_EINVAL, e.what()); return nullptr; }
+}
+
+agx_potential* agx_potential_from_coefs(int nr, int lmax, const double* radii,
+    const double* phi, const double* dphi)
+{
+    g_err.clear();
+    try {
+        if(nr < 2 || lmax < 0 || !radii || !phi || !dphi) throw std::invalid_argument("invalid coefficient arrays");
+        agx::Coefs c;
+        c.radii.assign(radii, radii + nr);
+        const int nc = (lmax + 1) * (lmax + 1);
+        c.Phi.resize(nc); c.dPhi.resize(nc);
+        for(int i = 0; i < nc; i++) {
+            c.Phi[i].assign(phi + (size_t)i * nr, phi + (size_t)(i + 1) * nr);
+            c.dPhi[i].assign(dphi + (size_t)i * nr, dphi + (size_t)(i + 1) * nr);
+        }
+        return makeFromCoefs(std::move(c));
+    } catch(std::exception& e) { fail(AGX_EINVAL, e.what()); return nullptr; }
+}
+
+void agx_potential_destroy(agx_potential* pot) { delete pot; }
+
+int agx_build_coef_file(const char* params, const char* path)
+{
+    g_err.clear();
+    if(!params || !path) return fail(AGX_EINVAL, "NULL argument");
+    try {
+        agx::Params P = agx::Params::parse(params);
+        if(!P.has("density")) throw std::invalid_argument("density=... is required");
+        P.checkOnly({"type", "density", "lmax", "mmax", "gridsizer", "rmin", "rmax", "symmetry",
+            "mass", "scaleradius", "gamma", "axisratioy", "axisratioz", "scaleheight", "surfacedensity",
+            "0.*", "1.*", "2.*", "3.*", "4.*", "5.*"});
+        auto dens = agx::makeDensity(P);
+        const int lmax = (int)P.num("lmax", 6);
+        agx::Coefs c = agx::multipoleFromDensity(*dens, lmax, (int)P.num("mmax", lmax),
+            agx::symmetryFromName(P.str("symmetry")), (int)P.num("gridsizer", 25),
+            P.num("rmin", 0), P.num("rmax", 0));
+        agx::writeCoefFile(path, c);
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EINVAL, e.what()); }
+}
+
+long agx_debug_tables(const char* coef_path, int which, int m, double* out)
+{
+    g_err.clear();
+    try {
+        agx::HostTables T = agx::buildTables(agx::readCoefFile(coef_path));
+        std::vector<double> v;
+        if(which == 0) v = T.gridR;
+        else if(which == 1) v = T.gridT;
+        else if(which == 2) {
+            auto it = std::find(T.slotM.begin(), T.slotM.end(), m);
+            if(it == T.slotM.end()) return 0;
+            const int s = (int)(it - T.slotM.begin());
+            const size_t n2 = (size_t)T.nr * T.nt;
+            v.resize(9 * n2);
+            for(int q = 0; q < 9; q++)
+                for(size_t i = 0; i < n2; i++)
+                    v[q * n2 + i] = T.nodes[(i * T.ns + s) * AGX_NODE_STRIDE + q];
+        } else if(which == 3)
+            v = { (double)T.logScaling, T.invPhi0, (double)T.lmax, (double)T.mmax, (double)T.mmin, (double)T.step };
+        else return -1;
+        if(out) std::copy(v.begin(), v.end(), out);
+        return (long)v.size();
+    } catch(std::exception& e) { fail(AGX_EINVAL, e.what()); return -1; }
+}
+
+int agx_potential_write(const agx_potential* pot, const char* path)
+{
+    g_err.clear();
+    if(!pot || !path) return fail(AGX_EINVAL, "NULL argument");
+    try { agx::writeCoefFile(path, pot->coefs); return AGX_OK; }
+    catch(std::exception& e) { return fail(AGX_EINVAL, e.what()); }
+}
+
+int agx_potential_coefs(const agx_potential* pot, int* nr, int* lmax, double* radii,
+    double* phi, double* dphi)
+{
+    g_err.clear();
+    if(!pot) return fail(AGX_EINVAL, "NULL potential");
+    const int n = (int)pot->coefs.radii.size(), L = pot->coefs.lmax();
+    if(nr) *nr = n;
+    if(lmax) *lmax = L;
+    if(radii) std::copy(pot->coefs.radii.begin(), pot->coefs.radii.end(), radii);
+    for(size_t c = 0; c < pot->coefs.Phi.size(); c++) {
+        if(phi) std::copy(pot->coefs.Phi[c].begin(), pot->coefs.Phi[c].end(), phi + c * n);
+        if(dphi) std::copy(pot->coefs.dPhi[c].begin(), pot->coefs.dPhi[c].end(), dphi + c * n);
+    }
+    return AGX_OK;
+}
+
+int agx_eval_many_device(const agx_potential* pot, long n, const double* d_pos, double t,
+    double* d_phi, double* d_grad, int flags, void* stream)
+{
+    (void)t;   // the Multipole potential is static
+    if(!pot) return fail(AGX_EINVAL, "NULL potential");
+    if(n < 0 || (n > 0 && !d_pos)) return fail(AGX_EINVAL, "invalid point array");
+    if(flags != AGX_PRECISION_FP64 && flags != AGX_PRECISION_FP32) return fail(AGX_EINVAL, "invalid flags");
+    hipError_t e = agxLaunchEval(pot->dev, flags, n, d_pos, d_phi, d_grad, (hipStream_t)stream);
+    if(e != hipSuccess) return fail(AGX_EDEVICE, std::string("eval launch: ") + hipGetErrorString(e));
+    return AGX_OK;
+}
+
+int agx_eval_many(const agx_potential* pot, long n, const double* pos, double t,
+    double* phi, double* grad, int flags)
+{
+    g_err.clear();
+    if(!pot) return fail(AGX_EINVAL, "NULL potential");
+    if(n == 0) return AGX_OK;
+    if(n < 0 || !pos) return fail(AGX_EINVAL, "invalid point array");
+    try {
+        HIPCHECK(hipSetDevice(pot->device));
+        DevBuf<double> dp(n * 3), dphi(phi ? n : 0), dg(grad ? n * 3 : 0);
+        HIPCHECK(hipMemcpy(dp.p, pos, n * 3 * 8, hipMemcpyHostToDevice));
+        int rc = agx_eval_many_device(pot, n, dp.p, t, dphi.p, dg.p, flags, nullptr);
+        if(rc) return rc;
+        HIPCHECK(hipDeviceSynchronize());
+        if(phi) HIPCHECK(hipMemcpy(phi, dphi.p, n * 8, hipMemcpyDeviceToHost));
+        if(grad) HIPCHECK(hipMemcpy(grad, dg.p, n * 3 * 8, hipMemcpyDeviceToHost));
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+double agx_eval_potential(const agx_potential* pot, const double pos[3], double t,
+    double deriv[3], double deriv2[6])
+{
+    if(deriv2) {
+        fail(AGX_EUNSUPPORTED, "second derivatives are not provided by this version");
+        for(int i = 0; i < 6; i++) deriv2[i] = NAN;
+    }
+    double phi = NAN, g[3];
+    if(agx_eval_many(pot, 1, pos, t, &phi, g, AGX_PRECISION_FP64) != AGX_OK) return NAN;
+    if(deriv) for(int i = 0; i < 3; i++) deriv[i] = g[i];
+    return phi;
+}
+
+int agx_set_orbit_launch(int num_blocks, int block_size)
+{
+    if(num_blocks < 0 || block_size < 0 || (block_size && (block_size % 64 || block_size > 256)))
+        return fail(AGX_EINVAL, "block size must be a multiple of 64 up to 256");
+    g_orbitBlocks = num_blocks;
+    g_orbitBlockSize = block_size;
+    return AGX_OK;
+}
+
+int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
+    const double* d_tstart, const double* d_ttotal, const int* d_trajsize,
+    const long* d_traj_offset, const int* d_order, double accuracy, double Omega,
+    long maxNumSteps, double* d_time, double* d_traj, double* d_final, long* d_nsteps,
+    long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags, void* stream)
+{
+    if(!pot) return fail(AGX_EINVAL, "NULL potential");
+    if(n == 0) return AGX_OK;
+    if(n < 0 || n > 0x7fffffffL || !d_ic || !d_ttotal || !d_trajsize)
+        return fail(AGX_EINVAL, "invalid orbit arrays");
+    if((d_time || d_traj) && !d_traj_offset) return fail(AGX_EINVAL, "trajectory output needs offsets");
+    if(!(accuracy > 0)) return fail(AGX_EINVAL, "accuracy must be positive");
+    if(!(Omega > -INFINITY && Omega < INFINITY)) return fail(AGX_EINVAL, "Omega must be finite");
+    if(maxNumSteps <= 0) return fail(AGX_EINVAL, "maxNumSteps must be positive");
+    if(flags != AGX_PRECISION_FP64 && flags != AGX_PRECISION_FP32) return fail(AGX_EINVAL, "invalid flags");
+    try {
+        AgxOrbitArgs A;
+        A.n = n; A.ic = d_ic; A.tstart = d_tstart; A.ttotal = d_ttotal; A.trajsize = d_trajsize;
+        A.trajOffset = d_traj_offset; A.order = d_order; A.accuracy = accuracy; A.Omega = Omega;
+        A.maxNumSteps = maxNumSteps; A.outTime = d_time; A.outTraj = d_traj; A.outFinal = d_final;
+        A.outNsteps = d_nsteps; A.outNattempts = d_nattempts; A.outNevals = d_nevals;
+        A.outMaxde = d_maxde; A.outStatus = d_status; A.counter = pot->dcounter;
+        int blocks, bs;
+        orbitGeometry(n, flags, blocks, bs);
+        HIPCHECK(hipMemsetAsync(pot->dcounter, 0, sizeof(unsigned int), (hipStream_t)stream));
+        hipError_t e = agxLaunchOrbit(pot->dev, flags, A, blocks, bs, (hipStream_t)stream);
+        if(e != hipSuccess) return fail(AGX_EDEVICE, std::string("orbit launch: ") + hipGetErrorString(e));
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+int agx_orbit_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, double accuracy, double Omega, long maxNumSteps,
+    double* out_time, double* out_traj, double* out_final, long* out_nsteps,
+    long* out_nattempts, long* out_nevals, double* out_maxde, int* out_status, int flags)
+{
+    g_err.clear();
+    if(!pot) return fail(AGX_EINVAL, "NULL potential");
+    if(n == 0) return AGX_OK;
+    if(n < 0 || !ic || !ttotal || !trajsize) return fail(AGX_EINVAL, "invalid orbit arrays");
+    std::vector<long> offs(n);
+    long rows = 0;
+    for(long i = 0; i < n; i++) {
+        if(trajsize[i] < 1) return fail(AGX_EUNSUPPORTED, "trajsize must be >= 1 (recording every "
+            "timestep, trajsize=0, is not provided by the batch interface)");
+        offs[i] = rows;
+        rows += trajsize[i];
+    }
+    try {
+        HIPCHECK(hipSetDevice(pot->device));
+        DevBuf<double> dic(n * 6), dts(tstart ? n : 0), dtt(n), dtime(out_time ? rows : 0),
+            dtraj(out_traj ? rows * 6 : 0), dfin(out_final ? n * 6 : 0), dmde(out_maxde ? n : 0);
+        DevBuf<int> dsz(n), dst(out_status ? n : 0);
+        DevBuf<long> doff(n), dns(out_nsteps ? n : 0), dna(out_nattempts ? n : 0), dne(out_nevals ? n : 0);
+        HIPCHECK(hipMemcpy(dic.p, ic, n * 48, hipMemcpyHostToDevice));
+        if(tstart) HIPCHECK(hipMemcpy(dts.p, tstart, n * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dtt.p, ttotal, n * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dsz.p, trajsize, n * 4, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(doff.p, offs.data(), n * 8, hipMemcpyHostToDevice));
+        int rc = agx_orbit_batch_device(pot, n, dic.p, dts.p, dtt.p, dsz.p, doff.p, nullptr,
+            accuracy, Omega, maxNumSteps, dtime.p, dtraj.p, dfin.p, dns.p, dna.p, dne.p, dmde.p,
+            dst.p, flags, nullptr);
+        if(rc) return rc;
+        HIPCHECK(hipDeviceSynchronize());
+        if(out_time) HIPCHECK(hipMemcpy(out_time, dtime.p, rows * 8, hipMemcpyDeviceToHost));
+        if(out_traj) HIPCHECK(hipMemcpy(out_traj, dtraj.p, rows * 48, hipMemcpyDeviceToHost));
+        if(out_final) HIPCHECK(hipMemcpy(out_final, dfin.p, n * 48, hipMemcpyDeviceToHost));
+        if(out_nsteps) HIPCHECK(hipMemcpy(out_nsteps, dns.p, n * 8, hipMemcpyDeviceToHost));
+        if(out_nattempts) HIPCHECK(hipMemcpy(out_nattempts, dna.p, n * 8, hipMemcpyDeviceToHost));
+        if(out_nevals) HIPCHECK(hipMemcpy(out_nevals, dne.p, n * 8, hipMemcpyDeviceToHost));
+        if(out_maxde) HIPCHECK(hipMemcpy(out_maxde, dmde.p, n * 8, hipMemcpyDeviceToHost));
+        if(out_status) HIPCHECK(hipMemcpy(out_status, dst.p, n * 4, hipMemcpyDeviceToHost));
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+// ---- drop-in aliases (src/interface_c.cpp:15-98) ----
+const char* agama_getError(void) { return agx_last_error(); }
+agama_Potential* agama_createPotential(const char* params) { return agx_potential_create(params); }
+void agama_deletePotential(agama_Potential* pot) { agx_potential_destroy(pot); }
+double agama_evalPotential(const agama_Potential* pot, const double pos[3], double time,
+    double deriv[3], double deriv2[6])
+{ return agx_eval_potential(pot, pos, time, deriv, deriv2); }
+
+}  // extern "C"
+
+// ---- NEMO acceleration plugin (src/interface_nemo.cpp:74-220) ----
+namespace {
+std::unique_ptr<agx_potential> g_nemoPot;
+double g_nemoOmega = 0;
+
+template<typename NumT>
+void nemoAcc(int ndim, double time, int nbody, const void* pos_, const int* active, void* pot_,
+    void* acc_, int flag)
+{
+    const NumT* pos = static_cast<const NumT*>(pos_);
+    NumT* pot = static_cast<NumT*>(pot_);
+    NumT* acc = static_cast<NumT*>(acc_);
+    double ca = 1, sa = 0;
+    if(g_nemoOmega != 0) { sa = sin(g_nemoOmega * time); ca = cos(g_nemoOmega * time); }
+    std::vector<long> idx;
+    std::vector<double> pts;
+    for(int i = 0; i < nbody; i++) {
+        if(active && (active[i] & 1) != 1) continue;
+        idx.push_back(i);
+        const double x = pos[i * ndim], y = pos[i * ndim + 1], z = ndim == 3 ? pos[i * ndim + 2] : 0;
+        pts.push_back(x * ca + y * sa);
+        pts.push_back(y * ca - x * sa);
+        pts.push_back(z);
+    }
+    std::vector<double> phi(idx.size()), g(idx.size() * 3);
+    if(idx.empty()) return;
+    if(agx_eval_many(g_nemoPot.get(), (long)idx.size(), pts.data(), time, phi.data(), g.data(),
+        AGX_PRECISION_FP64) != AGX_OK) {
+        fprintf(stderr, "agama_amd NEMO plugin: %s\n", agx_last_error());
+        return;
+    }
+    const bool addPot = flag & 1, addAcc = flag & 2;
+    for(size_t j = 0; j < idx.size(); j++) {
+        const long i = idx[j];
+        const NumT ax = (NumT)-(g[j * 3] * ca - g[j * 3 + 1] * sa),
+                   ay = (NumT)-(g[j * 3 + 1] * ca + g[j * 3] * sa), az = (NumT)-g[j * 3 + 2];
+        if(pot) { if(addPot) pot[i] += (NumT)phi[j]; else pot[i] = (NumT)phi[j]; }
+        if(addAcc) { acc[i * ndim] += ax; acc[i * ndim + 1] += ay; if(ndim == 3) acc[i * ndim + 2] += az; }
+        else { acc[i * ndim] = ax; acc[i * ndim + 1] = ay; if(ndim == 3) acc[i * ndim + 2] = az; }
+    }
+}
+
+void nemoAccel(int ndim, double t, int n, const void*, const void* pos, const void*,
+    const int* flags, void* pot, void* acc, int flag, char numtype)
+{
+    if(!g_nemoPot || (ndim != 2 && ndim != 3)) return;
+    if(numtype == 'f') nemoAcc<float>(ndim, t, n, pos, flags, pot, acc, flag);
+    else if(numtype == 'd') nemoAcc<double>(ndim, t, n, pos, flags, pot, acc, flag);
+}
+}  // namespace
+
+extern "C" void iniacceleration(const double* pars, int npar, const char* file,
+    agx_acc_pter* accel, bool* need_m, bool* need_v)
+{
+    if(!file || !file[0]) {
+        fprintf(stderr, "agama_amd NEMO plugin: should provide the coefficient file in accfile=...\n");
+        exit(1);
+    }
+    g_nemoOmega = npar >= 1 ? pars[0] : 0;
+    const double G = npar >= 2 ? pars[1] : 1;
+    if(G != 1) {
+        fprintf(stderr, "agama_amd NEMO plugin: G != 1 (unit conversion) is not supported\n");
+        exit(1);
+    }
+    agx_potential* p = agx_potential_create((std::string("type=Multipole, file=") + file).c_str());
+    if(!p) {
+        fprintf(stderr, "agama_amd NEMO plugin: %s\n", agx_last_error());
+        exit(1);
+    }
+    g_nemoPot.reset(p);
+    *accel = nemoAccel;
+    if(need_m) *need_m = false;
+    if(need_v) *need_v = false;
+}

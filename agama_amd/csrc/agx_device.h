@@ -1,0 +1,442 @@
+// Device functions for Multipole evaluation on gfx950 (FP64).
+//
+// Evaluation follows the reference routines (paths relative to its source tree):
+//   sincosA / atan2A        src/math_core.cpp:171-248
+//   binSearchD              src/math_core.cpp:250-280
+//   quintic Hermite passes  src/math_spline.cpp:653-718 (evalQuinticSplines)
+//   interior                src/potential_multipole.cpp:1969-2080 (MultipoleInterp2d::evalCyl)
+//                           with src/math_spline.cpp:2535-2597 (QuinticSpline2d::evalDeriv)
+//   power-law branches      src/potential_multipole.cpp:1600-1658, 722-848
+//   Fourier synthesis       src/potential_multipole.cpp:666-717, src/math_sphharm.cpp:134-152
+//   Car<->Cyl               src/coord.cpp:94-125, 607-614
+// Design points of the device version:
+//   - the cell search and the quintic weights depend only on the point, so they are computed
+//     once per evaluation and shared by all azimuthal harmonics (the reference recomputes the
+//     same values for each harmonic; the arithmetic is identical);
+//   - harmonics are streamed in ascending m with the cos/sin(m phi) recurrence advanced in the
+//     same loop, so no per-harmonic arrays live in registers (mmin = 0, every symmetry with
+//     y-reflection); expansions with sine terms (mmin < 0) take a generic path with small
+//     private arrays that keeps the reference's summation order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "agx_tables.h"
+
+namespace agxd {
+
+__device__ __forceinline__ double sqd(double x) { return x * x; }
+
+__device__ __forceinline__ void sincosA(double x, double& s, double& c)
+{
+    double y = fabs(x);
+    long quad = (long)(4 / M_PI * y);
+    quad = (quad + 1) >> 1;
+    y -= M_PI / 2 * quad;
+    int q13 = (int)(quad & 1), q02 = 1 - q13;
+    int signc = 1 - (int)(quad & 2), signs = signbit(x) ? -signc : signc;
+    double y2 = y * y;
+    double sy = y + y * (((((+1.5896230157654657e-10 * y2 - 2.5050747762857807e-8) * y2
+        + 2.7557313621385725e-6) * y2 - 1.9841269829589539e-4) * y2 + 8.3333333333221186e-3) * y2
+        - 0.1666666666666663073) * y2;
+    double cy = 1.0 + ((((((-1.1358536521387682e-11 * y2 + 2.0875700841974730e-9) * y2
+        - 2.7557314179296740e-7) * y2 + 2.4801587288851704e-5) * y2 - 0.0013888888888873056) * y2
+        + 0.0416666666666665950) * y2 - 0.5) * y2;
+    s = (q02 * sy + q13 * cy) * signs;
+    c = (q02 * cy - q13 * sy) * signc;
+}
+
+__device__ __forceinline__ double atan1A(double x)
+{
+    double x2 = x * x,
+    num = -403.11710541978266 + x2 * (-902.59989673314130 + x2 * (-707.35355100686270 +
+          x2 * (-230.47304101738868 + x2 * (-28.595211396994470 + x2 * -0.9023747369388881)))),
+    den = 1209.3513162593547 + x2 * (3433.4104799543807 + x2 * (3663.8135197609940 +
+          x2 * (1821.3627056982116 + x2 * (423.04454079463840 + x2 * (39.917377889601520 + x2)))));
+    return x * x2 * (num / den) + x;
+}
+
+__device__ __forceinline__ double atan2A(double y, double x)
+{
+    if(y == 0)
+        return copysign(M_PI * (signbit(x) ? 1.0 : 0.0), y);
+    bool negx = x < 0;
+    double signy = y >= 0 ? 1 : -1, absy = y * signy, absx = negx ? -x : x;
+    bool ybigger = absy > absx;
+    double res = atan1A(ybigger ? -x / y : y / x);
+    if(ybigger)   res += 0.5 * M_PI * signy;
+    else if(negx) res += M_PI * signy;
+    return res;
+}
+
+__device__ __forceinline__ int binSearchD(double x, const double* __restrict__ arr, int size)
+{
+    if(size < 1 || !(x >= arr[0])) return -1;
+    if(x > arr[size - 1] || size < 2) return size - 1;
+    int index = (int)((x - arr[0]) / (arr[size - 1] - arr[0]) * (size - 1));
+    int indhi = size - 1;
+    if(index == size - 1) return size - 2;
+    if(x >= arr[index]) {
+        if(x < arr[index + 1]) return index;
+    } else {
+        indhi = index;
+        index = 0;
+    }
+    while(indhi > index + 1) {
+        int i = (indhi + index) / 2;
+        if(arr[i] > x) indhi = i;
+        else index = i;
+    }
+    return index;
+}
+
+// weights of the quintic Hermite interpolant on [xl, xh] at x (value and first derivative)
+struct QuinticW {
+    bool atEnd;                 // x == xh: endpoint values are returned exactly
+    double dx, dx2, h, P, Q, R, Pp, Qp, Rp;
+    __device__ __forceinline__ void init(double x, double xl, double xh)
+    {
+        atEnd = x == xh;
+        dx = x - xl;
+        dx2 = .5 * dx * dx;
+        h = xh - xl;
+        const double hi = 1 / h, h2 = h * h, t = dx * hi, t2 = t * t, t3 = t * t2,
+            t1t = t * (1 - t), Px = 30 * t1t * hi;
+        P  = t3 * (10 - t * (15 - 6 * t));
+        Q  = t3 * (1 - t * 0.5) * h;
+        R  = t3 * (1 - t * (1.25 - 0.5 * t)) * h2;
+        Pp = Px * t1t;
+        Qp = t2 * (3 - t * 2);
+        Rp = t2 * (3 - t * (5 - 2.5 * t)) * h;
+    }
+    // value and derivative of one function given f, f', f'' at both ends
+    __device__ __forceinline__ void eval2(double fl, double fh, double f1l, double f1h,
+        double f2l, double f2h, double& f, double& df) const
+    {
+        if(atEnd) { f = fh; df = f1h; return; }
+        const double fd = fh - fl - 0.5 * h * (f1h + f1l),
+            f1d = f1h - f1l - 0.5 * h * (f2h + f2l), f2d = f2h - f2l;
+        f  = fl + P * fd + dx * f1l + Q * f1d + dx2 * f2l + R * f2d;
+        df = Pp * fd + f1l + Qp * f1d + dx * f2l + Rp * f2d;
+    }
+    __device__ __forceinline__ double eval1(double fl, double fh, double f1l, double f1h,
+        double f2l, double f2h) const
+    {
+        if(atEnd) return fh;
+        const double fd = fh - fl - 0.5 * h * (f1h + f1l),
+            f1d = f1h - f1l - 0.5 * h * (f2h + f2l), f2d = f2h - f2l;
+        return fl + P * fd + dx * f1l + Q * f1d + dx2 * f2l + R * f2d;
+    }
+};
+
+// 2-D quintic spline of one harmonic slot at a located cell: value and (ln r, tau) derivatives.
+// A, B point to the node records of corners (xi, yi) and (xi+1, yi); the (.., yi+1) corners
+// follow each at +rowStep doubles.  Returns z without the f_offset added back.
+template<typename T>
+__device__ __forceinline__ void splineSlot(const T* __restrict__ A, const T* __restrict__ B,
+    int rowStep, int offsetCorner, const QuinticW& wy, const QuinticW& wx,
+    double& z, double& zx, double& zy)
+{
+    // corners: ll = A, lu = A + rowStep, ul = B, uu = B + rowStep (x = ln r first index)
+    const T* ll = A; const T* lu = A + rowStep; const T* ul = B; const T* uu = B + rowStep;
+    const double off = offsetCorner == 0 ? (double)ll[0] : offsetCorner == 1 ? (double)lu[0] :
+                       offsetCorner == 2 ? (double)ul[0] : (double)uu[0];
+    // y-pass (tau) for the six functions {f, fx, fxx} at xlow/xupp, with f, f_y, f_yy ends
+    double F0, F1, F2, F3, F4, F5, dF0, dF1, dF2, dF3, dF4, dF5;
+    wy.eval2((double)ll[0] - off, (double)lu[0] - off, ll[2], lu[2], ll[5], lu[5], F0, dF0);
+    wy.eval2((double)ul[0] - off, (double)uu[0] - off, ul[2], uu[2], ul[5], uu[5], F1, dF1);
+    wy.eval2(ll[1], lu[1], ll[4], lu[4], ll[7], lu[7], F2, dF2);
+    wy.eval2(ul[1], uu[1], ul[4], uu[4], ul[7], uu[7], F3, dF3);
+    wy.eval2(ll[3], lu[3], ll[6], lu[6], ll[8], lu[8], F4, dF4);
+    wy.eval2(ul[3], uu[3], ul[6], uu[6], ul[8], uu[8], F5, dF5);
+    // x-pass (ln r)
+    wx.eval2(F0, F1, F2, F3, F4, F5, z, zx);
+    zy = wx.eval1(dF0, dF1, dF2, dF3, dF4, dF5);
+    z += off;
+}
+
+// interior evaluation in cylindrical coordinates: potential and (R, z, phi) gradient
+template<typename T>
+__device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const T* __restrict__ nodes,
+    double R, double z, double phi, double& pot, double& gR, double& gz, double& gphi)
+{
+    const double r = sqrt(R * R + z * z), logr = log(r), rplusRinv = 1. / (r + fabs(R)),
+        tau = R == 0 ? (z > 0 ? 1. : z < 0 ? -1. : 0.) : z * rplusRinv;
+    const int xi = binSearchD(logr, P.gridR, P.nr), yi = binSearchD(tau, P.gridT, P.nt);
+    if(xi < 0 || xi >= P.nr - 1 || yi < 0 || yi >= P.nt - 1) {
+        pot = gR = gz = gphi = NAN;
+        return;
+    }
+    const double xlow = P.gridR[xi], xupp = P.gridR[xi + 1], ylow = P.gridT[yi], yupp = P.gridT[yi + 1];
+    const int offsetCorner = logr == xupp ? (tau == yupp ? 3 : 2) : (tau == yupp ? 1 : 0);
+    QuinticW wy, wx;
+    wy.init(tau, ylow, yupp);
+    wx.init(logr, xlow, xupp);
+    const int ns = P.ns, rowStep = ns * AGX_NODE_STRIDE;
+    const T* baseA = nodes + ((size_t)(xi * P.nt + yi) * ns) * AGX_NODE_STRIDE;
+    const T* baseB = baseA + (size_t)P.nt * rowStep;
+
+    double trPot, trDr, trDth, trDphi = 0;
+    if(P.mmin == 0) {
+        // m = 0 is slot 0
+        double P0, d0r, d0t;
+        splineSlot<T>(baseA, baseB, rowStep, offsetCorner, wy, wx, P0, d0r, d0t);
+        if(P.logScaling) {
+            const double expX = exp(P0), val = 1 / (P.invPhi0 - expX), dPhidX = val * val * expX;
+            P0 = val;
+            d0r *= dPhidX;
+            d0t *= dPhidX;
+        }
+        trPot = P0; trDr = d0r; trDth = d0t;
+        if(ns > 1) {
+            // cos/sin(k phi) recurrence (trigMultiAngle), advanced up to each slot's m
+            double alpha, beta, sphi, cphi, s1 = 0, c1 = 1;
+            sincosA(phi, sphi, cphi);
+            sincosA(phi / 2, alpha, beta);
+            alpha *= alpha * 2;
+            beta = sphi;
+            int k = 0;
+            for(int s = 1; s < ns; s++) {
+                const int m = P.slotM[s];
+                while(k < m) {
+                    const double cn = c1 - (alpha * c1 + beta * s1);
+                    const double sn = s1 - (alpha * s1 - beta * c1);
+                    c1 = cn; s1 = sn; k++;
+                }
+                double Pm, dmr, dmt;
+                splineSlot<T>(baseA + s * AGX_NODE_STRIDE, baseB + s * AGX_NODE_STRIDE, rowStep,
+                    offsetCorner, wy, wx, Pm, dmr, dmt);
+                if(P.logScaling) {
+                    dmr = dmr * P0 + Pm * d0r;
+                    dmt = dmt * P0 + Pm * d0t;
+                    Pm *= P0;
+                }
+                const double trig = c1, dtrig = -m * s1;
+                trPot  += Pm * trig;
+                trDr   += dmr * trig;
+                trDth  += dmt * trig;
+                trDphi += Pm * dtrig;
+            }
+        }
+    } else {
+        // generic order (sine terms present): reference loop over mm = m - mmin ascending
+        double Cv[AGX_MAX_SLOTS], Cr[AGX_MAX_SLOTS], Ct[AGX_MAX_SLOTS];
+        for(int s = 0; s < ns; s++)
+            splineSlot<T>(baseA + s * AGX_NODE_STRIDE, baseB + s * AGX_NODE_STRIDE, rowStep,
+                offsetCorner, wy, wx, Cv[s], Cr[s], Ct[s]);
+        const int zs = P.zeroSlot;
+        if(P.logScaling) {
+            const double expX = exp(Cv[zs]), val = 1 / (P.invPhi0 - expX), dPhidX = val * val * expX;
+            Cv[zs] = val; Cr[zs] *= dPhidX; Ct[zs] *= dPhidX;
+            for(int s = 0; s < ns; s++) {
+                if(s == zs) continue;
+                Cr[s] = Cr[s] * Cv[zs] + Cv[s] * Cr[zs];
+                Ct[s] = Ct[s] * Cv[zs] + Cv[s] * Ct[zs];
+                Cv[s] *= Cv[zs];
+            }
+        }
+        double tc[AGX_MAX_M], ts[AGX_MAX_M];
+        {
+            double alpha, beta, sphi, cphi, s1 = 0, c1 = 1;
+            sincosA(phi, sphi, cphi);
+            sincosA(phi / 2, alpha, beta);
+            alpha *= alpha * 2;
+            beta = sphi;
+            for(int k = 0; k < P.mmax; k++) {
+                const double cn = c1 - (alpha * c1 + beta * s1);
+                const double sn = s1 - (alpha * s1 - beta * c1);
+                tc[k] = cn; ts[k] = sn; c1 = cn; s1 = sn;
+            }
+        }
+        trPot = Cv[zs]; trDr = Cr[zs]; trDth = Ct[zs];
+        for(int s = 0; s < ns; s++) {
+            const int m = P.slotM[s];
+            if(m == 0) continue;
+            const double trig = m > 0 ? tc[m - 1] : ts[-m - 1];
+            const double dtrig = m > 0 ? -m * ts[m - 1] : -m * tc[-m - 1];
+            trPot  += Cv[s] * trig;
+            trDr   += Cr[s] * trig;
+            trDth  += Ct[s] * trig;
+            trDphi += Cv[s] * dtrig;
+        }
+    }
+    pot = trPot;
+    const double rinv = 1 / r, r2inv = rinv * rinv;
+    const double drdR = R * r2inv, drdz = z * r2inv, dthdR = -tau * rinv, dthdz = rinv - rplusRinv;
+    gR = trDr * drdR + trDth * dthdR;
+    gz = trDr * drdz + trDth * dthdz;
+    gphi = trDphi;
+}
+
+// PowerLawMultipole::evalCyl: extrapolation inside rmin / outside rmax
+__device__ __noinline__ void powerLawEval(const AgxPowerLawDev& L, double R, double z, double phi,
+    double& pot, double& gR, double& gz, double& gphi)
+{
+    const double rsq = R * R + z * z, dlogr = log(rsq / L.r0sq) * 0.5;
+    const int lmax = (L.inner && rsq < L.r0sq * 1e-16) || (!L.inner && rsq > L.r0sq * 1e16) ? 0 : L.lmax;
+    // one (l,m) term: Phi_lm and d Phi_lm / d ln r
+    auto term = [&](int l, int m, double& Pv, double& dPv) {
+        const int c = l * (l + 1) + m;
+        const double s = L.S[c], u = L.U[c], w = L.W[c], v = L.inner ? l : -l - 1;
+        const double rv  = v != 0 ? exp(dlogr * v) : 1;
+        const double rs  = s != v ? (s != 0 ? exp(dlogr * s) : 1) : rv;
+        const double urs = u * rs * (s != v || u == 0 ? 1 : dlogr);
+        const double wrv = w * rv;
+        const double qr2 = v == 0 ? L.Q * rsq / L.r0sq : 0;
+        Pv  = urs + wrv + qr2;
+        dPv = urs * s + wrv * v + (s != v ? 0 : u * rs) + qr2 * 2;
+    };
+    if(lmax == 0) {
+        double P0, dP0;
+        term(0, 0, P0, dP0);
+        pot = P0;
+        const double rsqinv = rsq > 0 ? 1 / rsq : 0, Rr2 = rsq < INFINITY ? R * rsqinv : 0,
+            zr2 = rsq < INFINITY ? z * rsqinv : 0;
+        gR = dP0 * Rr2; gz = dP0 * zr2; gphi = 0;
+        return;
+    }
+    const double tau = z == 0 ? 0 : z / (sqrt(R * R + z * z) + R);
+    double vr, gr, gth, gph = 0;
+    if(L.lmax == 2 && L.mmin == 0 && L.step == 2) {
+        // sphHarmTransformInverseDeriv2 (potential_multipole.cpp:722-772)
+        const double C2 = sqrt(1.25), D2 = sqrt(3.75);
+        const double ct = 2 * tau / (1 + tau * tau), st = (1 - tau * tau) / (1 + tau * tau),
+            cc = ct * ct, cs = ct * st, ss = st * st, Y20 = (3 * C2 * cc - C2), dY20 = -6 * C2 * cs;
+        double P00, d00, P20, d20;
+        term(0, 0, P00, d00);
+        term(2, 0, P20, d20);
+        vr = Y20 * P20 + P00;
+        gr = Y20 * d20 + d00;
+        gth = dY20 * P20;
+        if(L.mmax == 2) {
+            double sp, cp, P22, d22;
+            term(2, 2, P22, d22);
+            const double Y22 = D2 * ss, dY22 = 2 * D2 * cs;
+            sincosA(2 * phi, sp, cp);
+            vr  += Y22 * P22 * cp;
+            gr  += Y22 * d22 * cp;
+            gth += dY22 * P22 * cp;
+            gph += Y22 * P22 * -2 * sp;
+        }
+    } else {
+        // sphHarmTransformInverseDeriv (potential_multipole.cpp:778-821) with the Legendre
+        // recurrence of sphHarmArray (math_sphharm.cpp:63-132) streamed in l
+        const int nm = L.mmax - L.mmin + 1;
+        double Cv[AGX_MAX_SLOTS], Cr[AGX_MAX_SLOTS], Ct[AGX_MAX_SLOTS];
+        const double ct = 2 * tau / (1 + tau * tau), st = (1 - tau * tau) / (1 + tau * tau);
+        for(int mm = 0; mm < nm; mm++) {
+            const int m = mm + L.mmin, lmin = L.lmin[m + L.mmax];
+            if(lmin > L.lmax) continue;
+            const double mul = m == 0 ? 2 * 1.77245385090551602729816748334 :
+                2 * 1.77245385090551602729816748334 * M_SQRT2;
+            Cv[mm] = Cr[mm] = Ct[mm] = 0;
+            const int am = m < 0 ? -m : m;
+            // legendrePmm
+            double fact2m = 1, dfact = 1;
+            for(int i = 2; i <= 2 * am; i++) fact2m *= i;
+            for(int i = 2 * am - 1; i > 1; i -= 2) dfact *= i;
+            double prefact = 0.5 / 1.77245385090551602729816748334 * sqrt((2 * am + 1) / fact2m);
+            double Pl, dPl;
+            if(am == 0) { Pl = prefact; dPl = 0; }
+            else if(am == 1) { Pl = -st * prefact; dPl = -ct * prefact; }
+            else {
+                const double coef = prefact * dfact * (am % 2 == 1 ? -1 : 1);
+                double sinm2 = 1, xb = st;
+                for(int n = am - 2; n; n >>= 1) { if(n % 2) sinm2 *= xb; xb *= xb; }
+                dPl = am * coef * sinm2 * st * ct;
+                Pl = coef * sinm2 * (st * st);
+            }
+            double Plm1 = Pl / prefact, Plm = ct * (2 * am + 1) * Plm1, Plm2 = 0;
+            for(int l = am; l <= L.lmax; l++) {
+                if(l > am) {
+                    if(l > am + 1) Plm = (ct * (2 * l - 1) * Plm1 - (l + am - 1) * Plm2) / (l - am);
+                    prefact *= sqrt((2 * l + 1.) / (2 * l - 1.) * (l - am) / (l + am));
+                    Pl = Plm * prefact;
+                    double d = 0;
+                    if(st >= 1e-8 || (am > 2 && st > 0)) d = (l * ct * Plm - (l + am) * Plm1) / st;
+                    else if(am == 0) d = -l * (l + 1) / 2 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
+                    else if(am == 1) d = -l * (l + 1) / 2 * (ct > 0 || l % 2 == 0 ? 1 : -1);
+                    else if(am == 2) d = l * (l + 1) * (l + 2) * (l - 1) / 4 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
+                    dPl = prefact * d;
+                    Plm2 = Plm1;
+                    Plm1 = Plm;
+                }
+                if(l >= lmin && (l - lmin) % L.step == 0 && l <= lmax) {
+                    double Pv, dPv;
+                    term(l, m, Pv, dPv);
+                    Cv[mm] += Pl * Pv * mul;
+                    Cr[mm] += Pl * dPv * mul;
+                    Ct[mm] += dPl * Pv * mul;
+                }
+            }
+        }
+        // fourierTransformAzimuth
+        const int z0 = -L.mmin;
+        vr = Cv[z0]; gr = Cr[z0]; gth = Ct[z0];
+        if(L.mmax > 0) {
+            double tc[AGX_MAX_M], ts[AGX_MAX_M];
+            double alpha, beta, sphi, cphi, s1 = 0, c1 = 1;
+            sincosA(phi, sphi, cphi);
+            sincosA(phi / 2, alpha, beta);
+            alpha *= alpha * 2;
+            beta = sphi;
+            for(int k = 0; k < L.mmax; k++) {
+                const double cn = c1 - (alpha * c1 + beta * s1);
+                const double sn = s1 - (alpha * s1 - beta * c1);
+                tc[k] = cn; ts[k] = sn; c1 = cn; s1 = sn;
+            }
+            for(int mm = 0; mm < nm; mm++) {
+                const int m = mm + L.mmin;
+                if(m == 0 || L.lmin[m + L.mmax] > L.lmax) continue;
+                const double trig = m > 0 ? tc[m - 1] : ts[-m - 1];
+                const double dtrig = m > 0 ? -m * ts[m - 1] : -m * tc[-m - 1];
+                vr  += Cv[mm] * trig;
+                gr  += Cr[mm] * trig;
+                gth += Ct[mm] * trig;
+                gph += Cv[mm] * dtrig;
+            }
+        }
+    }
+    pot = vr;
+    // transformDerivsSphToCyl
+    const double r2inv = 1 / (R * R + z * z), drdR = R * r2inv, drdz = z * r2inv;
+    gR = gr * drdR + gth * drdz;
+    gz = gr * drdz + gth * -drdR;
+    gphi = gph;
+}
+
+// Multipole::evalCyl dispatch (potential_multipole.cpp:1538-1548)
+template<typename T>
+__device__ __forceinline__ void multipoleCyl(const AgxMultipoleDev& P, const T* __restrict__ nodes,
+    double R, double z, double phi, double& pot, double& gR, double& gz, double& gphi)
+{
+    const double rsq = R * R + z * z;
+    if(rsq < P.rsqInner)
+        powerLawEval(P.inner, R, z, phi, pot, gR, gz, gphi);
+    else if(rsq > P.rsqOuter)
+        powerLawEval(P.outer, R, z, phi, pot, gR, gz, gphi);
+    else
+        interiorEval<T>(P, nodes, R, z, phi, pot, gR, gz, gphi);
+}
+
+// BasePotentialCyl::evalCar: Car -> Cyl, evaluate, gradient back to Car (+dPhi/dx)
+template<typename T>
+__device__ __forceinline__ void multipoleCar(const AgxMultipoleDev& P, const T* __restrict__ nodes,
+    double x, double y, double z, double& pot, double& gx, double& gy, double& gz)
+{
+    const double R = sqrt(x * x + y * y);
+    double dRdx, dRdy, dphidx, dphidy, phi;
+    if(R == 0) {
+        dRdx = dRdy = dphidx = dphidy = 1.;
+        phi = 0;
+    } else {
+        const double cosphi = x / R, sinphi = y / R;
+        dRdx = cosphi; dRdy = sinphi; dphidx = -sinphi / R; dphidy = cosphi / R;
+        phi = atan2A(y, x);
+    }
+    double gR, gzc, gphi;
+    multipoleCyl<T>(P, nodes, R, z, phi, pot, gR, gzc, gphi);
+    gx = gR * dRdx + gphi * dphidx;
+    gy = gR * dRdy + gphi * dphidy;
+    gz = gzc;
+}
+
+}  // namespace agxd

@@ -1,0 +1,43 @@
+// Device-visible layout of a Multipole potential, shared by the host builder (C++) and the HIP
+// kernels.  Plain structs of scalars and device pointers: passed to kernels by value.
+//
+// Interior (MultipoleInterp2d in the reference, src/potential_multipole.cpp:1854-2080):
+//   one 2-D quintic spline in (ln r, tau) per non-empty azimuthal harmonic m ("slot").
+//   All slots share the (ln r, tau) grid, so the node data are stored node-major:
+//     nodes[((i*nt + j)*ns + s)*AGX_NODE_STRIDE + q],  i < nr (ln r), j < nt (tau), s < ns,
+//     q = 0..8 : f, f_x, f_y, f_xx, f_xy, f_yy, f_xxy, f_xyy, f_xxyy   (q = 9: padding)
+//   so a point's 4 cell corners for all harmonics are two contiguous runs of 2*ns*80 bytes.
+// Extrapolation (PowerLawMultipole, src/potential_multipole.cpp:1585-1658): per (l,m) term
+//   S, U, W indexed by c = l*(l+1)+m, plus Q for the inner monopole.
+#pragma once
+
+#define AGX_NODE_STRIDE 10
+#define AGX_MAX_M 16
+#define AGX_MAX_SLOTS (2 * AGX_MAX_M + 1)
+
+struct AgxPowerLawDev {
+    double r0sq;
+    double Q;
+    int inner;        // 1: r < rmin branch (v = l), 0: r > rmax branch (v = -l-1)
+    int lmax;         // of the index set deduced from the non-zero U terms
+    int mmax, mmin, step;
+    int lmin[AGX_MAX_SLOTS];   // lmin(m) at [m + mmax]; > lmax means the harmonic is empty
+    const double* S;
+    const double* U;
+    const double* W;
+};
+
+struct AgxMultipoleDev {
+    int nr, nt, ns;             // radial nodes, tau nodes, stored harmonic slots
+    int mmax, mmin;             // azimuthal index range of the expansion
+    int logScaling;             // m=0 term stored as ln(invPhi0 - 1/Phi), others as Phi_m/Phi_0
+    double invPhi0;
+    double rsqInner, rsqOuter;  // (rmin*(1+eps))^2, (rmax*(1-eps))^2 : branch thresholds
+    int slotM[AGX_MAX_SLOTS];   // m of each stored slot, ascending as in the reference loop
+    int zeroSlot;               // slot holding m = 0
+    const double* gridR;        // ln r nodes [nr]
+    const double* gridT;        // tau nodes [nt]
+    const double* nodes;        // node-major spline data (see above)
+    const float* nodesF;        // the same data rounded to float (precision-split form), or null
+    AgxPowerLawDev inner, outer;
+};

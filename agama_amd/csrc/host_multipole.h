@@ -1,0 +1,71 @@
+// Host-side construction of Multipole potentials for the agama_amd library:
+//   density models -> spherical-harmonic coefficients (Poisson solver)
+//   coefficient text files (the reference's format) <-> coefficients
+//   coefficients -> 2-D quintic spline tables in the device layout of agx_tables.h
+// All of this runs once per potential on the CPU, as in the reference; evaluation and orbit
+// integration run on the GPU.
+#pragma once
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+#include "agx_tables.h"
+
+namespace agx {
+
+// symmetry flags, same values as the reference's coord::SymmetryType (src/coord.h:50-75)
+enum Sym { ST_NONE = 0, ST_XREFL = 1, ST_YREFL = 2, ST_ZREFL = 4, ST_REFL = 8, ST_ZROT = 16,
+    ST_ROT = 32, ST_UNKNOWN = 64, ST_BISYM = 12, ST_TRIAXIAL = 15, ST_AXISYM = 31,
+    ST_SPHERICAL = 63 };
+
+// case-insensitive key=value parameters (the reference's INI / keyword conventions)
+struct Params {
+    std::map<std::string, std::string> kv;
+    static Params parse(const std::string& text);   // "a=1, b=2" or one pair per line
+    bool has(const std::string& key) const;
+    std::string str(const std::string& key, const std::string& def = "") const;
+    double num(const std::string& key, double def) const;
+    void checkOnly(const std::vector<std::string>& allowed) const;   // rejects unknown keys
+};
+
+int symmetryFromName(const std::string& name);   // reference: getSymmetryTypeByName
+std::string symmetryName(int sym);                // reference: getSymmetryNameByType
+
+// spherical-harmonic coefficients on a radial grid: Phi[c][k], dPhi[c][k], c = l*(l+1)+m
+struct Coefs {
+    std::vector<double> radii;
+    std::vector<std::vector<double> > Phi, dPhi;
+    int lmax() const;
+};
+
+struct Density {
+    virtual ~Density() {}
+    virtual double rhoCyl(double R, double z, double phi) const = 0;
+    virtual int symmetry() const = 0;
+};
+std::unique_ptr<Density> makeDensity(const Params& p);   // Dehnen, Ferrers, Disk, NFW, Plummer,
+                                                         // or a sum given as "a+b" components
+
+Coefs multipoleFromDensity(const Density& dens, int lmax, int mmax, int symExp,
+    int gridSizeR, double rmin, double rmax);
+Coefs readCoefFile(const std::string& path);
+void writeCoefFile(const std::string& path, const Coefs& c);
+
+// host copy of the device tables
+struct HostPowerLaw {
+    double r0sq = 0, Q = 0;
+    int inner = 1, lmax = 0, mmax = 0, mmin = 0, step = 1;
+    std::vector<int> lmin;          // [m + mmax]
+    std::vector<double> S, U, W;    // indexed by c
+};
+struct HostTables {
+    int nr = 0, nt = 0, ns = 0, mmax = 0, mmin = 0, lmax = 0, step = 1, logScaling = 0;
+    double invPhi0 = 0, rsqInner = 0, rsqOuter = 0;
+    std::vector<int> slotM;
+    int zeroSlot = 0;
+    std::vector<double> gridR, gridT, nodes;   // nodes: see agx_tables.h
+    HostPowerLaw inner, outer;
+};
+HostTables buildTables(const Coefs& c);
+
+}  // namespace agx

@@ -1,0 +1,148 @@
+/*  agama_amd: MI355X (gfx950) Multipole potential evaluation and DOP853 orbit integration.
+
+    Plain C ABI: opaque handles, plain pointers and sizes, integer status codes, a
+    thread-local last-error string.  Every entry point names the reference (Agama) interface
+    that it replaces; paths are relative to the reference source tree.
+
+    Conventions (as in the reference):
+      - positions/velocities are Cartesian, rows of 3 (x,y,z) or 6 (x,y,z,vx,vy,vz), row-major;
+      - "grad" outputs are +dPhi/dx (the reference's BasePotential::eval gradient), while the
+        NEMO-style entry point returns accelerations -dPhi/dx;
+      - a non-zero Omega integrates in a frame rotating about z, inputs and outputs are given
+        in the rotating frame (src/orbit.h:219-232).
+    All computation runs on the current HIP device of the calling thread at creation time.
+    Functions taking host pointers copy to and from the device and synchronise; the _device
+    variants take device pointers and a hipStream_t (passed as void*) and are asynchronous.
+*/
+#ifndef AGAMA_AMD_H
+#define AGAMA_AMD_H
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct agx_potential agx_potential;
+
+/* status codes */
+#define AGX_OK            0
+#define AGX_EINVAL        1   /* invalid argument */
+#define AGX_EDEVICE       2   /* HIP runtime error */
+#define AGX_EUNSUPPORTED  3   /* valid in the reference, not provided by this library */
+
+/* precision of the spline tables used by the evaluation (flags argument) */
+#define AGX_PRECISION_FP64 0
+#define AGX_PRECISION_FP32 1  /* tables rounded to float; arithmetic, stages, error control FP64 */
+
+/* Message of the last failed call on this thread ("" if none).
+   Replaces agama_getError (src/interface_c.cpp:15-19), which is a process-global string. */
+const char* agx_last_error(void);
+
+/* Create a potential from key=value parameters, separated by commas or newlines
+   ("type=Multipole, file=coefs.ini" or "type=Multipole, density=Dehnen, gamma=1, ...").
+   Replaces agama_createPotential (src/interface_c.h:46, src/interface_c.cpp:31-54) and the
+   Multipole part of potential::createPotential (src/potential_factory.cpp:1357-1441).
+   Returns NULL on error. */
+agx_potential* agx_potential_create(const char* params);
+
+/* Create a Multipole potential directly from coefficient arrays: radii[nr] and
+   phi/dphi[c*nr + k] for c = l*(l+1)+m < (lmax+1)^2
+   (reference: Multipole::Multipole, src/potential_multipole.cpp:1491-1516). */
+agx_potential* agx_potential_from_coefs(int nr, int lmax, const double* radii,
+    const double* phi, const double* dphi);
+
+/* Replaces agama_deletePotential (src/interface_c.cpp:56-60). */
+void agx_potential_destroy(agx_potential* pot);
+
+/* Host-only construction (no GPU needed): build a Multipole expansion from key=value
+   parameters as agx_potential_create does and write its coefficient file.  This is the
+   reference's Potential(type='Multipole', density=...).export(path) sequence
+   (src/potential_multipole.cpp:1400-1441, src/potential_factory.cpp:909-929). */
+int agx_build_coef_file(const char* params, const char* path);
+
+/* Host-only diagnostic: the interpolation tables built from a coefficient file, in the
+   reference's per-harmonic layout.  which: 0 = ln r grid, 1 = tau grid, 2 = the 9 node arrays
+   (f, f_x, f_y, f_xx, f_xy, f_yy, f_xxy, f_xyy, f_xxyy, each nr*nt row-major) of harmonic m,
+   3 = {logScaling, invPhi0, lmax, mmax, mmin, step}.  Returns the number of doubles written
+   (or needed when out is NULL), or -1. */
+long agx_debug_tables(const char* coef_path, int which, int m, double* out);
+
+/* Write the coefficients in the reference's text format
+   (writePotential / writePotentialMultipole, src/potential_factory.cpp:909-929). */
+int agx_potential_write(const agx_potential* pot, const char* path);
+
+/* Query the coefficient arrays: returns nr and lmax; copies arrays when pointers are non-NULL
+   (sizes nr and (lmax+1)^2*nr).  Replaces Multipole::getCoefs' data access. */
+int agx_potential_coefs(const agx_potential* pot, int* nr, int* lmax, double* radii,
+    double* phi, double* dphi);
+
+/* Single-point potential and gradient.  deriv2 (Hessian) must be NULL in this version.
+   Replaces agama_evalPotential (src/interface_c.h:75, src/interface_c.cpp:68-98). */
+double agx_eval_potential(const agx_potential* pot, const double pos[3], double t,
+    double deriv[3], double deriv2[6]);
+
+/* Batched potential and gradient at n points (host buffers pos[n*3], phi[n], grad[n*3];
+   phi or grad may be NULL).  Replaces Potential.eval/potential/force over
+   BatchFunction (src/interface_python.cpp:3027-3207, 1206-1489). */
+int agx_eval_many(const agx_potential* pot, long n, const double* pos, double t,
+    double* phi, double* grad, int flags);
+
+/* Same with device buffers, asynchronous on `stream` (hipStream_t). */
+int agx_eval_many_device(const agx_potential* pot, long n, const double* d_pos, double t,
+    double* d_phi, double* d_grad, int flags, void* stream);
+
+/* Batch orbit integration with DOP853 in the given potential, restating agama.orbit
+   (src/interface_python.cpp:7689-8281) with dtype=float64 and separateTime output:
+     ic[n*6], tstart[n] (NULL: 0), ttotal[n], trajsize[n] (>=1) on the host;
+     out_time[rows], out_traj[rows*6], rows = sum(trajsize), orbit i at row offset
+     sum_{j<i} trajsize[j], NaN-padded after an early stop (storeOrbitArrays,
+     src/interface_python.cpp:6921-6959); out_final[n*6] = last recorded point;
+     out_nsteps / out_nattempts / out_nevals: accepted steps, attempts, RHS evaluations;
+     out_maxde: max |E/E0-1| over accepted steps (E - Omega*Lz when Omega != 0);
+     out_status: 0 done, 1 stepper failure (orbit.cpp:69-75), 2 maxNumSteps reached.
+   Any output pointer may be NULL.  Returns AGX_OK or an error code. */
+int agx_orbit_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, double accuracy, double Omega, long maxNumSteps,
+    double* out_time, double* out_traj, double* out_final, long* out_nsteps,
+    long* out_nattempts, long* out_nevals, double* out_maxde, int* out_status, int flags);
+
+/* Same with device buffers.  d_traj_offset[n] gives each orbit's first output row; d_order
+   (may be NULL) a processing order (permutation of 0..n-1), e.g. longest integrations
+   first.  Asynchronous on `stream`.  Calls that overlap on one potential handle race on its
+   work-queue counter; use one handle per concurrent stream. */
+int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
+    const double* d_tstart, const double* d_ttotal, const int* d_trajsize,
+    const long* d_traj_offset, const int* d_order, double accuracy, double Omega,
+    long maxNumSteps, double* d_time, double* d_traj, double* d_final, long* d_nsteps,
+    long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags,
+    void* stream);
+
+/* Launch geometry of the orbit kernel (workgroups, threads per workgroup); 0 keeps the
+   default.  Set by the autotuner (agama_amd.tune). */
+int agx_set_orbit_launch(int num_blocks, int block_size);
+
+/* ---- NEMO acceleration plugin (src/interface_nemo.cpp:47-60, 114-147, 180-220) ----
+   iniacceleration(pars, npar, file, &accel, &need_m, &need_v): pars[0] = Omega (pattern
+   speed), pars[1] = G is ignored unless 1; file names a Multipole coefficient file.  The
+   returned accel has the acc_pter signature:
+     accel(ndim, t, n, m, pos, vel, flags, pot, acc, flag, numtype)
+   with ndim = 2 or 3, numtype 'f' or 'd', flag bit 0: add to pot, bit 1: add to acc (else assign),
+   flags[i] & 1 selects active bodies (NULL: all); positions are in the rotating frame.  */
+typedef void (*agx_acc_pter)(int ndim, double t, int n, const void* m, const void* pos,
+    const void* vel, const int* flags, void* pot, void* acc, int flag, char numtype);
+void iniacceleration(const double* pars, int npar, const char* file, agx_acc_pter* accel,
+    bool* need_m, bool* need_v);
+
+/* ---- drop-in aliases of the reference's plain C interface (src/interface_c.h:15-85) ---- */
+typedef agx_potential agama_Potential;
+const char* agama_getError(void);
+agama_Potential* agama_createPotential(const char* params);
+void agama_deletePotential(agama_Potential* pot);
+double agama_evalPotential(const agama_Potential* pot, const double pos[3], double time,
+    double deriv[3], double deriv2[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

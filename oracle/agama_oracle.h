@@ -1,0 +1,93 @@
+/*  CPU oracle for the Multipole potential + DOP853 orbit integration path.
+
+    TEST INFRASTRUCTURE ONLY.  This library is a plain C++ restatement of the reference
+    algorithm (Agama) used as the checker in tests/, in __graft_entry__.smoke() and as the
+    timed CPU baseline in bench.py.  The product (agama_amd) never links, loads or calls it.
+
+    Every function in agama_oracle.cpp cites the reference file:line that it restates.
+    Parity status per function is listed in the header comment of agama_oracle.cpp.
+*/
+#pragma once
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_pot orc_pot;
+
+/* last error message of the calling thread ("" if none) */
+const char* orc_last_error(void);
+
+/* Multipole potential from an Agama-format coefficient text file
+   (reference: src/potential_factory.cpp:725-747 createMultipoleFromCoefs) */
+orc_pot* orc_multipole_from_file(const char* path);
+
+/* Multipole potential from arrays: radii[nr], phi[c*nr+k], dphi[c*nr+k], c < (lmax+1)^2
+   (reference: src/potential_multipole.cpp:1491-1516 Multipole::Multipole) */
+orc_pot* orc_multipole_from_arrays(int nr, int lmax, const double* radii,
+    const double* phi, const double* dphi);
+
+/* analytic NFW halo (reference: src/potential_analytic.cpp:55-75) */
+orc_pot* orc_nfw(double mass, double scale_radius);
+
+/* sum of components (reference: src/potential_composite.cpp:51-163); takes ownership */
+orc_pot* orc_composite(int n, orc_pot** components);
+
+void orc_free(orc_pot* p);
+
+/* introspection of the Multipole tables built from the coefficients (for table parity tests):
+   returns the number of doubles written (or needed, if out==NULL). which: 0=gridR(ln r),
+   1=gridT(tau), 2=spline node values for harmonic slot `m` (9 arrays of nr*nt, order
+   f,fx,fy,fxx,fxy,fyy,fxxy,fxyy,fxxyy), 3=scalars {logScaling, invPhi0, lmax, mmax, mmin, step} */
+long orc_multipole_tables(const orc_pot* p, int which, int m, double* out);
+
+/* potential and gradient (+dPhi/dx, not acceleration) at n points pos[n*3]; either output may
+   be NULL.  OpenMP over points with nthreads (<=0: default). */
+void orc_eval_many(const orc_pot* p, long n, const double* pos, double t,
+    double* phi, double* grad, int nthreads);
+
+/* batch orbit integration, restating agama.orbit(potential, ic, time, timestart, trajsize,
+   accuracy, Omega, maxNumSteps, dtype=float) (reference: src/interface_python.cpp:7689-8281).
+   ic[n*6], tstart[n], ttotal[n], trajsize[n] (>=1); traj_offset[n] = first output row of each
+   orbit in out_time / out_traj (rows of 6); out_* may be NULL where noted.
+   out_nsteps/out_nattempts/out_nevals: accepted steps, step attempts, RHS evaluations.
+   out_maxde: max over accepted steps of |E/E0-1| (Jacobi energy if Omega!=0).
+   out_status: 0 ok, 1 stepper failure (zero/non-finite step), 2 maxNumSteps reached.
+   Returns 0 on success, nonzero on error (see orc_last_error). */
+int orc_orbit_many(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, const long* traj_offset,
+    double accuracy, double Omega, long maxNumSteps,
+    double* out_time, double* out_traj, double* out_final,
+    long* out_nsteps, long* out_nattempts, long* out_nevals,
+    double* out_maxde, int* out_status, int nthreads);
+
+/* single DOP853 run of a generic ODE system given by a C callback (closed-form tests):
+   rhs(t, x, dxdt, ctx). Records the state at the end of every step and dense output at
+   `nsample` requested absolute times (sorted). Fixed-step mode when accuracy is INFINITY is
+   NOT special-cased (as in the reference, the error control then always accepts). */
+typedef void (*orc_rhs_fn)(double t, const double* x, double* dxdt, void* ctx);
+int orc_ode_dop853(int ndim, orc_rhs_fn rhs, void* ctx, const double* x0, double ttotal,
+    double accuracy, double* out_xend, int nsample, const double* tsample, double* out_sample,
+    long* out_nsteps);
+
+/* nsteps fixed DOP853 steps of length h (accuracy = infinity accepts every step); records the
+   state at the end of each step and the dense output at its midpoint */
+int orc_ode_dop853_fixed(int ndim, orc_rhs_fn rhs, void* ctx, const double* x0, double h,
+    int nsteps, double* out_states, double* out_mid);
+
+/* restated scalar helpers, exported for direct tests */
+void   orc_sincos(double x, double* s, double* c);
+double orc_atan2(double y, double x);
+long   orc_binsearch(double x, const double* arr, long size);
+double orc_lambertw(double x, int wm1branch);
+void   orc_gl_nodes(int n, double* nodes, double* weights);   /* on [-1,1], ascending */
+void   orc_sphharm_array(int lmax, int m, double tau, double* P, double* dP);
+void   orc_trig_multi_angle(double phi, int m, int needSine, double* out);
+/* 2-D quintic spline on a grid: build from f, fx, fy, fxy (row-major nx*ny), evaluate at
+   npts points (x,y) -> z, z_x, z_y.  Exposes QuinticSpline2d for node-exactness tests. */
+int orc_spline2d_eval(int nx, int ny, const double* xg, const double* yg, const double* f,
+    const double* fx, const double* fy, const double* fxy, long npts, const double* xy,
+    double* z, double* zx, double* zy);
+
+#ifdef __cplusplus
+}
+#endif
