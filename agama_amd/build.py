@@ -6,14 +6,14 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-OUT = os.path.join(HERE, "libagama_amd.so")
-BUILD = os.path.join(HERE, "_build")
+OUT = os.environ.get("AGX_BUILD_OUT", os.path.join(HERE, "libagama_amd.so"))
+BUILD = os.environ.get("AGX_BUILD_DIR", os.path.join(HERE, "_build"))
 ARCH = os.environ.get("AGX_OFFLOAD_ARCH", "gfx950")
 
 HOST_FLAGS = ["-std=c++17", "-O2", "-march=x86-64-v3", "-ffp-contract=off", "-fPIC", "-Wall"]
 HIP_FLAGS = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-Wno-unused-result"]
+             "-Wno-unused-result"] + os.environ.get("AGX_EXTRA_FLAGS", "").split()
 
 
 def _run(cmd):

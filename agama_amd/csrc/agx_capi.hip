@@ -15,16 +15,18 @@
 #include "agx_kernels.h"
 #include "host_multipole.h"
 
-hipError_t agxLaunchEval(const AgxMultipoleDev& P, int precision, long n, const double* pos,
-    double* phi, double* grad, hipStream_t stream);
-hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, int precision, const AgxOrbitArgs& A,
-    int numBlocks, int blockSize, hipStream_t stream);
+hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, long n,
+    const double* pos, double* phi, double* grad, hipStream_t stream);
+hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision,
+    const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream);
 int agxOrbitKernelBlocksPerCU(int precision, int blockSize);
+int agxOrbitKernelBlockSize();
 
 struct agx_potential {
     agx::Coefs coefs;
     agx::HostTables T;
-    AgxMultipoleDev dev;
+    AgxMultipoleDev dev;               // header with device pointers (host copy)
+    AgxMultipoleDev* ddev = nullptr;   // the same header in device memory
     void* dmem = nullptr;
     unsigned int* dcounter = nullptr;
     int device = 0;
@@ -34,7 +36,7 @@ struct agx_potential {
 namespace {
 
 thread_local std::string g_err;
-int g_orbitBlocks = 0, g_orbitBlockSize = 0;
+int g_orbitBlocks = 0;
 
 int fail(int code, const std::string& msg) { g_err = msg; return code; }
 
@@ -56,7 +58,8 @@ void upload(agx_potential& p)
     const size_t oR = take(T.gridR.size() * 8), oT = take(T.gridT.size() * 8),
         oN = take(T.nodes.size() * 8), oF = take(T.nodes.size() * 4),
         oIS = take(nc * 8), oIU = take(nc * 8), oIW = take(nc * 8),
-        oOS = take(nc * 8), oOU = take(nc * 8), oOW = take(nc * 8), oC = take(64);
+        oOS = take(nc * 8), oOU = take(nc * 8), oOW = take(nc * 8), oC = take(64),
+        oH = take(sizeof(AgxMultipoleDev));
     HIPCHECK(hipGetDevice(&p.device));
     HIPCHECK(hipMalloc(&p.dmem, off));
     char* base = static_cast<char*>(p.dmem);
@@ -93,6 +96,8 @@ void upload(agx_potential& p)
     fillPL(D.inner, T.inner, oIS, oIU, oIW);
     fillPL(D.outer, T.outer, oOS, oOU, oOW);
     p.dcounter = reinterpret_cast<unsigned int*>(base + oC);
+    p.ddev = reinterpret_cast<AgxMultipoleDev*>(base + oH);
+    put(oH, &D, sizeof(D));
 }
 
 agx_potential* makeFromCoefs(agx::Coefs&& c)
@@ -133,7 +138,7 @@ agx_potential* createFromParams(const std::string& text)
 
 int orbitGeometry(long n, int precision, int& blocks, int& bs)
 {
-    bs = g_orbitBlockSize > 0 ? g_orbitBlockSize : 256;
+    bs = agxOrbitKernelBlockSize();
     if(g_orbitBlocks > 0) blocks = g_orbitBlocks;
     else {
         int dev = 0, cus = 0;
@@ -266,7 +271,7 @@ int agx_eval_many_device(const agx_potential* pot, long n, const double* d_pos, 
     if(!pot) return fail(AGX_EINVAL, "NULL potential");
     if(n < 0 || (n > 0 && !d_pos)) return fail(AGX_EINVAL, "invalid point array");
     if(flags != AGX_PRECISION_FP64 && flags != AGX_PRECISION_FP32) return fail(AGX_EINVAL, "invalid flags");
-    hipError_t e = agxLaunchEval(pot->dev, flags, n, d_pos, d_phi, d_grad, (hipStream_t)stream);
+    hipError_t e = agxLaunchEval(pot->dev, pot->ddev, flags, n, d_pos, d_phi, d_grad, (hipStream_t)stream);
     if(e != hipSuccess) return fail(AGX_EDEVICE, std::string("eval launch: ") + hipGetErrorString(e));
     return AGX_OK;
 }
@@ -306,10 +311,10 @@ double agx_eval_potential(const agx_potential* pot, const double pos[3], double 
 
 int agx_set_orbit_launch(int num_blocks, int block_size)
 {
-    if(num_blocks < 0 || block_size < 0 || (block_size && (block_size % 64 || block_size > 256)))
-        return fail(AGX_EINVAL, "block size must be a multiple of 64 up to 256");
+    if(num_blocks < 0 || (block_size != 0 && block_size != agxOrbitKernelBlockSize()))
+        return fail(AGX_EINVAL, "num_blocks must be >= 0 and block_size 0 or " +
+            std::to_string(agxOrbitKernelBlockSize()));
     g_orbitBlocks = num_blocks;
-    g_orbitBlockSize = block_size;
     return AGX_OK;
 }
 
@@ -338,7 +343,7 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
         int blocks, bs;
         orbitGeometry(n, flags, blocks, bs);
         HIPCHECK(hipMemsetAsync(pot->dcounter, 0, sizeof(unsigned int), (hipStream_t)stream));
-        hipError_t e = agxLaunchOrbit(pot->dev, flags, A, blocks, bs, (hipStream_t)stream);
+        hipError_t e = agxLaunchOrbit(pot->dev, pot->ddev, flags, A, blocks, bs, (hipStream_t)stream);
         if(e != hipSuccess) return fail(AGX_EDEVICE, std::string("orbit launch: ") + hipGetErrorString(e));
         return AGX_OK;
     } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }

@@ -153,6 +153,32 @@ __device__ __forceinline__ void splineSlot(const T* __restrict__ A, const T* __r
     z += off;
 }
 
+// cos/sin(k phi) for k = 1..m by the recurrence of trigMultiAngle (math_sphharm.cpp:134-152),
+// started from (alpha, beta) = (2 sin^2(phi/2), sin(phi)); `c, s` hold the values for k and
+// are advanced in place up to m
+struct TrigRec {
+    double alpha, beta, c, s;
+    int k;
+    __device__ __forceinline__ void init(double phi)
+    {
+        double sphi, cphi;
+        sincosA(phi, sphi, cphi);
+        sincosA(phi / 2, alpha, beta);
+        alpha *= alpha * 2;
+        beta = sphi;
+        c = 1; s = 0; k = 0;
+    }
+    __device__ __forceinline__ void restart() { c = 1; s = 0; k = 0; }
+    __device__ __forceinline__ void advanceTo(int m)
+    {
+        while(k < m) {
+            const double cn = c - (alpha * c + beta * s);
+            const double sn = s - (alpha * s - beta * c);
+            c = cn; s = sn; k++;
+        }
+    }
+};
+
 // interior evaluation in cylindrical coordinates: potential and (R, z, phi) gradient
 template<typename T>
 __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const T* __restrict__ nodes,
@@ -174,88 +200,48 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const T* 
     const T* baseA = nodes + ((size_t)(xi * P.nt + yi) * ns) * AGX_NODE_STRIDE;
     const T* baseB = baseA + (size_t)P.nt * rowStep;
 
-    double trPot, trDr, trDth, trDphi = 0;
-    if(P.mmin == 0) {
-        // m = 0 is slot 0
-        double P0, d0r, d0t;
-        splineSlot<T>(baseA, baseB, rowStep, offsetCorner, wy, wx, P0, d0r, d0t);
-        if(P.logScaling) {
-            const double expX = exp(P0), val = 1 / (P.invPhi0 - expX), dPhidX = val * val * expX;
-            P0 = val;
-            d0r *= dPhidX;
-            d0t *= dPhidX;
-        }
-        trPot = P0; trDr = d0r; trDth = d0t;
-        if(ns > 1) {
-            // cos/sin(k phi) recurrence (trigMultiAngle), advanced up to each slot's m
-            double alpha, beta, sphi, cphi, s1 = 0, c1 = 1;
-            sincosA(phi, sphi, cphi);
-            sincosA(phi / 2, alpha, beta);
-            alpha *= alpha * 2;
-            beta = sphi;
-            int k = 0;
-            for(int s = 1; s < ns; s++) {
-                const int m = P.slotM[s];
-                while(k < m) {
-                    const double cn = c1 - (alpha * c1 + beta * s1);
-                    const double sn = s1 - (alpha * s1 - beta * c1);
-                    c1 = cn; s1 = sn; k++;
-                }
-                double Pm, dmr, dmt;
-                splineSlot<T>(baseA + s * AGX_NODE_STRIDE, baseB + s * AGX_NODE_STRIDE, rowStep,
-                    offsetCorner, wy, wx, Pm, dmr, dmt);
-                if(P.logScaling) {
-                    dmr = dmr * P0 + Pm * d0r;
-                    dmt = dmt * P0 + Pm * d0t;
-                    Pm *= P0;
-                }
-                const double trig = c1, dtrig = -m * s1;
-                trPot  += Pm * trig;
-                trDr   += dmr * trig;
-                trDth  += dmt * trig;
-                trDphi += Pm * dtrig;
-            }
-        }
-    } else {
-        // generic order (sine terms present): reference loop over mm = m - mmin ascending
-        double Cv[AGX_MAX_SLOTS], Cr[AGX_MAX_SLOTS], Ct[AGX_MAX_SLOTS];
-        for(int s = 0; s < ns; s++)
-            splineSlot<T>(baseA + s * AGX_NODE_STRIDE, baseB + s * AGX_NODE_STRIDE, rowStep,
-                offsetCorner, wy, wx, Cv[s], Cr[s], Ct[s]);
-        const int zs = P.zeroSlot;
-        if(P.logScaling) {
-            const double expX = exp(Cv[zs]), val = 1 / (P.invPhi0 - expX), dPhidX = val * val * expX;
-            Cv[zs] = val; Cr[zs] *= dPhidX; Ct[zs] *= dPhidX;
-            for(int s = 0; s < ns; s++) {
-                if(s == zs) continue;
-                Cr[s] = Cr[s] * Cv[zs] + Cv[s] * Cr[zs];
-                Ct[s] = Ct[s] * Cv[zs] + Cv[s] * Ct[zs];
-                Cv[s] *= Cv[zs];
-            }
-        }
-        double tc[AGX_MAX_M], ts[AGX_MAX_M];
-        {
-            double alpha, beta, sphi, cphi, s1 = 0, c1 = 1;
-            sincosA(phi, sphi, cphi);
-            sincosA(phi / 2, alpha, beta);
-            alpha *= alpha * 2;
-            beta = sphi;
-            for(int k = 0; k < P.mmax; k++) {
-                const double cn = c1 - (alpha * c1 + beta * s1);
-                const double sn = s1 - (alpha * s1 - beta * c1);
-                tc[k] = cn; ts[k] = sn; c1 = cn; s1 = sn;
-            }
-        }
-        trPot = Cv[zs]; trDr = Cr[zs]; trDth = Ct[zs];
+    // m = 0 first (it scales the other harmonics), then the others in ascending m as in the
+    // reference's Fourier synthesis: negative m (sine terms) restart the trig recurrence,
+    // positive m advance one running recurrence
+    const int zs = P.zeroSlot;
+    double P0, d0r, d0t;
+    splineSlot<T>(baseA + zs * AGX_NODE_STRIDE, baseB + zs * AGX_NODE_STRIDE, rowStep, offsetCorner,
+        wy, wx, P0, d0r, d0t);
+    if(P.logScaling) {
+        const double expX = exp(P0), val = 1 / (P.invPhi0 - expX), dPhidX = val * val * expX;
+        P0 = val;
+        d0r *= dPhidX;
+        d0t *= dPhidX;
+    }
+    double trPot = P0, trDr = d0r, trDth = d0t, trDphi = 0;
+    if(ns > 1) {
+        TrigRec pos, neg;
+        pos.init(phi);
+        neg.alpha = pos.alpha; neg.beta = pos.beta;
         for(int s = 0; s < ns; s++) {
+            if(s == zs) continue;
             const int m = P.slotM[s];
-            if(m == 0) continue;
-            const double trig = m > 0 ? tc[m - 1] : ts[-m - 1];
-            const double dtrig = m > 0 ? -m * ts[m - 1] : -m * tc[-m - 1];
-            trPot  += Cv[s] * trig;
-            trDr   += Cr[s] * trig;
-            trDth  += Ct[s] * trig;
-            trDphi += Cv[s] * dtrig;
+            double trig, dtrig;
+            if(m > 0) {
+                pos.advanceTo(m);
+                trig = pos.c; dtrig = -m * pos.s;
+            } else {
+                neg.restart();
+                neg.advanceTo(-m);
+                trig = neg.s; dtrig = -m * neg.c;
+            }
+            double Pm, dmr, dmt;
+            splineSlot<T>(baseA + s * AGX_NODE_STRIDE, baseB + s * AGX_NODE_STRIDE, rowStep,
+                offsetCorner, wy, wx, Pm, dmr, dmt);
+            if(P.logScaling) {
+                dmr = dmr * P0 + Pm * d0r;
+                dmt = dmt * P0 + Pm * d0t;
+                Pm *= P0;
+            }
+            trPot  += Pm * trig;
+            trDr   += dmr * trig;
+            trDth  += dmt * trig;
+            trDphi += Pm * dtrig;
         }
     }
     pot = trPot;
@@ -267,7 +253,7 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const T* 
 }
 
 // PowerLawMultipole::evalCyl: extrapolation inside rmin / outside rmax
-__device__ __noinline__ void powerLawEval(const AgxPowerLawDev& L, double R, double z, double phi,
+__device__ __forceinline__ void powerLawEval(const AgxPowerLawDev& L, double R, double z, double phi,
     double& pot, double& gR, double& gz, double& gphi)
 {
     const double rsq = R * R + z * z, dlogr = log(rsq / L.r0sq) * 0.5;
@@ -317,17 +303,22 @@ __device__ __noinline__ void powerLawEval(const AgxPowerLawDev& L, double R, dou
             gph += Y22 * P22 * -2 * sp;
         }
     } else {
-        // sphHarmTransformInverseDeriv (potential_multipole.cpp:778-821) with the Legendre
-        // recurrence of sphHarmArray (math_sphharm.cpp:63-132) streamed in l
-        const int nm = L.mmax - L.mmin + 1;
-        double Cv[AGX_MAX_SLOTS], Cr[AGX_MAX_SLOTS], Ct[AGX_MAX_SLOTS];
+        // sphHarmTransformInverseDeriv (potential_multipole.cpp:778-821): per harmonic m the
+        // Legendre recurrence of sphHarmArray (math_sphharm.cpp:63-132) streamed in l, then the
+        // Fourier synthesis in the reference's order (m = 0 first, then ascending m)
         const double ct = 2 * tau / (1 + tau * tau), st = (1 - tau * tau) / (1 + tau * tau);
-        for(int mm = 0; mm < nm; mm++) {
-            const int m = mm + L.mmin, lmin = L.lmin[m + L.mmax];
+        TrigRec pos, neg;
+        pos.init(phi);
+        neg.alpha = pos.alpha; neg.beta = pos.beta;
+        vr = gr = gth = 0;
+        for(int pass = 0; pass < 2; pass++)
+        for(int m = (pass == 0 ? 0 : L.mmin); m <= (pass == 0 ? 0 : L.mmax); m++) {
+            if(pass == 1 && m == 0) continue;
+            const int lmin = L.lmin[m + L.mmax];
             if(lmin > L.lmax) continue;
             const double mul = m == 0 ? 2 * 1.77245385090551602729816748334 :
                 2 * 1.77245385090551602729816748334 * M_SQRT2;
-            Cv[mm] = Cr[mm] = Ct[mm] = 0;
+            double Cv = 0, Cr = 0, Ct = 0;
             const int am = m < 0 ? -m : m;
             // legendrePmm
             double fact2m = 1, dfact = 1;
@@ -362,37 +353,19 @@ __device__ __noinline__ void powerLawEval(const AgxPowerLawDev& L, double R, dou
                 if(l >= lmin && (l - lmin) % L.step == 0 && l <= lmax) {
                     double Pv, dPv;
                     term(l, m, Pv, dPv);
-                    Cv[mm] += Pl * Pv * mul;
-                    Cr[mm] += Pl * dPv * mul;
-                    Ct[mm] += dPl * Pv * mul;
+                    Cv += Pl * Pv * mul;
+                    Cr += Pl * dPv * mul;
+                    Ct += dPl * Pv * mul;
                 }
             }
-        }
-        // fourierTransformAzimuth
-        const int z0 = -L.mmin;
-        vr = Cv[z0]; gr = Cr[z0]; gth = Ct[z0];
-        if(L.mmax > 0) {
-            double tc[AGX_MAX_M], ts[AGX_MAX_M];
-            double alpha, beta, sphi, cphi, s1 = 0, c1 = 1;
-            sincosA(phi, sphi, cphi);
-            sincosA(phi / 2, alpha, beta);
-            alpha *= alpha * 2;
-            beta = sphi;
-            for(int k = 0; k < L.mmax; k++) {
-                const double cn = c1 - (alpha * c1 + beta * s1);
-                const double sn = s1 - (alpha * s1 - beta * c1);
-                tc[k] = cn; ts[k] = sn; c1 = cn; s1 = sn;
-            }
-            for(int mm = 0; mm < nm; mm++) {
-                const int m = mm + L.mmin;
-                if(m == 0 || L.lmin[m + L.mmax] > L.lmax) continue;
-                const double trig = m > 0 ? tc[m - 1] : ts[-m - 1];
-                const double dtrig = m > 0 ? -m * ts[m - 1] : -m * tc[-m - 1];
-                vr  += Cv[mm] * trig;
-                gr  += Cr[mm] * trig;
-                gth += Ct[mm] * trig;
-                gph += Cv[mm] * dtrig;
-            }
+            if(m == 0) { vr = Cv; gr = Cr; gth = Ct; continue; }
+            double trig, dtrig;
+            if(m > 0) { pos.advanceTo(m); trig = pos.c; dtrig = -m * pos.s; }
+            else { neg.restart(); neg.advanceTo(-m); trig = neg.s; dtrig = -m * neg.c; }
+            vr  += Cv * trig;
+            gr  += Cr * trig;
+            gth += Ct * trig;
+            gph += Cv * dtrig;
         }
     }
     pot = vr;

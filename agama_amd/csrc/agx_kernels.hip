@@ -24,12 +24,16 @@ using namespace agxd;
 namespace {
 
 // ---------------- batched potential / gradient ----------------
+// The table header lives in device memory (uniform pointer -> scalar loads); passing it by
+// value would make the compiler copy its arrays into per-lane scratch at every launch.
 template<typename T>
-__global__ __launch_bounds__(256) void evalKernel(AgxMultipoleDev P, const T* __restrict__ nodes,
-    long n, const double* __restrict__ pos, double* __restrict__ phi, double* __restrict__ grad)
+__global__ __launch_bounds__(256) void evalKernel(const AgxMultipoleDev* __restrict__ Pp,
+    const T* __restrict__ nodes, long n, const double* __restrict__ pos, double* __restrict__ phi,
+    double* __restrict__ grad)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= n) return;
+    const AgxMultipoleDev& P = *Pp;
     const double x = pos[i * 3], y = pos[i * 3 + 1], z = pos[i * 3 + 2];
     double p, gx, gy, gz;
     multipoleCar<T>(P, nodes, x, y, z, p, gx, gy, gz);
@@ -87,13 +91,32 @@ constexpr double
 enum : int { PH_IDLE = -1, PH_INIT_A = 0, PH_INIT_B, PH_INIT_C, PH_S2, PH_S3, PH_S4, PH_S5, PH_S6,
     PH_S7, PH_S8, PH_S9, PH_S10, PH_S11, PH_S12, PH_FSAL };
 
+// Threads per workgroup of the orbit kernel (fixed: it sizes the LDS block below).
+constexpr int OBS = 256;
+// Wavefronts per SIMD the orbit kernel is compiled for (register budget 512/AGX_ORBIT_WAVES).
+#ifndef AGX_ORBIT_WAVES
+#define AGX_ORBIT_WAVES 2
+#endif
+
+// Per-lane state kept in LDS, field-major with a stride of OBS doubles so that a wavefront
+// touches consecutive addresses: the stage vectors k6..k10 (used in the later stage sums,
+// the error estimate and dense output) and the once-per-step bookkeeping.  Together with the
+// register state this keeps the kernel within 256 VGPRs (two wavefronts per SIMD) while two
+// workgroups share one CU's 160 KB of LDS.
+enum : int { F_K6 = 0, F_K7 = 6, F_K8 = 12, F_K9 = 18, F_K10 = 24, F_H1 = 30, F_ACC = 31, F_E0 = 32,
+    F_MAXDE = 33, F_T0 = 34, F_DTS = 35, F_LASTTB = 36, F_LASTH = 37, F_COUNT = 38 };
+
 struct Lane {
-    long oi;
-    int phase, nbad, S;
-    double x[6], k1[6], k2[6], k3[6], k4[6], k5[6], k6[6], k7[6], k8[6], k9[6], k10[6], y[6];
-    double toff, tb, tend, sgn, h, hnext, preverr, rem, h1, accInit, E0, maxde, dtS, t0;
-    double lastTb, lastH;
-    long nsteps, natt, nev, trajOff, actual;
+    double* sh;                // this lane's LDS column
+    int oi, phase, nbad, S, nsteps, natt, actual;
+    double x[6], k1[6], k2[6], k3[6], k4[6], k5[6], y[6];
+    double toff, tb, tend, h, hnext, preverr, rem;
+    __device__ __forceinline__ double& f(int field) const { return sh[field * OBS]; }
+    __device__ __forceinline__ double& k6(int i) const { return sh[(F_K6 + i) * OBS]; }
+    __device__ __forceinline__ double& k7(int i) const { return sh[(F_K7 + i) * OBS]; }
+    __device__ __forceinline__ double& k8(int i) const { return sh[(F_K8 + i) * OBS]; }
+    __device__ __forceinline__ double& k9(int i) const { return sh[(F_K9 + i) * OBS]; }
+    __device__ __forceinline__ double& k10(int i) const { return sh[(F_K10 + i) * OBS]; }
 };
 
 __device__ __forceinline__ double energyOf(const double* xv, double pot, double Omega)
@@ -109,17 +132,17 @@ __device__ __forceinline__ double denseOut(const Lane& L, int i, double off)
 {
     if(off == L.h) return L.y[i];
     if(off == 0) return L.x[i];
-    const double h = L.h;
+    const double h = L.h, k6 = L.k6(i), k7 = L.k7(i), k8 = L.k8(i), k9 = L.k9(i), k10 = L.k10(i);
     const double r1 = L.x[i], xd = L.y[i] - L.x[i], r2 = xd, xc = h * L.k1[i] - xd, r3 = xc,
         r4 = xd - h * L.k4[i] - xc,
-        r5 = h * (d41 * L.k1[i] + d46 * L.k6[i] + d47 * L.k7[i] + d48 * L.k8[i] + d49 * L.k9[i] +
-             d410 * L.k10[i] + d411 * L.k2[i] + d412 * L.k3[i] + d413 * L.k4[i]),
-        r6 = h * (d51 * L.k1[i] + d56 * L.k6[i] + d57 * L.k7[i] + d58 * L.k8[i] + d59 * L.k9[i] +
-             d510 * L.k10[i] + d511 * L.k2[i] + d512 * L.k3[i] + d513 * L.k4[i]),
-        r7 = h * (d61 * L.k1[i] + d66 * L.k6[i] + d67 * L.k7[i] + d68 * L.k8[i] + d69 * L.k9[i] +
-             d610 * L.k10[i] + d611 * L.k2[i] + d612 * L.k3[i] + d613 * L.k4[i]),
-        r8 = h * (d71 * L.k1[i] + d76 * L.k6[i] + d77 * L.k7[i] + d78 * L.k8[i] + d79 * L.k9[i] +
-             d710 * L.k10[i] + d711 * L.k2[i] + 0 * L.k3[i] + 0 * L.k4[i]);
+        r5 = h * (d41 * L.k1[i] + d46 * k6 + d47 * k7 + d48 * k8 + d49 * k9 +
+             d410 * k10 + d411 * L.k2[i] + d412 * L.k3[i] + d413 * L.k4[i]),
+        r6 = h * (d51 * L.k1[i] + d56 * k6 + d57 * k7 + d58 * k8 + d59 * k9 +
+             d510 * k10 + d511 * L.k2[i] + d512 * L.k3[i] + d513 * L.k4[i]),
+        r7 = h * (d61 * L.k1[i] + d66 * k6 + d67 * k7 + d68 * k8 + d69 * k9 +
+             d610 * k10 + d611 * L.k2[i] + d612 * L.k3[i] + d613 * L.k4[i]),
+        r8 = h * (d71 * L.k1[i] + d76 * k6 + d77 * k7 + d78 * k8 + d79 * k9 +
+             d710 * k10 + d711 * L.k2[i] + 0 * L.k3[i] + 0 * L.k4[i]);
     const double p = off / h, q = 1.0 - p;
     return r1 + p * (r2 + q * (r3 + p * (r4 + q * (r5 + p * (r6 + q * (r7 + p * r8))))));
 }
@@ -142,48 +165,58 @@ __device__ __forceinline__ void storeRow(const AgxOrbitArgs& A, long row, double
     if(A.outTraj) for(int c = 0; c < 6; c++) A.outTraj[row * 6 + c] = p[c];
 }
 
+__device__ __forceinline__ long trajOffsetOf(const AgxOrbitArgs& A, int oi)
+{
+    return A.trajOffset ? A.trajOffset[oi] : 0;
+}
+
 // RuntimeTrajectory::processTimestep (orbit.cpp:15-47) after an accepted step [tb, tb+h]
 __device__ __forceinline__ void sampleStep(const AgxOrbitArgs& A, Lane& L)
 {
     if(L.S < 2) return;   // a single final point is written when the orbit ends
     const double tbegin = L.tb, timestep = L.h;
-    if(L.t0 != L.t0) L.t0 = tbegin;
+    double t0 = L.f(F_T0);
+    if(t0 != t0) { t0 = tbegin; L.f(F_T0) = t0; }
+    const double dtS = L.f(F_DTS);
     const double sign = timestep >= 0 ? +1 : -1, tendS = tbegin + timestep;
-    const double dtr = ROUNDOFF * fmax(fmax(fabs(tendS), fabs(tbegin)), fabs(L.t0));
-    long iout = (long)ceil(sign * (tbegin - L.t0) / L.dtS);
-    const long iend = (long)((sign * (tendS - L.t0) + dtr) / L.dtS);
-    L.actual = iend + 1;
+    const double dtr = ROUNDOFF * fmax(fmax(fabs(tendS), fabs(tbegin)), fabs(t0));
+    long iout = (long)ceil(sign * (tbegin - t0) / dtS);
+    const long iend = (long)((sign * (tendS - t0) + dtr) / dtS);
+    L.actual = (int)min(iend + 1, (long)L.S + 1);
+    if(iout > iend) return;
+    const long base = trajOffsetOf(A, L.oi);
     for(; iout <= iend; iout++) {
-        const double timeout = sign * L.dtS * iout + L.t0;
+        const double timeout = sign * dtS * iout + t0;
         double off = fmin(fmax(sign * (timeout - tbegin), 0.0), sign * timestep) * sign;
         double d[6], o[6];
         for(int c = 0; c < 6; c++) d[c] = denseOut(L, c, off);
         toRotating(A.Omega, A.Omega * (off + tbegin), d, o);
-        if(iout >= 0 && iout < L.S) storeRow(A, L.trajOff + iout, timeout, o);
+        if(iout >= 0 && iout < L.S) storeRow(A, base + iout, timeout, o);
     }
 }
 
 __device__ __forceinline__ void finishOrbit(const AgxOrbitArgs& A, Lane& L, int status)
 {
-    const long oi = L.oi;
-    double fin[6];
+    const int oi = L.oi;
+    const long base = trajOffsetOf(A, oi);
     if(L.S == 1 || A.outFinal) {
-        if(L.actual > 0) {
-            // last recorded point: getSol(h) of the last processed step (or of the zero-length run)
-            toRotating(A.Omega, A.Omega * (L.lastH + L.lastTb), L.x, fin);
-        } else
+        double fin[6];
+        const double lastTb = L.f(F_LASTTB), lastH = L.f(F_LASTH);
+        if(L.actual > 0)   // last recorded point: getSol(h) of the last processed step
+            toRotating(A.Omega, A.Omega * (lastH + lastTb), L.x, fin);
+        else
             for(int c = 0; c < 6; c++) fin[c] = NAN;
-        if(L.S == 1) storeRow(A, L.trajOff, L.actual > 0 ? L.lastTb + L.lastH : NAN, fin);
-        if(A.outFinal) for(int c = 0; c < 6; c++) A.outFinal[oi * 6 + c] = fin[c];
+        if(L.S == 1) storeRow(A, base, L.actual > 0 ? lastTb + lastH : NAN, fin);
+        if(A.outFinal) for(int c = 0; c < 6; c++) A.outFinal[(long)oi * 6 + c] = fin[c];
     }
     for(long r = L.actual > 0 ? L.actual : 0; r < L.S; r++) {
         const double nanrow[6] = { NAN, NAN, NAN, NAN, NAN, NAN };
-        storeRow(A, L.trajOff + r, NAN, nanrow);
+        storeRow(A, base + r, NAN, nanrow);
     }
     if(A.outNsteps) A.outNsteps[oi] = L.nsteps;
     if(A.outNattempts) A.outNattempts[oi] = L.natt;
-    if(A.outNevals) A.outNevals[oi] = L.nev;
-    if(A.outMaxde) A.outMaxde[oi] = L.maxde;
+    if(A.outNevals) A.outNevals[oi] = 3 + 12 * (long)L.natt;   // initialisation + 12 per attempt
+    if(A.outMaxde) A.outMaxde[oi] = L.f(F_MAXDE);
     if(A.outStatus) A.outStatus[oi] = status;
     L.phase = PH_IDLE;
 }
@@ -192,23 +225,23 @@ __device__ __forceinline__ void fetchOrbit(const AgxOrbitArgs& A, Lane& L)
 {
     const unsigned int q = atomicAdd(A.counter, 1u);
     if((long)q >= A.n) { L.oi = -1; L.phase = PH_IDLE; return; }
-    const long oi = A.order ? (long)A.order[q] : (long)q;
+    const int oi = A.order ? A.order[q] : (int)q;
     L.oi = oi;
     L.tb = A.tstart ? A.tstart[oi] : 0.0;
     const double T = A.ttotal[oi];
     L.tend = L.tb + T;
-    L.sgn = T > 0 ? +1 : -1;
     L.S = A.trajsize[oi];
-    L.trajOff = A.trajOffset ? A.trajOffset[oi] : 0;
-    L.dtS = L.S > 1 ? fabs(T) / (L.S - 1) : INFINITY;
-    L.t0 = NAN;
+    L.f(F_DTS) = L.S > 1 ? fabs(T) / (L.S - 1) : INFINITY;
+    L.f(F_T0) = NAN;
+    L.f(F_MAXDE) = 0;
+    L.f(F_LASTTB) = L.tb;
+    L.f(F_LASTH) = 0;
     L.actual = 0;
     L.hnext = 0;
-    L.nsteps = L.natt = L.nev = 0;
-    L.maxde = 0;
-    L.lastTb = L.tb; L.lastH = 0;
+    L.h = 0;
+    L.nsteps = L.natt = 0;
     double ic[6];
-    for(int c = 0; c < 6; c++) ic[c] = A.ic[oi * 6 + c];
+    for(int c = 0; c < 6; c++) ic[c] = A.ic[(long)oi * 6 + c];
     if(A.Omega != 0) {   // orbit.cpp:104-113
         double ca = 1, sa = 0;
         sincosA(A.Omega * L.tb, sa, ca);
@@ -245,10 +278,12 @@ __device__ __forceinline__ void beginStep(const AgxOrbitArgs& A, Lane& L)
 // step accepted (doStep returned h): energy, sampling, advance (orbit.cpp:65-82)
 __device__ __forceinline__ void acceptStep(const AgxOrbitArgs& A, Lane& L, double pot)
 {
-    const double de = fabs(energyOf(L.y, pot, A.Omega) / L.E0 - 1);
-    if(de > L.maxde || de != de) L.maxde = de;
+    const double de = fabs(energyOf(L.y, pot, A.Omega) / L.f(F_E0) - 1);
+    const double mde = L.f(F_MAXDE);
+    if(de > mde || de != de) L.f(F_MAXDE) = de;
     sampleStep(A, L);
-    L.lastTb = L.tb; L.lastH = L.h;
+    L.f(F_LASTTB) = L.tb;
+    L.f(F_LASTH) = L.h;
     if(L.S == 1) L.actual = 1;
     for(int i = 0; i < 6; i++) { L.x[i] = L.y[i]; L.k1[i] = L.k4[i]; }
     L.tb = L.h == L.rem ? L.tend : L.tb + L.h;
@@ -259,10 +294,13 @@ __device__ __forceinline__ void acceptStep(const AgxOrbitArgs& A, Lane& L, doubl
 }
 
 template<typename T>
-__global__ __launch_bounds__(256) void orbitKernel(AgxMultipoleDev P, const T* __restrict__ nodes,
-    AgxOrbitArgs A)
+__global__ __launch_bounds__(OBS) __attribute__((amdgpu_waves_per_eu(AGX_ORBIT_WAVES, AGX_ORBIT_WAVES))) void orbitKernel(const AgxMultipoleDev* __restrict__ Pp,
+    const T* __restrict__ nodes, AgxOrbitArgs A)
 {
+    __shared__ double lds[F_COUNT * OBS];
+    const AgxMultipoleDev& P = *Pp;
     Lane L;
+    L.sh = lds + threadIdx.x;
     fetchOrbit(A, L);
     const double accRel = A.accuracy;
     while(true) {
@@ -280,47 +318,52 @@ __global__ __launch_bounds__(256) void orbitKernel(AgxMultipoleDev P, const T* _
         f[3] = -gx * ca + gy * sa;
         f[4] = -gy * ca - gx * sa;
         f[5] = -gz;
-        L.nev++;
         const double h = L.h;
         switch(L.phase) {
         case PH_INIT_A: {   // DOP853::init + first evaluation of initTimeStep (math_ode.cpp:35-105)
             for(int i = 0; i < 6; i++) L.k1[i] = f[i];
             const double Ekin = 0.5 * (L.x[3] * L.x[3] + L.x[4] * L.x[4] + L.x[5] * L.x[5]);
-            L.accInit = fmin(1., fabs(pot + Ekin) / fmax(fabs(pot), Ekin));
-            L.E0 = energyOf(L.x, pot, A.Omega);
+            L.f(F_ACC) = fmin(1., fabs(pot + Ekin) / fmax(fabs(pot), Ekin));
+            L.f(F_E0) = energyOf(L.x, pot, A.Omega);
             double nx = 0, nd = 0;
             for(int i = 0; i < 6; i++) { nx += L.x[i] * L.x[i]; nd += L.k1[i] * L.k1[i]; }
-            L.h1 = fmax(1e-12, fmin(1e12, sqrt(nx / nd) * 0.01));
-            for(int i = 0; i < 6; i++) L.y[i] = L.x[i] + L.h1 * L.k1[i];
-            L.toff = L.h1;
+            const double h1 = fmax(1e-12, fmin(1e12, sqrt(nx / nd) * 0.01));
+            L.f(F_H1) = h1;
+            for(int i = 0; i < 6; i++) L.y[i] = L.x[i] + h1 * L.k1[i];
+            L.toff = h1;
             L.phase = PH_INIT_B;
             break;
         }
-        case PH_INIT_B:
-            for(int i = 0; i < 6; i++) { L.k2[i] = f[i]; L.y[i] = L.x[i] + 0.5 * L.h1 * (L.k1[i] + L.k2[i]); }
-            L.toff = L.h1;
+        case PH_INIT_B: {
+            const double h1 = L.f(F_H1);
+            for(int i = 0; i < 6; i++) { L.k2[i] = f[i]; L.y[i] = L.x[i] + 0.5 * h1 * (L.k1[i] + L.k2[i]); }
+            L.toff = h1;
             L.phase = PH_INIT_C;
             break;
+        }
         case PH_INIT_C: {
+            const double h1 = L.f(F_H1);
             for(int i = 0; i < 6; i++) L.k3[i] = f[i];
             double h2 = INFINITY;
             for(int i = 0; i < 6; i++) {
                 const double d0 = fmax(0., fmax(fabs(L.x[i]), fabs(L.y[i])));
                 const double d1 = fmax(fabs(L.k1[i]), fabs(L.k3[i]));
-                const double d2 = fabs((3 * L.k2[i] - L.k1[i] - 2 * L.k3[i]) / L.h1);
-                const double d3 = fabs(6 * (L.k3[i] - L.k2[i]) / (L.h1 * L.h1));
+                const double d2 = fabs((3 * L.k2[i] - L.k1[i] - 2 * L.k3[i]) / h1);
+                const double d3 = fabs(6 * (L.k3[i] - L.k2[i]) / (h1 * h1));
                 const double hh = (d0 * d2 + d1 * d1) / (d1 * d3 + d2 * d2);
                 h2 = fmin(h2, hh);
             }
-            L.hnext = fmax(1e-12, fmin(1e12, pow(accRel, 1. / 8) * sqrt(h2))) * L.accInit;
+            L.hnext = fmax(1e-12, fmin(1e12, pow(accRel, 1. / 8) * sqrt(h2))) * L.f(F_ACC);
             // run() (orbit.cpp:52-62): a zero-length or non-finite request records the start
             if(A.ttotal[L.oi] == 0 || !(L.tend > -INFINITY && L.tend < INFINITY)) {
-                L.lastTb = L.tb; L.lastH = 0;
+                L.f(F_LASTTB) = L.tb;
+                L.f(F_LASTH) = 0;
                 if(L.S >= 2) {   // sampling interval 0: the start is recorded twice
                     double o[6];
                     toRotating(A.Omega, A.Omega * L.tb, L.x, o);
-                    storeRow(A, L.trajOff, L.tb, o);
-                    storeRow(A, L.trajOff + 1, L.tb, o);
+                    const long base = trajOffsetOf(A, L.oi);
+                    storeRow(A, base, L.tb, o);
+                    storeRow(A, base + 1, L.tb, o);
                     L.actual = 2;
                 } else
                     L.actual = 1;
@@ -349,50 +392,50 @@ __global__ __launch_bounds__(256) void orbitKernel(AgxMultipoleDev P, const T* _
             L.toff = c6 * h; L.phase = PH_S6; break;
         case PH_S6:
             for(int i = 0; i < 6; i++) {
-                L.k6[i] = f[i];
-                L.y[i] = L.x[i] + h * (a71 * L.k1[i] + a74 * L.k4[i] + a75 * L.k5[i] + a76 * L.k6[i]);
+                L.k6(i) = f[i];
+                L.y[i] = L.x[i] + h * (a71 * L.k1[i] + a74 * L.k4[i] + a75 * L.k5[i] + a76 * f[i]);
             }
             L.toff = c7 * h; L.phase = PH_S7; break;
         case PH_S7:
             for(int i = 0; i < 6; i++) {
-                L.k7[i] = f[i];
-                L.y[i] = L.x[i] + h * (a81 * L.k1[i] + a84 * L.k4[i] + a85 * L.k5[i] + a86 * L.k6[i] +
-                    a87 * L.k7[i]);
+                L.k7(i) = f[i];
+                L.y[i] = L.x[i] + h * (a81 * L.k1[i] + a84 * L.k4[i] + a85 * L.k5[i] + a86 * L.k6(i) +
+                    a87 * f[i]);
             }
             L.toff = c8 * h; L.phase = PH_S8; break;
         case PH_S8:
             for(int i = 0; i < 6; i++) {
-                L.k8[i] = f[i];
-                L.y[i] = L.x[i] + h * (a91 * L.k1[i] + a94 * L.k4[i] + a95 * L.k5[i] + a96 * L.k6[i] +
-                    a97 * L.k7[i] + a98 * L.k8[i]);
+                L.k8(i) = f[i];
+                L.y[i] = L.x[i] + h * (a91 * L.k1[i] + a94 * L.k4[i] + a95 * L.k5[i] + a96 * L.k6(i) +
+                    a97 * L.k7(i) + a98 * f[i]);
             }
             L.toff = c9 * h; L.phase = PH_S9; break;
         case PH_S9:
             for(int i = 0; i < 6; i++) {
-                L.k9[i] = f[i];
-                L.y[i] = L.x[i] + h * (a101 * L.k1[i] + a104 * L.k4[i] + a105 * L.k5[i] + a106 * L.k6[i] +
-                    a107 * L.k7[i] + a108 * L.k8[i] + a109 * L.k9[i]);
+                L.k9(i) = f[i];
+                L.y[i] = L.x[i] + h * (a101 * L.k1[i] + a104 * L.k4[i] + a105 * L.k5[i] + a106 * L.k6(i) +
+                    a107 * L.k7(i) + a108 * L.k8(i) + a109 * f[i]);
             }
             L.toff = c10 * h; L.phase = PH_S10; break;
         case PH_S10:
             for(int i = 0; i < 6; i++) {
-                L.k10[i] = f[i];
-                L.y[i] = L.x[i] + h * (a111 * L.k1[i] + a114 * L.k4[i] + a115 * L.k5[i] + a116 * L.k6[i] +
-                    a117 * L.k7[i] + a118 * L.k8[i] + a119 * L.k9[i] + a1110 * L.k10[i]);
+                L.k10(i) = f[i];
+                L.y[i] = L.x[i] + h * (a111 * L.k1[i] + a114 * L.k4[i] + a115 * L.k5[i] + a116 * L.k6(i) +
+                    a117 * L.k7(i) + a118 * L.k8(i) + a119 * L.k9(i) + a1110 * f[i]);
             }
             L.toff = c11 * h; L.phase = PH_S11; break;
         case PH_S11:   // k11 -> k2 registers
             for(int i = 0; i < 6; i++) {
                 L.k2[i] = f[i];
-                L.y[i] = L.x[i] + h * (a121 * L.k1[i] + a124 * L.k4[i] + a125 * L.k5[i] + a126 * L.k6[i] +
-                    a127 * L.k7[i] + a128 * L.k8[i] + a129 * L.k9[i] + a1210 * L.k10[i] + a1211 * L.k2[i]);
+                L.y[i] = L.x[i] + h * (a121 * L.k1[i] + a124 * L.k4[i] + a125 * L.k5[i] + a126 * L.k6(i) +
+                    a127 * L.k7(i) + a128 * L.k8(i) + a129 * L.k9(i) + a1210 * L.k10(i) + a1211 * f[i]);
             }
             L.toff = h; L.phase = PH_S12; break;
         case PH_S12:   // k12 -> k3 registers, b-sum -> k5 registers
             for(int i = 0; i < 6; i++) {
                 L.k3[i] = f[i];
-                L.k5[i] = b1 * L.k1[i] + b6 * L.k6[i] + b7 * L.k7[i] + b8 * L.k8[i] + b9 * L.k9[i] +
-                    b10 * L.k10[i] + b11 * L.k2[i] + b12 * L.k3[i];
+                L.k5[i] = b1 * L.k1[i] + b6 * L.k6(i) + b7 * L.k7(i) + b8 * L.k8(i) + b9 * L.k9(i) +
+                    b10 * L.k10(i) + b11 * L.k2[i] + b12 * f[i];
                 L.y[i] = L.x[i] + h * L.k5[i];
             }
             L.toff = h; L.phase = PH_FSAL; break;
@@ -404,9 +447,9 @@ __global__ __launch_bounds__(256) void orbitKernel(AgxMultipoleDev P, const T* _
             for(int i = 0; i < 6; i++) {
                 const double sk = (0. + accRel * fmax(fabs(L.x[i]), fabs(L.y[i]))) * accFac;
                 if(sk == 0) continue;
-                const double e3 = (L.k5[i] - bhh1 * L.k1[i] - bhh2 * L.k9[i] - bhh3 * L.k3[i]) / sk;
-                const double e5 = (er1 * L.k1[i] + er6 * L.k6[i] + er7 * L.k7[i] + er8 * L.k8[i] +
-                    er9 * L.k9[i] + er10 * L.k10[i] + er11 * L.k2[i] + er12 * L.k3[i]) / sk;
+                const double e3 = (L.k5[i] - bhh1 * L.k1[i] - bhh2 * L.k9(i) - bhh3 * L.k3[i]) / sk;
+                const double e5 = (er1 * L.k1[i] + er6 * L.k6(i) + er7 * L.k7(i) + er8 * L.k8(i) +
+                    er9 * L.k9(i) + er10 * L.k10(i) + er11 * L.k2[i] + er12 * L.k3[i]) / sk;
                 err3 += e3 * e3;
                 err5 += e5 * e5;
             }
@@ -444,20 +487,23 @@ __global__ __launch_bounds__(256) void orbitKernel(AgxMultipoleDev P, const T* _
 }  // namespace
 
 // ---------------- launchers (called from agx_capi.hip) ----------------
-hipError_t agxLaunchEval(const AgxMultipoleDev& P, int precision, long n, const double* pos,
-    double* phi, double* grad, hipStream_t stream)
+// P: host copy of the header (for the table pointers), dP: the same header in device memory
+hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, long n,
+    const double* pos, double* phi, double* grad, hipStream_t stream)
 {
     if(n <= 0) return hipSuccess;
     const int block = 256;
     const long grid = (n + block - 1) / block;
     if(precision == AGX_FP32_TABLES)
-        hipLaunchKernelGGL(evalKernel<float>, dim3((unsigned)grid), dim3(block), 0, stream, P, P.nodesF,
+        hipLaunchKernelGGL(evalKernel<float>, dim3((unsigned)grid), dim3(block), 0, stream, dP, P.nodesF,
             n, pos, phi, grad);
     else
-        hipLaunchKernelGGL(evalKernel<double>, dim3((unsigned)grid), dim3(block), 0, stream, P, P.nodes,
+        hipLaunchKernelGGL(evalKernel<double>, dim3((unsigned)grid), dim3(block), 0, stream, dP, P.nodes,
             n, pos, phi, grad);
     return hipGetLastError();
 }
+
+int agxOrbitKernelBlockSize() { return OBS; }
 
 int agxOrbitKernelBlocksPerCU(int precision, int blockSize)
 {
@@ -468,13 +514,13 @@ int agxOrbitKernelBlocksPerCU(int precision, int blockSize)
     return e == hipSuccess ? nb : 1;
 }
 
-hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, int precision, const AgxOrbitArgs& A,
-    int numBlocks, int blockSize, hipStream_t stream)
+hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision,
+    const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream)
 {
     if(A.n <= 0) return hipSuccess;
     if(precision == AGX_FP32_TABLES)
-        hipLaunchKernelGGL(orbitKernel<float>, dim3(numBlocks), dim3(blockSize), 0, stream, P, P.nodesF, A);
+        hipLaunchKernelGGL(orbitKernel<float>, dim3(numBlocks), dim3(blockSize), 0, stream, dP, P.nodesF, A);
     else
-        hipLaunchKernelGGL(orbitKernel<double>, dim3(numBlocks), dim3(blockSize), 0, stream, P, P.nodes, A);
+        hipLaunchKernelGGL(orbitKernel<double>, dim3(numBlocks), dim3(blockSize), 0, stream, dP, P.nodes, A);
     return hipGetLastError();
 }

@@ -118,8 +118,9 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
     long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags,
     void* stream);
 
-/* Launch geometry of the orbit kernel (workgroups, threads per workgroup); 0 keeps the
-   default.  Set by the autotuner (agama_amd.tune). */
+/* Launch geometry of the orbit kernel: number of workgroups (0 = one full wave of resident
+   workgroups on every CU) and threads per workgroup (0 or 256, the only size it is built for).
+   Set by the autotuner (agama_amd.tune). */
 int agx_set_orbit_launch(int num_blocks, int block_size);
 
 /* ---- NEMO acceleration plugin (src/interface_nemo.cpp:47-60, 114-147, 180-220) ----

@@ -1555,24 +1555,29 @@ void integrateOneOrbit(const orc_pot& pot, const double ic[6], double tstart, do
     stepper.init(posvel);
     double E0 = energyOf(posvel, sys.lastEpot, Omega);
     res.maxde = 0;
-    auto getSol = [&](double timeOffset, double out[6]) {   // orbit.cpp:119-133
+    // getSolNative (orbit.cpp:119-133) at an offset from the step that began at tbegin
+    auto getSolAt = [&](double timeOffset, double tbegin, double out[6]) {
         double data[6];
         for(int i=0; i<6; i++)
             data[i] = stepper.getSol(timeOffset, i);
         if(Omega) {
             double ca=1, sa=0;
-            sincos_r(Omega * (timeOffset + sys.timeBegin), sa, ca);
+            sincos_r(Omega * (timeOffset + tbegin), sa, ca);
             out[0] = data[0]*ca + data[1]*sa; out[1] = data[1]*ca - data[0]*sa; out[2] = data[2];
             out[3] = data[3]*ca + data[4]*sa; out[4] = data[4]*ca - data[3]*sa; out[5] = data[5];
         } else
             for(int i=0; i<6; i++) out[i] = data[i];
     };
+    auto getSol = [&](double timeOffset, double out[6]) { getSolAt(timeOffset, sys.timeBegin, out); };
+    double lastTb = NAN, lastH = 0;
     auto store = [&](long idx, double t, const double* p) {
         if(idx < 0 || idx >= trajsize) return;
         if(out_time) out_time[idx] = t;
         if(out_traj) for(int c=0; c<6; c++) out_traj[idx*6+c] = p[c];
     };
     auto processTimestep = [&](double tbegin, double timestep) {   // orbit.cpp:15-47
+        lastTb = tbegin;
+        lastH = timestep;
         if(t0!=t0)
             t0 = tbegin;
         double pt[6];
@@ -1628,7 +1633,11 @@ void integrateOneOrbit(const orc_pot& pot, const double ic[6], double tstart, do
             }
         }
     }
-    getSol(timeStep, res.fin);
+    // last recorded point, i.e. the trajectory value of a trajsize=1 request
+    if(lastTb == lastTb)
+        getSolAt(lastH, lastTb, res.fin);
+    else
+        for(int c=0; c<6; c++) res.fin[c] = NAN;
     // storeOrbitArrays (interface_python.cpp:6921-6959): NaN beyond the recorded size
     for(long idx = std::max<long>(actualSize, 0); idx < trajsize; idx++) {
         if(out_time) out_time[idx] = NAN;

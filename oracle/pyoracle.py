@@ -1,0 +1,140 @@
+"""ctypes wrapper of the CPU oracle (oracle/build/liboracle*.so) - TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker or the
+timed CPU baseline; the agama_amd product never imports it."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_vp = ctypes.c_void_p
+
+
+def build():
+    """compile the oracle (and oracle/_ref when the reference tree is present)"""
+    subprocess.check_call(["make", "-s", "-C", HERE])
+    subprocess.check_call(["sh", os.path.join(HERE, "ref_build.sh")])
+
+
+def _load(name):
+    path = os.path.join(HERE, "build", name)
+    if not os.path.exists(path):
+        raise ImportError(f"oracle library missing: {path} (run make -C oracle)")
+    lib = ctypes.CDLL(path)
+    sig = {
+        "orc_last_error": (ctypes.c_char_p, []),
+        "orc_multipole_from_file": (_vp, [ctypes.c_char_p]),
+        "orc_multipole_from_arrays": (_vp, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
+        "orc_free": (None, [_vp]),
+        "orc_multipole_tables": (ctypes.c_long, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
+        "orc_eval_many": (None, [_vp, ctypes.c_long, _vp, ctypes.c_double, _vp, _vp, ctypes.c_int]),
+        "orc_orbit_many": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp,
+                                          _vp, _vp, ctypes.c_int]),
+        "orc_ode_dop853": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                          _vp, ctypes.c_int, _vp, _vp, _vp]),
+        "orc_ode_dop853_fixed": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
+                                                _vp, _vp]),
+        "orc_sincos": (None, [ctypes.c_double, _vp, _vp]),
+        "orc_atan2": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+        "orc_binsearch": (ctypes.c_long, [ctypes.c_double, _vp, ctypes.c_long]),
+        "orc_lambertw": (ctypes.c_double, [ctypes.c_double, ctypes.c_int]),
+        "orc_gl_nodes": (None, [ctypes.c_int, _vp, _vp]),
+        "orc_sphharm_array": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _vp]),
+        "orc_trig_multi_angle": (None, [ctypes.c_double, ctypes.c_int, ctypes.c_int, _vp]),
+        "orc_spline2d_eval": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                             ctypes.c_long, _vp, _vp, _vp, _vp]),
+    }
+    for n, (res, args) in sig.items():
+        f = getattr(lib, n)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load("liboracle.so")          # strict build: the parity checker
+lib_fast = None                      # reference optimisation flags: the timed CPU baseline
+
+
+def fast():
+    global lib_fast
+    if lib_fast is None:
+        lib_fast = _load("liboracle_fast.so")
+    return lib_fast
+
+
+RHS = ctypes.CFUNCTYPE(None, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                       ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class OraclePotential:
+    def __init__(self, coef_file=None, coefs=None, library=None):
+        self.lib = library or lib
+        if coef_file is not None:
+            self.h = self.lib.orc_multipole_from_file(str(coef_file).encode())
+        else:
+            radii, Phi, dPhi = (np.ascontiguousarray(a, dtype=np.float64) for a in coefs)
+            lmax = int(round(np.sqrt(Phi.shape[0]))) - 1
+            self.h = self.lib.orc_multipole_from_arrays(len(radii), lmax, _p(radii), _p(Phi), _p(dPhi))
+        if not self.h:
+            raise RuntimeError("oracle: " + self.lib.orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_free(self.h)
+
+    def tables(self, which, m=0):
+        n = self.lib.orc_multipole_tables(self.h, which, m, None)
+        out = np.empty(max(n, 0))
+        if n > 0:
+            self.lib.orc_multipole_tables(self.h, which, m, _p(out))
+        return out
+
+    def eval(self, points, nthreads=0):
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        phi = np.empty(len(pts))
+        grad = np.empty((len(pts), 3))
+        self.lib.orc_eval_many(self.h, len(pts), _p(pts), 0.0, _p(phi), _p(grad), nthreads)
+        return phi, grad
+
+    def orbits(self, ic, ttotal, trajsize=1, tstart=None, accuracy=1e-8, Omega=0.0,
+               maxNumSteps=int(1e8), nthreads=0):
+        ic = np.ascontiguousarray(ic, dtype=np.float64).reshape(-1, 6)
+        n = len(ic)
+        tt = np.ascontiguousarray(np.broadcast_to(np.asarray(ttotal, dtype=np.float64), (n,)))
+        sz = np.ascontiguousarray(np.broadcast_to(np.asarray(trajsize, dtype=np.int32), (n,)))
+        ts = None if tstart is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(tstart, dtype=np.float64), (n,)))
+        offs = np.concatenate([[0], np.cumsum(sz)[:-1]]).astype(np.int64)
+        rows = int(sz.sum())
+        out = dict(time=np.empty(rows), traj=np.empty((rows, 6)), final=np.empty((n, 6)),
+                   nsteps=np.empty(n, np.int64), nattempts=np.empty(n, np.int64),
+                   nevals=np.empty(n, np.int64), maxde=np.empty(n), status=np.empty(n, np.int32))
+        rc = self.lib.orc_orbit_many(self.h, n, _p(ic), _p(ts), _p(tt), _p(sz), _p(offs), accuracy, Omega,
+                                     maxNumSteps, _p(out["time"]), _p(out["traj"]), _p(out["final"]),
+                                     _p(out["nsteps"]), _p(out["nattempts"]), _p(out["nevals"]),
+                                     _p(out["maxde"]), _p(out["status"]), nthreads)
+        if rc:
+            raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
+        out["offsets"] = offs
+        return out
+
+
+def ref_lib():
+    """the reference's own DOP853 (oracle/_ref), or None when it was not built"""
+    path = os.path.join(HERE, "_ref", "libref_dop853.so")
+    if not os.path.exists(path):
+        return None
+    r = ctypes.CDLL(path)
+    r.ref_ode_dop853.restype = ctypes.c_int
+    r.ref_ode_dop853.argtypes = [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _vp,
+                                 ctypes.c_int, _vp, _vp, _vp]
+    r.ref_ode_dop853_fixed.restype = ctypes.c_int
+    r.ref_ode_dop853_fixed.argtypes = [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _vp, _vp]
+    return r

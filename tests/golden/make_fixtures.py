@@ -1,0 +1,39 @@
+"""Writes the coefficient-file fixtures used by the parity tests.
+
+The coefficients are produced by agama_amd's host-side Poisson construction (no GPU needed,
+agx_build_coef_file) from the density profiles of the workloads; both the GPU path and the
+CPU oracle then load the same file, which is the parity boundary for evaluation and
+integration.  Run: python tests/golden/make_fixtures.py"""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from agama_amd import lib, _native  # noqa: E402
+
+FIXTURES = {
+    # triaxial Dehnen gamma=1, axis ratios 1:0.8:0.6, lmax=8, 50 radial nodes in [1e-3, 1e3]
+    "dehnen_triax_lmax8.coef": "type=Multipole, density=Dehnen, gamma=1, mass=1, scaleRadius=1, "
+                               "axisRatioY=0.8, axisRatioZ=0.6, lmax=8, gridSizeR=50, rmin=1e-3, rmax=1e3",
+    # spherical Plummer (closed-form potential -M/sqrt(r^2+a^2)) expanded with lmax=4
+    "plummer_lmax4.coef": "type=Multipole, density=Plummer, mass=1, scaleRadius=1, lmax=4, "
+                          "gridSizeR=40, rmin=1e-2, rmax=1e2, symmetry=Triaxial",
+    # axisymmetric composite: exponential disk + Hernquist bulge + NFW halo, lmax=12, 80 nodes
+    "disk_bulge_halo_lmax12.coef": "type=Multipole, density=Disk+Dehnen+NFW, lmax=12, gridSizeR=80, "
+                                   "rmin=1e-2, rmax=1e3, 0.mass=0.05, 0.scaleRadius=3, 0.scaleHeight=0.3, "
+                                   "1.mass=0.01, 1.scaleRadius=0.5, 1.gamma=1, 2.mass=1, 2.scaleRadius=20",
+}
+
+
+def main():
+    for name, params in FIXTURES.items():
+        path = os.path.join(HERE, name)
+        rc = lib.agx_build_coef_file(params.encode(), path.encode())
+        if rc:
+            raise RuntimeError(f"{name}: {_native.last_error()}")
+        print("wrote", path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    main()

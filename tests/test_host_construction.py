@@ -1,0 +1,116 @@
+"""CPU tests of the product's host side: the Multipole construction (Poisson solver ->
+coefficients, coefficients -> spline tables) and the C ABI surface.  No GPU needed."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from agama_amd import _native, lib
+from oracle.pyoracle import OraclePotential
+from tests.conftest import GOLDEN, ROOT
+
+FIXTURES = ["dehnen_triax_lmax8.coef", "disk_bulge_halo_lmax12.coef"]
+
+
+def product_tables(path, which, m=0):
+    n = lib.agx_debug_tables(path.encode(), which, m, None)
+    assert n >= 0, _native.last_error()
+    out = np.empty(n)
+    lib.agx_debug_tables(path.encode(), which, m, out.ctypes.data)
+    return out
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_tables_bitwise_equal_to_oracle(name):
+    """MultipoleInterp2d construction (potential_multipole.cpp:1854-1967): grids, scaling
+    constants and all nine node arrays of every harmonic agree bit for bit"""
+    path = os.path.join(GOLDEN, name)
+    orc = OraclePotential(path)
+    for which in (0, 1, 3):
+        assert np.array_equal(product_tables(path, which), orc.tables(which))
+    scal = orc.tables(3)
+    mmax = int(scal[3])
+    nonempty = 0
+    for m in range(-mmax, mmax + 1):
+        a, b = product_tables(path, 2, m), orc.tables(2, m)
+        assert np.array_equal(a, b), m
+        nonempty += a.size > 0
+    assert nonempty >= 1
+
+
+def read_coefs(path):
+    lines = open(path).read().splitlines()
+    i = lines.index("#Phi")
+    nr = int([ln for ln in lines if ln.startswith("gridSizeR=")][0].split("=")[1])
+    phi = np.array([[float(v) for v in ln.split()] for ln in lines[i + 2:i + 2 + nr]])
+    j = lines.index("#dPhi/dr")
+    dphi = np.array([[float(v) for v in ln.split()] for ln in lines[j + 2:j + 2 + nr]])
+    return phi[:, 0], phi[:, 1:].T, dphi[:, 1:].T
+
+
+def test_poisson_plummer_closed_form():
+    """spherical Plummer: Phi_00(r) = -M/sqrt(r^2+a^2), dPhi/dr = M r/(r^2+a^2)^1.5"""
+    r, P, dP = read_coefs(os.path.join(GOLDEN, "plummer_lmax4.coef"))
+    assert P.shape[0] == 1                  # spherical symmetry keeps only l=0
+    assert np.max(np.abs(P[0] / (-1 / np.sqrt(r ** 2 + 1)) - 1)) < 1e-9
+    assert np.max(np.abs(dP[0] / (r / (r ** 2 + 1) ** 1.5) - 1)) < 1e-8
+
+
+def test_poisson_hernquist_closed_form(tmp_path):
+    """spherical Hernquist (Dehnen gamma=1): Phi = -M/(r+a)"""
+    path = str(tmp_path / "h.coef")
+    assert lib.agx_build_coef_file(b"type=Multipole, density=Dehnen, gamma=1, mass=2, scaleRadius=1.5, "
+                                   b"lmax=0, gridSizeR=40, rmin=1e-3, rmax=1e3", path.encode()) == 0
+    r, P, dP = read_coefs(path)
+    assert np.max(np.abs(P[0] / (-2 / (r + 1.5)) - 1)) < 1e-6
+    assert np.max(np.abs(dP[0] / (2 / (r + 1.5) ** 2) - 1)) < 1e-5
+
+
+def test_triaxial_symmetry_pattern():
+    """triaxial symmetry: only even l and even m >= 0 are non-zero (math_sphharm.h:76-114)"""
+    r, P, dP = read_coefs(os.path.join(GOLDEN, "dehnen_triax_lmax8.coef"))
+    assert P.shape[0] == 81
+    for c in range(81):
+        l = int(np.sqrt(c))
+        m = c - l * (l + 1)
+        nz = np.any(P[c] != 0) or np.any(dP[c] != 0)
+        assert nz == (l % 2 == 0 and m >= 0 and m % 2 == 0), (l, m)
+
+
+def test_coefficient_file_roundtrip(tmp_path):
+    """file -> construction -> file reproduces the coefficients exactly (17 digits)"""
+    src = os.path.join(GOLDEN, "dehnen_triax_lmax8.coef")
+    a = read_coefs(src)
+    dst = str(tmp_path / "rt.coef")
+    params = ("type=Multipole, density=Dehnen, gamma=1, mass=1, scaleRadius=1, axisRatioY=0.8, "
+              "axisRatioZ=0.6, lmax=8, gridSizeR=50, rmin=1e-3, rmax=1e3")
+    assert lib.agx_build_coef_file(params.encode(), dst.encode()) == 0
+    b = read_coefs(dst)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_invalid_parameters_report_errors():
+    assert lib.agx_build_coef_file(b"type=Multipole, density=Nonsense", b"/tmp/x") != 0
+    assert "unknown density" in _native.last_error()
+    assert lib.agx_build_coef_file(b"type=Multipole, density=Dehnen, rmin=1, rmax=0.5", b"/tmp/x") != 0
+    assert lib.agx_build_coef_file(b"type=Multipole, density=Dehnen, bogus=3, rmin=1, rmax=5", b"/tmp/x") != 0
+    assert "unknown parameter" in _native.last_error()
+    assert lib.agx_debug_tables(b"/nonexistent.coef", 0, 0, None) == -1
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "agama_amd.h")).read()
+    header = re.sub(r"/\*.*?\*/", "", header, flags=re.S)
+    names = set(re.findall(r"\b([A-Za-z_]\w*)\s*\(", header)) - {"if", "defined", "sizeof"}
+    names = {n for n in names if n.startswith(("agx_", "agama_")) or n == "iniacceleration"}
+    names.discard("agx_acc_pter")
+    assert "agx_orbit_batch" in names and "agama_evalPotential" in names
+    out = subprocess.check_output(["nm", "-D", "--defined-only", _native.LIB_PATH]).decode()
+    exported = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    missing = sorted(names - exported)
+    assert not missing, missing
+    assert set(_native.EXPORTED) <= exported
