@@ -21,6 +21,9 @@ hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, i
     const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream);
 int agxOrbitKernelBlocksPerCU(int precision, int blockSize);
 int agxOrbitKernelBlockSize();
+size_t agxScheduleWorkspace(long n);
+hipError_t agxLaunchSchedule(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, long n,
+    const double* ic, const double* ttotal, int* order, void* work, hipStream_t stream);
 
 struct agx_potential {
     agx::Coefs coefs;
@@ -332,8 +335,26 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
     if(!(accuracy > 0)) return fail(AGX_EINVAL, "accuracy must be positive");
     if(!(Omega > -INFINITY && Omega < INFINITY)) return fail(AGX_EINVAL, "Omega must be finite");
     if(maxNumSteps <= 0) return fail(AGX_EINVAL, "maxNumSteps must be positive");
-    if(flags != AGX_PRECISION_FP64 && flags != AGX_PRECISION_FP32) return fail(AGX_EINVAL, "invalid flags");
+    const int precision = flags & AGX_PRECISION_MASK;
+    if((precision != AGX_PRECISION_FP64 && precision != AGX_PRECISION_FP32) ||
+        (flags & ~(AGX_PRECISION_MASK | AGX_ORDER_AS_GIVEN)))
+        return fail(AGX_EINVAL, "invalid flags");
     try {
+        hipStream_t st = (hipStream_t)stream;
+        void* work = nullptr;
+        if(!d_order && !(flags & AGX_ORDER_AS_GIVEN)) {
+            // longest expected integrations first (agx_schedule.hip); stream-ordered workspace
+            const size_t wb = agxScheduleWorkspace(n) + ((size_t)n * 4 + 255) / 256 * 256;
+            HIPCHECK(hipMallocAsync(&work, wb, st));
+            int* ord = reinterpret_cast<int*>(static_cast<char*>(work) + agxScheduleWorkspace(n));
+            hipError_t e = agxLaunchSchedule(pot->dev, pot->ddev, n, d_ic, d_ttotal, ord,
+                work, st);
+            if(e != hipSuccess) {
+                (void)hipFreeAsync(work, st);
+                return fail(AGX_EDEVICE, std::string("schedule: ") + hipGetErrorString(e));
+            }
+            d_order = ord;
+        }
         AgxOrbitArgs A;
         A.n = n; A.ic = d_ic; A.tstart = d_tstart; A.ttotal = d_ttotal; A.trajsize = d_trajsize;
         A.trajOffset = d_traj_offset; A.order = d_order; A.accuracy = accuracy; A.Omega = Omega;
@@ -341,9 +362,10 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
         A.outNsteps = d_nsteps; A.outNattempts = d_nattempts; A.outNevals = d_nevals;
         A.outMaxde = d_maxde; A.outStatus = d_status; A.counter = pot->dcounter;
         int blocks, bs;
-        orbitGeometry(n, flags, blocks, bs);
-        HIPCHECK(hipMemsetAsync(pot->dcounter, 0, sizeof(unsigned int), (hipStream_t)stream));
-        hipError_t e = agxLaunchOrbit(pot->dev, pot->ddev, flags, A, blocks, bs, (hipStream_t)stream);
+        orbitGeometry(n, precision, blocks, bs);
+        HIPCHECK(hipMemsetAsync(pot->dcounter, 0, sizeof(unsigned int), st));
+        hipError_t e = agxLaunchOrbit(pot->dev, pot->ddev, precision, A, blocks, bs, st);
+        if(work) HIPCHECK(hipFreeAsync(work, st));
         if(e != hipSuccess) return fail(AGX_EDEVICE, std::string("orbit launch: ") + hipGetErrorString(e));
         return AGX_OK;
     } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }

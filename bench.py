@@ -1,0 +1,162 @@
+"""Benchmark: orbit-steps/s of the batched FP64 DOP853 integrator in the triaxial Dehnen
+Multipole potential (lmax=8) on one MI355X.
+
+Workload ("batching at scale"): 1M orbits with initial conditions drawn from the triaxial
+Dehnen density (seed 42), integration times log-uniform between 1 and 300 circular periods,
+accuracy 1e-8, final state plus max relative energy drift per orbit, FP64 throughout.
+One bench step = one persistent-kernel launch that integrates the whole batch; inputs and
+outputs stay resident in HBM.  `value` = accepted orbit-steps / s over the timed steps.
+
+Prints one JSON line (rank 0).  `--gpus N` with N > 1 is not supported (single-GPU build).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "orbit-steps/s at 1M orbits: MLP surrogate vs FP64 Multipole at ΔE/E≤1e-6"
+
+# Algorithmic FP64 operations (+ - * / sqrt log exp each count 1) of the GPU path, counted
+# from the kernel's arithmetic for the triaxial lmax=8 expansion (5 harmonics), Omega = 0:
+#   per right-hand-side evaluation: Cartesian->cylindrical incl. atan2 38, branch test 3,
+#     interior setup 18, quintic weights 2x38, 5 harmonics x 229 (2-D quintic spline),
+#     4 x 16 (rescaling + Fourier sums), cos/sin recurrence 146, m=0 unscaling 7,
+#     chain rule 19, frame rotation 15                                     -> 1531
+#   per step attempt besides the 12 evaluations: stage sums 666, b-sum 102, error norm and
+#     step control 200, energy bookkeeping 10                             -> 978
+#   per orbit: 3 initial evaluations + first-step estimate ~80
+FLOPS_PER_EVAL = 1531
+FLOPS_PER_ATTEMPT_EXTRA = 978
+FLOPS_PER_ORBIT_INIT = 3 * FLOPS_PER_EVAL + 80
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix), AMD published figure
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--orbits", type=int, default=1_000_000)
+    ap.add_argument("--precision", choices=["fp64", "fp32tables"], default="fp64")
+    ap.add_argument("--cpu-orbits", type=int, default=20_000,
+                    help="size of the CPU-baseline sample (0 disables)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build targets one GPU; --gpus N with N > 1 is not supported "
+              "(orbits are independent and would shard by index, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import agama_amd
+    from agama_amd import lib, workloads
+
+    dev = torch.device("cuda:0")
+    coef = os.path.join(ROOT, "tests", "golden", "dehnen_triax_lmax8.coef")
+    pot = agama_amd.Potential(file=coef)
+    n = args.orbits
+    ic, T = workloads.triaxial_dehnen_orbits(n, lambda p: pot.potential(p), pot.coefs(), seed=42, uneven=True)
+    prec = agama_amd.PRECISION_FP64 if args.precision == "fp64" else agama_amd.PRECISION_FP32
+
+    d_ic = torch.from_numpy(ic).to(dev)
+    d_T = torch.from_numpy(T).to(dev)
+    d_sz = torch.ones(n, dtype=torch.int32, device=dev)
+    d_off = torch.arange(n, dtype=torch.int64, device=dev)
+    d_time = torch.empty(n, dtype=torch.float64, device=dev)
+    d_fin = torch.empty((n, 6), dtype=torch.float64, device=dev)
+    d_ns = torch.empty(n, dtype=torch.int64, device=dev)
+    d_na = torch.empty(n, dtype=torch.int64, device=dev)
+    d_ne = torch.empty(n, dtype=torch.int64, device=dev)
+    d_de = torch.empty(n, dtype=torch.float64, device=dev)
+    d_st = torch.empty(n, dtype=torch.int32, device=dev)
+    stream = torch.cuda.Stream()
+
+    def launch():
+        rc = lib.agx_orbit_batch_device(pot.handle, n, d_ic.data_ptr(), None, d_T.data_ptr(), d_sz.data_ptr(),
+                                        d_off.data_ptr(), None, 1e-8, 0.0, int(1e8), d_time.data_ptr(), None,
+                                        d_fin.data_ptr(), d_ns.data_ptr(), d_na.data_ptr(), d_ne.data_ptr(),
+                                        d_de.data_ptr(), d_st.data_ptr(), prec, stream.cuda_stream)
+        if rc:
+            raise RuntimeError(agama_amd._native.last_error())
+
+    with torch.cuda.stream(stream):
+        for _ in range(args.warmup):
+            launch()
+            print(f"warmup step done", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for k in range(args.steps):
+            launch()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    kernel_s = ev0.elapsed_time(ev1) / 1e3
+    ns = d_ns.cpu().numpy()
+    na = d_na.cpu().numpy()
+    de = d_de.cpu().numpy()
+    st = d_st.cpu().numpy()
+    steps_per_launch = int(ns.sum())
+    attempts = int(na.sum())
+    flops = attempts * (12 * FLOPS_PER_EVAL + FLOPS_PER_ATTEMPT_EXTRA) + n * FLOPS_PER_ORBIT_INIT
+    per_launch = kernel_s / args.steps
+    value = steps_per_launch * args.steps / wall
+    achieved = flops / per_launch / 1e12
+    print(f"orbits {n}: {steps_per_launch} accepted steps, {attempts} attempts per launch; "
+          f"kernel {per_launch*1e3:.1f} ms/launch, wall {wall:.2f} s; status counts {np.bincount(st)}; "
+          f"median/max dE/E {np.median(de):.2e}/{np.max(de):.2e}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if args.cpu_orbits > 0:
+        from oracle import pyoracle
+        threads = args.cpu_threads or os.cpu_count()
+        fast = pyoracle.OraclePotential(coef, library=pyoracle.fast())
+        m = min(args.cpu_orbits, n)
+        t0 = time.perf_counter()
+        r = fast.orbits(ic[:m], T[:m], trajsize=1, accuracy=1e-8, nthreads=threads)
+        dt = time.perf_counter() - t0
+        cpu = {"value": float(r["nsteps"].sum() / dt), "unit": "orbit-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"first {m} of the {n} orbits (same seeded ICs and times), C++ OpenMP "
+                         f"restatement built with -O2 (oracle/build/liboracle_fast.so), "
+                         f"{dt:.1f} s on {threads} threads"}
+        print(f"cpu baseline: {cpu['value']/1e6:.2f} Msteps/s on {threads} threads ({dt:.1f} s)",
+              file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "orbit-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64 (fp32 spline tables)",
+        "data": "synthetic",
+        "config": {"workload": "triaxial Dehnen Multipole lmax=8, 1M orbits, log-uniform 1-300 circular "
+                               "periods, DOP853 accuracy 1e-8, final state + max dE/E",
+                   "orbits": n, "accepted_steps_per_step": steps_per_launch,
+                   "attempts_per_step": attempts, "precision": args.precision},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "orbitKernel<double>", "kernel_ms": per_launch * 1e3},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

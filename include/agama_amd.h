@@ -31,9 +31,14 @@ typedef struct agx_potential agx_potential;
 #define AGX_EDEVICE       2   /* HIP runtime error */
 #define AGX_EUNSUPPORTED  3   /* valid in the reference, not provided by this library */
 
-/* precision of the spline tables used by the evaluation (flags argument) */
+/* flags: precision of the Multipole evaluation ... */
 #define AGX_PRECISION_FP64 0
 #define AGX_PRECISION_FP32 1  /* tables rounded to float; arithmetic, stages, error control FP64 */
+#define AGX_PRECISION_MASK 0xF
+/* ... and, for the orbit entry points, the processing order: by default the library orders
+   the batch longest-expected-integration first (|T| / dynamical time at the start point);
+   AGX_ORDER_AS_GIVEN keeps the input order (or the caller's d_order). */
+#define AGX_ORDER_AS_GIVEN 0x100
 
 /* Message of the last failed call on this thread ("" if none).
    Replaces agama_getError (src/interface_c.cpp:15-19), which is a process-global string. */
@@ -108,8 +113,9 @@ int agx_orbit_batch(const agx_potential* pot, long n, const double* ic, const do
     long* out_nattempts, long* out_nevals, double* out_maxde, int* out_status, int flags);
 
 /* Same with device buffers.  d_traj_offset[n] gives each orbit's first output row; d_order
-   (may be NULL) a processing order (permutation of 0..n-1), e.g. longest integrations
-   first.  Asynchronous on `stream`.  Calls that overlap on one potential handle race on its
+   (may be NULL) a processing order (permutation of 0..n-1); without it the library computes
+   the longest-first order itself (stream-ordered workspace) unless flags has
+   AGX_ORDER_AS_GIVEN.  Asynchronous on `stream`.  Calls that overlap on one potential handle race on its
    work-queue counter; use one handle per concurrent stream. */
 int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
     const double* d_tstart, const double* d_ttotal, const int* d_trajsize,
