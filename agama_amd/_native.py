@@ -44,6 +44,7 @@ def _load():
                                                   ctypes.c_double, ctypes.c_double, ctypes.c_long,
                                                   _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
         "agx_set_orbit_launch": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+        "agx_set_fetch": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
         "agama_getError": (ctypes.c_char_p, []),
         "agama_createPotential": (_vp, [ctypes.c_char_p]),
         "agama_deletePotential": (None, [_vp]),
@@ -62,7 +63,7 @@ EXPORTED = ["agx_last_error", "agx_potential_create", "agx_potential_from_coefs"
             "agx_potential_destroy", "agx_potential_write", "agx_build_coef_file",
             "agx_debug_tables", "agx_potential_coefs",
             "agx_eval_potential", "agx_eval_many", "agx_eval_many_device", "agx_orbit_batch",
-            "agx_orbit_batch_device", "agx_set_orbit_launch", "iniacceleration",
+            "agx_orbit_batch_device", "agx_set_orbit_launch", "agx_set_fetch", "iniacceleration",
             "agama_getError", "agama_createPotential", "agama_deletePotential",
             "agama_evalPotential"]
 

@@ -15,11 +15,11 @@
 #include "agx_kernels.h"
 #include "host_multipole.h"
 
-hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, long n,
-    const double* pos, double* phi, double* grad, hipStream_t stream);
-hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision,
+hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
+    long n, const double* pos, double* phi, double* grad, hipStream_t stream);
+hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
     const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream);
-int agxOrbitKernelBlocksPerCU(int precision, int blockSize);
+int agxOrbitKernelBlocksPerCU(int precision, int fetch);
 int agxOrbitKernelBlockSize();
 size_t agxScheduleWorkspace(long n);
 hipError_t agxLaunchSchedule(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, long n,
@@ -40,6 +40,7 @@ namespace {
 
 thread_local std::string g_err;
 int g_orbitBlocks = 0;
+int g_fetchEval = AGX_FETCH_VEC2, g_fetchOrbit = AGX_FETCH_VEC2;   // set by agx_set_fetch
 
 int fail(int code, const std::string& msg) { g_err = msg; return code; }
 
@@ -147,7 +148,7 @@ int orbitGeometry(long n, int precision, int& blocks, int& bs)
         int dev = 0, cus = 0;
         HIPCHECK(hipGetDevice(&dev));
         HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        const int perCU = std::max(1, agxOrbitKernelBlocksPerCU(precision, bs));
+        const int perCU = std::max(1, agxOrbitKernelBlocksPerCU(precision, g_fetchOrbit));
         blocks = cus * perCU;
     }
     const long need = (n + bs - 1) / bs;
@@ -274,7 +275,8 @@ int agx_eval_many_device(const agx_potential* pot, long n, const double* d_pos, 
     if(!pot) return fail(AGX_EINVAL, "NULL potential");
     if(n < 0 || (n > 0 && !d_pos)) return fail(AGX_EINVAL, "invalid point array");
     if(flags != AGX_PRECISION_FP64 && flags != AGX_PRECISION_FP32) return fail(AGX_EINVAL, "invalid flags");
-    hipError_t e = agxLaunchEval(pot->dev, pot->ddev, flags, n, d_pos, d_phi, d_grad, (hipStream_t)stream);
+    hipError_t e = agxLaunchEval(pot->dev, pot->ddev, flags, g_fetchEval, n, d_pos, d_phi, d_grad,
+        (hipStream_t)stream);
     if(e != hipSuccess) return fail(AGX_EDEVICE, std::string("eval launch: ") + hipGetErrorString(e));
     return AGX_OK;
 }
@@ -321,6 +323,15 @@ int agx_set_orbit_launch(int num_blocks, int block_size)
     return AGX_OK;
 }
 
+int agx_set_fetch(int eval_mode, int orbit_mode)
+{
+    auto ok = [](int m) { return m == AGX_FETCH_SCALAR || m == AGX_FETCH_VEC2 || m == AGX_FETCH_COOP; };
+    if(!ok(eval_mode) || !ok(orbit_mode)) return fail(AGX_EINVAL, "fetch mode must be 0, 1 or 2");
+    g_fetchEval = eval_mode;
+    g_fetchOrbit = orbit_mode;
+    return AGX_OK;
+}
+
 int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
     const double* d_tstart, const double* d_ttotal, const int* d_trajsize,
     const long* d_traj_offset, const int* d_order, double accuracy, double Omega,
@@ -364,7 +375,7 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
         int blocks, bs;
         orbitGeometry(n, precision, blocks, bs);
         HIPCHECK(hipMemsetAsync(pot->dcounter, 0, sizeof(unsigned int), st));
-        hipError_t e = agxLaunchOrbit(pot->dev, pot->ddev, precision, A, blocks, bs, st);
+        hipError_t e = agxLaunchOrbit(pot->dev, pot->ddev, precision, g_fetchOrbit, A, blocks, bs, st);
         if(work) HIPCHECK(hipFreeAsync(work, st));
         if(e != hipSuccess) return fail(AGX_EDEVICE, std::string("orbit launch: ") + hipGetErrorString(e));
         return AGX_OK;

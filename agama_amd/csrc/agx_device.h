@@ -23,6 +23,17 @@
 
 namespace agxd {
 
+// Table pointers are read from the header in device memory, so the compiler cannot infer their
+// address space and would emit flat (generic) accesses; these qualifiers make global-memory
+// and LDS accesses explicit.
+#define AGX_GLOBAL __attribute__((address_space(1)))
+#define AGX_LDS __attribute__((address_space(3)))
+template<typename T>
+__device__ __forceinline__ const T AGX_GLOBAL* gmem(const T* p) { return (const T AGX_GLOBAL*)p; }
+// native 16- and 8-byte vectors (the HIP vector classes cannot be copied across address spaces)
+typedef double vd2 __attribute__((ext_vector_type(2)));
+typedef float vf2 __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ double sqd(double x) { return x * x; }
 
 __device__ __forceinline__ void sincosA(double x, double& s, double& c)
@@ -67,8 +78,9 @@ __device__ __forceinline__ double atan2A(double y, double x)
     return res;
 }
 
-__device__ __forceinline__ int binSearchD(double x, const double* __restrict__ arr, int size)
+__device__ __forceinline__ int binSearchD(double x, const double* __restrict__ arrp, int size)
 {
+    const double AGX_GLOBAL* arr = gmem(arrp);
     if(size < 1 || !(x >= arr[0])) return -1;
     if(x > arr[size - 1] || size < 2) return size - 1;
     int index = (int)((x - arr[0]) / (arr[size - 1] - arr[0]) * (size - 1));
@@ -127,22 +139,123 @@ struct QuinticW {
     }
 };
 
-// 2-D quintic spline of one harmonic slot at a located cell: value and (ln r, tau) derivatives.
-// A, B point to the node records of corners (xi, yi) and (xi+1, yi); the (.., yi+1) corners
-// follow each at +rowStep doubles.  Returns z without the f_offset added back.
-template<typename T>
-__device__ __forceinline__ void splineSlot(const T* __restrict__ A, const T* __restrict__ B,
-    int rowStep, int offsetCorner, const QuinticW& wy, const QuinticW& wx,
-    double& z, double& zx, double& zy)
+// ---- node fetchers: how a lane obtains the 9 values of one node record (f, f_x, f_y, f_xx,
+// f_xy, f_yy, f_xxy, f_xyy, f_xxyy).  `rec` is the record's index in units of
+// AGX_NODE_STRIDE doubles.  Every record sits at an 80-byte (16-byte aligned) boundary.
+
+// one 8-byte load per value and lane
+struct FetchScalar {
+    const double AGX_GLOBAL* nodes;
+    static constexpr bool uniform = false;
+    __device__ __forceinline__ void record(int rec, double* v) const
+    {
+        const double AGX_GLOBAL* p = nodes + (size_t)rec * AGX_NODE_STRIDE;
+        for(int q = 0; q < 9; q++) v[q] = p[q];
+    }
+};
+
+// one 16-byte load per pair of values and lane
+struct FetchVec2 {
+    const vd2 AGX_GLOBAL* nodes;
+    static constexpr bool uniform = false;
+    __device__ __forceinline__ void record(int rec, double* v) const
+    {
+        const vd2 AGX_GLOBAL* p = nodes + (size_t)rec * (AGX_NODE_STRIDE / 2);
+        for(int q = 0; q < 5; q++) {
+            const vd2 t = p[q];
+            v[2 * q] = t.x;
+            if(2 * q + 1 < 9) v[2 * q + 1] = t.y;
+        }
+    }
+};
+
+// FP32 tables (precision-split form): one 16-byte load per four values and lane
+struct FetchFloat {
+    const float AGX_GLOBAL* nodes;
+    static constexpr bool uniform = false;
+    __device__ __forceinline__ void record(int rec, double* v) const
+    {
+        const vf2 AGX_GLOBAL* p2 = (const vf2 AGX_GLOBAL*)(nodes + (size_t)rec * AGX_NODE_STRIDE);
+        for(int q = 0; q < 5; q++) {
+            const vf2 t = p2[q];
+            v[2 * q] = t.x;
+            if(2 * q + 1 < 9) v[2 * q + 1] = t.y;
+        }
+    }
+};
+
+// Cooperative gather of a whole wavefront: the 64 lanes' records (64 x 80 bytes each) are
+// read with coalesced 16-byte loads - lane l fetches chunk k = l + 64 t of the wave's list -
+// and redistributed through LDS, so that one load instruction touches about 13 records'
+// cache lines instead of 64.  Divergent per-lane gathers are limited by the address unit at
+// about one cache line per cycle per CU.  The four corner records of a harmonic are fetched
+// as one batch (20 loads in flight).  All 64 lanes must call records4() together
+// (wave-uniform control flow); `lds` is this wavefront's buffer of 4 * 64 * 5 vd2.
+struct FetchCoop {
+    const vd2 AGX_GLOBAL* nodes;
+    vd2 AGX_LDS* lds;
+    int lane;
+    static constexpr bool uniform = true;
+    static constexpr int LDS_DOUBLE2_PER_WAVE = 4 * 64 * 5;
+    __device__ __forceinline__ void records4(int r0, int r1, int r2, int r3, double (&v)[4][10]) const
+    {
+        const int b0 = r0 * 5, b1 = r1 * 5, b2 = r2 * 5, b3 = r3 * 5;
+        vd2 t0[5], t1[5], t2[5], t3[5];
+#pragma unroll
+        for(int t = 0; t < 5; t++) {
+            const int k = lane + 64 * t, r = k / 5, c = k - 5 * r;
+            t0[t] = nodes[__shfl(b0, r, 64) + c];
+            t1[t] = nodes[__shfl(b1, r, 64) + c];
+            t2[t] = nodes[__shfl(b2, r, 64) + c];
+            t3[t] = nodes[__shfl(b3, r, 64) + c];
+        }
+#pragma unroll
+        for(int t = 0; t < 5; t++) {
+            const int k = lane + 64 * t;
+            lds[k] = t0[t]; lds[320 + k] = t1[t]; lds[640 + k] = t2[t]; lds[960 + k] = t3[t];
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for(int j = 0; j < 4; j++)
+#pragma unroll
+            for(int q = 0; q < 5; q++) {
+                const vd2 t = lds[j * 320 + lane * 5 + q];
+                v[j][2 * q] = t.x;
+                v[j][2 * q + 1] = t.y;
+            }
+        __builtin_amdgcn_wave_barrier();
+    }
+};
+
+// per-lane fetchers: the four records are independent loads scheduled together
+template<class F>
+__device__ __forceinline__ void fetch4(const F& f, int r0, int r1, int r2, int r3, double (&v)[4][10])
 {
-    // corners: ll = A, lu = A + rowStep, ul = B, uu = B + rowStep (x = ln r first index)
-    const T* ll = A; const T* lu = A + rowStep; const T* ul = B; const T* uu = B + rowStep;
-    const double off = offsetCorner == 0 ? (double)ll[0] : offsetCorner == 1 ? (double)lu[0] :
-                       offsetCorner == 2 ? (double)ul[0] : (double)uu[0];
+    f.record(r0, v[0]); f.record(r1, v[1]); f.record(r2, v[2]); f.record(r3, v[3]);
+}
+__device__ __forceinline__ void fetch4(const FetchCoop& f, int r0, int r1, int r2, int r3,
+    double (&v)[4][10])
+{
+    f.records4(r0, r1, r2, r3, v);
+}
+
+// 2-D quintic spline of one harmonic slot at a located cell: value and (ln r, tau) derivatives.
+// recLL is the record of corner (xi, yi); (xi, yi+1) follows at +ns records and (xi+1, yi) at
+// +nt*ns records.  The reference order of the nine node arrays (QuinticSpline2d::evalDeriv,
+// math_spline.cpp:2572-2597) maps to the record as q = 0..8 above.
+template<class F>
+__device__ __forceinline__ void splineSlot(const F& fetch, int recLL, int ns, int nt, int offsetCorner,
+    const QuinticW& wy, const QuinticW& wx, double& z, double& zx, double& zy)
+{
+    double v[4][10];
+    fetch4(fetch, recLL, recLL + ns, recLL + nt * ns, recLL + nt * ns + ns, v);
+    const double* ll = v[0]; const double* lu = v[1]; const double* ul = v[2]; const double* uu = v[3];
+    const double off = offsetCorner == 0 ? ll[0] : offsetCorner == 1 ? lu[0] :
+                       offsetCorner == 2 ? ul[0] : uu[0];
     // y-pass (tau) for the six functions {f, fx, fxx} at xlow/xupp, with f, f_y, f_yy ends
     double F0, F1, F2, F3, F4, F5, dF0, dF1, dF2, dF3, dF4, dF5;
-    wy.eval2((double)ll[0] - off, (double)lu[0] - off, ll[2], lu[2], ll[5], lu[5], F0, dF0);
-    wy.eval2((double)ul[0] - off, (double)uu[0] - off, ul[2], uu[2], ul[5], uu[5], F1, dF1);
+    wy.eval2(ll[0] - off, lu[0] - off, ll[2], lu[2], ll[5], lu[5], F0, dF0);
+    wy.eval2(ul[0] - off, uu[0] - off, ul[2], uu[2], ul[5], uu[5], F1, dF1);
     wy.eval2(ll[1], lu[1], ll[4], lu[4], ll[7], lu[7], F2, dF2);
     wy.eval2(ul[1], uu[1], ul[4], uu[4], ul[7], uu[7], F3, dF3);
     wy.eval2(ll[3], lu[3], ll[6], lu[6], ll[8], lu[8], F4, dF4);
@@ -179,34 +292,32 @@ struct TrigRec {
     }
 };
 
-// interior evaluation in cylindrical coordinates: potential and (R, z, phi) gradient
-template<typename T>
-__device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const T* __restrict__ nodes,
+// interior evaluation in cylindrical coordinates: potential and (R, z, phi) gradient.
+// Control flow is uniform across lanes (all harmonics are always evaluated; points outside
+// the grid read cell (0,0) and return NaN, as the reference's spline does), which the
+// cooperative fetcher requires.
+template<class F>
+__device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const F& fetch,
     double R, double z, double phi, double& pot, double& gR, double& gz, double& gphi)
 {
     const double r = sqrt(R * R + z * z), logr = log(r), rplusRinv = 1. / (r + fabs(R)),
         tau = R == 0 ? (z > 0 ? 1. : z < 0 ? -1. : 0.) : z * rplusRinv;
     const int xi = binSearchD(logr, P.gridR, P.nr), yi = binSearchD(tau, P.gridT, P.nt);
-    if(xi < 0 || xi >= P.nr - 1 || yi < 0 || yi >= P.nt - 1) {
-        pot = gR = gz = gphi = NAN;
-        return;
-    }
-    const double xlow = P.gridR[xi], xupp = P.gridR[xi + 1], ylow = P.gridT[yi], yupp = P.gridT[yi + 1];
+    const bool inGrid = xi >= 0 && xi < P.nr - 1 && yi >= 0 && yi < P.nt - 1;
+    const int xc = inGrid ? xi : 0, yc = inGrid ? yi : 0;
+    const double xlow = P.gridR[xc], xupp = P.gridR[xc + 1], ylow = P.gridT[yc], yupp = P.gridT[yc + 1];
     const int offsetCorner = logr == xupp ? (tau == yupp ? 3 : 2) : (tau == yupp ? 1 : 0);
     QuinticW wy, wx;
     wy.init(tau, ylow, yupp);
     wx.init(logr, xlow, xupp);
-    const int ns = P.ns, rowStep = ns * AGX_NODE_STRIDE;
-    const T* baseA = nodes + ((size_t)(xi * P.nt + yi) * ns) * AGX_NODE_STRIDE;
-    const T* baseB = baseA + (size_t)P.nt * rowStep;
+    const int ns = P.ns, nt = P.nt, rec0 = (xc * nt + yc) * ns;
 
     // m = 0 first (it scales the other harmonics), then the others in ascending m as in the
     // reference's Fourier synthesis: negative m (sine terms) restart the trig recurrence,
     // positive m advance one running recurrence
     const int zs = P.zeroSlot;
     double P0, d0r, d0t;
-    splineSlot<T>(baseA + zs * AGX_NODE_STRIDE, baseB + zs * AGX_NODE_STRIDE, rowStep, offsetCorner,
-        wy, wx, P0, d0r, d0t);
+    splineSlot(fetch, rec0 + zs, ns, nt, offsetCorner, wy, wx, P0, d0r, d0t);
     if(P.logScaling) {
         const double expX = exp(P0), val = 1 / (P.invPhi0 - expX), dPhidX = val * val * expX;
         P0 = val;
@@ -231,8 +342,7 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const T* 
                 trig = neg.s; dtrig = -m * neg.c;
             }
             double Pm, dmr, dmt;
-            splineSlot<T>(baseA + s * AGX_NODE_STRIDE, baseB + s * AGX_NODE_STRIDE, rowStep,
-                offsetCorner, wy, wx, Pm, dmr, dmt);
+            splineSlot(fetch, rec0 + s, ns, nt, offsetCorner, wy, wx, Pm, dmr, dmt);
             if(P.logScaling) {
                 dmr = dmr * P0 + Pm * d0r;
                 dmt = dmt * P0 + Pm * d0t;
@@ -244,12 +354,12 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const T* 
             trDphi += Pm * dtrig;
         }
     }
-    pot = trPot;
     const double rinv = 1 / r, r2inv = rinv * rinv;
     const double drdR = R * r2inv, drdz = z * r2inv, dthdR = -tau * rinv, dthdz = rinv - rplusRinv;
-    gR = trDr * drdR + trDth * dthdR;
-    gz = trDr * drdz + trDth * dthdz;
-    gphi = trDphi;
+    pot = inGrid ? trPot : NAN;
+    gR = inGrid ? trDr * drdR + trDth * dthdR : NAN;
+    gz = inGrid ? trDr * drdz + trDth * dthdz : NAN;
+    gphi = inGrid ? trDphi : NAN;
 }
 
 // PowerLawMultipole::evalCyl: extrapolation inside rmin / outside rmax
@@ -261,7 +371,7 @@ __device__ __forceinline__ void powerLawEval(const AgxPowerLawDev& L, double R, 
     // one (l,m) term: Phi_lm and d Phi_lm / d ln r
     auto term = [&](int l, int m, double& Pv, double& dPv) {
         const int c = l * (l + 1) + m;
-        const double s = L.S[c], u = L.U[c], w = L.W[c], v = L.inner ? l : -l - 1;
+        const double s = gmem(L.S)[c], u = gmem(L.U)[c], w = gmem(L.W)[c], v = L.inner ? l : -l - 1;
         const double rv  = v != 0 ? exp(dlogr * v) : 1;
         const double rs  = s != v ? (s != 0 ? exp(dlogr * s) : 1) : rv;
         const double urs = u * rs * (s != v || u == 0 ? 1 : dlogr);
@@ -376,23 +486,24 @@ __device__ __forceinline__ void powerLawEval(const AgxPowerLawDev& L, double R, 
     gphi = gph;
 }
 
-// Multipole::evalCyl dispatch (potential_multipole.cpp:1538-1548)
-template<typename T>
-__device__ __forceinline__ void multipoleCyl(const AgxMultipoleDev& P, const T* __restrict__ nodes,
+// Multipole::evalCyl dispatch (potential_multipole.cpp:1538-1548).  The interior spline is
+// evaluated by every lane (uniform control flow for the cooperative fetcher); lanes in the
+// power-law regions then overwrite the result.
+template<class F>
+__device__ __forceinline__ void multipoleCyl(const AgxMultipoleDev& P, const F& fetch,
     double R, double z, double phi, double& pot, double& gR, double& gz, double& gphi)
 {
     const double rsq = R * R + z * z;
+    interiorEval(P, fetch, R, z, phi, pot, gR, gz, gphi);
     if(rsq < P.rsqInner)
         powerLawEval(P.inner, R, z, phi, pot, gR, gz, gphi);
     else if(rsq > P.rsqOuter)
         powerLawEval(P.outer, R, z, phi, pot, gR, gz, gphi);
-    else
-        interiorEval<T>(P, nodes, R, z, phi, pot, gR, gz, gphi);
 }
 
 // BasePotentialCyl::evalCar: Car -> Cyl, evaluate, gradient back to Car (+dPhi/dx)
-template<typename T>
-__device__ __forceinline__ void multipoleCar(const AgxMultipoleDev& P, const T* __restrict__ nodes,
+template<class F>
+__device__ __forceinline__ void multipoleCar(const AgxMultipoleDev& P, const F& fetch,
     double x, double y, double z, double& pot, double& gx, double& gy, double& gz)
 {
     const double R = sqrt(x * x + y * y);
@@ -406,7 +517,7 @@ __device__ __forceinline__ void multipoleCar(const AgxMultipoleDev& P, const T* 
         phi = atan2A(y, x);
     }
     double gR, gzc, gphi;
-    multipoleCyl<T>(P, nodes, R, z, phi, pot, gR, gzc, gphi);
+    multipoleCyl(P, fetch, R, z, phi, pot, gR, gzc, gphi);
     gx = gR * dRdx + gphi * dphidx;
     gy = gR * dRdy + gphi * dphidy;
     gz = gzc;

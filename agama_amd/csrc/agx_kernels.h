@@ -24,3 +24,7 @@ struct AgxOrbitArgs {
 };
 
 enum AgxPrecision { AGX_FP64 = 0, AGX_FP32_TABLES = 1 };
+
+// how FP64 node records are fetched: per-lane 8-byte loads, per-lane 16-byte loads, or the
+// wavefront-cooperative gather through LDS (agx_device.h)
+enum AgxFetch { AGX_FETCH_SCALAR = 0, AGX_FETCH_VEC2 = 1, AGX_FETCH_COOP = 2 };

@@ -26,17 +26,36 @@ namespace {
 // ---------------- batched potential / gradient ----------------
 // The table header lives in device memory (uniform pointer -> scalar loads); passing it by
 // value would make the compiler copy its arrays into per-lane scratch at every launch.
-template<typename T>
+// MODE: AgxFetch for FP64 tables, or -1 for the FP32 tables.  Every lane of the grid takes
+// part in the evaluation (out-of-range lanes evaluate their block's first point and discard
+// it) so that the cooperative fetcher sees wave-uniform control flow.
+template<int MODE>
 __global__ __launch_bounds__(256) void evalKernel(const AgxMultipoleDev* __restrict__ Pp,
-    const T* __restrict__ nodes, long n, const double* __restrict__ pos, double* __restrict__ phi,
-    double* __restrict__ grad)
+    long n, const double* __restrict__ pos, double* __restrict__ phi, double* __restrict__ grad)
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n) return;
+    __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? 4 * FetchCoop::LDS_DOUBLE2_PER_WAVE : 1];
     const AgxMultipoleDev& P = *Pp;
+    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool inRange = i0 < n;
+    const long i = inRange ? i0 : (long)blockIdx.x * blockDim.x;
     const double x = pos[i * 3], y = pos[i * 3 + 1], z = pos[i * 3 + 2];
     double p, gx, gy, gz;
-    multipoleCar<T>(P, nodes, x, y, z, p, gx, gy, gz);
+    if constexpr(MODE == AGX_FETCH_COOP) {
+        FetchCoop f{ gmem(reinterpret_cast<const vd2*>(P.nodes)),
+            (vd2 AGX_LDS*)(stage + (threadIdx.x >> 6) * FetchCoop::LDS_DOUBLE2_PER_WAVE),
+            (int)(threadIdx.x & 63) };
+        multipoleCar(P, f, x, y, z, p, gx, gy, gz);
+    } else if constexpr(MODE == AGX_FETCH_VEC2) {
+        FetchVec2 f{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
+        multipoleCar(P, f, x, y, z, p, gx, gy, gz);
+    } else if constexpr(MODE == AGX_FETCH_SCALAR) {
+        FetchScalar f{ gmem(P.nodes) };
+        multipoleCar(P, f, x, y, z, p, gx, gy, gz);
+    } else {
+        FetchFloat f{ gmem(P.nodesF) };
+        multipoleCar(P, f, x, y, z, p, gx, gy, gz);
+    }
+    if(!inRange) return;
     if(phi) phi[i] = p;
     if(grad) { grad[i * 3] = gx; grad[i * 3 + 1] = gy; grad[i * 3 + 2] = gz; }
 }
@@ -95,7 +114,7 @@ enum : int { PH_IDLE = -1, PH_INIT_A = 0, PH_INIT_B, PH_INIT_C, PH_S2, PH_S3, PH
 constexpr int OBS = 256;
 // Wavefronts per SIMD the orbit kernel is compiled for (register budget 512/AGX_ORBIT_WAVES).
 #ifndef AGX_ORBIT_WAVES
-#define AGX_ORBIT_WAVES 2
+#define AGX_ORBIT_WAVES 1
 #endif
 
 // Per-lane state kept in LDS, field-major with a stride of OBS doubles so that a wavefront
@@ -224,7 +243,12 @@ __device__ __forceinline__ void finishOrbit(const AgxOrbitArgs& A, Lane& L, int 
 __device__ __forceinline__ void fetchOrbit(const AgxOrbitArgs& A, Lane& L)
 {
     const unsigned int q = atomicAdd(A.counter, 1u);
-    if((long)q >= A.n) { L.oi = -1; L.phase = PH_IDLE; return; }
+    if((long)q >= A.n) {
+        L.oi = -1;
+        L.phase = PH_IDLE;
+        L.y[0] = 1; L.y[1] = L.y[2] = L.y[3] = L.y[4] = L.y[5] = 0;   // placeholder point
+        return;
+    }
     const int oi = A.order ? A.order[q] : (int)q;
     L.oi = oi;
     L.tb = A.tstart ? A.tstart[oi] : 0.0;
@@ -293,11 +317,34 @@ __device__ __forceinline__ void acceptStep(const AgxOrbitArgs& A, Lane& L, doubl
     beginStep(A, L);
 }
 
-template<typename T>
-__global__ __launch_bounds__(OBS) __attribute__((amdgpu_waves_per_eu(AGX_ORBIT_WAVES, AGX_ORBIT_WAVES))) void orbitKernel(const AgxMultipoleDev* __restrict__ Pp,
-    const T* __restrict__ nodes, AgxOrbitArgs A)
+// one right-hand-side potential gradient in the inertial frame (orbit.cpp:182-208)
+template<int MODE>
+__device__ __forceinline__ void rhsEval(const AgxMultipoleDev& P, vd2* stage, double x, double y,
+    double z, double& pot, double& gx, double& gy, double& gz)
+{
+    if constexpr(MODE == AGX_FETCH_COOP) {
+        FetchCoop f{ gmem(reinterpret_cast<const vd2*>(P.nodes)),
+            (vd2 AGX_LDS*)(stage + (threadIdx.x >> 6) * FetchCoop::LDS_DOUBLE2_PER_WAVE),
+            (int)(threadIdx.x & 63) };
+        multipoleCar(P, f, x, y, z, pot, gx, gy, gz);
+    } else if constexpr(MODE == AGX_FETCH_VEC2) {
+        FetchVec2 f{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
+        multipoleCar(P, f, x, y, z, pot, gx, gy, gz);
+    } else if constexpr(MODE == AGX_FETCH_SCALAR) {
+        FetchScalar f{ gmem(P.nodes) };
+        multipoleCar(P, f, x, y, z, pot, gx, gy, gz);
+    } else {
+        FetchFloat f{ gmem(P.nodesF) };
+        multipoleCar(P, f, x, y, z, pot, gx, gy, gz);
+    }
+}
+
+template<int MODE>
+__global__ __launch_bounds__(OBS) __attribute__((amdgpu_waves_per_eu(AGX_ORBIT_WAVES, AGX_ORBIT_WAVES)))
+void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
 {
     __shared__ double lds[F_COUNT * OBS];
+    __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? (OBS / 64) * FetchCoop::LDS_DOUBLE2_PER_WAVE : 1];
     const AgxMultipoleDev& P = *Pp;
     Lane L;
     L.sh = lds + threadIdx.x;
@@ -306,13 +353,14 @@ __global__ __launch_bounds__(OBS) __attribute__((amdgpu_waves_per_eu(AGX_ORBIT_W
     while(true) {
         const bool active = L.phase != PH_IDLE;
         if(!__any(active)) break;
-        if(!active) continue;
-        // ---- one right-hand-side evaluation (OrbitIntegrator<Car>::eval, orbit.cpp:182-208)
+        // ---- one right-hand-side evaluation (OrbitIntegrator<Car>::eval, orbit.cpp:182-208),
+        // executed by every lane of the wavefront (idle lanes evaluate a placeholder point)
         double ca = 1, sa = 0;
-        if(A.Omega != 0) sincosA(A.Omega * (L.tb + L.toff), sa, ca);
+        if(A.Omega != 0 && active) sincosA(A.Omega * (L.tb + L.toff), sa, ca);
         double pot, gx, gy, gz;
-        multipoleCar<T>(P, nodes, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa, L.y[2],
+        rhsEval<MODE>(P, stage, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa, L.y[2],
             pot, gx, gy, gz);
+        if(!active) continue;
         double f[6];
         f[0] = L.y[3]; f[1] = L.y[4]; f[2] = L.y[5];
         f[3] = -gx * ca + gy * sa;
@@ -487,40 +535,56 @@ __global__ __launch_bounds__(OBS) __attribute__((amdgpu_waves_per_eu(AGX_ORBIT_W
 }  // namespace
 
 // ---------------- launchers (called from agx_capi.hip) ----------------
-// P: host copy of the header (for the table pointers), dP: the same header in device memory
-hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, long n,
-    const double* pos, double* phi, double* grad, hipStream_t stream)
+// P: host copy of the header, dP: the same header in device memory
+hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
+    long n, const double* pos, double* phi, double* grad, hipStream_t stream)
 {
+    (void)P;
     if(n <= 0) return hipSuccess;
     const int block = 256;
-    const long grid = (n + block - 1) / block;
+    const dim3 grid((unsigned)((n + block - 1) / block));
     if(precision == AGX_FP32_TABLES)
-        hipLaunchKernelGGL(evalKernel<float>, dim3((unsigned)grid), dim3(block), 0, stream, dP, P.nodesF,
-            n, pos, phi, grad);
+        hipLaunchKernelGGL(evalKernel<-1>, grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
+    else if(fetch == AGX_FETCH_COOP)
+        hipLaunchKernelGGL(evalKernel<AGX_FETCH_COOP>, grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
+    else if(fetch == AGX_FETCH_VEC2)
+        hipLaunchKernelGGL(evalKernel<AGX_FETCH_VEC2>, grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     else
-        hipLaunchKernelGGL(evalKernel<double>, dim3((unsigned)grid), dim3(block), 0, stream, dP, P.nodes,
-            n, pos, phi, grad);
+        hipLaunchKernelGGL(evalKernel<AGX_FETCH_SCALAR>, grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     return hipGetLastError();
 }
 
 int agxOrbitKernelBlockSize() { return OBS; }
 
-int agxOrbitKernelBlocksPerCU(int precision, int blockSize)
+int agxOrbitKernelBlocksPerCU(int precision, int fetch)
 {
     int nb = 0;
-    hipError_t e = precision == AGX_FP32_TABLES ?
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernel<float>, blockSize, 0) :
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernel<double>, blockSize, 0);
+    hipError_t e;
+    if(precision == AGX_FP32_TABLES)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernel<-1>, OBS, 0);
+    else if(fetch == AGX_FETCH_COOP)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernel<AGX_FETCH_COOP>, OBS, 0);
+    else if(fetch == AGX_FETCH_VEC2)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernel<AGX_FETCH_VEC2>, OBS, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernel<AGX_FETCH_SCALAR>, OBS, 0);
     return e == hipSuccess ? nb : 1;
 }
 
-hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision,
+hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
     const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream)
 {
+    (void)P;
     if(A.n <= 0) return hipSuccess;
+    if(blockSize != OBS) return hipErrorInvalidValue;
+    const dim3 g(numBlocks), b(OBS);
     if(precision == AGX_FP32_TABLES)
-        hipLaunchKernelGGL(orbitKernel<float>, dim3(numBlocks), dim3(blockSize), 0, stream, dP, P.nodesF, A);
+        hipLaunchKernelGGL(orbitKernel<-1>, g, b, 0, stream, dP, A);
+    else if(fetch == AGX_FETCH_COOP)
+        hipLaunchKernelGGL(orbitKernel<AGX_FETCH_COOP>, g, b, 0, stream, dP, A);
+    else if(fetch == AGX_FETCH_VEC2)
+        hipLaunchKernelGGL(orbitKernel<AGX_FETCH_VEC2>, g, b, 0, stream, dP, A);
     else
-        hipLaunchKernelGGL(orbitKernel<double>, dim3(numBlocks), dim3(blockSize), 0, stream, dP, P.nodes, A);
+        hipLaunchKernelGGL(orbitKernel<AGX_FETCH_SCALAR>, g, b, 0, stream, dP, A);
     return hipGetLastError();
 }

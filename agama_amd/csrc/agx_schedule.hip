@@ -15,16 +15,15 @@ using namespace agxd;
 
 namespace {
 
-template<typename T>
-__global__ __launch_bounds__(256) void costKernel(const AgxMultipoleDev* __restrict__ Pp,
-    const T* __restrict__ nodes, long n, const double* __restrict__ ic, const double* __restrict__ ttotal,
+__global__ __launch_bounds__(256) void costKernel(const AgxMultipoleDev* __restrict__ Pp, long n, const double* __restrict__ ic, const double* __restrict__ ttotal,
     float* __restrict__ key, int* __restrict__ idx)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= n) return;
     const double x = ic[i * 6], y = ic[i * 6 + 1], z = ic[i * 6 + 2];
     double p, gx, gy, gz;
-    multipoleCar<T>(*Pp, nodes, x, y, z, p, gx, gy, gz);
+    FetchVec2 f{ gmem(reinterpret_cast<const vd2*>(Pp->nodes)) };
+    multipoleCar(*Pp, f, x, y, z, p, gx, gy, gz);
     const double r = sqrt(x * x + y * y + z * z), g = sqrt(gx * gx + gy * gy + gz * gz);
     double c = fabs(ttotal[i]) * sqrt(g / fmax(r, 1e-300));
     if(!(c >= 0 && c < 3e38)) c = 3e38;   // NaN / inf inputs go first and fail fast
@@ -57,8 +56,8 @@ hipError_t agxLaunchSchedule(const AgxMultipoleDev& P, const AgxMultipoleDev* dP
     void* cubTemp = w + 3 * a;
     size_t cubBytes = agxScheduleWorkspace(n) - 3 * a - 256;
     const int block = 256;
-    hipLaunchKernelGGL(costKernel<double>, dim3((unsigned)((n + block - 1) / block)), dim3(block), 0, stream,
-        dP, P.nodes, n, ic, ttotal, keyIn, idxIn);
+    hipLaunchKernelGGL(costKernel, dim3((unsigned)((n + block - 1) / block)), dim3(block), 0, stream,
+        dP, n, ic, ttotal, keyIn, idxIn);
     hipError_t e = hipGetLastError();
     if(e != hipSuccess) return e;
     return hipcub::DeviceRadixSort::SortPairsDescending(cubTemp, cubBytes, keyIn, keyOut, idxIn, order,

@@ -45,7 +45,75 @@ def parse():
     ap.add_argument("--cpu-orbits", type=int, default=20_000,
                     help="size of the CPU-baseline sample (0 disables)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--mode", choices=["orbit", "eval"], default="orbit",
+                    help="orbit: the batched integration (default); eval: potential + force at "
+                         "1M points drawn from the density (secondary line)")
     return ap.parse_args()
+
+
+# algorithmic FP64 operations of one potential+gradient evaluation (eval mode): the
+# right-hand-side count above without the frame rotation
+FLOPS_PER_POINT = FLOPS_PER_EVAL - 15
+
+
+def bench_eval(args):
+    import numpy as np
+    import torch
+    import agama_amd
+    from agama_amd import lib, workloads
+    dev = torch.device("cuda:0")
+    coef = os.path.join(ROOT, "tests", "golden", "dehnen_triax_lmax8.coef")
+    pot = agama_amd.Potential(file=coef)
+    n = 1_000_000
+    pts_np = workloads.density_points(n, seed=7)
+    pts = torch.from_numpy(pts_np).to(dev)
+    phi = torch.empty(n, dtype=torch.float64, device=dev)
+    grad = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    prec = agama_amd.PRECISION_FP64 if args.precision == "fp64" else agama_amd.PRECISION_FP32
+    stream = torch.cuda.Stream()
+    reps = max(args.steps, 1) * 20
+
+    def launch():
+        rc = lib.agx_eval_many_device(pot.handle, n, pts.data_ptr(), 0.0, phi.data_ptr(), grad.data_ptr(),
+                                      prec, stream.cuda_stream)
+        if rc:
+            raise RuntimeError(agama_amd._native.last_error())
+    with torch.cuda.stream(stream):
+        for _ in range(max(args.warmup, 1) * 5):
+            launch()
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(reps):
+            launch()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    per = ev0.elapsed_time(ev1) / 1e3 / reps
+    achieved = n * FLOPS_PER_POINT / per / 1e12
+    cpu = None
+    if args.cpu_orbits > 0:
+        from oracle import pyoracle
+        threads = args.cpu_threads or os.cpu_count()
+        fast = pyoracle.OraclePotential(coef, library=pyoracle.fast())
+        t0 = time.perf_counter()
+        for _ in range(3):
+            fast.eval(pts_np, nthreads=threads)
+        dt = (time.perf_counter() - t0) / 3
+        cpu = {"value": n / dt, "unit": "evaluations/s", "cores": threads, "kind": "port",
+               "sample": f"the same 1M points, C++ OpenMP restatement (-O2), {threads} threads"}
+    out = {"metric": "potential+force evaluations/s at 1M points (triaxial Dehnen Multipole lmax=8)",
+           "value": n / per, "unit": "evaluations/s", "n_gpus": 1, "steps": reps, "warmup": args.warmup,
+           "ms_per_step": wall / reps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f64 (fp32 spline tables)",
+           "data": "synthetic",
+           "config": {"workload": "Phi and grad Phi at 1M points drawn from the triaxial Dehnen density"},
+           "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                        "kernel": "evalKernel<double>", "kernel_ms": per * 1e3},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -54,6 +122,8 @@ def main():
         print("bench.py: this build targets one GPU; --gpus N with N > 1 is not supported "
               "(orbits are independent and would shard by index, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "eval":
+        return bench_eval(args)
     import numpy as np
     import torch
     import agama_amd
@@ -112,6 +182,8 @@ def main():
     per_launch = kernel_s / args.steps
     value = steps_per_launch * args.steps / wall
     achieved = flops / per_launch / 1e12
+    print(f"steps per orbit: median {np.median(ns):.0f}, 99.9% {np.quantile(ns, 0.999):.0f}, max {ns.max()}",
+          file=sys.stderr, flush=True)
     print(f"orbits {n}: {steps_per_launch} accepted steps, {attempts} attempts per launch; "
           f"kernel {per_launch*1e3:.1f} ms/launch, wall {wall:.2f} s; status counts {np.bincount(st)}; "
           f"median/max dE/E {np.median(de):.2e}/{np.max(de):.2e}", file=sys.stderr, flush=True)

@@ -129,6 +129,11 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
    Set by the autotuner (agama_amd.tune). */
 int agx_set_orbit_launch(int num_blocks, int block_size);
 
+/* How the FP64 spline node records are gathered by the evaluation kernel and by the orbit
+   kernel: 0 = one 8-byte load per value and lane, 1 = 16-byte loads per lane, 2 = a
+   wavefront-cooperative gather through LDS.  Results are identical; set by the autotuner. */
+int agx_set_fetch(int eval_mode, int orbit_mode);
+
 /* ---- NEMO acceleration plugin (src/interface_nemo.cpp:47-60, 114-147, 180-220) ----
    iniacceleration(pars, npar, file, &accel, &need_m, &need_v): pars[0] = Omega (pattern
    speed), pars[1] = G is ignored unless 1; file names a Multipole coefficient file.  The
