@@ -33,6 +33,9 @@ FLOPS_PER_EVAL = 1531
 FLOPS_PER_ATTEMPT_EXTRA = 978
 FLOPS_PER_ORBIT_INIT = 3 * FLOPS_PER_EVAL + 80
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix), AMD published figure
+# HBM bytes per launch of the orbit kernel on this workload, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs scaled by a 1 GiB copy (profiles/orbitKernel_1M_uneven_triaxial_dehnen_lmax8.md)
+ORBIT_TRAFFIC_BYTES = 566e6 + 291e6
 
 
 def parse():
@@ -45,9 +48,10 @@ def parse():
     ap.add_argument("--cpu-orbits", type=int, default=20_000,
                     help="size of the CPU-baseline sample (0 disables)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["orbit", "eval"], default="orbit",
+    ap.add_argument("--mode", choices=["orbit", "eval", "copy"], default="orbit",
                     help="orbit: the batched integration (default); eval: potential + force at "
-                         "1M points drawn from the density (secondary line)")
+                         "1M points drawn from the density (secondary line); copy: one 1 GiB "
+                         "device-to-device copy, the reference for HBM byte counters")
     return ap.parse_args()
 
 
@@ -124,6 +128,15 @@ def main():
         sys.exit(2)
     if args.mode == "eval":
         return bench_eval(args)
+    if args.mode == "copy":
+        import torch
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "copy", "bytes_read": a.numel() * 8, "bytes_written": b.numel() * 8}))
+        return
     import numpy as np
     import torch
     import agama_amd
@@ -223,8 +236,9 @@ def main():
                    "orbits": n, "accepted_steps_per_step": steps_per_launch,
                    "attempts_per_step": attempts, "precision": args.precision},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "orbitKernel<double>", "kernel_ms": per_launch * 1e3},
+                     "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": ORBIT_TRAFFIC_BYTES if (n == 1_000_000 and args.precision == "fp64") else None,
+                     "kernel": "orbitKernel (FP64)", "kernel_ms": per_launch * 1e3},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
