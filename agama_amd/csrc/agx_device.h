@@ -100,44 +100,45 @@ __device__ __forceinline__ int binSearchD(double x, const double* __restrict__ a
     return index;
 }
 
-// weights of the quintic Hermite interpolant on [xl, xh] at x (value and first derivative)
-struct QuinticW {
+// weights of the quintic Hermite interpolant on [xl, xh] at x (value and first derivative),
+// in the arithmetic type R of the evaluation (double, or float for the precision-split form)
+template<typename R>
+struct QuinticWT {
     bool atEnd;                 // x == xh: endpoint values are returned exactly
-    double dx, dx2, h, P, Q, R, Pp, Qp, Rp;
-    __device__ __forceinline__ void init(double x, double xl, double xh)
+    R dx, dx2, h, P, Q, Rr, Pp, Qp, Rp;
+    __device__ __forceinline__ void init(R x, R xl, R xh)
     {
         atEnd = x == xh;
         dx = x - xl;
-        dx2 = .5 * dx * dx;
+        dx2 = R(.5) * dx * dx;
         h = xh - xl;
-        const double hi = 1 / h, h2 = h * h, t = dx * hi, t2 = t * t, t3 = t * t2,
-            t1t = t * (1 - t), Px = 30 * t1t * hi;
-        P  = t3 * (10 - t * (15 - 6 * t));
-        Q  = t3 * (1 - t * 0.5) * h;
-        R  = t3 * (1 - t * (1.25 - 0.5 * t)) * h2;
+        const R hi = R(1) / h, h2 = h * h, t = dx * hi, t2 = t * t, t3 = t * t2,
+            t1t = t * (R(1) - t), Px = R(30) * t1t * hi;
+        P  = t3 * (R(10) - t * (R(15) - R(6) * t));
+        Q  = t3 * (R(1) - t * R(0.5)) * h;
+        Rr = t3 * (R(1) - t * (R(1.25) - R(0.5) * t)) * h2;
         Pp = Px * t1t;
-        Qp = t2 * (3 - t * 2);
-        Rp = t2 * (3 - t * (5 - 2.5 * t)) * h;
+        Qp = t2 * (R(3) - t * R(2));
+        Rp = t2 * (R(3) - t * (R(5) - R(2.5) * t)) * h;
     }
     // value and derivative of one function given f, f', f'' at both ends
-    __device__ __forceinline__ void eval2(double fl, double fh, double f1l, double f1h,
-        double f2l, double f2h, double& f, double& df) const
+    __device__ __forceinline__ void eval2(R fl, R fh, R f1l, R f1h, R f2l, R f2h, R& f, R& df) const
     {
         if(atEnd) { f = fh; df = f1h; return; }
-        const double fd = fh - fl - 0.5 * h * (f1h + f1l),
-            f1d = f1h - f1l - 0.5 * h * (f2h + f2l), f2d = f2h - f2l;
-        f  = fl + P * fd + dx * f1l + Q * f1d + dx2 * f2l + R * f2d;
+        const R fd = fh - fl - R(0.5) * h * (f1h + f1l),
+            f1d = f1h - f1l - R(0.5) * h * (f2h + f2l), f2d = f2h - f2l;
+        f  = fl + P * fd + dx * f1l + Q * f1d + dx2 * f2l + Rr * f2d;
         df = Pp * fd + f1l + Qp * f1d + dx * f2l + Rp * f2d;
     }
-    __device__ __forceinline__ double eval1(double fl, double fh, double f1l, double f1h,
-        double f2l, double f2h) const
+    __device__ __forceinline__ R eval1(R fl, R fh, R f1l, R f1h, R f2l, R f2h) const
     {
         if(atEnd) return fh;
-        const double fd = fh - fl - 0.5 * h * (f1h + f1l),
-            f1d = f1h - f1l - 0.5 * h * (f2h + f2l), f2d = f2h - f2l;
-        return fl + P * fd + dx * f1l + Q * f1d + dx2 * f2l + R * f2d;
+        const R fd = fh - fl - R(0.5) * h * (f1h + f1l),
+            f1d = f1h - f1l - R(0.5) * h * (f2h + f2l), f2d = f2h - f2l;
+        return fl + P * fd + dx * f1l + Q * f1d + dx2 * f2l + Rr * f2d;
     }
 };
+typedef QuinticWT<double> QuinticW;
 
 // ---- node fetchers: how a lane obtains the 9 values of one node record (f, f_x, f_y, f_xx,
 // f_xy, f_yy, f_xxy, f_xyy, f_xxyy).  `rec` is the record's index in units of
@@ -145,6 +146,7 @@ struct QuinticW {
 
 // one 8-byte load per value and lane
 struct FetchScalar {
+    typedef double real;
     const double AGX_GLOBAL* nodes;
     static constexpr bool uniform = false;
     __device__ __forceinline__ void record(int rec, double* v) const
@@ -156,6 +158,7 @@ struct FetchScalar {
 
 // one 16-byte load per pair of values and lane
 struct FetchVec2 {
+    typedef double real;
     const vd2 AGX_GLOBAL* nodes;
     static constexpr bool uniform = false;
     __device__ __forceinline__ void record(int rec, double* v) const
@@ -169,17 +172,18 @@ struct FetchVec2 {
     }
 };
 
-// FP32 tables (precision-split form): one 16-byte load per four values and lane
+// FP32 tables (precision-split form): the spline is evaluated in float arithmetic
 struct FetchFloat {
+    typedef float real;
     const float AGX_GLOBAL* nodes;
     static constexpr bool uniform = false;
-    __device__ __forceinline__ void record(int rec, double* v) const
+    __device__ __forceinline__ void record(int rec, float* v) const
     {
         const vf2 AGX_GLOBAL* p2 = (const vf2 AGX_GLOBAL*)(nodes + (size_t)rec * AGX_NODE_STRIDE);
         for(int q = 0; q < 5; q++) {
             const vf2 t = p2[q];
             v[2 * q] = t.x;
-            if(2 * q + 1 < 9) v[2 * q + 1] = t.y;
+            v[2 * q + 1] = t.y;
         }
     }
 };
@@ -192,6 +196,7 @@ struct FetchFloat {
 // as one batch (20 loads in flight).  All 64 lanes must call records4() together
 // (wave-uniform control flow); `lds` is this wavefront's buffer of 4 * 64 * 5 vd2.
 struct FetchCoop {
+    typedef double real;
     const vd2 AGX_GLOBAL* nodes;
     vd2 AGX_LDS* lds;
     int lane;
@@ -229,7 +234,8 @@ struct FetchCoop {
 
 // per-lane fetchers: the four records are independent loads scheduled together
 template<class F>
-__device__ __forceinline__ void fetch4(const F& f, int r0, int r1, int r2, int r3, double (&v)[4][10])
+__device__ __forceinline__ void fetch4(const F& f, int r0, int r1, int r2, int r3,
+    typename F::real (&v)[4][10])
 {
     f.record(r0, v[0]); f.record(r1, v[1]); f.record(r2, v[2]); f.record(r3, v[3]);
 }
@@ -243,17 +249,17 @@ __device__ __forceinline__ void fetch4(const FetchCoop& f, int r0, int r1, int r
 // recLL is the record of corner (xi, yi); (xi, yi+1) follows at +ns records and (xi+1, yi) at
 // +nt*ns records.  The reference order of the nine node arrays (QuinticSpline2d::evalDeriv,
 // math_spline.cpp:2572-2597) maps to the record as q = 0..8 above.
-template<class F>
+template<class F, typename R = typename F::real>
 __device__ __forceinline__ void splineSlot(const F& fetch, int recLL, int ns, int nt, int offsetCorner,
-    const QuinticW& wy, const QuinticW& wx, double& z, double& zx, double& zy)
+    const QuinticWT<R>& wy, const QuinticWT<R>& wx, R& z, R& zx, R& zy)
 {
-    double v[4][10];
+    R v[4][10];
     fetch4(fetch, recLL, recLL + ns, recLL + nt * ns, recLL + nt * ns + ns, v);
-    const double* ll = v[0]; const double* lu = v[1]; const double* ul = v[2]; const double* uu = v[3];
-    const double off = offsetCorner == 0 ? ll[0] : offsetCorner == 1 ? lu[0] :
-                       offsetCorner == 2 ? ul[0] : uu[0];
+    const R* ll = v[0]; const R* lu = v[1]; const R* ul = v[2]; const R* uu = v[3];
+    const R off = offsetCorner == 0 ? ll[0] : offsetCorner == 1 ? lu[0] :
+                  offsetCorner == 2 ? ul[0] : uu[0];
     // y-pass (tau) for the six functions {f, fx, fxx} at xlow/xupp, with f, f_y, f_yy ends
-    double F0, F1, F2, F3, F4, F5, dF0, dF1, dF2, dF3, dF4, dF5;
+    R F0, F1, F2, F3, F4, F5, dF0, dF1, dF2, dF3, dF4, dF5;
     wy.eval2(ll[0] - off, lu[0] - off, ll[2], lu[2], ll[5], lu[5], F0, dF0);
     wy.eval2(ul[0] - off, uu[0] - off, ul[2], uu[2], ul[5], uu[5], F1, dF1);
     wy.eval2(ll[1], lu[1], ll[4], lu[4], ll[7], lu[7], F2, dF2);
@@ -269,24 +275,26 @@ __device__ __forceinline__ void splineSlot(const F& fetch, int recLL, int ns, in
 // cos/sin(k phi) for k = 1..m by the recurrence of trigMultiAngle (math_sphharm.cpp:134-152),
 // started from (alpha, beta) = (2 sin^2(phi/2), sin(phi)); `c, s` hold the values for k and
 // are advanced in place up to m
-struct TrigRec {
-    double alpha, beta, c, s;
+template<typename R>
+struct TrigRecT {
+    R alpha, beta, c, s;
     int k;
     __device__ __forceinline__ void init(double phi)
     {
-        double sphi, cphi;
+        double sphi, cphi, a, b;
         sincosA(phi, sphi, cphi);
-        sincosA(phi / 2, alpha, beta);
-        alpha *= alpha * 2;
-        beta = sphi;
+        sincosA(phi / 2, a, b);
+        a *= a * 2;
+        alpha = (R)a;
+        beta = (R)sphi;
         c = 1; s = 0; k = 0;
     }
     __device__ __forceinline__ void restart() { c = 1; s = 0; k = 0; }
     __device__ __forceinline__ void advanceTo(int m)
     {
         while(k < m) {
-            const double cn = c - (alpha * c + beta * s);
-            const double sn = s - (alpha * s - beta * c);
+            const R cn = c - (alpha * c + beta * s);
+            const R sn = s - (alpha * s - beta * c);
             c = cn; s = sn; k++;
         }
     }
@@ -305,43 +313,44 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const F& 
     const int xi = binSearchD(logr, P.gridR, P.nr), yi = binSearchD(tau, P.gridT, P.nt);
     const bool inGrid = xi >= 0 && xi < P.nr - 1 && yi >= 0 && yi < P.nt - 1;
     const int xc = inGrid ? xi : 0, yc = inGrid ? yi : 0;
+    typedef typename F::real Rt;
     const double xlow = P.gridR[xc], xupp = P.gridR[xc + 1], ylow = P.gridT[yc], yupp = P.gridT[yc + 1];
     const int offsetCorner = logr == xupp ? (tau == yupp ? 3 : 2) : (tau == yupp ? 1 : 0);
-    QuinticW wy, wx;
-    wy.init(tau, ylow, yupp);
-    wx.init(logr, xlow, xupp);
+    QuinticWT<Rt> wy, wx;
+    wy.init((Rt)tau, (Rt)ylow, (Rt)yupp);
+    wx.init((Rt)logr, (Rt)xlow, (Rt)xupp);
     const int ns = P.ns, nt = P.nt, rec0 = (xc * nt + yc) * ns;
 
     // m = 0 first (it scales the other harmonics), then the others in ascending m as in the
     // reference's Fourier synthesis: negative m (sine terms) restart the trig recurrence,
     // positive m advance one running recurrence
     const int zs = P.zeroSlot;
-    double P0, d0r, d0t;
+    Rt P0, d0r, d0t;
     splineSlot(fetch, rec0 + zs, ns, nt, offsetCorner, wy, wx, P0, d0r, d0t);
     if(P.logScaling) {
-        const double expX = exp(P0), val = 1 / (P.invPhi0 - expX), dPhidX = val * val * expX;
+        const Rt expX = exp(P0), val = Rt(1) / ((Rt)P.invPhi0 - expX), dPhidX = val * val * expX;
         P0 = val;
         d0r *= dPhidX;
         d0t *= dPhidX;
     }
-    double trPot = P0, trDr = d0r, trDth = d0t, trDphi = 0;
+    Rt trPot = P0, trDr = d0r, trDth = d0t, trDphi = 0;
     if(ns > 1) {
-        TrigRec pos, neg;
+        TrigRecT<Rt> pos, neg;
         pos.init(phi);
         neg.alpha = pos.alpha; neg.beta = pos.beta;
         for(int s = 0; s < ns; s++) {
             if(s == zs) continue;
             const int m = P.slotM[s];
-            double trig, dtrig;
+            Rt trig, dtrig;
             if(m > 0) {
                 pos.advanceTo(m);
-                trig = pos.c; dtrig = -m * pos.s;
+                trig = pos.c; dtrig = Rt(-m) * pos.s;
             } else {
                 neg.restart();
                 neg.advanceTo(-m);
-                trig = neg.s; dtrig = -m * neg.c;
+                trig = neg.s; dtrig = Rt(-m) * neg.c;
             }
-            double Pm, dmr, dmt;
+            Rt Pm, dmr, dmt;
             splineSlot(fetch, rec0 + s, ns, nt, offsetCorner, wy, wx, Pm, dmr, dmt);
             if(P.logScaling) {
                 dmr = dmr * P0 + Pm * d0r;
@@ -356,10 +365,10 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const F& 
     }
     const double rinv = 1 / r, r2inv = rinv * rinv;
     const double drdR = R * r2inv, drdz = z * r2inv, dthdR = -tau * rinv, dthdz = rinv - rplusRinv;
-    pot = inGrid ? trPot : NAN;
-    gR = inGrid ? trDr * drdR + trDth * dthdR : NAN;
-    gz = inGrid ? trDr * drdz + trDth * dthdz : NAN;
-    gphi = inGrid ? trDphi : NAN;
+    pot = inGrid ? (double)trPot : NAN;
+    gR = inGrid ? (double)trDr * drdR + (double)trDth * dthdR : NAN;
+    gz = inGrid ? (double)trDr * drdz + (double)trDth * dthdz : NAN;
+    gphi = inGrid ? (double)trDphi : NAN;
 }
 
 // PowerLawMultipole::evalCyl: extrapolation inside rmin / outside rmax
@@ -417,7 +426,7 @@ __device__ __forceinline__ void powerLawEval(const AgxPowerLawDev& L, double R, 
         // Legendre recurrence of sphHarmArray (math_sphharm.cpp:63-132) streamed in l, then the
         // Fourier synthesis in the reference's order (m = 0 first, then ascending m)
         const double ct = 2 * tau / (1 + tau * tau), st = (1 - tau * tau) / (1 + tau * tau);
-        TrigRec pos, neg;
+        TrigRecT<double> pos, neg;
         pos.init(phi);
         neg.alpha = pos.alpha; neg.beta = pos.beta;
         vr = gr = gth = 0;

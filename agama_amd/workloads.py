@@ -86,6 +86,31 @@ def triaxial_dehnen_orbits(n, potential_fn, coefs, seed=42, periods=100.0, uneve
     return np.hstack([pos, vel]), nper * tcirc
 
 
+DISK_BULGE_HALO = ("type=Multipole, density=Disk+Dehnen+NFW, lmax=12, gridSizeR=80, rmin=1e-2, "
+                   "rmax=1e3, 0.mass=0.05, 0.scaleRadius=3, 0.scaleHeight=0.3, 1.mass=0.01, "
+                   "1.scaleRadius=0.5, 1.gamma=1, 2.mass=1, 2.scaleRadius=20")
+
+
+def disk_orbits(n, force_fn, seed=42, periods=100.0, Rd=3.0, hz=0.3, sigma_frac=0.1):
+    """near-circular disk orbits in an axisymmetric potential: cylindrical radius from the
+    exponential surface density (R = Rd * Gamma(2) deviate), z from the sech^2 profile of
+    scale height hz, azimuth uniform; velocities v_phi = v_circ(R) plus Gaussian dispersions of
+    sigma_frac * v_circ(R) in all three components, v_circ^2 = R dPhi/dR in the midplane.
+    force_fn(points[n,3]) -> -grad Phi.  Integration time: `periods` circular periods at R."""
+    gen = rng(seed)
+    R = Rd * gen.gamma(2.0, size=n)
+    R = np.clip(R, 0.05, None)
+    z = hz * np.arctanh(np.clip(2 * gen.random(n) - 1, -1 + 1e-15, 1 - 1e-15))
+    ph = gen.uniform(0, 2 * np.pi, n)
+    mid = np.stack([R, np.zeros(n), np.zeros(n)], 1)
+    vc = np.sqrt(np.maximum(-R * np.asarray(force_fn(mid))[:, 0], 0.0))
+    vR, vz, vphi = (gen.normal(size=(3, n)) * sigma_frac * vc)
+    vphi = vphi + vc
+    c, s = np.cos(ph), np.sin(ph)
+    ic = np.stack([R * c, R * s, z, vR * c - vphi * s, vR * s + vphi * c, vz], 1)
+    return ic, periods * 2 * np.pi * R / vc
+
+
 def density_points(n, seed=7):
     """points drawn from the triaxial Dehnen density (for potential/force evaluation)"""
     return sample_dehnen_positions(n, rng(seed))

@@ -33,7 +33,9 @@ typedef struct agx_potential agx_potential;
 
 /* flags: precision of the Multipole evaluation ... */
 #define AGX_PRECISION_FP64 0
-#define AGX_PRECISION_FP32 1  /* tables rounded to float; arithmetic, stages, error control FP64 */
+#define AGX_PRECISION_FP32 1  /* splines, harmonic scaling and Fourier synthesis in float from float
+                                 tables; coordinates, extrapolation, stages, error control and
+                                 energy stay FP64 */
 #define AGX_PRECISION_MASK 0xF
 /* ... and, for the orbit entry points, the processing order: by default the library orders
    the batch longest-expected-integration first (|T| / dynamical time at the start point);

@@ -1,0 +1,80 @@
+"""Precision split on the composite disk + bulge + halo Multipole (lmax=12, 80 radial nodes):
+the FP32 form evaluates the quintic splines, the harmonic scaling and the Fourier synthesis in
+float arithmetic from float tables; coordinates, chain rule, power-law extrapolation, DOP853
+stage sums, error control and energy stay FP64.  Every figure is measured against the FP64
+CPU oracle and printed; the bounds written here are the stated error of the FP32 form."""
+import os
+
+import numpy as np
+import pytest
+
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+COEF = os.path.join(GOLDEN, "disk_bulge_halo_lmax12.coef")
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import agama_amd
+    from oracle.pyoracle import OraclePotential
+    return agama_amd.Potential(file=COEF), OraclePotential(COEF)
+
+
+def disk_ics(orc, n, seed, periods):
+    from agama_amd import workloads
+    return workloads.disk_orbits(n, lambda p: -orc.eval(p)[1], seed=seed, periods=periods)
+
+
+def test_fp64_form_matches_oracle(setup):
+    import agama_amd
+    gpu, orc = setup
+    ic, _ = disk_ics(orc, 100_000, seed=31, periods=1)
+    pts = ic[:, :3]
+    phi, grad = gpu._eval(pts, True, True, precision=agama_amd.PRECISION_FP64)
+    ophi, og = orc.eval(pts)
+    assert np.max(np.abs(phi / ophi - 1)) <= 1e-12
+    assert np.max(np.linalg.norm(grad - og, axis=1) / np.linalg.norm(og, axis=1)) <= 1e-12
+
+
+def test_fp32_form_force_error(setup):
+    """force and potential error of the FP32 form at disk points (the orbit sample's own)"""
+    import agama_amd
+    gpu, orc = setup
+    ic, _ = disk_ics(orc, 100_000, seed=32, periods=1)
+    pts = ic[:, :3]
+    phi, grad = gpu._eval(pts, True, True, precision=agama_amd.PRECISION_FP32)
+    ophi, og = orc.eval(pts)
+    ef = np.linalg.norm(grad - og, axis=1) / np.linalg.norm(og, axis=1)
+    ep = np.abs(phi / ophi - 1)
+    print(f"FP32 form: force rel err median {np.median(ef):.2e} 99% {np.quantile(ef, .99):.2e} "
+          f"max {ef.max():.2e}; potential rel err median {np.median(ep):.2e} max {ep.max():.2e}")
+    assert np.median(ef) <= 1e-6 and ef.max() <= 1e-4
+    assert ep.max() <= 1e-6
+
+
+def test_fp32_form_energy_drift(setup):
+    """near-circular disk orbits over 100 circular periods at accuracy 1e-8: energy drift of
+    the FP32 form, with the energy measured by the FP64 oracle at start and end"""
+    import agama_amd
+    gpu, orc = setup
+    ic, T = disk_ics(orc, 512, seed=33, periods=100)
+    res = {}
+    for name, prec in (("fp64", agama_amd.PRECISION_FP64), ("fp32", agama_amd.PRECISION_FP32)):
+        info = {}
+        agama_amd.orbit(potential=gpu, ic=ic, time=T, accuracy=1e-8, info=info, precision=prec)
+        assert np.all(info["status"] == 0)
+        fin = info["final"]
+        E0 = orc.eval(ic[:, :3])[0] + 0.5 * np.sum(ic[:, 3:] ** 2, 1)
+        E1 = orc.eval(fin[:, :3])[0] + 0.5 * np.sum(fin[:, 3:] ** 2, 1)
+        res[name] = (np.abs(E1 / E0 - 1), fin, info["nsteps"])
+    d64, f64, n64 = res["fp64"]
+    d32, f32, n32 = res["fp32"]
+    pos = np.linalg.norm(f32[:, :3] - f64[:, :3], axis=1) / np.linalg.norm(f64[:, :3], axis=1)
+    print(f"energy drift (FP64 oracle) fp64 form: median {np.median(d64):.2e} max {d64.max():.2e}; "
+          f"fp32 form: median {np.median(d32):.2e} max {d32.max():.2e}; "
+          f"fp32-vs-fp64 final position median {np.median(pos):.2e}; "
+          f"steps ratio {n32.sum() / n64.sum():.3f}")
+    assert np.median(d64) <= 1e-9
+    assert np.median(d32) <= 1e-6 and d32.max() <= 1e-4
