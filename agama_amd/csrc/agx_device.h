@@ -106,13 +106,16 @@ template<typename R>
 struct QuinticWT {
     bool atEnd;                 // x == xh: endpoint values are returned exactly
     R dx, dx2, h, P, Q, Rr, Pp, Qp, Rp;
-    __device__ __forceinline__ void init(R x, R xl, R xh)
+    // the offsets x - xl and xh - xl are formed in double before rounding to R, so that the
+    // float form's weights vary smoothly with x inside the cell
+    __device__ __forceinline__ void init(double x, double xl, double xh)
     {
         atEnd = x == xh;
-        dx = x - xl;
+        dx = (R)(x - xl);
         dx2 = R(.5) * dx * dx;
-        h = xh - xl;
-        const R hi = R(1) / h, h2 = h * h, t = dx * hi, t2 = t * t, t3 = t * t2,
+        h = (R)(xh - xl);
+        const double hiD = 1 / (xh - xl);
+        const R hi = (R)hiD, h2 = h * h, t = (R)((x - xl) * hiD), t2 = t * t, t3 = t * t2,
             t1t = t * (R(1) - t), Px = R(30) * t1t * hi;
         P  = t3 * (R(10) - t * (R(15) - R(6) * t));
         Q  = t3 * (R(1) - t * R(0.5)) * h;
@@ -317,8 +320,8 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const F& 
     const double xlow = P.gridR[xc], xupp = P.gridR[xc + 1], ylow = P.gridT[yc], yupp = P.gridT[yc + 1];
     const int offsetCorner = logr == xupp ? (tau == yupp ? 3 : 2) : (tau == yupp ? 1 : 0);
     QuinticWT<Rt> wy, wx;
-    wy.init((Rt)tau, (Rt)ylow, (Rt)yupp);
-    wx.init((Rt)logr, (Rt)xlow, (Rt)xupp);
+    wy.init(tau, ylow, yupp);
+    wx.init(logr, xlow, xupp);
     const int ns = P.ns, nt = P.nt, rec0 = (xc * nt + yc) * ns;
 
     // m = 0 first (it scales the other harmonics), then the others in ascending m as in the

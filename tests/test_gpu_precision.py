@@ -56,25 +56,33 @@ def test_fp32_form_force_error(setup):
 
 def test_fp32_form_energy_drift(setup):
     """near-circular disk orbits over 100 circular periods at accuracy 1e-8: energy drift of
-    the FP32 form, with the energy measured by the FP64 oracle at start and end"""
+    both forms, with the energy measured by the FP64 oracle at start and end, next to the
+    oracle's own integration; the FP32 form's extra steps (its force noise feeds the error
+    estimate) are printed"""
     import agama_amd
     gpu, orc = setup
     ic, T = disk_ics(orc, 512, seed=33, periods=100)
+
+    def drift(fin):
+        E0 = orc.eval(ic[:, :3])[0] + 0.5 * np.sum(ic[:, 3:] ** 2, 1)
+        E1 = orc.eval(fin[:, :3])[0] + 0.5 * np.sum(fin[:, 3:] ** 2, 1)
+        return np.abs(E1 / E0 - 1)
+
     res = {}
     for name, prec in (("fp64", agama_amd.PRECISION_FP64), ("fp32", agama_amd.PRECISION_FP32)):
         info = {}
         agama_amd.orbit(potential=gpu, ic=ic, time=T, accuracy=1e-8, info=info, precision=prec)
         assert np.all(info["status"] == 0)
-        fin = info["final"]
-        E0 = orc.eval(ic[:, :3])[0] + 0.5 * np.sum(ic[:, 3:] ** 2, 1)
-        E1 = orc.eval(fin[:, :3])[0] + 0.5 * np.sum(fin[:, 3:] ** 2, 1)
-        res[name] = (np.abs(E1 / E0 - 1), fin, info["nsteps"])
+        res[name] = (drift(info["final"]), info["final"], info["nsteps"])
+    ref = orc.orbits(ic, T, trajsize=1, accuracy=1e-8)
+    dref = drift(ref["final"])
     d64, f64, n64 = res["fp64"]
     d32, f32, n32 = res["fp32"]
     pos = np.linalg.norm(f32[:, :3] - f64[:, :3], axis=1) / np.linalg.norm(f64[:, :3], axis=1)
-    print(f"energy drift (FP64 oracle) fp64 form: median {np.median(d64):.2e} max {d64.max():.2e}; "
+    print(f"energy drift (FP64 oracle) oracle: median {np.median(dref):.2e} max {dref.max():.2e}; "
+          f"fp64 form: median {np.median(d64):.2e} max {d64.max():.2e}; "
           f"fp32 form: median {np.median(d32):.2e} max {d32.max():.2e}; "
           f"fp32-vs-fp64 final position median {np.median(pos):.2e}; "
-          f"steps ratio {n32.sum() / n64.sum():.3f}")
-    assert np.median(d64) <= 1e-9
+          f"steps fp32/fp64 {n32.sum() / n64.sum():.3f}")
+    assert np.median(d64) <= 1.5 * np.median(dref) + 1e-13
     assert np.median(d32) <= 1e-6 and d32.max() <= 1e-4
