@@ -12,7 +12,8 @@ import numpy as np
 from . import _native
 from ._native import AgamaAmdError, check, lib, ptr
 
-__all__ = ["Potential", "Density", "orbit", "AgamaAmdError", "PRECISION_FP64", "PRECISION_FP32"]
+__all__ = ["Potential", "Density", "orbit", "Surrogate", "AgamaAmdError", "PRECISION_FP64",
+           "PRECISION_FP32"]
 
 PRECISION_FP64 = 0
 PRECISION_FP32 = 1
@@ -255,3 +256,6 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
     for i, (a, b) in enumerate(parts):
         res[i, 0], res[i, 1] = a, b
     return res[0] if single else res
+
+
+from .surrogate import Surrogate  # noqa: E402

@@ -45,6 +45,18 @@ def _load():
                                                   _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
         "agx_set_orbit_launch": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
         "agx_set_fetch": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+        "agx_surrogate_train": (_vp, [_vp, ctypes.c_char_p]),
+        "agx_surrogate_create": (_vp, [_vp, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                       ctypes.c_double, ctypes.c_double]),
+        "agx_surrogate_destroy": (None, [_vp]),
+        "agx_surrogate_info": (ctypes.c_int, [_vp, _vp, _vp]),
+        "agx_surrogate_loss_history": (ctypes.c_long, [_vp, _vp, ctypes.c_long]),
+        "agx_surrogate_eval_many": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp]),
+        "agx_surrogate_eval_many_device": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp]),
+        "agx_surrogate_network": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp]),
+        "agx_surrogate_train_grad": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp]),
+        "agx_debug_gemm": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
         "agama_getError": (ctypes.c_char_p, []),
         "agama_createPotential": (_vp, [ctypes.c_char_p]),
         "agama_deletePotential": (None, [_vp]),
@@ -64,6 +76,10 @@ EXPORTED = ["agx_last_error", "agx_potential_create", "agx_potential_from_coefs"
             "agx_debug_tables", "agx_potential_coefs",
             "agx_eval_potential", "agx_eval_many", "agx_eval_many_device", "agx_orbit_batch",
             "agx_orbit_batch_device", "agx_set_orbit_launch", "agx_set_fetch", "iniacceleration",
+            "agx_surrogate_train", "agx_surrogate_create", "agx_surrogate_destroy",
+            "agx_surrogate_info", "agx_surrogate_loss_history", "agx_surrogate_eval_many",
+            "agx_surrogate_eval_many_device", "agx_surrogate_network", "agx_surrogate_train_grad",
+            "agx_debug_gemm",
             "agama_getError", "agama_createPotential", "agama_deletePotential",
             "agama_evalPotential"]
 

@@ -8,11 +8,13 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
 #include "../../include/agama_amd.h"
 #include "agx_kernels.h"
+#include "agx_surrogate.h"
 #include "host_multipole.h"
 
 hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
@@ -104,11 +106,11 @@ void upload(agx_potential& p)
     put(oH, &D, sizeof(D));
 }
 
-agx_potential* makeFromCoefs(agx::Coefs&& c)
+agx_potential* makeFromCoefs(agx::Coefs&& c, bool allowLowOrder = false)
 {
     std::unique_ptr<agx_potential> p(new agx_potential);
     p->coefs = std::move(c);
-    p->T = agx::buildTables(p->coefs);
+    p->T = agx::buildTables(p->coefs, allowLowOrder);
     upload(*p);
     return p.release();
 }
@@ -423,6 +425,361 @@ int agx_orbit_batch(const agx_potential* pot, long n, const double* ic, const do
         if(out_nevals) HIPCHECK(hipMemcpy(out_nevals, dne.p, n * 8, hipMemcpyDeviceToHost));
         if(out_maxde) HIPCHECK(hipMemcpy(out_maxde, dmde.p, n * 8, hipMemcpyDeviceToHost));
         if(out_status) HIPCHECK(hipMemcpy(out_status, dst.p, n * 4, hipMemcpyDeviceToHost));
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+}  // extern "C"
+
+// ---------------- MLP surrogate (no counterpart in the reference) ----------------
+struct agx_surrogate {
+    const agx_potential* pot = nullptr;         // the full FP64 Multipole (not owned)
+    std::unique_ptr<agx_potential> base;        // its l <= baseLmax truncation
+    std::vector<float> theta;                   // flat parameters (AGX_P_* layout)
+    int baseLmax = 2;
+    float lnScale = 1, outScale = 1;
+    double rmin = 1e-2, rmax = 1e2;
+    std::vector<float> lossHist;
+    void* dmem = nullptr;                       // packed inference weights + header
+    AgxMlpDev* dM = nullptr;
+    ~agx_surrogate() { if(dmem) (void)hipFree(dmem); }
+};
+
+namespace {
+
+uint16_t bf16Bits(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7FFF + ((u >> 16) & 1);   // round to nearest even (finite inputs)
+    return (uint16_t)(u >> 16);
+}
+
+// inference layout of the parameters (agx_mlp.h) uploaded into one allocation
+void packInference(agx_surrogate& S)
+{
+    const float* th = S.theta.data();
+    const size_t tileBytes = 8 * 16 * 64 * 16;    // one layer: 8 m-tiles x 16 k-blocks x 64 lanes x 16 B
+    const size_t nf = 256 * 4 + 256 + 3 * 256 + 256;
+    std::vector<uint16_t> tiles(6 * tileBytes / 2);
+    const int woff[3] = { AGX_P_W2, AGX_P_W3, AGX_P_W4 }, boff[3] = { AGX_P_B2, AGX_P_B3, AGX_P_B4 };
+    for(int l = 0; l < 3; l++) {
+        const float* W = th + woff[l];
+        uint16_t* F = tiles.data() + (size_t)l * tileBytes / 2;
+        uint16_t* B = tiles.data() + (size_t)(3 + l) * tileBytes / 2;
+        for(int mt = 0; mt < 8; mt++) for(int kb = 0; kb < 16; kb++) for(int lane = 0; lane < 64; lane++)
+            for(int p = 0; p < 8; p++) {
+                const size_t e = (((size_t)mt * 16 + kb) * 64 + lane) * 8 + p;
+                const int row = 32 * mt + (lane & 31), k = agxMlpKNeuron(16 * kb + 8 * (lane >> 5) + p);
+                F[e] = bf16Bits(W[row * 256 + k]);        // W[out = row][in = k]
+                B[e] = bf16Bits(W[k * 256 + row]);        // W^T[in = row][out = k]
+            }
+    }
+    std::vector<float> fl(nf);
+    float* w1p = fl.data();
+    float* b1p = w1p + 1024;
+    float* bp = b1p + 256;
+    float* w5p = bp + 768;
+    for(int mt = 0; mt < 8; mt++) for(int g = 0; g < 2; g++) for(int j = 0; j < 16; j++) {
+        const int idx = (mt * 2 + g) * 16 + j, o = agxMlpNeuron(mt, g, j);
+        for(int f = 0; f < 4; f++) w1p[idx * 4 + f] = th[AGX_P_W1 + o * 4 + f];
+        b1p[idx] = th[AGX_P_B1 + o];
+        for(int l = 0; l < 3; l++) bp[l * 256 + idx] = th[boff[l] + o];
+        w5p[idx] = th[AGX_P_W5 + o];
+    }
+    const size_t oT = 0, oF = oT + 6 * tileBytes, oH = oF + align256(nf * 4), total = oH + align256(sizeof(AgxMlpDev));
+    HIPCHECK(hipSetDevice(S.pot->device));
+    if(!S.dmem) HIPCHECK(hipMalloc(&S.dmem, total));
+    char* base = static_cast<char*>(S.dmem);
+    HIPCHECK(hipMemcpy(base + oT, tiles.data(), 6 * tileBytes, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(base + oF, fl.data(), nf * 4, hipMemcpyHostToDevice));
+    AgxMlpDev M;
+    memset(&M, 0, sizeof(M));
+    for(int l = 0; l < 3; l++) {
+        M.fwdA[l] = reinterpret_cast<const agx_bf16x8*>(base + oT + l * tileBytes);
+        M.bwdA[l] = reinterpret_cast<const agx_bf16x8*>(base + oT + (3 + l) * tileBytes);
+        M.bp[l] = reinterpret_cast<const float*>(base + oF) + 1024 + 256 + l * 256;
+    }
+    M.w1p = reinterpret_cast<const float*>(base + oF);
+    M.b1p = M.w1p + 1024;
+    M.w5p = M.w1p + 1024 + 256 + 768;
+    M.b5 = th[AGX_P_B5];
+    M.lnScale = S.lnScale;
+    M.outScale = S.outScale;
+    M.rminTrain = S.rmin;
+    M.rmaxTrain = S.rmax;
+    S.dM = reinterpret_cast<AgxMlpDev*>(base + oH);
+    HIPCHECK(hipMemcpy(S.dM, &M, sizeof(M), hipMemcpyHostToDevice));
+}
+
+agx_surrogate* makeSurrogate(const agx_potential* pot, int baseLmax, float lnScale, float outScale,
+    double rmin, double rmax)
+{
+    if(baseLmax < 0 || baseLmax > 2) throw std::invalid_argument("surrogate: base_lmax must be 0, 1 or 2");
+    if(!(rmin > 0 && rmax > rmin)) throw std::invalid_argument("surrogate: invalid radius range");
+    HIPCHECK(hipSetDevice(pot->device));
+    std::unique_ptr<agx_surrogate> S(new agx_surrogate);
+    S->pot = pot;
+    S->baseLmax = baseLmax;
+    S->lnScale = lnScale;
+    S->outScale = outScale;
+    S->rmin = rmin;
+    S->rmax = rmax;
+    S->base.reset(makeFromCoefs(agx::truncateCoefs(pot->coefs, baseLmax), true));
+    return S.release();
+}
+
+// device buffers of the training step (freed on destruction)
+struct TrainAlloc {
+    std::vector<void*> ptrs;
+    template<typename T> T* get(size_t n) {
+        void* p = nullptr;
+        HIPCHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+    ~TrainAlloc() { for(void* p : ptrs) (void)hipFree(p); }
+};
+
+void allocTrainStep(TrainAlloc& A, AgxTrainBuffers& T, int batch, long nbatches, long histLen)
+{
+    const size_t E = (size_t)batch * 256;
+    T.batch = batch;
+    T.nbatches = nbatches;
+    T.theta = A.get<float>(AGX_MLP_NPARAM);
+    T.grad = A.get<float>(AGX_MLP_NPARAM);
+    T.m1 = A.get<float>(AGX_MLP_NPARAM);
+    T.m2 = A.get<float>(AGX_MLP_NPARAM);
+    T.Wb = A.get<uint16_t>(3 * 65536);
+    for(int l = 0; l < 4; l++) { T.H[l] = A.get<uint16_t>(E); T.S[l] = A.get<float>(E); }
+    T.Z = A.get<float>(E);
+    T.G = A.get<float>(E);
+    T.Gb = A.get<uint16_t>(E);
+    T.step = A.get<long>(1);
+    T.lossHist = A.get<float>(histLen);
+    HIPCHECK(hipMemset(T.m1, 0, AGX_MLP_NPARAM * 4));
+    HIPCHECK(hipMemset(T.m2, 0, AGX_MLP_NPARAM * 4));
+    HIPCHECK(hipMemset(T.step, 0, 8));
+    HIPCHECK(hipMemset(T.lossHist, 0, histLen * 4));
+}
+
+std::vector<float> initParams(unsigned long long seed)
+{
+    std::vector<float> th(AGX_MLP_NPARAM, 0.f);
+    std::mt19937_64 gen(seed);
+    std::normal_distribution<double> nd(0., 1.);
+    for(int i = 0; i < 256 * 4; i++) th[AGX_P_W1 + i] = (float)(nd(gen) * sqrt(2. / 4));
+    const int woff[3] = { AGX_P_W2, AGX_P_W3, AGX_P_W4 };
+    for(int l = 0; l < 3; l++) for(int i = 0; i < 65536; i++) th[woff[l] + i] = (float)(nd(gen) * sqrt(2. / 256));
+    for(int i = 0; i < 256; i++) th[AGX_P_W5 + i] = (float)(nd(gen) * sqrt(1. / 256));
+    return th;
+}
+
+}  // namespace
+
+extern "C" {
+
+agx_surrogate* agx_surrogate_create(const agx_potential* pot, const float* theta, int base_lmax,
+    float ln_scale, float out_scale, double rmin, double rmax)
+{
+    g_err.clear();
+    if(!pot || !theta) { fail(AGX_EINVAL, "NULL argument"); return nullptr; }
+    try {
+        std::unique_ptr<agx_surrogate> S(makeSurrogate(pot, base_lmax, ln_scale, out_scale, rmin, rmax));
+        S->theta.assign(theta, theta + AGX_MLP_NPARAM);
+        packInference(*S);
+        return S.release();
+    } catch(std::exception& e) { fail(AGX_EINVAL, e.what()); return nullptr; }
+}
+
+agx_surrogate* agx_surrogate_train(const agx_potential* pot, const char* params)
+{
+    g_err.clear();
+    if(!pot) { fail(AGX_EINVAL, "NULL potential"); return nullptr; }
+    try {
+        agx::Params P = agx::Params::parse(params ? params : "");
+        P.checkOnly({"samples", "batch", "steps", "lr", "seed", "baselmax", "rmin", "rmax", "warmup"});
+        const int batch = (int)P.num("batch", 16384);
+        const long steps = (long)P.num("steps", 6000);
+        const double rmin = P.num("rmin", 1e-2), rmax = P.num("rmax", 1e2);
+        const unsigned long long seed = (unsigned long long)P.num("seed", 1);
+        if(batch < 512 || batch % 512) throw std::invalid_argument("surrogate: batch must be a multiple of 512");
+        if(steps < 1) throw std::invalid_argument("surrogate: steps must be positive");
+        const long nb = std::max<long>(1, (long)P.num("samples", 20e6) / batch), N = nb * batch;
+        const float lnScale = (float)std::max(fabs(log(rmin)), fabs(log(rmax)));
+        std::unique_ptr<agx_surrogate> S(makeSurrogate(pot, (int)P.num("baselmax", 2), lnScale, 1.f, rmin, rmax));
+        hipStream_t st;
+        HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        std::unique_ptr<void, void(*)(void*)> stGuard((void*)st, [](void* s) { (void)hipStreamDestroy((hipStream_t)s); });
+        TrainAlloc A;
+        float* feat = A.get<float>(N * 4);
+        float* target = A.get<float>(N);
+        {   // training set: FP64 Multipole and its l <= baseLmax part at N points
+            TrainAlloc tmp;
+            double* pos = tmp.get<double>(N * 3);
+            double* phi = tmp.get<double>(N);
+            double* phib = tmp.get<double>(N);
+            double* q = tmp.get<double>(N);
+            double* sumsq = tmp.get<double>(1);
+            HIPCHECK(hipMemsetAsync(sumsq, 0, 8, st));
+            HIPCHECK(agxLaunchSamplePoints(N, seed, rmin, rmax, pos, st));
+            HIPCHECK(agxLaunchEval(pot->dev, pot->ddev, AGX_FP64, AGX_FETCH_VEC2, N, pos, phi, nullptr, st));
+            HIPCHECK(agxLaunchEval(S->base->dev, S->base->ddev, AGX_FP64, AGX_FETCH_VEC2, N, pos, phib, nullptr, st));
+            HIPCHECK(agxLaunchResidual(N, pos, phi, phib, lnScale, feat, q, sumsq, st));
+            double ss = 0;
+            HIPCHECK(hipMemcpyAsync(&ss, sumsq, 8, hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+            const double rms = sqrt(ss / N);
+            if(!(rms > 0)) throw std::runtime_error("surrogate: the potential has no l > base_lmax part to learn");
+            S->outScale = (float)rms;
+            HIPCHECK(agxLaunchTarget(N, q, 1 / (double)S->outScale, target, st));
+            HIPCHECK(hipStreamSynchronize(st));
+        }
+        AgxTrainBuffers T;
+        allocTrainStep(A, T, batch, nb, steps);
+        T.feat = feat;
+        T.target = target;
+        S->theta = initParams(seed);
+        HIPCHECK(hipMemcpy(T.theta, S->theta.data(), AGX_MLP_NPARAM * 4, hipMemcpyHostToDevice));
+        HIPCHECK(agxLaunchPackTrain(T.theta, T.Wb, st));
+        AgxTrainHyper hp;
+        hp.lr = (float)P.num("lr", 1e-3);
+        hp.beta1 = 0.9f; hp.beta2 = 0.999f; hp.eps = 1e-8f;
+        hp.totalSteps = steps;
+        hp.warmup = (long)P.num("warmup", 200);
+        // one training step captured as a graph and replayed
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = agxEnqueueTrainStep(T, hp, st);
+        HIPCHECK(hipStreamEndCapture(st, &graph));
+        HIPCHECK(ce);
+        std::unique_ptr<void, void(*)(void*)> gGuard((void*)graph, [](void* g) { (void)hipGraphDestroy((hipGraph_t)g); });
+        HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        std::unique_ptr<void, void(*)(void*)> eGuard((void*)exec, [](void* g) { (void)hipGraphExecDestroy((hipGraphExec_t)g); });
+        for(long s = 0; s < steps; s++) HIPCHECK(hipGraphLaunch(exec, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        HIPCHECK(hipMemcpy(S->theta.data(), T.theta, AGX_MLP_NPARAM * 4, hipMemcpyDeviceToHost));
+        S->lossHist.resize(steps);
+        HIPCHECK(hipMemcpy(S->lossHist.data(), T.lossHist, steps * 4, hipMemcpyDeviceToHost));
+        packInference(*S);
+        return S.release();
+    } catch(std::exception& e) { fail(AGX_EDEVICE, e.what()); return nullptr; }
+}
+
+void agx_surrogate_destroy(agx_surrogate* s) { delete s; }
+
+int agx_surrogate_info(const agx_surrogate* s, float* theta, double* meta)
+{
+    if(!s) return fail(AGX_EINVAL, "NULL surrogate");
+    if(theta) memcpy(theta, s->theta.data(), AGX_MLP_NPARAM * 4);
+    if(meta) {
+        meta[0] = s->baseLmax; meta[1] = s->lnScale; meta[2] = s->outScale; meta[3] = s->rmin;
+        meta[4] = s->rmax; meta[5] = (double)s->lossHist.size();
+        meta[6] = s->lossHist.empty() ? NAN : s->lossHist.back();
+        meta[7] = AGX_MLP_NPARAM;
+    }
+    return AGX_OK;
+}
+
+long agx_surrogate_loss_history(const agx_surrogate* s, float* out, long cap)
+{
+    if(!s) return -fail(AGX_EINVAL, "NULL surrogate");
+    const long n = std::min<long>(cap, (long)s->lossHist.size());
+    if(out && n > 0) memcpy(out, s->lossHist.data(), n * 4);
+    return (long)s->lossHist.size();
+}
+
+int agx_surrogate_eval_many_device(const agx_surrogate* s, long n, const double* d_pos, double* d_phi,
+    double* d_grad, void* stream)
+{
+    if(!s) return fail(AGX_EINVAL, "NULL surrogate");
+    if(n < 0 || (n > 0 && !d_pos)) return fail(AGX_EINVAL, "invalid arrays");
+    hipError_t e = agxLaunchSurrogateEval(s->dM, s->base->ddev, n, d_pos, d_phi, d_grad, (hipStream_t)stream);
+    return e == hipSuccess ? AGX_OK : fail(AGX_EDEVICE, hipGetErrorString(e));
+}
+
+int agx_surrogate_eval_many(const agx_surrogate* s, long n, const double* pos, double* phi, double* grad)
+{
+    g_err.clear();
+    if(!s) return fail(AGX_EINVAL, "NULL surrogate");
+    if(n == 0) return AGX_OK;
+    if(n < 0 || !pos) return fail(AGX_EINVAL, "invalid arrays");
+    try {
+        HIPCHECK(hipSetDevice(s->pot->device));
+        DevBuf<double> dp(n * 3), dphi(phi ? n : 0), dg(grad ? n * 3 : 0);
+        HIPCHECK(hipMemcpy(dp.p, pos, n * 24, hipMemcpyHostToDevice));
+        HIPCHECK(agxLaunchSurrogateEval(s->dM, s->base->ddev, n, dp.p, dphi.p, dg.p, nullptr));
+        HIPCHECK(hipDeviceSynchronize());
+        if(phi) HIPCHECK(hipMemcpy(phi, dphi.p, n * 8, hipMemcpyDeviceToHost));
+        if(grad) HIPCHECK(hipMemcpy(grad, dg.p, n * 24, hipMemcpyDeviceToHost));
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+int agx_surrogate_network(const agx_surrogate* s, long n, const float* feat, float* out, float* dfeat)
+{
+    g_err.clear();
+    if(!s) return fail(AGX_EINVAL, "NULL surrogate");
+    if(n == 0) return AGX_OK;
+    if(n < 0 || !feat) return fail(AGX_EINVAL, "invalid arrays");
+    try {
+        HIPCHECK(hipSetDevice(s->pot->device));
+        DevBuf<float> df(n * 4), dout(out ? n : 0), dd(dfeat ? n * 4 : 0);
+        HIPCHECK(hipMemcpy(df.p, feat, n * 16, hipMemcpyHostToDevice));
+        HIPCHECK(agxLaunchMlpFeatures(s->dM, n, df.p, dout.p, dd.p, nullptr));
+        HIPCHECK(hipDeviceSynchronize());
+        if(out) HIPCHECK(hipMemcpy(out, dout.p, n * 4, hipMemcpyDeviceToHost));
+        if(dfeat) HIPCHECK(hipMemcpy(dfeat, dd.p, n * 16, hipMemcpyDeviceToHost));
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+int agx_surrogate_train_grad(const agx_surrogate* s, long n, const float* feat, const float* target,
+    float* loss, float* grad)
+{
+    g_err.clear();
+    if(!s) return fail(AGX_EINVAL, "NULL surrogate");
+    if(n < 512 || n % 512 || !feat || !target) return fail(AGX_EINVAL, "n must be a positive multiple of 512");
+    try {
+        HIPCHECK(hipSetDevice(s->pot->device));
+        TrainAlloc A;
+        AgxTrainBuffers T;
+        allocTrainStep(A, T, (int)n, 1, 1);
+        float* df = A.get<float>(n * 4);
+        float* dt = A.get<float>(n);
+        HIPCHECK(hipMemcpy(df, feat, n * 16, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dt, target, n * 4, hipMemcpyHostToDevice));
+        T.feat = df;
+        T.target = dt;
+        HIPCHECK(hipMemcpy(T.theta, s->theta.data(), AGX_MLP_NPARAM * 4, hipMemcpyHostToDevice));
+        HIPCHECK(agxLaunchPackTrain(T.theta, T.Wb, nullptr));
+        AgxTrainHyper hp;
+        memset(&hp, 0, sizeof(hp));   // lr = 0: loss and gradient only
+        HIPCHECK(agxEnqueueTrainStep(T, hp, nullptr));
+        HIPCHECK(hipDeviceSynchronize());
+        if(loss) HIPCHECK(hipMemcpy(loss, T.lossHist, 4, hipMemcpyDeviceToHost));
+        if(grad) HIPCHECK(hipMemcpy(grad, T.grad, AGX_MLP_NPARAM * 4, hipMemcpyDeviceToHost));
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+int agx_debug_gemm(int M, int N, int K, int splits, int a_kcontig, int b_kcontig, const unsigned short* A,
+    const unsigned short* B, float* C)
+{
+    g_err.clear();
+    if(M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) return fail(AGX_EINVAL, "invalid arguments");
+    try {
+        DevBuf<uint16_t> da((size_t)M * K), db((size_t)K * N);
+        DevBuf<float> dc((size_t)M * N);
+        HIPCHECK(hipMemcpy(da.p, A, (size_t)M * K * 2, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(db.p, B, (size_t)K * N * 2, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemset(dc.p, 0, (size_t)M * N * 4));
+        hipError_t e = agxLaunchGemm(M, N, K, splits, a_kcontig != 0, b_kcontig != 0, da.p,
+            a_kcontig ? K : M, db.p, b_kcontig ? K : N, dc.p, N, splits > 1 ? 2 : 0, nullptr);
+        if(e != hipSuccess) return fail(AGX_EINVAL, std::string("gemm: ") + hipGetErrorString(e));
+        HIPCHECK(hipDeviceSynchronize());
+        HIPCHECK(hipMemcpy(C, dc.p, (size_t)M * N * 4, hipMemcpyDeviceToHost));
         return AGX_OK;
     } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
 }

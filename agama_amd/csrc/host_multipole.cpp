@@ -875,7 +875,19 @@ void writeCoefFile(const std::string& path, const Coefs& C)
     }
 }
 
-HostTables buildTables(const Coefs& C)
+Coefs truncateCoefs(const Coefs& C, int lmax)
+{
+    Coefs t = C;
+    const int L = C.lmax();
+    for(int l = lmax + 1; l <= L; l++)
+        for(int m = -l; m <= l; m++) {
+            std::fill(t.Phi[l * (l + 1) + m].begin(), t.Phi[l * (l + 1) + m].end(), 0.);
+            std::fill(t.dPhi[l * (l + 1) + m].begin(), t.dPhi[l * (l + 1) + m].end(), 0.);
+        }
+    return t;
+}
+
+HostTables buildTables(const Coefs& C, bool allowLowOrder)
 {
     const int nr = (int)C.radii.size();
     Ind ind = indicesFromCoefs2(C.Phi, C.dPhi);
@@ -884,7 +896,7 @@ HostTables buildTables(const Coefs& C)
         correct &= (int)C.Phi[c].size() == nr && (int)C.dPhi[c].size() == nr;
     for(int k = 1; correct && k < nr; k++) correct &= C.radii[k] > C.radii[k-1];
     if(!correct) throw std::invalid_argument("Multipole: invalid radial grid");
-    if(ind.lmax <= 2)
+    if(ind.lmax <= 2 && !allowLowOrder)
         throw std::invalid_argument("Multipole: expansions with lmax<=2 (the reference's 1-d spline "
             "variant) are not supported by this build");
     if(ind.mmax > AGX_MAX_M) throw std::invalid_argument("Multipole: mmax > 16 is not supported");

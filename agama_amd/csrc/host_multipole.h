@@ -66,6 +66,10 @@ struct HostTables {
     std::vector<double> gridR, gridT, nodes;   // nodes: see agx_tables.h
     HostPowerLaw inner, outer;
 };
-HostTables buildTables(const Coefs& c);
+// allowLowOrder: also build the 2-d tables for expansions with lmax <= 2 (the reference uses
+// 1-d splines there; the 2-d form is used internally, e.g. for the surrogate's l<=2 baseline)
+HostTables buildTables(const Coefs& c, bool allowLowOrder = false);
+// the expansion with every term of degree l > lmax set to zero
+Coefs truncateCoefs(const Coefs& c, int lmax);
 
 }  // namespace agx

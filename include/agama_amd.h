@@ -156,6 +156,45 @@ void agama_deletePotential(agama_Potential* pot);
 double agama_evalPotential(const agama_Potential* pot, const double pos[3], double time,
     double deriv[3], double deriv2[6]);
 
+
+/* ---- MLP potential surrogate (no counterpart in the reference) ----
+   Phi_NN(x) = Phi_base(x) (1 + out_scale * MLP(|x|/r, |y|/r, |z|/r, ln r / ln_scale)), with
+   Phi_base the potential's expansion truncated to l <= base_lmax (FP64) and MLP a 4-256-256-
+   256-256-1 SiLU network evaluated in bf16 on MFMA with fp32 accumulation; the force is the
+   exact gradient of the network by a hand-written backward.  Parameters: AGX_MLP_NPARAM
+   floats, W1[256x4] b1 W2[256x256] b2 W3 b3 W4 b4 w5[256] b5, weights row-major [out][in]. */
+#define AGX_MLP_NPARAM_C 198913
+typedef struct agx_surrogate agx_surrogate;
+/* train on the GPU: params "samples=20e6, batch=16384, steps=6000, lr=1e-3, seed=1,
+   baseLmax=2, rmin=1e-2, rmax=1e2, warmup=200" (all optional).  Samples have ln r uniform in
+   [ln rmin, ln rmax] and isotropic directions; targets are the FP64 Multipole's relative
+   residual over the base; Adam with cosine decay; one training step is a replayed HIP graph.
+   The potential must outlive the surrogate. */
+agx_surrogate* agx_surrogate_train(const agx_potential* pot, const char* params);
+/* from given parameters (e.g. saved by a caller from agx_surrogate_info) */
+agx_surrogate* agx_surrogate_create(const agx_potential* pot, const float* theta, int base_lmax,
+                                    float ln_scale, float out_scale, double rmin, double rmax);
+void agx_surrogate_destroy(agx_surrogate* s);
+/* theta[AGX_MLP_NPARAM_C] and meta[8] = {base_lmax, ln_scale, out_scale, rmin, rmax,
+   training steps, last training loss, AGX_MLP_NPARAM_C}; either may be NULL */
+int agx_surrogate_info(const agx_surrogate* s, float* theta, double* meta);
+/* training loss per step (mean squared error of the scaled residual); returns the count */
+long agx_surrogate_loss_history(const agx_surrogate* s, float* out, long cap);
+/* Phi_NN and +grad Phi_NN at n points (host / device arrays) */
+int agx_surrogate_eval_many(const agx_surrogate* s, long n, const double* pos, double* phi, double* grad);
+int agx_surrogate_eval_many_device(const agx_surrogate* s, long n, const double* d_pos,
+                                   double* d_phi, double* d_grad, void* stream);
+/* the raw network on features feat[n][4] (float): out[n] and d out / d feat [n][4] */
+int agx_surrogate_network(const agx_surrogate* s, long n, const float* feat, float* out, float* dfeat);
+/* one forward/backward of the training kernels on a batch (n a multiple of 512): mean
+   squared error and its gradient w.r.t. the parameters (no update) */
+int agx_surrogate_train_grad(const agx_surrogate* s, long n, const float* feat, const float* target,
+                             float* loss, float* grad);
+/* the training GEMM on host arrays: C[M][N] = sum_k A(m,k) B(k,n), bf16 bit patterns,
+   A row-major [M][K] if a_kcontig else [K][M], B [N][K] if b_kcontig else [K][N] */
+int agx_debug_gemm(int M, int N, int K, int splits, int a_kcontig, int b_kcontig,
+                   const unsigned short* A, const unsigned short* B, float* C);
+
 #ifdef __cplusplus
 }
 #endif
