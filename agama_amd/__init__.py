@@ -168,7 +168,7 @@ class Potential:
 
 def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, accuracy=1e-8,
           maxNumSteps=None, Omega=0.0, dtype="float32", separateTime=False, method="dop853",
-          verbose=False, info=None, precision=PRECISION_FP64):
+          verbose=False, info=None, precision=PRECISION_FP64, max_rel_de=1e-6):
     """Integrate orbits with DOP853 on the GPU - reference: agama.orbit
     (src/interface_python.cpp:7689-8281, output conventions :7941-7961).
 
@@ -176,8 +176,14 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
     trajsize a tuple (times[N,S], traj[N,S,6]); with separateTime=False an (N,2) object array
     of [times, traj] pairs; a single orbit (ic of length 6) drops the leading dimension.
     `info`, if a dict, receives per-orbit diagnostics (nsteps, nattempts, nevals, maxde, status,
-    final) that the reference does not return."""
-    if not isinstance(potential, Potential):
+    final) that the reference does not return.
+
+    `potential` may also be a `Surrogate`: orbits are then integrated with the MLP surrogate
+    force while their energy error (judged by the FP64 Multipole) stays within `max_rel_de`
+    and they stay within the training radii, and otherwise integrated again with the FP64
+    Multipole (status + 8; info["n_fallback"] counts them)."""
+    surrogate = isinstance(potential, Surrogate)
+    if not surrogate and not isinstance(potential, Potential):
         raise TypeError("Argument 'potential' must be a valid Potential object")
     if str(method).lower() != "dop853":
         raise AgamaAmdError("only method='dop853' is provided by this version")
@@ -226,11 +232,20 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
     nev = np.empty(n, dtype=np.int64)
     maxde = np.empty(n)
     status = np.empty(n, dtype=np.int32)
-    check(lib.agx_orbit_batch(potential.handle, n, ptr(ic2), ptr(ts), ptr(tt), ptr(sz), float(accuracy),
-                              float(Omega), maxsteps, ptr(times), ptr(traj), ptr(final), ptr(nsteps),
-                              ptr(natt), ptr(nev), ptr(maxde), ptr(status), int(precision)), "orbit")
+    nfb = np.zeros(1, dtype=np.int64)
+    if surrogate:
+        check(lib.agx_surrogate_orbit_batch(potential._h, n, ptr(ic2), ptr(ts), ptr(tt), ptr(sz), float(accuracy),
+                                            float(Omega), maxsteps, float(max_rel_de), ptr(times), ptr(traj),
+                                            ptr(final), ptr(nsteps), ptr(natt), ptr(nev), ptr(maxde), ptr(status),
+                                            ptr(nfb)), "surrogate orbit")
+    else:
+        check(lib.agx_orbit_batch(potential.handle, n, ptr(ic2), ptr(ts), ptr(tt), ptr(sz), float(accuracy),
+                                  float(Omega), maxsteps, ptr(times), ptr(traj), ptr(final), ptr(nsteps),
+                                  ptr(natt), ptr(nev), ptr(maxde), ptr(status), int(precision)), "orbit")
     if isinstance(info, dict):
         info.update(nsteps=nsteps, nattempts=natt, nevals=nev, maxde=maxde, status=status, final=final)
+        if surrogate:
+            info["n_fallback"] = int(nfb[0])
     if not have_traj:
         return final[0] if single else final
 

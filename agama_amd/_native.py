@@ -55,6 +55,13 @@ def _load():
         "agx_surrogate_eval_many_device": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp]),
         "agx_surrogate_network": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp]),
         "agx_surrogate_train_grad": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp]),
+        "agx_surrogate_orbit_batch": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, ctypes.c_double,
+                                                     ctypes.c_double, ctypes.c_long, ctypes.c_double, _vp, _vp,
+                                                     _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "agx_surrogate_orbit_batch_device": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp,
+                                                            ctypes.c_double, ctypes.c_double, ctypes.c_long,
+                                                            ctypes.c_double, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                            _vp, _vp, _vp, _vp]),
         "agx_debug_gemm": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
         "agama_getError": (ctypes.c_char_p, []),
@@ -79,7 +86,7 @@ EXPORTED = ["agx_last_error", "agx_potential_create", "agx_potential_from_coefs"
             "agx_surrogate_train", "agx_surrogate_create", "agx_surrogate_destroy",
             "agx_surrogate_info", "agx_surrogate_loss_history", "agx_surrogate_eval_many",
             "agx_surrogate_eval_many_device", "agx_surrogate_network", "agx_surrogate_train_grad",
-            "agx_debug_gemm",
+            "agx_surrogate_orbit_batch", "agx_surrogate_orbit_batch_device", "agx_debug_gemm",
             "agama_getError", "agama_createPotential", "agama_deletePotential",
             "agama_evalPotential"]
 

@@ -764,6 +764,115 @@ int agx_surrogate_train_grad(const agx_surrogate* s, long n, const float* feat, 
     } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
 }
 
+int agx_surrogate_orbit_batch_device(const agx_surrogate* s, long n, const double* d_ic,
+    const double* d_tstart, const double* d_ttotal, const int* d_trajsize, const long* d_traj_offset,
+    double accuracy, double Omega, long maxNumSteps, double max_rel_de, double* d_time, double* d_traj,
+    double* d_final, long* d_nsteps, long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status,
+    long* n_fallback, void* stream)
+{
+    if(!s) return fail(AGX_EINVAL, "NULL surrogate");
+    if(n_fallback) *n_fallback = 0;
+    if(n == 0) return AGX_OK;
+    if(n < 0 || n > 0x7fffffffL || !d_ic || !d_ttotal || !d_trajsize)
+        return fail(AGX_EINVAL, "invalid orbit arrays");
+    if((d_time || d_traj) && !d_traj_offset) return fail(AGX_EINVAL, "trajectory output needs offsets");
+    if(!(accuracy > 0)) return fail(AGX_EINVAL, "accuracy must be positive");
+    if(!(Omega > -INFINITY && Omega < INFINITY)) return fail(AGX_EINVAL, "Omega must be finite");
+    if(maxNumSteps <= 0) return fail(AGX_EINVAL, "maxNumSteps must be positive");
+    if(!(max_rel_de > 0)) return fail(AGX_EINVAL, "max_rel_de must be positive");
+    const agx_potential* pot = s->pot;
+    try {
+        hipStream_t st = (hipStream_t)stream;
+        const size_t schedBytes = agxScheduleWorkspace(n), ordBytes = ((size_t)n * 4 + 255) / 256 * 256;
+        void* work = nullptr;
+        HIPCHECK(hipMallocAsync(&work, schedBytes + 2 * ordBytes + 256, st));
+        char* w = static_cast<char*>(work);
+        int* ord = reinterpret_cast<int*>(w + schedBytes);
+        int* list = reinterpret_cast<int*>(w + schedBytes + ordBytes);
+        unsigned int* nfb = reinterpret_cast<unsigned int*>(w + schedBytes + 2 * ordBytes);
+        auto release = [&]() { (void)hipFreeAsync(work, st); };
+        hipError_t e = agxLaunchSchedule(pot->dev, pot->ddev, n, d_ic, d_ttotal, ord, work, st);
+        if(e != hipSuccess) { release(); return fail(AGX_EDEVICE, std::string("schedule: ") + hipGetErrorString(e)); }
+        AgxOrbitArgs A;
+        A.n = n; A.ic = d_ic; A.tstart = d_tstart; A.ttotal = d_ttotal; A.trajsize = d_trajsize;
+        A.trajOffset = d_traj_offset; A.order = ord; A.accuracy = accuracy; A.Omega = Omega;
+        A.maxNumSteps = maxNumSteps; A.outTime = d_time; A.outTraj = d_traj; A.outFinal = d_final;
+        A.outNsteps = d_nsteps; A.outNattempts = d_nattempts; A.outNevals = d_nevals;
+        A.outMaxde = d_maxde; A.outStatus = d_status; A.counter = pot->dcounter;
+        AgxSurrogateOrbitArgs SA;
+        SA.maxRelDE = max_rel_de; SA.rmin = s->rmin; SA.rmax = s->rmax; SA.fallback = list; SA.nfallback = nfb;
+        int dev = 0, cus = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const long need = (n + 63) / 64;
+        const int blocks = (int)std::min<long>(need, (long)cus * agxOrbitSurrogateBlocksPerCU());
+        HIPCHECK(hipMemsetAsync(pot->dcounter, 0, sizeof(unsigned int), st));
+        HIPCHECK(hipMemsetAsync(nfb, 0, sizeof(unsigned int), st));
+        e = agxLaunchOrbitSurrogate(pot->ddev, s->base->ddev, s->dM, A, SA, blocks, st);
+        if(e != hipSuccess) { release(); return fail(AGX_EDEVICE, std::string("surrogate orbit launch: ") + hipGetErrorString(e)); }
+        unsigned int nf = 0;
+        HIPCHECK(hipMemcpyAsync(&nf, nfb, sizeof(nf), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        if(nf > 0) {   // the fallback orbits, integrated again with the FP64 Multipole
+            A.n = nf;
+            A.order = list;
+            int fb, bs;
+            orbitGeometry(nf, AGX_FP64, fb, bs);
+            HIPCHECK(hipMemsetAsync(pot->dcounter, 0, sizeof(unsigned int), st));
+            e = agxLaunchOrbit(pot->dev, pot->ddev, AGX_FP64, g_fetchOrbit, A, fb, bs, st);
+            if(e == hipSuccess) e = agxLaunchMarkFallback(list, nf, d_status, st);
+            if(e != hipSuccess) { release(); return fail(AGX_EDEVICE, std::string("fallback launch: ") + hipGetErrorString(e)); }
+        }
+        release();
+        if(n_fallback) *n_fallback = nf;
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+int agx_surrogate_orbit_batch(const agx_surrogate* s, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, double accuracy, double Omega, long maxNumSteps,
+    double max_rel_de, double* out_time, double* out_traj, double* out_final, long* out_nsteps,
+    long* out_nattempts, long* out_nevals, double* out_maxde, int* out_status, long* n_fallback)
+{
+    g_err.clear();
+    if(!s) return fail(AGX_EINVAL, "NULL surrogate");
+    if(n_fallback) *n_fallback = 0;
+    if(n == 0) return AGX_OK;
+    if(n < 0 || !ic || !ttotal || !trajsize) return fail(AGX_EINVAL, "invalid orbit arrays");
+    std::vector<long> offs(n);
+    long rows = 0;
+    for(long i = 0; i < n; i++) {
+        if(trajsize[i] < 1) return fail(AGX_EUNSUPPORTED, "trajsize must be >= 1");
+        offs[i] = rows;
+        rows += trajsize[i];
+    }
+    try {
+        HIPCHECK(hipSetDevice(s->pot->device));
+        DevBuf<double> dic(n * 6), dts(tstart ? n : 0), dtt(n), dtime(out_time ? rows : 0),
+            dtraj(out_traj ? rows * 6 : 0), dfin(out_final ? n * 6 : 0), dmde(out_maxde ? n : 0);
+        DevBuf<int> dsz(n), dst(out_status ? n : 0);
+        DevBuf<long> doff(n), dns(out_nsteps ? n : 0), dna(out_nattempts ? n : 0), dne(out_nevals ? n : 0);
+        HIPCHECK(hipMemcpy(dic.p, ic, n * 48, hipMemcpyHostToDevice));
+        if(tstart) HIPCHECK(hipMemcpy(dts.p, tstart, n * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dtt.p, ttotal, n * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dsz.p, trajsize, n * 4, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(doff.p, offs.data(), n * 8, hipMemcpyHostToDevice));
+        int rc = agx_surrogate_orbit_batch_device(s, n, dic.p, dts.p, dtt.p, dsz.p, doff.p, accuracy, Omega,
+            maxNumSteps, max_rel_de, dtime.p, dtraj.p, dfin.p, dns.p, dna.p, dne.p, dmde.p, dst.p, n_fallback, nullptr);
+        if(rc) return rc;
+        HIPCHECK(hipDeviceSynchronize());
+        if(out_time) HIPCHECK(hipMemcpy(out_time, dtime.p, rows * 8, hipMemcpyDeviceToHost));
+        if(out_traj) HIPCHECK(hipMemcpy(out_traj, dtraj.p, rows * 48, hipMemcpyDeviceToHost));
+        if(out_final) HIPCHECK(hipMemcpy(out_final, dfin.p, n * 48, hipMemcpyDeviceToHost));
+        if(out_nsteps) HIPCHECK(hipMemcpy(out_nsteps, dns.p, n * 8, hipMemcpyDeviceToHost));
+        if(out_nattempts) HIPCHECK(hipMemcpy(out_nattempts, dna.p, n * 8, hipMemcpyDeviceToHost));
+        if(out_nevals) HIPCHECK(hipMemcpy(out_nevals, dne.p, n * 8, hipMemcpyDeviceToHost));
+        if(out_maxde) HIPCHECK(hipMemcpy(out_maxde, dmde.p, n * 8, hipMemcpyDeviceToHost));
+        if(out_status) HIPCHECK(hipMemcpy(out_status, dst.p, n * 4, hipMemcpyDeviceToHost));
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
 int agx_debug_gemm(int M, int N, int K, int splits, int a_kcontig, int b_kcontig, const unsigned short* A,
     const unsigned short* B, float* C)
 {

@@ -44,3 +44,15 @@ hipError_t agxLaunchSamplePoints(long n, unsigned long long seed, double rmin, d
 hipError_t agxLaunchResidual(long n, const double* pos, const double* phi, const double* phib, float lnScale,
     float* feat, double* q, double* sumsq, hipStream_t s);
 hipError_t agxLaunchTarget(long n, const double* q, double inv, float* target, hipStream_t s);
+
+// surrogate orbit integration: per-orbit fallback to the FP64 Multipole
+struct AgxSurrogateOrbitArgs {
+    double maxRelDE;          // fallback when |E/E0 - 1| (energy from the FP64 Multipole) exceeds this
+    double rmin, rmax;        // ... or when an evaluation point leaves [rmin, rmax]
+    int* fallback;            // orbit indices handed over to the FP64 kernel
+    unsigned int* nfallback;  // their count (zero on entry)
+};
+int agxOrbitSurrogateBlocksPerCU();
+hipError_t agxLaunchOrbitSurrogate(const AgxMultipoleDev* dP, const AgxMultipoleDev* dBase, const AgxMlpDev* dM,
+    const AgxOrbitArgs& A, const AgxSurrogateOrbitArgs& SA, int numBlocks, hipStream_t stream);
+hipError_t agxLaunchMarkFallback(const int* list, long n, int* status, hipStream_t stream);

@@ -190,6 +190,24 @@ int agx_surrogate_network(const agx_surrogate* s, long n, const float* feat, flo
    squared error and its gradient w.r.t. the parameters (no update) */
 int agx_surrogate_train_grad(const agx_surrogate* s, long n, const float* feat, const float* target,
                              float* loss, float* grad);
+/* Orbit integration with the surrogate force, the arguments and outputs of agx_orbit_batch
+   plus max_rel_de: every orbit is integrated with the surrogate while its energy error
+   |E/E0 - 1|, measured with the FP64 Multipole at every accepted step, stays at or below
+   max_rel_de and its evaluation points stay within the training radii; otherwise it is
+   integrated again from its initial conditions with the FP64 Multipole (its status gets
+   +8).  n_fallback (may be NULL) receives the number of such orbits.  The device variant
+   synchronises the stream once, to size the fallback launch. */
+int agx_surrogate_orbit_batch(const agx_surrogate* s, long n, const double* ic, const double* tstart,
+                              const double* ttotal, const int* trajsize, double accuracy, double Omega,
+                              long maxNumSteps, double max_rel_de, double* out_time, double* out_traj,
+                              double* out_final, long* out_nsteps, long* out_nattempts, long* out_nevals,
+                              double* out_maxde, int* out_status, long* n_fallback);
+int agx_surrogate_orbit_batch_device(const agx_surrogate* s, long n, const double* d_ic,
+                                     const double* d_tstart, const double* d_ttotal, const int* d_trajsize,
+                                     const long* d_traj_offset, double accuracy, double Omega,
+                                     long maxNumSteps, double max_rel_de, double* d_time, double* d_traj,
+                                     double* d_final, long* d_nsteps, long* d_nattempts, long* d_nevals,
+                                     double* d_maxde, int* d_status, long* n_fallback, void* stream);
 /* the training GEMM on host arrays: C[M][N] = sum_k A(m,k) B(k,n), bf16 bit patterns,
    A row-major [M][K] if a_kcontig else [K][M], B [N][K] if b_kcontig else [K][N] */
 int agx_debug_gemm(int M, int N, int K, int splits, int a_kcontig, int b_kcontig,

@@ -157,3 +157,66 @@ def test_short_training_run(pot, tmp_path):
     sur2 = agama_amd.Surrogate.load(pot, path)
     phi2, g2 = sur2._eval(pts[:1000], True, True)
     assert np.array_equal(phi2, phi[:1000]) and np.array_equal(g2, g[:1000])
+
+
+@pytest.fixture(scope="module")
+def trained(pot):
+    import agama_amd
+    return agama_amd.Surrogate.train(pot, samples=4e6, steps=3000, batch=16384, seed=3)
+
+
+def test_surrogate_orbits_and_fallback(pot, trained):
+    """every orbit either keeps its FP64-judged energy error within max_rel_de with the
+    surrogate force, or is integrated again with the FP64 Multipole - and then equals the
+    plain FP64 integration exactly"""
+    import agama_amd
+    from agama_amd import workloads
+    from oracle.pyoracle import OraclePotential
+    orc = OraclePotential(COEF)
+    ic, T = workloads.triaxial_dehnen_orbits(512, lambda p: orc.eval(p)[0], pot.coefs(), seed=31, periods=10)
+    ic[0, :3] = [150.0, 0.0, 0.0]          # outside the training radii: must fall back
+    info_s, info_p = {}, {}
+    agama_amd.orbit(potential=trained, ic=ic, time=T, accuracy=1e-8, info=info_s, max_rel_de=1e-6)
+    agama_amd.orbit(potential=pot, ic=ic, time=T, accuracy=1e-8, info=info_p)
+    st = info_s["status"]
+    fb = (st & 8) != 0
+    assert fb[0]
+    assert info_s["n_fallback"] == fb.sum()
+    assert np.all((st & ~8) == 0) and np.all(info_p["status"] == 0)
+    assert np.array_equal(info_s["final"][fb], info_p["final"][fb])
+    assert np.array_equal(info_s["nsteps"][fb], info_p["nsteps"][fb])
+    ok = ~fb
+    assert np.all(info_s["maxde"][ok] <= 1e-6)
+    pos = np.linalg.norm(info_s["final"][ok, :3] - info_p["final"][ok, :3], axis=1) / \
+        np.linalg.norm(info_p["final"][ok, :3], axis=1)
+    steps_ratio = info_s["nsteps"][ok].sum() / max(info_p["nsteps"][ok].sum(), 1)
+    print(f"surrogate orbits: {ok.sum()} of {len(ok)} kept the surrogate (dE/E <= 1e-6 over 10 "
+          f"periods); their final position vs FP64 median {np.median(pos) if ok.any() else np.nan:.2e}; "
+          f"steps surrogate/FP64 {steps_ratio:.3f}; max dE/E surrogate {info_s['maxde'][ok].max() if ok.any() else np.nan:.2e}")
+
+
+def test_surrogate_orbits_loose_threshold(pot, trained):
+    """with a loose energy threshold the surrogate integrates every in-range orbit itself.
+    The bf16 rounding in the network makes its force rough on small scales, which the DOP853
+    error control at accuracy 1e-8 answers with much shorter steps: the run is bounded by
+    maxNumSteps, and the step counts and the FP64-judged energy error are reported"""
+    import agama_amd
+    from agama_amd import workloads
+    from oracle.pyoracle import OraclePotential
+    orc = OraclePotential(COEF)
+    ic, T = workloads.triaxial_dehnen_orbits(32, lambda p: orc.eval(p)[0], pot.coefs(), seed=32, periods=1)
+    info, ref = {}, {}
+    agama_amd.orbit(potential=trained, ic=ic, time=T, accuracy=1e-8, info=info, max_rel_de=1.0,
+                    maxNumSteps=4000)
+    agama_amd.orbit(potential=pot, ic=ic, time=T, accuracy=1e-8, info=ref)
+    st = info["status"]
+    fb = (st & 8) != 0
+    capped = (st & ~8) == 2
+    assert np.all(np.isin(st & ~8, (0, 2)))
+    sel = ~fb & ~capped
+    mde = info["maxde"][~fb]
+    print(f"loose threshold, 1 period: {fb.sum()} fallbacks (radius range), {capped.sum()} reached "
+          f"maxNumSteps; steps surrogate/FP64 on completed orbits "
+          f"{info['nsteps'][sel].sum() / max(ref['nsteps'][sel].sum(), 1):.1f}; surrogate dE/E median "
+          f"{np.median(mde):.2e} max {mde.max():.2e}")
+    assert np.median(mde) < 1e-2
