@@ -6,9 +6,13 @@
 // (agx_mlp.h), the l <= 2 FP64 base is evaluated per lane, and the force is assembled in FP64.
 // The FP64 Multipole stays the judge: it gives the initial energy and the energy after every
 // accepted step, so the recorded |E/E0 - 1| is the true energy error of the surrogate orbit.
+// The three evaluations that start an orbit (initial energy, first-step estimate) use the FP64
+// Multipole, so the first step size is free of the network's rounding noise.
 // An orbit falls back - it is handed over to the FP64 kernel and integrated again from its
-// initial conditions - as soon as that error exceeds the threshold or an evaluation point
-// leaves the training radii.
+// initial conditions - as soon as that error exceeds the threshold, an evaluation point
+// leaves the training radii, or an accepted step is 64 times shorter than the first one (the
+// bf16 rounding in the network makes its force rough on small scales, and the error control
+// can answer that with ever shorter steps).
 //
 // The stage sequence of every lane is aligned to the wavefront's iteration counter (the first
 // stage of an attempt starts when it % 12 == 0; a lane that has just initialised waits), so
@@ -42,6 +46,7 @@ __global__ __launch_bounds__(SBS) void orbitSurrogateKernel(const AgxMultipoleDe
     const double accRel = A.accuracy, rmin2 = SA.rmin * SA.rmin, rmax2 = SA.rmax * SA.rmax;
     const double kOut = M.outScale;
     unsigned int it = 0;
+    double hRef = 0;   // first step of the lane's current orbit
     while(true) {
         const bool active = L.phase != PH_IDLE;
         if(!__any(active)) break;
@@ -74,25 +79,30 @@ __global__ __launch_bounds__(SBS) void orbitSurrogateKernel(const AgxMultipoleDe
         mlpChain(M, x, y, z, df, ox, oy, oz);
         const double fac = 1 + kOut * (double)out, pk = pb * kOut;
         const double gx = gbx * fac + pk * ox, gy = gby * fac + pk * oy, gz = gbz * fac + pk * oz;
-        // ---- the FP64 Multipole as judge: initial energy and energy of accepted steps
-        double pot = pb * fac;
-        if(run && (L.phase == PH_INIT_A || L.phase == PH_FSAL)) {
+        // ---- the FP64 Multipole: initialisation, and judge of the energy of accepted steps
+        double pot = pb * fac, fx = gx, fy = gy, fz = gz;
+        const bool init = L.phase == PH_INIT_A || L.phase == PH_INIT_B || L.phase == PH_INIT_C;
+        if(run && (init || L.phase == PH_FSAL)) {
             FetchVec2 fp{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
             double jx, jy, jz;
             multipoleCar(P, fp, x, y, z, pot, jx, jy, jz);
+            if(init) { fx = jx; fy = jy; fz = jz; }
         }
         if(!run) continue;
         const double r2 = x * x + y * y + z * z;
         bool fallback = !(r2 >= rmin2 && r2 <= rmax2);
         double f[6];
         f[0] = L.y[3]; f[1] = L.y[4]; f[2] = L.y[5];
-        f[3] = -gx * ca + gy * sa;
-        f[4] = -gy * ca - gx * sa;
-        f[5] = -gz;
+        f[3] = -fx * ca + fy * sa;
+        f[4] = -fy * ca - fx * sa;
+        f[5] = -fz;
+        const int phaseBefore = L.phase, stepsBefore = L.nsteps;
         advancePhase(A, L, f, pot, accRel);
+        if(phaseBefore == PH_INIT_C) hRef = fabs(L.h);
         // maxde is updated at accepted steps (also by the one that ends the orbit)
         const double mde = L.f(F_MAXDE);
         if(!(mde <= SA.maxRelDE)) fallback = true;
+        if(L.phase != PH_IDLE && L.nsteps != stepsBefore && fabs(L.f(F_LASTH)) * 64 < hRef) fallback = true;
         if(fallback) {
             const unsigned int slot = atomicAdd(SA.nfallback, 1u);
             SA.fallback[slot] = L.oi;

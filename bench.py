@@ -1,11 +1,20 @@
-"""Benchmark: orbit-steps/s of the batched FP64 DOP853 integrator in the triaxial Dehnen
-Multipole potential (lmax=8) on one MI355X.
+"""Benchmark: orbit-steps/s of the batched DOP853 integrator in the triaxial Dehnen Multipole
+potential (lmax=8) on one MI355X - the MLP surrogate path against the FP64 Multipole path,
+both held to a per-orbit energy error of at most 1e-6.
 
 Workload ("batching at scale"): 1M orbits with initial conditions drawn from the triaxial
 Dehnen density (seed 42), integration times log-uniform between 1 and 300 circular periods,
-accuracy 1e-8, final state plus max relative energy drift per orbit, FP64 throughout.
-One bench step = one persistent-kernel launch that integrates the whole batch; inputs and
-outputs stay resident in HBM.  `value` = accepted orbit-steps / s over the timed steps.
+accuracy 1e-8, final state plus max relative energy error per orbit, FP64 integrator.
+One bench step = one integration of the whole batch; inputs and outputs stay resident in HBM.
+
+Default mode runs both forms on the same initial conditions:
+  FP64 Multipole: one persistent-kernel launch per step (its time and HIP-event kernel time);
+  MLP surrogate: the network is first trained on the GPU (not timed), then each step is one
+    surrogate-orbit call - the surrogate kernel with the FP64 Multipole as energy judge, and
+    the FP64 kernel for the orbits that exceed dE/E = 1e-6 or leave the training radii.
+`value` = accepted orbit-steps / s of the surrogate path (the steps of the final results over
+the whole call); the FP64 path's figure and the ratio are reported next to it.  `--mode
+fp64` runs the FP64 path alone (its figure is then `value`).
 
 Prints one JSON line (rank 0).  `--gpus N` with N > 1 is not supported (single-GPU build).
 """
@@ -48,10 +57,14 @@ def parse():
     ap.add_argument("--cpu-orbits", type=int, default=20_000,
                     help="size of the CPU-baseline sample (0 disables)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["orbit", "eval", "copy"], default="orbit",
-                    help="orbit: the batched integration (default); eval: potential + force at "
-                         "1M points drawn from the density (secondary line); copy: one 1 GiB "
+    ap.add_argument("--mode", choices=["orbit", "fp64", "eval", "mlp", "copy"], default="orbit",
+                    help="orbit: surrogate and FP64 paths (default); fp64: the FP64 path alone; "
+                         "eval: potential + force at 1M points drawn from the density; mlp: the "
+                         "surrogate's fused MFMA evaluation at 1M points; copy: one 1 GiB "
                          "device-to-device copy, the reference for HBM byte counters")
+    ap.add_argument("--train-steps", type=int, default=6000)
+    ap.add_argument("--train-samples", type=float, default=20e6)
+    ap.add_argument("--max-rel-de", type=float, default=1e-6)
     return ap.parse_args()
 
 
@@ -124,7 +137,7 @@ def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build targets one GPU; --gpus N with N > 1 is not supported "
-              "(orbits are independent and would shard by index, see DESIGN.md)", file=sys.stderr)
+              "(orbits are independent and would shard by orbit index)", file=sys.stderr)
         sys.exit(2)
     if args.mode == "eval":
         return bench_eval(args)
@@ -137,6 +150,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"mode": "copy", "bytes_read": a.numel() * 8, "bytes_written": b.numel() * 8}))
         return
+    if args.mode == "mlp":
+        return bench_mlp(args)
     import numpy as np
     import torch
     import agama_amd
@@ -162,7 +177,7 @@ def main():
     d_st = torch.empty(n, dtype=torch.int32, device=dev)
     stream = torch.cuda.Stream()
 
-    def launch():
+    def launch_fp64():
         rc = lib.agx_orbit_batch_device(pot.handle, n, d_ic.data_ptr(), None, d_T.data_ptr(), d_sz.data_ptr(),
                                         d_off.data_ptr(), None, 1e-8, 0.0, int(1e8), d_time.data_ptr(), None,
                                         d_fin.data_ptr(), d_ns.data_ptr(), d_na.data_ptr(), d_ne.data_ptr(),
@@ -170,21 +185,26 @@ def main():
         if rc:
             raise RuntimeError(agama_amd._native.last_error())
 
-    with torch.cuda.stream(stream):
-        for _ in range(args.warmup):
-            launch()
-            print(f"warmup step done", file=sys.stderr, flush=True)
-        torch.cuda.synchronize()
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        ev0.record(stream)
-        for k in range(args.steps):
-            launch()
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-    kernel_s = ev0.elapsed_time(ev1) / 1e3
+    def timed(launch, label):
+        with torch.cuda.stream(stream):
+            for _ in range(args.warmup):
+                launch()
+                torch.cuda.synchronize()
+                print(f"{label}: warmup step done", file=sys.stderr, flush=True)
+            torch.cuda.synchronize()
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            ev0.record(stream)
+            for _ in range(args.steps):
+                launch()
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+        print(f"{label}: {args.steps} timed step(s) done in {wall:.2f} s", file=sys.stderr, flush=True)
+        return wall, ev0.elapsed_time(ev1) / 1e3
+
+    wall, kernel_s = timed(launch_fp64, "FP64 Multipole")
     ns = d_ns.cpu().numpy()
     na = d_na.cpu().numpy()
     de = d_de.cpu().numpy()
@@ -193,13 +213,63 @@ def main():
     attempts = int(na.sum())
     flops = attempts * (12 * FLOPS_PER_EVAL + FLOPS_PER_ATTEMPT_EXTRA) + n * FLOPS_PER_ORBIT_INIT
     per_launch = kernel_s / args.steps
-    value = steps_per_launch * args.steps / wall
+    fp64_value = steps_per_launch * args.steps / wall
     achieved = flops / per_launch / 1e12
     print(f"steps per orbit: median {np.median(ns):.0f}, 99.9% {np.quantile(ns, 0.999):.0f}, max {ns.max()}",
           file=sys.stderr, flush=True)
     print(f"orbits {n}: {steps_per_launch} accepted steps, {attempts} attempts per launch; "
           f"kernel {per_launch*1e3:.1f} ms/launch, wall {wall:.2f} s; status counts {np.bincount(st)}; "
           f"median/max dE/E {np.median(de):.2e}/{np.max(de):.2e}", file=sys.stderr, flush=True)
+
+    surrogate = None
+    value = fp64_value
+    if args.mode == "orbit" and args.precision == "fp64":
+        t0 = time.perf_counter()
+        sur = agama_amd.Surrogate.train(pot, samples=args.train_samples, steps=args.train_steps)
+        train_s = time.perf_counter() - t0
+        hist = sur.loss_history()
+        print(f"surrogate trained in {train_s:.1f} s: loss {hist[:20].mean():.3e} -> {hist[-50:].mean():.3e}",
+              file=sys.stderr, flush=True)
+        nfb = np.zeros(1, dtype=np.int64)
+
+        def launch_sur():
+            rc = lib.agx_surrogate_orbit_batch_device(
+                sur._h, n, d_ic.data_ptr(), None, d_T.data_ptr(), d_sz.data_ptr(), d_off.data_ptr(), 1e-8, 0.0,
+                int(1e8), float(args.max_rel_de), d_time.data_ptr(), None, d_fin.data_ptr(), d_ns.data_ptr(),
+                d_na.data_ptr(), d_ne.data_ptr(), d_de.data_ptr(), d_st.data_ptr(), nfb.ctypes.data,
+                stream.cuda_stream)
+            if rc:
+                raise RuntimeError(agama_amd._native.last_error())
+
+        swall, _ = timed(launch_sur, "MLP surrogate")
+        sns = d_ns.cpu().numpy()
+        sde = d_de.cpu().numpy()
+        sst = d_st.cpu().numpy()
+        value = float(sns.sum()) * args.steps / swall
+        nf = int(nfb[0])
+        # surrogate accuracy at points drawn from the density (judged by the FP64 Multipole)
+        pts = workloads.density_points(200_000, seed=9)
+        r = np.linalg.norm(pts, axis=1)
+        pts = pts[(r > 0.01) & (r < 100)]
+        ophi, og = pot._eval(pts, True, True)
+        sphi, sg = sur._eval(pts, True, True)
+        ferr = np.linalg.norm(sg - og, axis=1) / np.linalg.norm(og, axis=1)
+        perr = np.abs(sphi / ophi - 1)
+        surrogate = {
+            "value": value, "fp64_multipole_value": fp64_value, "surrogate_over_fp64": value / fp64_value,
+            "max_rel_de": args.max_rel_de, "fallback_orbits": nf, "fallback_fraction": nf / n,
+            "orbits_kept_on_surrogate": n - nf,
+            "max_dE_over_E": float(np.max(sde)), "status_counts": np.bincount(sst & 7).tolist(),
+            "force_err_median": float(np.median(ferr)), "force_err_p99": float(np.quantile(ferr, 0.99)),
+            "potential_err_median": float(np.median(perr)),
+            "training": {"samples": int(args.train_samples), "steps": args.train_steps, "batch": 16384,
+                         "seconds": train_s, "final_loss": float(hist[-50:].mean())},
+            "wall_s_per_step": swall / args.steps,
+        }
+        print(f"surrogate path: {value/1e6:.1f} Msteps/s vs FP64 {fp64_value/1e6:.1f} Msteps/s; "
+              f"{nf} of {n} orbits fell back (dE/E > {args.max_rel_de:g} or outside the training radii); "
+              f"force err median {np.median(ferr):.2e}, potential err median {np.median(perr):.2e}",
+              file=sys.stderr, flush=True)
 
     cpu = None
     if args.cpu_orbits > 0:
@@ -212,8 +282,8 @@ def main():
         dt = time.perf_counter() - t0
         cpu = {"value": float(r["nsteps"].sum() / dt), "unit": "orbit-steps/s", "cores": threads,
                "kind": "port",
-               "sample": f"first {m} of the {n} orbits (same seeded ICs and times), C++ OpenMP "
-                         f"restatement built with -O2 (oracle/build/liboracle_fast.so), "
+               "sample": f"first {m} of the {n} orbits (same seeded ICs and times), FP64 Multipole, C++ "
+                         f"OpenMP restatement built with -O2 (oracle/build/liboracle_fast.so), "
                          f"{dt:.1f} s on {threads} threads"}
         print(f"cpu baseline: {cpu['value']/1e6:.2f} Msteps/s on {threads} threads ({dt:.1f} s)",
               file=sys.stderr, flush=True)
@@ -225,22 +295,80 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": (surrogate["wall_s_per_step"] if surrogate else wall / args.steps) * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f64 (fp32 spline tables)",
         "data": "synthetic",
         "config": {"workload": "triaxial Dehnen Multipole lmax=8, 1M orbits, log-uniform 1-300 circular "
-                               "periods, DOP853 accuracy 1e-8, final state + max dE/E",
+                               "periods, DOP853 accuracy 1e-8, final state + max dE/E <= 1e-6",
                    "orbits": n, "accepted_steps_per_step": steps_per_launch,
-                   "attempts_per_step": attempts, "precision": args.precision},
+                   "attempts_per_step": attempts, "precision": args.precision,
+                   "path": "MLP surrogate with FP64 fallback" if surrogate else "FP64 Multipole"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": ORBIT_TRAFFIC_BYTES if (n == 1_000_000 and args.precision == "fp64") else None,
                      "kernel": "orbitKernel (FP64)", "kernel_ms": per_launch * 1e3},
         "cpu_baseline": cpu,
     }
+    if surrogate:
+        out["surrogate"] = surrogate
+        out["fp64_multipole"] = {"value": fp64_value, "ms_per_step": wall / args.steps * 1e3}
+    print(json.dumps(out), flush=True)
+
+
+# algorithmic bf16 MFMA operations of one surrogate evaluation: 3 hidden 256x256 products in
+# the forward and 3 in the input-gradient backward (2 flops per multiply-add)
+MLP_MFMA_FLOPS_PER_EVAL = 6 * 2 * 256 * 256
+PEAK_BF16_TFLOPS = 2500.0    # MI355X dense bf16 MFMA, AMD published figure
+
+
+def bench_mlp(args):
+    """the surrogate's fused evaluation kernel on 1M points: evaluations/s and MFMA rate"""
+    import numpy as np
+    import torch
+    import agama_amd
+    from agama_amd import lib, workloads
+    dev = torch.device("cuda:0")
+    coef = os.path.join(ROOT, "tests", "golden", "dehnen_triax_lmax8.coef")
+    pot = agama_amd.Potential(file=coef)
+    sur = agama_amd.Surrogate.train(pot, samples=min(args.train_samples, 4e6), steps=min(args.train_steps, 1000))
+    n = 1_000_000
+    pts = torch.from_numpy(workloads.density_points(n, seed=7)).to(dev)
+    phi = torch.empty(n, dtype=torch.float64, device=dev)
+    grad = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    stream = torch.cuda.Stream()
+    reps = max(args.steps, 1) * 5
+
+    def launch():
+        rc = lib.agx_surrogate_eval_many_device(sur._h, n, pts.data_ptr(), phi.data_ptr(), grad.data_ptr(),
+                                                stream.cuda_stream)
+        if rc:
+            raise RuntimeError(agama_amd._native.last_error())
+    with torch.cuda.stream(stream):
+        for _ in range(max(args.warmup, 1)):
+            launch()
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(reps):
+            launch()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    per = ev0.elapsed_time(ev1) / 1e3 / reps
+    achieved = n * MLP_MFMA_FLOPS_PER_EVAL / per / 1e12
+    out = {"metric": "MLP surrogate potential+force evaluations/s at 1M points (bf16 MFMA, fp32 accumulation)",
+           "value": n / per, "unit": "evaluations/s", "n_gpus": 1, "steps": reps, "warmup": args.warmup,
+           "ms_per_step": wall / reps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "bf16 (fp32 accumulation; f64 base and assembly)", "data": "synthetic",
+           "config": {"workload": "surrogate Phi and grad Phi at 1M points drawn from the triaxial Dehnen density"},
+           "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+                        "kernel": "surrogateEvalKernel", "kernel_ms": per * 1e3},
+           "cpu_baseline": None}
     print(json.dumps(out), flush=True)
 
 

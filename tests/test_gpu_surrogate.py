@@ -196,27 +196,28 @@ def test_surrogate_orbits_and_fallback(pot, trained):
 
 
 def test_surrogate_orbits_loose_threshold(pot, trained):
-    """with a loose energy threshold the surrogate integrates every in-range orbit itself.
-    The bf16 rounding in the network makes its force rough on small scales, which the DOP853
-    error control at accuracy 1e-8 answers with much shorter steps: the run is bounded by
-    maxNumSteps, and the step counts and the FP64-judged energy error are reported"""
+    """with a loose energy threshold the surrogate integrates in-range orbits itself until the
+    step size collapses (the bf16 rounding in the network makes its force rough on small
+    scales, which the error control at accuracy 1e-8 answers with ever shorter steps); the
+    orbits that keep the surrogate, their step counts and their FP64-judged energy error are
+    reported, and every result is either a surrogate orbit or the exact FP64 integration"""
     import agama_amd
     from agama_amd import workloads
     from oracle.pyoracle import OraclePotential
     orc = OraclePotential(COEF)
-    ic, T = workloads.triaxial_dehnen_orbits(32, lambda p: orc.eval(p)[0], pot.coefs(), seed=32, periods=1)
+    ic, T = workloads.triaxial_dehnen_orbits(256, lambda p: orc.eval(p)[0], pot.coefs(), seed=32, periods=2)
     info, ref = {}, {}
-    agama_amd.orbit(potential=trained, ic=ic, time=T, accuracy=1e-8, info=info, max_rel_de=1.0,
-                    maxNumSteps=4000)
+    agama_amd.orbit(potential=trained, ic=ic, time=T, accuracy=1e-8, info=info, max_rel_de=1.0)
     agama_amd.orbit(potential=pot, ic=ic, time=T, accuracy=1e-8, info=ref)
     st = info["status"]
     fb = (st & 8) != 0
-    capped = (st & ~8) == 2
-    assert np.all(np.isin(st & ~8, (0, 2)))
-    sel = ~fb & ~capped
-    mde = info["maxde"][~fb]
-    print(f"loose threshold, 1 period: {fb.sum()} fallbacks (radius range), {capped.sum()} reached "
-          f"maxNumSteps; steps surrogate/FP64 on completed orbits "
-          f"{info['nsteps'][sel].sum() / max(ref['nsteps'][sel].sum(), 1):.1f}; surrogate dE/E median "
-          f"{np.median(mde):.2e} max {mde.max():.2e}")
-    assert np.median(mde) < 1e-2
+    assert np.all((st & ~8) == 0)
+    assert np.array_equal(info["final"][fb], ref["final"][fb])
+    ok = ~fb
+    msg = f"loose threshold, 2 periods: {ok.sum()} of {len(ok)} orbits kept the surrogate"
+    if ok.any():
+        mde = info["maxde"][ok]
+        msg += (f"; their steps surrogate/FP64 {info['nsteps'][ok].sum() / ref['nsteps'][ok].sum():.2f}, "
+                f"dE/E median {np.median(mde):.2e} max {mde.max():.2e}")
+        assert np.median(mde) < 1e-2
+    print(msg)
