@@ -71,6 +71,8 @@ class Potential:
               lmax=8, gridSizeR=50, rmin=1e-3, rmax=1e3)
     Potential(type='Multipole', density=Density(...), lmax=..., ...)
     Potential(coefs=(radii, Phi, dPhi))   # arrays of shape [nr], [(lmax+1)^2, nr] x 2
+    Potential(dict(type='Multipole', ...), dict(type='NFW', mass=1, scaleRadius=20))
+                                          # Composite: one Multipole plus an analytic NFW halo
 
     A string density name is expanded from the density profile by the Poisson solver
     (the reference instead expands the analytic potential of a named model).
@@ -78,10 +80,18 @@ class Potential:
 
     def __init__(self, *args, **kwargs):
         self._h = None
+        if args and all(isinstance(a, dict) for a in args) and not kwargs:
+            text = "; ".join(self._text(dict(a)) for a in args)
+            h = lib.agx_potential_create(text.encode())
+            if not h:
+                raise AgamaAmdError("Potential construction failed: " + _native.last_error())
+            self._h = h
+            return
         if len(args) == 1 and isinstance(args[0], str) and not kwargs:
             kwargs = {"file": args[0]}
         elif args:
-            raise TypeError("Potential accepts only named arguments (or a single file name)")
+            raise TypeError("Potential accepts only named arguments (or a single file name, or "
+                            "dictionaries of the components of a composite)")
         coefs = kwargs.pop("coefs", None)
         if coefs is not None:
             radii, Phi, dPhi = (np.ascontiguousarray(a, dtype=np.float64) for a in coefs)
@@ -89,22 +99,25 @@ class Potential:
             lmax = int(round(np.sqrt(nc))) - 1
             h = lib.agx_potential_from_coefs(len(radii), lmax, ptr(radii), ptr(Phi), ptr(dPhi))
         else:
-            items = []
-            for k, v in kwargs.items():
-                kl = k.lower()
-                if kl == "density" and isinstance(v, Density):
-                    items += v._params()
-                elif kl in _KEYS:
-                    items.append((_KEYS[kl], v))
-                else:
-                    raise TypeError(f"unknown argument '{k}'")
-            if not any(k == "type" for k, _ in items):
-                items.insert(0, ("type", "Multipole"))
-            text = ", ".join(f"{k}={_fmt(v)}" for k, v in items)
-            h = lib.agx_potential_create(text.encode())
+            h = lib.agx_potential_create(self._text(kwargs).encode())
         if not h:
             raise AgamaAmdError("Potential construction failed: " + _native.last_error())
         self._h = h
+
+    @staticmethod
+    def _text(kwargs):
+        items = []
+        for k, v in kwargs.items():
+            kl = k.lower()
+            if kl == "density" and isinstance(v, Density):
+                items += v._params()
+            elif kl in _KEYS:
+                items.append((_KEYS[kl], v))
+            else:
+                raise TypeError(f"unknown argument '{k}'")
+        if not any(k == "type" for k, _ in items):
+            items.insert(0, ("type", "Multipole"))
+        return ", ".join(f"{k}={_fmt(v)}" for k, v in items)
 
     def __del__(self):
         if getattr(self, "_h", None) and lib is not None:

@@ -35,6 +35,7 @@ struct agx_potential {
     void* dmem = nullptr;
     unsigned int* dcounter = nullptr;
     int device = 0;
+    double nfwMass = 0, nfwScale = 1;  // optional analytic NFW component
     ~agx_potential() { if(dmem) (void)hipFree(dmem); }
 };
 
@@ -87,6 +88,8 @@ void upload(agx_potential& p)
     D.rsqInner = T.rsqInner; D.rsqOuter = T.rsqOuter;
     for(int s = 0; s < T.ns; s++) D.slotM[s] = T.slotM[s];
     D.zeroSlot = T.zeroSlot;
+    D.nfwMass = p.nfwMass;
+    D.nfwScale = p.nfwScale;
     D.gridR = reinterpret_cast<const double*>(base + oR);
     D.gridT = reinterpret_cast<const double*>(base + oT);
     D.nodes = reinterpret_cast<const double*>(base + oN);
@@ -106,27 +109,54 @@ void upload(agx_potential& p)
     put(oH, &D, sizeof(D));
 }
 
-agx_potential* makeFromCoefs(agx::Coefs&& c, bool allowLowOrder = false)
+agx_potential* makeFromCoefs(agx::Coefs&& c, bool allowLowOrder = false, double nfwMass = 0,
+    double nfwScale = 1)
 {
     std::unique_ptr<agx_potential> p(new agx_potential);
+    p->nfwMass = nfwMass;
+    p->nfwScale = nfwScale;
     p->coefs = std::move(c);
     p->T = agx::buildTables(p->coefs, allowLowOrder);
     upload(*p);
     return p.release();
 }
 
-// "type=Multipole, file=..." or "type=Multipole, density=..., lmax=..., gridSizeR=..., ..."
+// "type=Multipole, file=..." or "type=Multipole, density=..., lmax=..., gridSizeR=..., ...",
+// optionally followed by "; type=NFW, mass=..., scaleRadius=..." (a two-component Composite,
+// src/potential_composite.cpp; components separated by ';')
 agx_potential* createFromParams(const std::string& text)
 {
-    agx::Params P = agx::Params::parse(text);
-    std::string type = P.str("type", "Multipole");
-    std::string tl = type;
-    for(auto& ch : tl) ch = (char)tolower(ch);
-    if(tl != "multipole")
-        throw std::invalid_argument("only type=Multipole is provided by this library (got " + type + ")");
+    std::vector<std::string> parts;
+    size_t b = 0;
+    for(size_t e; (e = text.find(';', b)) != std::string::npos; b = e + 1) parts.push_back(text.substr(b, e - b));
+    parts.push_back(text.substr(b));
+    const agx::Params* mp = nullptr;
+    std::vector<agx::Params> comps;
+    for(const std::string& t : parts) comps.push_back(agx::Params::parse(t));
+    double nfwMass = 0, nfwScale = 1;
+    int nfwCount = 0;
+    for(const agx::Params& P : comps) {
+        std::string tl = P.str("type", "Multipole");
+        for(auto& ch : tl) ch = (char)tolower(ch);
+        if(tl == "multipole") {
+            if(mp) throw std::invalid_argument("a composite may hold only one Multipole component");
+            mp = &P;
+        } else if(tl == "nfw") {
+            P.checkOnly({"type", "mass", "scaleradius"});
+            nfwMass = P.num("mass", 1);
+            nfwScale = P.num("scaleradius", 1);
+            if(!(nfwScale > 0) || !(nfwMass > -INFINITY && nfwMass < INFINITY))
+                throw std::invalid_argument("NFW: invalid mass or scaleRadius");
+            if(++nfwCount > 1) throw std::invalid_argument("a composite may hold only one NFW component");
+        } else
+            throw std::invalid_argument("only type=Multipole (plus an optional type=NFW component) is "
+                "provided by this library (got " + P.str("type") + ")");
+    }
+    if(!mp) throw std::invalid_argument("a Multipole component is required");
+    const agx::Params& P = *mp;
     if(P.has("file")) {
         P.checkOnly({"type", "file"});
-        return makeFromCoefs(agx::readCoefFile(P.str("file")));
+        return makeFromCoefs(agx::readCoefFile(P.str("file")), false, nfwMass, nfwScale);
     }
     if(!P.has("density"))
         throw std::invalid_argument("Multipole needs either file=... or density=...");
@@ -139,7 +169,7 @@ agx_potential* createFromParams(const std::string& text)
     const int nr = (int)P.num("gridsizer", 25);
     const int sym = agx::symmetryFromName(P.str("symmetry"));
     return makeFromCoefs(agx::multipoleFromDensity(*dens, lmax, mmax, sym, nr,
-        P.num("rmin", 0), P.num("rmax", 0)));
+        P.num("rmin", 0), P.num("rmax", 0)), false, nfwMass, nfwScale);
 }
 
 int orbitGeometry(long n, int precision, int& blocks, int& bs)
@@ -525,7 +555,7 @@ agx_surrogate* makeSurrogate(const agx_potential* pot, int baseLmax, float lnSca
     S->outScale = outScale;
     S->rmin = rmin;
     S->rmax = rmax;
-    S->base.reset(makeFromCoefs(agx::truncateCoefs(pot->coefs, baseLmax), true));
+    S->base.reset(makeFromCoefs(agx::truncateCoefs(pot->coefs, baseLmax), true, pot->nfwMass, pot->nfwScale));
     return S.release();
 }
 

@@ -513,6 +513,18 @@ __device__ __forceinline__ void multipoleCyl(const AgxMultipoleDev& P, const F& 
         powerLawEval(P.outer, R, z, phi, pot, gR, gz, gphi);
 }
 
+// NFW potential and radial derivative (src/potential_analytic.cpp:55-75, with its rational
+// approximations of ln(1+x)/x and its derivative at small r/a, which set the float values)
+__device__ __forceinline__ void nfwDeriv(double mass, double a, double r, double& pot, double& der)
+{
+    const double q = r / a;
+    const double lnr = r == INFINITY ? 0. : q > 0.016 ? log(1 + q) / r :
+        (1 + q * (1 + 11. / 60 * q)) / (1 + q * (1.5 + q * (0.6 + q * 0.05))) / a;
+    pot = -mass * lnr;
+    der = mass * (q > 0.013 ? (lnr - 1 / (r + a)) / r :
+        (0.5 + 17. / 96 * q) / (1 + q * (27. / 16 + q * (0.75 + 11. / 160 * q))) / (a * a));
+}
+
 // BasePotentialCyl::evalCar: Car -> Cyl, evaluate, gradient back to Car (+dPhi/dx)
 template<class F>
 __device__ __forceinline__ void multipoleCar(const AgxMultipoleDev& P, const F& fetch,
@@ -533,6 +545,18 @@ __device__ __forceinline__ void multipoleCar(const AgxMultipoleDev& P, const F& 
     gx = gR * dRdx + gphi * dphidx;
     gy = gR * dRdy + gphi * dphidy;
     gz = gzc;
+    if(P.nfwMass != 0) {
+        // the spherical component is evaluated in Cartesian coordinates and added first,
+        // the cylindrical one's converted gradient after it (potential_composite.cpp:134-151)
+        const double r = sqrt(x * x + y * y + z * z);
+        double pn, dn;
+        nfwDeriv(P.nfwMass, P.nfwScale, r, pn, dn);
+        const double xr = r == 0 ? 0. : x / r, yr = r == 0 ? 0. : y / r, zr = r == 0 ? 0. : z / r;
+        pot = pot + pn;
+        gx = xr * dn + gx;
+        gy = yr * dn + gy;
+        gz = zr * dn + gz;
+    }
 }
 
 }  // namespace agxd

@@ -40,4 +40,7 @@ struct AgxMultipoleDev {
     const double* nodes;        // node-major spline data (see above)
     const float* nodesF;        // the same data rounded to float (precision-split form), or null
     AgxPowerLawDev inner, outer;
+    // optional analytic NFW halo added to the expansion (a two-component Composite, as in
+    // src/potential_composite.cpp:51-163): Phi = -M ln(1 + r/a) / r; nfwMass == 0: none
+    double nfwMass, nfwScale;
 };

@@ -111,6 +111,35 @@ def disk_orbits(n, force_fn, seed=42, periods=100.0, Rd=3.0, hz=0.3, sigma_frac=
     return ic, periods * 2 * np.pi * R / vc
 
 
+BAR_OMEGA = 40.0
+BAR_HALO = [dict(type="Multipole", density="Ferrers", mass=0.02, scaleRadius=3,
+                 axisRatioY=1 / 3, axisRatioZ=0.8 / 3, lmax=16, gridSizeR=25, rmin=0.01, rmax=100),
+            dict(type="NFW", mass=1, scaleRadius=20)]
+
+
+def bar_halo_orbits(n, force_fn, seed=42, Omega=BAR_OMEGA, rmin=0.3, rmax=10.0):
+    """initial conditions in the frame rotating with the bar (pattern speed Omega about z):
+    cylindrical radius log-uniform in [rmin, rmax] around the bar, z Gaussian with sigma
+    0.1 R, azimuth uniform; inertial velocity = prograde circular speed v_c(R) (from the
+    midplane radial force along x) plus 10% Gaussian dispersion, converted to the rotating
+    frame (v - Omega z x r).  Integration times log-uniform in [5, 200] bar periods 2 pi/Omega.
+    force_fn(points[n,3]) -> -grad Phi."""
+    gen = rng(seed)
+    R = np.exp(gen.uniform(np.log(rmin), np.log(rmax), n))
+    z = 0.1 * R * gen.normal(size=n)
+    ph = gen.uniform(0, 2 * np.pi, n)
+    mid = np.stack([R, np.zeros(n), np.zeros(n)], 1)
+    vc = np.sqrt(np.maximum(-R * np.asarray(force_fn(mid))[:, 0], 0.0))
+    vR, vz, vphi = gen.normal(size=(3, n)) * 0.1 * vc
+    vphi = vphi + vc
+    c, s = np.cos(ph), np.sin(ph)
+    x, y = R * c, R * s
+    vx, vy = vR * c - vphi * s, vR * s + vphi * c
+    ic = np.stack([x, y, z, vx + Omega * y, vy - Omega * x, vz], 1)
+    T = np.exp(gen.uniform(np.log(5.0), np.log(200.0), n)) * 2 * np.pi / Omega
+    return ic, T
+
+
 def density_points(n, seed=7):
     """points drawn from the triaxial Dehnen density (for potential/force evaluation)"""
     return sample_dehnen_positions(n, rng(seed))

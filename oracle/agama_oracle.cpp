@@ -1155,6 +1155,87 @@ struct OrcMultipole : public orc_pot {
     }
 };
 
+// NFW::evalDeriv (potential_analytic.cpp:55-75) through evalAndConvertSph (coord.cpp:923-956)
+struct OrcNFW : public orc_pot {
+    double mass, scaleRadius;
+    OrcNFW(double m, double a) : mass(m), scaleRadius(a) {}
+    void evalDeriv(double r, double* potential, double* deriv) const
+    {
+        double rrel = r / scaleRadius;
+        double ln_over_r = r==INFINITY ? 0 :
+            rrel > 0.016 ? log(1 + rrel) / r :
+            (1 + rrel * (1 + 11./60 * rrel)) / (1 + rrel * (1.5 + rrel * (0.6 + rrel * 0.05))) / scaleRadius;
+        if(potential)
+            *potential = -mass * ln_over_r;
+        if(deriv)
+            *deriv = mass * (rrel > 0.013 ?
+                (ln_over_r - 1/(r+scaleRadius)) / r :
+                (0.5 + 17./96 * rrel) / (1 + rrel * (27./16 + rrel * (0.75 + 11./160 * rrel))) /
+                pow_2(scaleRadius));
+    }
+    virtual void evalCar(const double p[3], double* phi, double grad[3], double) const
+    {
+        const double r = sqrt(pow_2(p[0]) + pow_2(p[1]) + pow_2(p[2]));
+        if(!grad) { evalDeriv(r, phi, NULL); return; }
+        double der;
+        evalDeriv(r, phi, &der);
+        double xr = p[0]/r, yr = p[1]/r, zr = p[2]/r;
+        if(r==0) xr = yr = zr = 0;
+        grad[0] = xr*der; grad[1] = yr*der; grad[2] = zr*der;
+    }
+};
+
+// Composite::evalCar = evalComponents<Cyl,Sph> (potential_composite.cpp:51-163, 308-313):
+// Multipole components (cylindrical) share one Car->Cyl transformation and their summed
+// gradient is converted after the loop; spherically symmetric ones are evaluated in Car inside
+// the loop.  Potentials are summed in component order starting from 0.
+struct OrcComposite : public orc_pot {
+    std::vector<orc_pot*> comps;
+    ~OrcComposite() { for(orc_pot* c : comps) delete c; }
+    virtual void evalCar(const double p[3], double* phi, double grad[3], double t) const
+    {
+        if(phi) *phi = 0;
+        if(grad) grad[0] = grad[1] = grad[2] = 0;
+        bool haveCyl = false;
+        PosCyl pc;
+        double dRdx = 0, dRdy = 0, dphidx = 0, dphidy = 0;
+        GradCyl gsum;
+        gsum.dR = gsum.dz = gsum.dphi = 0;
+        for(const orc_pot* c : comps) {
+            const OrcMultipole* mp = dynamic_cast<const OrcMultipole*>(c);
+            if(mp) {
+                if(!haveCyl) {   // coord.cpp:94-125 (toPosDeriv<Car,Cyl>)
+                    const double R = sqrt(pow_2(p[0]) + pow_2(p[1]));
+                    if(R==0) {
+                        dRdx = dRdy = dphidx = dphidy = 1.;
+                        pc.R = 0; pc.z = p[2]; pc.phi = 0;
+                    } else {
+                        const double cosphi = p[0]/R, sinphi = p[1]/R;
+                        dRdx = cosphi; dRdy = sinphi; dphidx = -sinphi/R; dphidy = cosphi/R;
+                        pc.R = R; pc.z = p[2]; pc.phi = atan2_r(p[1], p[0]);
+                    }
+                    haveCyl = true;
+                }
+                double pot;
+                GradCyl g;
+                mp->evalCyl(pc, &pot, &g);
+                if(phi) *phi += pot;
+                gsum.dR += g.dR; gsum.dz += g.dz; gsum.dphi += g.dphi;
+            } else {
+                double pot, g[3];
+                c->evalCar(p, &pot, g, t);
+                if(phi) *phi += pot;
+                if(grad) { grad[0] += g[0]; grad[1] += g[1]; grad[2] += g[2]; }
+            }
+        }
+        if(haveCyl && grad) {   // coord.cpp:607-614 (toGrad<Cyl,Car>)
+            grad[0] += gsum.dR*dRdx + gsum.dphi*dphidx;
+            grad[1] += gsum.dR*dRdy + gsum.dphi*dphidy;
+            grad[2] += gsum.dz;
+        }
+    }
+};
+
 std::vector<std::string> splitWS(const std::string& s)
 {
     std::vector<std::string> out;
@@ -1681,6 +1762,22 @@ orc_pot* orc_multipole_from_arrays(int nr, int lmax, const double* radii,
         }
         return new OrcMultipole(r, P, dP);
     } catch(std::exception& e) { g_err = e.what(); return NULL; }
+}
+
+orc_pot* orc_nfw(double mass, double scale_radius)
+{
+    g_err.clear();
+    if(!(scale_radius > 0)) { g_err = "NFW: scale radius must be positive"; return NULL; }
+    return new OrcNFW(mass, scale_radius);
+}
+
+orc_pot* orc_composite(int n, orc_pot** components)
+{
+    g_err.clear();
+    if(n <= 0 || !components) { g_err = "composite: no components"; return NULL; }
+    OrcComposite* c = new OrcComposite;
+    c->comps.assign(components, components + n);
+    return c;
 }
 
 void orc_free(orc_pot* p) { delete p; }

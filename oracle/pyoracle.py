@@ -28,6 +28,8 @@ def _load(name):
         "orc_multipole_from_file": (_vp, [ctypes.c_char_p]),
         "orc_multipole_from_arrays": (_vp, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
         "orc_free": (None, [_vp]),
+        "orc_nfw": (_vp, [ctypes.c_double, ctypes.c_double]),
+        "orc_composite": (_vp, [ctypes.c_int, _vp]),
         "orc_multipole_tables": (ctypes.c_long, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
         "orc_eval_many": (None, [_vp, ctypes.c_long, _vp, ctypes.c_double, _vp, _vp, ctypes.c_int]),
         "orc_orbit_many": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
@@ -74,7 +76,10 @@ def _p(a):
 
 
 class OraclePotential:
-    def __init__(self, coef_file=None, coefs=None, library=None):
+    """the oracle's Multipole (from a coefficient file or arrays), optionally composed with an
+    analytic NFW halo: nfw=(mass, scaleRadius)"""
+
+    def __init__(self, coef_file=None, coefs=None, library=None, nfw=None):
         self.lib = library or lib
         if coef_file is not None:
             self.h = self.lib.orc_multipole_from_file(str(coef_file).encode())
@@ -84,6 +89,14 @@ class OraclePotential:
             self.h = self.lib.orc_multipole_from_arrays(len(radii), lmax, _p(radii), _p(Phi), _p(dPhi))
         if not self.h:
             raise RuntimeError("oracle: " + self.lib.orc_last_error().decode())
+        if nfw is not None:
+            halo = self.lib.orc_nfw(float(nfw[0]), float(nfw[1]))
+            if not halo:
+                self.lib.orc_free(self.h)
+                self.h = None
+                raise RuntimeError("oracle: " + self.lib.orc_last_error().decode())
+            comps = (ctypes.c_void_p * 2)(self.h, halo)
+            self.h = self.lib.orc_composite(2, ctypes.cast(comps, ctypes.c_void_p))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -23,11 +23,18 @@ FIXTURES = {
     "disk_bulge_halo_lmax12.coef": "type=Multipole, density=Disk+Dehnen+NFW, lmax=12, gridSizeR=80, "
                                    "rmin=1e-2, rmax=1e3, 0.mass=0.05, 0.scaleRadius=3, 0.scaleHeight=0.3, "
                                    "1.mass=0.01, 1.scaleRadius=0.5, 1.gamma=1, 2.mass=1, 2.scaleRadius=20",
+    # Ferrers bar a=3, b=1, c=0.8 (axis ratios 1/3, 0.8/3), M=0.02, lmax=16 (even l and m for
+    # its triaxial symmetry), 25 radial nodes in [0.01, 100]; used with an NFW halo in a composite
+    "ferrers_bar_lmax16.coef": "type=Multipole, density=Ferrers, mass=0.02, scaleRadius=3, "
+                               "axisRatioY=0.3333333333333333, axisRatioZ=0.26666666666666666, "
+                               "lmax=16, gridSizeR=25, rmin=0.01, rmax=100",
 }
 
 
-def main():
+def main(only=None):
     for name, params in FIXTURES.items():
+        if only and name not in only:
+            continue
         path = os.path.join(HERE, name)
         rc = lib.agx_build_coef_file(params.encode(), path.encode())
         if rc:
@@ -36,4 +43,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])
