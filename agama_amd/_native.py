@@ -5,6 +5,14 @@ loudly when it is missing - there is no CPU fallback for the product path."""
 import ctypes
 import os
 
+# PyTorch-ROCm bundles its own HIP runtime under the same soname as the system one.  If this
+# library were loaded first, torch would later load its copy as a second runtime in the same
+# process, which then finds no GPU; loading torch first makes both use one runtime.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libagama_amd.so")
 
@@ -64,6 +72,13 @@ def _load():
                                                             _vp, _vp, _vp, _vp]),
         "agx_debug_gemm": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
+        "agx_set_orbit_waves": (ctypes.c_int, [ctypes.c_int]),
+        "agx_get_orbit_config": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+        "agx_orbit_graph_create": (_vp, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                         _vp, ctypes.c_int]),
+        "agx_orbit_graph_launch": (ctypes.c_int, [_vp, _vp]),
+        "agx_orbit_graph_destroy": (None, [_vp]),
         "agama_getError": (ctypes.c_char_p, []),
         "agama_createPotential": (_vp, [ctypes.c_char_p]),
         "agama_deletePotential": (None, [_vp]),

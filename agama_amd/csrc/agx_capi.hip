@@ -20,8 +20,9 @@
 hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
     long n, const double* pos, double* phi, double* grad, hipStream_t stream);
 hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
-    const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream);
-int agxOrbitKernelBlocksPerCU(int precision, int fetch);
+    int waves, const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream);
+int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves);
+hipError_t agxLaunchFillU32(unsigned int* p, long n, unsigned int v, hipStream_t stream);
 int agxOrbitKernelBlockSize();
 size_t agxScheduleWorkspace(long n);
 hipError_t agxLaunchSchedule(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, long n,
@@ -44,6 +45,7 @@ namespace {
 thread_local std::string g_err;
 int g_orbitBlocks = 0;
 int g_fetchEval = AGX_FETCH_VEC2, g_fetchOrbit = AGX_FETCH_VEC2;   // set by agx_set_fetch
+int g_orbitWaves = 1;                                                // set by agx_set_orbit_waves
 
 int fail(int code, const std::string& msg) { g_err = msg; return code; }
 
@@ -180,7 +182,7 @@ int orbitGeometry(long n, int precision, int& blocks, int& bs)
         int dev = 0, cus = 0;
         HIPCHECK(hipGetDevice(&dev));
         HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        const int perCU = std::max(1, agxOrbitKernelBlocksPerCU(precision, g_fetchOrbit));
+        const int perCU = std::max(1, agxOrbitKernelBlocksPerCU(precision, g_fetchOrbit, g_orbitWaves));
         blocks = cus * perCU;
     }
     const long need = (n + bs - 1) / bs;
@@ -364,6 +366,22 @@ int agx_set_fetch(int eval_mode, int orbit_mode)
     return AGX_OK;
 }
 
+int agx_set_orbit_waves(int waves)
+{
+    if(waves != 1 && waves != 2) return fail(AGX_EINVAL, "waves must be 1 or 2");
+    g_orbitWaves = waves;
+    return AGX_OK;
+}
+
+int agx_get_orbit_config(int* num_blocks, int* fetch_eval, int* fetch_orbit, int* waves)
+{
+    if(num_blocks) *num_blocks = g_orbitBlocks;
+    if(fetch_eval) *fetch_eval = g_fetchEval;
+    if(fetch_orbit) *fetch_orbit = g_fetchOrbit;
+    if(waves) *waves = g_orbitWaves;
+    return AGX_OK;
+}
+
 int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
     const double* d_tstart, const double* d_ttotal, const int* d_trajsize,
     const long* d_traj_offset, const int* d_order, double accuracy, double Omega,
@@ -406,13 +424,91 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
         A.outMaxde = d_maxde; A.outStatus = d_status; A.counter = pot->dcounter;
         int blocks, bs;
         orbitGeometry(n, precision, blocks, bs);
-        HIPCHECK(hipMemsetAsync(pot->dcounter, 0, sizeof(unsigned int), st));
-        hipError_t e = agxLaunchOrbit(pot->dev, pot->ddev, precision, g_fetchOrbit, A, blocks, bs, st);
+        HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
+        hipError_t e = agxLaunchOrbit(pot->dev, pot->ddev, precision, g_fetchOrbit, g_orbitWaves, A, blocks, bs, st);
         if(work) HIPCHECK(hipFreeAsync(work, st));
         if(e != hipSuccess) return fail(AGX_EDEVICE, std::string("orbit launch: ") + hipGetErrorString(e));
         return AGX_OK;
     } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
 }
+
+}  // extern "C"
+
+// one integration of a fixed batch captured as a HIP graph (counter fill + persistent kernel)
+struct agx_orbit_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void* work = nullptr;          // work order (n ints) + private queue counter
+    int device = 0;
+    ~agx_orbit_graph()
+    {
+        if(exec) (void)hipGraphExecDestroy(exec);
+        if(graph) (void)hipGraphDestroy(graph);
+        if(work) (void)hipFree(work);
+    }
+};
+
+extern "C" {
+
+agx_orbit_graph* agx_orbit_graph_create(const agx_potential* pot, long n, const double* d_ic,
+    const double* d_tstart, const double* d_ttotal, const int* d_trajsize, const long* d_traj_offset,
+    double accuracy, double Omega, long maxNumSteps, double* d_time, double* d_traj, double* d_final,
+    long* d_nsteps, long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags)
+{
+    g_err.clear();
+    if(!pot) { fail(AGX_EINVAL, "NULL potential"); return nullptr; }
+    if(n <= 0 || n > 0x7fffffffL || !d_ic || !d_ttotal || !d_trajsize) { fail(AGX_EINVAL, "invalid orbit arrays"); return nullptr; }
+    if((d_time || d_traj) && !d_traj_offset) { fail(AGX_EINVAL, "trajectory output needs offsets"); return nullptr; }
+    if(!(accuracy > 0) || !(Omega > -INFINITY && Omega < INFINITY) || maxNumSteps <= 0) {
+        fail(AGX_EINVAL, "invalid accuracy, Omega or maxNumSteps"); return nullptr;
+    }
+    const int precision = flags & AGX_PRECISION_MASK;
+    if((precision != AGX_PRECISION_FP64 && precision != AGX_PRECISION_FP32) ||
+        (flags & ~(AGX_PRECISION_MASK | AGX_ORDER_AS_GIVEN))) { fail(AGX_EINVAL, "invalid flags"); return nullptr; }
+    try {
+        std::unique_ptr<agx_orbit_graph> G(new agx_orbit_graph);
+        HIPCHECK(hipSetDevice(pot->device));
+        G->device = pot->device;
+        const size_t ordBytes = ((size_t)n * 4 + 255) / 256 * 256, schedBytes = agxScheduleWorkspace(n);
+        HIPCHECK(hipMalloc(&G->work, ordBytes + 256 + schedBytes));
+        char* w = static_cast<char*>(G->work);
+        int* ord = reinterpret_cast<int*>(w);
+        unsigned int* counter = reinterpret_cast<unsigned int*>(w + ordBytes);
+        const int* order = nullptr;
+        if(!(flags & AGX_ORDER_AS_GIVEN)) {   // the work order, computed once from the initial conditions
+            HIPCHECK(agxLaunchSchedule(pot->dev, pot->ddev, n, d_ic, d_ttotal, ord, w + ordBytes + 256, nullptr));
+            HIPCHECK(hipDeviceSynchronize());
+            order = ord;
+        }
+        AgxOrbitArgs A;
+        A.n = n; A.ic = d_ic; A.tstart = d_tstart; A.ttotal = d_ttotal; A.trajsize = d_trajsize;
+        A.trajOffset = d_traj_offset; A.order = order; A.accuracy = accuracy; A.Omega = Omega;
+        A.maxNumSteps = maxNumSteps; A.outTime = d_time; A.outTraj = d_traj; A.outFinal = d_final;
+        A.outNsteps = d_nsteps; A.outNattempts = d_nattempts; A.outNevals = d_nevals;
+        A.outMaxde = d_maxde; A.outStatus = d_status; A.counter = counter;
+        int blocks, bs;
+        orbitGeometry(n, precision, blocks, bs);
+        hipStream_t st;
+        HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        std::unique_ptr<void, void(*)(void*)> sg((void*)st, [](void* q) { (void)hipStreamDestroy((hipStream_t)q); });
+        HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = agxLaunchFillU32(counter, 1, 0u, st);
+        if(e == hipSuccess) e = agxLaunchOrbit(pot->dev, pot->ddev, precision, g_fetchOrbit, g_orbitWaves, A, blocks, bs, st);
+        HIPCHECK(hipStreamEndCapture(st, &G->graph));
+        HIPCHECK(e);
+        HIPCHECK(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
+        return G.release();
+    } catch(std::exception& e) { fail(AGX_EDEVICE, e.what()); return nullptr; }
+}
+
+int agx_orbit_graph_launch(agx_orbit_graph* g, void* stream)
+{
+    if(!g) return fail(AGX_EINVAL, "NULL graph");
+    hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
+    return e == hipSuccess ? AGX_OK : fail(AGX_EDEVICE, hipGetErrorString(e));
+}
+
+void agx_orbit_graph_destroy(agx_orbit_graph* g) { delete g; }
 
 int agx_orbit_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
     const double* ttotal, const int* trajsize, double accuracy, double Omega, long maxNumSteps,
@@ -836,8 +932,8 @@ int agx_surrogate_orbit_batch_device(const agx_surrogate* s, long n, const doubl
         HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         const long need = (n + 63) / 64;
         const int blocks = (int)std::min<long>(need, (long)cus * agxOrbitSurrogateBlocksPerCU());
-        HIPCHECK(hipMemsetAsync(pot->dcounter, 0, sizeof(unsigned int), st));
-        HIPCHECK(hipMemsetAsync(nfb, 0, sizeof(unsigned int), st));
+        HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
+        HIPCHECK(agxLaunchFillU32(nfb, 1, 0u, st));
         e = agxLaunchOrbitSurrogate(pot->ddev, s->base->ddev, s->dM, A, SA, blocks, st);
         if(e != hipSuccess) { release(); return fail(AGX_EDEVICE, std::string("surrogate orbit launch: ") + hipGetErrorString(e)); }
         unsigned int nf = 0;
@@ -848,8 +944,8 @@ int agx_surrogate_orbit_batch_device(const agx_surrogate* s, long n, const doubl
             A.order = list;
             int fb, bs;
             orbitGeometry(nf, AGX_FP64, fb, bs);
-            HIPCHECK(hipMemsetAsync(pot->dcounter, 0, sizeof(unsigned int), st));
-            e = agxLaunchOrbit(pot->dev, pot->ddev, AGX_FP64, g_fetchOrbit, A, fb, bs, st);
+            HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
+            e = agxLaunchOrbit(pot->dev, pot->ddev, AGX_FP64, g_fetchOrbit, g_orbitWaves, A, fb, bs, st);
             if(e == hipSuccess) e = agxLaunchMarkFallback(list, nf, d_status, st);
             if(e != hipSuccess) { release(); return fail(AGX_EDEVICE, std::string("fallback launch: ") + hipGetErrorString(e)); }
         }

@@ -63,10 +63,6 @@ __global__ __launch_bounds__(256) void evalKernel(const AgxMultipoleDev* __restr
 
 // Threads per workgroup of the orbit kernel (fixed: it sizes the LDS block below).
 constexpr int OBS = 256;
-// Wavefronts per SIMD the orbit kernel is compiled for (register budget 512/AGX_ORBIT_WAVES).
-#ifndef AGX_ORBIT_WAVES
-#define AGX_ORBIT_WAVES 1
-#endif
 
 // one right-hand-side potential gradient in the inertial frame (orbit.cpp:182-208)
 template<int MODE>
@@ -90,8 +86,10 @@ __device__ __forceinline__ void rhsEval(const AgxMultipoleDev& P, vd2* stage, do
     }
 }
 
-template<int MODE>
-__global__ __launch_bounds__(OBS) __attribute__((amdgpu_waves_per_eu(AGX_ORBIT_WAVES, AGX_ORBIT_WAVES)))
+// WAVES: wavefronts per SIMD the kernel is compiled for (register budget 512 / WAVES); both
+// variants exist so that the autotuner can choose at run time.
+template<int MODE, int WAVES>
+__global__ __launch_bounds__(OBS) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
 void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
 {
     __shared__ double lds[F_COUNT * OBS];
@@ -146,35 +144,44 @@ hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, in
 
 int agxOrbitKernelBlockSize() { return OBS; }
 
-int agxOrbitKernelBlocksPerCU(int precision, int fetch)
+namespace {
+typedef void (*OrbitKernelFn)(const AgxMultipoleDev*, AgxOrbitArgs);
+OrbitKernelFn orbitKernelFor(int precision, int fetch, int waves)
+{
+    const bool two = waves == 2;
+    if(precision == AGX_FP32_TABLES) return two ? orbitKernel<-1, 2> : orbitKernel<-1, 1>;
+    if(fetch == AGX_FETCH_COOP) return two ? orbitKernel<AGX_FETCH_COOP, 2> : orbitKernel<AGX_FETCH_COOP, 1>;
+    if(fetch == AGX_FETCH_VEC2) return two ? orbitKernel<AGX_FETCH_VEC2, 2> : orbitKernel<AGX_FETCH_VEC2, 1>;
+    return two ? orbitKernel<AGX_FETCH_SCALAR, 2> : orbitKernel<AGX_FETCH_SCALAR, 1>;
+}
+
+__global__ void fillU32Kernel(unsigned int* p, long n, unsigned int v)
+{
+    for(long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+}  // namespace
+
+int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves)
 {
     int nb = 0;
-    hipError_t e;
-    if(precision == AGX_FP32_TABLES)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernel<-1>, OBS, 0);
-    else if(fetch == AGX_FETCH_COOP)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernel<AGX_FETCH_COOP>, OBS, 0);
-    else if(fetch == AGX_FETCH_VEC2)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernel<AGX_FETCH_VEC2>, OBS, 0);
-    else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernel<AGX_FETCH_SCALAR>, OBS, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernelFor(precision, fetch, waves), OBS, 0);
     return e == hipSuccess ? nb : 1;
 }
 
 hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
-    const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream)
+    int waves, const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream)
 {
     (void)P;
     if(A.n <= 0) return hipSuccess;
     if(blockSize != OBS) return hipErrorInvalidValue;
-    const dim3 g(numBlocks), b(OBS);
-    if(precision == AGX_FP32_TABLES)
-        hipLaunchKernelGGL(orbitKernel<-1>, g, b, 0, stream, dP, A);
-    else if(fetch == AGX_FETCH_COOP)
-        hipLaunchKernelGGL(orbitKernel<AGX_FETCH_COOP>, g, b, 0, stream, dP, A);
-    else if(fetch == AGX_FETCH_VEC2)
-        hipLaunchKernelGGL(orbitKernel<AGX_FETCH_VEC2>, g, b, 0, stream, dP, A);
-    else
-        hipLaunchKernelGGL(orbitKernel<AGX_FETCH_SCALAR>, g, b, 0, stream, dP, A);
+    hipLaunchKernelGGL(orbitKernelFor(precision, fetch, waves), dim3(numBlocks), dim3(OBS), 0, stream, dP, A);
+    return hipGetLastError();
+}
+
+// a fill kernel instead of hipMemsetAsync: graph-capturable and safe to replay
+hipError_t agxLaunchFillU32(unsigned int* p, long n, unsigned int v, hipStream_t stream)
+{
+    if(n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fillU32Kernel, dim3(1), dim3(64), 0, stream, p, n, v);
     return hipGetLastError();
 }

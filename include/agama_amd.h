@@ -157,6 +157,30 @@ double agama_evalPotential(const agama_Potential* pot, const double pos[3], doub
     double deriv[3], double deriv2[6]);
 
 
+
+/* Orbit-kernel launch configuration chosen by the autotuner (agama_amd/tune.py): wavefronts
+   per SIMD the persistent kernel is compiled for (1: 512 registers per lane, 2: 256), and the
+   current settings of agx_set_orbit_launch / agx_set_fetch / agx_set_orbit_waves. */
+int agx_set_orbit_waves(int waves);
+int agx_get_orbit_config(int* num_blocks, int* fetch_eval, int* fetch_orbit, int* waves);
+
+/* One integration of a fixed batch of device arrays (arguments as agx_orbit_batch_device)
+   captured as a HIP graph - a queue-counter fill kernel and the persistent kernel - for
+   callers that integrate the same batch layout repeatedly (e.g. after updating d_ic in
+   place).  The work order is computed once, at creation, from the initial conditions then
+   present; the graph owns its queue counter, so graphs of one potential may run
+   concurrently.  No counterpart in the reference (its orbit() is host-synchronous). */
+typedef struct agx_orbit_graph agx_orbit_graph;
+agx_orbit_graph* agx_orbit_graph_create(const agx_potential* pot, long n, const double* d_ic,
+                                        const double* d_tstart, const double* d_ttotal,
+                                        const int* d_trajsize, const long* d_traj_offset,
+                                        double accuracy, double Omega, long maxNumSteps,
+                                        double* d_time, double* d_traj, double* d_final, long* d_nsteps,
+                                        long* d_nattempts, long* d_nevals, double* d_maxde,
+                                        int* d_status, int flags);
+int agx_orbit_graph_launch(agx_orbit_graph* g, void* stream);
+void agx_orbit_graph_destroy(agx_orbit_graph* g);
+
 /* ---- MLP potential surrogate (no counterpart in the reference) ----
    Phi_NN(x) = Phi_base(x) (1 + out_scale * MLP(|x|/r, |y|/r, |z|/r, ln r / ln_scale)), with
    Phi_base the potential's expansion truncated to l <= base_lmax (FP64) and MLP a 4-256-256-
