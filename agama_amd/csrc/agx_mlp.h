@@ -88,64 +88,69 @@ struct WTiles {
     }
 };
 
-// one hidden layer in the MFMA result layout: out = bf16(silu(W in + b)), silu' -> LDS slot
-__device__ __forceinline__ void mlpHidden(const WTiles& Aw, agx_gbl_float* __restrict__ bias,
-    const agx_bf16x8 (&in)[16], agx_bf16x8 (&out)[16], agx_lds_bf16x8* sslot, int lane, int g)
+// One 256x256 product for the wavefront's 32 points: for every m-tile mt, acc = init(mt) +
+// W(mt rows) x in, then epi(mt, acc).  MODE 0 issues one m-tile's 16 weight tiles and waits for
+// them (fewest registers; used inside the orbit kernel).  MODE 1 works on two m-tiles at a time
+// (two independent accumulation chains) and loads the next pair's tiles while the current pair
+// is multiplied (about 256 more registers; used by the evaluation kernel).
+template<int MODE, class Init, class Epi>
+__device__ __forceinline__ void mlpProduct(const WTiles& Aw, const agx_bf16x8 (&in)[16], int lane,
+    Init init, Epi epi)
 {
+    if constexpr(MODE == 0) {
 #pragma unroll
-    for(int mt = 0; mt < 8; mt++) {
-        agx_f32x16 acc;
+        for(int mt = 0; mt < 8; mt++) {
+            agx_f32x16 acc = init(mt);
 #pragma unroll
-        for(int j = 0; j < 16; j++) acc[j] = bias[(mt * 2 + g) * 16 + j];
+            for(int kb = 0; kb < 16; kb++)
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aw.tile(mt, kb, lane), in[kb], acc, 0, 0, 0);
+            epi(mt, acc);
+            // keep the next m-tile's weight loads from being hoisted here (register pressure)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else {
+        agx_bf16x8 P0[16], P1[16], Q0[16], Q1[16];
 #pragma unroll
-        for(int kb = 0; kb < 16; kb++)
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aw.tile(mt, kb, lane), in[kb], acc, 0, 0, 0);
+        for(int kb = 0; kb < 16; kb++) { P0[kb] = Aw.tile(0, kb, lane); P1[kb] = Aw.tile(1, kb, lane); }
 #pragma unroll
-        for(int q = 0; q < 2; q++) {
-            agx_bf16x8 hv, sv;
+        for(int mp = 0; mp < 8; mp += 4) {
 #pragma unroll
-            for(int p = 0; p < 8; p++) {
-                float h, s;
-                siluBoth(acc[8 * q + p], h, s);
-                hv[p] = (__bf16)h;
-                sv[p] = (__bf16)s;
+            for(int kb = 0; kb < 16; kb++) { Q0[kb] = Aw.tile(mp + 2, kb, lane); Q1[kb] = Aw.tile(mp + 3, kb, lane); }
+            {
+                agx_f32x16 a0 = init(mp), a1 = init(mp + 1);
+#pragma unroll
+                for(int kb = 0; kb < 16; kb++) {
+                    a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P0[kb], in[kb], a0, 0, 0, 0);
+                    a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P1[kb], in[kb], a1, 0, 0, 0);
+                }
+                epi(mp, a0);
+                epi(mp + 1, a1);
             }
-            out[2 * mt + q] = hv;
-            sslot[(mt * 2 + q) * 64 + lane] = sv;
+            __builtin_amdgcn_sched_barrier(0);
+            if(mp + 4 < 8) {
+#pragma unroll
+                for(int kb = 0; kb < 16; kb++) { P0[kb] = Aw.tile(mp + 4, kb, lane); P1[kb] = Aw.tile(mp + 5, kb, lane); }
+            }
+            {
+                agx_f32x16 a0 = init(mp + 2), a1 = init(mp + 3);
+#pragma unroll
+                for(int kb = 0; kb < 16; kb++) {
+                    a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Q0[kb], in[kb], a0, 0, 0, 0);
+                    a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Q1[kb], in[kb], a1, 0, 0, 0);
+                }
+                epi(mp + 2, a0);
+                epi(mp + 3, a1);
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
-        // keep the next m-tile's weight loads from being hoisted here (register pressure)
-        __builtin_amdgcn_sched_barrier(0);
-    }
-}
-
-// one backward layer: out = bf16((W^T in) * silu'), silu' from the LDS slot
-__device__ __forceinline__ void mlpBack(const WTiles& Aw, const agx_bf16x8 (&in)[16],
-    agx_bf16x8 (&out)[16], const agx_lds_bf16x8* sslot, int lane)
-{
-#pragma unroll
-    for(int mt = 0; mt < 8; mt++) {
-        agx_f32x16 acc;
-#pragma unroll
-        for(int j = 0; j < 16; j++) acc[j] = 0.f;
-#pragma unroll
-        for(int kb = 0; kb < 16; kb++)
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aw.tile(mt, kb, lane), in[kb], acc, 0, 0, 0);
-#pragma unroll
-        for(int q = 0; q < 2; q++) {
-            const agx_bf16x8 sv = sslot[(mt * 2 + q) * 64 + lane];
-            agx_bf16x8 gv;
-#pragma unroll
-            for(int p = 0; p < 8; p++) gv[p] = (__bf16)(acc[8 * q + p] * (float)sv[p]);
-            out[2 * mt + q] = gv;
-        }
-        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
 // Fused forward + input-gradient backward for the 32 points of this wavefront: lane l works
 // on point l & 31 with features feat[4] (both halves of the wavefront pass the same values).
 // Returns the network output and d out / d feat in every lane (both halves agree).
-// sbuf: AGX_MLP_LDS_VEC entries of LDS private to this wavefront.
+// sbuf: AGX_MLP_LDS_VEC entries of LDS private to this wavefront.  MODE: see mlpProduct.
+template<int MODE = 0>
 __device__ __forceinline__ void mlpWave(const AgxMlpDev& M, const float feat[4],
     agx_lds_bf16x8* sbuf, float& out, float dfeat[4])
 {
@@ -174,65 +179,98 @@ __device__ __forceinline__ void mlpWave(const AgxMlpDev& M, const float feat[4],
             sbuf[(mt * 2 + q) * 64 + lane] = sv;
             __builtin_amdgcn_sched_barrier(0);
         }
-    // ---- layers 2 and 3
-    mlpHidden(WTiles(M.fwdA[0]), gF(M.bp[0]), ha, hb, sbuf + 1 * 1024, lane, g);
-    mlpHidden(WTiles(M.fwdA[1]), gF(M.bp[1]), hb, ha, sbuf + 2 * 1024, lane, g);
-    const WTiles A4(M.fwdA[2]);
-    agx_gbl_float* b4p = gF(M.bp[2]);
+    // ---- hidden layers 2 and 3: bias init, SiLU epilogue, derivatives to LDS
+    auto hidden = [&](int li, const agx_bf16x8 (&in)[16], agx_bf16x8 (&outv)[16]) {
+        agx_gbl_float* bias = gF(M.bp[li]);
+        agx_lds_bf16x8* sslot = sbuf + (li + 1) * 1024;
+        mlpProduct<MODE>(WTiles(M.fwdA[li]), in, lane,
+            [&](int mt) {
+                agx_f32x16 acc;
+#pragma unroll
+                for(int j = 0; j < 16; j++) acc[j] = bias[(mt * 2 + g) * 16 + j];
+                return acc;
+            },
+            [&](int mt, const agx_f32x16& acc) {
+#pragma unroll
+                for(int q = 0; q < 2; q++) {
+                    agx_bf16x8 hv, sv;
+#pragma unroll
+                    for(int p = 0; p < 8; p++) {
+                        float h, s;
+                        siluBoth(acc[8 * q + p], h, s);
+                        hv[p] = (__bf16)h;
+                        sv[p] = (__bf16)s;
+                    }
+                    outv[2 * mt + q] = hv;
+                    sslot[(mt * 2 + q) * 64 + lane] = sv;
+                }
+            });
+    };
+    hidden(0, ha, hb);
+    hidden(1, hb, ha);
     // ---- layer 4 and the output layer; the backward starts here: g4 = w5 * silu'(z4)
     float op = 0.f;
+    {
+        agx_gbl_float* b4p = gF(M.bp[2]);
+        mlpProduct<MODE>(WTiles(M.fwdA[2]), ha, lane,
+            [&](int mt) {
+                agx_f32x16 acc;
 #pragma unroll
-    for(int mt = 0; mt < 8; mt++) {
-        agx_f32x16 acc;
+                for(int j = 0; j < 16; j++) acc[j] = b4p[(mt * 2 + g) * 16 + j];
+                return acc;
+            },
+            [&](int mt, const agx_f32x16& acc) {
 #pragma unroll
-        for(int j = 0; j < 16; j++) acc[j] = b4p[(mt * 2 + g) * 16 + j];
+                for(int q = 0; q < 2; q++) {
+                    agx_bf16x8 gv;
 #pragma unroll
-        for(int kb = 0; kb < 16; kb++)
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A4.tile(mt, kb, lane), ha[kb], acc, 0, 0, 0);
-#pragma unroll
-        for(int q = 0; q < 2; q++) {
-            agx_bf16x8 gv;
-#pragma unroll
-            for(int p = 0; p < 8; p++) {
-                const int idx = (mt * 2 + g) * 16 + 8 * q + p;
-                float h, s;
-                siluBoth(acc[8 * q + p], h, s);
-                const float w5 = w5p[idx];
-                op += w5 * (float)(__bf16)h;
-                gv[p] = (__bf16)(w5 * (float)(__bf16)s);
-            }
-            hb[2 * mt + q] = gv;
-        }
-        __builtin_amdgcn_sched_barrier(0);
+                    for(int p = 0; p < 8; p++) {
+                        const int idx = (mt * 2 + g) * 16 + 8 * q + p;
+                        float h, s;
+                        siluBoth(acc[8 * q + p], h, s);
+                        const float w5 = w5p[idx];
+                        op += w5 * (float)(__bf16)h;
+                        gv[p] = (__bf16)(w5 * (float)(__bf16)s);
+                    }
+                    hb[2 * mt + q] = gv;
+                }
+            });
     }
     // ---- backward through layers 4 and 3 (bf16 gradients), then 2 (float, into W1^T)
-    mlpBack(WTiles(M.bwdA[2]), hb, ha, sbuf + 2 * 1024, lane);
-    mlpBack(WTiles(M.bwdA[1]), ha, hb, sbuf + 1 * 1024, lane);
-    const WTiles B2(M.bwdA[0]);
+    auto zero = [](int) { agx_f32x16 acc; for(int j = 0; j < 16; j++) acc[j] = 0.f; return acc; };
+    auto back = [&](int li, const agx_bf16x8 (&in)[16], agx_bf16x8 (&outv)[16]) {
+        const agx_lds_bf16x8* sslot = sbuf + li * 1024;
+        mlpProduct<MODE>(WTiles(M.bwdA[li]), in, lane, zero,
+            [&](int mt, const agx_f32x16& acc) {
+#pragma unroll
+                for(int q = 0; q < 2; q++) {
+                    const agx_bf16x8 sv = sslot[(mt * 2 + q) * 64 + lane];
+                    agx_bf16x8 gv;
+#pragma unroll
+                    for(int p = 0; p < 8; p++) gv[p] = (__bf16)(acc[8 * q + p] * (float)sv[p]);
+                    outv[2 * mt + q] = gv;
+                }
+            });
+    };
+    back(2, hb, ha);
+    back(1, ha, hb);
     float d0 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f;
+    mlpProduct<MODE>(WTiles(M.bwdA[0]), hb, lane, zero,
+        [&](int mt, const agx_f32x16& acc) {
 #pragma unroll
-    for(int mt = 0; mt < 8; mt++) {
-        agx_f32x16 acc;
+            for(int q = 0; q < 2; q++) {
+                const agx_bf16x8 sv = sbuf[(mt * 2 + q) * 64 + lane];
 #pragma unroll
-        for(int j = 0; j < 16; j++) acc[j] = 0.f;
-#pragma unroll
-        for(int kb = 0; kb < 16; kb++)
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(B2.tile(mt, kb, lane), hb[kb], acc, 0, 0, 0);
-#pragma unroll
-        for(int q = 0; q < 2; q++) {
-            const agx_bf16x8 sv = sbuf[(mt * 2 + q) * 64 + lane];
-#pragma unroll
-            for(int p = 0; p < 8; p++) {
-                const int idx = (mt * 2 + g) * 16 + 8 * q + p;
-                const float gval = acc[8 * q + p] * (float)sv[p];
-                d0 += w1p[idx * 4] * gval;
-                d1 += w1p[idx * 4 + 1] * gval;
-                d2 += w1p[idx * 4 + 2] * gval;
-                d3 += w1p[idx * 4 + 3] * gval;
+                for(int p = 0; p < 8; p++) {
+                    const int idx = (mt * 2 + g) * 16 + 8 * q + p;
+                    const float gval = acc[8 * q + p] * (float)sv[p];
+                    d0 += w1p[idx * 4] * gval;
+                    d1 += w1p[idx * 4 + 1] * gval;
+                    d2 += w1p[idx * 4 + 2] * gval;
+                    d3 += w1p[idx * 4 + 3] * gval;
+                }
             }
-        }
-        __builtin_amdgcn_sched_barrier(0);
-    }
+        });
     // the two halves of the wavefront hold complementary neurons of the same point
     out = op + __shfl_xor(op, 32) + M.b5;
     dfeat[0] = d0 + __shfl_xor(d0, 32);

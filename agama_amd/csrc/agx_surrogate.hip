@@ -21,6 +21,10 @@ namespace {
 
 typedef __bf16 bf16;
 
+#ifndef AGX_MLP_EVAL_MODE
+#define AGX_MLP_EVAL_MODE 1   // paired m-tiles with weight prefetch (agx_mlp.h mlpProduct)
+#endif
+
 // ---------------- fused evaluation ----------------
 // One wavefront per block; 32 points per pass; both halves of the wavefront evaluate the
 // FP64 baseline of their point and the half with g == 0 writes the result.
@@ -39,7 +43,7 @@ __global__ __launch_bounds__(64) void surrogateEvalKernel(const AgxMlpDev* __res
         const double x = pos[i * 3], y = pos[i * 3 + 1], z = pos[i * 3 + 2];
         float f[4], out, df[4];
         mlpFeatures(M, x, y, z, f);
-        mlpWave(M, f, (agx_lds_bf16x8*)sbuf, out, df);
+        mlpWave<AGX_MLP_EVAL_MODE>(M, f, (agx_lds_bf16x8*)sbuf, out, df);
         double pb, gbx, gby, gbz, ox, oy, oz;
         FetchVec2 fb{ gmem(reinterpret_cast<const vd2*>(B.nodes)) };
         multipoleCar(B, fb, x, y, z, pb, gbx, gby, gbz);
@@ -69,7 +73,7 @@ __global__ __launch_bounds__(64) void mlpFeatureKernel(const AgxMlpDev* __restri
         if(!valid) i = tile * 32;
         float f[4], o, df[4];
         for(int c = 0; c < 4; c++) f[c] = feat[i * 4 + c];
-        mlpWave(M, f, (agx_lds_bf16x8*)sbuf, o, df);
+        mlpWave<AGX_MLP_EVAL_MODE>(M, f, (agx_lds_bf16x8*)sbuf, o, df);
         if(valid && lane < 32) {
             if(out) out[i] = o;
             if(dfeat) for(int c = 0; c < 4; c++) dfeat[i * 4 + c] = df[c];
