@@ -21,7 +21,7 @@ hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, in
     long n, const double* pos, double* phi, double* grad, hipStream_t stream);
 hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
     int waves, const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream);
-int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves);
+int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves, bool nfw);
 hipError_t agxLaunchFillU32(unsigned int* p, long n, unsigned int v, hipStream_t stream);
 int agxOrbitKernelBlockSize();
 size_t agxScheduleWorkspace(long n);
@@ -174,7 +174,7 @@ agx_potential* createFromParams(const std::string& text)
         P.num("rmin", 0), P.num("rmax", 0)), false, nfwMass, nfwScale);
 }
 
-int orbitGeometry(long n, int precision, int& blocks, int& bs)
+int orbitGeometry(long n, int precision, bool nfw, int& blocks, int& bs)
 {
     bs = agxOrbitKernelBlockSize();
     if(g_orbitBlocks > 0) blocks = g_orbitBlocks;
@@ -182,7 +182,7 @@ int orbitGeometry(long n, int precision, int& blocks, int& bs)
         int dev = 0, cus = 0;
         HIPCHECK(hipGetDevice(&dev));
         HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        const int perCU = std::max(1, agxOrbitKernelBlocksPerCU(precision, g_fetchOrbit, g_orbitWaves));
+        const int perCU = std::max(1, agxOrbitKernelBlocksPerCU(precision, g_fetchOrbit, g_orbitWaves, nfw));
         blocks = cus * perCU;
     }
     const long need = (n + bs - 1) / bs;
@@ -423,7 +423,7 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
         A.outNsteps = d_nsteps; A.outNattempts = d_nattempts; A.outNevals = d_nevals;
         A.outMaxde = d_maxde; A.outStatus = d_status; A.counter = pot->dcounter;
         int blocks, bs;
-        orbitGeometry(n, precision, blocks, bs);
+        orbitGeometry(n, precision, pot->nfwMass != 0, blocks, bs);
         HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
         hipError_t e = agxLaunchOrbit(pot->dev, pot->ddev, precision, g_fetchOrbit, g_orbitWaves, A, blocks, bs, st);
         if(work) HIPCHECK(hipFreeAsync(work, st));
@@ -487,7 +487,7 @@ agx_orbit_graph* agx_orbit_graph_create(const agx_potential* pot, long n, const 
         A.outNsteps = d_nsteps; A.outNattempts = d_nattempts; A.outNevals = d_nevals;
         A.outMaxde = d_maxde; A.outStatus = d_status; A.counter = counter;
         int blocks, bs;
-        orbitGeometry(n, precision, blocks, bs);
+        orbitGeometry(n, precision, pot->nfwMass != 0, blocks, bs);
         hipStream_t st;
         HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         std::unique_ptr<void, void(*)(void*)> sg((void*)st, [](void* q) { (void)hipStreamDestroy((hipStream_t)q); });
@@ -943,7 +943,7 @@ int agx_surrogate_orbit_batch_device(const agx_surrogate* s, long n, const doubl
             A.n = nf;
             A.order = list;
             int fb, bs;
-            orbitGeometry(nf, AGX_FP64, fb, bs);
+            orbitGeometry(nf, AGX_FP64, pot->nfwMass != 0, fb, bs);
             HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
             e = agxLaunchOrbit(pot->dev, pot->ddev, AGX_FP64, g_fetchOrbit, g_orbitWaves, A, fb, bs, st);
             if(e == hipSuccess) e = agxLaunchMarkFallback(list, nf, d_status, st);

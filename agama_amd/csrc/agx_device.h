@@ -78,9 +78,10 @@ __device__ __forceinline__ double atan2A(double y, double x)
     return res;
 }
 
-__device__ __forceinline__ int binSearchD(double x, const double* __restrict__ arrp, int size)
+// arr: a global-memory pointer, or a generic one (e.g. to an LDS copy of the grid)
+template<class GP>
+__device__ __forceinline__ int binSearchD(double x, GP arr, int size)
 {
-    const double AGX_GLOBAL* arr = gmem(arrp);
     if(size < 1 || !(x >= arr[0])) return -1;
     if(x > arr[size - 1] || size < 2) return size - 1;
     int index = (int)((x - arr[0]) / (arr[size - 1] - arr[0]) * (size - 1));
@@ -307,17 +308,18 @@ struct TrigRecT {
 // Control flow is uniform across lanes (all harmonics are always evaluated; points outside
 // the grid read cell (0,0) and return NaN, as the reference's spline does), which the
 // cooperative fetcher requires.
-template<class F>
+template<class F, class GP>
 __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const F& fetch,
-    double R, double z, double phi, double& pot, double& gR, double& gz, double& gphi)
+    double R, double z, double phi, double& pot, double& gR, double& gz, double& gphi,
+    GP gridR, GP gridT)
 {
     const double r = sqrt(R * R + z * z), logr = log(r), rplusRinv = 1. / (r + fabs(R)),
         tau = R == 0 ? (z > 0 ? 1. : z < 0 ? -1. : 0.) : z * rplusRinv;
-    const int xi = binSearchD(logr, P.gridR, P.nr), yi = binSearchD(tau, P.gridT, P.nt);
+    const int xi = binSearchD(logr, gridR, P.nr), yi = binSearchD(tau, gridT, P.nt);
     const bool inGrid = xi >= 0 && xi < P.nr - 1 && yi >= 0 && yi < P.nt - 1;
     const int xc = inGrid ? xi : 0, yc = inGrid ? yi : 0;
     typedef typename F::real Rt;
-    const double xlow = P.gridR[xc], xupp = P.gridR[xc + 1], ylow = P.gridT[yc], yupp = P.gridT[yc + 1];
+    const double xlow = gridR[xc], xupp = gridR[xc + 1], ylow = gridT[yc], yupp = gridT[yc + 1];
     const int offsetCorner = logr == xupp ? (tau == yupp ? 3 : 2) : (tau == yupp ? 1 : 0);
     QuinticWT<Rt> wy, wx;
     wy.init(tau, ylow, yupp);
@@ -501,12 +503,16 @@ __device__ __forceinline__ void powerLawEval(const AgxPowerLawDev& L, double R, 
 // Multipole::evalCyl dispatch (potential_multipole.cpp:1538-1548).  The interior spline is
 // evaluated by every lane (uniform control flow for the cooperative fetcher); lanes in the
 // power-law regions then overwrite the result.
-template<class F>
+// gridR/gridT: the ln r and tau grids, as global-memory pointers or generic ones (an LDS
+// copy); null: the header's arrays
+template<class F, class GP = const double AGX_GLOBAL*>
 __device__ __forceinline__ void multipoleCyl(const AgxMultipoleDev& P, const F& fetch,
-    double R, double z, double phi, double& pot, double& gR, double& gz, double& gphi)
+    double R, double z, double phi, double& pot, double& gR, double& gz, double& gphi,
+    GP gridR = nullptr, GP gridT = nullptr)
 {
     const double rsq = R * R + z * z;
-    interiorEval(P, fetch, R, z, phi, pot, gR, gz, gphi);
+    interiorEval(P, fetch, R, z, phi, pot, gR, gz, gphi, gridR ? gridR : GP(gmem(P.gridR)),
+        gridT ? gridT : GP(gmem(P.gridT)));
     if(rsq < P.rsqInner)
         powerLawEval(P.inner, R, z, phi, pot, gR, gz, gphi);
     else if(rsq > P.rsqOuter)
@@ -515,6 +521,21 @@ __device__ __forceinline__ void multipoleCyl(const AgxMultipoleDev& P, const F& 
 
 // NFW potential and radial derivative (src/potential_analytic.cpp:55-75, with its rational
 // approximations of ln(1+x)/x and its derivative at small r/a, which set the float values)
+// Grids in LDS: a block copies the ln r and tau grids (at most AGX_LDS_GRID doubles together)
+// so that the binary searches of every evaluation read LDS instead of making dependent L2
+// round trips.  Returns the pointers to use (the header's global arrays if they do not fit).
+constexpr int AGX_LDS_GRID = 256;
+__device__ __forceinline__ void stageGrids(const AgxMultipoleDev& P, double* lds, const double*& gR,
+    const double*& gT)
+{
+    const int nr = P.nr, nt = P.nt;
+    if(nr + nt > AGX_LDS_GRID) { gR = P.gridR; gT = P.gridT; return; }
+    for(int i = threadIdx.x; i < nr + nt; i += blockDim.x) lds[i] = i < nr ? P.gridR[i] : P.gridT[i - nr];
+    __syncthreads();
+    gR = lds;
+    gT = lds + nr;
+}
+
 __device__ __forceinline__ void nfwDeriv(double mass, double a, double r, double& pot, double& der)
 {
     const double q = r / a;
@@ -526,9 +547,12 @@ __device__ __forceinline__ void nfwDeriv(double mass, double a, double r, double
 }
 
 // BasePotentialCyl::evalCar: Car -> Cyl, evaluate, gradient back to Car (+dPhi/dx)
-template<class F>
+// NFW: whether the optional analytic halo is handled (kernels for potentials without one are
+// instantiated with false: keeping x, y, z live to the end costs a wavefront of occupancy)
+template<class F, class GP = const double AGX_GLOBAL*, bool NFW = true>
 __device__ __forceinline__ void multipoleCar(const AgxMultipoleDev& P, const F& fetch,
-    double x, double y, double z, double& pot, double& gx, double& gy, double& gz)
+    double x, double y, double z, double& pot, double& gx, double& gy, double& gz,
+    GP gridR = nullptr, GP gridT = nullptr)
 {
     const double R = sqrt(x * x + y * y);
     double dRdx, dRdy, dphidx, dphidy, phi;
@@ -541,11 +565,11 @@ __device__ __forceinline__ void multipoleCar(const AgxMultipoleDev& P, const F& 
         phi = atan2A(y, x);
     }
     double gR, gzc, gphi;
-    multipoleCyl(P, fetch, R, z, phi, pot, gR, gzc, gphi);
+    multipoleCyl(P, fetch, R, z, phi, pot, gR, gzc, gphi, gridR, gridT);
     gx = gR * dRdx + gphi * dphidx;
     gy = gR * dRdy + gphi * dphidy;
     gz = gzc;
-    if(P.nfwMass != 0) {
+    if(NFW && P.nfwMass != 0) {
         // the spherical component is evaluated in Cartesian coordinates and added first,
         // the cylindrical one's converted gradient after it (potential_composite.cpp:134-151)
         const double r = sqrt(x * x + y * y + z * z);

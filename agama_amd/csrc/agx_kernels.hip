@@ -25,17 +25,24 @@ using namespace agxd;
 namespace {
 
 // ---------------- batched potential / gradient ----------------
+// minimum wavefronts per SIMD the evaluation kernel is compiled for
+#ifndef AGX_EVAL_WAVES
+#define AGX_EVAL_WAVES 2
+#endif
 // The table header lives in device memory (uniform pointer -> scalar loads); passing it by
 // value would make the compiler copy its arrays into per-lane scratch at every launch.
 // MODE: AgxFetch for FP64 tables, or -1 for the FP32 tables.  Every lane of the grid takes
 // part in the evaluation (out-of-range lanes evaluate their block's first point and discard
 // it) so that the cooperative fetcher sees wave-uniform control flow.
-template<int MODE>
-__global__ __launch_bounds__(256) void evalKernel(const AgxMultipoleDev* __restrict__ Pp,
+template<int MODE, bool NFW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AGX_EVAL_WAVES))) void evalKernel(const AgxMultipoleDev* __restrict__ Pp,
     long n, const double* __restrict__ pos, double* __restrict__ phi, double* __restrict__ grad)
 {
     __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? 4 * FetchCoop::LDS_DOUBLE2_PER_WAVE : 1];
     const AgxMultipoleDev& P = *Pp;
+    // short-lived blocks: the grids are read from global memory (staging them costs more)
+    const double AGX_GLOBAL* gR = gmem(P.gridR);
+    const double AGX_GLOBAL* gT = gmem(P.gridT);
     const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool inRange = i0 < n;
     const long i = inRange ? i0 : (long)blockIdx.x * blockDim.x;
@@ -45,16 +52,16 @@ __global__ __launch_bounds__(256) void evalKernel(const AgxMultipoleDev* __restr
         FetchCoop f{ gmem(reinterpret_cast<const vd2*>(P.nodes)),
             (vd2 AGX_LDS*)(stage + (threadIdx.x >> 6) * FetchCoop::LDS_DOUBLE2_PER_WAVE),
             (int)(threadIdx.x & 63) };
-        multipoleCar(P, f, x, y, z, p, gx, gy, gz);
+        multipoleCar<decltype(f), const double AGX_GLOBAL*, NFW>(P, f, x, y, z, p, gx, gy, gz, gR, gT);
     } else if constexpr(MODE == AGX_FETCH_VEC2) {
         FetchVec2 f{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
-        multipoleCar(P, f, x, y, z, p, gx, gy, gz);
+        multipoleCar<decltype(f), const double AGX_GLOBAL*, NFW>(P, f, x, y, z, p, gx, gy, gz, gR, gT);
     } else if constexpr(MODE == AGX_FETCH_SCALAR) {
         FetchScalar f{ gmem(P.nodes) };
-        multipoleCar(P, f, x, y, z, p, gx, gy, gz);
+        multipoleCar<decltype(f), const double AGX_GLOBAL*, NFW>(P, f, x, y, z, p, gx, gy, gz, gR, gT);
     } else {
         FetchFloat f{ gmem(P.nodesF) };
-        multipoleCar(P, f, x, y, z, p, gx, gy, gz);
+        multipoleCar<decltype(f), const double AGX_GLOBAL*, NFW>(P, f, x, y, z, p, gx, gy, gz, gR, gT);
     }
     if(!inRange) return;
     if(phi) phi[i] = p;
@@ -63,53 +70,65 @@ __global__ __launch_bounds__(256) void evalKernel(const AgxMultipoleDev* __restr
 
 // Threads per workgroup of the orbit kernel (fixed: it sizes the LDS block below).
 constexpr int OBS = 256;
+#ifndef AGX_ALIGN_PHASES
+#define AGX_ALIGN_PHASES 1
+#endif
 
 // one right-hand-side potential gradient in the inertial frame (orbit.cpp:182-208)
-template<int MODE>
+template<int MODE, bool NFW>
 __device__ __forceinline__ void rhsEval(const AgxMultipoleDev& P, vd2* stage, double x, double y,
-    double z, double& pot, double& gx, double& gy, double& gz)
+    double z, double& pot, double& gx, double& gy, double& gz, const double* gR, const double* gT)
 {
     if constexpr(MODE == AGX_FETCH_COOP) {
         FetchCoop f{ gmem(reinterpret_cast<const vd2*>(P.nodes)),
             (vd2 AGX_LDS*)(stage + (threadIdx.x >> 6) * FetchCoop::LDS_DOUBLE2_PER_WAVE),
             (int)(threadIdx.x & 63) };
-        multipoleCar(P, f, x, y, z, pot, gx, gy, gz);
+        multipoleCar<decltype(f), const double*, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     } else if constexpr(MODE == AGX_FETCH_VEC2) {
         FetchVec2 f{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
-        multipoleCar(P, f, x, y, z, pot, gx, gy, gz);
+        multipoleCar<decltype(f), const double*, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     } else if constexpr(MODE == AGX_FETCH_SCALAR) {
         FetchScalar f{ gmem(P.nodes) };
-        multipoleCar(P, f, x, y, z, pot, gx, gy, gz);
+        multipoleCar<decltype(f), const double*, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     } else {
         FetchFloat f{ gmem(P.nodesF) };
-        multipoleCar(P, f, x, y, z, pot, gx, gy, gz);
+        multipoleCar<decltype(f), const double*, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     }
 }
 
 // WAVES: wavefronts per SIMD the kernel is compiled for (register budget 512 / WAVES); both
 // variants exist so that the autotuner can choose at run time.
-template<int MODE, int WAVES>
+template<int MODE, int WAVES, bool NFW>
 __global__ __launch_bounds__(OBS) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
 void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
 {
     __shared__ double lds[F_COUNT * OBS];
     __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? (OBS / 64) * FetchCoop::LDS_DOUBLE2_PER_WAVE : 1];
+    __shared__ double gridLds[AGX_LDS_GRID];
     const AgxMultipoleDev& P = *Pp;
+    const double *gR, *gT;
+    stageGrids(P, gridLds, gR, gT);
     LaneT<OBS> L;
     L.sh = lds + threadIdx.x;
     fetchOrbit(A, L);
     const double accRel = A.accuracy;
+    unsigned int it = 0;
     while(true) {
         const bool active = L.phase != PH_IDLE;
         if(!__any(active)) break;
+        // Stage sequences are aligned to the wavefront's iteration counter: an attempt's first
+        // stage starts when it % 12 == 0 (a lane that has just initialised waits), so all lanes
+        // take the same branch of the phase machine below instead of several in turn.
+        const bool run = active && !(L.phase == PH_S2 && AGX_ALIGN_PHASES && it % 12 != 0);
+        it++;
         // ---- one right-hand-side evaluation (OrbitIntegrator<Car>::eval, orbit.cpp:182-208),
         // executed by every lane of the wavefront (idle lanes evaluate a placeholder point)
         double ca = 1, sa = 0;
         if(A.Omega != 0 && active) sincosA(A.Omega * (L.tb + L.toff), sa, ca);
         double pot, gx, gy, gz;
-        rhsEval<MODE>(P, stage, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa, L.y[2],
-            pot, gx, gy, gz);
-        if(!active) continue;
+        rhsEval<MODE, NFW>(P, stage, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa, L.y[2],
+            pot, gx, gy, gz, gR, gT);
+        if(!run) continue;
         double f[6];
         f[0] = L.y[3]; f[1] = L.y[4]; f[2] = L.y[5];
         f[3] = -gx * ca + gy * sa;
@@ -131,14 +150,19 @@ hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, in
     if(n <= 0) return hipSuccess;
     const int block = 256;
     const dim3 grid((unsigned)((n + block - 1) / block));
-    if(precision == AGX_FP32_TABLES)
-        hipLaunchKernelGGL(evalKernel<-1>, grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
+    if(P.nfwMass != 0) {   // composite with an NFW halo: 16-byte per-lane fetch (or FP32 tables)
+        if(precision == AGX_FP32_TABLES)
+            hipLaunchKernelGGL((evalKernel<-1, true>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
+        else
+            hipLaunchKernelGGL((evalKernel<AGX_FETCH_VEC2, true>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
+    } else if(precision == AGX_FP32_TABLES)
+        hipLaunchKernelGGL((evalKernel<-1, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     else if(fetch == AGX_FETCH_COOP)
-        hipLaunchKernelGGL(evalKernel<AGX_FETCH_COOP>, grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
+        hipLaunchKernelGGL((evalKernel<AGX_FETCH_COOP, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     else if(fetch == AGX_FETCH_VEC2)
-        hipLaunchKernelGGL(evalKernel<AGX_FETCH_VEC2>, grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
+        hipLaunchKernelGGL((evalKernel<AGX_FETCH_VEC2, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     else
-        hipLaunchKernelGGL(evalKernel<AGX_FETCH_SCALAR>, grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
+        hipLaunchKernelGGL((evalKernel<AGX_FETCH_SCALAR, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     return hipGetLastError();
 }
 
@@ -146,13 +170,15 @@ int agxOrbitKernelBlockSize() { return OBS; }
 
 namespace {
 typedef void (*OrbitKernelFn)(const AgxMultipoleDev*, AgxOrbitArgs);
-OrbitKernelFn orbitKernelFor(int precision, int fetch, int waves)
+OrbitKernelFn orbitKernelFor(int precision, int fetch, int waves, bool nfw)
 {
+    if(nfw)   // composite with an NFW halo: 16-byte per-lane fetch (or FP32 tables), 1 wave/SIMD
+        return precision == AGX_FP32_TABLES ? orbitKernel<-1, 1, true> : orbitKernel<AGX_FETCH_VEC2, 1, true>;
     const bool two = waves == 2;
-    if(precision == AGX_FP32_TABLES) return two ? orbitKernel<-1, 2> : orbitKernel<-1, 1>;
-    if(fetch == AGX_FETCH_COOP) return two ? orbitKernel<AGX_FETCH_COOP, 2> : orbitKernel<AGX_FETCH_COOP, 1>;
-    if(fetch == AGX_FETCH_VEC2) return two ? orbitKernel<AGX_FETCH_VEC2, 2> : orbitKernel<AGX_FETCH_VEC2, 1>;
-    return two ? orbitKernel<AGX_FETCH_SCALAR, 2> : orbitKernel<AGX_FETCH_SCALAR, 1>;
+    if(precision == AGX_FP32_TABLES) return two ? orbitKernel<-1, 2, false> : orbitKernel<-1, 1, false>;
+    if(fetch == AGX_FETCH_COOP) return two ? orbitKernel<AGX_FETCH_COOP, 2, false> : orbitKernel<AGX_FETCH_COOP, 1, false>;
+    if(fetch == AGX_FETCH_VEC2) return two ? orbitKernel<AGX_FETCH_VEC2, 2, false> : orbitKernel<AGX_FETCH_VEC2, 1, false>;
+    return two ? orbitKernel<AGX_FETCH_SCALAR, 2, false> : orbitKernel<AGX_FETCH_SCALAR, 1, false>;
 }
 
 __global__ void fillU32Kernel(unsigned int* p, long n, unsigned int v)
@@ -161,10 +187,10 @@ __global__ void fillU32Kernel(unsigned int* p, long n, unsigned int v)
 }
 }  // namespace
 
-int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves)
+int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves, bool nfw)
 {
     int nb = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernelFor(precision, fetch, waves), OBS, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernelFor(precision, fetch, waves, nfw), OBS, 0);
     return e == hipSuccess ? nb : 1;
 }
 
@@ -174,7 +200,8 @@ hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, i
     (void)P;
     if(A.n <= 0) return hipSuccess;
     if(blockSize != OBS) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(orbitKernelFor(precision, fetch, waves), dim3(numBlocks), dim3(OBS), 0, stream, dP, A);
+    hipLaunchKernelGGL(orbitKernelFor(precision, fetch, waves, P.nfwMass != 0), dim3(numBlocks), dim3(OBS), 0,
+        stream, dP, A);
     return hipGetLastError();
 }
 
