@@ -253,12 +253,12 @@ __device__ __forceinline__ void fetch4(const FetchCoop& f, int r0, int r1, int r
 // recLL is the record of corner (xi, yi); (xi, yi+1) follows at +ns records and (xi+1, yi) at
 // +nt*ns records.  The reference order of the nine node arrays (QuinticSpline2d::evalDeriv,
 // math_spline.cpp:2572-2597) maps to the record as q = 0..8 above.
-template<class F, typename R = typename F::real>
-__device__ __forceinline__ void splineSlot(const F& fetch, int recLL, int ns, int nt, int offsetCorner,
+// value and (x, y) derivatives of one harmonic slot from its four corner records v
+// (ll, lu, ul, uu): QuinticSpline2d::evalDeriv (src/math_spline.cpp:2535-2597)
+template<typename R>
+__device__ __forceinline__ void splineSlotFrom(const R (&v)[4][10], int offsetCorner,
     const QuinticWT<R>& wy, const QuinticWT<R>& wx, R& z, R& zx, R& zy)
 {
-    R v[4][10];
-    fetch4(fetch, recLL, recLL + ns, recLL + nt * ns, recLL + nt * ns + ns, v);
     const R* ll = v[0]; const R* lu = v[1]; const R* ul = v[2]; const R* uu = v[3];
     const R off = offsetCorner == 0 ? ll[0] : offsetCorner == 1 ? lu[0] :
                   offsetCorner == 2 ? ul[0] : uu[0];
@@ -274,6 +274,22 @@ __device__ __forceinline__ void splineSlot(const F& fetch, int recLL, int ns, in
     wx.eval2(F0, F1, F2, F3, F4, F5, z, zx);
     zy = wx.eval1(dF0, dF1, dF2, dF3, dF4, dF5);
     z += off;
+}
+
+// the four corner records of slot record recLL (cell corner (i, j)): (i,j), (i,j+1), (i+1,j), (i+1,j+1)
+template<class F, typename R = typename F::real>
+__device__ __forceinline__ void fetchSlot(const F& fetch, int recLL, int ns, int nt, R (&v)[4][10])
+{
+    fetch4(fetch, recLL, recLL + ns, recLL + nt * ns, recLL + nt * ns + ns, v);
+}
+
+template<class F, typename R = typename F::real>
+__device__ __forceinline__ void splineSlot(const F& fetch, int recLL, int ns, int nt, int offsetCorner,
+    const QuinticWT<R>& wy, const QuinticWT<R>& wx, R& z, R& zx, R& zy)
+{
+    R v[4][10];
+    fetchSlot(fetch, recLL, ns, nt, v);
+    splineSlotFrom(v, offsetCorner, wy, wx, z, zx, zy);
 }
 
 // cos/sin(k phi) for k = 1..m by the recurrence of trigMultiAngle (math_sphharm.cpp:134-152),
@@ -308,7 +324,9 @@ struct TrigRecT {
 // Control flow is uniform across lanes (all harmonics are always evaluated; points outside
 // the grid read cell (0,0) and return NaN, as the reference's spline does), which the
 // cooperative fetcher requires.
-template<class F, class GP>
+// PF: request the next harmonic's corner records before evaluating the current one (software
+// pipelining: more registers, one L2 round trip fewer on the critical path per harmonic)
+template<class F, class GP, bool PF = false>
 __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const F& fetch,
     double R, double z, double phi, double& pot, double& gR, double& gz, double& gphi,
     GP gridR, GP gridT)
@@ -331,7 +349,14 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const F& 
     // positive m advance one running recurrence
     const int zs = P.zeroSlot;
     Rt P0, d0r, d0t;
-    splineSlot(fetch, rec0 + zs, ns, nt, offsetCorner, wy, wx, P0, d0r, d0t);
+    Rt va[4][10], vb[4][10];
+    const int s0 = zs == 0 ? 1 : 0;
+    if constexpr(PF) {
+        fetchSlot(fetch, rec0 + zs, ns, nt, va);
+        if(s0 < ns) fetchSlot(fetch, rec0 + s0, ns, nt, vb);
+        splineSlotFrom(va, offsetCorner, wy, wx, P0, d0r, d0t);
+    } else
+        splineSlot(fetch, rec0 + zs, ns, nt, offsetCorner, wy, wx, P0, d0r, d0t);
     if(P.logScaling) {
         const Rt expX = exp(P0), val = Rt(1) / ((Rt)P.invPhi0 - expX), dPhidX = val * val * expX;
         P0 = val;
@@ -343,8 +368,8 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const F& 
         TrigRecT<Rt> pos, neg;
         pos.init(phi);
         neg.alpha = pos.alpha; neg.beta = pos.beta;
-        for(int s = 0; s < ns; s++) {
-            if(s == zs) continue;
+        // harmonic slot s (not the m = 0 slot) from its corner records v
+        auto addSlot = [&](int s, const Rt (&v)[4][10]) {
             const int m = P.slotM[s];
             Rt trig, dtrig;
             if(m > 0) {
@@ -356,7 +381,7 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const F& 
                 trig = neg.s; dtrig = Rt(-m) * neg.c;
             }
             Rt Pm, dmr, dmt;
-            splineSlot(fetch, rec0 + s, ns, nt, offsetCorner, wy, wx, Pm, dmr, dmt);
+            splineSlotFrom(v, offsetCorner, wy, wx, Pm, dmr, dmt);
             if(P.logScaling) {
                 dmr = dmr * P0 + Pm * d0r;
                 dmt = dmt * P0 + Pm * d0t;
@@ -366,6 +391,27 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const F& 
             trDr   += dmr * trig;
             trDth  += dmt * trig;
             trDphi += Pm * dtrig;
+        };
+        if constexpr(PF) {
+        // vb holds slot s on entry; the two buffers alternate
+        for(int s = s0; s < ns; ) {
+            int s2 = s + 1;
+            if(s2 == zs) s2++;
+            if(s2 < ns) fetchSlot(fetch, rec0 + s2, ns, nt, va);
+            addSlot(s, vb);
+            if(s2 >= ns) break;
+            int s3 = s2 + 1;
+            if(s3 == zs) s3++;
+            if(s3 < ns) fetchSlot(fetch, rec0 + s3, ns, nt, vb);
+            addSlot(s2, va);
+            s = s3;
+        }
+        } else {
+        for(int s = 0; s < ns; s++) {
+            if(s == zs) continue;
+            fetchSlot(fetch, rec0 + s, ns, nt, va);
+            addSlot(s, va);
+        }
         }
     }
     const double rinv = 1 / r, r2inv = rinv * rinv;
@@ -505,13 +551,13 @@ __device__ __forceinline__ void powerLawEval(const AgxPowerLawDev& L, double R, 
 // power-law regions then overwrite the result.
 // gridR/gridT: the ln r and tau grids, as global-memory pointers or generic ones (an LDS
 // copy); null: the header's arrays
-template<class F, class GP = const double AGX_GLOBAL*>
+template<class F, class GP = const double AGX_GLOBAL*, bool PF = false>
 __device__ __forceinline__ void multipoleCyl(const AgxMultipoleDev& P, const F& fetch,
     double R, double z, double phi, double& pot, double& gR, double& gz, double& gphi,
     GP gridR = nullptr, GP gridT = nullptr)
 {
     const double rsq = R * R + z * z;
-    interiorEval(P, fetch, R, z, phi, pot, gR, gz, gphi, gridR ? gridR : GP(gmem(P.gridR)),
+    interiorEval<F, GP, PF>(P, fetch, R, z, phi, pot, gR, gz, gphi, gridR ? gridR : GP(gmem(P.gridR)),
         gridT ? gridT : GP(gmem(P.gridT)));
     if(rsq < P.rsqInner)
         powerLawEval(P.inner, R, z, phi, pot, gR, gz, gphi);
@@ -549,7 +595,7 @@ __device__ __forceinline__ void nfwDeriv(double mass, double a, double r, double
 // BasePotentialCyl::evalCar: Car -> Cyl, evaluate, gradient back to Car (+dPhi/dx)
 // NFW: whether the optional analytic halo is handled (kernels for potentials without one are
 // instantiated with false: keeping x, y, z live to the end costs a wavefront of occupancy)
-template<class F, class GP = const double AGX_GLOBAL*, bool NFW = true>
+template<class F, class GP = const double AGX_GLOBAL*, bool NFW = true, bool PF = false>
 __device__ __forceinline__ void multipoleCar(const AgxMultipoleDev& P, const F& fetch,
     double x, double y, double z, double& pot, double& gx, double& gy, double& gz,
     GP gridR = nullptr, GP gridT = nullptr)
@@ -565,7 +611,7 @@ __device__ __forceinline__ void multipoleCar(const AgxMultipoleDev& P, const F& 
         phi = atan2A(y, x);
     }
     double gR, gzc, gphi;
-    multipoleCyl(P, fetch, R, z, phi, pot, gR, gzc, gphi, gridR, gridT);
+    multipoleCyl<F, GP, PF>(P, fetch, R, z, phi, pot, gR, gzc, gphi, gridR, gridT);
     gx = gR * dRdx + gphi * dphidx;
     gy = gR * dRdy + gphi * dphidy;
     gz = gzc;

@@ -6,6 +6,7 @@
 // different block sizes (the LDS stride) share the code.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "agx_device.h"
 #include "agx_kernels.h"
 
@@ -67,14 +68,36 @@ enum : int { PH_IDLE = -1, PH_INIT_A = 0, PH_INIT_B, PH_INIT_C, PH_S2, PH_S3, PH
 // register state this keeps the kernel within 256 VGPRs (two wavefronts per SIMD) while two
 // workgroups share one CU's 160 KB of LDS.
 enum : int { F_K6 = 0, F_K7 = 6, F_K8 = 12, F_K9 = 18, F_K10 = 24, F_H1 = 30, F_ACC = 31, F_E0 = 32,
-    F_MAXDE = 33, F_T0 = 34, F_DTS = 35, F_LASTTB = 36, F_LASTH = 37, F_COUNT = 38 };
+    F_MAXDE = 33, F_T0 = 34, F_DTS = 35, F_LASTTB = 36, F_LASTH = 37, F_COUNT = 38,
+    // with the stage vectors k1..k5 and the step's start x in LDS as well
+    F_X = 38, F_K1 = 44, F_K2 = 50, F_K3 = 56, F_K4 = 62, F_K5 = 68, F_COUNT_K = 74 };
 
+// a 6-vector stored in a lane's LDS column (stride = block size)
 template<int STRIDE>
+struct LdsVec {
+    double* p;
+    __device__ __forceinline__ double& operator[](int i) const { return p[i * STRIDE]; }
+};
+
+// KLDS: keep x and k1..k5 in LDS too (fewer registers live across the potential evaluation,
+// for kernels whose occupancy is set by registers and that have LDS to spare)
+template<int STRIDE, bool KLDS = false>
 struct LaneT {
+    typedef typename std::conditional<KLDS, LdsVec<STRIDE>, double[6]>::type Vec;
+    static constexpr int LDS_FIELDS = KLDS ? F_COUNT_K : F_COUNT;
     double* sh;                // this lane's LDS column
     int oi, phase, nbad, S, nsteps, natt, actual;
-    double x[6], k1[6], k2[6], k3[6], k4[6], k5[6], y[6];
+    Vec x, k1, k2, k3, k4, k5;
+    double y[6];
     double toff, tb, tend, h, hnext, preverr, rem;
+    __device__ __forceinline__ void attach(double* col)
+    {
+        sh = col;
+        if constexpr(KLDS) {
+            x.p = col + F_X * STRIDE; k1.p = col + F_K1 * STRIDE; k2.p = col + F_K2 * STRIDE;
+            k3.p = col + F_K3 * STRIDE; k4.p = col + F_K4 * STRIDE; k5.p = col + F_K5 * STRIDE;
+        }
+    }
     __device__ __forceinline__ double& f(int field) const { return sh[field * STRIDE]; }
     __device__ __forceinline__ double& k6(int i) const { return sh[(F_K6 + i) * STRIDE]; }
     __device__ __forceinline__ double& k7(int i) const { return sh[(F_K7 + i) * STRIDE]; }
@@ -83,7 +106,8 @@ struct LaneT {
     __device__ __forceinline__ double& k10(int i) const { return sh[(F_K10 + i) * STRIDE]; }
 };
 
-__device__ __forceinline__ double energyOf(const double* xv, double pot, double Omega)
+template<class V>
+__device__ __forceinline__ double energyOf(const V& xv, double pot, double Omega)
 {
     double E = pot + 0.5 * (xv[3] * xv[3] + xv[4] * xv[4] + xv[5] * xv[5]);
     if(Omega != 0) E -= Omega * (xv[0] * xv[4] - xv[1] * xv[3]);
@@ -113,7 +137,8 @@ __device__ __forceinline__ double denseOut(const Lane& L, int i, double off)
 }
 
 // rotate an inertial-frame state into the rotating frame at angle ang (orbit.cpp:119-133)
-__device__ __forceinline__ void toRotating(double Omega, double ang, const double* d, double* o)
+template<class V>
+__device__ __forceinline__ void toRotating(double Omega, double ang, const V& d, double* o)
 {
     if(Omega != 0) {
         double ca = 1, sa = 0;

@@ -70,6 +70,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AGX_EVAL_WA
 
 // Threads per workgroup of the orbit kernel (fixed: it sizes the LDS block below).
 constexpr int OBS = 256;
+#ifndef AGX_K_IN_LDS
+#define AGX_K_IN_LDS 0
+#endif
 #ifndef AGX_ALIGN_PHASES
 #define AGX_ALIGN_PHASES 1
 #endif
@@ -83,16 +86,16 @@ __device__ __forceinline__ void rhsEval(const AgxMultipoleDev& P, vd2* stage, do
         FetchCoop f{ gmem(reinterpret_cast<const vd2*>(P.nodes)),
             (vd2 AGX_LDS*)(stage + (threadIdx.x >> 6) * FetchCoop::LDS_DOUBLE2_PER_WAVE),
             (int)(threadIdx.x & 63) };
-        multipoleCar<decltype(f), const double*, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
+        multipoleCar<decltype(f), const double*, NFW, true>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     } else if constexpr(MODE == AGX_FETCH_VEC2) {
         FetchVec2 f{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
-        multipoleCar<decltype(f), const double*, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
+        multipoleCar<decltype(f), const double*, NFW, true>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     } else if constexpr(MODE == AGX_FETCH_SCALAR) {
         FetchScalar f{ gmem(P.nodes) };
-        multipoleCar<decltype(f), const double*, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
+        multipoleCar<decltype(f), const double*, NFW, true>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     } else {
         FetchFloat f{ gmem(P.nodesF) };
-        multipoleCar<decltype(f), const double*, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
+        multipoleCar<decltype(f), const double*, NFW, true>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     }
 }
 
@@ -102,14 +105,17 @@ template<int MODE, int WAVES, bool NFW>
 __global__ __launch_bounds__(OBS) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
 void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
 {
-    __shared__ double lds[F_COUNT * OBS];
+    // at one wavefront per SIMD the block has the CU's LDS to itself: the stage vectors k1..k5
+    // and x live there too (the cooperative fetcher needs its own LDS staging area instead)
+    typedef LaneT<OBS, AGX_K_IN_LDS && WAVES == 1 && MODE != AGX_FETCH_COOP> Lane;
+    __shared__ double lds[Lane::LDS_FIELDS * OBS];
     __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? (OBS / 64) * FetchCoop::LDS_DOUBLE2_PER_WAVE : 1];
     __shared__ double gridLds[AGX_LDS_GRID];
     const AgxMultipoleDev& P = *Pp;
     const double *gR, *gT;
     stageGrids(P, gridLds, gR, gT);
-    LaneT<OBS> L;
-    L.sh = lds + threadIdx.x;
+    Lane L;
+    L.attach(lds + threadIdx.x);
     fetchOrbit(A, L);
     const double accRel = A.accuracy;
     unsigned int it = 0;

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(SBS) void orbitSurrogateKernel(const AgxMultipoleDe
     const AgxMlpDev& M = *Mp;
     const int lane = threadIdx.x & 63;
     LaneT<SBS> L;
-    L.sh = lds + threadIdx.x;
+    L.attach(lds + threadIdx.x);
     fetchOrbit(A, L);
     const double accRel = A.accuracy, rmin2 = SA.rmin * SA.rmin, rmax2 = SA.rmax * SA.rmax;
     const double kOut = M.outScale;
