@@ -681,6 +681,7 @@ void allocTrainStep(TrainAlloc& A, AgxTrainBuffers& T, int batch, long nbatches,
     T.Z = A.get<float>(E);
     T.G = A.get<float>(E);
     T.Gb = A.get<uint16_t>(E);
+    T.dout = A.get<float>(batch);
     T.step = A.get<long>(1);
     T.lossHist = A.get<float>(histLen);
     HIPCHECK(hipMemset(T.m1, 0, AGX_MLP_NPARAM * 4));

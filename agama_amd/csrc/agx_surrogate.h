@@ -26,6 +26,7 @@ struct AgxTrainBuffers {
     float* Z;                 // GEMM output [batch][256]
     float* G;                 // gradient w.r.t. pre-activations [batch][256]
     void* Gb;                 // its bf16 copy
+    float* dout;              // d loss / d out per sample [batch]
     long* step;               // step counter (device)
     float* lossHist;          // per-step mean squared error (device, accumulated)
 };

@@ -182,34 +182,55 @@ __global__ void biasSiluKernel(const float* __restrict__ Z, const float* __restr
 
 // output layer, loss and the start of the backward; one wavefront per sample.
 //   out = w5 . H4 + b5;  e = out - target;  loss += e^2 / batch;  dout = 2 e / batch
-//   G = dout * w5 * S4 (float), Gb = bf16(G);  dw5 += dout * H4;  db5 += dout
+//   G = dout * w5 * S4 (float), Gb = bf16(G);  db5 += dout  (dw5 by dw5Kernel from dout)
+// Sums over the block's four samples are formed in LDS before one atomic per block.
 __global__ __launch_bounds__(256) void outputLossKernel(const float* __restrict__ theta,
     const bf16* __restrict__ H4, const float* __restrict__ S4, const float* __restrict__ target,
     const long* __restrict__ step, long nbatches, int batch, float* __restrict__ G, bf16* __restrict__ Gb,
-    float* __restrict__ grad, float* __restrict__ lossHist)
+    float* __restrict__ dout, float* __restrict__ grad, float* __restrict__ lossHist)
 {
+    __shared__ float red[2][4];
     const long off = batchOffset(step, nbatches, batch);
-    const int lane = threadIdx.x & 63;
-    const long b = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if(b >= batch) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long b = (long)blockIdx.x * 4 + w;
+    const bool valid = b < batch;
+    const long bb = valid ? b : 0;
     const float* w5 = theta + AGX_P_W5;
     float part = 0.f;
-    for(int q = 0; q < 4; q++) part += w5[lane * 4 + q] * (float)H4[b * 256 + lane * 4 + q];
+    for(int q = 0; q < 4; q++) part += w5[lane * 4 + q] * (float)H4[bb * 256 + lane * 4 + q];
     for(int sh = 32; sh >= 1; sh >>= 1) part += __shfl_xor(part, sh);
     const float out = part + theta[AGX_P_B5];
-    const float e = out - target[off + b];
-    const float dout = 2.f * e / (float)batch;
+    const float e = out - target[off + bb];
+    const float d = valid ? 2.f * e / (float)batch : 0.f;
     if(lane == 0) {
-        atomicAdd(lossHist + *step, e * e / (float)batch);
-        atomicAdd(grad + AGX_P_B5, dout);
+        red[0][w] = valid ? e * e / (float)batch : 0.f;
+        red[1][w] = d;
+        if(valid) dout[b] = d;
     }
-    for(int q = 0; q < 4; q++) {
-        const long idx = b * 256 + lane * 4 + q;
-        const float gv = dout * w5[lane * 4 + q] * S4[idx];
-        G[idx] = gv;
-        Gb[idx] = (bf16)gv;
-        atomicAdd(grad + AGX_P_W5 + lane * 4 + q, dout * (float)H4[idx]);
+    if(valid)
+        for(int q = 0; q < 4; q++) {
+            const long idx = b * 256 + lane * 4 + q;
+            const float gv = d * w5[lane * 4 + q] * S4[idx];
+            G[idx] = gv;
+            Gb[idx] = (bf16)gv;
+        }
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        atomicAdd(lossHist + *step, red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+        atomicAdd(grad + AGX_P_B5, red[1][0] + red[1][1] + red[1][2] + red[1][3]);
     }
+}
+
+// output-layer weight gradient dw5[o] += sum_b dout[b] H4[b][o] (atomic partial sums over
+// row chunks, one thread per neuron)
+__global__ __launch_bounds__(256) void dw5Kernel(const bf16* __restrict__ H4, const float* __restrict__ dout,
+    int batch, int rowsPer, float* __restrict__ grad)
+{
+    const int o = threadIdx.x;
+    const long r0 = (long)blockIdx.x * rowsPer;
+    float s = 0.f;
+    for(long r = r0; r < r0 + rowsPer && r < batch; r++) s += dout[r] * (float)H4[r * 256 + o];
+    atomicAdd(grad + AGX_P_W5 + o, s);
 }
 
 // bias gradient: column sums of G [batch][256] (atomic partial sums over row chunks)
@@ -405,7 +426,8 @@ hipError_t agxEnqueueTrainStep(const AgxTrainBuffers& T, const AgxTrainHyper& hp
             (bf16*)T.H[l + 1], T.S[l + 1]));
     }
     AGX_TRY(hipLaunchKernelGGL(outputLossKernel, dim3((B + 3) / 4), dim3(256), 0, s, T.theta, (const bf16*)T.H[3],
-        T.S[3], T.target, T.step, T.nbatches, B, T.G, (bf16*)T.Gb, T.grad, T.lossHist));
+        T.S[3], T.target, T.step, T.nbatches, B, T.G, (bf16*)T.Gb, T.dout, T.grad, T.lossHist));
+    AGX_TRY(hipLaunchKernelGGL(dw5Kernel, dim3(cg), dim3(256), 0, s, (const bf16*)T.H[3], T.dout, B, rowsPer, T.grad));
     // backward: layers 4, 3, 2
     const int woff[3] = { AGX_P_W2, AGX_P_W3, AGX_P_W4 };
     for(int l = 2; l >= 0; l--) {
