@@ -44,7 +44,7 @@ FLOPS_PER_ORBIT_INIT = 3 * FLOPS_PER_EVAL + 80
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix), AMD published figure
 # HBM bytes per launch of the orbit kernel on this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs scaled by a 1 GiB copy (profiles/orbitKernel_1M_uneven_triaxial_dehnen_lmax8.md)
-ORBIT_TRAFFIC_BYTES = 566e6 + 291e6
+ORBIT_TRAFFIC_BYTES = 566e6 + 289e6
 
 
 def parse():
