@@ -16,6 +16,21 @@ DEHNEN_TRIAXIAL = dict(type="Multipole", density="Dehnen", gamma=1, mass=1, scal
                        axisRatioY=0.8, axisRatioZ=0.6, lmax=8, gridSizeR=50, rmin=1e-3, rmax=1e3)
 
 
+def read_coef_file(path):
+    """(radii[nr], Phi[(lmax+1)^2, nr], dPhi[...]) from an Agama Multipole coefficient file
+    (the '#Phi' and '#dPhi/dr' blocks; src/potential_factory.cpp:621-641 format)"""
+    lines = open(path).read().splitlines()
+    blocks = []
+    for tag in ("#Phi", "#dPhi/dr"):
+        i = lines.index(tag) + 2          # skip the column header
+        rows = []
+        while i < len(lines) and lines[i].strip():
+            rows.append([float(v) for v in lines[i].split()])
+            i += 1
+        blocks.append(np.array(rows))
+    return blocks[0][:, 0], blocks[0][:, 1:].T.copy(), blocks[1][:, 1:].T.copy()
+
+
 def rng(seed=42):
     return np.random.Generator(np.random.Philox(seed))
 

@@ -1565,7 +1565,182 @@ struct DOP853 {
 };
 
 // orbit.cpp:182-208  OrbitIntegrator<Car>::eval (integration in the inertial frame)
-struct OrbitSys : public OdeSys {
+// math_ode.h:150-166, math_ode.cpp:426-701 : 8th-order Runge-Kutta-Nystrom (Dormand-Prince
+// RKN6(8)) for x'' = f(t, x), with a PI step controller and a quintic Taylor dense output.
+// The system supplies eval2: the acceleration from positions xv[0..2] (and, with accFac, the
+// accuracy factor from xv[3..5]); the first-order eval is used by initTimeStep.
+struct OdeSys2 : public OdeSys {
+    virtual void eval2(double t, const double* xv, double* d2xdt2, double* af) const = 0;
+};
+
+struct DPRKN8 {
+    const OdeSys2& odeSystem;
+    const int NDIM;
+    const double accRel;
+    double prevTimeStep, nextTimeStep, qold;
+    std::vector<double> state;      // x, v, a, jerk, and the 4th, 5th Taylor coefficients
+    long nattempts;
+    DPRKN8(const OdeSys2& sys, double _accRel) : odeSystem(sys), NDIM(sys.size()),
+        accRel(10 * pow(_accRel, 0.9)), prevTimeStep(0), nextTimeStep(0), qold(0),
+        state(NDIM * 3), nattempts(0) {}
+
+    // math_ode.cpp:437-448
+    void init(const double stateNew[])
+    {
+        for(int d=0; d<NDIM; d++)
+            state[d] = stateNew[d];
+        double accFac = 1;
+        odeSystem.eval2(0, stateNew, &state[NDIM], nextTimeStep==0 ? &accFac : NULL);
+        if(nextTimeStep == 0)
+            nextTimeStep = initTimeStep(odeSystem, stateNew, 0, accRel) * accFac;
+        qold = 0.0001;
+    }
+
+    // math_ode.cpp:450-677
+    double doStep(double maxTimeStep)
+    {
+        if(maxTimeStep == 0)
+            return 0;
+        const int nv = NDIM/2;
+        static const double
+        c1 = 1.0/20, c2 = 1.0/10, c3 = 3.0/10, c4 = 1.0/2, c5 = 7.0/10, c6 = 9.0/10, c7 = 1.0, c8 = 1.0,
+        a21 = 1.0/800, a31 = 1.0/600, a32 = 1.0/300, a41 = 9.0/200, a42 = -9.0/100, a43 = 9.0/100,
+        a51 = -66701.0/197352, a52 = 28325.0/32892, a53 = -2665.0/5482, a54 = 2170.0/24669,
+        a61 = 227015747.0/304251000, a62 = -54897451.0/30425100, a63 = 12942349.0/10141700,
+        a64 = -9499.0/304251, a65 = 539.0/9250,
+        a71 = -1131891597.0/901789000, a72 = 41964921.0/12882700, a73 = -6663147.0/3220675,
+        a74 = 270954.0/644135, a75 = -108.0/5875, a76 = 114.0/1645,
+        a81 = 13836959.0/3667458, a82 = -17731450.0/1833729, a83 = 1063919505.0/156478208,
+        a84 = -33213845.0/39119552, a85 = 13335.0/28544, a86 = -705.0/14272, a87 = 1645.0/57088,
+        a91 = 223.0/7938, a93 = 1175.0/8064, a94 = 925.0/6048, a95 = 41.0/448, a96 = 925.0/14112,
+        a97 = 1175.0/72576,
+        b1 = 223.0/7938, b3 = 1175.0/8064, b4 = 925.0/6048, b5 = 41.0/448, b6 = 925.0/14112,
+        b7 = 1175.0/72576,
+        bp1 = 223.0/7938, bp3 = 5875.0/36288, bp4 = 4625.0/21168, bp5 = 41.0/224,
+        bp6 = 4625.0/21168, bp7 = 5875.0/36288, bp8 = 223.0/7938,
+        bt1 = 223.0/7938 - 7987313.0/109941300, bt3 = 1175.0/8064 - 1610737.0/44674560,
+        bt4 = 925.0/6048 - 10023263.0/33505920, bt5 = 41.0/448 + 497221.0/12409600,
+        bt6 = 925.0/14112 - 10023263.0/78180480, bt7 = 1175.0/72576 - 1610737.0/402071040,
+        bpt1 = 223.0/7938 - 7987313.0/109941300, bpt3 = 5875.0/36288 - 1610737.0/40207104,
+        bpt4 = 4625.0/21168 - 10023263.0/23454144, bpt5 = 41.0/224 + 497221.0/6204800,
+        bpt6 = 4625.0/21168 - 10023263.0/23454144, bpt7 = 5875.0/36288 - 1610737.0/40207104,
+        bpt8 = 223.0/7938 + 4251941.0/54970650, bpt9 = -3.0/20;
+        // stage storage: xt doubles as the evaluation point (its first nv entries)
+        std::vector<double> tmp(nv * 13);
+        double *xt = &tmp[0], *x = &state[0], *v = x + nv, *k1 = v + nv, *j1 = k1 + nv,
+            *k2 = xt + nv, *k3 = k2 + nv, *k4 = k3 + nv, *k5 = k4 + nv, *k6 = k5 + nv, *k7 = k6 + nv,
+            *k8 = k7 + nv, *k9 = k8 + nv, *xn = k9 + nv, *vn = xn + nv, *kn = vn + nv, *jn = kn + nv;
+        double sign = maxTimeStep >= 0 ? +1 : -1;
+        double h = fmin(maxTimeStep * sign, nextTimeStep) * sign;
+        int nbad = 0, niter = 0;
+        double preverr = INFINITY;
+        while(true) {
+            if(h==0 || !isFinite(h))
+                return 0;
+            nattempts++;
+            for(int i=0; i<nv; i++) xt[i] = x[i] + h * (c1 * v[i] + h * (a21 * k1[i]));
+            odeSystem.eval2(c1 * h, xt, k2, NULL);
+            for(int i=0; i<nv; i++) xt[i] = x[i] + h * (c2 * v[i] + h * (a31 * k1[i] + a32 * k2[i]));
+            odeSystem.eval2(c2 * h, xt, k3, NULL);
+            for(int i=0; i<nv; i++) xt[i] = x[i] + h * (c3 * v[i] + h * (a41 * k1[i] + a42 * k2[i] + a43 * k3[i]));
+            odeSystem.eval2(c3 * h, xt, k4, NULL);
+            for(int i=0; i<nv; i++) xt[i] = x[i] + h * (c4 * v[i] + h *
+                (a51 * k1[i] + a52 * k2[i] + a53 * k3[i] + a54 * k4[i]));
+            odeSystem.eval2(c4 * h, xt, k5, NULL);
+            for(int i=0; i<nv; i++) xt[i] = x[i] + h * (c5 * v[i] + h *
+                (a61 * k1[i] + a62 * k2[i] + a63 * k3[i] + a64 * k4[i] + a65 * k5[i]));
+            odeSystem.eval2(c5 * h, xt, k6, NULL);
+            for(int i=0; i<nv; i++) xt[i] = x[i] + h * (c6 * v[i] + h *
+                (a71 * k1[i] + a72 * k2[i] + a73 * k3[i] + a74 * k4[i] + a75 * k5[i] + a76 * k6[i]));
+            odeSystem.eval2(c6 * h, xt, k7, NULL);
+            for(int i=0; i<nv; i++) xt[i] = x[i] + h * (c7 * v[i] + h *
+                (a81 * k1[i] + a82 * k2[i] + a83 * k3[i] + a84 * k4[i] + a85 * k5[i] + a86 * k6[i] +
+                 a87 * k7[i]));
+            odeSystem.eval2(c7 * h, xt, k8, NULL);
+            for(int i=0; i<nv; i++) xt[i] = x[i] + h * (c8 * v[i] + h *
+                (a91 * k1[i] + a93 * k3[i] + a94 * k4[i] + a95 * k5[i] + a96 * k6[i] + a97 * k7[i]));
+            odeSystem.eval2(c8 * h, xt, k9, NULL);
+            for(int i=0; i<nv; i++) {
+                xn[i] = x[i] + h * (v[i] + h *
+                    (b1 * k1[i] + b3 * k3[i] + b4 * k4[i] + b5 * k5[i] + b6 * k6[i] + b7 * k7[i]));
+                vn[i] = v[i] + h *
+                    (bp1* k1[i] + bp3* k3[i] + bp4* k4[i] + bp5* k5[i] + bp6* k6[i] + bp7* k7[i] + bp8* k8[i]);
+            }
+            double accFac = 1.0;
+            odeSystem.eval2(h, xn, kn, &accFac);
+            double err = 0.0;
+            for(int i=0; i<nv; i++) {
+                double denom = (accRel * fmax(fabs(x[i]), fabs(xn[i]))) * accFac;
+                if(denom!=0) {
+                    double xtilde = pow_2(h) * (bt1 * k1[i] + bt3 * k3[i] + bt4 * k4[i] + bt5 * k5[i] +
+                        bt6 * k6[i] + bt7 * k7[i]);
+                    err += pow_2(xtilde / denom);
+                }
+                denom = (accRel * fmax(fabs(v[i]), fabs(vn[i]))) * accFac;
+                if(denom!=0) {
+                    double vtilde = h * (bpt1* k1[i] + bpt3* k3[i] + bpt4* k4[i] + bpt5* k5[i] +
+                        bpt6* k6[i] + bpt7* k7[i] + bpt8* k8[i] + bpt9* k9[i]);
+                    err += pow_2(vtilde / denom);
+                }
+            }
+            err = sqrt(err / NDIM);
+            const double gamma = 0.9, qmax = 10.0, qmin = 0.2, beta1 = 7./80, beta2 = 4./80;
+            double q1 = pow(err, beta1);
+            if(err <= 1 || ++niter >= 12) {
+                double q = accRel * accFac==INFINITY ? 0 :
+                    fmin(1 / qmin, fmax(1 / qmax, q1 / pow(qold, beta2) / gamma));
+                nextTimeStep = fabs(h) / q;
+                qold = err;
+                break;
+            }
+            if(err > 0.5*preverr) {
+                if(++nbad >= 2)
+                    q1 = 1/qmin;
+            } else
+                nbad = 0;
+            preverr = err;
+            h /= fmin(1 / qmin, q1 / gamma);
+        }
+        // math_ode.cpp:656-677: Taylor coefficients of the dense output around the step's end
+        for(int i=0; i<nv; i++) {
+            jn[i] = ((
+                -60 * (x[i] - xn[i]) / h +
+                -24 *  v[i] -  36 * vn[i]) / h +
+                -3  * k1[i] +   9 * kn[i]) / h;
+            state[4*nv+i] = ((
+                -360 * (x[i] - xn[i]) / h +
+                -168 *  v[i] - 192 * vn[i]) / h +
+                -24  * k1[i] +  36 * kn[i]) / pow_2(h);
+            state[5*nv+i] = ((
+                -720 * (x[i] - xn[i]) / h +
+                -360 *  v[i] - 360 * vn[i]) / h +
+                -60  * k1[i] +  60 * kn[i]) / (h*h*h);
+            x [i] = xn[i];
+            v [i] = vn[i];
+            k1[i] = kn[i];
+            j1[i] = jn[i];
+        }
+        prevTimeStep = h;
+        return h;
+    }
+
+    // math_ode.cpp:680-701
+    double getSol(double timeOffset, unsigned int ind) const
+    {
+        double dt = timeOffset - prevTimeStep;
+        if(dt == 0)
+            return state[ind];
+        unsigned int nv = NDIM/2;
+        int i = ind % nv;
+        if(ind < nv)
+            return state[i] + dt * (state[i+nv] + dt * (1./2) * (state[i+2*nv] + dt * (1./3) *
+                (state[i+3*nv] + dt * (1./4) * (state[i+4*nv] + dt * (1./5) * state[i+5*nv]))));
+        return state[i+nv] + dt * (state[i+2*nv] + dt * (1./2) * (state[i+3*nv] + dt * (1./3) *
+            (state[i+4*nv] + dt * (1./4) * state[i+5*nv])));
+    }
+};
+
+struct OrbitSys : public OdeSys2 {
     const orc_pot& potential;
     double Omega, timeBegin;
     mutable double lastEpot;
@@ -1594,6 +1769,25 @@ struct OrbitSys : public OdeSys {
             lastEpot = Epot;
         }
     }
+    // orbit.cpp:217-250 (OrbitIntegrator<Car>::eval2, without the Hermite jerk)
+    virtual void eval2(double timeOffset, const double xv[], double d2xdt2[], double* accFac) const
+    {
+        nevals++;
+        double ca=1, sa=0;
+        if(Omega)
+            sincos_r(Omega * (timeBegin + timeOffset), sa, ca);
+        double Epot, grad[3];
+        double pos[3] = { xv[0]*ca + xv[1]*sa, xv[1]*ca - xv[0]*sa, xv[2] };
+        potential.evalCar(pos, accFac ? &Epot : NULL, grad, timeBegin + timeOffset);
+        d2xdt2[0] = -grad[0]*ca + grad[1]*sa;
+        d2xdt2[1] = -grad[1]*ca - grad[0]*sa;
+        d2xdt2[2] = -grad[2];
+        if(accFac) {
+            double Ekin = 0.5 * (pow_2(xv[3]) + pow_2(xv[4]) + pow_2(xv[5]));
+            *accFac = fmin(1, fabs(Epot + Ekin) / fmax(fabs(Epot), Ekin));
+            lastEpot = Epot;
+        }
+    }
 };
 
 // energy (Jacobi integral when the frame rotates) of an inertial-frame state
@@ -1608,13 +1802,15 @@ inline double energyOf(const double* xv, double Epot, double Omega)
 // per-orbit body of interface_python.cpp:8107-8202 with trajectory stored as float64
 struct OrbitResult { long nsteps, nattempts, nevals; double maxde; int status; double fin[6]; };
 
-void integrateOneOrbit(const orc_pot& pot, const double ic[6], double tstart, double ttotal,
+// method: 0 = DOP853, 1 = DPRKN8 (OrbitIntParams::Method, orbit.h:116-129, 186-195)
+template<class Stepper>
+void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart, double ttotal,
     int trajsize, double accuracy, double Omega, long maxNumSteps,
     double* out_time, double* out_traj, OrbitResult& res)
 {
     const double ROUNDOFF = 10*DBL_EPSILON;
     OrbitSys sys(pot, Omega);
-    DOP853 stepper(sys, accuracy);
+    Stepper stepper(sys, accuracy);
     double samplingInterval = trajsize>1 ? fabs(ttotal) / (trajsize-1) :
         trajsize==1 ? INFINITY : 0;
     long actualSize = 0;
@@ -1729,10 +1925,37 @@ void integrateOneOrbit(const orc_pot& pot, const double ic[6], double tstart, do
     res.nevals = sys.nevals;
 }
 
+void integrateOneOrbit(const orc_pot& pot, const double ic[6], double tstart, double ttotal,
+    int trajsize, double accuracy, double Omega, long maxNumSteps,
+    double* out_time, double* out_traj, OrbitResult& res, int method=0)
+{
+    if(method == 1)
+        integrateOneOrbitWith<DPRKN8>(pot, ic, tstart, ttotal, trajsize, accuracy, Omega, maxNumSteps,
+            out_time, out_traj, res);
+    else
+        integrateOneOrbitWith<DOP853>(pot, ic, tstart, ttotal, trajsize, accuracy, Omega, maxNumSteps,
+            out_time, out_traj, res);
+}
+
 struct CallbackSys : public OdeSys {
     int n; orc_rhs_fn fn; void* ctx; double tb;
     virtual int size() const { return n; }
     virtual void eval(double t, const double* x, double* dxdt, double*) const { fn(tb + t, x, dxdt, ctx); }
+};
+
+// a second-order system x'' = a(t, x) given by a callback; its first-order form
+// restates IOdeSystem2ndOrder::eval (math_ode.cpp:13-19)
+struct CallbackSys2 : public OdeSys2 {
+    int n; orc_rhs_fn fn; void* ctx; double tb;
+    virtual int size() const { return n; }
+    virtual void eval2(double t, const double* xv, double* d2xdt2, double*) const { fn(tb + t, xv, d2xdt2, ctx); }
+    virtual void eval(double t, const double* w, double* dwdt, double* af) const
+    {
+        const int N = n / 2;
+        for(int i=0; i<N; i++)
+            dwdt[i] = w[i + N];
+        eval2(t, w, dwdt + N, af);
+    }
 };
 
 }  // namespace
@@ -1827,6 +2050,18 @@ int orc_orbit_many(const orc_pot* p, long n, const double* ic, const double* tst
     long* out_nsteps, long* out_nattempts, long* out_nevals,
     double* out_maxde, int* out_status, int nthreads)
 {
+    return orc_orbit_many_method(p, n, ic, tstart, ttotal, trajsize, traj_offset, accuracy, Omega,
+        maxNumSteps, out_time, out_traj, out_final, out_nsteps, out_nattempts, out_nevals, out_maxde,
+        out_status, nthreads, 0);
+}
+
+int orc_orbit_many_method(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, const long* traj_offset,
+    double accuracy, double Omega, long maxNumSteps,
+    double* out_time, double* out_traj, double* out_final,
+    long* out_nsteps, long* out_nattempts, long* out_nevals,
+    double* out_maxde, int* out_status, int nthreads, int method)
+{
     g_err.clear();
     for(long i=0; i<n; i++)
         if(trajsize[i] < 1) { g_err = "trajsize must be >= 1"; return 1; }
@@ -1841,7 +2076,7 @@ int orc_orbit_many(const orc_pot* p, long n, const double* ic, const double* tst
             integrateOneOrbit(*p, &ic[orb*6], tstart ? tstart[orb] : 0, ttotal[orb],
                 trajsize[orb], accuracy, Omega, maxNumSteps,
                 out_time ? out_time + traj_offset[orb] : NULL,
-                out_traj ? out_traj + traj_offset[orb]*6 : NULL, r);
+                out_traj ? out_traj + traj_offset[orb]*6 : NULL, r, method);
         } catch(std::exception&) {
             fail = 1;
             continue;
@@ -1863,6 +2098,31 @@ int orc_ode_dop853(int ndim, orc_rhs_fn rhs, void* ctx, const double* x0, double
 {
     CallbackSys sys; sys.n = ndim; sys.fn = rhs; sys.ctx = ctx; sys.tb = 0;
     DOP853 st(sys, accuracy);
+    st.init(x0);
+    double tb = 0; long ns = 0; int is = 0;
+    while(tb != ttotal) {
+        double rem = ttotal - tb;
+        double h = st.doStep(rem);
+        if(!(h > 0)) return 1;
+        while(is < nsample && tsample[is] <= tb + h) {
+            for(int i=0; i<ndim; i++) out_sample[is*ndim+i] = st.getSol(tsample[is] - tb, i);
+            is++;
+        }
+        tb = h == rem ? ttotal : tb + h;
+        sys.tb = tb;
+        ns++;
+    }
+    for(int i=0; i<ndim; i++) out_xend[i] = st.getSol(st.prevTimeStep, i);
+    if(out_nsteps) *out_nsteps = ns;
+    return 0;
+}
+
+int orc_ode_dprkn8(int ndim, orc_rhs_fn acc, void* ctx, const double* x0, double ttotal,
+    double accuracy, double* out_xend, int nsample, const double* tsample, double* out_sample,
+    long* out_nsteps)
+{
+    CallbackSys2 sys; sys.n = ndim; sys.fn = acc; sys.ctx = ctx; sys.tb = 0;
+    DPRKN8 st(sys, accuracy);
     st.init(x0);
     double tb = 0; long ns = 0; int is = 0;
     while(tb != ttotal) {

@@ -59,12 +59,27 @@ int orc_orbit_many(const orc_pot* p, long n, const double* ic, const double* tst
     double* out_time, double* out_traj, double* out_final,
     long* out_nsteps, long* out_nattempts, long* out_nevals,
     double* out_maxde, int* out_status, int nthreads);
+/* the same with the integrator chosen: method 0 = DOP853, 1 = DPRKN8
+   (reference: OrbitIntParams::Method, src/orbit.h:116-129; src/math_ode.cpp:426-701) */
+int orc_orbit_many_method(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, const long* traj_offset,
+    double accuracy, double Omega, long maxNumSteps,
+    double* out_time, double* out_traj, double* out_final,
+    long* out_nsteps, long* out_nattempts, long* out_nevals,
+    double* out_maxde, int* out_status, int nthreads, int method);
 
 /* single DOP853 run of a generic ODE system given by a C callback (closed-form tests):
    rhs(t, x, dxdt, ctx). Records the state at the end of every step and dense output at
    `nsample` requested absolute times (sorted). Fixed-step mode when accuracy is INFINITY is
    NOT special-cased (as in the reference, the error control then always accepts). */
 typedef void (*orc_rhs_fn)(double t, const double* x, double* dxdt, void* ctx);
+/* DPRKN8 (reference: src/math_ode.cpp:426-701) on x'' = acc(t, x): ndim = 2N (positions then
+   velocities); the callback fills N accelerations from the first N entries of x; otherwise
+   as orc_ode_dop853 */
+int orc_ode_dprkn8(int ndim, orc_rhs_fn acc, void* ctx, const double* x0, double ttotal,
+    double accuracy, double* out_xend, int nsample, const double* tsample, double* out_sample,
+    long* out_nsteps);
+
 int orc_ode_dop853(int ndim, orc_rhs_fn rhs, void* ctx, const double* x0, double ttotal,
     double accuracy, double* out_xend, int nsample, const double* tsample, double* out_sample,
     long* out_nsteps);

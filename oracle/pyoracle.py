@@ -35,6 +35,11 @@ def _load(name):
         "orc_orbit_many": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp,
                                           _vp, _vp, ctypes.c_int]),
+        "orc_orbit_many_method": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
+                                                 ctypes.c_double, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                 _vp, _vp, ctypes.c_int, ctypes.c_int]),
+        "orc_ode_dprkn8": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                          _vp, ctypes.c_int, _vp, _vp, _vp]),
         "orc_ode_dop853": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                           _vp, ctypes.c_int, _vp, _vp, _vp]),
         "orc_ode_dop853_fixed": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
@@ -117,7 +122,7 @@ class OraclePotential:
         return phi, grad
 
     def orbits(self, ic, ttotal, trajsize=1, tstart=None, accuracy=1e-8, Omega=0.0,
-               maxNumSteps=int(1e8), nthreads=0):
+               maxNumSteps=int(1e8), nthreads=0, method="dop853"):
         ic = np.ascontiguousarray(ic, dtype=np.float64).reshape(-1, 6)
         n = len(ic)
         tt = np.ascontiguousarray(np.broadcast_to(np.asarray(ttotal, dtype=np.float64), (n,)))
@@ -129,10 +134,11 @@ class OraclePotential:
         out = dict(time=np.empty(rows), traj=np.empty((rows, 6)), final=np.empty((n, 6)),
                    nsteps=np.empty(n, np.int64), nattempts=np.empty(n, np.int64),
                    nevals=np.empty(n, np.int64), maxde=np.empty(n), status=np.empty(n, np.int32))
-        rc = self.lib.orc_orbit_many(self.h, n, _p(ic), _p(ts), _p(tt), _p(sz), _p(offs), accuracy, Omega,
-                                     maxNumSteps, _p(out["time"]), _p(out["traj"]), _p(out["final"]),
-                                     _p(out["nsteps"]), _p(out["nattempts"]), _p(out["nevals"]),
-                                     _p(out["maxde"]), _p(out["status"]), nthreads)
+        m = {"dop853": 0, "dprkn8": 1}[method.lower()]
+        rc = self.lib.orc_orbit_many_method(self.h, n, _p(ic), _p(ts), _p(tt), _p(sz), _p(offs), accuracy,
+                                            Omega, maxNumSteps, _p(out["time"]), _p(out["traj"]),
+                                            _p(out["final"]), _p(out["nsteps"]), _p(out["nattempts"]),
+                                            _p(out["nevals"]), _p(out["maxde"]), _p(out["status"]), nthreads, m)
         if rc:
             raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
         out["offsets"] = offs
@@ -150,4 +156,7 @@ def ref_lib():
                                  ctypes.c_int, _vp, _vp, _vp]
     r.ref_ode_dop853_fixed.restype = ctypes.c_int
     r.ref_ode_dop853_fixed.argtypes = [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _vp, _vp]
+    if hasattr(r, "ref_ode_dprkn8"):
+        r.ref_ode_dprkn8.restype = ctypes.c_int
+        r.ref_ode_dprkn8.argtypes = r.ref_ode_dop853.argtypes
     return r

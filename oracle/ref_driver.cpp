@@ -17,6 +17,15 @@ public:
     { (void)af; fn(tb + t, x, dxdt, ctx); }
     virtual unsigned int size() const { return n; }
 };
+
+class CallbackOde2 : public math::IOdeSystem2ndOrder {
+public:
+    int n; ref_rhs_fn fn; void* ctx; double tb;
+    CallbackOde2(int _n, ref_rhs_fn _fn, void* _ctx) : n(_n), fn(_fn), ctx(_ctx), tb(0) {}
+    virtual void eval2(const double t, const double xv[], double d2xdt2[], double* d3, double* af) const
+    { (void)d3; (void)af; fn(tb + t, xv, d2xdt2, ctx); }
+    virtual unsigned int size() const { return n; }
+};
 }
 
 extern "C" {
@@ -28,6 +37,35 @@ int ref_ode_dop853(int ndim, ref_rhs_fn rhs, void* ctx, const double* x0, double
 {
     CallbackOde sys(ndim, rhs, ctx);
     math::OdeStepperDOP853 st(sys, accuracy);
+    st.init(x0);
+    double tb = 0, hlast = 0;
+    long ns = 0;
+    int is = 0;
+    while(tb != ttotal) {
+        double rem = ttotal - tb;
+        double h = st.doStep(rem);
+        if(!(h > 0)) return 1;
+        while(is < nsample && tsample[is] <= tb + h) {
+            for(int i=0; i<ndim; i++) out_sample[is*ndim+i] = st.getSol(tsample[is] - tb, i);
+            is++;
+        }
+        tb = h == rem ? ttotal : tb + h;
+        sys.tb = tb;
+        hlast = h;
+        ns++;
+    }
+    for(int i=0; i<ndim; i++) out_xend[i] = st.getSol(hlast, i);
+    if(out_nsteps) *out_nsteps = ns;
+    return 0;
+}
+
+/* the reference's DPRKN8 on x'' = acc(t, x), ndim = 2N; as ref_ode_dop853 */
+int ref_ode_dprkn8(int ndim, ref_rhs_fn acc, void* ctx, const double* x0, double ttotal,
+    double accuracy, double* out_xend, int nsample, const double* tsample, double* out_sample,
+    long* out_nsteps)
+{
+    CallbackOde2 sys(ndim, acc, ctx);
+    math::OdeStepperDPRKN8 st(sys, accuracy);
     st.init(x0);
     double tb = 0, hlast = 0;
     long ns = 0;
