@@ -17,6 +17,7 @@ __all__ = ["Potential", "Density", "orbit", "Surrogate", "AgamaAmdError", "PRECI
 
 PRECISION_FP64 = 0
 PRECISION_FP32 = 1
+METHOD_DPRKN8 = 0x200      # flag of the orbit entry points (AGX_METHOD_DPRKN8)
 
 # keyword spellings accepted by the reference (case-insensitive) and used by the library
 _KEYS = {k.lower(): k for k in ["type", "file", "density", "lmax", "mmax", "gridSizeR", "rmin", "rmax",
@@ -182,7 +183,7 @@ class Potential:
 def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, accuracy=1e-8,
           maxNumSteps=None, Omega=0.0, dtype="float32", separateTime=False, method="dop853",
           verbose=False, info=None, precision=PRECISION_FP64, max_rel_de=1e-6):
-    """Integrate orbits with DOP853 on the GPU - reference: agama.orbit
+    """Integrate orbits with DOP853 or DPRKN8 (method=) on the GPU - reference: agama.orbit
     (src/interface_python.cpp:7689-8281, output conventions :7941-7961).
 
     Returns the trajectory output as in the reference: with separateTime=True and equal
@@ -198,8 +199,16 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
     surrogate = isinstance(potential, Surrogate)
     if not surrogate and not isinstance(potential, Potential):
         raise TypeError("Argument 'potential' must be a valid Potential object")
-    if str(method).lower() != "dop853":
-        raise AgamaAmdError("only method='dop853' is provided by this version")
+    # method names as parsed by the reference (interface_python.cpp:7913-7926; stringsEqual is
+    # case-insensitive)
+    mname = str(method).lower() if isinstance(method, str) else None
+    if mname == "hermite":
+        raise AgamaAmdError("method='hermite' is not provided by this version")
+    if mname not in ("dop853", "dprkn8"):
+        raise ValueError("Unknown ODE integation method (valid values are 'dop853', 'dprkn8' or 'hermite')")
+    if surrogate and mname != "dop853":
+        raise AgamaAmdError("the surrogate path integrates with method='dop853' only")
+    method_flag = METHOD_DPRKN8 if mname == "dprkn8" else 0
     icarr = np.ascontiguousarray(ic, dtype=np.float64)
     if not (icarr.shape == (6,) or (icarr.ndim == 2 and icarr.shape[1] == 6)):
         raise TypeError("Argument 'ic' does not contain a valid array of length Nx6")
@@ -254,7 +263,8 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
     else:
         check(lib.agx_orbit_batch(potential.handle, n, ptr(ic2), ptr(ts), ptr(tt), ptr(sz), float(accuracy),
                                   float(Omega), maxsteps, ptr(times), ptr(traj), ptr(final), ptr(nsteps),
-                                  ptr(natt), ptr(nev), ptr(maxde), ptr(status), int(precision)), "orbit")
+                                  ptr(natt), ptr(nev), ptr(maxde), ptr(status), int(precision) | method_flag),
+              "orbit")
     if isinstance(info, dict):
         info.update(nsteps=nsteps, nattempts=natt, nevals=nev, maxde=maxde, status=status, final=final)
         if surrogate:

@@ -22,6 +22,9 @@ hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, in
 hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
     int waves, const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream);
 int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves, bool nfw);
+hipError_t agxLaunchOrbitRKN(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision,
+    const AgxOrbitArgs& A, int numBlocks, hipStream_t stream);
+int agxOrbitKernelRKNBlocksPerCU(int precision, bool nfw);
 hipError_t agxLaunchFillU32(unsigned int* p, long n, unsigned int v, hipStream_t stream);
 int agxOrbitKernelBlockSize();
 size_t agxScheduleWorkspace(long n);
@@ -174,20 +177,29 @@ agx_potential* createFromParams(const std::string& text)
         P.num("rmin", 0), P.num("rmax", 0)), false, nfwMass, nfwScale);
 }
 
-int orbitGeometry(long n, int precision, bool nfw, int& blocks, int& bs)
+int orbitGeometry(long n, int precision, bool nfw, int& blocks, int& bs, bool rkn = false)
 {
-    bs = agxOrbitKernelBlockSize();
+    bs = agxOrbitKernelBlockSize();   // the DOP853 and DPRKN8 kernels share the block size
     if(g_orbitBlocks > 0) blocks = g_orbitBlocks;
     else {
         int dev = 0, cus = 0;
         HIPCHECK(hipGetDevice(&dev));
         HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        const int perCU = std::max(1, agxOrbitKernelBlocksPerCU(precision, g_fetchOrbit, g_orbitWaves, nfw));
+        const int perCU = std::max(1, rkn ? agxOrbitKernelRKNBlocksPerCU(precision, nfw) :
+            agxOrbitKernelBlocksPerCU(precision, g_fetchOrbit, g_orbitWaves, nfw));
         blocks = cus * perCU;
     }
     const long need = (n + bs - 1) / bs;
     if(need < blocks) blocks = (int)std::max<long>(need, 1);
     return 0;
+}
+
+// the persistent orbit kernel of the method named in flags (DOP853 by default)
+hipError_t launchOrbitMethod(const agx_potential* pot, int precision, int flags, const AgxOrbitArgs& A,
+    int blocks, int bs, hipStream_t st)
+{
+    if(flags & AGX_METHOD_DPRKN8) return agxLaunchOrbitRKN(pot->dev, pot->ddev, precision, A, blocks, st);
+    return agxLaunchOrbit(pot->dev, pot->ddev, precision, g_fetchOrbit, g_orbitWaves, A, blocks, bs, st);
 }
 
 template<typename T>
@@ -398,7 +410,7 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
     if(maxNumSteps <= 0) return fail(AGX_EINVAL, "maxNumSteps must be positive");
     const int precision = flags & AGX_PRECISION_MASK;
     if((precision != AGX_PRECISION_FP64 && precision != AGX_PRECISION_FP32) ||
-        (flags & ~(AGX_PRECISION_MASK | AGX_ORDER_AS_GIVEN)))
+        (flags & ~(AGX_PRECISION_MASK | AGX_ORDER_AS_GIVEN | AGX_METHOD_DPRKN8)))
         return fail(AGX_EINVAL, "invalid flags");
     try {
         hipStream_t st = (hipStream_t)stream;
@@ -423,9 +435,9 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
         A.outNsteps = d_nsteps; A.outNattempts = d_nattempts; A.outNevals = d_nevals;
         A.outMaxde = d_maxde; A.outStatus = d_status; A.counter = pot->dcounter;
         int blocks, bs;
-        orbitGeometry(n, precision, pot->nfwMass != 0, blocks, bs);
+        orbitGeometry(n, precision, pot->nfwMass != 0, blocks, bs, flags & AGX_METHOD_DPRKN8);
         HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
-        hipError_t e = agxLaunchOrbit(pot->dev, pot->ddev, precision, g_fetchOrbit, g_orbitWaves, A, blocks, bs, st);
+        hipError_t e = launchOrbitMethod(pot, precision, flags, A, blocks, bs, st);
         if(work) HIPCHECK(hipFreeAsync(work, st));
         if(e != hipSuccess) return fail(AGX_EDEVICE, std::string("orbit launch: ") + hipGetErrorString(e));
         return AGX_OK;
@@ -464,7 +476,7 @@ agx_orbit_graph* agx_orbit_graph_create(const agx_potential* pot, long n, const 
     }
     const int precision = flags & AGX_PRECISION_MASK;
     if((precision != AGX_PRECISION_FP64 && precision != AGX_PRECISION_FP32) ||
-        (flags & ~(AGX_PRECISION_MASK | AGX_ORDER_AS_GIVEN))) { fail(AGX_EINVAL, "invalid flags"); return nullptr; }
+        (flags & ~(AGX_PRECISION_MASK | AGX_ORDER_AS_GIVEN | AGX_METHOD_DPRKN8))) { fail(AGX_EINVAL, "invalid flags"); return nullptr; }
     try {
         std::unique_ptr<agx_orbit_graph> G(new agx_orbit_graph);
         HIPCHECK(hipSetDevice(pot->device));
@@ -487,13 +499,13 @@ agx_orbit_graph* agx_orbit_graph_create(const agx_potential* pot, long n, const 
         A.outNsteps = d_nsteps; A.outNattempts = d_nattempts; A.outNevals = d_nevals;
         A.outMaxde = d_maxde; A.outStatus = d_status; A.counter = counter;
         int blocks, bs;
-        orbitGeometry(n, precision, pot->nfwMass != 0, blocks, bs);
+        orbitGeometry(n, precision, pot->nfwMass != 0, blocks, bs, flags & AGX_METHOD_DPRKN8);
         hipStream_t st;
         HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         std::unique_ptr<void, void(*)(void*)> sg((void*)st, [](void* q) { (void)hipStreamDestroy((hipStream_t)q); });
         HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipError_t e = agxLaunchFillU32(counter, 1, 0u, st);
-        if(e == hipSuccess) e = agxLaunchOrbit(pot->dev, pot->ddev, precision, g_fetchOrbit, g_orbitWaves, A, blocks, bs, st);
+        if(e == hipSuccess) e = launchOrbitMethod(pot, precision, flags, A, blocks, bs, st);
         HIPCHECK(hipStreamEndCapture(st, &G->graph));
         HIPCHECK(e);
         HIPCHECK(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));

@@ -85,6 +85,7 @@ template<int STRIDE, bool KLDS = false>
 struct LaneT {
     typedef typename std::conditional<KLDS, LdsVec<STRIDE>, double[6]>::type Vec;
     static constexpr int LDS_FIELDS = KLDS ? F_COUNT_K : F_COUNT;
+    static constexpr int EVALS_PER_ATTEMPT = 12;
     double* sh;                // this lane's LDS column
     int oi, phase, nbad, S, nsteps, natt, actual;
     Vec x, k1, k2, k3, k4, k5;
@@ -207,7 +208,7 @@ __device__ __forceinline__ void finishOrbit(const AgxOrbitArgs& A, Lane& L, int 
     }
     if(A.outNsteps) A.outNsteps[oi] = L.nsteps;
     if(A.outNattempts) A.outNattempts[oi] = L.natt;
-    if(A.outNevals) A.outNevals[oi] = 3 + 12 * (long)L.natt;   // initialisation + 12 per attempt
+    if(A.outNevals) A.outNevals[oi] = 3 + Lane::EVALS_PER_ATTEMPT * (long)L.natt;   // initialisation + attempts
     if(A.outMaxde) A.outMaxde[oi] = L.f(F_MAXDE);
     if(A.outStatus) A.outStatus[oi] = status;
     L.phase = PH_IDLE;

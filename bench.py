@@ -40,6 +40,11 @@ METRIC = "orbit-steps/s at 1M orbits: MLP surrogate vs FP64 Multipole at Î”E/Eâ‰
 #   per orbit: 3 initial evaluations + first-step estimate ~80
 FLOPS_PER_EVAL = 1531
 FLOPS_PER_ATTEMPT_EXTRA = 978
+# DPRKN8 (--method dprkn8): 9 evaluations per attempt; besides them the position-only stage
+#   sums 324, x/v b-sums 96, error norm and PI control 190, energy bookkeeping 10     -> 620
+EVALS_PER_ATTEMPT = {"dop853": 12, "dprkn8": 9}
+FLOPS_PER_ATTEMPT_EXTRA_BY_METHOD = {"dop853": FLOPS_PER_ATTEMPT_EXTRA, "dprkn8": 620}
+METHOD_FLAG = {"dop853": 0, "dprkn8": 0x200}
 FLOPS_PER_ORBIT_INIT = 3 * FLOPS_PER_EVAL + 80
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix), AMD published figure
 # HBM bytes per launch of the orbit kernel on this workload, from rocprofv3 FETCH_SIZE and
@@ -62,6 +67,8 @@ def parse():
                          "eval: potential + force at 1M points drawn from the density; mlp: the "
                          "surrogate's fused MFMA evaluation at 1M points; copy: one 1 GiB "
                          "device-to-device copy, the reference for HBM byte counters")
+    ap.add_argument("--method", choices=["dop853", "dprkn8"], default="dop853",
+                    help="integrator of the FP64 Multipole path (the surrogate path uses DOP853)")
     ap.add_argument("--train-steps", type=int, default=6000)
     ap.add_argument("--train-samples", type=float, default=20e6)
     ap.add_argument("--max-rel-de", type=float, default=1e-6)
@@ -181,7 +188,8 @@ def main():
         rc = lib.agx_orbit_batch_device(pot.handle, n, d_ic.data_ptr(), None, d_T.data_ptr(), d_sz.data_ptr(),
                                         d_off.data_ptr(), None, 1e-8, 0.0, int(1e8), d_time.data_ptr(), None,
                                         d_fin.data_ptr(), d_ns.data_ptr(), d_na.data_ptr(), d_ne.data_ptr(),
-                                        d_de.data_ptr(), d_st.data_ptr(), prec, stream.cuda_stream)
+                                        d_de.data_ptr(), d_st.data_ptr(), prec | METHOD_FLAG[args.method],
+                                        stream.cuda_stream)
         if rc:
             raise RuntimeError(agama_amd._native.last_error())
 
@@ -204,14 +212,15 @@ def main():
         print(f"{label}: {args.steps} timed step(s) done in {wall:.2f} s", file=sys.stderr, flush=True)
         return wall, ev0.elapsed_time(ev1) / 1e3
 
-    wall, kernel_s = timed(launch_fp64, "FP64 Multipole")
+    wall, kernel_s = timed(launch_fp64, f"FP64 Multipole ({args.method})")
     ns = d_ns.cpu().numpy()
     na = d_na.cpu().numpy()
     de = d_de.cpu().numpy()
     st = d_st.cpu().numpy()
     steps_per_launch = int(ns.sum())
     attempts = int(na.sum())
-    flops = attempts * (12 * FLOPS_PER_EVAL + FLOPS_PER_ATTEMPT_EXTRA) + n * FLOPS_PER_ORBIT_INIT
+    flops = attempts * (EVALS_PER_ATTEMPT[args.method] * FLOPS_PER_EVAL +
+                        FLOPS_PER_ATTEMPT_EXTRA_BY_METHOD[args.method]) + n * FLOPS_PER_ORBIT_INIT
     per_launch = kernel_s / args.steps
     fp64_value = steps_per_launch * args.steps / wall
     achieved = flops / per_launch / 1e12
@@ -223,7 +232,7 @@ def main():
 
     surrogate = None
     value = fp64_value
-    if args.mode == "orbit" and args.precision == "fp64":
+    if args.mode == "orbit" and args.precision == "fp64" and args.method == "dop853":
         t0 = time.perf_counter()
         sur = agama_amd.Surrogate.train(pot, samples=args.train_samples, steps=args.train_steps)
         train_s = time.perf_counter() - t0
@@ -278,7 +287,7 @@ def main():
         fast = pyoracle.OraclePotential(coef, library=pyoracle.fast())
         m = min(args.cpu_orbits, n)
         t0 = time.perf_counter()
-        r = fast.orbits(ic[:m], T[:m], trajsize=1, accuracy=1e-8, nthreads=threads)
+        r = fast.orbits(ic[:m], T[:m], trajsize=1, accuracy=1e-8, nthreads=threads, method=args.method)
         dt = time.perf_counter() - t0
         cpu = {"value": float(r["nsteps"].sum() / dt), "unit": "orbit-steps/s", "cores": threads,
                "kind": "port",
@@ -302,14 +311,16 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f64 (fp32 spline tables)",
         "data": "synthetic",
         "config": {"workload": "triaxial Dehnen Multipole lmax=8, 1M orbits, log-uniform 1-300 circular "
-                               "periods, DOP853 accuracy 1e-8, final state + max dE/E <= 1e-6",
+                               f"periods, {args.method.upper()} accuracy 1e-8, final state + max dE/E <= 1e-6",
                    "orbits": n, "accepted_steps_per_step": steps_per_launch,
                    "attempts_per_step": attempts, "precision": args.precision,
                    "path": "MLP surrogate with FP64 fallback" if surrogate else "FP64 Multipole"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": ORBIT_TRAFFIC_BYTES if (n == 1_000_000 and args.precision == "fp64") else None,
-                     "kernel": "orbitKernel (FP64)", "kernel_ms": per_launch * 1e3},
+                     "traffic": ORBIT_TRAFFIC_BYTES if (n == 1_000_000 and args.precision == "fp64"
+                                                        and args.method == "dop853") else None,
+                     "kernel": "orbitKernel (FP64)" if args.method == "dop853" else "orbitKernelRKN (FP64)",
+                     "kernel_ms": per_launch * 1e3},
         "cpu_baseline": cpu,
     }
     if surrogate:

@@ -41,6 +41,11 @@ typedef struct agx_potential agx_potential;
    the batch longest-expected-integration first (|T| / dynamical time at the start point);
    AGX_ORDER_AS_GIVEN keeps the input order (or the caller's d_order). */
 #define AGX_ORDER_AS_GIVEN 0x100
+/* ... and the integration method (OrbitIntParams::Method, src/orbit.h:116-129): DOP853 by
+   default, or the 8th-order Runge-Kutta-Nystrom method DPRKN8 (src/math_ode.cpp:426-701),
+   which evaluates positions only (9 evaluations per step attempt instead of 12).  The DPRKN8
+   kernel gathers FP64 node records with 16-byte loads whatever agx_set_fetch chose. */
+#define AGX_METHOD_DPRKN8 0x200
 
 /* Message of the last failed call on this thread ("" if none).
    Replaces agama_getError (src/interface_c.cpp:15-19), which is a process-global string. */
@@ -99,7 +104,7 @@ int agx_eval_many(const agx_potential* pot, long n, const double* pos, double t,
 int agx_eval_many_device(const agx_potential* pot, long n, const double* d_pos, double t,
     double* d_phi, double* d_grad, int flags, void* stream);
 
-/* Batch orbit integration with DOP853 in the given potential, restating agama.orbit
+/* Batch orbit integration with DOP853 (or DPRKN8, flags) in the given potential, restating agama.orbit
    (src/interface_python.cpp:7689-8281) with dtype=float64 and separateTime output:
      ic[n*6], tstart[n] (NULL: 0), ttotal[n], trajsize[n] (>=1) on the host;
      out_time[rows], out_traj[rows*6], rows = sum(trajsize), orbit i at row offset
