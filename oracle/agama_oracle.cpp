@@ -52,6 +52,7 @@ const double ROOT3_DBL_EPS = 6.0554544523933429e-06;
 const double SQRTPI = 1.77245385090551602729816748334;
 
 template<typename T> inline T pow_2(T x) { return x*x; }
+template<typename T> inline T pow_3(T x) { return x*x*x; }
 inline bool isFinite(double x) { return x>-INFINITY && x<INFINITY; }
 inline double signum(double x) { return x>0 ? 1 : x<0 ? -1 : 0; }
 
@@ -464,7 +465,8 @@ struct QuinticSpline2d {
         }
     }
 
-    void evalDeriv(const double x, const double y, double* z, double* z_x, double* z_y) const
+    void evalDeriv(const double x, const double y, double* z, double* z_x, double* z_y,
+        double* z_xx = NULL, double* z_xy = NULL, double* z_yy = NULL) const
     {
         const long nx = xval.size(), ny = yval.size(),
             xi = binSearch(x, &xval.front(), nx),
@@ -474,6 +476,9 @@ struct QuinticSpline2d {
             if(z) *z = NAN;
             if(z_x) *z_x = NAN;
             if(z_y) *z_y = NAN;
+            if(z_xx) *z_xx = NAN;
+            if(z_xy) *z_xy = NAN;
+            if(z_yy) *z_yy = NAN;
             return;
         }
         const double
@@ -489,15 +494,20 @@ struct QuinticSpline2d {
             f1u[6] = { fy  [ilu], fy  [iuu], fxy [ilu], fxy [iuu], fxxy [ilu], fxxy [iuu] },
             f2l[6] = { fyy [ill], fyy [iul], fxyy[ill], fxyy[iul], fxxyy[ill], fxxyy[iul] },
             f2u[6] = { fyy [ilu], fyy [iuu], fxyy[ilu], fxyy[iuu], fxxyy[ilu], fxxyy[iuu] };
-        bool der = z_y!=NULL;
-        double F[6], dF[6];
-        evalQuinticSplines<6>(y, ylow, yupp, fl, fu, f1l, f1u, f2l, f2u, F, der? dF : NULL, NULL);
-        evalQuinticSplines<1>(x, xlow, xupp, &F[0], &F[1], &F[2], &F[3], &F[4], &F[5], z, z_x, NULL);
+        bool der = z_y!=NULL || z_xy!=NULL;
+        bool der2 = z_yy!=NULL;
+        double F[6], dF[6], d2F[6];
+        evalQuinticSplines<6>(y, ylow, yupp, fl, fu, f1l, f1u, f2l, f2u, F, der? dF : NULL,
+            der2? d2F : NULL);
+        evalQuinticSplines<1>(x, xlow, xupp, &F[0], &F[1], &F[2], &F[3], &F[4], &F[5], z, z_x, z_xx);
         if(z)
             *z += f_offset;
-        if(z_y)
+        if(z_y || z_xy)
             evalQuinticSplines<1>(x, xlow, xupp, &dF[0], &dF[1], &dF[2], &dF[3], &dF[4], &dF[5],
-                z_y, NULL, NULL);
+                z_y, z_xy, NULL);
+        if(z_yy)
+            evalQuinticSplines<1>(x, xlow, xupp, &d2F[0], &d2F[1], &d2F[2], &d2F[3], &d2F[4], &d2F[5],
+                z_yy, NULL, NULL);
     }
 };
 
@@ -580,18 +590,20 @@ SphHarmIndices getIndicesFromCoefs2(const std::vector<std::vector<double> >& P,
 double factorial_d(int n) { double r=1; for(int i=2;i<=n;i++) r*=i; return r; }
 double dfactorial_d(int n) { double r=1; for(int i=n;i>1;i-=2) r*=i; return r; }
 
-// math_sphharm.cpp:22-61  normalised P_m^m and its theta-derivatives
+// math_sphharm.cpp:22-61  normalised P_m^m and its first and second theta-derivatives
 inline void legendrePmm(int m, double costheta, double sintheta,
-    double& prefact, double* value, double* der)
+    double& prefact, double* value, double* der, double* der2 = NULL)
 {
     prefact = 0.5/SQRTPI * sqrt( (2*m+1) / factorial_d(2*m) );
     if(m == 0) {
         if(der) *der = 0;
+        if(der2) *der2 = 0;
         *value = prefact;
         return;
     }
     if(m == 1) {
         if(der) *der = -costheta * prefact;
+        if(der2) *der2 = sintheta * prefact;
         *value = -sintheta * prefact;
         return;
     }
@@ -599,27 +611,33 @@ inline void legendrePmm(int m, double costheta, double sintheta,
     double sinm2 = ipow(sintheta, m-2);
     if(der)
         *der = m * coef * sinm2 * sintheta * costheta;
+    if(der2)
+        *der2 = m * coef * sinm2 * (m * pow_2(costheta) - 1);
     *value = coef * sinm2 * pow_2(sintheta);
 }
 
-// math_sphharm.cpp:63-132 (values and first derivatives)
+// math_sphharm.cpp:63-132 (values, first and optionally second derivatives)
 void sphHarmArray(const unsigned int lmax, const unsigned int m, const double tau,
-    double* resultArray, double* derivArray)
+    double* resultArray, double* derivArray, double* deriv2Array = NULL)
 {
-    if(m>lmax || resultArray==NULL)
+    if(m>lmax || resultArray==NULL || (deriv2Array!=NULL && derivArray==NULL))
         throw std::domain_error("Invalid parameters in sphHarmArray");
     if(lmax==0) {
         resultArray[0] = 0.5/SQRTPI;
         if(derivArray) derivArray[0] = 0;
+        if(deriv2Array) deriv2Array[0] = 0;
         return;
     }
     const double ct =      2 * tau  / (1 + tau*tau);
     const double st = (1 - tau*tau) / (1 + tau*tau);
     double prefact;
-    legendrePmm(m, ct, st, prefact, resultArray, derivArray);
+    legendrePmm(m, ct, st, prefact, resultArray, derivArray, deriv2Array);
     if(lmax == m)
         return;
     double Plm1 = resultArray[0] / prefact, Plm = ct * (2*m+1) * Plm1, Plm2 = 0;
+    // second derivatives of the un-normalised m=1 functions near the poles come from their
+    // own recurrence (the general expression cancels there)
+    double d2Plm1 = st, d2Plm2 = 0, d2Plm = 12 * ct * st;
     const double EPS = 1e-8;
     for(int l=m+1; l<=(int)lmax; l++) {
         unsigned int ind = l-m;
@@ -638,6 +656,25 @@ void sphHarmArray(const unsigned int lmax, const unsigned int m, const double ta
             else if(m==2)
                 dPlm = l*(l+1)*(l+2)*(l-1)/4 * st * (ct>0 || l%2==1 ? 1 : -1);
             derivArray[ind] = prefact * dPlm;
+        }
+        if(deriv2Array!=NULL) {
+            if(st >= EPS || (m>2 && st>0))
+                deriv2Array[ind] = ct * derivArray[ind] / (-st) - (l*(l+1)-pow_2(m/st)) * resultArray[ind];
+            else if(m==0)
+                deriv2Array[ind] = -l*(l+1)/2 * prefact * (ct>0 || l%2==0 ? 1 : -1);
+            else if(m==1) {
+                if(l>(int)m+1) {
+                    double twodPlm1 = -l*(l-1) * (ct>0 || l%2==1 ? 1 : -1);
+                    d2Plm = ( (2*l-1) * (ct * (d2Plm1 - Plm1) - st * twodPlm1) - l * d2Plm2) / (l-1);
+                }
+                deriv2Array[ind] = prefact * d2Plm;
+                d2Plm2 = d2Plm1;
+                d2Plm1 = d2Plm;
+            }
+            else if(m==2)
+                deriv2Array[ind] = l*(l+1)*(l+2)*(l-1)/4 * prefact * (ct>0 || l%2==0 ? 1 : -1);
+            else
+                deriv2Array[ind] = 0;
         }
         Plm2 = Plm1;
         Plm1 = Plm;
@@ -666,20 +703,32 @@ void trigMultiAngle(const double phi, const unsigned int m, const bool needSine,
 
 struct GradSph { double dr, dtheta, dphi; };
 struct GradCyl { double dR, dz, dphi; };
+// coord.h: second derivatives in spherical (or the (ln r, tau) stand-in), cylindrical and
+// Cartesian coordinates
+struct HessSph { double dr2, dtheta2, dphi2, drdtheta, drdphi, dthetadphi; };
+struct HessCyl { double dR2, dz2, dphi2, dRdz, dzdphi, dRdphi; };
+struct HessCar { double dx2, dy2, dz2, dxdy, dydz, dxdz; };
 
-// potential_multipole.cpp:666-717 (value and gradient)
+// potential_multipole.cpp:666-717 (value, gradient and Hessian; C_m holds 1, 3 or 6
+// quantities per harmonic: Phi, d/dr, d/dtheta, d2/dr2, d2/drdtheta, d2/dtheta2)
 void fourierTransformAzimuth(const SphHarmIndices& ind, const double phi,
-    const double C_m[], double *val, GradSph *grad)
+    const double C_m[], double *val, GradSph *grad, HessSph *hess = NULL)
 {
     const int mmin = ind.mmin();
     const int nm = ind.mmax - mmin + 1;
-    const int numQuantities = grad!=NULL ? 3 : 1;
+    const int numQuantities = hess!=NULL ? 6 : grad!=NULL ? 3 : 1;
     if(val)
         *val = C_m[-mmin];
     if(grad) {
         grad->dr     = C_m[-mmin+nm];
         grad->dtheta = C_m[-mmin+nm*2];
         grad->dphi   = 0;
+    }
+    if(hess) {
+        hess->dr2      = C_m[-mmin+nm*3];
+        hess->drdtheta = C_m[-mmin+nm*4];
+        hess->dtheta2  = C_m[-mmin+nm*5];
+        hess->drdphi = hess->dthetadphi = hess->dphi2 = 0;
     }
     if(ind.mmax == 0)
         return;
@@ -694,6 +743,7 @@ void fourierTransformAzimuth(const SphHarmIndices& ind, const double phi,
             continue;
         double trig  = m>0 ? trig_m[m-1] : trig_m[ind.mmax-m-1];
         double dtrig = m>0 ? -m*trig_m[ind.mmax+m-1] : -m*trig_m[-m-1];
+        double d2trig = -m*m*trig;
         if(val)
             *val += C_m[mm] * trig;
         if(grad) {
@@ -701,14 +751,23 @@ void fourierTransformAzimuth(const SphHarmIndices& ind, const double phi,
             grad->dtheta += C_m[mm+nm*2] *  trig;
             grad->dphi   += C_m[mm]      * dtrig;
         }
+        if(hess) {
+            hess->dr2        += C_m[mm+nm*3] *   trig;
+            hess->drdtheta   += C_m[mm+nm*4] *   trig;
+            hess->dtheta2    += C_m[mm+nm*5] *   trig;
+            hess->drdphi     += C_m[mm+nm  ] *  dtrig;
+            hess->dthetadphi += C_m[mm+nm*2] *  dtrig;
+            hess->dphi2      += C_m[mm]      * d2trig;
+        }
     }
 }
 
 struct PosCyl { double R, z, phi; };
 
-// potential_multipole.cpp:722-772 (value and gradient; lmax=2, mmin=0, step=2 special case)
+// potential_multipole.cpp:722-772 (value, gradient, Hessian; lmax=2, mmin=0, step=2 special case)
 void sphHarmTransformInverseDeriv2(const SphHarmIndices& ind, const PosCyl& pos,
-    const double C_lm[], const double dC_lm[], double *val, GradSph *grad)
+    const double C_lm[], const double dC_lm[], const double d2C_lm[], double *val, GradSph *grad,
+    HessSph *hess)
 {
     const int i00 = SphHarmIndices::index(0,0), i20 = SphHarmIndices::index(2,0),
         i22 = SphHarmIndices::index(2,2);
@@ -719,7 +778,8 @@ void sphHarmTransformInverseDeriv2(const SphHarmIndices& ind, const PosCyl& pos,
     st  = (1 - tau*tau) / (1 + tau*tau),
     cc  = ct * ct, cs = ct * st, ss = st * st,
     Y20  = (3*C2 * cc - C2),
-    dY20 = -6*C2 * cs;
+    dY20 = -6*C2 * cs,
+    d2Y20 = -12*C2 * cc + 6*C2;
     if(val)
         *val = Y20 * C_lm[i20] + C_lm[i00];
     if(grad) {
@@ -727,8 +787,14 @@ void sphHarmTransformInverseDeriv2(const SphHarmIndices& ind, const PosCyl& pos,
         grad->dtheta = dY20 * C_lm[i20];
         grad->dphi   = 0;
     }
+    if(hess) {
+        hess->dr2      =   Y20 * d2C_lm[i20] + d2C_lm[i00];
+        hess->drdtheta =  dY20 *  dC_lm[i20];
+        hess->dtheta2  = d2Y20 *   C_lm[i20];
+        hess->drdphi = hess->dthetadphi = hess->dphi2 = 0;
+    }
     if(ind.mmax == 2) {
-        double sp, cp, Y22 = D2 * ss, dY22 = 2*D2 * cs;
+        double sp, cp, Y22 = D2 * ss, dY22 = 2*D2 * cs, d2Y22 = 4*D2 * cc - 2*D2;
         sincos_r(2*pos.phi, sp, cp);
         if(val)
             *val += Y22 * C_lm[i22] * cp;
@@ -737,23 +803,33 @@ void sphHarmTransformInverseDeriv2(const SphHarmIndices& ind, const PosCyl& pos,
             grad->dtheta += dY22 *  C_lm[i22] * cp;
             grad->dphi   +=  Y22 *  C_lm[i22] * -2*sp;
         }
+        if(hess) {
+            hess->dr2        +=   Y22 * d2C_lm[i22] *    cp;
+            hess->drdtheta   +=  dY22 *  dC_lm[i22] *    cp;
+            hess->dtheta2    += d2Y22 *   C_lm[i22] *    cp;
+            hess->drdphi     +=   Y22 *  dC_lm[i22] * -2*sp;
+            hess->dthetadphi +=  dY22 *   C_lm[i22] * -2*sp;
+            hess->dphi2      +=   Y22 *   C_lm[i22] * -4*cp;
+        }
     }
 }
 
-// potential_multipole.cpp:778-821 (value and gradient)
+// potential_multipole.cpp:778-821 (value, gradient and Hessian)
 void sphHarmTransformInverseDeriv(const SphHarmIndices& ind, const PosCyl& pos,
-    const double C_lm[], const double dC_lm[], double *val, GradSph *grad)
+    const double C_lm[], const double dC_lm[], const double d2C_lm[], double *val, GradSph *grad,
+    HessSph *hess)
 {
     if(ind.lmax==2 && ind.mmin()==0 && ind.step==2) {
-        sphHarmTransformInverseDeriv2(ind, pos, C_lm, dC_lm, val, grad);
+        sphHarmTransformInverseDeriv2(ind, pos, C_lm, dC_lm, d2C_lm, val, grad, hess);
         return;
     }
-    const int numQuantities = grad!=NULL ? 3 : 1;
+    const int numQuantities = hess!=NULL ? 6 : grad!=NULL ? 3 : 1;
     int sizeC = 6 * (2*ind.mmax+1), sizeP = ind.lmax+1;
     std::vector<double> buf(sizeC + 3*sizeP);
     double* C_m = &buf[0];
     double* P_lm = C_m + sizeC;
     double* dP_lm = numQuantities>=3 ? P_lm + sizeP : NULL;
+    double* d2P_lm = numQuantities==6 ? P_lm + sizeP*2 : NULL;
     const double tau = pos.z == 0 ? 0 : pos.z / (sqrt(pow_2(pos.R) + pow_2(pos.z)) + pos.R);
     const int nm = ind.mmax - ind.mmin() + 1;
     for(int mm=0; mm<nm; mm++) {
@@ -765,7 +841,7 @@ void sphHarmTransformInverseDeriv(const SphHarmIndices& ind, const PosCyl& pos,
         for(int q=0; q<numQuantities; q++)
             C_m[mm + q*nm] = 0;
         int absm = abs(m);
-        sphHarmArray(ind.lmax, absm, tau, P_lm, dP_lm);
+        sphHarmArray(ind.lmax, absm, tau, P_lm, dP_lm, d2P_lm);
         for(int l=lmin; l<=ind.lmax; l+=ind.step) {
             unsigned int c = SphHarmIndices::index(l, m), p = l-absm;
             C_m[mm] += P_lm[p] * C_lm[c] * mul;
@@ -773,13 +849,43 @@ void sphHarmTransformInverseDeriv(const SphHarmIndices& ind, const PosCyl& pos,
                 C_m[mm + nm  ] +=  P_lm[p] * dC_lm[c] * mul;
                 C_m[mm + nm*2] += dP_lm[p] *  C_lm[c] * mul;
             }
+            if(numQuantities==6) {
+                C_m[mm + nm*3] +=   P_lm[p] * d2C_lm[c] * mul;
+                C_m[mm + nm*4] +=  dP_lm[p] *  dC_lm[c] * mul;
+                C_m[mm + nm*5] += d2P_lm[p] *   C_lm[c] * mul;
+            }
         }
     }
-    fourierTransformAzimuth(ind, pos.phi, C_m, val, grad);
+    fourierTransformAzimuth(ind, pos.phi, C_m, val, grad, hess);
 }
 
-// potential_multipole.cpp:824-848 (gradient part)
-void transformDerivsSphToCyl(const PosCyl& pos, const GradSph& gradSph, GradCyl* gradCyl)
+// coord.cpp:779-799 (toHess<Sph,Cyl>), written out for the (ln r, theta) stand-in coordinates
+HessCyl toHessSphCyl(const GradSph& g, const HessSph& h, double drdR, double drdz, double dthetadR,
+    double dthetadz, double d2rdR2, double d2rdRdz, double d2rdz2, double d2thetadR2,
+    double d2thetadRdz, double d2thetadz2)
+{
+    HessCyl d;
+    d.dR2 =
+        (h.dr2     *drdR + h.drdtheta*dthetadR) * drdR +
+        (h.drdtheta*drdR + h.dtheta2 *dthetadR) * dthetadR +
+        g.dr*d2rdR2 + g.dtheta*d2thetadR2;
+    d.dRdz =
+        (h.dr2     *drdz + h.drdtheta*dthetadz) * drdR +
+        (h.drdtheta*drdz + h.dtheta2 *dthetadz) * dthetadR +
+        g.dr*d2rdRdz + g.dtheta*d2thetadRdz;
+    d.dz2 =
+        (h.dr2     *drdz + h.drdtheta*dthetadz) * drdz +
+        (h.drdtheta*drdz + h.dtheta2 *dthetadz) * dthetadz +
+        g.dr*d2rdz2 + g.dtheta*d2thetadz2;
+    d.dRdphi = h.drdphi*drdR + h.dthetadphi*dthetadR;
+    d.dzdphi = h.drdphi*drdz + h.dthetadphi*dthetadz;
+    d.dphi2  = h.dphi2;
+    return d;
+}
+
+// potential_multipole.cpp:824-848
+void transformDerivsSphToCyl(const PosCyl& pos, const GradSph& gradSph, GradCyl* gradCyl,
+    const HessSph* hessSph = NULL, HessCyl* hessCyl = NULL)
 {
     const double r2inv = 1 / (pow_2(pos.R) + pow_2(pos.z));
     double drdR = pos.R * r2inv, drdz = pos.z * r2inv;
@@ -787,6 +893,12 @@ void transformDerivsSphToCyl(const PosCyl& pos, const GradSph& gradSph, GradCyl*
     gradCyl->dR = gradSph.dr*drdR + gradSph.dtheta*dthetadR;
     gradCyl->dz = gradSph.dr*drdz + gradSph.dtheta*dthetadz;
     gradCyl->dphi = gradSph.dphi;
+    if(hessCyl) {
+        const double d2rdR2 = pow_2(drdz) - pow_2(drdR), d2rdRdz = -2 * drdR * drdz,
+            d2rdz2 = -d2rdR2, d2thetadR2 = d2rdRdz, d2thetadRdz = -d2rdR2, d2thetadz2 = -d2rdRdz;
+        *hessCyl = toHessSphCyl(gradSph, *hessSph, drdR, drdz, dthetadR, dthetadz,
+            d2rdR2, d2rdRdz, d2rdz2, d2thetadR2, d2thetadRdz, d2thetadz2);
+    }
 }
 
 // potential_multipole.cpp:526-539
@@ -861,10 +973,12 @@ struct PowerLawMultipole {
     std::vector<double> S, U, W;
     double Q;
 
-    void evalCyl(const PosCyl& pos, double* potential, GradCyl* grad) const
+    // potential_multipole.cpp:1600-1658; a Hessian request needs grad as well
+    void evalCyl(const PosCyl& pos, double* potential, GradCyl* grad, HessCyl* hess = NULL) const
     {
         bool needGrad = grad!=NULL;
-        std::vector<double> Phi_lm(ind.size()), dPhi_lm(ind.size());
+        bool needHess = hess!=NULL;
+        std::vector<double> Phi_lm(ind.size()), dPhi_lm(ind.size()), d2Phi_lm(ind.size());
         double rsq   = pow_2(pos.R) + pow_2(pos.z);
         double dlogr = log(rsq / r0sq) * 0.5;
         int lmax = (inner && rsq < r0sq*1e-16) || (!inner && rsq > r0sq*1e16) ? 0 : ind.lmax;
@@ -880,6 +994,8 @@ struct PowerLawMultipole {
                 Phi_lm[c] = urs + wrv + qr2;
                 if(needGrad)
                     dPhi_lm[c] = urs*s + wrv*v + (s!=v ? 0 : u*rs) + qr2*2;
+                if(needHess)
+                    d2Phi_lm[c] = urs*s*s + wrv*v*v + (s!=v ? 0 : 2*s*u*rs) + qr2*4;
             }
         if(lmax == 0) {
             if(potential)
@@ -892,13 +1008,21 @@ struct PowerLawMultipole {
                 grad->dz = dPhi_lm[0] * zr2;
                 grad->dphi = 0;
             }
+            if(hess) {
+                double d2 = d2Phi_lm[0] - 2 * dPhi_lm[0];
+                hess->dR2 = d2 * pow_2(Rr2) + dPhi_lm[0] * rsqinv;
+                hess->dz2 = d2 * pow_2(zr2) + dPhi_lm[0] * rsqinv;
+                hess->dRdz= d2 * Rr2 * zr2;
+                hess->dRdphi = hess->dzdphi = hess->dphi2 = 0;
+            }
             return;
         }
         GradSph gradSph;
-        sphHarmTransformInverseDeriv(ind, pos, &Phi_lm[0], &dPhi_lm[0], potential,
-            needGrad ? &gradSph : NULL);
+        HessSph hessSph;
+        sphHarmTransformInverseDeriv(ind, pos, &Phi_lm[0], &dPhi_lm[0], &d2Phi_lm[0], potential,
+            needGrad ? &gradSph : NULL, needHess ? &hessSph : NULL);
         if(needGrad)
-            transformDerivsSphToCyl(pos, gradSph, grad);
+            transformDerivsSphToCyl(pos, gradSph, grad, needHess ? &hessSph : NULL, hess);
     }
 };
 
@@ -1030,7 +1154,8 @@ struct MultipoleInterp2d {
         }
     }
 
-    void evalCyl(const PosCyl& pos, double* potential, GradCyl* grad) const
+    // potential_multipole.cpp:1969-2080; a Hessian request needs grad as well
+    void evalCyl(const PosCyl& pos, double* potential, GradCyl* grad, HessCyl* hess = NULL) const
     {
         const double
             r         = sqrt(pow_2(pos.R) + pow_2(pos.z)),
@@ -1038,22 +1163,32 @@ struct MultipoleInterp2d {
             rplusRinv = 1. / (r + fabs(pos.R)),
             tau       = pos.R==0 ? signum(pos.z) : pos.z * rplusRinv;
         const int mmin = ind.mmin(), nm = ind.mmax - mmin + 1;
-        const int numQuantities = grad!=NULL ? 3 : 1;
-        double C_m[3*64];
-        double *Phi = C_m, *dlnr = C_m+nm, *dtau = C_m+nm*2;
+        const int numQuantities = hess!=NULL ? 6 : grad!=NULL ? 3 : 1;
+        double C_m[6*64];
+        double *Phi = C_m, *dlnr = C_m+nm, *dtau = C_m+nm*2,
+            *dlnr2 = C_m+nm*3, *dlnrdtau = C_m+nm*4, *dtau2 = C_m+nm*5;
         for(int mm=0; mm<nm; mm++) {
             int m = mm + mmin;
             if(ind.lmin(m) > ind.lmax)
                 continue;
             spl[m+ind.mmax].evalDeriv(logr, tau, &Phi[mm],
                 numQuantities>=3 ? &dlnr[mm] : NULL,
-                numQuantities>=3 ? &dtau[mm] : NULL);
+                numQuantities>=3 ? &dtau[mm] : NULL,
+                numQuantities==6 ? &dlnr2[mm] : NULL,
+                numQuantities==6 ? &dlnrdtau[mm] : NULL,
+                numQuantities==6 ? &dtau2[mm] : NULL);
         }
         if(logScaling) {
             double expX = exp(Phi[-mmin]), val = 1 / (invPhi0 - expX);
             Phi[-mmin] = val;
             if(numQuantities>=3) {
                 double dPhidX = pow_2(val) * expX;
+                if(numQuantities==6) {
+                    double d2PhidX2 = dPhidX * val * (invPhi0 + expX);
+                    dlnr2   [-mmin] = dPhidX * dlnr2   [-mmin] + d2PhidX2 * dlnr[-mmin] * dlnr[-mmin];
+                    dtau2   [-mmin] = dPhidX * dtau2   [-mmin] + d2PhidX2 * dtau[-mmin] * dtau[-mmin];
+                    dlnrdtau[-mmin] = dPhidX * dlnrdtau[-mmin] + d2PhidX2 * dlnr[-mmin] * dtau[-mmin];
+                }
                 dlnr[-mmin] *= dPhidX;
                 dtau[-mmin] *= dPhidX;
             }
@@ -1061,6 +1196,12 @@ struct MultipoleInterp2d {
                 int m = mm + mmin;
                 if(m==0 || ind.lmin(m) > ind.lmax)
                     continue;
+                if(numQuantities==6) {
+                    dlnr2[mm] = dlnr2[mm] * Phi[-mmin] + Phi[mm] * dlnr2[-mmin] + 2 * dlnr[mm] * dlnr[-mmin];
+                    dtau2[mm] = dtau2[mm] * Phi[-mmin] + Phi[mm] * dtau2[-mmin] + 2 * dtau[mm] * dtau[-mmin];
+                    dlnrdtau[mm] = dlnrdtau[mm] * Phi[-mmin] + Phi[mm] * dlnrdtau[-mmin] +
+                        dlnr[mm] * dtau[-mmin] + dtau[mm] * dlnr[-mmin];
+                }
                 if(numQuantities>=3) {
                     dlnr[mm] = dlnr[mm] * Phi[-mmin] + Phi[mm] * dlnr[-mmin];
                     dtau[mm] = dtau[mm] * Phi[-mmin] + Phi[mm] * dtau[-mmin];
@@ -1070,7 +1211,9 @@ struct MultipoleInterp2d {
         }
         double trPot;
         GradSph trGrad;
-        fourierTransformAzimuth(ind, pos.phi, C_m, &trPot, numQuantities>=3 ? &trGrad : NULL);
+        HessSph trHess;
+        fourierTransformAzimuth(ind, pos.phi, C_m, &trPot, numQuantities>=3 ? &trGrad : NULL,
+            numQuantities==6 ? &trHess : NULL);
         if(potential)
             *potential = trPot;
         if(numQuantities==1)
@@ -1078,9 +1221,19 @@ struct MultipoleInterp2d {
         const double rinv = 1/r, r2inv = pow_2(rinv);
         double drdR = pos.R * r2inv, drdz = pos.z * r2inv;
         double dthetadR = -tau * rinv, dthetadz = rinv - rplusRinv;
-        grad->dR = trGrad.dr*drdR + trGrad.dtheta*dthetadR;
-        grad->dz = trGrad.dr*drdz + trGrad.dtheta*dthetadz;
-        grad->dphi = trGrad.dphi;
+        if(grad) {
+            grad->dR = trGrad.dr*drdR + trGrad.dtheta*dthetadR;
+            grad->dz = trGrad.dr*drdz + trGrad.dtheta*dthetadz;
+            grad->dphi = trGrad.dphi;
+        }
+        if(hess) {
+            const double d2rdR2 = pow_2(drdz) - pow_2(drdR), d2rdRdz = -2 * drdR * drdz,
+                d2rdz2 = -d2rdR2, d2thetadR2 = pos.z * r2inv * rinv,
+                d2thetadRdz = pow_2(dthetadR) - pow_2(drdR) * r * rplusRinv,
+                d2thetadz2 = -d2thetadR2 - dthetadR * rplusRinv;
+            *hess = toHessSphCyl(trGrad, trHess, drdR, drdz, dthetadR, dthetadz,
+                d2rdR2, d2rdRdz, d2rdz2, d2thetadR2, d2thetadRdz, d2thetadz2);
+        }
     }
 };
 
@@ -1091,9 +1244,60 @@ struct orc_pot {
     virtual ~orc_pot() {}
     // BasePotential::eval(PosCar,...) (potential_base.h:153-156): gradient = +dPhi/dx
     virtual void evalCar(const double pos[3], double* phi, double grad[3], double t) const = 0;
+    // the same with the Hessian hess = {dx2, dy2, dz2, dxdy, dydz, dxdz} (coord.h HessCar)
+    virtual void evalCarHess(const double pos[3], double* phi, double grad[3], double hess[6],
+        double t) const = 0;
 };
 
 namespace {
+
+// coord.cpp:94-125  toPosDeriv<Car,Cyl> with the first and second derivatives of (R, phi)
+struct CarCylDeriv {
+    double dRdx, dRdy, dphidx, dphidy, d2Rdx2, d2Rdy2, d2Rdxdy, d2phidx2, d2phidy2, d2phidxdy;
+};
+inline CarCylDeriv carToCyl(const double p[3], PosCyl& pc)
+{
+    CarCylDeriv d;
+    const double R2 = pow_2(p[0]) + pow_2(p[1]), R = sqrt(R2);
+    if(R==0) {
+        d.dRdx = d.dRdy = d.dphidx = d.dphidy = 1.;
+        d.d2Rdx2 = d.d2Rdy2 = d.d2Rdxdy = d.d2phidx2 = d.d2phidy2 = d.d2phidxdy = 1.;
+        pc.R = 0; pc.z = p[2]; pc.phi = 0;
+        return d;
+    }
+    const double cosphi = p[0]/R, sinphi = p[1]/R;
+    d.dRdx = cosphi; d.dRdy = sinphi; d.dphidx = -sinphi/R; d.dphidy = cosphi/R;
+    d.d2Rdx2 = pow_2(sinphi)/R;
+    d.d2Rdy2 = pow_2(cosphi)/R;
+    d.d2Rdxdy = -sinphi*cosphi/R;
+    d.d2phidx2 = 2*sinphi*cosphi/R2;
+    d.d2phidy2 = -d.d2phidx2;
+    d.d2phidxdy = (pow_2(sinphi)-pow_2(cosphi))/R2;
+    pc.R = R; pc.z = p[2]; pc.phi = atan2_r(p[1], p[0]);
+    return d;
+}
+// coord.cpp:607-614 (toGrad<Cyl,Car>) and 700-720 (toHess<Cyl,Car>)
+inline void cylToCar(const CarCylDeriv& d, const GradCyl& g, const HessCyl& h, double grad[3], double hess[6])
+{
+    grad[0] = g.dR*d.dRdx + g.dphi*d.dphidx;
+    grad[1] = g.dR*d.dRdy + g.dphi*d.dphidy;
+    grad[2] = g.dz;
+    hess[0] =
+        (h.dR2   *d.dRdx + h.dRdphi*d.dphidx) * d.dRdx +
+        (h.dRdphi*d.dRdx + h.dphi2 *d.dphidx) * d.dphidx +
+        g.dR*d.d2Rdx2 + g.dphi*d.d2phidx2;
+    hess[3] =
+        (h.dR2   *d.dRdy + h.dRdphi*d.dphidy) * d.dRdx +
+        (h.dRdphi*d.dRdy + h.dphi2 *d.dphidy) * d.dphidx +
+        g.dR*d.d2Rdxdy + g.dphi*d.d2phidxdy;
+    hess[1] =
+        (h.dR2   *d.dRdy + h.dRdphi*d.dphidy) * d.dRdy +
+        (h.dRdphi*d.dRdy + h.dphi2 *d.dphidy) * d.dphidy +
+        g.dR*d.d2Rdy2 + g.dphi*d.d2phidy2;
+    hess[5] = h.dRdz*d.dRdx + h.dzdphi*d.dphidx;
+    hess[4] = h.dRdz*d.dRdy + h.dzdphi*d.dphidy;
+    hess[2] = h.dz2;
+}
 
 // potential_multipole.cpp:1491-1516, 1538-1548 ; potential_base.h:240-243 ; coord.h:831-854
 struct OrcMultipole : public orc_pot {
@@ -1120,15 +1324,26 @@ struct OrcMultipole : public orc_pot {
         asymptOuter = initAsympt(gridRadii, Phi, dPhi, false);
     }
 
-    void evalCyl(const PosCyl& pos, double* potential, GradCyl* deriv) const
+    void evalCyl(const PosCyl& pos, double* potential, GradCyl* deriv, HessCyl* deriv2 = NULL) const
     {
         double rsq = pow_2(pos.R) + pow_2(pos.z);
         if(rsq < pow_2(gridRadii.front() * (1+ROOT3_DBL_EPS)))
-            asymptInner.evalCyl(pos, potential, deriv);
+            asymptInner.evalCyl(pos, potential, deriv, deriv2);
         else if(rsq > pow_2(gridRadii.back() * (1-ROOT3_DBL_EPS)))
-            asymptOuter.evalCyl(pos, potential, deriv);
+            asymptOuter.evalCyl(pos, potential, deriv, deriv2);
         else
-            impl->evalCyl(pos, potential, deriv);
+            impl->evalCyl(pos, potential, deriv, deriv2);
+    }
+
+    // coord.cpp:94-125 (toPosDeriv<Car,Cyl> with second derivatives), 700-720 (toHess<Cyl,Car>)
+    virtual void evalCarHess(const double p[3], double* phi, double grad[3], double hess[6], double) const
+    {
+        PosCyl pc;
+        CarCylDeriv cd = carToCyl(p, pc);
+        GradCyl gc;
+        HessCyl hc;
+        evalCyl(pc, phi, &gc, &hc);
+        cylToCar(cd, gc, hc, grad, hess);
     }
 
     // coord.cpp:94-125 (toPosDeriv<Car,Cyl>) and coord.cpp:607-614 (toGrad<Cyl,Car>)
@@ -1159,7 +1374,7 @@ struct OrcMultipole : public orc_pot {
 struct OrcNFW : public orc_pot {
     double mass, scaleRadius;
     OrcNFW(double m, double a) : mass(m), scaleRadius(a) {}
-    void evalDeriv(double r, double* potential, double* deriv) const
+    void evalDeriv(double r, double* potential, double* deriv, double* deriv2 = NULL) const
     {
         double rrel = r / scaleRadius;
         double ln_over_r = r==INFINITY ? 0 :
@@ -1172,6 +1387,31 @@ struct OrcNFW : public orc_pot {
                 (ln_over_r - 1/(r+scaleRadius)) / r :
                 (0.5 + 17./96 * rrel) / (1 + rrel * (27./16 + rrel * (0.75 + 11./160 * rrel))) /
                 pow_2(scaleRadius));
+        if(deriv2)
+            *deriv2 = -mass * (rrel > 0.010 ?
+                (2*ln_over_r - (2*scaleRadius + 3*r) / pow_2(scaleRadius+r) ) / pow_2(r) :
+                1 / (1.5 + rrel * (27./8 + rrel * (351./160 + 183./640 * rrel))) / pow_3(scaleRadius) );
+    }
+    // coord.cpp:923-956 (evalAndConvertSph<Car> with the Hessian)
+    virtual void evalCarHess(const double p[3], double* phi, double grad[3], double hess[6], double) const
+    {
+        const double r = sqrt(pow_2(p[0]) + pow_2(p[1]) + pow_2(p[2]));
+        double der, der2;
+        evalDeriv(r, phi, &der, &der2);
+        double xr = p[0]/r, yr = p[1]/r, zr = p[2]/r;
+        if(r==0) xr = yr = zr = 0;
+        grad[0] = xr*der; grad[1] = yr*der; grad[2] = zr*der;
+        double der_over_r = der/r, dd = der2 - der_over_r;
+        if(r==0) {
+            dd = 0;
+            if(der==0) der_over_r = der2;
+        }
+        hess[0] = pow_2(xr)*dd + der_over_r;
+        hess[1] = pow_2(yr)*dd + der_over_r;
+        hess[2] = pow_2(zr)*dd + der_over_r;
+        hess[3] = xr*yr*dd;
+        hess[4] = yr*zr*dd;
+        hess[5] = xr*zr*dd;
     }
     virtual void evalCar(const double p[3], double* phi, double grad[3], double) const
     {
@@ -1232,6 +1472,45 @@ struct OrcComposite : public orc_pot {
             grad[0] += gsum.dR*dRdx + gsum.dphi*dphidx;
             grad[1] += gsum.dR*dRdy + gsum.dphi*dphidy;
             grad[2] += gsum.dz;
+        }
+    }
+    // evalComponents<Cyl,Sph> with second derivatives: the cylindrical components' gradients
+    // and Hessians are summed in cylindrical coordinates and converted once after the loop
+    virtual void evalCarHess(const double p[3], double* phi, double grad[3], double hess[6], double t) const
+    {
+        if(phi) *phi = 0;
+        for(int i=0; i<3; i++) grad[i] = 0;
+        for(int i=0; i<6; i++) hess[i] = 0;
+        bool haveCyl = false;
+        PosCyl pc;
+        CarCylDeriv cd;
+        GradCyl gsum = {0, 0, 0};
+        HessCyl hsum = {0, 0, 0, 0, 0, 0};
+        for(const orc_pot* c : comps) {
+            const OrcMultipole* mp = dynamic_cast<const OrcMultipole*>(c);
+            if(mp) {
+                if(!haveCyl) { cd = carToCyl(p, pc); haveCyl = true; }
+                double pot;
+                GradCyl g;
+                HessCyl h;
+                mp->evalCyl(pc, &pot, &g, &h);
+                if(phi) *phi += pot;
+                gsum.dR += g.dR; gsum.dz += g.dz; gsum.dphi += g.dphi;
+                hsum.dR2 += h.dR2; hsum.dz2 += h.dz2; hsum.dphi2 += h.dphi2;
+                hsum.dRdz += h.dRdz; hsum.dzdphi += h.dzdphi; hsum.dRdphi += h.dRdphi;
+            } else {
+                double pot, g[3], h[6];
+                c->evalCarHess(p, &pot, g, h, t);
+                if(phi) *phi += pot;
+                for(int i=0; i<3; i++) grad[i] += g[i];
+                for(int i=0; i<6; i++) hess[i] += h[i];
+            }
+        }
+        if(haveCyl) {
+            double g[3], h[6];
+            cylToCar(cd, gsum, hsum, g, h);
+            for(int i=0; i<3; i++) grad[i] += g[i];
+            for(int i=0; i<6; i++) hess[i] += h[i];
         }
     }
 };
@@ -2043,6 +2322,22 @@ void orc_eval_many(const orc_pot* p, long n, const double* pos, double t,
     }
 }
 
+void orc_eval_hess_many(const orc_pot* p, long n, const double* pos, double t,
+    double* phi, double* grad, double* hess, int nthreads)
+{
+#ifdef _OPENMP
+    if(nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(nthreads)
+#endif
+    for(long i=0; i<n; i++) {
+        double ph, g[3], h[6];
+        p->evalCarHess(&pos[i*3], &ph, g, h, t);
+        if(phi) phi[i] = ph;
+        if(grad) for(int c=0; c<3; c++) grad[i*3+c] = g[c];
+        if(hess) for(int c=0; c<6; c++) hess[i*6+c] = h[c];
+    }
+}
+
 int orc_orbit_many(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, const int* trajsize, const long* traj_offset,
     double accuracy, double Omega, long maxNumSteps,
@@ -2166,6 +2461,8 @@ long orc_binsearch(double x, const double* arr, long size) { return binSearch(x,
 double orc_lambertw(double x, int wm1) { return lambertW(x, wm1 != 0); }
 void orc_gl_nodes(int n, double* nodes, double* weights) { prepareIntegrationTableGL(-1, 1, n, nodes, weights); }
 void orc_sphharm_array(int lmax, int m, double tau, double* P, double* dP) { sphHarmArray(lmax, m, tau, P, dP); }
+void orc_sphharm_array2(int lmax, int m, double tau, double* P, double* dP, double* d2P)
+{ sphHarmArray(lmax, m, tau, P, dP, d2P); }
 void orc_trig_multi_angle(double phi, int m, int needSine, double* out) { trigMultiAngle(phi, m, needSine!=0, out); }
 
 int orc_spline2d_eval(int nx, int ny, const double* xg, const double* yg, const double* f,
@@ -2179,6 +2476,21 @@ int orc_spline2d_eval(int nx, int ny, const double* xg, const double* yg, const 
             std::vector<double>(fy, fy+n2), std::vector<double>(fxy, fxy+n2));
         for(long i=0; i<npts; i++)
             s.evalDeriv(xy[2*i], xy[2*i+1], &z[i], &zx[i], &zy[i]);
+        return 0;
+    } catch(std::exception& e) { g_err = e.what(); return 1; }
+}
+
+int orc_spline2d_eval2(int nx, int ny, const double* xg, const double* yg, const double* f,
+    const double* fx, const double* fy, const double* fxy, long npts, const double* xy,
+    double* z, double* zx, double* zy, double* zxx, double* zxy, double* zyy)
+{
+    try {
+        size_t n2 = (size_t)nx*ny;
+        QuinticSpline2d s(std::vector<double>(xg, xg+nx), std::vector<double>(yg, yg+ny),
+            std::vector<double>(f, f+n2), std::vector<double>(fx, fx+n2),
+            std::vector<double>(fy, fy+n2), std::vector<double>(fxy, fxy+n2));
+        for(long i=0; i<npts; i++)
+            s.evalDeriv(xy[2*i], xy[2*i+1], &z[i], &zx[i], &zy[i], &zxx[i], &zxy[i], &zyy[i]);
         return 0;
     } catch(std::exception& e) { g_err = e.what(); return 1; }
 }

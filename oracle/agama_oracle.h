@@ -45,6 +45,12 @@ long orc_multipole_tables(const orc_pot* p, int which, int m, double* out);
 void orc_eval_many(const orc_pot* p, long n, const double* pos, double t,
     double* phi, double* grad, int nthreads);
 
+/* potential, gradient and Hessian {dx2, dy2, dz2, dxdy, dydz, dxdz} of Phi (not of the
+   acceleration) at n points; any output may be NULL.  BasePotential::eval with deriv2
+   (potential_base.h:153-156). */
+void orc_eval_hess_many(const orc_pot* p, long n, const double* pos, double t,
+    double* phi, double* grad, double* hess, int nthreads);
+
 /* batch orbit integration, restating agama.orbit(potential, ic, time, timestart, trajsize,
    accuracy, Omega, maxNumSteps, dtype=float) (reference: src/interface_python.cpp:7689-8281).
    ic[n*6], tstart[n], ttotal[n], trajsize[n] (>=1); traj_offset[n] = first output row of each
@@ -96,12 +102,17 @@ long   orc_binsearch(double x, const double* arr, long size);
 double orc_lambertw(double x, int wm1branch);
 void   orc_gl_nodes(int n, double* nodes, double* weights);   /* on [-1,1], ascending */
 void   orc_sphharm_array(int lmax, int m, double tau, double* P, double* dP);
+void   orc_sphharm_array2(int lmax, int m, double tau, double* P, double* dP, double* d2P);
 void   orc_trig_multi_angle(double phi, int m, int needSine, double* out);
 /* 2-D quintic spline on a grid: build from f, fx, fy, fxy (row-major nx*ny), evaluate at
    npts points (x,y) -> z, z_x, z_y.  Exposes QuinticSpline2d for node-exactness tests. */
 int orc_spline2d_eval(int nx, int ny, const double* xg, const double* yg, const double* f,
     const double* fx, const double* fy, const double* fxy, long npts, const double* xy,
     double* z, double* zx, double* zy);
+/* the same with the second derivatives z_xx, z_xy, z_yy */
+int orc_spline2d_eval2(int nx, int ny, const double* xg, const double* yg, const double* f,
+    const double* fx, const double* fy, const double* fxy, long npts, const double* xy,
+    double* z, double* zx, double* zy, double* zxx, double* zxy, double* zyy);
 
 #ifdef __cplusplus
 }

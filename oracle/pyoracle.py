@@ -32,6 +32,7 @@ def _load(name):
         "orc_composite": (_vp, [ctypes.c_int, _vp]),
         "orc_multipole_tables": (ctypes.c_long, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
         "orc_eval_many": (None, [_vp, ctypes.c_long, _vp, ctypes.c_double, _vp, _vp, ctypes.c_int]),
+        "orc_eval_hess_many": (None, [_vp, ctypes.c_long, _vp, ctypes.c_double, _vp, _vp, _vp, ctypes.c_int]),
         "orc_orbit_many": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp,
                                           _vp, _vp, ctypes.c_int]),
@@ -50,6 +51,9 @@ def _load(name):
         "orc_lambertw": (ctypes.c_double, [ctypes.c_double, ctypes.c_int]),
         "orc_gl_nodes": (None, [ctypes.c_int, _vp, _vp]),
         "orc_sphharm_array": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _vp]),
+        "orc_sphharm_array2": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _vp, _vp]),
+        "orc_spline2d_eval2": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                              ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "orc_trig_multi_angle": (None, [ctypes.c_double, ctypes.c_int, ctypes.c_int, _vp]),
         "orc_spline2d_eval": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
                                              ctypes.c_long, _vp, _vp, _vp, _vp]),
@@ -120,6 +124,15 @@ class OraclePotential:
         grad = np.empty((len(pts), 3))
         self.lib.orc_eval_many(self.h, len(pts), _p(pts), 0.0, _p(phi), _p(grad), nthreads)
         return phi, grad
+
+    def eval_hess(self, points, nthreads=0):
+        """potential, gradient and Hessian {dx2, dy2, dz2, dxdy, dydz, dxdz} of Phi"""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        phi = np.empty(len(pts))
+        grad = np.empty((len(pts), 3))
+        hess = np.empty((len(pts), 6))
+        self.lib.orc_eval_hess_many(self.h, len(pts), _p(pts), 0.0, _p(phi), _p(grad), _p(hess), nthreads)
+        return phi, grad, hess
 
     def orbits(self, ic, ttotal, trajsize=1, tstart=None, accuracy=1e-8, Omega=0.0,
                maxNumSteps=int(1e8), nthreads=0, method="dop853"):
