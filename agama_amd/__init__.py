@@ -167,17 +167,40 @@ class Potential:
         """acceleration -grad(Phi) - reference: Potential.force"""
         return -self._eval(points, False, True, t, precision)[1]
 
+    def _eval_hess(self, points, t=0.0):
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        single = pts.shape == (3,)
+        pts2 = pts.reshape(-1, 3)
+        n = pts2.shape[0]
+        phi, grad, hess = np.empty(n), np.empty((n, 3)), np.empty((n, 6))
+        check(lib.agx_eval_hess_many(self._h, n, ptr(pts2), float(t), ptr(phi), ptr(grad), ptr(hess),
+                                     PRECISION_FP64), "eval_hess")
+        if single:
+            return phi[0], grad[0], hess[0]
+        return phi, grad, hess
+
     def eval(self, points, pot=False, acc=False, der=False, t=0.0, precision=PRECISION_FP64):
-        """reference: Potential.eval(points, pot=, acc=, der=); der (Hessian) is not provided"""
+        """reference: Potential.eval(points, pot=, acc=, der=) (interface_python.cpp:3114-3207):
+        potential, acceleration -dPhi/dx and force derivatives -d2Phi/dxdy in the order
+        xx, yy, zz, xy, yz, zx (FP64 when der=True)"""
         if der:
-            raise AgamaAmdError("second derivatives (der=True) are not provided by this version")
-        phi, grad = self._eval(points, pot, acc, t, precision)
+            phi, grad, hess = self._eval_hess(points, t)
+        else:
+            phi, grad = self._eval(points, pot, acc, t, precision)
         out = []
         if pot:
             out.append(phi)
         if acc:
             out.append(-grad)
+        if der:
+            out.append(-hess)
         return out[0] if len(out) == 1 else tuple(out)
+
+    def forceDeriv(self, points, t=0.0):
+        """reference: Potential.forceDeriv (deprecated there in favour of eval(acc=True, der=True)):
+        (force, force derivatives)"""
+        _, grad, hess = self._eval_hess(points, t)
+        return -grad, -hess
 
 
 def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, accuracy=1e-8,

@@ -26,6 +26,8 @@ hipError_t agxLaunchOrbitRKN(const AgxMultipoleDev& P, const AgxMultipoleDev* dP
     const AgxOrbitArgs& A, int numBlocks, hipStream_t stream);
 int agxOrbitKernelRKNBlocksPerCU(int precision, bool nfw);
 hipError_t agxLaunchFillU32(unsigned int* p, long n, unsigned int v, hipStream_t stream);
+hipError_t agxLaunchEvalHess(const AgxMultipoleDev* dP, long n, const double* pos, double* phi,
+    double* grad, double* hess, hipStream_t stream);
 int agxOrbitKernelBlockSize();
 size_t agxScheduleWorkspace(long n);
 hipError_t agxLaunchSchedule(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, long n,
@@ -327,6 +329,40 @@ int agx_eval_many_device(const agx_potential* pot, long n, const double* d_pos, 
     return AGX_OK;
 }
 
+int agx_eval_hess_many_device(const agx_potential* pot, long n, const double* d_pos, double t,
+    double* d_phi, double* d_grad, double* d_hess, int flags, void* stream)
+{
+    (void)t;   // the Multipole potential is static
+    if(!pot) return fail(AGX_EINVAL, "NULL potential");
+    if(n < 0 || (n > 0 && !d_pos)) return fail(AGX_EINVAL, "invalid point array");
+    if(flags == AGX_PRECISION_FP32) return fail(AGX_EUNSUPPORTED, "second derivatives are evaluated in FP64 only");
+    if(flags != AGX_PRECISION_FP64) return fail(AGX_EINVAL, "invalid flags");
+    hipError_t e = agxLaunchEvalHess(pot->ddev, n, d_pos, d_phi, d_grad, d_hess, (hipStream_t)stream);
+    if(e != hipSuccess) return fail(AGX_EDEVICE, std::string("hessian launch: ") + hipGetErrorString(e));
+    return AGX_OK;
+}
+
+int agx_eval_hess_many(const agx_potential* pot, long n, const double* pos, double t,
+    double* phi, double* grad, double* hess, int flags)
+{
+    g_err.clear();
+    if(!pot) return fail(AGX_EINVAL, "NULL potential");
+    if(n == 0) return AGX_OK;
+    if(n < 0 || !pos) return fail(AGX_EINVAL, "invalid point array");
+    try {
+        HIPCHECK(hipSetDevice(pot->device));
+        DevBuf<double> dp(n * 3), dphi(phi ? n : 0), dg(grad ? n * 3 : 0), dh(hess ? n * 6 : 0);
+        HIPCHECK(hipMemcpy(dp.p, pos, n * 3 * 8, hipMemcpyHostToDevice));
+        int rc = agx_eval_hess_many_device(pot, n, dp.p, t, dphi.p, dg.p, dh.p, flags, nullptr);
+        if(rc) return rc;
+        HIPCHECK(hipDeviceSynchronize());
+        if(phi) HIPCHECK(hipMemcpy(phi, dphi.p, n * 8, hipMemcpyDeviceToHost));
+        if(grad) HIPCHECK(hipMemcpy(grad, dg.p, n * 3 * 8, hipMemcpyDeviceToHost));
+        if(hess) HIPCHECK(hipMemcpy(hess, dh.p, n * 6 * 8, hipMemcpyDeviceToHost));
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
 int agx_eval_many(const agx_potential* pot, long n, const double* pos, double t,
     double* phi, double* grad, int flags)
 {
@@ -350,12 +386,15 @@ int agx_eval_many(const agx_potential* pot, long n, const double* pos, double t,
 double agx_eval_potential(const agx_potential* pot, const double pos[3], double t,
     double deriv[3], double deriv2[6])
 {
-    if(deriv2) {
-        fail(AGX_EUNSUPPORTED, "second derivatives are not provided by this version");
-        for(int i = 0; i < 6; i++) deriv2[i] = NAN;
-    }
-    double phi = NAN, g[3];
-    if(agx_eval_many(pot, 1, pos, t, &phi, g, AGX_PRECISION_FP64) != AGX_OK) return NAN;
+    double phi = NAN, g[3], h[6];
+    if(deriv2) {   // {dx2, dy2, dz2, dxdy, dydz, dxdz} of Phi, as interface_c.cpp:81-88
+        if(agx_eval_hess_many(pot, 1, pos, t, &phi, g, h, AGX_PRECISION_FP64) != AGX_OK) {
+            for(int i = 0; i < 6; i++) deriv2[i] = NAN;
+            return NAN;
+        }
+        for(int i = 0; i < 6; i++) deriv2[i] = h[i];
+    } else if(agx_eval_many(pot, 1, pos, t, &phi, g, AGX_PRECISION_FP64) != AGX_OK)
+        return NAN;
     if(deriv) for(int i = 0; i < 3; i++) deriv[i] = g[i];
     return phi;
 }

@@ -89,7 +89,8 @@ int agx_potential_write(const agx_potential* pot, const char* path);
 int agx_potential_coefs(const agx_potential* pot, int* nr, int* lmax, double* radii,
     double* phi, double* dphi);
 
-/* Single-point potential and gradient.  deriv2 (Hessian) must be NULL in this version.
+/* Single-point potential, gradient (+dPhi/dx) and, when deriv2 is non-NULL, Hessian
+   {dx2, dy2, dz2, dxdy, dydz, dxdz} of Phi.
    Replaces agama_evalPotential (src/interface_c.h:75, src/interface_c.cpp:68-98). */
 double agx_eval_potential(const agx_potential* pot, const double pos[3], double t,
     double deriv[3], double deriv2[6]);
@@ -103,6 +104,16 @@ int agx_eval_many(const agx_potential* pot, long n, const double* pos, double t,
 /* Same with device buffers, asynchronous on `stream` (hipStream_t). */
 int agx_eval_many_device(const agx_potential* pot, long n, const double* d_pos, double t,
     double* d_phi, double* d_grad, int flags, void* stream);
+
+/* Batched potential, gradient and Hessian hess[n*6] = {dx2, dy2, dz2, dxdy, dydz, dxdz} of Phi
+   (any output may be NULL; FP64 only: AGX_PRECISION_FP32 returns AGX_EUNSUPPORTED).  Replaces
+   BasePotential::eval with deriv2 behind Potential.eval(..., der=True) and forceDeriv
+   (src/interface_python.cpp:3027-3207, 3106-3112; the Python layer returns -hess, the force
+   derivatives). */
+int agx_eval_hess_many(const agx_potential* pot, long n, const double* pos, double t,
+    double* phi, double* grad, double* hess, int flags);
+int agx_eval_hess_many_device(const agx_potential* pot, long n, const double* d_pos, double t,
+    double* d_phi, double* d_grad, double* d_hess, int flags, void* stream);
 
 /* Batch orbit integration with DOP853 (or DPRKN8, flags) in the given potential, restating agama.orbit
    (src/interface_python.cpp:7689-8281) with dtype=float64 and separateTime output:
