@@ -62,9 +62,10 @@ def parse():
     ap.add_argument("--cpu-orbits", type=int, default=20_000,
                     help="size of the CPU-baseline sample (0 disables)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["orbit", "fp64", "eval", "mlp", "copy"], default="orbit",
+    ap.add_argument("--mode", choices=["orbit", "fp64", "eval", "hess", "mlp", "copy"], default="orbit",
                     help="orbit: surrogate and FP64 paths (default); fp64: the FP64 path alone; "
-                         "eval: potential + force at 1M points drawn from the density; mlp: the "
+                         "eval: potential + force at 1M points drawn from the density; hess: the same "
+                         "with the Hessian; mlp: the "
                          "surrogate's fused MFMA evaluation at 1M points; copy: one 1 GiB "
                          "device-to-device copy, the reference for HBM byte counters")
     ap.add_argument("--method", choices=["dop853", "dprkn8"], default="dop853",
@@ -78,9 +79,14 @@ def parse():
 # algorithmic FP64 operations of one potential+gradient evaluation (eval mode): the
 # right-hand-side count above without the frame rotation
 FLOPS_PER_POINT = FLOPS_PER_EVAL - 15
+# ... and of one potential+gradient+Hessian evaluation (hess mode): per harmonic 9 quintic passes
+#   with second derivatives (35 each) + corner offset 5 -> 320, log-scaling products 26 and
+#   Fourier sums 22 for m != 0; 5 harmonics -> 1,840; weights 80, cylindrical and Cartesian
+#   gradient/Hessian transforms incl. atan2 and the (ln r, tau) second derivatives 280  -> 2,200
+FLOPS_PER_POINT_HESS = 2200
 
 
-def bench_eval(args):
+def bench_eval(args, hess=False):
     import numpy as np
     import torch
     import agama_amd
@@ -93,13 +99,20 @@ def bench_eval(args):
     pts = torch.from_numpy(pts_np).to(dev)
     phi = torch.empty(n, dtype=torch.float64, device=dev)
     grad = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    hes = torch.empty((n, 6), dtype=torch.float64, device=dev) if hess else None
     prec = agama_amd.PRECISION_FP64 if args.precision == "fp64" else agama_amd.PRECISION_FP32
+    if hess and prec != agama_amd.PRECISION_FP64:
+        raise SystemExit("bench.py: the Hessian path is FP64 only")
     stream = torch.cuda.Stream()
     reps = max(args.steps, 1) * 20
 
     def launch():
-        rc = lib.agx_eval_many_device(pot.handle, n, pts.data_ptr(), 0.0, phi.data_ptr(), grad.data_ptr(),
-                                      prec, stream.cuda_stream)
+        if hess:
+            rc = lib.agx_eval_hess_many_device(pot.handle, n, pts.data_ptr(), 0.0, phi.data_ptr(),
+                                               grad.data_ptr(), hes.data_ptr(), prec, stream.cuda_stream)
+        else:
+            rc = lib.agx_eval_many_device(pot.handle, n, pts.data_ptr(), 0.0, phi.data_ptr(), grad.data_ptr(),
+                                          prec, stream.cuda_stream)
         if rc:
             raise RuntimeError(agama_amd._native.last_error())
     with torch.cuda.stream(stream):
@@ -115,7 +128,7 @@ def bench_eval(args):
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
     per = ev0.elapsed_time(ev1) / 1e3 / reps
-    achieved = n * FLOPS_PER_POINT / per / 1e12
+    achieved = n * (FLOPS_PER_POINT_HESS if hess else FLOPS_PER_POINT) / per / 1e12
     cpu = None
     if args.cpu_orbits > 0:
         from oracle import pyoracle
@@ -123,19 +136,21 @@ def bench_eval(args):
         fast = pyoracle.OraclePotential(coef, library=pyoracle.fast())
         t0 = time.perf_counter()
         for _ in range(3):
-            fast.eval(pts_np, nthreads=threads)
+            (fast.eval_hess if hess else fast.eval)(pts_np, nthreads=threads)
         dt = (time.perf_counter() - t0) / 3
         cpu = {"value": n / dt, "unit": "evaluations/s", "cores": threads, "kind": "port",
                "sample": f"the same 1M points, C++ OpenMP restatement (-O2), {threads} threads"}
-    out = {"metric": "potential+force evaluations/s at 1M points (triaxial Dehnen Multipole lmax=8)",
+    what = "potential+force+Hessian" if hess else "potential+force"
+    out = {"metric": f"{what} evaluations/s at 1M points (triaxial Dehnen Multipole lmax=8)",
            "value": n / per, "unit": "evaluations/s", "n_gpus": 1, "steps": reps, "warmup": args.warmup,
            "ms_per_step": wall / reps * 1e3, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f64 (fp32 spline tables)",
            "data": "synthetic",
-           "config": {"workload": "Phi and grad Phi at 1M points drawn from the triaxial Dehnen density"},
+           "config": {"workload": ("Phi, grad Phi and the Hessian" if hess else "Phi and grad Phi") +
+                                  " at 1M points drawn from the triaxial Dehnen density"},
            "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                        "kernel": "evalKernel<double>", "kernel_ms": per * 1e3},
+                        "kernel": "evalHessKernel" if hess else "evalKernel<double>", "kernel_ms": per * 1e3},
            "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
@@ -146,8 +161,8 @@ def main():
         print("bench.py: this build targets one GPU; --gpus N with N > 1 is not supported "
               "(orbits are independent and would shard by orbit index)", file=sys.stderr)
         sys.exit(2)
-    if args.mode == "eval":
-        return bench_eval(args)
+    if args.mode in ("eval", "hess"):
+        return bench_eval(args, hess=args.mode == "hess")
     if args.mode == "copy":
         import torch
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
