@@ -28,6 +28,7 @@
       - Lambert W is an own Halley iteration (reference: GSL gsl_sf_lambert_W0/Wm1).
 */
 #include "agama_oracle.h"
+#include <functional>
 #include <cmath>
 #include <cfloat>
 #include <cstring>
@@ -2019,6 +2020,143 @@ struct DPRKN8 {
     }
 };
 
+// x'' = A(t) x + B(t) x' with 3x3 matrices (IOdeSystem2ndOrderLinear, math_ode.h:74-86);
+// t is measured from the start of the current step
+struct LinSys2 {
+    virtual ~LinSys2() {}
+    virtual void evalMat(double t, double a[9], double b[9]) const = 0;
+};
+
+// math_ode.cpp:1017-1250  implicit Gauss-Legendre collocation with 4 nodes (order 8) for
+// nvec independent 3-d vectors; no error control.  state per vector: x, x', and the
+// interpolation coefficients p, q, r, s (3 values each)
+struct Ode2StepperGL4 {
+    const LinSys2& sys;
+    const int nvec;
+    double prevTimeStep;
+    bool newstep;
+    std::vector<double> state;
+    Ode2StepperGL4(const LinSys2& s, int nv) : sys(s), nvec(nv), prevTimeStep(0), newstep(true),
+        state(18 * nv) {}
+
+    void init(const double* w)   // nvec blocks of {x[3], x'[3]}
+    {
+        state.assign(18 * nvec, 0);
+        for(int v=0; v<nvec; v++)
+            for(int i=0; i<6; i++) state[v*18 + i] = w[v*6 + i];
+        newstep = true;
+    }
+
+    double getSol(double timeOffset, int i) const
+    {
+        const double T = timeOffset - prevTimeStep;
+        const int vec = i / 6;
+        i -= vec * 6;
+        if(T == 0) return state[vec*18 + i];
+        const int d = i % 3;
+        const double *x = &state[vec*18 + d], *xd = x + 3, *p = xd + 3, *q = p + 3, *r = q + 3, *s = r + 3;
+        if(i == d)
+            return (*x) + T * ((*xd) + T * ((*p) + T * ((*q) + T * ((*r) + T * (*s)))));
+        return (*xd) + T * (2 * (*p) + T * (3 * (*q) + T * 4 * ((*r) + T * 5 * (*s))));
+    }
+
+    double doStep(double dt)
+    {
+        const double dt2 = dt*dt, dt3 = dt2*dt, idt = 1./dt, idt2 = idt*idt, idt3 = idt2*idt;
+        // collocation nodes (Gauss-Legendre, 4 points on [0,1]) and the derived weights
+        static const double g0 = 0.069431844202973713, g1 = 0.33000947820757187, g2 = 1-g1, g3 = 1-g0;
+        static const double
+            su0 = 0.5 * g0 * g0, sv0 = su0 * g0 * -2/3, sw0 = su0 * g0 * (g1*2/3 - g0/6),
+            sz0 = su0 * g0 * (g0 * (g1+g2)/6 - g0*g0/15 - g1*g2*2/3),
+            su1 = 0.5 * g1 * g1, sv1 = su1 *(g1/3 - g0), sw1 = su1 * g1 * (g0*2/3 - g1/6),
+            sz1 = su1 * g1 * (g1 * (g0+g2)/6 - g1*g1/15 - g0*g2*2/3),
+            su2 = 0.5 * g2 * g2, sv2 = su2 *(g2/3 - g0), sw2 = su2 *(g2/3 * (g2/2 - g0 - g1) + g0*g1),
+            sz2 = su2 * g2 * (g2 * (g0+g1)/6 - g2*g2/15 - g0*g1*2/3),
+            su3 = 0.5 * g3 * g3, sv3 = su3 *(g3/3 - g0), sw3 = su3 *(g3/3 * (g3/2 - g0 - g1) + g0*g1),
+            sz3 = su3 *(g3 * (g3 * (g3*0.6-g0-g1-g2)/6 + (g0*g1+g1*g2+g0*g2)/3) - g0*g1*g2),
+            du0 = g0, dv0 = g0 * -g0 / 2, dw0 = g0 * g0 * (g1 / 2 - g0 / 6),
+            dz0 = g0 * g0 * (g0 * (-g0 / 12 + g1 / 6 + g2 / 6) - g1 * g2 / 2),
+            du1 = g1, dv1 = g1 * (g1 / 2 - g0), dw1 = g1 * g1 * (g0 / 2 - g1 / 6),
+            dz1 = g1 * g1 * (g1 * (-g1 / 12 + g0 / 6 + g2 / 6) - g0 * g2 / 2),
+            du2 = g2, dv2 = g2 * (g2 / 2 - g0), dw2 = g2 * (g2 * (g2 / 3 - g1 / 2 - g0 / 2) + g0 * g1),
+            dz2 = g2 * g2 * (g2 * (-g2 / 12 + g0 / 6 + g1 / 6) - g0 * g1 / 2),
+            du3 = g3, dv3 = g3 * (g3 / 2 - g0), dw3 = g3 * (g3 * (g3 / 3 - g1 / 2 - g0 / 2) + g0 * g1),
+            dz3 = g3 * (g3 * (g0*g1/2 + g0*g2/2 + g1*g2/2 - g3 * (g0/3 + g1/3 + g2/3 - g3/4)) - g0*g1*g2),
+            mvu = 1 / (g1 - g0), mwu = 1 / (g2 - g0) / (g2 - g1), mwv = 1 / (g2 - g1),
+            mzu = 1 / (g3 - g0) / (g3 - g1) / (g3 - g2), mzv = 1 / (g3 - g1) / (g3 - g2), mzw = 1 / (g3 - g2),
+            mpu = 1./2, mpv = 1./2 * (1 - g0), mpw = 1./2 * (1 - g0) * (1 - g1),
+            mpz = 1./2 * (1 - g0) * (1 - g1) * (1 - g2),
+            mqv = 1./6, mqw = 1./6 * (2 - g0 - g1),
+            mqz = 1./6 * ((1-g0) * (1-g1) + (1-g1) * (1-g2) + (1-g2) * (1-g0)),
+            mrw = 1./12, mrz = 1./12 * (3 - g0 - g1 - g2), msz = 1./20;
+        double a[4][9], b[4][9];
+        sys.evalMat(dt * g0, a[0], b[0]);
+        sys.evalMat(dt * g1, a[1], b[1]);
+        sys.evalMat(dt * g2, a[2], b[2]);
+        sys.evalMat(dt * g3, a[3], b[3]);
+        // x''(h) = u + (h-g0) (v + (h-g1) (w + (h-g2) z)), solved by fixed-point iteration
+        auto rhs = [&](int k, const double* xh, const double* dh, int d) {
+            double r = 0;
+            for(int j=0; j<3; j++) r += a[k][d*3+j] * xh[j] + b[k][d*3+j] * dh[j];
+            return r;
+        };
+        for(int vec=0; vec<nvec; vec++) {
+            double *x = &state[vec*18], *xd = x + 3, *p = xd + 3, *q = p + 3, *r = q + 3, *sc = r + 3;
+            double x0[3], x1[3], x2[3], x3[3], u[3], v[3], w[3], z[3], xh[3], dh[3];
+            for(int d=0; d<3; d++) {
+                x0[d] = x[d] + xd[d] * dt * g0;
+                x1[d] = x[d] + xd[d] * dt * g1;
+                x2[d] = x[d] + xd[d] * dt * g2;
+                x3[d] = x[d] + xd[d] * dt * g3;
+                // predictor from the previous step's interpolation coefficients
+                double Q = dt*q[d], R = dt2*r[d], S = dt3*sc[d];
+                u[d] = 2  * p[d] + 6*g0 * Q + 12*g0*g0 * R + 20*g0*g0*g0 * S;
+                v[d] = 6  * Q + 12*(g0+g1) * R + 20*(g0*g0+g0*g1+g1*g1) * S;
+                w[d] = 12 * R + 20*(g0+g1+g2) * S;
+                z[d] = 20 * S;
+            }
+            const int niter = newstep ? 8 : 4;
+            for(int it=0; it<niter; it++) {
+                for(int d=0; d<3; d++) {
+                    xh[d] = x0[d] + dt2 * (su0 * u[d] + sv0 * v[d] + sw0 * w[d] + sz0 * z[d]);
+                    dh[d] = xd[d] + dt  * (du0 * u[d] + dv0 * v[d] + dw0 * w[d] + dz0 * z[d]);
+                }
+                for(int d=0; d<3; d++) u[d] = rhs(0, xh, dh, d);
+                for(int d=0; d<3; d++) {
+                    xh[d] = x1[d] + dt2 * (su1 * u[d] + sv1 * v[d] + sw1 * w[d] + sz1 * z[d]);
+                    dh[d] = xd[d] + dt  * (du1 * u[d] + dv1 * v[d] + dw1 * w[d] + dz1 * z[d]);
+                }
+                for(int d=0; d<3; d++) v[d] = (rhs(1, xh, dh, d) - u[d]) * mvu;
+                for(int d=0; d<3; d++) {
+                    xh[d] = x2[d] + dt2 * (su2 * u[d] + sv2 * v[d] + sw2 * w[d] + sz2 * z[d]);
+                    dh[d] = xd[d] + dt  * (du2 * u[d] + dv2 * v[d] + dw2 * w[d] + dz2 * z[d]);
+                }
+                for(int d=0; d<3; d++) w[d] = (rhs(2, xh, dh, d) - u[d]) * mwu - v[d] * mwv;
+                for(int d=0; d<3; d++) {
+                    xh[d] = x3[d] + dt2 * (su3 * u[d] + sv3 * v[d] + sw3 * w[d] + sz3 * z[d]);
+                    dh[d] = xd[d] + dt  * (du3 * u[d] + dv3 * v[d] + dw3 * w[d] + dz3 * z[d]);
+                }
+                for(int d=0; d<3; d++) z[d] = (rhs(3, xh, dh, d) - u[d]) * mzu - v[d] * mzv - w[d] * mzw;
+            }
+            for(int d=0; d<3; d++) {
+                const double P = mpz * z[d] + mpw * w[d] + mpv * v[d] + mpu * u[d],
+                    Q = mqz * z[d] + mqw * w[d] + mqv * v[d],
+                    R = mrz * z[d] + mrw * w[d],
+                    S = msz * z[d];
+                p[d] = P;
+                q[d] = Q * idt;
+                r[d] = R * idt2;
+                sc[d] = S * idt3;
+                x[d]  += (    P - 2 * Q + 3 * R - 4 * S) * dt2 + xd[d] * dt;
+                xd[d] += (2 * P - 3 * Q + 4 * R - 5 * S) * dt;
+            }
+        }
+        prevTimeStep = dt;
+        newstep = false;
+        return dt;
+    }
+};
+
 struct OrbitSys : public OdeSys2 {
     const orc_pot& potential;
     double Omega, timeBegin;
@@ -2081,11 +2219,20 @@ inline double energyOf(const double* xv, double Epot, double Omega)
 // per-orbit body of interface_python.cpp:8107-8202 with trajectory stored as float64
 struct OrbitResult { long nsteps, nattempts, nevals; double maxde; int status; double fin[6]; };
 
+// outputs of RuntimeVariational (orbit_variational.cpp): the six deviation vectors sampled like
+// the trajectory (dev[(vec*trajsize + row)*6 + c], NaN-padded), and the per-step log-magnitude
+// series of the fiducial deviation vector (time since start, log|w|)
+struct VarOutput {
+    double* dev = nullptr;
+    bool wantLyap = false;
+    std::vector<std::pair<double, double> > logDev;
+};
+
 // method: 0 = DOP853, 1 = DPRKN8 (OrbitIntParams::Method, orbit.h:116-129, 186-195)
 template<class Stepper>
 void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart, double ttotal,
     int trajsize, double accuracy, double Omega, long maxNumSteps,
-    double* out_time, double* out_traj, OrbitResult& res)
+    double* out_time, double* out_traj, OrbitResult& res, VarOutput* var = nullptr)
 {
     const double ROUNDOFF = 10*DBL_EPSILON;
     OrbitSys sys(pot, Omega);
@@ -2164,12 +2311,133 @@ void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart
             store(actualSize++, tbegin + timestep, pt);
         }
     };
+    // RuntimeVariational (orbit_variational.cpp:63-268, inertial-frame form): the variational
+    // equation Dx'' = -Hess(x(t)) Dx along the trajectory, with the Hessian taken at the dense
+    // output of the current step and rotated into the inertial frame (evalMat, :85-119)
+    const bool wantVar = var && (var->dev || var->wantLyap);
+    const int numVectors = var && var->dev ? 6 : 1;
+    static const double DEV_VEC_INIT[6] =
+        { sqrt(7./90), sqrt(11./90), sqrt(13./90), sqrt(17./90), sqrt(19./90), sqrt(23./90) };
+    double vT0 = NAN, vTimeBegin = NAN, vTimeStep = NAN, addLogDevVec = 0;
+    long devSize = 0;
+    struct FnLin : public LinSys2 {
+        std::function<void(double, double*, double*)> f;
+        virtual void evalMat(double t, double a[9], double b[9]) const { f(t, a, b); }
+    } vsys;
+    vsys.f = [&](double timeOffset, double* a, double* b) {
+        double pt[6], g[3], h[6];
+        getSolAt(timeOffset, vTimeBegin, pt);
+        pot.evalCarHess(pt, NULL, g, h, vTimeBegin + timeOffset);
+        for(int i=0; i<9; i++) b[i] = 0;
+        double ca=1, sa=0;
+        if(Omega)
+            sincos_r(Omega * (vTimeBegin + timeOffset), sa, ca);
+        const double dx2 = h[0], dy2 = h[1], dz2 = h[2], dxdy = h[3], dydz = h[4], dxdz = h[5];
+        a[0] = -dx2  * (ca * ca) - dy2 * (sa * sa) + 2 * dxdy * (ca * sa);
+        a[1] = -dxdy * (ca * ca - sa * sa) - (dx2 - dy2) * (ca * sa);
+        a[2] = -dxdz *  ca + dydz *  sa;
+        a[3] = a[1];
+        a[4] = -dy2  * (ca * ca) - dx2 * (sa * sa) - 2 * dxdy * (ca * sa);
+        a[5] = -dydz *  ca - dxdz *  sa;
+        a[6] = a[2];
+        a[7] = a[5];
+        a[8] = -dz2;
+    };
+    Ode2StepperGL4 vsolver(vsys, numVectors);
+    auto logMagnitude = [](const double* x) {
+        double norm = 0.;
+        for(int i=0; i<6; i++) norm += pow_2(x[i]);
+        return 0.5 * log(norm);
+    };
+    // storeSolution (:201-254): write the vectors at row ind (ind < 0: none), return log|w|
+    auto storeSolution = [&](double timeOffset, long ind) {
+        double Q[36];
+        for(int i=0; i<numVectors*6; i++) Q[i] = vsolver.getSol(timeOffset, i);
+        const double mul = exp(addLogDevVec);
+        double fid[6] = {0, 0, 0, 0, 0, 0};
+        for(int vec=0; vec<numVectors; vec++) {
+            if(var->dev && ind >= 0) {
+                devSize = ind + 1;
+                if(ind < trajsize) {
+                    double ca=1, sa=0;
+                    if(Omega)
+                        sincos_r(Omega * (vTimeBegin + timeOffset), sa, ca);
+                    const double* q = Q + vec*6;
+                    double* o = var->dev + ((long)vec * trajsize + ind) * 6;
+                    o[0] = (q[0] * ca + q[1] * sa) * mul;
+                    o[1] = (q[1] * ca - q[0] * sa) * mul;
+                    o[2] = (q[2] * mul);
+                    o[3] = (q[3] * ca + q[4] * sa) * mul;
+                    o[4] = (q[4] * ca - q[3] * sa) * mul;
+                    o[5] = (q[5] * mul);
+                }
+            }
+            if(numVectors == 6)
+                for(int i=0; i<6; i++) fid[i] += Q[vec*6+i] * DEV_VEC_INIT[vec];
+        }
+        return logMagnitude(numVectors == 1 ? Q : fid);
+    };
+    auto renormalize = [&]() {   // :256-308 (rescale, no orthogonalisation)
+        double dv[36];
+        double lg = -INFINITY;
+        for(int vec=0; vec<numVectors; vec++) {
+            for(int d=0; d<6; d++) dv[vec*6+d] = vsolver.getSol(vTimeStep, vec*6+d);
+            lg = fmax(lg, logMagnitude(dv + vec*6));
+        }
+        const double mult = exp(-lg);
+        addLogDevVec += lg;
+        for(int i=0; i<numVectors*6; i++) dv[i] *= mult;
+        vsolver.init(dv);
+    };
+    auto varStep = [&](double tbegin, double timestep) {   // processTimestep (:121-199)
+        vTimeBegin = tbegin;
+        vTimeStep = timestep;
+        const double tend = tbegin + timestep;
+        if(vT0 != vT0) {
+            vT0 = tbegin;
+            if(numVectors == 6) {   // identity, rotated into the inertial frame at the start
+                double dv[36] = {0};
+                double ca, sa;
+                sincos_r(Omega * tbegin, sa, ca);
+                dv[0] = dv[7] = dv[21] = dv[28] = ca;
+                dv[1] = dv[22] = sa;
+                dv[6] = dv[27] = -sa;
+                dv[14] = dv[35] = 1;
+                vsolver.init(dv);
+                storeSolution(0, 0);
+            } else
+                vsolver.init(DEV_VEC_INIT);
+        }
+        vsolver.doStep(timestep);
+        if(var->dev && samplingInterval > 0 && samplingInterval < INFINITY) {
+            const double sign = timestep>=0 ? +1 : -1;
+            const double dtroundoff = ROUNDOFF * fmax(fmax(fabs(tend), fabs(tbegin)), fabs(vT0));
+            long iout = static_cast<long>(ceil( sign * (tbegin-vT0) / samplingInterval));
+            const long iend = static_cast<long>((sign * (tend-vT0) + dtroundoff) / samplingInterval);
+            iout = std::max<long>(1, iout);
+            for(; iout <= iend; iout++) {
+                const double x = (sign * iout * samplingInterval - (tbegin-vT0)) * sign, hi = sign * timestep;
+                const double timeOffset = (x > hi ? hi : x > 0.0 ? x : 0.0) * sign;
+                storeSolution(timeOffset, iout);
+            }
+        }
+        long iout = -1;
+        if(var->dev) {
+            if(samplingInterval == 0) iout = devSize;
+            else if(samplingInterval == INFINITY) iout = 0;
+        }
+        const double logDevVec = storeSolution(vTimeStep, iout);
+        var->logDev.push_back(std::make_pair(fabs(tend - vT0), logDevVec + addLogDevVec));
+        if(logDevVec > 10)
+            renormalize();
+    };
     // run (orbit.cpp:52-84)
     res.status = 0;
     long numSteps = 0;
     double timeEnd = sys.timeBegin + ttotal, timeStep = 0;
     if(ttotal==0 || !isFinite(timeEnd)) {
         processTimestep(sys.timeBegin, timeStep);
+        if(wantVar) varStep(sys.timeBegin, timeStep);
     } else {
         double sign = ttotal>0 ? +1 : -1;
         while(sys.timeBegin != timeEnd) {
@@ -2182,6 +2450,7 @@ void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart
             double de = fabs(energyOf(&stepper.state[0], sys.lastEpot, Omega) / E0 - 1);
             if(de > res.maxde || de != de) res.maxde = de;
             processTimestep(sys.timeBegin, timeStep);
+            if(wantVar) varStep(sys.timeBegin, timeStep);
             sys.timeBegin = timeStep == timeRemaining ? timeEnd : sys.timeBegin + timeStep;
             if(++numSteps >= maxNumSteps) {
                 res.status = 2;
@@ -2199,6 +2468,10 @@ void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart
         if(out_time) out_time[idx] = NAN;
         if(out_traj) for(int c=0; c<6; c++) out_traj[idx*6+c] = NAN;
     }
+    if(var && var->dev)
+        for(int vec=0; vec<6; vec++)
+            for(long idx = devSize; idx < trajsize; idx++)
+                for(int c=0; c<6; c++) var->dev[((long)vec * trajsize + idx) * 6 + c] = NAN;
     res.nsteps = numSteps;
     res.nattempts = stepper.nattempts;
     res.nevals = sys.nevals;
@@ -2206,14 +2479,14 @@ void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart
 
 void integrateOneOrbit(const orc_pot& pot, const double ic[6], double tstart, double ttotal,
     int trajsize, double accuracy, double Omega, long maxNumSteps,
-    double* out_time, double* out_traj, OrbitResult& res, int method=0)
+    double* out_time, double* out_traj, OrbitResult& res, int method=0, VarOutput* var=nullptr)
 {
     if(method == 1)
         integrateOneOrbitWith<DPRKN8>(pot, ic, tstart, ttotal, trajsize, accuracy, Omega, maxNumSteps,
-            out_time, out_traj, res);
+            out_time, out_traj, res, var);
     else
         integrateOneOrbitWith<DOP853>(pot, ic, tstart, ttotal, trajsize, accuracy, Omega, maxNumSteps,
-            out_time, out_traj, res);
+            out_time, out_traj, res, var);
 }
 
 struct CallbackSys : public OdeSys {
@@ -2387,6 +2660,41 @@ int orc_orbit_many_method(const orc_pot* p, long n, const double* ic, const doub
     return fail;
 }
 
+int orc_orbit_many_var(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, const long* traj_offset,
+    double accuracy, double Omega, long maxNumSteps,
+    double* out_time, double* out_traj, double* out_final, double* out_dev,
+    long* out_nsteps, long* out_maxlogdev, int nthreads, int method)
+{
+    g_err.clear();
+    for(long i=0; i<n; i++)
+        if(trajsize[i] < 1) { g_err = "trajsize must be >= 1"; return 1; }
+    int fail = 0;
+#ifdef _OPENMP
+    if(nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for(long orb=0; orb<n; orb++) {
+        OrbitResult r;
+        VarOutput v;
+        v.dev = out_dev ? out_dev + traj_offset[orb] * 36 : NULL;
+        try {
+            integrateOneOrbit(*p, &ic[orb*6], tstart ? tstart[orb] : 0, ttotal[orb],
+                trajsize[orb], accuracy, Omega, maxNumSteps,
+                out_time ? out_time + traj_offset[orb] : NULL,
+                out_traj ? out_traj + traj_offset[orb]*6 : NULL, r, method, &v);
+        } catch(std::exception&) {
+            fail = 1;
+            continue;
+        }
+        if(out_final) for(int c=0; c<6; c++) out_final[orb*6+c] = r.fin[c];
+        if(out_nsteps) out_nsteps[orb] = r.nsteps;
+        if(out_maxlogdev) out_maxlogdev[orb] = (long)v.logDev.size();
+    }
+    if(fail) g_err = "exception during orbit integration";
+    return fail;
+}
+
 int orc_ode_dop853(int ndim, orc_rhs_fn rhs, void* ctx, const double* x0, double ttotal,
     double accuracy, double* out_xend, int nsample, const double* tsample, double* out_sample,
     long* out_nsteps)
@@ -2409,6 +2717,33 @@ int orc_ode_dop853(int ndim, orc_rhs_fn rhs, void* ctx, const double* x0, double
     }
     for(int i=0; i<ndim; i++) out_xend[i] = st.getSol(st.prevTimeStep, i);
     if(out_nsteps) *out_nsteps = ns;
+    return 0;
+}
+
+namespace {
+struct CallbackLin : public LinSys2 {
+    orc_mat_fn fn; void* ctx; double tb;
+    CallbackLin(orc_mat_fn f, void* c) : fn(f), ctx(c), tb(0) {}
+    virtual void evalMat(double t, double a[9], double b[9]) const { fn(tb + t, a, b, ctx); }
+};
+}
+
+int orc_ode_gl4(int nvec, orc_mat_fn mat, void* ctx, const double* x0, int nsteps, const double* steps,
+    double* out_states, int nfrac, const double* frac, double* out_dense)
+{
+    CallbackLin sys(mat, ctx);
+    Ode2StepperGL4 st(sys, nvec);
+    st.init(x0);
+    double tb = 0;
+    for(int s=0; s<nsteps; s++) {
+        st.doStep(steps[s]);
+        for(int i=0; i<nvec*6; i++) out_states[(long)s*nvec*6+i] = st.getSol(steps[s], i);
+        for(int k=0; k<nfrac; k++)
+            for(int i=0; i<nvec*6; i++)
+                out_dense[((long)s*nfrac+k)*nvec*6+i] = st.getSol(steps[s] * frac[k], i);
+        tb += steps[s];
+        sys.tb = tb;
+    }
     return 0;
 }
 

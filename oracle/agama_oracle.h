@@ -45,6 +45,16 @@ long orc_multipole_tables(const orc_pot* p, int which, int m, double* out);
 void orc_eval_many(const orc_pot* p, long n, const double* pos, double t,
     double* phi, double* grad, int nthreads);
 
+/* orc_orbit_many_method plus the variational equation of agama.orbit(der=True)
+   (orbit_variational.cpp, RuntimeVariational with six deviation vectors): out_dev holds, per
+   orbit at row offset traj_offset*36, the six deviation vectors [vec][row][6] sampled like
+   the trajectory (NaN-padded); out_nlogdev the number of recorded log-magnitude points. */
+int orc_orbit_many_var(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, const long* traj_offset,
+    double accuracy, double Omega, long maxNumSteps,
+    double* out_time, double* out_traj, double* out_final, double* out_dev,
+    long* out_nsteps, long* out_nlogdev, int nthreads, int method);
+
 /* potential, gradient and Hessian {dx2, dy2, dz2, dxdy, dydz, dxdz} of Phi (not of the
    acceleration) at n points; any output may be NULL.  BasePotential::eval with deriv2
    (potential_base.h:153-156). */
@@ -79,6 +89,12 @@ int orc_orbit_many_method(const orc_pot* p, long n, const double* ic, const doub
    `nsample` requested absolute times (sorted). Fixed-step mode when accuracy is INFINITY is
    NOT special-cased (as in the reference, the error control then always accepts). */
 typedef void (*orc_rhs_fn)(double t, const double* x, double* dxdt, void* ctx);
+/* x'' = A(t) x + B(t) x' (3x3 row-major A, B written by the callback) */
+typedef void (*orc_mat_fn)(double t, double* a, double* b, void* ctx);
+/* Ode2StepperGL4<3> (math_ode.cpp:1017-1250) for nvec vectors x0[nvec*6] = {x, x'}: steps of
+   the given lengths; end state of every step and dense output at fractions frac[] of each step */
+int orc_ode_gl4(int nvec, orc_mat_fn mat, void* ctx, const double* x0, int nsteps, const double* steps,
+    double* out_states, int nfrac, const double* frac, double* out_dense);
 /* DPRKN8 (reference: src/math_ode.cpp:426-701) on x'' = acc(t, x): ndim = 2N (positions then
    velocities); the callback fills N accelerations from the first N entries of x; otherwise
    as orc_ode_dop853 */

@@ -43,6 +43,11 @@ def _load(name):
                                           _vp, ctypes.c_int, _vp, _vp, _vp]),
         "orc_ode_dop853": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                           _vp, ctypes.c_int, _vp, _vp, _vp]),
+        "orc_orbit_many_var": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp,
+                                              ctypes.c_int, ctypes.c_int]),
+        "orc_ode_gl4": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
+                                       _vp, _vp]),
         "orc_ode_dop853_fixed": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
                                                 _vp, _vp]),
         "orc_sincos": (None, [ctypes.c_double, _vp, _vp]),
@@ -77,6 +82,8 @@ def fast():
 
 
 RHS = ctypes.CFUNCTYPE(None, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                       ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
+MAT = ctypes.CFUNCTYPE(None, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                        ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
 
 
@@ -158,6 +165,34 @@ class OraclePotential:
         return out
 
 
+def _orbits_var(self, ic, ttotal, trajsize=1, tstart=None, accuracy=1e-8, Omega=0.0,
+                maxNumSteps=int(1e8), nthreads=0, method="dop853"):
+    """orbits with the six deviation vectors of agama.orbit(der=True): dict with time, traj,
+    final, dev (n, 6, S, 6) for equal trajsize, nsteps"""
+    ic = np.ascontiguousarray(ic, dtype=np.float64).reshape(-1, 6)
+    n = len(ic)
+    tt = np.ascontiguousarray(np.broadcast_to(np.asarray(ttotal, dtype=np.float64), (n,)))
+    sz = np.ascontiguousarray(np.broadcast_to(np.asarray(trajsize, dtype=np.int32), (n,)))
+    ts = None if tstart is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(tstart, dtype=np.float64), (n,)))
+    offs = np.concatenate([[0], np.cumsum(sz)[:-1]]).astype(np.int64)
+    rows = int(sz.sum())
+    out = dict(time=np.empty(rows), traj=np.empty((rows, 6)), final=np.empty((n, 6)),
+               dev=np.empty(rows * 36), nsteps=np.empty(n, np.int64), nlogdev=np.empty(n, np.int64))
+    m = {"dop853": 0, "dprkn8": 1}[method.lower()]
+    rc = self.lib.orc_orbit_many_var(self.h, n, _p(ic), _p(ts), _p(tt), _p(sz), _p(offs), accuracy, Omega,
+                                     maxNumSteps, _p(out["time"]), _p(out["traj"]), _p(out["final"]),
+                                     _p(out["dev"]), _p(out["nsteps"]), _p(out["nlogdev"]), nthreads, m)
+    if rc:
+        raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
+    if np.all(sz == sz[0]):
+        out["dev"] = out["dev"].reshape(n, 6, int(sz[0]), 6)
+    return out
+
+
+OraclePotential.orbits_var = _orbits_var
+
+
 def ref_lib():
     """the reference's own DOP853 (oracle/_ref), or None when it was not built"""
     path = os.path.join(HERE, "_ref", "libref_dop853.so")
@@ -172,4 +207,7 @@ def ref_lib():
     if hasattr(r, "ref_ode_dprkn8"):
         r.ref_ode_dprkn8.restype = ctypes.c_int
         r.ref_ode_dprkn8.argtypes = r.ref_ode_dop853.argtypes
+    if hasattr(r, "ref_ode_gl4"):
+        r.ref_ode_gl4.restype = ctypes.c_int
+        r.ref_ode_gl4.argtypes = [ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, _vp]
     return r

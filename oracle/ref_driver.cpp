@@ -26,9 +26,41 @@ public:
     { (void)d3; (void)af; fn(tb + t, xv, d2xdt2, ctx); }
     virtual unsigned int size() const { return n; }
 };
+
+typedef void (*ref_mat_fn)(double t, double* a, double* b, void* ctx);
+
+class CallbackLinear : public math::IOdeSystem2ndOrderLinear {
+public:
+    ref_mat_fn fn; void* ctx; double tb;
+    CallbackLinear(ref_mat_fn _fn, void* _ctx) : fn(_fn), ctx(_ctx), tb(0) {}
+    virtual void evalMat(const double t, double a[], double b[]) const { fn(tb + t, a, b, ctx); }
+    virtual unsigned int size() const { return 6; }
+};
 }
 
 extern "C" {
+
+/* the reference's Ode2StepperGL4<3> on x'' = A(t) x + B(t) x' for nvec vectors (x0: nvec*6
+   values, each {x, x'}); steps of the given lengths; records the end state of every step and
+   the dense output at the fractions frac[0..nfrac) of every step */
+int ref_ode_gl4(int nvec, ref_mat_fn mat, void* ctx, const double* x0, int nsteps, const double* steps,
+    double* out_states, int nfrac, const double* frac, double* out_dense)
+{
+    CallbackLinear sys(mat, ctx);
+    math::Ode2StepperGL4<3> st(sys, nvec);
+    st.init(x0);
+    double tb = 0;
+    for(int s=0; s<nsteps; s++) {
+        st.doStep(steps[s]);
+        for(int i=0; i<nvec*6; i++) out_states[(long)s*nvec*6+i] = st.getSol(steps[s], i);
+        for(int k=0; k<nfrac; k++)
+            for(int i=0; i<nvec*6; i++)
+                out_dense[((long)s*nfrac+k)*nvec*6+i] = st.getSol(steps[s] * frac[k], i);
+        tb += steps[s];
+        sys.tb = tb;
+    }
+    return 0;
+}
 
 /* adaptive integration over [0, ttotal]; dense output at sorted tsample[]; end state */
 int ref_ode_dop853(int ndim, ref_rhs_fn rhs, void* ctx, const double* x0, double ttotal,
