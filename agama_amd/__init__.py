@@ -205,7 +205,7 @@ class Potential:
 
 def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, accuracy=1e-8,
           maxNumSteps=None, Omega=0.0, dtype="float32", separateTime=False, method="dop853",
-          verbose=False, info=None, precision=PRECISION_FP64, max_rel_de=1e-6):
+          verbose=False, info=None, precision=PRECISION_FP64, max_rel_de=1e-6, der=False):
     """Integrate orbits with DOP853 or DPRKN8 (method=) on the GPU - reference: agama.orbit
     (src/interface_python.cpp:7689-8281, output conventions :7941-7961).
 
@@ -218,7 +218,12 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
     `potential` may also be a `Surrogate`: orbits are then integrated with the MLP surrogate
     force while their energy error (judged by the FP64 Multipole) stays within `max_rel_de`
     and they stay within the training radii, and otherwise integrated again with the FP64
-    Multipole (status + 8; info["n_fallback"] counts them)."""
+    Multipole (status + 8; info["n_fallback"] counts them).
+
+    der=True also integrates the variational equation (src/orbit_variational.cpp) and returns
+    the six deviation vectors sampled like the trajectory, as an extra item: an array
+    (N, 6, trajsize, 6) with separateTime=True and equal trajsize, otherwise an (N, 6) object
+    array of (trajsize, 6) arrays (interface_python.cpp:8035-8047, 8205-8232)."""
     surrogate = isinstance(potential, Surrogate)
     if not surrogate and not isinstance(potential, Potential):
         raise TypeError("Argument 'potential' must be a valid Potential object")
@@ -232,6 +237,14 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
     if surrogate and mname != "dop853":
         raise AgamaAmdError("the surrogate path integrates with method='dop853' only")
     method_flag = METHOD_DPRKN8 if mname == "dprkn8" else 0
+    if der not in (False, True, 0, 1):
+        raise TypeError("Argument 'der' must be a boolean or an int 0/1")
+    der = bool(der)
+    if der and trajsize is None:
+        raise RuntimeError("Derivatives are requested, but the trajectory itself is not: provide "
+                           "trajsize=... for array output or dtype=object for Orbit object output")
+    if der and surrogate:
+        raise AgamaAmdError("der=True is integrated with the FP64 Multipole only")
     icarr = np.ascontiguousarray(ic, dtype=np.float64)
     if not (icarr.shape == (6,) or (icarr.ndim == 2 and icarr.shape[1] == 6)):
         raise TypeError("Argument 'ic' does not contain a valid array of length Nx6")
@@ -283,6 +296,12 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
                                             float(Omega), maxsteps, float(max_rel_de), ptr(times), ptr(traj),
                                             ptr(final), ptr(nsteps), ptr(natt), ptr(nev), ptr(maxde), ptr(status),
                                             ptr(nfb)), "surrogate orbit")
+    elif der:
+        dev = np.empty(int(sz.sum()) * 36)
+        check(lib.agx_orbit_var_batch(potential.handle, n, ptr(ic2), ptr(ts), ptr(tt), ptr(sz), float(accuracy),
+                                      float(Omega), maxsteps, ptr(times), ptr(traj), ptr(final), ptr(dev),
+                                      ptr(nsteps), ptr(natt), ptr(nev), ptr(maxde), ptr(status),
+                                      int(precision) | method_flag), "orbit")
     else:
         check(lib.agx_orbit_batch(potential.handle, n, ptr(ic2), ptr(ts), ptr(tt), ptr(sz), float(accuracy),
                                   float(Omega), maxsteps, ptr(times), ptr(traj), ptr(final), ptr(nsteps),
@@ -305,17 +324,31 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
         S = int(sz[0])
         t_out = times.reshape(n, S)
         x_out = conv(traj.reshape(n, S, 6))
+        if der:
+            d_out = conv(dev.reshape(n, 6, S, 6))
+            return (t_out[0], x_out[0], d_out[0]) if single else (t_out, x_out, d_out)
         return (t_out[0], x_out[0]) if single else (t_out, x_out)
     parts = [(times[offs[i]:offs[i + 1]], conv(traj[offs[i]:offs[i + 1]])) for i in range(n)]
+    darr = None
+    if der:   # (N, 6) object array of (trajsize, 6) arrays
+        darr = np.empty((n, 6), dtype=object)
+        for i in range(n):
+            block = dev[offs[i] * 36:offs[i + 1] * 36].reshape(6, sz[i], 6)
+            for k in range(6):
+                darr[i, k] = conv(block[k])
     if separateTime:
         tarr = np.empty(n, dtype=object)
         xarr = np.empty(n, dtype=object)
         for i, (a, b) in enumerate(parts):
             tarr[i], xarr[i] = a, b
+        if der:
+            return (tarr[0], xarr[0], darr[0]) if single else (tarr, xarr, darr)
         return (tarr[0], xarr[0]) if single else (tarr, xarr)
     res = np.empty((n, 2), dtype=object)
     for i, (a, b) in enumerate(parts):
         res[i, 0], res[i, 1] = a, b
+    if der:
+        return (res[0], darr[0]) if single else (res, darr)
     return res[0] if single else res
 
 

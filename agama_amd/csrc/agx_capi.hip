@@ -25,6 +25,10 @@ int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves, bool nfw);
 hipError_t agxLaunchOrbitRKN(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision,
     const AgxOrbitArgs& A, int numBlocks, hipStream_t stream);
 int agxOrbitKernelRKNBlocksPerCU(int precision, bool nfw);
+int agxOrbitVarKernelBlocksPerCU(bool nfw);
+size_t agxOrbitVarStateDoubles(int numBlocks);
+hipError_t agxLaunchOrbitVar(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, const AgxOrbitArgs& A,
+    int numBlocks, double* varState, double* dev, hipStream_t stream);
 hipError_t agxLaunchFillU32(unsigned int* p, long n, unsigned int v, hipStream_t stream);
 hipError_t agxLaunchEvalHess(const AgxMultipoleDev* dP, long n, const double* pos, double* phi,
     double* grad, double* hess, hipStream_t stream);
@@ -433,10 +437,13 @@ int agx_get_orbit_config(int* num_blocks, int* fetch_eval, int* fetch_orbit, int
     return AGX_OK;
 }
 
-int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
+}  // extern "C"
+
+// the batch entry points' common body; d_dev non-null: the variational kernel (der=True)
+static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic,
     const double* d_tstart, const double* d_ttotal, const int* d_trajsize,
     const long* d_traj_offset, const int* d_order, double accuracy, double Omega,
-    long maxNumSteps, double* d_time, double* d_traj, double* d_final, long* d_nsteps,
+    long maxNumSteps, double* d_time, double* d_traj, double* d_final, double* d_dev, long* d_nsteps,
     long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags, void* stream)
 {
     if(!pot) return fail(AGX_EINVAL, "NULL potential");
@@ -451,6 +458,9 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
     if((precision != AGX_PRECISION_FP64 && precision != AGX_PRECISION_FP32) ||
         (flags & ~(AGX_PRECISION_MASK | AGX_ORDER_AS_GIVEN | AGX_METHOD_DPRKN8)))
         return fail(AGX_EINVAL, "invalid flags");
+    if(d_dev && (precision != AGX_PRECISION_FP64 || (flags & AGX_METHOD_DPRKN8)))
+        return fail(AGX_EUNSUPPORTED, "deviation vectors are integrated with DOP853 in FP64 only");
+    if(d_dev && !d_traj_offset) return fail(AGX_EINVAL, "deviation vectors need trajectory offsets");
     try {
         hipStream_t st = (hipStream_t)stream;
         void* work = nullptr;
@@ -476,11 +486,51 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
         int blocks, bs;
         orbitGeometry(n, precision, pot->nfwMass != 0, blocks, bs, flags & AGX_METHOD_DPRKN8);
         HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
-        hipError_t e = launchOrbitMethod(pot, precision, flags, A, blocks, bs, st);
+        hipError_t e;
+        if(d_dev) {
+            // the kernel's LDS allows one block per CU; per-lane variational state in a
+            // stream-ordered workspace
+            int dev = 0, cus = 0;
+            HIPCHECK(hipGetDevice(&dev));
+            HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            int vb = cus * std::max(1, agxOrbitVarKernelBlocksPerCU(pot->nfwMass != 0));
+            const long need = (n + bs - 1) / bs;
+            if(need < vb) vb = (int)std::max<long>(need, 1);
+            void* vstate = nullptr;
+            HIPCHECK(hipMallocAsync(&vstate, agxOrbitVarStateDoubles(vb) * sizeof(double), st));
+            e = agxLaunchOrbitVar(pot->dev, pot->ddev, A, vb, static_cast<double*>(vstate), d_dev, st);
+            HIPCHECK(hipFreeAsync(vstate, st));
+        } else
+            e = launchOrbitMethod(pot, precision, flags, A, blocks, bs, st);
         if(work) HIPCHECK(hipFreeAsync(work, st));
         if(e != hipSuccess) return fail(AGX_EDEVICE, std::string("orbit launch: ") + hipGetErrorString(e));
         return AGX_OK;
     } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+extern "C" {
+
+int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
+    const double* d_tstart, const double* d_ttotal, const int* d_trajsize,
+    const long* d_traj_offset, const int* d_order, double accuracy, double Omega,
+    long maxNumSteps, double* d_time, double* d_traj, double* d_final, long* d_nsteps,
+    long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags, void* stream)
+{
+    return orbitBatchDevice(pot, n, d_ic, d_tstart, d_ttotal, d_trajsize, d_traj_offset, d_order, accuracy,
+        Omega, maxNumSteps, d_time, d_traj, d_final, nullptr, d_nsteps, d_nattempts, d_nevals, d_maxde,
+        d_status, flags, stream);
+}
+
+int agx_orbit_var_batch_device(const agx_potential* pot, long n, const double* d_ic,
+    const double* d_tstart, const double* d_ttotal, const int* d_trajsize,
+    const long* d_traj_offset, const int* d_order, double accuracy, double Omega,
+    long maxNumSteps, double* d_time, double* d_traj, double* d_final, double* d_dev, long* d_nsteps,
+    long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags, void* stream)
+{
+    if(!d_dev) return fail(AGX_EINVAL, "NULL deviation-vector output");
+    return orbitBatchDevice(pot, n, d_ic, d_tstart, d_ttotal, d_trajsize, d_traj_offset, d_order, accuracy,
+        Omega, maxNumSteps, d_time, d_traj, d_final, d_dev, d_nsteps, d_nattempts, d_nevals, d_maxde,
+        d_status, flags, stream);
 }
 
 }  // extern "C"
@@ -561,9 +611,11 @@ int agx_orbit_graph_launch(agx_orbit_graph* g, void* stream)
 
 void agx_orbit_graph_destroy(agx_orbit_graph* g) { delete g; }
 
-int agx_orbit_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
+}  // extern "C"
+
+static int orbitBatchHost(const agx_potential* pot, long n, const double* ic, const double* tstart,
     const double* ttotal, const int* trajsize, double accuracy, double Omega, long maxNumSteps,
-    double* out_time, double* out_traj, double* out_final, long* out_nsteps,
+    double* out_time, double* out_traj, double* out_final, double* out_dev, long* out_nsteps,
     long* out_nattempts, long* out_nevals, double* out_maxde, int* out_status, int flags)
 {
     g_err.clear();
@@ -581,7 +633,8 @@ int agx_orbit_batch(const agx_potential* pot, long n, const double* ic, const do
     try {
         HIPCHECK(hipSetDevice(pot->device));
         DevBuf<double> dic(n * 6), dts(tstart ? n : 0), dtt(n), dtime(out_time ? rows : 0),
-            dtraj(out_traj ? rows * 6 : 0), dfin(out_final ? n * 6 : 0), dmde(out_maxde ? n : 0);
+            dtraj(out_traj ? rows * 6 : 0), dfin(out_final ? n * 6 : 0), dmde(out_maxde ? n : 0),
+            ddev(out_dev ? rows * 36 : 0);
         DevBuf<int> dsz(n), dst(out_status ? n : 0);
         DevBuf<long> doff(n), dns(out_nsteps ? n : 0), dna(out_nattempts ? n : 0), dne(out_nevals ? n : 0);
         HIPCHECK(hipMemcpy(dic.p, ic, n * 48, hipMemcpyHostToDevice));
@@ -589,8 +642,8 @@ int agx_orbit_batch(const agx_potential* pot, long n, const double* ic, const do
         HIPCHECK(hipMemcpy(dtt.p, ttotal, n * 8, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(dsz.p, trajsize, n * 4, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(doff.p, offs.data(), n * 8, hipMemcpyHostToDevice));
-        int rc = agx_orbit_batch_device(pot, n, dic.p, dts.p, dtt.p, dsz.p, doff.p, nullptr,
-            accuracy, Omega, maxNumSteps, dtime.p, dtraj.p, dfin.p, dns.p, dna.p, dne.p, dmde.p,
+        int rc = orbitBatchDevice(pot, n, dic.p, dts.p, dtt.p, dsz.p, doff.p, nullptr,
+            accuracy, Omega, maxNumSteps, dtime.p, dtraj.p, dfin.p, ddev.p, dns.p, dna.p, dne.p, dmde.p,
             dst.p, flags, nullptr);
         if(rc) return rc;
         HIPCHECK(hipDeviceSynchronize());
@@ -602,8 +655,30 @@ int agx_orbit_batch(const agx_potential* pot, long n, const double* ic, const do
         if(out_nevals) HIPCHECK(hipMemcpy(out_nevals, dne.p, n * 8, hipMemcpyDeviceToHost));
         if(out_maxde) HIPCHECK(hipMemcpy(out_maxde, dmde.p, n * 8, hipMemcpyDeviceToHost));
         if(out_status) HIPCHECK(hipMemcpy(out_status, dst.p, n * 4, hipMemcpyDeviceToHost));
+        if(out_dev) HIPCHECK(hipMemcpy(out_dev, ddev.p, rows * 36 * 8, hipMemcpyDeviceToHost));
         return AGX_OK;
     } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+extern "C" {
+
+int agx_orbit_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, double accuracy, double Omega, long maxNumSteps,
+    double* out_time, double* out_traj, double* out_final, long* out_nsteps,
+    long* out_nattempts, long* out_nevals, double* out_maxde, int* out_status, int flags)
+{
+    return orbitBatchHost(pot, n, ic, tstart, ttotal, trajsize, accuracy, Omega, maxNumSteps, out_time,
+        out_traj, out_final, nullptr, out_nsteps, out_nattempts, out_nevals, out_maxde, out_status, flags);
+}
+
+int agx_orbit_var_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, double accuracy, double Omega, long maxNumSteps,
+    double* out_time, double* out_traj, double* out_final, double* out_dev, long* out_nsteps,
+    long* out_nattempts, long* out_nevals, double* out_maxde, int* out_status, int flags)
+{
+    if(!out_dev) return fail(AGX_EINVAL, "NULL deviation-vector output");
+    return orbitBatchHost(pot, n, ic, tstart, ttotal, trajsize, accuracy, Omega, maxNumSteps, out_time,
+        out_traj, out_final, out_dev, out_nsteps, out_nattempts, out_nevals, out_maxde, out_status, flags);
 }
 
 }  // extern "C"

@@ -142,6 +142,24 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
     long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags,
     void* stream);
 
+/* agx_orbit_batch[_device] with the variational equation: the six deviation vectors of
+   agama.orbit(der=True) (src/orbit_variational.cpp RuntimeVariational, six vectors started from
+   the identity and advanced with Ode2StepperGL4<3>, src/math_ode.cpp:1017-1250), sampled at
+   the trajectory's times: out_dev holds, per orbit at row offset sum_{j<i} trajsize[j] * 36,
+   [vector][row][6] in the rotating frame, NaN-padded like the trajectory.  Vector k is the
+   derivative of the phase-space point with respect to initial coordinate k.  DOP853 and FP64
+   only (other flags return AGX_EUNSUPPORTED).  The device form allocates its per-lane state
+   (1.2 KB per resident lane) stream-ordered on `stream`. */
+int agx_orbit_var_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, double accuracy, double Omega, long maxNumSteps,
+    double* out_time, double* out_traj, double* out_final, double* out_dev, long* out_nsteps,
+    long* out_nattempts, long* out_nevals, double* out_maxde, int* out_status, int flags);
+int agx_orbit_var_batch_device(const agx_potential* pot, long n, const double* d_ic,
+    const double* d_tstart, const double* d_ttotal, const int* d_trajsize,
+    const long* d_traj_offset, const int* d_order, double accuracy, double Omega,
+    long maxNumSteps, double* d_time, double* d_traj, double* d_final, double* d_dev, long* d_nsteps,
+    long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags, void* stream);
+
 /* Launch geometry of the orbit kernel: number of workgroups (0 = one full wave of resident
    workgroups on every CU) and threads per workgroup (0 or 256, the only size it is built for).
    Set by the autotuner (agama_amd.tune). */
