@@ -205,7 +205,7 @@ class Potential:
 
 def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, accuracy=1e-8,
           maxNumSteps=None, Omega=0.0, dtype="float32", separateTime=False, method="dop853",
-          verbose=False, info=None, precision=PRECISION_FP64, max_rel_de=1e-6, der=False):
+          verbose=False, info=None, precision=PRECISION_FP64, max_rel_de=1e-6, der=False, lyapunov=False):
     """Integrate orbits with DOP853 or DPRKN8 (method=) on the GPU - reference: agama.orbit
     (src/interface_python.cpp:7689-8281, output conventions :7941-7961).
 
@@ -223,7 +223,10 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
     der=True also integrates the variational equation (src/orbit_variational.cpp) and returns
     the six deviation vectors sampled like the trajectory, as an extra item: an array
     (N, 6, trajsize, 6) with separateTime=True and equal trajsize, otherwise an (N, 6) object
-    array of (trajsize, 6) arrays (interface_python.cpp:8035-8047, 8205-8232)."""
+    array of (trajsize, 6) arrays (interface_python.cpp:8035-8047, 8205-8232).
+    lyapunov=True adds an (N, 2) array: the Lyapunov exponent in units of the orbital period
+    T_circ(E) and the chaos onset time in the same unit (0 and inf for regular orbits,
+    orbit_variational.cpp:360-448) as the last item."""
     surrogate = isinstance(potential, Surrogate)
     if not surrogate and not isinstance(potential, Potential):
         raise TypeError("Argument 'potential' must be a valid Potential object")
@@ -245,6 +248,11 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
                            "trajsize=... for array output or dtype=object for Orbit object output")
     if der and surrogate:
         raise AgamaAmdError("der=True is integrated with the FP64 Multipole only")
+    if lyapunov not in (False, True, 0, 1):
+        raise TypeError("Argument 'lyapunov' must be a boolean")
+    lyapunov = bool(lyapunov)
+    if lyapunov and surrogate:
+        raise AgamaAmdError("lyapunov=True is integrated with the FP64 Multipole only")
     icarr = np.ascontiguousarray(ic, dtype=np.float64)
     if not (icarr.shape == (6,) or (icarr.ndim == 2 and icarr.shape[1] == 6)):
         raise TypeError("Argument 'ic' does not contain a valid array of length Nx6")
@@ -296,11 +304,12 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
                                             float(Omega), maxsteps, float(max_rel_de), ptr(times), ptr(traj),
                                             ptr(final), ptr(nsteps), ptr(natt), ptr(nev), ptr(maxde), ptr(status),
                                             ptr(nfb)), "surrogate orbit")
-    elif der:
-        dev = np.empty(int(sz.sum()) * 36)
+    elif der or lyapunov:
+        dev = np.empty(int(sz.sum()) * 36) if der else None
+        lyap = np.empty((n, 2)) if lyapunov else None
         check(lib.agx_orbit_var_batch(potential.handle, n, ptr(ic2), ptr(ts), ptr(tt), ptr(sz), float(accuracy),
                                       float(Omega), maxsteps, ptr(times), ptr(traj), ptr(final), ptr(dev),
-                                      ptr(nsteps), ptr(natt), ptr(nev), ptr(maxde), ptr(status),
+                                      ptr(lyap), ptr(nsteps), ptr(natt), ptr(nev), ptr(maxde), ptr(status),
                                       int(precision) | method_flag), "orbit")
     else:
         check(lib.agx_orbit_batch(potential.handle, n, ptr(ic2), ptr(ts), ptr(tt), ptr(sz), float(accuracy),
@@ -312,7 +321,19 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
         if surrogate:
             info["n_fallback"] = int(nfb[0])
     if not have_traj:
+        if lyapunov:
+            return lyap[0] if single else lyap
         return final[0] if single else final
+    out = _trajectory_outputs(n, single, sz, times, traj, dev if der else None, out_dtype, separateTime)
+    if lyapunov:
+        out = (out if isinstance(out, tuple) else (out,)) + ((lyap[0] if single else lyap),)
+    return out
+
+
+def _trajectory_outputs(n, single, sz, times, traj, dev, out_dtype, separateTime):
+    """the trajectory (and deviation-vector) outputs in the reference's layouts
+    (interface_python.cpp:7995-8047)"""
+    der = dev is not None
 
     def conv(a):
         if out_dtype in (np.complex64, np.complex128):

@@ -47,10 +47,11 @@ def _load():
                                                 ctypes.c_int, _vp]),
         "agx_orbit_var_batch": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp,
-                                               _vp, _vp, _vp, ctypes.c_int]),
+                                               _vp, _vp, _vp, _vp, ctypes.c_int]),
         "agx_orbit_var_batch_device": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp,
                                                       ctypes.c_double, ctypes.c_double, ctypes.c_long, _vp, _vp,
                                                       _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+        "agx_lyapunov_fit": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp]),
         "agx_eval_hess_many": (ctypes.c_int, [_vp, ctypes.c_long, _vp, ctypes.c_double, _vp, _vp, _vp,
                                               ctypes.c_int]),
         "agx_eval_hess_many_device": (ctypes.c_int, [_vp, ctypes.c_long, _vp, ctypes.c_double, _vp, _vp, _vp,
@@ -107,7 +108,7 @@ EXPORTED = ["agx_last_error", "agx_potential_create", "agx_potential_from_coefs"
             "agx_potential_destroy", "agx_potential_write", "agx_build_coef_file",
             "agx_debug_tables", "agx_potential_coefs",
             "agx_eval_potential", "agx_eval_many", "agx_eval_many_device", "agx_eval_hess_many",
-            "agx_eval_hess_many_device", "agx_orbit_var_batch", "agx_orbit_var_batch_device", "agx_orbit_batch",
+            "agx_eval_hess_many_device", "agx_orbit_var_batch", "agx_orbit_var_batch_device", "agx_lyapunov_fit", "agx_orbit_batch",
             "agx_orbit_batch_device", "agx_set_orbit_launch", "agx_set_fetch", "iniacceleration",
             "agx_surrogate_train", "agx_surrogate_create", "agx_surrogate_destroy",
             "agx_surrogate_info", "agx_surrogate_loss_history", "agx_surrogate_eval_many",

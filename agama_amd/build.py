@@ -25,9 +25,10 @@ def build(verbose_resources=False):
     os.makedirs(BUILD, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     objs = []
-    o = os.path.join(BUILD, "host_multipole.o")
-    _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, "host_multipole.cpp"), "-o", o])
-    objs.append(o)
+    for src in ("host_multipole.cpp", "host_lyapunov.cpp"):
+        o = os.path.join(BUILD, src.replace(".cpp", ".o"))
+        _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
     for src in ("agx_kernels.hip", "agx_orbit_rkn.hip", "agx_hessian.hip", "agx_orbit_var.hip", "agx_schedule.hip",
                 "agx_surrogate.hip", "agx_orbit_surrogate.hip", "agx_capi.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))

@@ -16,6 +16,7 @@
 #include "agx_kernels.h"
 #include "agx_surrogate.h"
 #include "host_multipole.h"
+#include "host_lyapunov.h"
 
 hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
     long n, const double* pos, double* phi, double* grad, hipStream_t stream);
@@ -28,7 +29,10 @@ int agxOrbitKernelRKNBlocksPerCU(int precision, bool nfw);
 int agxOrbitVarKernelBlocksPerCU(bool nfw);
 size_t agxOrbitVarStateDoubles(int numBlocks);
 hipError_t agxLaunchOrbitVar(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, const AgxOrbitArgs& A,
-    int numBlocks, double* varState, double* dev, hipStream_t stream);
+    int numBlocks, double* varState, double* dev, const double* period, const double* maxTime, double* bins,
+    int* nbins, double* lastTime, int maxBins, hipStream_t stream);
+hipError_t agxLaunchTcirc(const AgxMultipoleDev* dP, long n, const double* ic, const double* tstart,
+    double Omega, double* period, hipStream_t stream);
 hipError_t agxLaunchFillU32(unsigned int* p, long n, unsigned int v, hipStream_t stream);
 hipError_t agxLaunchEvalHess(const AgxMultipoleDev* dP, long n, const double* pos, double* phi,
     double* grad, double* hess, hipStream_t stream);
@@ -94,7 +98,7 @@ void upload(agx_potential& p)
     HIPCHECK(hipMemset(base + oC, 0, 64));
     AgxMultipoleDev& D = p.dev;
     memset(&D, 0, sizeof(D));
-    D.nr = T.nr; D.nt = T.nt; D.ns = T.ns; D.mmax = T.mmax; D.mmin = T.mmin;
+    D.nr = T.nr; D.nt = T.nt; D.ns = T.ns; D.mmax = T.mmax; D.mmin = T.mmin; D.sym = T.sym;
     D.logScaling = T.logScaling; D.invPhi0 = T.invPhi0;
     D.rsqInner = T.rsqInner; D.rsqOuter = T.rsqOuter;
     for(int s = 0; s < T.ns; s++) D.slotM[s] = T.slotM[s];
@@ -439,12 +443,24 @@ int agx_get_orbit_config(int* num_blocks, int* fetch_eval, int* fetch_orbit, int
 
 }  // extern "C"
 
-// the batch entry points' common body; d_dev non-null: the variational kernel (der=True)
+// Lyapunov bookkeeping buffers of the variational kernel (device pointers; bins null: none)
+struct LyapDev {
+    const double* period = nullptr;
+    const double* maxTime = nullptr;
+    double* bins = nullptr;
+    int* nbins = nullptr;
+    double* lastTime = nullptr;
+    int maxBins = 0;
+};
+
+// the batch entry points' common body; d_dev non-null or ly.bins non-null: the variational
+// kernel (der=True: six vectors; lyapunov only: one)
 static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic,
     const double* d_tstart, const double* d_ttotal, const int* d_trajsize,
     const long* d_traj_offset, const int* d_order, double accuracy, double Omega,
     long maxNumSteps, double* d_time, double* d_traj, double* d_final, double* d_dev, long* d_nsteps,
-    long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags, void* stream)
+    long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags, void* stream,
+    const LyapDev& ly = LyapDev())
 {
     if(!pot) return fail(AGX_EINVAL, "NULL potential");
     if(n == 0) return AGX_OK;
@@ -458,7 +474,8 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
     if((precision != AGX_PRECISION_FP64 && precision != AGX_PRECISION_FP32) ||
         (flags & ~(AGX_PRECISION_MASK | AGX_ORDER_AS_GIVEN | AGX_METHOD_DPRKN8)))
         return fail(AGX_EINVAL, "invalid flags");
-    if(d_dev && (precision != AGX_PRECISION_FP64 || (flags & AGX_METHOD_DPRKN8)))
+    const bool variational = d_dev || ly.bins;
+    if(variational && (precision != AGX_PRECISION_FP64 || (flags & AGX_METHOD_DPRKN8)))
         return fail(AGX_EUNSUPPORTED, "deviation vectors are integrated with DOP853 in FP64 only");
     if(d_dev && !d_traj_offset) return fail(AGX_EINVAL, "deviation vectors need trajectory offsets");
     try {
@@ -487,7 +504,7 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
         orbitGeometry(n, precision, pot->nfwMass != 0, blocks, bs, flags & AGX_METHOD_DPRKN8);
         HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
         hipError_t e;
-        if(d_dev) {
+        if(variational) {
             // the kernel's LDS allows one block per CU; per-lane variational state in a
             // stream-ordered workspace
             int dev = 0, cus = 0;
@@ -498,7 +515,8 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
             if(need < vb) vb = (int)std::max<long>(need, 1);
             void* vstate = nullptr;
             HIPCHECK(hipMallocAsync(&vstate, agxOrbitVarStateDoubles(vb) * sizeof(double), st));
-            e = agxLaunchOrbitVar(pot->dev, pot->ddev, A, vb, static_cast<double*>(vstate), d_dev, st);
+            e = agxLaunchOrbitVar(pot->dev, pot->ddev, A, vb, static_cast<double*>(vstate), d_dev, ly.period,
+                ly.maxTime, ly.bins, ly.nbins, ly.lastTime, ly.maxBins, st);
             HIPCHECK(hipFreeAsync(vstate, st));
         } else
             e = launchOrbitMethod(pot, precision, flags, A, blocks, bs, st);
@@ -616,7 +634,8 @@ void agx_orbit_graph_destroy(agx_orbit_graph* g) { delete g; }
 static int orbitBatchHost(const agx_potential* pot, long n, const double* ic, const double* tstart,
     const double* ttotal, const int* trajsize, double accuracy, double Omega, long maxNumSteps,
     double* out_time, double* out_traj, double* out_final, double* out_dev, long* out_nsteps,
-    long* out_nattempts, long* out_nevals, double* out_maxde, int* out_status, int flags)
+    long* out_nattempts, long* out_nevals, double* out_maxde, int* out_status, int flags,
+    double* out_lyap = nullptr, const double* maxTimeIn = nullptr)
 {
     g_err.clear();
     if(!pot) return fail(AGX_EINVAL, "NULL potential");
@@ -635,18 +654,76 @@ static int orbitBatchHost(const agx_potential* pot, long n, const double* ic, co
         DevBuf<double> dic(n * 6), dts(tstart ? n : 0), dtt(n), dtime(out_time ? rows : 0),
             dtraj(out_traj ? rows * 6 : 0), dfin(out_final ? n * 6 : 0), dmde(out_maxde ? n : 0),
             ddev(out_dev ? rows * 36 : 0);
-        DevBuf<int> dsz(n), dst(out_status ? n : 0);
+        const bool lyap = out_lyap != nullptr;
+        const int MAXB = 128;   // bins span at least 1% of the duration: at most 101 of them
+        DevBuf<double> dper(lyap ? n : 0), dmaxt(lyap ? n : 0), dbins(lyap ? n * 2 * MAXB : 0),
+            dlast(lyap ? n : 0);
+        DevBuf<int> dnb(lyap ? n : 0);
+        const bool needStatus = out_status || lyap;
+        DevBuf<int> dsz(n), dst(needStatus ? n : 0);
         DevBuf<long> doff(n), dns(out_nsteps ? n : 0), dna(out_nattempts ? n : 0), dne(out_nevals ? n : 0);
         HIPCHECK(hipMemcpy(dic.p, ic, n * 48, hipMemcpyHostToDevice));
         if(tstart) HIPCHECK(hipMemcpy(dts.p, tstart, n * 8, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(dtt.p, ttotal, n * 8, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(dsz.p, trajsize, n * 4, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(doff.p, offs.data(), n * 8, hipMemcpyHostToDevice));
+        LyapDev ly;
+        if(lyap) {
+            // bin width from the requested duration (or the caller's); orbit periods on the device
+            std::vector<double> mt(n);
+            for(long i = 0; i < n; i++) mt[i] = maxTimeIn ? maxTimeIn[i] : fabs(ttotal[i]);
+            HIPCHECK(hipMemcpy(dmaxt.p, mt.data(), n * 8, hipMemcpyHostToDevice));
+            HIPCHECK(agxLaunchTcirc(pot->ddev, n, dic.p, dts.p, Omega, dper.p, nullptr));
+            ly.period = dper.p; ly.maxTime = dmaxt.p; ly.bins = dbins.p; ly.nbins = dnb.p;
+            ly.lastTime = dlast.p; ly.maxBins = MAXB;
+        }
         int rc = orbitBatchDevice(pot, n, dic.p, dts.p, dtt.p, dsz.p, doff.p, nullptr,
             accuracy, Omega, maxNumSteps, dtime.p, dtraj.p, dfin.p, ddev.p, dns.p, dna.p, dne.p, dmde.p,
-            dst.p, flags, nullptr);
+            dst.p, flags, nullptr, ly);
         if(rc) return rc;
         HIPCHECK(hipDeviceSynchronize());
+        if(lyap) {
+            std::vector<double> per(n), bins((size_t)n * 2 * MAXB), last(n);
+            std::vector<int> nb(n), stat(n);
+            HIPCHECK(hipMemcpy(per.data(), dper.p, n * 8, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(bins.data(), dbins.p, (size_t)n * 2 * MAXB * 8, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(last.data(), dlast.p, n * 8, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(nb.data(), dnb.p, n * 4, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(stat.data(), dst.p, n * 4, hipMemcpyDeviceToHost));
+            // orbits that stopped early were binned with the requested duration: integrate them
+            // again with the duration they reached (the bin width is 1% of it)
+            std::vector<long> redo;
+            if(!maxTimeIn)
+                for(long i = 0; i < n; i++)
+                    if(stat[i] != 0 && last[i] == last[i] && last[i] != fabs(ttotal[i])) redo.push_back(i);
+            std::vector<double> redoLyap(redo.size() * 2);
+            if(!redo.empty()) {
+                const long m = (long)redo.size();
+                std::vector<double> ic2(m * 6), ts2(tstart ? m : 0), tt2(m), mt2(m);
+                std::vector<int> sz2(m, 1);
+                for(long j = 0; j < m; j++) {
+                    const long i = redo[j];
+                    for(int c = 0; c < 6; c++) ic2[j * 6 + c] = ic[i * 6 + c];
+                    if(tstart) ts2[j] = tstart[i];
+                    tt2[j] = ttotal[i];
+                    mt2[j] = last[i];
+                }
+                // the same form of the variational equation (six vectors when der=True)
+                std::vector<double> dev2(out_dev ? m * 36 : 0);
+                int rc2 = orbitBatchHost(pot, m, ic2.data(), tstart ? ts2.data() : nullptr, tt2.data(), sz2.data(),
+                    accuracy, Omega, maxNumSteps, nullptr, nullptr, nullptr, out_dev ? dev2.data() : nullptr,
+                    nullptr, nullptr, nullptr, nullptr, nullptr, flags, redoLyap.data(), mt2.data());
+                if(rc2) return rc2;
+            }
+            for(long i = 0; i < n; i++) {
+                const double* b = bins.data() + (size_t)i * 2 * MAXB;
+                agx::lyapunovFromBins(b, b + MAXB, std::min(nb[i], MAXB), last[i], per[i], out_lyap + i * 2);
+            }
+            for(size_t j = 0; j < redo.size(); j++) {
+                out_lyap[redo[j] * 2] = redoLyap[j * 2];
+                out_lyap[redo[j] * 2 + 1] = redoLyap[j * 2 + 1];
+            }
+        }
         if(out_time) HIPCHECK(hipMemcpy(out_time, dtime.p, rows * 8, hipMemcpyDeviceToHost));
         if(out_traj) HIPCHECK(hipMemcpy(out_traj, dtraj.p, rows * 48, hipMemcpyDeviceToHost));
         if(out_final) HIPCHECK(hipMemcpy(out_final, dfin.p, n * 48, hipMemcpyDeviceToHost));
@@ -671,14 +748,24 @@ int agx_orbit_batch(const agx_potential* pot, long n, const double* ic, const do
         out_traj, out_final, nullptr, out_nsteps, out_nattempts, out_nevals, out_maxde, out_status, flags);
 }
 
+int agx_lyapunov_fit(const double* t, const double* lnw, int nbins, double max_time, double period,
+    double* out)
+{
+    g_err.clear();
+    if(!out || (nbins > 0 && (!t || !lnw))) return fail(AGX_EINVAL, "NULL argument");
+    agx::lyapunovFromBins(t, lnw, nbins, max_time, period, out);
+    return AGX_OK;
+}
+
 int agx_orbit_var_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
     const double* ttotal, const int* trajsize, double accuracy, double Omega, long maxNumSteps,
-    double* out_time, double* out_traj, double* out_final, double* out_dev, long* out_nsteps,
-    long* out_nattempts, long* out_nevals, double* out_maxde, int* out_status, int flags)
+    double* out_time, double* out_traj, double* out_final, double* out_dev, double* out_lyap,
+    long* out_nsteps, long* out_nattempts, long* out_nevals, double* out_maxde, int* out_status, int flags)
 {
-    if(!out_dev) return fail(AGX_EINVAL, "NULL deviation-vector output");
+    if(!out_dev && !out_lyap) return fail(AGX_EINVAL, "neither deviation vectors nor Lyapunov exponents requested");
     return orbitBatchHost(pot, n, ic, tstart, ttotal, trajsize, accuracy, Omega, maxNumSteps, out_time,
-        out_traj, out_final, out_dev, out_nsteps, out_nattempts, out_nevals, out_maxde, out_status, flags);
+        out_traj, out_final, out_dev, out_nsteps, out_nattempts, out_nevals, out_maxde, out_status, flags,
+        out_lyap);
 }
 
 }  // extern "C"

@@ -362,6 +362,33 @@ __device__ __forceinline__ void nfwDeriv2(double mass, double a, double r, doubl
         1 / (1.5 + q * (27. / 8 + q * (351. / 160 + 183. / 640 * q))) / (a * a * a));
 }
 
+// BasePotential::eval(PosCyl) with gradient {dR, dz, dphi} and Hessian (Cyl layout) for the
+// Multipole (+ optional NFW, evalAndConvertSph<Cyl>, coord.cpp:958-990): the components are
+// summed in order, the Multipole first (Composite::evalCyl, potential_composite.cpp:315-320)
+template<class F>
+__device__ __forceinline__ void multipoleCylHess(const AgxMultipoleDev& P, const F& fetch, double R,
+    double z, double phi, double& pot, double (&gc)[3], double (&hc)[6])
+{
+    const double rsq = R * R + z * z;
+    if(rsq < P.rsqInner) powerLawEvalHess(P.inner, R, z, phi, pot, gc, hc);
+    else if(rsq > P.rsqOuter) powerLawEvalHess(P.outer, R, z, phi, pot, gc, hc);
+    else interiorEvalHess(P, fetch, R, z, phi, pot, gc, hc);
+    if(P.nfwMass != 0) {
+        const double r = sqrt(R * R + z * z);
+        double pn, dn, d2n;
+        nfwDeriv2(P.nfwMass, P.nfwScale, r, pn, dn, d2n);
+        const double Rr = r == 0 ? 0. : R / r, zr = r == 0 ? 0. : z / r;
+        double dor = dn / r, dd = d2n - dor;
+        if(r == 0) { dd = 0; if(dn == 0) dor = d2n; }
+        pot = pot + pn;
+        gc[0] = gc[0] + Rr * dn;
+        gc[1] = gc[1] + zr * dn;
+        hc[0] = hc[0] + (Rr * Rr * dd + dor);
+        hc[1] = hc[1] + (zr * zr * dd + dor);
+        hc[3] = hc[3] + Rr * zr * dd;
+    }
+}
+
 // BasePotential::eval(PosCar) with deriv2 for the Multipole (+ optional NFW) potential:
 // gradient g = +dPhi/dx and Hessian h in the Car layout
 template<class F>

@@ -1,7 +1,9 @@
 // Persistent DOP853 orbit kernel with the variational equation: the six deviation vectors of
-// agama.orbit(der=True), restating RuntimeVariational (src/orbit_variational.cpp:63-308,
-// inertial-frame form) with the Gauss-Legendre stepper Ode2StepperGL4<3>
-// (src/math_ode.cpp:1017-1250).
+// agama.orbit(der=True) and the deviation-vector growth behind lyapunov=True, restating
+// RuntimeVariational (src/orbit_variational.cpp:63-404, inertial-frame form) with the
+// Gauss-Legendre stepper Ode2StepperGL4<3> (src/math_ode.cpp:1017-1250), and the orbital
+// period T_circ(E) that sets the Lyapunov exponent's unit (src/potential_utils.cpp:264-289,
+// 608-619; src/math_core.cpp:866-1040; src/potential_base.cpp:257-283, 529-575).
 //
 // The orbit part is the DOP853 phase machine of agx_dop853.h.  After each accepted step a lane
 // takes four more evaluations - the Hessian at the step's dense-output positions for the
@@ -10,9 +12,13 @@
 // when they have grown by e^10.  An iteration in which any lane of the wavefront needs a
 // Hessian evaluates the Hessian path for the whole wavefront (it also returns the potential
 // and gradient the other lanes need), so control flow stays uniform.
-// Per-lane variational state (150 doubles: GL4 state of six vectors, the four A matrices,
+// Per-lane variational state (160 doubles: GL4 state of up to six vectors, the four A matrices,
 // the four positions and bookkeeping) lives in global memory, field-major across the grid's
 // lanes so that a wavefront's accesses are contiguous.
+// NV = 6: the deviation vectors start as the identity and their weighted sum is the fiducial
+// vector; NV = 1 (lyapunov=True without der=True): one vector with the fixed initial direction.
+// The log-magnitude series of the fiducial vector is coarse-grained on the fly into the bins
+// of calcLyapunovExponent (orbit_variational.cpp:372-404); the fit runs on the host.
 #include <hip/hip_runtime.h>
 #include "agx_device.h"
 #include "agx_hessian.h"
@@ -22,7 +28,8 @@
 namespace agxd {
 
 enum : int { V_GL = 0, V_AM = 108, V_PX = 132, V_T0 = 144, V_ADDLOG = 145, V_DEVSIZE = 146, V_NEWSTEP = 147,
-    V_PREVDT = 148, V_COUNT = 150 };
+    V_PREVDT = 148, V_PERIOD = 149, V_BSTART = 150, V_BOFF = 151, V_BSUM = 152, V_BSIZE = 153,
+    V_PREVT = 154, V_PREVV = 155, V_NBINS = 156, V_NPTS = 157, V_MINBIN = 158, V_COUNT = 160 };
 enum : int { PV_0 = 40, PV_1, PV_2, PV_3 };
 
 // Gauss-Legendre nodes on [0, 1] and the derived weights (math_ode.cpp:1081-1131)
@@ -63,18 +70,30 @@ __device__ __forceinline__ double devVecInit(int i)
            i == 3 ? 0.4346134936801766 : i == 4 ? 0.45946829173634074 : 0.5055250296034367;
 }
 
-template<int STRIDE>
+// outputs of the Lyapunov bookkeeping (null pointers: not requested)
+struct AgxLyapOut {
+    const double* period;   // [n] orbital period T_circ(E) of each orbit's starting point
+    const double* maxTime;  // [n] duration used for the bin width (the requested |T|)
+    double* bins;           // [n][2][maxBins]: bin times and ln w
+    int* nbins;             // [n]
+    double* lastTime;       // [n] time of the last recorded point since the start (NaN: none)
+    int maxBins;
+};
+
+template<int STRIDE, int NV>
 struct LaneVarT : public LaneT<STRIDE, false> {
     typedef LaneT<STRIDE, false> Base;
+    static constexpr int NVEC = NV;
     double* vg;      // this lane's variational state (field-major, stride vstride)
     long vstride;
     double* dev;     // deviation-vector output (n rows * 36 doubles), or null
+    AgxLyapOut ly;
     __device__ __forceinline__ double& v(int field) const { return vg[(long)field * vstride]; }
 };
 
 // GL4 getSol (math_ode.cpp:1040-1065) for component i (0..35) at offset timeOffset of the last step
-template<int S>
-__device__ __forceinline__ double gl4Sol(const LaneVarT<S>& L, double timeOffset, int i)
+template<int S, int NV>
+__device__ __forceinline__ double gl4Sol(const LaneVarT<S, NV>& L, double timeOffset, int i)
 {
     const double T = timeOffset - L.v(V_PREVDT);
     const int vec = i / 6, c = i - vec * 6;
@@ -96,21 +115,22 @@ __device__ __forceinline__ double logMagnitude6(const double* x)
 
 // storeSolution (orbit_variational.cpp:201-254): write the six vectors at row ind (ind < 0: none)
 // in the rotating frame, return the log-magnitude of the fiducial vector
-template<int S>
-__device__ __forceinline__ double storeSolution(const AgxOrbitArgs& A, LaneVarT<S>& L, double tbegin,
+template<int S, int NV>
+__device__ __forceinline__ double storeSolution(const AgxOrbitArgs& A, LaneVarT<S, NV>& L, double tbegin,
     double timeOffset, long ind)
 {
     const double mul = exp(L.v(V_ADDLOG));
     double fid[6] = { 0, 0, 0, 0, 0, 0 };
     double ca = 1, sa = 0;
     if(A.Omega != 0) sincosA(A.Omega * (tbegin + timeOffset), sa, ca);
-    const bool write = L.dev && ind >= 0;
+    const bool write = NV == 6 && L.dev && ind >= 0;
     if(write) {
         L.v(V_DEVSIZE) = (double)(ind + 1);
     }
-    for(int vec = 0; vec < 6; vec++) {
+    for(int vec = 0; vec < NV; vec++) {
         double q[6];
         for(int c = 0; c < 6; c++) q[c] = gl4Sol(L, timeOffset, vec * 6 + c);
+        if(NV == 1) return logMagnitude6(q);
         if(write && ind < L.S) {
             double* o = L.dev + (trajOffsetOf(A, L.oi) * 36 + ((long)vec * L.S + ind) * 6);
             o[0] = (q[0] * ca + q[1] * sa) * mul;
@@ -126,10 +146,18 @@ __device__ __forceinline__ double storeSolution(const AgxOrbitArgs& A, LaneVarT<
     return logMagnitude6(fid);
 }
 
-// six deviation vectors = the identity rotated into the inertial frame at tbegin (:136-155)
-template<int S>
-__device__ __forceinline__ void varInit(const AgxOrbitArgs& A, LaneVarT<S>& L, double tbegin)
+// six deviation vectors = the identity rotated into the inertial frame at tbegin (:136-155), or
+// the single vector along the fixed initial direction (:157-160)
+template<int S, int NV>
+__device__ __forceinline__ void varInit(const AgxOrbitArgs& A, LaneVarT<S, NV>& L, double tbegin)
 {
+    L.v(V_NEWSTEP) = 1;
+    L.v(V_PREVDT) = 0;
+    L.v(V_T0) = tbegin;
+    if(NV == 1) {
+        for(int i = 0; i < 18; i++) L.v(V_GL + i) = i < 6 ? devVecInit(i) : 0;
+        return;
+    }
     double ca, sa;
     sincosA(A.Omega * tbegin, sa, ca);
     for(int i = 0; i < 108; i++) L.v(V_GL + i) = 0;
@@ -138,15 +166,12 @@ __device__ __forceinline__ void varInit(const AgxOrbitArgs& A, LaneVarT<S>& L, d
     set(1, sa); set(22, sa);
     set(6, -sa); set(27, -sa);
     set(14, 1); set(35, 1);
-    L.v(V_NEWSTEP) = 1;
-    L.v(V_PREVDT) = 0;
-    L.v(V_T0) = tbegin;
     storeSolution(A, L, tbegin, 0, 0);
 }
 
 // Ode2StepperGL4::doStep for the six vectors with the stored A matrices (B = 0)
-template<int S>
-__device__ __forceinline__ void gl4Step(LaneVarT<S>& L, double dt)
+template<int S, int NV>
+__device__ __forceinline__ void gl4Step(LaneVarT<S, NV>& L, double dt)
 {
     using namespace gl4;
     const double dt2 = dt * dt, dt3 = dt2 * dt, idt = 1. / dt, idt2 = idt * idt, idt3 = idt2 * idt;
@@ -158,7 +183,7 @@ __device__ __forceinline__ void gl4Step(LaneVarT<S>& L, double dt)
         a[k][6] = a[k][2]; a[k][7] = a[k][5]; a[k][8] = L.v(f + 5);
     }
     const int niter = L.v(V_NEWSTEP) != 0 ? 8 : 4;
-    for(int vec = 0; vec < 6; vec++) {
+    for(int vec = 0; vec < NV; vec++) {
         const int b = V_GL + vec * 18;
         double x[3], xd[3], p[3], q[3], r[3], s[3];
         for(int d = 0; d < 3; d++) {
@@ -208,62 +233,127 @@ __device__ __forceinline__ void gl4Step(LaneVarT<S>& L, double dt)
 }
 
 // renormalize (orbit_variational.cpp:256-308, rescaling without orthogonalisation)
-template<int S>
-__device__ __forceinline__ void varRenormalize(LaneVarT<S>& L, double timeStep)
+template<int S, int NV>
+__device__ __forceinline__ void varRenormalize(LaneVarT<S, NV>& L, double timeStep)
 {
     double lg = -INFINITY;
-    for(int vec = 0; vec < 6; vec++) {
+    for(int vec = 0; vec < NV; vec++) {
         double q[6];
         for(int c = 0; c < 6; c++) q[c] = gl4Sol(L, timeStep, vec * 6 + c);
         lg = fmax(lg, logMagnitude6(q));
     }
     const double mult = exp(-lg);
     L.v(V_ADDLOG) = L.v(V_ADDLOG) + lg;
-    double vals[36];
-    for(int i = 0; i < 36; i++) vals[i] = gl4Sol(L, timeStep, i) * mult;
-    for(int i = 0; i < 108; i++) L.v(V_GL + i) = 0;
-    for(int vec = 0; vec < 6; vec++)
+    double vals[6 * NV];
+    for(int i = 0; i < 6 * NV; i++) vals[i] = gl4Sol(L, timeStep, i) * mult;
+    for(int i = 0; i < 18 * NV; i++) L.v(V_GL + i) = 0;
+    for(int vec = 0; vec < NV; vec++)
         for(int c = 0; c < 6; c++) L.v(V_GL + vec * 18 + c) = vals[vec * 6 + c];
     L.v(V_NEWSTEP) = 1;
 }
 
+// calcLyapunovExponent's coarse-graining (orbit_variational.cpp:380-404), fed one point
+// (time since start, ln|w|) at a time: each new point closes the pair (previous, new)
+template<int S, int NV>
+__device__ __forceinline__ void binPoint(LaneVarT<S, NV>& L, double tp, double vp)
+{
+    const double np = L.v(V_NPTS);
+    L.v(V_NPTS) = np + 1;
+    if(np > 0) {
+        const double ti = L.v(V_PREVT), vi = L.v(V_PREVV);
+        double bstart = L.v(V_BSTART);
+        if(bstart != bstart) {
+            bstart = ti;
+            L.v(V_BSTART) = ti;
+            L.v(V_BOFF) = vi;
+            L.v(V_BSUM) = 0;
+            L.v(V_BSIZE) = 0;
+        }
+        const double sum = L.v(V_BSUM) + (tp - ti) * exp(0.5 * (vi + vp) - L.v(V_BOFF));
+        const double size = L.v(V_BSIZE) + 1;
+        L.v(V_BSUM) = sum;
+        L.v(V_BSIZE) = size;
+        if(size >= 5 && tp >= bstart + L.v(V_MINBIN)) {
+            const int nb = (int)L.v(V_NBINS);
+            if(nb < L.ly.maxBins) {
+                double* b = L.ly.bins + (long)L.oi * 2 * L.ly.maxBins;
+                b[nb] = 0.5 * (bstart + tp);
+                b[L.ly.maxBins + nb] = log(sum / (tp - bstart)) + L.v(V_BOFF);
+            }
+            L.v(V_NBINS) = nb + 1;
+            L.v(V_BSTART) = NAN;
+        }
+    }
+    L.v(V_PREVT) = tp;
+    L.v(V_PREVV) = vp;
+}
+
+// the series ended: its last pair closes the open bin when it holds at least 5 points
+template<int S, int NV>
+__device__ __forceinline__ void binFinish(LaneVarT<S, NV>& L)
+{
+    const double bstart = L.v(V_BSTART);
+    if(bstart == bstart && L.v(V_BSIZE) >= 5) {
+        const int nb = (int)L.v(V_NBINS);
+        const double tp = L.v(V_PREVT);
+        if(nb < L.ly.maxBins) {
+            double* b = L.ly.bins + (long)L.oi * 2 * L.ly.maxBins;
+            b[nb] = 0.5 * (bstart + tp);
+            b[L.ly.maxBins + nb] = log(L.v(V_BSUM) / (tp - bstart)) + L.v(V_BOFF);
+        }
+        L.v(V_NBINS) = nb + 1;
+    }
+    L.ly.nbins[L.oi] = (int)L.v(V_NBINS);
+    L.ly.lastTime[L.oi] = L.v(V_NPTS) > 0 ? L.v(V_PREVT) : NAN;
+}
+
 // the generic bookkeeping plus the deviation-vector rows that were not recorded (NaN), and the
 // zero-length case where the start is recorded (orbit.cpp:52-57 with RuntimeVariational)
-template<int S>
-__device__ __forceinline__ void finishOrbit(const AgxOrbitArgs& A, LaneVarT<S>& L, int status)
+template<int S, int NV>
+__device__ __forceinline__ void finishOrbit(const AgxOrbitArgs& A, LaneVarT<S, NV>& L, int status)
 {
-    if(L.dev) {
-        const bool zeroLength = status == 0 && L.v(V_T0) != L.v(V_T0) &&
-            (A.ttotal[L.oi] == 0 || !(L.tend > -INFINITY && L.tend < INFINITY));
-        if(zeroLength) {
-            varInit(A, L, L.tb);
-            // doStep(0) leaves x and x' unchanged; sampling interval 0 records the start again
-            if(L.S >= 2) storeSolution(A, L, L.tb, 0, (long)L.v(V_DEVSIZE));
-        }
+    const bool zeroLength = status == 0 && L.v(V_T0) != L.v(V_T0) &&
+        (A.ttotal[L.oi] == 0 || !(L.tend > -INFINITY && L.tend < INFINITY));
+    if(zeroLength) {
+        varInit(A, L, L.tb);
+        // doStep(0) leaves x and x' unchanged; sampling interval 0 records the start again
+        if(NV == 6 && L.dev && L.S >= 2) storeSolution(A, L, L.tb, 0, (long)L.v(V_DEVSIZE));
+        if(L.ly.bins) binPoint(L, fabs(L.tb - L.v(V_T0)), storeSolution(A, L, L.tb, 0, -1));
+    }
+    if(L.ly.bins) binFinish(L);
+    if(NV == 6 && L.dev) {
         const long done = L.v(V_T0) == L.v(V_T0) ? (long)L.v(V_DEVSIZE) : 0;
         const long base = trajOffsetOf(A, L.oi) * 36;
         for(int vec = 0; vec < 6; vec++)
             for(long r = done; r < L.S; r++)
                 for(int c = 0; c < 6; c++) L.dev[base + ((long)vec * L.S + r) * 6 + c] = NAN;
     }
-    finishOrbit<typename LaneVarT<S>::Base>(A, L, status);
+    finishOrbit<typename LaneVarT<S, NV>::Base>(A, L, status);
 }
 
-template<int S>
-__device__ __forceinline__ void fetchOrbit(const AgxOrbitArgs& A, LaneVarT<S>& L)
+template<int S, int NV>
+__device__ __forceinline__ void fetchOrbit(const AgxOrbitArgs& A, LaneVarT<S, NV>& L)
 {
-    fetchOrbit<typename LaneVarT<S>::Base>(A, L);
+    fetchOrbit<typename LaneVarT<S, NV>::Base>(A, L);
     if(L.phase != PH_IDLE) {
         L.v(V_T0) = NAN;
         L.v(V_ADDLOG) = 0;
         L.v(V_DEVSIZE) = 0;
+        if(L.ly.bins) {
+            const double period = L.ly.period[L.oi];
+            L.v(V_PERIOD) = period;
+            L.v(V_MINBIN) = fmax(L.ly.maxTime[L.oi] * 0.01, period);
+            L.v(V_BSTART) = NAN;
+            L.v(V_NBINS) = 0;
+            L.v(V_NPTS) = 0;
+        }
     }
 }
 
 // accepted orbit step: energy, trajectory sampling and the step's dense-output positions at the
 // collocation times; the orbit state advances, its clock after the variational step
-template<int S>
-__device__ __forceinline__ void acceptStep(const AgxOrbitArgs& A, LaneVarT<S>& L, double pot)
+template<int S, int NV>
+__device__ __forceinline__ void acceptStep(const AgxOrbitArgs& A, LaneVarT<S, NV>& L, double pot)
 {
     const double de = fabs(energyOf(L.y, pot, A.Omega) / L.f(F_E0) - 1);
     const double mde = L.f(F_MAXDE);
@@ -282,8 +372,8 @@ __device__ __forceinline__ void acceptStep(const AgxOrbitArgs& A, LaneVarT<S>& L
 
 // one Hessian (Car layout) at the k-th collocation position, rotated into the inertial frame
 // with the same angle as the evaluation point; after the fourth: the variational step
-template<int S>
-__device__ __forceinline__ void advanceVar(const AgxOrbitArgs& A, LaneVarT<S>& L, const double (&hs)[6],
+template<int S, int NV>
+__device__ __forceinline__ void advanceVar(const AgxOrbitArgs& A, LaneVarT<S, NV>& L, const double (&hs)[6],
     double ca, double sa)
 {
     const int k = L.phase - PV_0;
@@ -306,7 +396,7 @@ __device__ __forceinline__ void advanceVar(const AgxOrbitArgs& A, LaneVarT<S>& L
     if(L.v(V_T0) != L.v(V_T0)) varInit(A, L, tbegin);
     gl4Step(L, timestep);
     const double t0 = L.v(V_T0);
-    if(L.dev && L.S > 1) {
+    if(NV == 6 && L.dev && L.S > 1) {
         const double dtS = L.f(F_DTS);
         const double sign = timestep >= 0 ? +1 : -1;
         const double dtr = ROUNDOFF * fmax(fmax(fabs(tend), fabs(tbegin)), fabs(t0));
@@ -318,7 +408,8 @@ __device__ __forceinline__ void advanceVar(const AgxOrbitArgs& A, LaneVarT<S>& L
             storeSolution(A, L, tbegin, (xo > hi ? hi : xo > 0.0 ? xo : 0.0) * sign, iout);
         }
     }
-    const double logDevVec = storeSolution(A, L, tbegin, timestep, L.dev && L.S == 1 ? 0 : -1);
+    const double logDevVec = storeSolution(A, L, tbegin, timestep, NV == 6 && L.dev && L.S == 1 ? 0 : -1);
+    if(L.ly.bins) binPoint(L, fabs(tend - t0), logDevVec + L.v(V_ADDLOG));
     if(logDevVec > 10) varRenormalize(L, timestep);
     // the orbit's clock and step count (orbit.cpp:79-82)
     L.tb = L.h == L.rem ? L.tend : L.tb + L.h;
@@ -326,6 +417,161 @@ __device__ __forceinline__ void advanceVar(const AgxOrbitArgs& A, LaneVarT<S>& L
     if(L.nsteps >= A.maxNumSteps) { finishOrbit(A, L, 2); return; }
     if(L.tb == L.tend) { finishOrbit(A, L, 0); return; }
     beginStep(A, L);
+}
+
+// ---- orbital period of each orbit's starting point (the Lyapunov exponent's unit) ----
+
+// Axisymmetrized<BasePotential>::evalCyl at (R, z): potential, dPhi/dR and d2Phi/dR2 averaged
+// over 2*8+1 azimuths, 9 with doubled weights for y-reflection symmetric potentials
+// (potential_base.cpp:257-283, 529-575)
+template<class F>
+__device__ __forceinline__ void axisymmetrizedEval(const AgxMultipoleDev& P, const F& fetch, double R,
+    double z, double& Phi, double& dR, double& dR2)
+{
+    double p, g[3], h[6];
+    constexpr int ST_YREFL = 2, ST_ZROT = 16;
+    if((P.sym & ST_ZROT) == ST_ZROT || (R == 0 && z == 0) || R == INFINITY || fabs(z) == INFINITY) {
+        multipoleCylHess(P, fetch, R, z, 0., p, g, h);
+        Phi = p; dR = g[0]; dR2 = h[0];
+        return;
+    }
+    const int nphi = 8;
+    const bool ysym = (P.sym & ST_YREFL) == ST_YREFL;
+    const int npoints = ysym ? nphi + 1 : 2 * nphi + 1;
+    double a0 = 0, a1 = 0, a3 = 0;
+    for(int i = 0; i < npoints; i++) {
+        multipoleCylHess(P, fetch, R, z, M_PI / (nphi + 0.5) * i, p, g, h);
+        const double w = ysym && i > 0 ? 2 : 1;
+        a0 += p * w;
+        a1 += g[0] * w;
+        a3 += h[0] * w;
+    }
+    Phi = a0 * (1. / (2 * nphi + 1));
+    dR = a1 * (1. / (2 * nphi + 1));
+    dR2 = a3 * (1. / (2 * nphi + 1));
+}
+
+__device__ __forceinline__ double hermiteInterpD(double x, double x1, double x2, double y1, double y2,
+    double dy1, double dy2)
+{
+    const double t = (x - x1) / (x2 - x1);
+    return (1 - t) * (1 - t) * ((1 + 2 * t) * y1 + t * dy1 * (x2 - x1))
+         + t * t * ((3 - 2 * t) * y2 + (t - 1) * dy2 * (x2 - x1));
+}
+
+// math_core.cpp:866-888
+__device__ __forceinline__ double hermiteInterpMonotoneD(double x, double x1, double x2, double f1, double f2,
+    double dfdx1, double dfdx2)
+{
+    if(!(fabs(dfdx1 + dfdx2) < INFINITY) || (dfdx1 >= 0 && dfdx2 < 0) || (dfdx2 >= 0 && dfdx1 < 0)) return NAN;
+    const double dx = x2 - x1, ai = 1 / (-6 * (f2 - f1) / dx + 3 * (dfdx1 + dfdx2)),
+        ba = -1 + (dfdx2 - dfdx1) * ai, ca = dfdx1 * ai, D = ba * ba - 4 * ca;
+    if(!(fabs(ba + D) < INFINITY)) return NAN;
+    if(D >= 0) {
+        const double sqD = sqrt(D), t1 = 0.5 * (-ba - sqD), t2 = 0.5 * (-ba + sqD);
+        if((t1 >= 0 && t1 <= 1) || (t2 >= 0 && t2 <= 1)) return NAN;
+    }
+    return hermiteInterpD(x, x1, x2, f1, f2, dfdx1, dfdx2);
+}
+
+// RcircRootFinder (potential_utils.cpp:264-289) of u = ln R seen through ScalingInf
+// (u = 1/(1-s) - 1/s, math_core.cpp:302-306) as ScaledFnc (math_core.h:212-227) does
+template<class F>
+__device__ __forceinline__ void rcircScaled(const AgxMultipoleDev& P, const F& fetch, double E, double s,
+    double& val, double& der)
+{
+    const double duds = 1 / ((1 - s) * (1 - s)) + 1 / (s * s);
+    const double R = exp(1 / (1 - s) - 1 / s);
+    double Phi, dR, dR2;
+    axisymmetrizedEval(P, fetch, R, 0., Phi, dR, dR2);
+    const double v = 0.5 * R * dR;
+    val = Phi - E + (fabs(v) < INFINITY ? v : 0);
+    if(!(fabs(val) < INFINITY)) {
+        if(E == -INFINITY) val = Phi == E ? 0 : +1.0;
+        else if(Phi == -INFINITY) val = -1.0;
+        else if(R >= 1e100) val = +1.0;
+    }
+    der = ((1.5 * dR + 0.5 * R * dR2) * R) * duds;
+}
+
+// findRoot (math_core.cpp:895-1040) on s in [0, 1], relative tolerance 1e-10, and
+// T_circ = 2 pi R / v_circ(R) (potential_utils.h:83-87, potential_utils.cpp:608-619)
+template<class F>
+__device__ double tcircOf(const AgxMultipoleDev& P, const F& fetch, double E)
+{
+    const double reltoler = 1e-10, DELTA_HERMITE = 1e-12;
+    const int MAXITER = 64;
+    double a = 0, b = 1, fa, fb, fdera, fderb;
+    rcircScaled(P, fetch, E, a, fa, fdera);
+    rcircScaled(P, fetch, E, b, fb, fderb);
+    double sroot;
+    if((fa < 0.0 && fb < 0.0) || (fa > 0.0 && fb > 0.0) || !(fabs(fa + fb) < INFINITY)) sroot = NAN;
+    else if(fa == 0) sroot = fb == 0 ? 0.5 * (a + b) : a;
+    else if(fb == 0) sroot = b;
+    else {
+        double c = a, fc = fa, fderc = fdera, d = b - c, e = b - c;
+        if(fabs(fc) < fabs(fb)) {
+            a = b; b = c; c = a; fa = fb; fb = fc; fc = fa; fdera = fderb; fderb = fderc; fderc = fdera;
+        }
+        int numIter = 0;
+        bool converged = false, done = false;
+        const double abstoler = fabs(0. - 1.) * reltoler;
+        while(!converged) {
+            const double tol = 0.5 * 2.2204460492503131e-16 * fabs(b), cminusb = c - b, m = 0.5 * cminusb;
+            if(fb == 0 || fabs(m) <= tol) { done = true; break; }
+            if(fabs(e) < tol || fabs(fa) <= fabs(fb)) { d = m; e = m; }
+            else {
+                double dd = NAN;
+                if(fderb * fderc > 0) dd = hermiteInterpMonotoneD(0, fb, fc, 0, cminusb, 1 / fderb, 1 / fderc);
+                if(fabs(dd) < INFINITY && fmin(fabs(dd), fabs(cminusb - dd)) > fabs(cminusb) * DELTA_HERMITE)
+                    d = dd;
+                else {
+                    double p, q, r, sr = fb / fa;
+                    if(a == c) { p = 2 * m * sr; q = 1 - sr; }
+                    else {
+                        q = fa / fc; r = fb / fc;
+                        p = sr * (2 * m * q * (q - r) - (b - a) * (r - 1));
+                        q = (q - 1) * (r - 1) * (sr - 1);
+                    }
+                    if(p > 0) q = -q; else p = -p;
+                    if(2 * p < fmin(3 * m * q - fabs(tol * q), fabs(e * q))) { e = d; d = p / q; }
+                    else { d = m; e = m; }
+                }
+            }
+            a = b; fa = fb; fdera = fderb;
+            if(fabs(d) > tol) b += d; else b += (m > 0 ? +tol : -tol);
+            rcircScaled(P, fetch, E, b, fb, fderb);
+            if(!(fabs(fb) < INFINITY)) { b = NAN; done = true; break; }
+            if((fb < 0 && fc < 0) || (fb > 0 && fc > 0)) { c = a; fc = fa; fderc = fdera; d = b - c; e = b - c; }
+            if(fabs(fc) < fabs(fb)) {
+                a = b; b = c; c = a; fa = fb; fb = fc; fc = fa; fdera = fderb; fderb = fderc; fderc = fdera;
+            }
+            numIter++;
+            if(fabs(b - c) <= abstoler) {
+                converged = true;
+                const double offset = fb * (b - c) / (fc - fb);
+                if((offset > 0 && offset < c - b) || (offset < 0 && offset > c - b)) b += offset;
+            } else {
+                const double offset = -fb / fderb;
+                const bool bracketed = (offset > 0 && offset < c - b) || (offset < 0 && offset > c - b);
+                if(bracketed && fabs(offset) < abstoler) { converged = true; b += offset; }
+            }
+            if(numIter >= MAXITER) converged = true;
+        }
+        (void)done;
+        sroot = b;
+    }
+    const double R = exp(1 / (1 - sroot) - 1 / sroot);
+    double vc;
+    double Phi, dR, dR2;
+    if(R == 0) {
+        axisymmetrizedEval(P, fetch, 0., 0., Phi, dR, dR2);
+        vc = fabs(Phi) < INFINITY ? 0 : INFINITY;
+    } else {
+        axisymmetrizedEval(P, fetch, R, 0., Phi, dR, dR2);
+        vc = sqrt(R * dR);
+    }
+    return R / vc * 2 * M_PI;
 }
 
 }  // namespace agxd
@@ -336,12 +582,44 @@ namespace {
 
 constexpr int VBS = 256;
 
-// varState: V_COUNT * (gridDim.x * VBS) doubles; dev: deviation-vector output
-template<bool NFW>
-__global__ __launch_bounds__(VBS) __attribute__((amdgpu_waves_per_eu(1, 1)))
-void orbitVarKernel(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A, double* varState, double* dev)
+// period[i] = T_circ(E) at orbit i's starting point as the orbit integrator reports it: the
+// initial condition taken into the inertial frame at tstart and back (orbit.cpp:104-133), its
+// energy Phi + v^2/2 (orbit_variational.cpp:131-135); non-finite values become 0
+__global__ __launch_bounds__(256) void tcircKernel(const AgxMultipoleDev* __restrict__ Pp, long n,
+    const double* __restrict__ ic, const double* __restrict__ tstart, double Omega, double* __restrict__ period)
 {
-    typedef LaneVarT<VBS> Lane;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    const AgxMultipoleDev& P = *Pp;
+    double x[6];
+    for(int c = 0; c < 6; c++) x[c] = ic[i * 6 + c];
+    if(Omega != 0) {
+        const double tb = tstart ? tstart[i] : 0.0;
+        double ca = 1, sa = 0;
+        sincosA(Omega * tb, sa, ca);
+        double w[6];
+        w[0] = x[0] * ca - x[1] * sa; w[1] = x[1] * ca + x[0] * sa; w[2] = x[2];
+        w[3] = x[3] * ca - x[4] * sa; w[4] = x[4] * ca + x[3] * sa; w[5] = x[5];
+        sincosA(Omega * (0 + tb), sa, ca);
+        x[0] = w[0] * ca + w[1] * sa; x[1] = w[1] * ca - w[0] * sa; x[2] = w[2];
+        x[3] = w[3] * ca + w[4] * sa; x[4] = w[4] * ca - w[3] * sa; x[5] = w[5];
+    }
+    FetchVec2 fv{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
+    double pot, gx, gy, gz;
+    multipoleCar<FetchVec2, const double AGX_GLOBAL*, true>(P, fv, x[0], x[1], x[2], pot, gx, gy, gz);
+    const double E = pot + 0.5 * (x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+    const double T = tcircOf(P, fv, E);
+    period[i] = fabs(T) < INFINITY ? T : 0;
+}
+
+// varState: V_COUNT * (gridDim.x * VBS) doubles; dev: deviation-vector output (NV = 6);
+// ly: Lyapunov bookkeeping (ly.bins null: none)
+template<bool NFW, int NV>
+__global__ __launch_bounds__(VBS) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void orbitVarKernel(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A, double* varState, double* dev,
+    AgxLyapOut ly)
+{
+    typedef LaneVarT<VBS, NV> Lane;
     __shared__ double lds[Lane::LDS_FIELDS * VBS];
     __shared__ double gridLds[AGX_LDS_GRID];
     const AgxMultipoleDev& P = *Pp;
@@ -352,6 +630,7 @@ void orbitVarKernel(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A, doub
     L.vstride = (long)gridDim.x * VBS;
     L.vg = varState + (long)blockIdx.x * VBS + threadIdx.x;
     L.dev = dev;
+    L.ly = ly;
     fetchOrbit(A, L);
     const double accRel = A.accuracy;
     unsigned int it = 0;
@@ -390,20 +669,37 @@ void orbitVarKernel(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A, doub
 int agxOrbitVarKernelBlocksPerCU(bool nfw)
 {
     int nb = 0;
-    hipError_t e = nfw ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitVarKernel<true>, VBS, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitVarKernel<false>, VBS, 0);
+    hipError_t e = nfw ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitVarKernel<true, 6>, VBS, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitVarKernel<false, 6>, VBS, 0);
     return e == hipSuccess && nb > 0 ? nb : 1;
 }
 
 size_t agxOrbitVarStateDoubles(int numBlocks) { return (size_t)V_COUNT * numBlocks * VBS; }
 
+hipError_t agxLaunchTcirc(const AgxMultipoleDev* dP, long n, const double* ic, const double* tstart,
+    double Omega, double* period, hipStream_t stream)
+{
+    if(n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tcircKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, dP, n, ic, tstart,
+        Omega, period);
+    return hipGetLastError();
+}
+
+// dev non-null: six deviation vectors (der=True), else one vector; bins non-null: Lyapunov
+// bookkeeping with period[n], maxTime[n] in and bins[n][2][maxBins], nbins[n], lastTime[n] out
 hipError_t agxLaunchOrbitVar(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, const AgxOrbitArgs& A,
-    int numBlocks, double* varState, double* dev, hipStream_t stream)
+    int numBlocks, double* varState, double* dev, const double* period, const double* maxTime, double* bins,
+    int* nbins, double* lastTime, int maxBins, hipStream_t stream)
 {
     if(A.n <= 0) return hipSuccess;
-    if(P.nfwMass != 0)
-        hipLaunchKernelGGL((orbitVarKernel<true>), dim3(numBlocks), dim3(VBS), 0, stream, dP, A, varState, dev);
-    else
-        hipLaunchKernelGGL((orbitVarKernel<false>), dim3(numBlocks), dim3(VBS), 0, stream, dP, A, varState, dev);
+    AgxLyapOut ly{ period, maxTime, bins, nbins, lastTime, maxBins };
+    const dim3 g(numBlocks), b(VBS);
+    if(dev) {
+        if(P.nfwMass != 0) hipLaunchKernelGGL((orbitVarKernel<true, 6>), g, b, 0, stream, dP, A, varState, dev, ly);
+        else hipLaunchKernelGGL((orbitVarKernel<false, 6>), g, b, 0, stream, dP, A, varState, dev, ly);
+    } else {
+        if(P.nfwMass != 0) hipLaunchKernelGGL((orbitVarKernel<true, 1>), g, b, 0, stream, dP, A, varState, dev, ly);
+        else hipLaunchKernelGGL((orbitVarKernel<false, 1>), g, b, 0, stream, dP, A, varState, dev, ly);
+    }
     return hipGetLastError();
 }

@@ -31,6 +31,7 @@ struct AgxMultipoleDev {
     int nr, nt, ns;             // radial nodes, tau nodes, stored harmonic slots
     int mmax, mmin;             // azimuthal index range of the expansion
     int logScaling;             // m=0 term stored as ln(invPhi0 - 1/Phi), others as Phi_m/Phi_0
+    int sym;                    // symmetry of the expansion (bits of the reference's coord::SymmetryType)
     double invPhi0;
     double rsqInner, rsqOuter;  // (rmin*(1+eps))^2, (rmax*(1-eps))^2 : branch thresholds
     int slotM[AGX_MAX_SLOTS];   // m of each stored slot, ascending as in the reference loop

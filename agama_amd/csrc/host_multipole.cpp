@@ -902,7 +902,7 @@ HostTables buildTables(const Coefs& C, bool allowLowOrder)
     if(ind.mmax > AGX_MAX_M) throw std::invalid_argument("Multipole: mmax > 16 is not supported");
 
     HostTables T;
-    T.nr = nr; T.lmax = ind.lmax; T.mmax = ind.mmax; T.mmin = ind.mmin(); T.step = ind.step;
+    T.nr = nr; T.lmax = ind.lmax; T.mmax = ind.mmax; T.mmin = ind.mmin(); T.step = ind.step; T.sym = ind.sym;
     // MultipoleInterp2d ctor (potential_multipole.cpp:1854-1967)
     double s, U, W, Q;
     extrapolationCoefs(C.Phi[0][0], C.Phi[0][1], C.dPhi[0][0], C.dPhi[0][1], C.radii[0], C.radii[1],

@@ -60,6 +60,7 @@ struct HostPowerLaw {
 };
 struct HostTables {
     int nr = 0, nt = 0, ns = 0, mmax = 0, mmin = 0, lmax = 0, step = 1, logScaling = 0;
+    int sym = 0;                    // symmetry of the expansion (coord::SymmetryType bits)
     double invPhi0 = 0, rsqInner = 0, rsqOuter = 0;
     std::vector<int> slotM;
     int zeroSlot = 0;

@@ -142,18 +142,33 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
     long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags,
     void* stream);
 
-/* agx_orbit_batch[_device] with the variational equation: the six deviation vectors of
-   agama.orbit(der=True) (src/orbit_variational.cpp RuntimeVariational, six vectors started from
-   the identity and advanced with Ode2StepperGL4<3>, src/math_ode.cpp:1017-1250), sampled at
-   the trajectory's times: out_dev holds, per orbit at row offset sum_{j<i} trajsize[j] * 36,
-   [vector][row][6] in the rotating frame, NaN-padded like the trajectory.  Vector k is the
-   derivative of the phase-space point with respect to initial coordinate k.  DOP853 and FP64
-   only (other flags return AGX_EUNSUPPORTED).  The device form allocates its per-lane state
-   (1.2 KB per resident lane) stream-ordered on `stream`. */
+/* agx_orbit_batch with the variational equation (src/orbit_variational.cpp RuntimeVariational,
+   deviation vectors advanced with Ode2StepperGL4<3>, src/math_ode.cpp:1017-1250):
+   out_dev (agama.orbit(der=True)): the six deviation vectors started from the identity, sampled
+     at the trajectory's times: per orbit at row offset sum_{j<i} trajsize[j] * 36,
+     [vector][row][6] in the rotating frame, NaN-padded like the trajectory; vector k is the
+     derivative of the phase-space point with respect to initial coordinate k;
+   out_lyap[n*2] (agama.orbit(lyapunov=True)): the Lyapunov exponent in units of the orbital
+     period T_circ(E) of the starting point and the onset time of chaos in the same unit
+     (calcLyapunovExponent, orbit_variational.cpp:360-448): 0 and infinity for regular orbits,
+     NaN for orbits with fewer than 5 bins of the deviation-vector growth.  Without out_dev the
+     growth is followed with one vector, as the reference does.  The fit is a Levenberg-Marquardt
+     of this library (the reference uses GSL's); orbits that stop early are integrated twice.
+   At least one of out_dev, out_lyap must be non-NULL.  DOP853 and FP64 only (other flags return
+   AGX_EUNSUPPORTED). */
 int agx_orbit_var_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
     const double* ttotal, const int* trajsize, double accuracy, double Omega, long maxNumSteps,
-    double* out_time, double* out_traj, double* out_final, double* out_dev, long* out_nsteps,
-    long* out_nattempts, long* out_nevals, double* out_maxde, int* out_status, int flags);
+    double* out_time, double* out_traj, double* out_final, double* out_dev, double* out_lyap,
+    long* out_nsteps, long* out_nattempts, long* out_nevals, double* out_maxde, int* out_status, int flags);
+/* The fitting step of the Lyapunov exponent on already binned data (host only): t[nbins],
+   lnw[nbins] the bin times and log mean deviation magnitudes, max_time the last recorded time,
+   period T_circ; out[2] as out_lyap above.  Replaces the fit and decisions of
+   calcLyapunovExponent (src/orbit_variational.cpp:406-448). */
+int agx_lyapunov_fit(const double* t, const double* lnw, int nbins, double max_time, double period,
+    double* out);
+/* Deviation vectors (out_dev required) with device buffers, asynchronous on `stream`; the
+   per-lane state (1.3 KB per resident lane) is allocated stream-ordered.  The Lyapunov fit
+   needs the host form. */
 int agx_orbit_var_batch_device(const agx_potential* pot, long n, const double* d_ic,
     const double* d_tstart, const double* d_ttotal, const int* d_trajsize,
     const long* d_traj_offset, const int* d_order, double accuracy, double Omega,

@@ -1248,6 +1248,10 @@ struct orc_pot {
     // the same with the Hessian hess = {dx2, dy2, dz2, dxdy, dydz, dxdz} (coord.h HessCar)
     virtual void evalCarHess(const double pos[3], double* phi, double grad[3], double hess[6],
         double t) const = 0;
+    // BasePotential::eval(PosCyl) with gradient and Hessian in cylindrical coordinates
+    virtual void evalCylHess(double R, double z, double phi, double* pot, double grad[3], double hess[6]) const = 0;
+    // coord::SymmetryType bits (potential_base.h symmetry())
+    virtual int symmetry() const = 0;
 };
 
 namespace {
@@ -1336,6 +1340,18 @@ struct OrcMultipole : public orc_pot {
             impl->evalCyl(pos, potential, deriv, deriv2);
     }
 
+    virtual int symmetry() const { return impl->ind.sym; }
+    // grad {dR, dz, dphi}, hess {dR2, dz2, dphi2, dRdz, dzdphi, dRdphi}
+    virtual void evalCylHess(double R, double z, double phi, double* pot, double grad[3], double hess[6]) const
+    {
+        PosCyl pc = {R, z, phi};
+        GradCyl g;
+        HessCyl h;
+        evalCyl(pc, pot, &g, &h);
+        grad[0] = g.dR; grad[1] = g.dz; grad[2] = g.dphi;
+        hess[0] = h.dR2; hess[1] = h.dz2; hess[2] = h.dphi2; hess[3] = h.dRdz; hess[4] = h.dzdphi; hess[5] = h.dRdphi;
+    }
+
     // coord.cpp:94-125 (toPosDeriv<Car,Cyl> with second derivatives), 700-720 (toHess<Cyl,Car>)
     virtual void evalCarHess(const double p[3], double* phi, double grad[3], double hess[6], double) const
     {
@@ -1392,6 +1408,26 @@ struct OrcNFW : public orc_pot {
             *deriv2 = -mass * (rrel > 0.010 ?
                 (2*ln_over_r - (2*scaleRadius + 3*r) / pow_2(scaleRadius+r) ) / pow_2(r) :
                 1 / (1.5 + rrel * (27./8 + rrel * (351./160 + 183./640 * rrel))) / pow_3(scaleRadius) );
+    }
+    virtual int symmetry() const { return ST_SPHERICAL; }
+    // coord.cpp:958-990 (evalAndConvertSph<Cyl>)
+    virtual void evalCylHess(double R, double z, double, double* pot, double grad[3], double hess[6]) const
+    {
+        const double r = sqrt(pow_2(R) + pow_2(z));
+        double der, der2;
+        evalDeriv(r, pot, &der, &der2);
+        double Rr = R/r, zr = z/r;
+        if(r==0) Rr = zr = 0;
+        grad[0] = Rr*der; grad[1] = zr*der; grad[2] = 0;
+        double der_over_r = der/r, dd = der2 - der_over_r;
+        if(r==0) {
+            dd = 0;
+            if(der==0) der_over_r = der2;
+        }
+        hess[0] = pow_2(Rr)*dd + der_over_r;
+        hess[1] = pow_2(zr)*dd + der_over_r;
+        hess[3] = Rr*zr*dd;
+        hess[2] = hess[4] = hess[5] = 0;
     }
     // coord.cpp:923-956 (evalAndConvertSph<Car> with the Hessian)
     virtual void evalCarHess(const double p[3], double* phi, double grad[3], double hess[6], double) const
@@ -1473,6 +1509,28 @@ struct OrcComposite : public orc_pot {
             grad[0] += gsum.dR*dRdx + gsum.dphi*dphidx;
             grad[1] += gsum.dR*dRdy + gsum.dphi*dphidy;
             grad[2] += gsum.dz;
+        }
+    }
+    virtual int symmetry() const
+    {
+        int sym = ST_SPHERICAL;
+        for(const orc_pot* c : comps) sym &= c->symmetry();
+        return sym;
+    }
+    // Composite::evalCyl = evalComponents<Car,Sph> (potential_composite.cpp:315-320): every
+    // component here prefers cylindrical or the input's coordinates, so all are evaluated at
+    // the cylindrical point and summed in component order
+    virtual void evalCylHess(double R, double z, double phi, double* pot, double grad[3], double hess[6]) const
+    {
+        if(pot) *pot = 0;
+        for(int i=0; i<3; i++) grad[i] = 0;
+        for(int i=0; i<6; i++) hess[i] = 0;
+        for(const orc_pot* c : comps) {
+            double p, g[3], h[6];
+            c->evalCylHess(R, z, phi, &p, g, h);
+            if(pot) *pot += p;
+            for(int i=0; i<3; i++) grad[i] += g[i];
+            for(int i=0; i<6; i++) hess[i] += h[i];
         }
     }
     // evalComponents<Cyl,Sph> with second derivatives: the cylindrical components' gradients
@@ -2020,6 +2078,203 @@ struct DPRKN8 {
     }
 };
 
+// ---- orbital period for the Lyapunov exponent's units ----
+// Axisymmetrized<BasePotential>::evalCyl (potential_base.cpp:257-283, 529-575): the potential,
+// dPhi/dR, dPhi/dz and, when hessOut, d2Phi/dR2, d2Phi/dz2, d2Phi/dRdz averaged over 2*8+1
+// azimuths (9 with doubled weights when the potential is symmetric under y -> -y)
+void axisymmetrizedEval(const orc_pot& pot, double R, double z, double* Phi, double* dR, double* dR2)
+{
+    const int sym = pot.symmetry();
+    double p, g[3], h[6];
+    if(has(sym, ST_ZROT) || (R==0 && z==0) || R==INFINITY || fabs(z)==INFINITY) {
+        pot.evalCylHess(R, z, 0, &p, g, h);
+        if(Phi) *Phi = p;
+        if(dR) *dR = g[0];
+        if(dR2) *dR2 = h[0];
+        return;
+    }
+    const int nphi = 8;
+    const bool ysym = has(sym, ST_YREFL);
+    const int npoints = ysym ? nphi+1 : 2*nphi+1;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for(int i=0; i<npoints; i++) {
+        pot.evalCylHess(R, z, M_PI/(nphi+0.5) * i, &p, g, h);
+        const double vals[6] = { p, g[0], g[1], h[0], h[1], h[3] };
+        for(int k=0; k<6; k++) acc[k] += vals[k] * (ysym && i>0 ? 2 : 1);
+    }
+    for(int k=0; k<6; k++) acc[k] *= 1./(2*nphi+1);
+    if(Phi) *Phi = acc[0];
+    if(dR) *dR = acc[1];
+    if(dR2) *dR2 = acc[3];
+}
+
+// math_core.h:84-89 (cubic Hermite interpolation) and math_core.cpp:866-888 (its monotonic form)
+inline double hermiteInterp(double x, double x1, double x2, double y1, double y2, double dy1, double dy2)
+{
+    const double t = (x-x1) / (x2-x1);
+    return pow_2(1-t) * ( (1+2*t) * y1 +   t   * dy1 * (x2-x1) )
+         + pow_2(t)   * ( (3-2*t) * y2 + (t-1) * dy2 * (x2-x1) );
+}
+inline double hermiteInterpMonotone(double x, double x1, double x2,
+    double f1, double f2, double dfdx1, double dfdx2)
+{
+    if(!isFinite(dfdx1+dfdx2) || (dfdx1>=0 && dfdx2<0) || (dfdx2>=0 && dfdx1<0))
+        return NAN;
+    const double dx = x2-x1,
+        ai = 1 / (-6 * (f2-f1) / dx + 3 * (dfdx1 + dfdx2)),  ba = -1 + (dfdx2 - dfdx1) * ai,  ca = dfdx1 * ai,
+        D  = pow_2(ba) - 4 * ca;
+    if(!isFinite(ba+D))
+        return NAN;
+    if(D >= 0) {
+        double sqD = sqrt(D);
+        double t1 = 0.5 * (-ba-sqD);
+        double t2 = 0.5 * (-ba+sqD);
+        if( (t1>=0 && t1<=1) || (t2>=0 && t2<=1) )
+            return NAN;
+    }
+    return hermiteInterp(x, x1, x2, f1, f2, dfdx1, dfdx2);
+}
+
+// math_core.cpp:895-1040  Brent's method with Hermite steps from the derivatives
+double findRootDeriv(const std::function<void(double, double*, double*)>& fnc,
+    const double xlower, const double xupper, const double reltoler)
+{
+    const int MAXITER = 64;
+    const double DELTA_HERMITE = 1e-12;
+    double a = xlower, b = xupper, fa, fb, fdera = NAN, fderb = NAN;
+    fnc(a, &fa, &fdera);
+    fnc(b, &fb, &fderb);
+    if((fa < 0.0 && fb < 0.0) || (fa > 0.0 && fb > 0.0) || !isFinite(fa+fb))
+        return NAN;
+    if(fa==0)
+        return (fb==0) ? 0.5 * (a+b) : a;
+    if(fb==0)
+        return b;
+    double c = a, fc = fa, fderc = fdera, d = b - c, e = b - c;
+    if(fabs(fc) < fabs(fb)) {
+        a = b; b = c; c = a;
+        fa = fb; fb = fc; fc = fa;
+        fdera = fderb; fderb = fderc; fderc = fdera;
+    }
+    int numIter = 0;
+    bool converged = false;
+    const double abstoler = fabs(xlower-xupper) * reltoler;
+    do {
+        const double tol = 0.5 * DBL_EPSILON * fabs(b);
+        const double cminusb = c-b, m = 0.5 * cminusb;
+        if(fb == 0 || fabs(m) <= tol)
+            return b;
+        if(fabs(e) < tol || fabs(fa) <= fabs(fb)) {
+            d = m;
+            e = m;
+        } else {
+            double dd = NAN;
+            if(fderb * fderc > 0)
+                dd = hermiteInterpMonotone(0, fb, fc, 0, cminusb, 1/fderb, 1/fderc);
+            if(isFinite(dd) && fmin(fabs(dd), fabs(cminusb-dd)) > fabs(cminusb) * DELTA_HERMITE) {
+                d = dd;
+            } else {
+                double p, q, r, s = fb / fa;
+                if(a == c) {
+                    p = 2 * m * s;
+                    q = 1 - s;
+                } else {
+                    q = fa / fc;
+                    r = fb / fc;
+                    p = s * (2 * m * q * (q - r) - (b - a) * (r - 1));
+                    q = (q - 1) * (r - 1) * (s - 1);
+                }
+                if(p > 0)
+                    q = -q;
+                else
+                    p = -p;
+                if(2 * p < fmin(3 * m * q - fabs(tol * q), fabs(e * q))) {
+                    e = d;
+                    d = p / q;
+                } else {
+                    d = m;
+                    e = m;
+                }
+            }
+        }
+        a = b;
+        fa = fb;
+        fdera = fderb;
+        if(fabs(d) > tol)
+            b += d;
+        else
+            b += (m > 0 ? +tol : -tol);
+        fnc(b, &fb, &fderb);
+        if(!isFinite(fb))
+            return NAN;
+        if((fb < 0 && fc < 0) || (fb > 0 && fc > 0)) {
+            c = a; fc = fa; fderc = fdera;
+            d = b - c;
+            e = b - c;
+        }
+        if(fabs(fc) < fabs(fb)) {
+            a = b; b = c; c = a;
+            fa = fb; fb = fc; fc = fa;
+            fdera = fderb; fderb = fderc; fderc = fdera;
+        }
+        numIter++;
+        if(fabs(b-c) <= abstoler) {
+            converged = true;
+            double offset = fb*(b-c)/(fc-fb);
+            if((offset>0 && offset<c-b) || (offset<0 && offset>c-b))
+                b += offset;
+        } else {
+            double offset = -fb / fderb;
+            bool bracketed = (offset>0 && offset<c-b) || (offset<0 && offset>c-b);
+            if(bracketed && fabs(offset) < abstoler) {
+                converged = true;
+                b += offset;
+            }
+        }
+        if(numIter >= MAXITER)
+            converged = true;
+    } while(!converged);
+    return b;
+}
+
+// potential_utils.cpp:264-289, 608-619; potential_utils.h:83-87: T_circ(E) = 2 pi R_circ / v_circ
+double T_circ(const orc_pot& pot, double E)
+{
+    const double HUGE_NUMBER = 1e100, ACCURACY_ROOT = 1e-10;
+    // RcircRootFinder on u = ln R, mapped to s in [0,1] by ScalingInf: u = 1/(1-s) - 1/s
+    auto rootFnc = [&](double s, double* val, double* der) {
+        const double duds = 1 / pow_2(1-s) + 1 / pow_2(s);
+        const double logR = 1 / (1-s) - 1 / s;
+        double Phi, dR, dR2, R = exp(logR);
+        axisymmetrizedEval(pot, R, 0, &Phi, &dR, &dR2);
+        double v = 0.5 * R * dR;
+        *val = Phi - E + (isFinite(v) ? v : 0);
+        if(!isFinite(*val)) {
+            if(E==-INFINITY)
+                *val = Phi==E ? 0 : +1.0;
+            else if(Phi==-INFINITY)
+                *val = -1.0;
+            else if(R>=HUGE_NUMBER)
+                *val = +1.0;
+        }
+        double dfdu = (1.5 * dR + 0.5 * R * dR2) * R;
+        *der = dfdu * duds;
+    };
+    const double s = findRootDeriv(rootFnc, 0, 1, ACCURACY_ROOT);
+    const double R = exp(1 / (1-s) - 1 / s);
+    double vc;
+    if(R==0) {
+        double Phi0;
+        axisymmetrizedEval(pot, 0, 0, &Phi0, NULL, NULL);
+        vc = isFinite(Phi0) ? 0 : INFINITY;
+    } else {
+        double dR;
+        axisymmetrizedEval(pot, R, 0, NULL, &dR, NULL);
+        vc = sqrt(R * dR);
+    }
+    return R / vc * 2*M_PI;
+}
+
 // x'' = A(t) x + B(t) x' with 3x3 matrices (IOdeSystem2ndOrderLinear, math_ode.h:74-86);
 // t is measured from the start of the current step
 struct LinSys2 {
@@ -2225,6 +2480,7 @@ struct OrbitResult { long nsteps, nattempts, nevals; double maxde; int status; d
 struct VarOutput {
     double* dev = nullptr;
     bool wantLyap = false;
+    double orbitalPeriod = NAN;
     std::vector<std::pair<double, double> > logDev;
 };
 
@@ -2395,6 +2651,13 @@ void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart
         const double tend = tbegin + timestep;
         if(vT0 != vT0) {
             vT0 = tbegin;
+            // orbital period at the energy of the starting point (orbit_variational.cpp:131-135)
+            double p0[6], Epot;
+            getSolAt(0, tbegin, p0);
+            pot.evalCar(p0, &Epot, NULL, tbegin);
+            var->orbitalPeriod = T_circ(pot, Epot + 0.5*(p0[3]*p0[3]+p0[4]*p0[4]+p0[5]*p0[5]));
+            if(!isFinite(var->orbitalPeriod))
+                var->orbitalPeriod = 0;
             if(numVectors == 6) {   // identity, rotated into the inertial frame at the start
                 double dv[36] = {0};
                 double ca, sa;
@@ -2659,6 +2922,76 @@ int orc_orbit_many_method(const orc_pot* p, long n, const double* ic, const doub
     if(fail) g_err = "exception during orbit integration";
     return fail;
 }
+
+// calcLyapunovExponent, first part (orbit_variational.cpp:372-404): the log-magnitude series
+// coarse-grained into bins of at least 5 points and max(1% of the duration, one period)
+int lyapunovBins(const std::vector<std::pair<double, double> >& series, double orbitalPeriod,
+    int maxBins, double* t, double* lnw, double* maxTimeOut)
+{
+    if(series.empty()) { *maxTimeOut = NAN; return 0; }
+    const double maxTime = series.back().first;
+    *maxTimeOut = maxTime;
+    const int MIN_BIN_SIZE = 5;
+    const double minBinTime = fmax(maxTime*0.01, orbitalPeriod);
+    double binStart = NAN, logOffset = 0, sumDevVecBin = 0;
+    int binSize = 0, nb = 0;
+    for(size_t i=0, size=series.size()-1; i<size; i++) {
+        const double ti = series[i].first, vi = series[i].second;
+        const double tp = series[i+1].first, vp = series[i+1].second;
+        if(binStart != binStart) {
+            binStart = ti;
+            logOffset = vi;
+            sumDevVecBin = 0;
+            binSize = 0;
+        }
+        sumDevVecBin += (tp-ti) * exp(0.5 * (vi + vp) - logOffset);
+        binSize++;
+        if(binSize >= MIN_BIN_SIZE && (tp >= binStart + minBinTime || i+1 >= size)) {
+            if(nb < maxBins) {
+                t[nb] = 0.5 * (binStart + tp);
+                lnw[nb] = log(sumDevVecBin / (tp - binStart)) + logOffset;
+            }
+            nb++;
+            binStart = NAN;
+        }
+    }
+    return nb;
+}
+
+int orc_orbit_many_lyap(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, const long* traj_offset,
+    double accuracy, double Omega, long maxNumSteps, double* out_dev, int maxBins,
+    double* out_bins, int* out_nbins, double* out_period, double* out_maxtime, int nthreads)
+{
+    g_err.clear();
+    for(long i=0; i<n; i++)
+        if(trajsize[i] < 1) { g_err = "trajsize must be >= 1"; return 1; }
+    int fail = 0;
+#ifdef _OPENMP
+    if(nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for(long orb=0; orb<n; orb++) {
+        OrbitResult r;
+        VarOutput v;
+        v.dev = out_dev ? out_dev + traj_offset[orb] * 36 : NULL;
+        v.wantLyap = true;
+        try {
+            integrateOneOrbit(*p, &ic[orb*6], tstart ? tstart[orb] : 0, ttotal[orb],
+                trajsize[orb], accuracy, Omega, maxNumSteps, NULL, NULL, r, 0, &v);
+        } catch(std::exception&) {
+            fail = 1;
+            continue;
+        }
+        double* b = out_bins + (long)orb * 2 * maxBins;
+        out_nbins[orb] = lyapunovBins(v.logDev, v.orbitalPeriod, maxBins, b, b + maxBins, &out_maxtime[orb]);
+        out_period[orb] = v.orbitalPeriod;
+    }
+    if(fail) g_err = "exception during orbit integration";
+    return fail;
+}
+
+double orc_tcirc(const orc_pot* p, double E) { return T_circ(*p, E); }
 
 int orc_orbit_many_var(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, const int* trajsize, const long* traj_offset,

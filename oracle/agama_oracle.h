@@ -55,6 +55,18 @@ int orc_orbit_many_var(const orc_pot* p, long n, const double* ic, const double*
     double* out_time, double* out_traj, double* out_final, double* out_dev,
     long* out_nsteps, long* out_nlogdev, int nthreads, int method);
 
+/* agama.orbit(lyapunov=True[, der=True]) up to the fit: per orbit the orbital period T_circ(E)
+   of the starting point (potential_utils.h:83-87), the coarse-grained log-magnitude series of
+   the deviation vector (orbit_variational.cpp:372-404) as out_bins[orb][2][maxBins] = (t, ln w),
+   their count out_nbins (may exceed maxBins: then truncated) and the last recorded time;
+   out_dev as in orc_orbit_many_var when non-NULL (six vectors), else a single vector. */
+int orc_orbit_many_lyap(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, const long* traj_offset,
+    double accuracy, double Omega, long maxNumSteps, double* out_dev, int maxBins,
+    double* out_bins, int* out_nbins, double* out_period, double* out_maxtime, int nthreads);
+/* T_circ(E) of the potential (R_circ by findRoot over ln R, potential_utils.cpp:264-289, 608-619) */
+double orc_tcirc(const orc_pot* p, double E);
+
 /* potential, gradient and Hessian {dx2, dy2, dz2, dxdy, dydz, dxdz} of Phi (not of the
    acceleration) at n points; any output may be NULL.  BasePotential::eval with deriv2
    (potential_base.h:153-156). */

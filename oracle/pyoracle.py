@@ -46,6 +46,10 @@ def _load(name):
         "orc_orbit_many_var": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
                                               ctypes.c_double, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp,
                                               ctypes.c_int, ctypes.c_int]),
+        "orc_orbit_many_lyap": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_long, _vp, ctypes.c_int, _vp, _vp, _vp,
+                                               _vp, ctypes.c_int]),
+        "orc_tcirc": (ctypes.c_double, [_vp, ctypes.c_double]),
         "orc_ode_gl4": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                        _vp, _vp]),
         "orc_ode_dop853_fixed": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
@@ -191,6 +195,99 @@ def _orbits_var(self, ic, ttotal, trajsize=1, tstart=None, accuracy=1e-8, Omega=
 
 
 OraclePotential.orbits_var = _orbits_var
+
+MAX_LYAP_BINS = 128
+
+
+def _orbits_lyap(self, ic, ttotal, tstart=None, accuracy=1e-8, Omega=0.0, maxNumSteps=int(1e8), der=False,
+                 nthreads=0):
+    """agama.orbit(lyapunov=True[, der=True]) as the oracle restates it: per orbit the
+    (lambda * T_circ, t_chaos / T_circ) pair, with the bins, period and last time behind it"""
+    ic = np.ascontiguousarray(ic, dtype=np.float64).reshape(-1, 6)
+    n = len(ic)
+    tt = np.ascontiguousarray(np.broadcast_to(np.asarray(ttotal, dtype=np.float64), (n,)))
+    sz = np.ones(n, np.int32)
+    ts = None if tstart is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(tstart, dtype=np.float64), (n,)))
+    offs = np.arange(n, dtype=np.int64)
+    dev = np.empty(n * 36) if der else None
+    bins = np.empty((n, 2, MAX_LYAP_BINS))
+    nb = np.empty(n, np.int32)
+    period = np.empty(n)
+    last = np.empty(n)
+    rc = self.lib.orc_orbit_many_lyap(self.h, n, _p(ic), _p(ts), _p(tt), _p(sz), _p(offs), accuracy, Omega,
+                                      maxNumSteps, _p(dev), MAX_LYAP_BINS, _p(bins), _p(nb), _p(period),
+                                      _p(last), nthreads)
+    if rc:
+        raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
+    out = np.array([lyapunov_from_bins(bins[i, 0, :nb[i]], bins[i, 1, :nb[i]], last[i], period[i])
+                    for i in range(n)])
+    return dict(lyap=out, bins=bins, nbins=nb, period=period, last=last)
+
+
+OraclePotential.orbits_lyap = _orbits_lyap
+OraclePotential.tcirc = lambda self, E: self.lib.orc_tcirc(self.h, float(E))
+
+
+def _logexpm1overx(x):
+    x = np.asarray(x, dtype=np.float64)
+    with np.errstate(over="ignore", divide="ignore", invalid="ignore"):
+        return np.where(np.abs(x) < 1e-2, x * (0.5 + x * (1. / 24 - (1. / 2880) * x * x)),
+                        np.where(x < 33, np.log(np.expm1(np.minimum(x, 33)) / np.where(x == 0, 1, x)),
+                                 x - np.log(np.abs(x))))
+
+
+def _derlogexpm1overx(x):
+    x = np.asarray(x, dtype=np.float64)
+    with np.errstate(over="ignore", divide="ignore", invalid="ignore"):
+        return np.where(np.abs(x) < 1e-2, 0.5 + x * (1. / 12 - (1. / 720) * x * x),
+                        -1 / np.where(x == 0, 1, x) - 1 / np.expm1(-x))
+
+
+def lyapunov_from_bins(t, lnw, max_time, period):
+    """calcLyapunovExponent's fit and decisions (src/orbit_variational.cpp:406-448) on the bins:
+    gsl_fit_linear's running-mean slope/intercept as the starting point, then a MINPACK
+    Levenberg-Marquardt (scipy's 'lm', the algorithm behind GSL's lmsder) of the model
+    A + ln t + [t >= tch] (F(lambda t) - F(lambda tch)), F(x) = ln((e^x - 1)/x), iterated to
+    the least-squares minimum (the reference stops GSL's iterations at a 1e-6 relative change
+    of the parameters)"""
+    from scipy.optimize import least_squares
+    t = np.asarray(t, dtype=np.float64)
+    lnw = np.asarray(lnw, dtype=np.float64)
+    if not (max_time == max_time) or len(t) < 5:
+        return np.array([np.nan, np.nan])
+    mx = my = dx2 = dxdy = 0.0
+    for i in range(len(t)):
+        mx += (t[i] - mx) / (i + 1.0)
+        my += (lnw[i] - my) / (i + 1.0)
+    for i in range(len(t)):
+        dx, dy = t[i] - mx, lnw[i] - my
+        dx2 += (dx * dx - dx2) / (i + 1.0)
+        dxdy += (dx * dy - dxdy) / (i + 1.0)
+    slope = dxdy / dx2
+    p0 = np.array([slope, my - mx * slope, max_time * 0.5])
+
+    def fun(p):
+        lam, A, tch = p
+        r = A + np.log(t) + np.where(t < tch, 0, _logexpm1overx(lam * t) - _logexpm1overx(lam * tch)) - lnw
+        return np.where(np.isfinite(r), r, 1e10)
+
+    def jac(p):
+        lam, A, tch = p
+        J = np.empty((len(t), 3))
+        J[:, 0] = np.where(t < tch, 0, (t - tch) * _derlogexpm1overx(lam * t))
+        J[:, 1] = 1
+        J[:, 2] = np.where(t < tch, 0, -lam * _derlogexpm1overx(lam * tch))
+        return J
+    res = least_squares(fun, p0, jac=jac, method="lm", xtol=1e-14, ftol=1e-15, gtol=1e-15, max_nfev=2000)
+    lam, _, tch = res.x
+    if lam < 0 or tch >= t[-2]:
+        return np.array([0.0, np.inf])
+    if tch < t[0]:
+        tch = 0.0
+    if float(_logexpm1overx(lam * t[-1]) - _logexpm1overx(lam * tch)) < np.log(2):
+        return np.array([0.0, np.inf])
+    return np.array([lam * period, tch / period])
 
 
 def ref_lib():
