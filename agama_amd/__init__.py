@@ -226,7 +226,13 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
     array of (trajsize, 6) arrays (interface_python.cpp:8035-8047, 8205-8232).
     lyapunov=True adds an (N, 2) array: the Lyapunov exponent in units of the orbital period
     T_circ(E) and the chaos onset time in the same unit (0 and inf for regular orbits,
-    orbit_variational.cpp:360-448) as the last item."""
+    orbit_variational.cpp:360-448) as the last item.
+
+    trajsize=0 records every integration step (the start, then each step's end; orbit.cpp:38-45)
+    and returns arrays of each orbit's own length.  dtype=object returns agama.Orbit objects
+    (`Orbit`: quintic-spline interpolators of the trajectory, interface_python.cpp:6605-6880)
+    instead of arrays, recorded at every step unless trajsize is given (:7864-7869); with der=True
+    the deviation vectors become an (N, 6) array of Orbit objects too."""
     surrogate = isinstance(potential, Surrogate)
     if not surrogate and not isinstance(potential, Potential):
         raise TypeError("Argument 'potential' must be a valid Potential object")
@@ -243,7 +249,8 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
     if der not in (False, True, 0, 1):
         raise TypeError("Argument 'der' must be a boolean or an int 0/1")
     der = bool(der)
-    if der and trajsize is None:
+    obj = dtype is object or (isinstance(dtype, str) and dtype in ("object", "O")) or dtype is np.object_
+    if der and trajsize is None and not obj:
         raise RuntimeError("Derivatives are requested, but the trajectory itself is not: provide "
                            "trajsize=... for array output or dtype=object for Orbit object output")
     if der and surrogate:
@@ -266,6 +273,7 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
         tt = np.full(n, float(tt))
     elif tt.shape != (n,):
         raise ValueError("Argument 'time' must either be a scalar or have the same length as the number of points in the initial conditions")
+    tt = np.ascontiguousarray(tt)
     ts = None
     if timestart is not None:
         ts = np.asarray(timestart, dtype=np.float64)
@@ -278,17 +286,25 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
         sz = np.full(n, int(sz), dtype=np.int32) if sz.ndim == 0 else np.ascontiguousarray(sz, dtype=np.int32)
         if sz.shape != (n,) or np.any(sz < 0):
             raise ValueError("Argument 'trajsize', if provided, must either be a nonnegative integer or an array of integers with the same length as the number of points in the initial conditions")
-        if np.any(sz == 0):
-            raise AgamaAmdError("trajsize=0 (every timestep) is not provided by this version")
+    elif obj:   # Orbit objects without trajsize: every timestep (interface_python.cpp:7864-7869)
+        sz = np.zeros(n, dtype=np.int32)
+        have_traj = True
     else:
         sz = np.ones(n, dtype=np.int32)
-    if dtype in ("float32", np.float32, "float", float, "float64", np.float64, "double"):
+    if obj:
+        out_dtype = object
+    elif dtype in ("float32", np.float32, "float", float, "float64", np.float64, "double"):
         out_dtype = np.float32 if dtype in ("float32", np.float32) else np.float64
     elif dtype in ("complex64", "complex128", "complex", np.complex64, np.complex128):
         out_dtype = np.complex64 if dtype in ("complex64", np.complex64) else np.complex128
     else:
         raise AgamaAmdError(f"dtype={dtype} is not provided by this version")
     maxsteps = int(1e8) if maxNumSteps is None else int(maxNumSteps)
+    if have_traj and (obj or np.any(sz == 0)):
+        if surrogate:
+            raise AgamaAmdError("trajsize=0 and dtype=object are integrated with the FP64 Multipole only")
+        return _recorded_orbits(potential, n, single, ic2, ts, tt, sz, accuracy, Omega, maxsteps, method_flag,
+                                precision, der, lyapunov, obj, out_dtype, separateTime, info)
     rows = int(sz.sum())
     times = np.empty(rows)
     traj = np.empty((rows, 6))
@@ -330,10 +346,58 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
     return out
 
 
-def _trajectory_outputs(n, single, sz, times, traj, dev, out_dtype, separateTime):
+def _recorded_orbits(potential, n, single, ic2, ts, tt, sz, accuracy, Omega, maxsteps, method_flag, precision,
+                     der, lyapunov, obj, out_dtype, separateTime, info):
+    """orbit() with every-step recording (some trajsize 0) or Orbit-object output: the batch is
+    recorded in device memory (agx_orbit_record_batch), then returned as arrays of each orbit's
+    length or as Orbit objects over that record (interface_python.cpp:8165-8232)"""
+    from .orbitobj import Orbit, _Record
+    rows = np.empty(n, dtype=np.int64)
+    final = np.empty((n, 6))
+    nsteps = np.empty(n, dtype=np.int64)
+    natt = np.empty(n, dtype=np.int64)
+    nev = np.empty(n, dtype=np.int64)
+    maxde = np.empty(n)
+    status = np.empty(n, dtype=np.int32)
+    h = lib.agx_orbit_record_batch(potential.handle, n, ptr(ic2), ptr(ts), ptr(tt), ptr(sz), float(accuracy),
+                                   float(Omega), maxsteps, int(der), ptr(rows), ptr(final), ptr(nsteps), ptr(natt),
+                                   ptr(nev), ptr(maxde), ptr(status), int(precision) | method_flag)
+    rec = _Record(h, n, der)
+    lyap = None
+    if lyapunov:   # the exponent from the one-vector variational equation, as without a record
+        lyap = np.empty((n, 2))
+        check(lib.agx_orbit_var_batch(potential.handle, n, ptr(ic2), ptr(ts), ptr(tt),
+                                      ptr(np.ones(n, dtype=np.int32)), float(accuracy), float(Omega), maxsteps,
+                                      None, None, None, None, ptr(lyap), None, None, None, None, None,
+                                      int(precision) | method_flag), "orbit")
+    if isinstance(info, dict):
+        info.update(nsteps=nsteps, nattempts=natt, nevals=nev, maxde=maxde, status=status, final=final)
+    if obj:
+        orbits = np.empty(n, dtype=object)
+        for i in range(n):
+            orbits[i] = Orbit._make(rec, i, Omega)
+        out = orbits[0] if single else orbits
+        if der:
+            darr = np.empty((n, 6), dtype=object)
+            for i in range(n):
+                for k in range(6):
+                    darr[i, k] = Orbit._make(rec, n + 6 * i + k, Omega)
+            out = (out, darr[0] if single else darr)
+    else:
+        times, traj, dev = rec.fetch(rows, der)
+        out = _trajectory_outputs(n, single, rows, times, traj, dev, out_dtype, separateTime, uniform=False)
+    if lyapunov:
+        out = (out if isinstance(out, tuple) else (out,)) + ((lyap[0] if single else lyap),)
+    return out
+
+
+def _trajectory_outputs(n, single, sz, times, traj, dev, out_dtype, separateTime, uniform=None):
     """the trajectory (and deviation-vector) outputs in the reference's layouts
-    (interface_python.cpp:7995-8047)"""
+    (interface_python.cpp:7995-8047); uniform: one trajsize > 0 for all orbits (the 2d/3d
+    layout of separateTime=True)"""
     der = dev is not None
+    if uniform is None:
+        uniform = bool(np.all(sz == sz[0]))
 
     def conv(a):
         if out_dtype in (np.complex64, np.complex128):
@@ -341,7 +405,7 @@ def _trajectory_outputs(n, single, sz, times, traj, dev, out_dtype, separateTime
         return a.astype(out_dtype)
 
     offs = np.concatenate([[0], np.cumsum(sz)])
-    if separateTime and np.all(sz == sz[0]):
+    if separateTime and uniform:
         S = int(sz[0])
         t_out = times.reshape(n, S)
         x_out = conv(traj.reshape(n, S, 6))
@@ -374,3 +438,4 @@ def _trajectory_outputs(n, single, sz, times, traj, dev, out_dtype, separateTime
 
 
 from .surrogate import Surrogate  # noqa: E402
+from .orbitobj import Orbit  # noqa: E402,F401

@@ -36,6 +36,10 @@ hipError_t agxLaunchTcirc(const AgxMultipoleDev* dP, long n, const double* ic, c
 hipError_t agxLaunchFillU32(unsigned int* p, long n, unsigned int v, hipStream_t stream);
 hipError_t agxLaunchEvalHess(const AgxMultipoleDev* dP, long n, const double* pos, double* phi,
     double* grad, double* hess, hipStream_t stream);
+hipError_t agxLaunchBuildSplines(const AgxSeries* dSeries, long ns, double Omega, double* nodes,
+    double* pivots, int* count, int* reversed, hipStream_t stream);
+hipError_t agxLaunchEvalSplines(const AgxSeries* dSeries, const int* count, const double* nodes, double Omega,
+    long m, const long* which, const double* t, int mode, double* out, hipStream_t stream);
 int agxOrbitKernelBlockSize();
 size_t agxScheduleWorkspace(long n);
 hipError_t agxLaunchSchedule(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, long n,
@@ -766,6 +770,325 @@ int agx_orbit_var_batch(const agx_potential* pot, long n, const double* ic, cons
     return orbitBatchHost(pot, n, ic, tstart, ttotal, trajsize, accuracy, Omega, maxNumSteps, out_time,
         out_traj, out_final, out_dev, out_nsteps, out_nattempts, out_nevals, out_maxde, out_status, flags,
         out_lyap);
+}
+
+}  // extern "C"
+
+// ---------------- every-step recording and interpolated orbits ----------------
+// The trajectory (and deviation vectors) of a batch kept in device memory: orbits with
+// trajsize 0 record the start and the end of every step (orbit.cpp:38-45), the others trajsize
+// rows at regular intervals.  The number of steps is known only at the end, so the first
+// integration reserves a fixed block per every-step orbit, and an orbit that took more steps
+// is integrated again with a block of its exact size (the integration does not depend on what
+// is recorded, so the second one repeats the first step for step).  The quintic splines of
+// agama.Orbit objects are built from these rows on first use (agx_orbit_interp.hip).
+struct agx_orbit_record {
+    int device = 0;
+    long n = 0, deviceRows = 0;
+    double Omega = 0;
+    bool der = false;
+    std::vector<long> off, cap, rows;   // per orbit: first device row, rows reserved, rows returned
+    std::unique_ptr<DevBuf<double>> time, traj, dev;
+    // interpolators: series 0..n-1 are the trajectories, n + 6*i + k the k-th deviation vector
+    // of orbit i
+    std::unique_ptr<DevBuf<AgxSeries>> series;
+    std::unique_ptr<DevBuf<double>> nodes;
+    std::unique_ptr<DevBuf<int>> count;
+    std::vector<long> nodeOff;
+    std::vector<int> hcount, hrev;
+    long numSeries() const { return n * (der ? 7 : 1); }
+};
+
+namespace {
+
+long g_recordRows = 0;   // first-pass row budget of agx_orbit_record_batch (0: default)
+
+// rows an every-step orbit recorded: the start and each step's end; a zero-length (or
+// non-finite) request records the start twice; a failure before the first step records nothing
+long everyStepRows(long nsteps, double tstart, double ttotal)
+{
+    if(nsteps > 0) return nsteps + 1;
+    const double te = tstart + ttotal;
+    return ttotal == 0 || !(te > -INFINITY && te < INFINITY) ? 2 : 0;
+}
+
+void buildSplines(agx_orbit_record& R)
+{
+    if(R.series) return;
+    const long ns = R.numSeries();
+    std::vector<AgxSeries> S(ns);
+    R.nodeOff.assign(ns, 0);
+    long nodes = 0;
+    for(long s = 0; s < ns; s++) {
+        const long i = s < R.n ? s : (s - R.n) / 6, k = s < R.n ? -1 : (s - R.n) % 6;
+        S[s].t = R.time->p + R.off[i];
+        S[s].v = k < 0 ? R.traj->p + R.off[i] * 6 : R.dev->p + R.off[i] * 36 + k * R.cap[i] * 6;
+        S[s].rows = R.rows[i];
+        S[s].node = nodes;
+        R.nodeOff[s] = nodes;
+        nodes += R.rows[i];
+    }
+    std::unique_ptr<DevBuf<AgxSeries>> dS(new DevBuf<AgxSeries>(std::max<long>(ns, 1)));
+    std::unique_ptr<DevBuf<double>> dN(new DevBuf<double>(std::max<long>(nodes, 1) * AGX_NODE_DOUBLES));
+    std::unique_ptr<DevBuf<int>> dC(new DevBuf<int>(std::max<long>(ns, 1)));
+    DevBuf<double> piv(std::max<long>(nodes, 1));
+    DevBuf<int> rev(std::max<long>(ns, 1));
+    if(ns) HIPCHECK(hipMemcpy(dS->p, S.data(), ns * sizeof(AgxSeries), hipMemcpyHostToDevice));
+    HIPCHECK(agxLaunchBuildSplines(dS->p, ns, R.Omega, dN->p, piv.p, dC->p, rev.p, nullptr));
+    R.hcount.assign(ns, 0);
+    R.hrev.assign(ns, 0);
+    if(ns) {
+        HIPCHECK(hipMemcpy(R.hcount.data(), dC->p, ns * 4, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(R.hrev.data(), rev.p, ns * 4, hipMemcpyDeviceToHost));
+    }
+    R.series = std::move(dS);
+    R.nodes = std::move(dN);
+    R.count = std::move(dC);
+}
+
+}  // namespace
+
+extern "C" {
+
+agx_orbit_record* agx_orbit_record_batch(const agx_potential* pot, long n, const double* ic,
+    const double* tstart, const double* ttotal, const int* trajsize, double accuracy, double Omega,
+    long maxNumSteps, int der, long* out_rows, double* out_final, long* out_nsteps, long* out_nattempts,
+    long* out_nevals, double* out_maxde, int* out_status, int flags)
+{
+    g_err.clear();
+    if(!pot) { fail(AGX_EINVAL, "NULL potential"); return nullptr; }
+    if(n < 0 || n > 0x7fffffffL || (n > 0 && (!ic || !ttotal || !trajsize))) {
+        fail(AGX_EINVAL, "invalid orbit arrays");
+        return nullptr;
+    }
+    for(long i = 0; i < n; i++)
+        if(trajsize[i] < 0) { fail(AGX_EINVAL, "trajsize must be >= 0"); return nullptr; }
+    if(maxNumSteps <= 0) { fail(AGX_EINVAL, "maxNumSteps must be positive"); return nullptr; }
+    try {
+        HIPCHECK(hipSetDevice(pot->device));
+        std::unique_ptr<agx_orbit_record> R(new agx_orbit_record);
+        R->device = pot->device;
+        R->n = n;
+        R->Omega = Omega;
+        R->der = der != 0;
+        R->off.assign(n, 0); R->cap.assign(n, 0); R->rows.assign(n, 0);
+        long nEvery = 0;
+        for(long i = 0; i < n; i++) nEvery += trajsize[i] == 0;
+        // first-pass block of an every-step orbit: a share of 2^24 rows (2^21 with deviation
+        // vectors, or the budget set by agx_set_record_rows), between 2 and 65536 rows, and no
+        // more than the step limit allows
+        const long budget = g_recordRows > 0 ? g_recordRows : R->der ? (1L << 21) : (1L << 24);
+        long cap0 = nEvery ? std::max(g_recordRows > 0 ? 2L : 64L, std::min(budget / nEvery, 1L << 16)) : 0;
+        cap0 = std::min(cap0, std::max(2L, maxNumSteps + 1));
+        std::vector<int> ksz(n);
+        long total = 0;
+        for(long i = 0; i < n; i++) {
+            const long c = trajsize[i] > 0 ? trajsize[i] : cap0;
+            R->cap[i] = c;
+            R->off[i] = total;
+            total += c;
+            ksz[i] = trajsize[i] > 0 ? trajsize[i] : -(int)c;
+        }
+        std::vector<double> fin(n * 6), mde(n);
+        std::vector<long> ns(n), na(n), ne(n);
+        std::vector<int> st(n);
+        // one integration of the orbits idx (all when empty) into the record's rows
+        auto integrate = [&](const std::vector<long>& idx) -> int {
+            const long m = idx.empty() ? n : (long)idx.size();
+            auto at = [&](long j) { return idx.empty() ? j : idx[j]; };
+            std::vector<double> hic(m * 6), hts(tstart ? m : 0), htt(m);
+            std::vector<int> hsz(m);
+            std::vector<long> hoff(m);
+            for(long j = 0; j < m; j++) {
+                const long i = at(j);
+                for(int c = 0; c < 6; c++) hic[j * 6 + c] = ic[i * 6 + c];
+                if(tstart) hts[j] = tstart[i];
+                htt[j] = ttotal[i];
+                hsz[j] = ksz[i];
+                hoff[j] = R->off[i];
+            }
+            DevBuf<double> dic(m * 6), dts(tstart ? m : 0), dtt(m), dfin(m * 6), dmde(m);
+            DevBuf<int> dsz(m), dst(m);
+            DevBuf<long> doff(m), dns(m), dna(m), dne(m);
+            HIPCHECK(hipMemcpy(dic.p, hic.data(), m * 48, hipMemcpyHostToDevice));
+            if(tstart) HIPCHECK(hipMemcpy(dts.p, hts.data(), m * 8, hipMemcpyHostToDevice));
+            HIPCHECK(hipMemcpy(dtt.p, htt.data(), m * 8, hipMemcpyHostToDevice));
+            HIPCHECK(hipMemcpy(dsz.p, hsz.data(), m * 4, hipMemcpyHostToDevice));
+            HIPCHECK(hipMemcpy(doff.p, hoff.data(), m * 8, hipMemcpyHostToDevice));
+            int rc = orbitBatchDevice(pot, m, dic.p, dts.p, dtt.p, dsz.p, doff.p, nullptr, accuracy, Omega,
+                maxNumSteps, R->time->p, R->traj->p, dfin.p, R->der ? R->dev->p : nullptr, dns.p, dna.p,
+                dne.p, dmde.p, dst.p, flags, nullptr);
+            if(rc) return rc;
+            HIPCHECK(hipDeviceSynchronize());
+            std::vector<double> f2(m * 6), d2(m);
+            std::vector<long> s2(m), a2(m), e2(m);
+            std::vector<int> t2(m);
+            HIPCHECK(hipMemcpy(f2.data(), dfin.p, m * 48, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(d2.data(), dmde.p, m * 8, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(s2.data(), dns.p, m * 8, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(a2.data(), dna.p, m * 8, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(e2.data(), dne.p, m * 8, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(t2.data(), dst.p, m * 4, hipMemcpyDeviceToHost));
+            for(long j = 0; j < m; j++) {
+                const long i = at(j);
+                for(int c = 0; c < 6; c++) fin[i * 6 + c] = f2[j * 6 + c];
+                mde[i] = d2[j]; ns[i] = s2[j]; na[i] = a2[j]; ne[i] = e2[j]; st[i] = t2[j];
+            }
+            return AGX_OK;
+        };
+        R->time.reset(new DevBuf<double>(std::max<long>(total, 1)));
+        R->traj.reset(new DevBuf<double>(std::max<long>(total, 1) * 6));
+        R->dev.reset(new DevBuf<double>(R->der ? std::max<long>(total, 1) * 36 : 0));
+        R->deviceRows = total;
+        if(n > 0) {
+            int rc = integrate({});
+            if(rc) return nullptr;
+        }
+        std::vector<long> redo;
+        for(long i = 0; i < n; i++) {
+            if(trajsize[i] > 0) { R->rows[i] = trajsize[i]; continue; }
+            const long need = everyStepRows(ns[i], tstart ? tstart[i] : 0, ttotal[i]);
+            if(need > R->cap[i]) redo.push_back(i);
+            else R->rows[i] = need;
+        }
+        if(!redo.empty()) {
+            // blocks of the exact size (plus a margin) appended after the first pass's rows
+            long extra = 0;
+            for(long i : redo) extra += everyStepRows(ns[i], tstart ? tstart[i] : 0, ttotal[i]) + 16;
+            const long total2 = total + extra;
+            std::unique_ptr<DevBuf<double>> t2(new DevBuf<double>(total2)), x2(new DevBuf<double>(total2 * 6)),
+                v2(new DevBuf<double>(R->der ? total2 * 36 : 0));
+            HIPCHECK(hipMemcpy(t2->p, R->time->p, total * 8, hipMemcpyDeviceToDevice));
+            HIPCHECK(hipMemcpy(x2->p, R->traj->p, total * 48, hipMemcpyDeviceToDevice));
+            if(R->der) HIPCHECK(hipMemcpy(v2->p, R->dev->p, total * 288, hipMemcpyDeviceToDevice));
+            R->time = std::move(t2); R->traj = std::move(x2); R->dev = std::move(v2);
+            long o = total;
+            for(long i : redo) {
+                const long c = everyStepRows(ns[i], tstart ? tstart[i] : 0, ttotal[i]) + 16;
+                R->off[i] = o; R->cap[i] = c; ksz[i] = -(int)c;
+                o += c;
+            }
+            R->deviceRows = total2;
+            int rc = integrate(redo);
+            if(rc) return nullptr;
+            for(long i : redo) {
+                const long need = everyStepRows(ns[i], tstart ? tstart[i] : 0, ttotal[i]);
+                if(need > R->cap[i]) {
+                    fail(AGX_EDEVICE, "an orbit took a different number of steps when integrated again");
+                    return nullptr;
+                }
+                R->rows[i] = need;
+            }
+        }
+        for(long i = 0; i < n; i++) {
+            if(out_rows) out_rows[i] = R->rows[i];
+            if(out_final) for(int c = 0; c < 6; c++) out_final[i * 6 + c] = fin[i * 6 + c];
+            if(out_nsteps) out_nsteps[i] = ns[i];
+            if(out_nattempts) out_nattempts[i] = na[i];
+            if(out_nevals) out_nevals[i] = ne[i];
+            if(out_maxde) out_maxde[i] = mde[i];
+            if(out_status) out_status[i] = st[i];
+        }
+        return R.release();
+    } catch(std::exception& e) { fail(AGX_EDEVICE, e.what()); return nullptr; }
+}
+
+int agx_orbit_record_fetch(const agx_orbit_record* R, double* out_time, double* out_traj, double* out_dev)
+{
+    g_err.clear();
+    if(!R) return fail(AGX_EINVAL, "NULL record");
+    if(out_dev && !R->der) return fail(AGX_EINVAL, "the record holds no deviation vectors");
+    try {
+        HIPCHECK(hipSetDevice(R->device));
+        const long D = R->deviceRows;
+        std::vector<double> t(out_time ? D : 0), x(out_traj ? D * 6 : 0), v(out_dev ? D * 36 : 0);
+        if(out_time && D) HIPCHECK(hipMemcpy(t.data(), R->time->p, D * 8, hipMemcpyDeviceToHost));
+        if(out_traj && D) HIPCHECK(hipMemcpy(x.data(), R->traj->p, D * 48, hipMemcpyDeviceToHost));
+        if(out_dev && D) HIPCHECK(hipMemcpy(v.data(), R->dev->p, D * 288, hipMemcpyDeviceToHost));
+        long o = 0;
+        for(long i = 0; i < R->n; i++) {
+            const long r = R->rows[i], a = R->off[i];
+            if(out_time) std::copy(t.begin() + a, t.begin() + a + r, out_time + o);
+            if(out_traj) std::copy(x.begin() + a * 6, x.begin() + (a + r) * 6, out_traj + o * 6);
+            if(out_dev)   // six vectors of r rows each (the device block reserves cap rows per vector)
+                for(int k = 0; k < 6; k++)
+                    std::copy(v.begin() + a * 36 + k * R->cap[i] * 6, v.begin() + a * 36 + (k * R->cap[i] + r) * 6,
+                        out_dev + o * 36 + k * r * 6);
+            o += r;
+        }
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+long agx_orbit_record_num_series(const agx_orbit_record* R) { return R ? R->numSeries() : -1; }
+
+int agx_set_record_rows(long rows)
+{
+    if(rows < 0) return fail(AGX_EINVAL, "rows must be >= 0");
+    g_recordRows = rows;
+    return AGX_OK;
+}
+
+int agx_orbit_record_series_info(agx_orbit_record* R, long* out_nodes, int* out_reversed)
+{
+    g_err.clear();
+    if(!R) return fail(AGX_EINVAL, "NULL record");
+    try {
+        HIPCHECK(hipSetDevice(R->device));
+        buildSplines(*R);
+        for(long s = 0; s < R->numSeries(); s++) {
+            if(out_nodes) out_nodes[s] = R->hcount[s];
+            if(out_reversed) out_reversed[s] = R->hrev[s];
+        }
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+int agx_orbit_record_node_times(agx_orbit_record* R, long series, double* out_t)
+{
+    g_err.clear();
+    if(!R || !out_t) return fail(AGX_EINVAL, "NULL argument");
+    if(series < 0 || series >= R->numSeries()) return fail(AGX_EINVAL, "series index out of range");
+    try {
+        HIPCHECK(hipSetDevice(R->device));
+        buildSplines(*R);
+        const long k = R->hcount[series];
+        if(k) HIPCHECK(hipMemcpy2D(out_t, 8, R->nodes->p + R->nodeOff[series] * AGX_NODE_DOUBLES,
+            AGX_NODE_DOUBLES * 8, 8, k, hipMemcpyDeviceToHost));
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+int agx_orbit_record_interp(agx_orbit_record* R, long m, const long* series, const double* t, int mode,
+    double* out)
+{
+    g_err.clear();
+    if(!R) return fail(AGX_EINVAL, "NULL record");
+    if(m < 0 || (m > 0 && (!series || !t || !out))) return fail(AGX_EINVAL, "invalid query arrays");
+    if(mode != 0 && mode != 1) return fail(AGX_EINVAL, "mode must be 0 or 1");
+    const long ns = R->numSeries();
+    for(long q = 0; q < m; q++)
+        if(series[q] < 0 || series[q] >= ns) return fail(AGX_EINVAL, "series index out of range");
+    if(m == 0) return AGX_OK;
+    try {
+        HIPCHECK(hipSetDevice(R->device));
+        buildSplines(*R);
+        const int width = mode == 0 ? 6 : 9;
+        DevBuf<long> dq(m);
+        DevBuf<double> dt(m), dout(m * width);
+        HIPCHECK(hipMemcpy(dq.p, series, m * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dt.p, t, m * 8, hipMemcpyHostToDevice));
+        HIPCHECK(agxLaunchEvalSplines(R->series->p, R->count->p, R->nodes->p, R->Omega, m, dq.p, dt.p, mode,
+            dout.p, nullptr));
+        HIPCHECK(hipMemcpy(out, dout.p, m * width * 8, hipMemcpyDeviceToHost));
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+void agx_orbit_record_free(agx_orbit_record* R)
+{
+    if(R) (void)hipSetDevice(R->device);
+    delete R;
 }
 
 }  // extern "C"

@@ -161,29 +161,54 @@ __device__ __forceinline__ long trajOffsetOf(const AgxOrbitArgs& A, int oi)
     return A.trajOffset ? A.trajOffset[oi] : 0;
 }
 
-// RuntimeTrajectory::processTimestep (orbit.cpp:15-47) after an accepted step [tb, tb+h]
+// row capacity of an orbit's trajectory block: trajsize S >= 1 rows, or -S rows when S < 0
+// (sampling interval 0: every step is recorded)
+__device__ __forceinline__ long rowCapacity(int S) { return S < 0 ? -(long)S : (long)S; }
+
+// RuntimeTrajectory::processTimestep (orbit.cpp:15-47) after an accepted step [tb, tb+h]:
+// rows at regular intervals (trajsize S >= 2), or with sampling interval 0 (S < 0) the start of
+// the first step and then the end of every step (orbit.cpp:38-45), in a block of -S rows whose
+// later rows are not written (the host integrates such an orbit again with a block of
+// nsteps + 1 rows).  Both share one loop over the rows a step produces.
 template<class Lane>
 __device__ __forceinline__ void sampleStep(const AgxOrbitArgs& A, Lane& L)
 {
-    if(L.S < 2) return;   // a single final point is written when the orbit ends
+    if(L.S == 0 || L.S == 1) return;   // a single final point is written when the orbit ends
+    const bool every = L.S < 0;
     const double tbegin = L.tb, timestep = L.h;
-    double t0 = L.f(F_T0);
-    if(t0 != t0) { t0 = tbegin; L.f(F_T0) = t0; }
-    const double dtS = L.f(F_DTS);
-    const double sign = timestep >= 0 ? +1 : -1, tendS = tbegin + timestep;
-    const double dtr = ROUNDOFF * fmax(fmax(fabs(tendS), fabs(tbegin)), fabs(t0));
-    long iout = (long)ceil(sign * (tbegin - t0) / dtS);
-    const long iend = (long)((sign * (tendS - t0) + dtr) / dtS);
-    L.actual = (int)min(iend + 1, (long)L.S + 1);
+    double t0 = 0, dtS = 0, sign = 1;
+    long iout, iend;
+    if(every) {
+        iend = (long)L.nsteps + 1;
+        iout = L.nsteps == 0 ? 0 : iend;
+        L.actual = 1;
+    } else {
+        t0 = L.f(F_T0);
+        if(t0 != t0) { t0 = tbegin; L.f(F_T0) = t0; }
+        dtS = L.f(F_DTS);
+        sign = timestep >= 0 ? +1 : -1;
+        const double tendS = tbegin + timestep;
+        const double dtr = ROUNDOFF * fmax(fmax(fabs(tendS), fabs(tbegin)), fabs(t0));
+        iout = (long)ceil(sign * (tbegin - t0) / dtS);
+        iend = (long)((sign * (tendS - t0) + dtr) / dtS);
+        L.actual = (int)min(iend + 1, (long)L.S + 1);
+    }
     if(iout > iend) return;
-    const long base = trajOffsetOf(A, L.oi);
+    const long base = trajOffsetOf(A, L.oi), lim = rowCapacity(L.S);
     for(; iout <= iend; iout++) {
-        const double timeout = sign * dtS * iout + t0;
-        double off = fmin(fmax(sign * (timeout - tbegin), 0.0), sign * timestep) * sign;
+        double timeout, off;
+        if(every) {
+            const bool last = iout == iend;
+            timeout = last ? tbegin + timestep : tbegin;
+            off = last ? timestep : 0.0;
+        } else {
+            timeout = sign * dtS * iout + t0;
+            off = fmin(fmax(sign * (timeout - tbegin), 0.0), sign * timestep) * sign;
+        }
         double d[6], o[6];
         for(int c = 0; c < 6; c++) d[c] = denseOut(L, c, off);
         toRotating(A.Omega, A.Omega * (off + tbegin), d, o);
-        if(iout >= 0 && iout < L.S) storeRow(A, base + iout, timeout, o);
+        if(iout >= 0 && iout < lim) storeRow(A, base + iout, timeout, o);
     }
 }
 
@@ -342,7 +367,7 @@ __device__ __forceinline__ void advancePhase(const AgxOrbitArgs& A, Lane& L, con
         if(A.ttotal[L.oi] == 0 || !(L.tend > -INFINITY && L.tend < INFINITY)) {
             L.f(F_LASTTB) = L.tb;
             L.f(F_LASTH) = 0;
-            if(L.S >= 2) {   // sampling interval 0: the start is recorded twice
+            if(L.S >= 2 || L.S < 0) {   // sampling interval 0: the start is recorded twice
                 double o[6];
                 toRotating(A.Omega, A.Omega * L.tb, L.x, o);
                 const long base = trajOffsetOf(A, L.oi);

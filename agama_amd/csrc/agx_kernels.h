@@ -7,7 +7,7 @@ struct AgxOrbitArgs {
     const double* ic;        // n x 6 (x, y, z, vx, vy, vz), rotating-frame input as in the reference
     const double* tstart;    // n or null (0)
     const double* ttotal;    // n
-    const int* trajsize;     // n (>= 1)
+    const int* trajsize;     // n: >= 1 rows at regular intervals; -c: every step, c rows reserved
     const long* trajOffset;  // n: first output row of each orbit
     const int* order;        // n or null: processing order (a permutation of 0..n-1)
     double accuracy, Omega;
@@ -28,3 +28,16 @@ enum AgxPrecision { AGX_FP64 = 0, AGX_FP32_TABLES = 1 };
 // how FP64 node records are fetched: per-lane 8-byte loads, per-lane 16-byte loads, or the
 // wavefront-cooperative gather through LDS (agx_device.h)
 enum AgxFetch { AGX_FETCH_SCALAR = 0, AGX_FETCH_VEC2 = 1, AGX_FETCH_COOP = 2 };
+
+// one recorded series - an orbit's trajectory or one of its deviation vectors - behind the
+// quintic-spline interpolators of Orbit objects (agx_orbit_interp.hip)
+struct AgxSeries {
+    const double* t;         // timestamps of the rows
+    const double* v;         // rows x 6: position and velocity part, rotating frame
+    long rows;               // rows in the block; recording stops at the first NaN timestamp
+    long node;               // first node of the series in the node array
+};
+
+// spline node: time, position (3), first derivative (3), second derivative (3) in the
+// inertial frame
+enum : int { AGX_NODE_DOUBLES = 10 };

@@ -92,6 +92,21 @@ __device__ __forceinline__ double denseOutRKN(const LaneRKN<STRIDE>& L, const do
 template<int STRIDE>
 __device__ __forceinline__ void sampleStepRKN(const AgxOrbitArgs& A, LaneRKN<STRIDE>& L, const double* kn)
 {
+    if(L.S < 0) {   // sampling interval 0: getSol(0) of the first step, then every step's end
+        const long cap = rowCapacity(L.S), base = trajOffsetOf(A, L.oi), row = (long)L.nsteps + 1;
+        double d[6], o[6];
+        if(L.nsteps == 0) {
+            for(int c = 0; c < 6; c++) d[c] = denseOutRKN(L, kn, c, 0.0);
+            toRotating(A.Omega, A.Omega * L.tb, d, o);
+            storeRow(A, base, L.tb, o);
+        }
+        if(row < cap) {
+            toRotating(A.Omega, A.Omega * (L.h + L.tb), L.y, o);
+            storeRow(A, base + row, L.tb + L.h, o);
+        }
+        L.actual = 1;
+        return;
+    }
     if(L.S < 2) return;
     const double tbegin = L.tb, timestep = L.h;
     double t0 = L.f(F_T0);
@@ -200,7 +215,7 @@ __device__ __forceinline__ void advanceRKN(const AgxOrbitArgs& A, LaneRKN<STRIDE
         if(A.ttotal[L.oi] == 0 || !(L.tend > -INFINITY && L.tend < INFINITY)) {
             L.f(F_LASTTB) = L.tb;
             L.f(F_LASTH) = 0;
-            if(L.S >= 2) {
+            if(L.S >= 2 || L.S < 0) {
                 double o[6];
                 toRotating(A.Omega, A.Omega * L.tb, L.x, o);
                 const long base = trajOffsetOf(A, L.oi);

@@ -131,8 +131,9 @@ __device__ __forceinline__ double storeSolution(const AgxOrbitArgs& A, LaneVarT<
         double q[6];
         for(int c = 0; c < 6; c++) q[c] = gl4Sol(L, timeOffset, vec * 6 + c);
         if(NV == 1) return logMagnitude6(q);
-        if(write && ind < L.S) {
-            double* o = L.dev + (trajOffsetOf(A, L.oi) * 36 + ((long)vec * L.S + ind) * 6);
+        const long cap = rowCapacity(L.S);
+        if(write && ind < cap) {
+            double* o = L.dev + (trajOffsetOf(A, L.oi) * 36 + ((long)vec * cap + ind) * 6);
             o[0] = (q[0] * ca + q[1] * sa) * mul;
             o[1] = (q[1] * ca - q[0] * sa) * mul;
             o[2] = (q[2] * mul);
@@ -317,7 +318,7 @@ __device__ __forceinline__ void finishOrbit(const AgxOrbitArgs& A, LaneVarT<S, N
     if(zeroLength) {
         varInit(A, L, L.tb);
         // doStep(0) leaves x and x' unchanged; sampling interval 0 records the start again
-        if(NV == 6 && L.dev && L.S >= 2) storeSolution(A, L, L.tb, 0, (long)L.v(V_DEVSIZE));
+        if(NV == 6 && L.dev && (L.S >= 2 || L.S < 0)) storeSolution(A, L, L.tb, 0, (long)L.v(V_DEVSIZE));
         if(L.ly.bins) binPoint(L, fabs(L.tb - L.v(V_T0)), storeSolution(A, L, L.tb, 0, -1));
     }
     if(L.ly.bins) binFinish(L);
@@ -408,7 +409,9 @@ __device__ __forceinline__ void advanceVar(const AgxOrbitArgs& A, LaneVarT<S, NV
             storeSolution(A, L, tbegin, (xo > hi ? hi : xo > 0.0 ? xo : 0.0) * sign, iout);
         }
     }
-    const double logDevVec = storeSolution(A, L, tbegin, timestep, NV == 6 && L.dev && L.S == 1 ? 0 : -1);
+    // one row (trajsize 1), the next row (sampling interval 0, trajsize < 0 here) or none
+    const long ind = !(NV == 6 && L.dev) ? -1 : L.S == 1 ? 0 : L.S < 0 ? (long)L.v(V_DEVSIZE) : -1;
+    const double logDevVec = storeSolution(A, L, tbegin, timestep, ind);
     if(L.ly.bins) binPoint(L, fabs(tend - t0), logDevVec + L.v(V_ADDLOG));
     if(logDevVec > 10) varRenormalize(L, timestep);
     // the orbit's clock and step count (orbit.cpp:79-82)

@@ -175,6 +175,49 @@ int agx_orbit_var_batch_device(const agx_potential* pot, long n, const double* d
     long maxNumSteps, double* d_time, double* d_traj, double* d_final, double* d_dev, long* d_nsteps,
     long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags, void* stream);
 
+/* ---- every-step trajectories and interpolated orbits (agama.orbit with trajsize=0 or
+   dtype=object; src/interface_python.cpp:7577-7584, 7864-7869, 8121-8127, 6605-6880) ---- */
+typedef struct agx_orbit_record agx_orbit_record;
+/* Integrate a batch and keep what it recorded in device memory: orbit i records trajsize[i]
+   rows at regular intervals (as agx_orbit_batch) or, when trajsize[i] == 0, the start and the
+   end of every accepted step (RuntimeTrajectory with sampling interval 0, src/orbit.cpp:38-45),
+   in the rotating frame; der != 0 also records the six deviation vectors at the same times
+   (DOP853, FP64 only, as agx_orbit_var_batch).  out_rows[n]: rows of orbit i (trajsize[i], or
+   the recorded count when it is 0).  The other outputs are those of agx_orbit_batch; any may be
+   NULL.  Returns NULL on error (agx_last_error).  An every-step orbit that needs more rows than
+   the first pass reserved (a share of 2^24 rows) is integrated a second time. */
+agx_orbit_record* agx_orbit_record_batch(const agx_potential* pot, long n, const double* ic,
+    const double* tstart, const double* ttotal, const int* trajsize, double accuracy, double Omega,
+    long maxNumSteps, int der, long* out_rows, double* out_final, long* out_nsteps, long* out_nattempts,
+    long* out_nevals, double* out_maxde, int* out_status, int flags);
+/* The recorded rows, orbit after orbit (orbit i at row offset sum_{j<i} out_rows[j]):
+   out_time[rows], out_traj[rows*6], out_dev[rows*36] as [vector][row][6] per orbit (any may
+   be NULL; out_dev needs der). */
+int agx_orbit_record_fetch(const agx_orbit_record* rec, double* out_time, double* out_traj,
+    double* out_dev);
+/* Interpolated orbits (agama.Orbit, createOrbitObject, interface_python.cpp:6793-6880): each
+   series - the trajectory of orbit i (series i) or its k-th deviation vector (series n + 6*i + k,
+   with der) - becomes three quintic splines x(t), y(t), z(t) in the inertial frame through the
+   recorded positions with the velocities as derivatives (repeated timestamps skipped, time
+   order restored for backward integration); built on the device on first use.
+   num_series: n or 7n.  series_info: nodes per series (the Orbit's length) and whether it was
+   recorded backward in time.  node_times: the node times of one series in increasing order. */
+long agx_orbit_record_num_series(const agx_orbit_record* rec);
+/* Rows the first pass of agx_orbit_record_batch shares among its every-step orbits (0: the
+   default 2^24, or 2^21 with deviation vectors); a small budget sends most orbits through the
+   second integration. */
+int agx_set_record_rows(long rows);
+int agx_orbit_record_series_info(agx_orbit_record* rec, long* out_nodes, int* out_reversed);
+int agx_orbit_record_node_times(agx_orbit_record* rec, long series, double* out_t);
+/* Evaluate m queries (series[q], t[q]) on the device.  mode 0 (Orbit.__call__,
+   interface_python.cpp:6643-6714): out[m*6] position and velocity rotated back into the
+   rotating frame at angle Omega*t, NaN outside the recorded time span; mode 1 (the x, y, z
+   Spline members): out[m*9] = x, y, z, their first and their second derivatives in the inertial
+   frame, continued linearly beyond the ends (QuinticSpline::evalDeriv, math_spline.cpp:1405-1429). */
+int agx_orbit_record_interp(agx_orbit_record* rec, long m, const long* series, const double* t, int mode,
+    double* out);
+void agx_orbit_record_free(agx_orbit_record* rec);
+
 /* Launch geometry of the orbit kernel: number of workgroups (0 = one full wave of resident
    workgroups on every CU) and threads per workgroup (0 or 256, the only size it is built for).
    Set by the autotuner (agama_amd.tune). */

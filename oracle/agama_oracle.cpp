@@ -2472,7 +2472,7 @@ inline double energyOf(const double* xv, double Epot, double Omega)
 
 // orbit.cpp:52-84 (run), 98-133 (init/getSol for Car), 15-47 (RuntimeTrajectory), and the
 // per-orbit body of interface_python.cpp:8107-8202 with trajectory stored as float64
-struct OrbitResult { long nsteps, nattempts, nevals; double maxde; int status; double fin[6]; };
+struct OrbitResult { long nsteps, nattempts, nevals; double maxde; int status; double fin[6]; long rows; };
 
 // outputs of RuntimeVariational (orbit_variational.cpp): the six deviation vectors sampled like
 // the trajectory (dev[(vec*trajsize + row)*6 + c], NaN-padded), and the per-step log-magnitude
@@ -2488,9 +2488,12 @@ struct VarOutput {
 template<class Stepper>
 void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart, double ttotal,
     int trajsize, double accuracy, double Omega, long maxNumSteps,
-    double* out_time, double* out_traj, OrbitResult& res, VarOutput* var = nullptr)
+    double* out_time, double* out_traj, OrbitResult& res, VarOutput* var = nullptr, long capacity = -1)
 {
     const double ROUNDOFF = 10*DBL_EPSILON;
+    // rows available in the output block: trajsize, or `capacity` (trajsize 0 records every
+    // step; rows beyond the block are counted but not written)
+    const long rowsMax = capacity >= 0 ? capacity : trajsize;
     OrbitSys sys(pot, Omega);
     Stepper stepper(sys, accuracy);
     double samplingInterval = trajsize>1 ? fabs(ttotal) / (trajsize-1) :
@@ -2530,7 +2533,7 @@ void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart
     auto getSol = [&](double timeOffset, double out[6]) { getSolAt(timeOffset, sys.timeBegin, out); };
     double lastTb = NAN, lastH = 0;
     auto store = [&](long idx, double t, const double* p) {
-        if(idx < 0 || idx >= trajsize) return;
+        if(idx < 0 || idx >= rowsMax) return;
         if(out_time) out_time[idx] = t;
         if(out_traj) for(int c=0; c<6; c++) out_traj[idx*6+c] = p[c];
     };
@@ -2614,12 +2617,12 @@ void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart
         for(int vec=0; vec<numVectors; vec++) {
             if(var->dev && ind >= 0) {
                 devSize = ind + 1;
-                if(ind < trajsize) {
+                if(ind < rowsMax) {
                     double ca=1, sa=0;
                     if(Omega)
                         sincos_r(Omega * (vTimeBegin + timeOffset), sa, ca);
                     const double* q = Q + vec*6;
-                    double* o = var->dev + ((long)vec * trajsize + ind) * 6;
+                    double* o = var->dev + ((long)vec * rowsMax + ind) * 6;
                     o[0] = (q[0] * ca + q[1] * sa) * mul;
                     o[1] = (q[1] * ca - q[0] * sa) * mul;
                     o[2] = (q[2] * mul);
@@ -2734,22 +2737,24 @@ void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart
     if(var && var->dev)
         for(int vec=0; vec<6; vec++)
             for(long idx = devSize; idx < trajsize; idx++)
-                for(int c=0; c<6; c++) var->dev[((long)vec * trajsize + idx) * 6 + c] = NAN;
+                for(int c=0; c<6; c++) var->dev[((long)vec * rowsMax + idx) * 6 + c] = NAN;
     res.nsteps = numSteps;
+    res.rows = actualSize;
     res.nattempts = stepper.nattempts;
     res.nevals = sys.nevals;
 }
 
 void integrateOneOrbit(const orc_pot& pot, const double ic[6], double tstart, double ttotal,
     int trajsize, double accuracy, double Omega, long maxNumSteps,
-    double* out_time, double* out_traj, OrbitResult& res, int method=0, VarOutput* var=nullptr)
+    double* out_time, double* out_traj, OrbitResult& res, int method=0, VarOutput* var=nullptr,
+    long capacity=-1)
 {
     if(method == 1)
         integrateOneOrbitWith<DPRKN8>(pot, ic, tstart, ttotal, trajsize, accuracy, Omega, maxNumSteps,
-            out_time, out_traj, res, var);
+            out_time, out_traj, res, var, capacity);
     else
         integrateOneOrbitWith<DOP853>(pot, ic, tstart, ttotal, trajsize, accuracy, Omega, maxNumSteps,
-            out_time, out_traj, res, var);
+            out_time, out_traj, res, var, capacity);
 }
 
 struct CallbackSys : public OdeSys {
@@ -3026,6 +3031,140 @@ int orc_orbit_many_var(const orc_pot* p, long n, const double* ic, const double*
     }
     if(fail) g_err = "exception during orbit integration";
     return fail;
+}
+
+int orc_orbit_steps(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, const long* capacity, const long* traj_offset, double accuracy, double Omega,
+    long maxNumSteps, int method, int der, double* out_time, double* out_traj, double* out_dev,
+    long* out_rows, long* out_nsteps, int* out_status, int nthreads)
+{
+    g_err.clear();
+    int fail = 0;
+#ifdef _OPENMP
+    if(nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for(long orb=0; orb<n; orb++) {
+        OrbitResult r;
+        VarOutput v;
+        v.dev = der && out_dev ? out_dev + traj_offset[orb] * 36 : NULL;
+        try {
+            integrateOneOrbit(*p, &ic[orb*6], tstart ? tstart[orb] : 0, ttotal[orb],
+                0, accuracy, Omega, maxNumSteps,
+                out_time ? out_time + traj_offset[orb] : NULL,
+                out_traj ? out_traj + traj_offset[orb]*6 : NULL, r, method, der ? &v : NULL, capacity[orb]);
+        } catch(std::exception&) {
+            fail = 1;
+            continue;
+        }
+        if(out_rows) out_rows[orb] = r.rows;
+        if(out_nsteps) out_nsteps[orb] = r.nsteps;
+        if(out_status) out_status[orb] = r.status;
+    }
+    if(fail) g_err = "exception during orbit integration";
+    return fail;
+}
+
+// createOrbitObject (interface_python.cpp:6793-6880) and Orbit.__call__ (:6643-6714) on one
+// recorded series: rows into the inertial frame at angle Omega*t, repeated timestamps skipped,
+// time order restored when integrated backward; three quintic splines from values and first
+// derivatives (QuinticSpline(x, f, df), math_spline.cpp:1376-1389); evaluated with
+// QuinticSpline::evalDeriv (math_spline.cpp:1405-1429) and binSearch (math_core.cpp:251-281)
+static long binSearchRef(double x, const double* arr, long size)
+{
+    if(size < 1 || !(x >= arr[0])) return -1;
+    if(x > arr[size-1] || size < 2) return size-1;
+    long index = static_cast<long>((x-arr[0]) / (arr[size-1]-arr[0]) * (size-1));
+    long indhi = size-1;
+    if(index == size-1) return size-2;
+    if(x >= arr[index]) {
+        if(x < arr[index+1]) return index;
+    } else {
+        indhi = index;
+        index = 0;
+    }
+    while(indhi > index + 1) {
+        long i = (indhi + index)/2;
+        if(arr[i] > x) indhi = i;
+        else index = i;
+    }
+    return index;
+}
+
+struct Spline1dRef {
+    std::vector<double> x, f, d, d2;
+    void evalDeriv(double t, double* val, double* der, double* der2) const
+    {
+        const long size = x.size();
+        if(size == 0 || t != t) { *val = *der = *der2 = NAN; return; }
+        const long index = binSearchRef(t, &x[0], size);
+        if(index < 0) {
+            *val = f[0] + (d[0]==0 ? 0 : d[0] * (t-x[0])); *der = d[0]; *der2 = 0; return;
+        }
+        if(index >= size-1) {
+            *val = f[size-1] + (d[size-1]==0 ? 0 : d[size-1] * (t-x[size-1])); *der = d[size-1]; *der2 = 0;
+            return;
+        }
+        evalQuinticSplines<1>(t, x[index], x[index+1], &f[index], &f[index+1], &d[index], &d[index+1],
+            &d2[index], &d2[index+1], val, der, der2);
+    }
+};
+
+int orc_orbit_interp(long rows, const double* t, const double* traj, double Omega, long m, const double* tq,
+    double* out, double* out_spl, long* out_nodes, int* out_reversed)
+{
+    g_err.clear();
+    const bool reversed = rows > 1 && t[1] < t[0];
+    std::vector<double> tt, cx, cy, cz, vx, vy, vz;
+    for(long index=0; index<rows; index++) {
+        const long r = reversed ? rows-1-index : index;
+        const double ti = t[r];
+        const double* p = traj + r*6;
+        if(!tt.empty() && ti == tt.back()) continue;
+        tt.push_back(ti);
+        double ca, sa;
+        sincos_r(Omega * ti, sa, ca);
+        cx.push_back(p[0] * ca - p[1] * sa);
+        cy.push_back(p[1] * ca + p[0] * sa);
+        cz.push_back(p[2]);
+        vx.push_back(p[3] * ca - p[4] * sa);
+        vy.push_back(p[4] * ca + p[3] * sa);
+        vz.push_back(p[5]);
+    }
+    Spline1dRef S[3];
+    const std::vector<double>* fv[3] = { &cx, &cy, &cz };
+    const std::vector<double>* dv[3] = { &vx, &vy, &vz };
+    for(int k=0; k<3; k++) {
+        S[k].x = tt; S[k].f = *fv[k]; S[k].d = *dv[k];
+        S[k].d2 = constructQuinticSpline(tt, S[k].f, S[k].d);
+    }
+    if(out_nodes) *out_nodes = (long)tt.size();
+    if(out_reversed) *out_reversed = reversed ? 1 : 0;
+    const double tmin = tt.empty() ? NAN : tt.front(), tmax = tt.empty() ? NAN : tt.back();
+    for(long q=0; q<m; q++) {
+        const double tv = tq[q];
+        double v[3], d[3], d2[3];
+        for(int k=0; k<3; k++) S[k].evalDeriv(tv, &v[k], &d[k], &d2[k]);
+        if(out_spl)
+            for(int k=0; k<3; k++) {
+                out_spl[q*9+k] = tt.empty() ? NAN : v[k];
+                out_spl[q*9+3+k] = tt.empty() ? NAN : d[k];
+                out_spl[q*9+6+k] = tt.empty() ? NAN : d2[k];
+            }
+        if(!out) continue;
+        if(tv >= tmin && tv <= tmax) {
+            double ca=1, sa=0;
+            if(Omega != 0) sincos_r(Omega * tv, sa, ca);
+            out[q*6+0] =  v[0] * ca + v[1] * sa;
+            out[q*6+1] =  v[1] * ca - v[0] * sa;
+            out[q*6+2] =  v[2];
+            out[q*6+3] =  d[0] * ca + d[1] * sa;
+            out[q*6+4] =  d[1] * ca - d[0] * sa;
+            out[q*6+5] =  d[2];
+        } else
+            for(int c=0; c<6; c++) out[q*6+c] = NAN;
+    }
+    return 0;
 }
 
 int orc_ode_dop853(int ndim, orc_rhs_fn rhs, void* ctx, const double* x0, double ttotal,

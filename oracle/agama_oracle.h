@@ -89,6 +89,18 @@ int orc_orbit_many(const orc_pot* p, long n, const double* ic, const double* tst
     double* out_maxde, int* out_status, int nthreads);
 /* the same with the integrator chosen: method 0 = DOP853, 1 = DPRKN8
    (reference: OrbitIntParams::Method, src/orbit.h:116-129; src/math_ode.cpp:426-701) */
+/* every-step recording (trajsize 0, orbit.cpp:38-45): orbit i writes at most capacity[i] rows
+   at traj_offset[i] (with der: six deviation vectors [vec][capacity[i]][6] at traj_offset[i]*36);
+   out_rows = the rows it recorded (possibly more than capacity) */
+int orc_orbit_steps(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, const long* capacity, const long* traj_offset, double accuracy, double Omega,
+    long maxNumSteps, int method, int der, double* out_time, double* out_traj, double* out_dev,
+    long* out_rows, long* out_nsteps, int* out_status, int nthreads);
+/* agama.Orbit on one recorded series (createOrbitObject + Orbit.__call__): out[m*6] rotating
+   frame, NaN outside the span; out_spl[m*9] the inertial-frame splines x,y,z, their first and
+   second derivatives (linear continuation outside) */
+int orc_orbit_interp(long rows, const double* t, const double* traj, double Omega, long m, const double* tq,
+    double* out, double* out_spl, long* out_nodes, int* out_reversed);
 int orc_orbit_many_method(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, const int* trajsize, const long* traj_offset,
     double accuracy, double Omega, long maxNumSteps,

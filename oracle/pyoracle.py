@@ -50,6 +50,11 @@ def _load(name):
                                                ctypes.c_double, ctypes.c_long, _vp, ctypes.c_int, _vp, _vp, _vp,
                                                _vp, ctypes.c_int]),
         "orc_tcirc": (ctypes.c_double, [_vp, ctypes.c_double]),
+        "orc_orbit_steps": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_long, ctypes.c_int, ctypes.c_int, _vp, _vp,
+                                           _vp, _vp, _vp, _vp, ctypes.c_int]),
+        "orc_orbit_interp": (ctypes.c_int, [ctypes.c_long, _vp, _vp, ctypes.c_double, ctypes.c_long, _vp, _vp,
+                                            _vp, _vp, _vp]),
         "orc_ode_gl4": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                        _vp, _vp]),
         "orc_ode_dop853_fixed": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
@@ -226,6 +231,66 @@ def _orbits_lyap(self, ic, ttotal, tstart=None, accuracy=1e-8, Omega=0.0, maxNum
 
 
 OraclePotential.orbits_lyap = _orbits_lyap
+
+
+def _orbits_steps(self, ic, ttotal, tstart=None, accuracy=1e-8, Omega=0.0, maxNumSteps=int(1e8),
+                  method="dop853", der=False, nthreads=0):
+    """agama.orbit(trajsize=0) as the oracle restates it: every step recorded; returns per orbit
+    (times, traj[rows, 6]) or, with der, (times, traj, dev[6, rows, 6]), and the step counts.
+    Two passes: the first counts the rows, the second records them."""
+    ic = np.ascontiguousarray(ic, dtype=np.float64).reshape(-1, 6)
+    n = len(ic)
+    tt = np.ascontiguousarray(np.broadcast_to(np.asarray(ttotal, dtype=np.float64), (n,)))
+    ts = None if tstart is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(tstart, dtype=np.float64), (n,)))
+    m = {"dop853": 0, "dprkn8": 1}[method.lower()]
+    rows = np.empty(n, np.int64)
+    nsteps = np.empty(n, np.int64)
+    status = np.empty(n, np.int32)
+    zero = np.zeros(n, np.int64)
+    rc = self.lib.orc_orbit_steps(self.h, n, _p(ic), _p(ts), _p(tt), _p(zero), _p(zero), accuracy, Omega,
+                                  maxNumSteps, m, int(der), None, None, None, _p(rows), _p(nsteps), _p(status),
+                                  nthreads)
+    if rc:
+        raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
+    cap = rows.copy()
+    offs = np.concatenate([[0], np.cumsum(cap)[:-1]]).astype(np.int64)
+    total = int(cap.sum())
+    t = np.empty(total)
+    x = np.empty((total, 6))
+    d = np.empty(total * 36) if der else None
+    rc = self.lib.orc_orbit_steps(self.h, n, _p(ic), _p(ts), _p(tt), _p(cap), _p(offs), accuracy, Omega,
+                                  maxNumSteps, m, int(der), _p(t), _p(x), _p(d), _p(rows), _p(nsteps),
+                                  _p(status), nthreads)
+    if rc:
+        raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
+    out = []
+    for i in range(n):
+        a, r = offs[i], rows[i]
+        item = (t[a:a + r], x[a:a + r])
+        if der:
+            item += (d[a * 36:(a + r) * 36].reshape(6, r, 6),)
+        out.append(item)
+    return dict(series=out, rows=rows, nsteps=nsteps, status=status)
+
+
+OraclePotential.orbits_steps = _orbits_steps
+
+
+def orbit_interp(t, traj, Omega, tq):
+    """agama.Orbit built from one recorded series (times t, rows traj) and evaluated at tq:
+    (rotating-frame points [m, 6] with NaN outside, inertial splines [m, 9] = value, first and
+    second derivative of x, y, z, number of nodes, reversed)"""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    traj = np.ascontiguousarray(traj, dtype=np.float64).reshape(-1, 6)
+    tq = np.ascontiguousarray(np.atleast_1d(tq), dtype=np.float64)
+    out = np.empty((len(tq), 6))
+    spl = np.empty((len(tq), 9))
+    nodes = np.empty(1, np.int64)
+    rev = np.empty(1, np.int32)
+    lib.orc_orbit_interp(len(t), _p(t), _p(traj), float(Omega), len(tq), _p(tq), _p(out), _p(spl), _p(nodes),
+                         _p(rev))
+    return out, spl, int(nodes[0]), bool(rev[0])
 OraclePotential.tcirc = lambda self, E: self.lib.orc_tcirc(self.h, float(E))
 
 
