@@ -40,6 +40,8 @@ hipError_t agxLaunchBuildSplines(const AgxSeries* dSeries, long ns, double Omega
     double* pivots, int* count, int* reversed, hipStream_t stream);
 hipError_t agxLaunchEvalSplines(const AgxSeries* dSeries, const int* count, const double* nodes, double Omega,
     long m, const long* which, const double* t, int mode, double* out, hipStream_t stream);
+hipError_t agxLaunchGatherRows(long nseg, const long* srcRow, const long* dstRow, const long* rows, int width,
+    const double* src, double* dst, hipStream_t stream);
 int agxOrbitKernelBlockSize();
 size_t agxScheduleWorkspace(long n);
 hipError_t agxLaunchSchedule(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, long n,
@@ -457,6 +459,14 @@ struct LyapDev {
     int maxBins = 0;
 };
 
+// overflow pool of every-step recording (AgxOrbitArgs::poolCounter and the fields after it)
+struct RecordPool {
+    int* chunkDir = nullptr;
+    unsigned int* counter = nullptr;
+    long base = 0;
+    int chunk = 0, maxChunks = 0, chunks = 0;
+};
+
 // the batch entry points' common body; d_dev non-null or ly.bins non-null: the variational
 // kernel (der=True: six vectors; lyapunov only: one)
 static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic,
@@ -464,7 +474,7 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
     const long* d_traj_offset, const int* d_order, double accuracy, double Omega,
     long maxNumSteps, double* d_time, double* d_traj, double* d_final, double* d_dev, long* d_nsteps,
     long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags, void* stream,
-    const LyapDev& ly = LyapDev())
+    const LyapDev& ly = LyapDev(), const RecordPool& pool = RecordPool())
 {
     if(!pot) return fail(AGX_EINVAL, "NULL potential");
     if(n == 0) return AGX_OK;
@@ -504,6 +514,10 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
         A.maxNumSteps = maxNumSteps; A.outTime = d_time; A.outTraj = d_traj; A.outFinal = d_final;
         A.outNsteps = d_nsteps; A.outNattempts = d_nattempts; A.outNevals = d_nevals;
         A.outMaxde = d_maxde; A.outStatus = d_status; A.counter = pot->dcounter;
+        if(pool.counter) {
+            A.chunkDir = pool.chunkDir; A.poolCounter = pool.counter; A.poolBase = pool.base;
+            A.poolChunk = pool.chunk; A.maxChunks = pool.maxChunks; A.poolChunks = pool.chunks;
+        }
         int blocks, bs;
         orbitGeometry(n, precision, pot->nfwMass != 0, blocks, bs, flags & AGX_METHOD_DPRKN8);
         HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
@@ -872,28 +886,44 @@ agx_orbit_record* agx_orbit_record_batch(const agx_potential* pot, long n, const
         R->Omega = Omega;
         R->der = der != 0;
         R->off.assign(n, 0); R->cap.assign(n, 0); R->rows.assign(n, 0);
+        if(n == 0) {
+            R->time.reset(new DevBuf<double>(1));
+            R->traj.reset(new DevBuf<double>(6));
+            R->dev.reset(new DevBuf<double>(R->der ? 36 : 0));
+            return R.release();
+        }
         long nEvery = 0;
         for(long i = 0; i < n; i++) nEvery += trajsize[i] == 0;
-        // first-pass block of an every-step orbit: a share of 2^24 rows (2^21 with deviation
-        // vectors, or the budget set by agx_set_record_rows), between 2 and 65536 rows, and no
-        // more than the step limit allows
-        const long budget = g_recordRows > 0 ? g_recordRows : R->der ? (1L << 21) : (1L << 24);
-        long cap0 = nEvery ? std::max(g_recordRows > 0 ? 2L : 64L, std::min(budget / nEvery, 1L << 16)) : 0;
+        // Rows for every-step orbits: a budget of 2^28 rows (15 GB; 2^25 with deviation vectors,
+        // 11.5 GB; or the budget set by agx_set_record_rows), half of it as a block per orbit
+        // (64 to 32768 rows, at most the step limit + 1) and as much again (at most the rest of
+        // the budget) as an overflow pool of chunks of the block's size (at most 1024 rows) that
+        // lanes take as their orbits outgrow the block, up to 64 chunks per orbit.
+        const long budget = g_recordRows > 0 ? g_recordRows : R->der ? (1L << 25) : (1L << 28);
+        long cap0 = nEvery ? std::max(g_recordRows > 0 ? 2L : 64L, std::min(budget / (2 * nEvery), 1L << 15)) : 0;
         cap0 = std::min(cap0, std::max(2L, maxNumSteps + 1));
+        const long CH = std::max(4L, std::min(cap0, 1024L));
+        const int MAXCH = 64;
         std::vector<int> ksz(n);
-        long total = 0;
+        long blocks = 0;
         for(long i = 0; i < n; i++) {
             const long c = trajsize[i] > 0 ? trajsize[i] : cap0;
             R->cap[i] = c;
-            R->off[i] = total;
-            total += c;
+            R->off[i] = blocks;
+            blocks += c;
             ksz[i] = trajsize[i] > 0 ? trajsize[i] : -(int)c;
         }
+        const long poolRows = nEvery == 0 ? 0 : g_recordRows > 0 ? budget - nEvery * cap0 :
+            std::min(budget - nEvery * cap0, nEvery * cap0);
+        const long poolChunks = std::max(0L, poolRows / CH);
+        const long rows1 = blocks + poolChunks * CH;
         std::vector<double> fin(n * 6), mde(n);
         std::vector<long> ns(n), na(n), ne(n);
         std::vector<int> st(n);
-        // one integration of the orbits idx (all when empty) into the record's rows
-        auto integrate = [&](const std::vector<long>& idx) -> int {
+        // one integration of the orbits idx (all when empty) into the blocks at R->off with
+        // capacities ksz, writing into time/traj/dev
+        auto integrate = [&](const std::vector<long>& idx, double* dtime, double* dtraj, double* ddev,
+                             const RecordPool& pool) -> int {
             const long m = idx.empty() ? n : (long)idx.size();
             auto at = [&](long j) { return idx.empty() ? j : idx[j]; };
             std::vector<double> hic(m * 6), hts(tstart ? m : 0), htt(m);
@@ -916,8 +946,8 @@ agx_orbit_record* agx_orbit_record_batch(const agx_potential* pot, long n, const
             HIPCHECK(hipMemcpy(dsz.p, hsz.data(), m * 4, hipMemcpyHostToDevice));
             HIPCHECK(hipMemcpy(doff.p, hoff.data(), m * 8, hipMemcpyHostToDevice));
             int rc = orbitBatchDevice(pot, m, dic.p, dts.p, dtt.p, dsz.p, doff.p, nullptr, accuracy, Omega,
-                maxNumSteps, R->time->p, R->traj->p, dfin.p, R->der ? R->dev->p : nullptr, dns.p, dna.p,
-                dne.p, dmde.p, dst.p, flags, nullptr);
+                maxNumSteps, dtime, dtraj, dfin.p, ddev, dns.p, dna.p, dne.p, dmde.p, dst.p, flags, nullptr,
+                LyapDev(), pool);
             if(rc) return rc;
             HIPCHECK(hipDeviceSynchronize());
             std::vector<double> f2(m * 6), d2(m);
@@ -936,50 +966,102 @@ agx_orbit_record* agx_orbit_record_batch(const agx_potential* pot, long n, const
             }
             return AGX_OK;
         };
-        R->time.reset(new DevBuf<double>(std::max<long>(total, 1)));
-        R->traj.reset(new DevBuf<double>(std::max<long>(total, 1) * 6));
-        R->dev.reset(new DevBuf<double>(R->der ? std::max<long>(total, 1) * 36 : 0));
-        R->deviceRows = total;
-        if(n > 0) {
-            int rc = integrate({});
-            if(rc) return nullptr;
+        // first pass: blocks + pool
+        DevBuf<double> t1(rows1), x1(rows1 * 6), v1(R->der ? rows1 * 36 : 0);
+        DevBuf<int> dir(nEvery ? n * MAXCH : 0);
+        DevBuf<unsigned int> counter(1);
+        RecordPool pool;
+        if(nEvery) {
+            HIPCHECK(agxLaunchFillU32(reinterpret_cast<unsigned int*>(dir.p), n * MAXCH, 0xFFFFFFFFu, nullptr));
+            HIPCHECK(agxLaunchFillU32(counter.p, 1, 0u, nullptr));
+            pool.chunkDir = dir.p; pool.counter = counter.p; pool.base = blocks; pool.chunk = (int)CH;
+            pool.maxChunks = MAXCH; pool.chunks = (int)std::min<long>(poolChunks, 0x7fffffffL);
         }
-        std::vector<long> redo;
+        if(integrate({}, t1.p, x1.p, R->der ? v1.p : nullptr, pool)) return nullptr;
+        std::vector<int> hdir;
+        std::vector<long> need(n, 0), redo;
+        for(long i = 0; i < n; i++)
+            need[i] = trajsize[i] > 0 ? trajsize[i] : everyStepRows(ns[i], tstart ? tstart[i] : 0, ttotal[i]);
         for(long i = 0; i < n; i++) {
-            if(trajsize[i] > 0) { R->rows[i] = trajsize[i]; continue; }
-            const long need = everyStepRows(ns[i], tstart ? tstart[i] : 0, ttotal[i]);
-            if(need > R->cap[i]) redo.push_back(i);
-            else R->rows[i] = need;
-        }
-        if(!redo.empty()) {
-            // blocks of the exact size (plus a margin) appended after the first pass's rows
-            long extra = 0;
-            for(long i : redo) extra += everyStepRows(ns[i], tstart ? tstart[i] : 0, ttotal[i]) + 16;
-            const long total2 = total + extra;
-            std::unique_ptr<DevBuf<double>> t2(new DevBuf<double>(total2)), x2(new DevBuf<double>(total2 * 6)),
-                v2(new DevBuf<double>(R->der ? total2 * 36 : 0));
-            HIPCHECK(hipMemcpy(t2->p, R->time->p, total * 8, hipMemcpyDeviceToDevice));
-            HIPCHECK(hipMemcpy(x2->p, R->traj->p, total * 48, hipMemcpyDeviceToDevice));
-            if(R->der) HIPCHECK(hipMemcpy(v2->p, R->dev->p, total * 288, hipMemcpyDeviceToDevice));
-            R->time = std::move(t2); R->traj = std::move(x2); R->dev = std::move(v2);
-            long o = total;
-            for(long i : redo) {
-                const long c = everyStepRows(ns[i], tstart ? tstart[i] : 0, ttotal[i]) + 16;
-                R->off[i] = o; R->cap[i] = c; ksz[i] = -(int)c;
-                o += c;
+            if(trajsize[i] > 0 || need[i] <= R->cap[i]) continue;
+            if(hdir.empty()) {
+                hdir.resize(n * MAXCH);
+                HIPCHECK(hipMemcpy(hdir.data(), dir.p, n * MAXCH * 4, hipMemcpyDeviceToHost));
             }
-            R->deviceRows = total2;
-            int rc = integrate(redo);
-            if(rc) return nullptr;
+            const long chunks = (need[i] - R->cap[i] + CH - 1) / CH;
+            bool ok = chunks <= MAXCH;
+            for(long k = 0; ok && k < chunks; k++) ok = hdir[i * MAXCH + k] >= 0;
+            if(!ok) redo.push_back(i);
+        }
+        // orbits the pool could not hold: integrated again with blocks of their size
+        std::vector<long> off2(n, -1);
+        long rows2 = 0;
+        for(long i : redo) { off2[i] = rows2; rows2 += need[i] + 16; }
+        DevBuf<double> t2(rows2), x2(rows2 * 6), v2(R->der ? rows2 * 36 : 0);
+        if(!redo.empty()) {
+            std::vector<long> off1 = R->off;
+            for(long i : redo) { R->off[i] = off2[i]; ksz[i] = -(int)(need[i] + 16); }
+            if(integrate(redo, t2.p, x2.p, R->der ? v2.p : nullptr, RecordPool())) return nullptr;
             for(long i : redo) {
-                const long need = everyStepRows(ns[i], tstart ? tstart[i] : 0, ttotal[i]);
-                if(need > R->cap[i]) {
+                if(everyStepRows(ns[i], tstart ? tstart[i] : 0, ttotal[i]) != need[i]) {
                     fail(AGX_EDEVICE, "an orbit took a different number of steps when integrated again");
                     return nullptr;
                 }
-                R->rows[i] = need;
+            }
+            R->off = off1;
+        }
+        // gather every orbit's rows into one compact buffer: segments from the first pass
+        // (block, then its pool chunks in order) or from the second
+        std::vector<long> dstOff(n);
+        long total = 0;
+        for(long i = 0; i < n; i++) { dstOff[i] = total; total += need[i]; }
+        struct Seg { std::vector<long> src, dst, cnt; };
+        Seg a1, a6, ad1, a2s, a26, ad2;   // time/traj rows and dev rows, first and second pass
+        for(long i = 0; i < n; i++) {
+            const long r = need[i];
+            if(off2[i] >= 0) {
+                a2s.src.push_back(off2[i]); a2s.dst.push_back(dstOff[i]); a2s.cnt.push_back(r);
+                for(int k = 0; R->der && k < 6; k++) {
+                    ad2.src.push_back(off2[i] * 6 + k * (need[i] + 16));
+                    ad2.dst.push_back(dstOff[i] * 6 + k * r); ad2.cnt.push_back(r);
+                }
+                continue;
+            }
+            const long inBlock = std::min(r, R->cap[i]);
+            a1.src.push_back(R->off[i]); a1.dst.push_back(dstOff[i]); a1.cnt.push_back(inBlock);
+            for(int k = 0; R->der && k < 6; k++) {
+                ad1.src.push_back(R->off[i] * 6 + k * R->cap[i]);
+                ad1.dst.push_back(dstOff[i] * 6 + k * r); ad1.cnt.push_back(inBlock);
+            }
+            for(long c = 0, left = r - inBlock; left > 0; c++, left -= CH) {
+                const long chunk = hdir[i * MAXCH + c], len = std::min<long>(left, CH);
+                const long at = inBlock + c * CH;
+                a1.src.push_back(blocks + chunk * CH); a1.dst.push_back(dstOff[i] + at); a1.cnt.push_back(len);
+                for(int k = 0; R->der && k < 6; k++) {
+                    ad1.src.push_back((blocks + chunk * CH) * 6 + k * CH);
+                    ad1.dst.push_back(dstOff[i] * 6 + k * r + at); ad1.cnt.push_back(len);
+                }
             }
         }
+        std::unique_ptr<DevBuf<double>> ct(new DevBuf<double>(std::max<long>(total, 1))),
+            cx(new DevBuf<double>(std::max<long>(total, 1) * 6)),
+            cv(new DevBuf<double>(R->der ? std::max<long>(total, 1) * 36 : 0));
+        auto gather = [&](const Seg& g, int width, const double* src, double* dst) {
+            const long k = (long)g.src.size();
+            if(!k) return;
+            DevBuf<long> ds(k), dd(k), dc(k);
+            HIPCHECK(hipMemcpy(ds.p, g.src.data(), k * 8, hipMemcpyHostToDevice));
+            HIPCHECK(hipMemcpy(dd.p, g.dst.data(), k * 8, hipMemcpyHostToDevice));
+            HIPCHECK(hipMemcpy(dc.p, g.cnt.data(), k * 8, hipMemcpyHostToDevice));
+            HIPCHECK(agxLaunchGatherRows(k, ds.p, dd.p, dc.p, width, src, dst, nullptr));
+            HIPCHECK(hipDeviceSynchronize());
+        };
+        gather(a1, 1, t1.p, ct->p); gather(a1, 6, x1.p, cx->p);
+        gather(a2s, 1, t2.p, ct->p); gather(a2s, 6, x2.p, cx->p);
+        if(R->der) { gather(ad1, 6, v1.p, cv->p); gather(ad2, 6, v2.p, cv->p); }
+        R->time = std::move(ct); R->traj = std::move(cx); R->dev = std::move(cv);
+        R->deviceRows = total;
+        for(long i = 0; i < n; i++) { R->off[i] = dstOff[i]; R->cap[i] = need[i]; R->rows[i] = need[i]; }
         for(long i = 0; i < n; i++) {
             if(out_rows) out_rows[i] = R->rows[i];
             if(out_final) for(int c = 0; c < 6; c++) out_final[i * 6 + c] = fin[i * 6 + c];
@@ -1000,21 +1082,41 @@ int agx_orbit_record_fetch(const agx_orbit_record* R, double* out_time, double* 
     if(out_dev && !R->der) return fail(AGX_EINVAL, "the record holds no deviation vectors");
     try {
         HIPCHECK(hipSetDevice(R->device));
-        const long D = R->deviceRows;
-        std::vector<double> t(out_time ? D : 0), x(out_traj ? D * 6 : 0), v(out_dev ? D * 36 : 0);
-        if(out_time && D) HIPCHECK(hipMemcpy(t.data(), R->time->p, D * 8, hipMemcpyDeviceToHost));
-        if(out_traj && D) HIPCHECK(hipMemcpy(x.data(), R->traj->p, D * 48, hipMemcpyDeviceToHost));
-        if(out_dev && D) HIPCHECK(hipMemcpy(v.data(), R->dev->p, D * 288, hipMemcpyDeviceToHost));
+        // the blocks hold unused rows (reserved, or left by a first pass): gather the recorded
+        // rows into one contiguous buffer on the device, then copy that
+        const long n = R->n;
+        long total = 0;
+        for(long i = 0; i < n; i++) total += R->rows[i];
+        if(total == 0) return AGX_OK;
+        const long nseg = out_dev ? 7 * n : n;
+        std::vector<long> src(nseg), dst(nseg), cnt(nseg);
         long o = 0;
-        for(long i = 0; i < R->n; i++) {
-            const long r = R->rows[i], a = R->off[i];
-            if(out_time) std::copy(t.begin() + a, t.begin() + a + r, out_time + o);
-            if(out_traj) std::copy(x.begin() + a * 6, x.begin() + (a + r) * 6, out_traj + o * 6);
-            if(out_dev)   // six vectors of r rows each (the device block reserves cap rows per vector)
-                for(int k = 0; k < 6; k++)
-                    std::copy(v.begin() + a * 36 + k * R->cap[i] * 6, v.begin() + a * 36 + (k * R->cap[i] + r) * 6,
-                        out_dev + o * 36 + k * r * 6);
-            o += r;
+        for(long i = 0; i < n; i++) {
+            src[i] = R->off[i]; dst[i] = o; cnt[i] = R->rows[i];
+            if(out_dev)   // six vectors of rows[i] rows each; the device block has cap[i] per vector
+                for(int k = 0; k < 6; k++) {
+                    src[n + i * 6 + k] = R->off[i] * 6 + k * R->cap[i];
+                    dst[n + i * 6 + k] = o * 6 + k * R->rows[i];
+                    cnt[n + i * 6 + k] = R->rows[i];
+                }
+            o += R->rows[i];
+        }
+        DevBuf<long> dsrc(nseg), ddst(nseg), dcnt(nseg);
+        HIPCHECK(hipMemcpy(dsrc.p, src.data(), nseg * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(ddst.p, dst.data(), nseg * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dcnt.p, cnt.data(), nseg * 8, hipMemcpyHostToDevice));
+        DevBuf<double> buf(out_dev ? total * 36 : total * 6);
+        if(out_time) {
+            HIPCHECK(agxLaunchGatherRows(n, dsrc.p, ddst.p, dcnt.p, 1, R->time->p, buf.p, nullptr));
+            HIPCHECK(hipMemcpy(out_time, buf.p, total * 8, hipMemcpyDeviceToHost));
+        }
+        if(out_traj) {
+            HIPCHECK(agxLaunchGatherRows(n, dsrc.p, ddst.p, dcnt.p, 6, R->traj->p, buf.p, nullptr));
+            HIPCHECK(hipMemcpy(out_traj, buf.p, total * 48, hipMemcpyDeviceToHost));
+        }
+        if(out_dev) {
+            HIPCHECK(agxLaunchGatherRows(6 * n, dsrc.p + n, ddst.p + n, dcnt.p + n, 6, R->dev->p, buf.p, nullptr));
+            HIPCHECK(hipMemcpy(out_dev, buf.p, total * 288, hipMemcpyDeviceToHost));
         }
         return AGX_OK;
     } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }

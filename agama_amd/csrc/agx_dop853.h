@@ -165,11 +165,43 @@ __device__ __forceinline__ long trajOffsetOf(const AgxOrbitArgs& A, int oi)
 // (sampling interval 0: every step is recorded)
 __device__ __forceinline__ long rowCapacity(int S) { return S < 0 ? -(long)S : (long)S; }
 
+// output row of an every-step orbit's row `row` counted from the end of its block: a row of the
+// overflow pool, taking a new chunk when the row starts one (the lane writes its rows in order,
+// so the chunk directory entry is set before it is read); -1 when there is no pool or it is
+// exhausted (the host then integrates the orbit again with a block of its size)
+__device__ __forceinline__ long overflowRow(int* chunkDir, unsigned int* poolCounter, long poolBase, int poolChunk,
+    int maxChunks, int poolChunks, int oi, long row)
+{
+    if(!poolCounter) return -1;
+    const long k = row / poolChunk, r = row - k * poolChunk;
+    if(k >= maxChunks) return -1;
+    int* dir = chunkDir + (long)oi * maxChunks + k;
+    int c;
+    if(r == 0) {
+        const unsigned int q = atomicAdd(poolCounter, 1u);
+        c = q < (unsigned int)poolChunks ? (int)q : -1;
+        *dir = c;
+    } else
+        c = *dir;
+    return c < 0 ? -1 : poolBase + (long)c * poolChunk + r;
+}
+
+__device__ __forceinline__ long overflowRow(const AgxOrbitArgs& A, int oi, long row)
+{
+    return overflowRow(A.chunkDir, A.poolCounter, A.poolBase, A.poolChunk, A.maxChunks, A.poolChunks, oi, row);
+}
+
+// the same as a call (fewer registers held across it in kernels where that matters)
+__device__ __noinline__ long overflowRowCall(int* chunkDir, unsigned int* poolCounter, long poolBase,
+    int poolChunk, int maxChunks, int poolChunks, int oi, long row)
+{
+    return overflowRow(chunkDir, poolCounter, poolBase, poolChunk, maxChunks, poolChunks, oi, row);
+}
+
 // RuntimeTrajectory::processTimestep (orbit.cpp:15-47) after an accepted step [tb, tb+h]:
 // rows at regular intervals (trajsize S >= 2), or with sampling interval 0 (S < 0) the start of
-// the first step and then the end of every step (orbit.cpp:38-45), in a block of -S rows whose
-// later rows are not written (the host integrates such an orbit again with a block of
-// nsteps + 1 rows).  Both share one loop over the rows a step produces.
+// the first step and then the end of every step (orbit.cpp:38-45), in a block of -S rows
+// continued in the overflow pool.  Both share one loop over the rows a step produces.
 template<class Lane>
 __device__ __forceinline__ void sampleStep(const AgxOrbitArgs& A, Lane& L)
 {
@@ -209,6 +241,10 @@ __device__ __forceinline__ void sampleStep(const AgxOrbitArgs& A, Lane& L)
         for(int c = 0; c < 6; c++) d[c] = denseOut(L, c, off);
         toRotating(A.Omega, A.Omega * (off + tbegin), d, o);
         if(iout >= 0 && iout < lim) storeRow(A, base + iout, timeout, o);
+        else if(every) {
+            const long r = overflowRow(A, L.oi, iout - lim);
+            if(r >= 0) storeRow(A, r, timeout, o);
+        }
     }
 }
 

@@ -21,6 +21,14 @@ struct AgxOrbitArgs {
     double* outMaxde;        // n, or null
     int* outStatus;          // n, or null
     unsigned int* counter;   // work-queue head, zero on entry
+    // overflow pool of every-step recording (trajsize < 0): rows beyond an orbit's block go to
+    // chunks of poolChunk rows, numbered from poolCounter (zero on entry) and placed at row
+    // poolBase + chunk * poolChunk of the output buffers; chunkDir[oi * maxChunks + k] holds the
+    // chunk of the orbit's k-th overflow chunk (-1 on entry).  poolCounter null: no pool.
+    int* chunkDir = nullptr;
+    unsigned int* poolCounter = nullptr;
+    long poolBase = 0;
+    int poolChunk = 0, maxChunks = 0, poolChunks = 0;
 };
 
 enum AgxPrecision { AGX_FP64 = 0, AGX_FP32_TABLES = 1 };

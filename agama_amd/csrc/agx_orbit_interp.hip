@@ -176,7 +176,31 @@ __global__ __launch_bounds__(256) void evalSplinesKernel(const AgxSeries* __rest
     o[5] = df[2];
 }
 
+// copy row segments: segment s moves rows[s] rows of `width` doubles from row srcRow[s] of src
+// to row dstRow[s] of dst; one workgroup per segment at a time (coalesced within a segment)
+__global__ __launch_bounds__(256) void gatherRowsKernel(long nseg, const long* __restrict__ srcRow,
+    const long* __restrict__ dstRow, const long* __restrict__ rows, int width, const double* __restrict__ src,
+    double* __restrict__ dst)
+{
+    for(long s = blockIdx.x; s < nseg; s += gridDim.x) {
+        const long len = rows[s] * width;
+        const double* a = src + srcRow[s] * width;
+        double* b = dst + dstRow[s] * width;
+        for(long k = threadIdx.x; k < len; k += blockDim.x) b[k] = a[k];
+    }
+}
+
 }  // namespace
+
+hipError_t agxLaunchGatherRows(long nseg, const long* srcRow, const long* dstRow, const long* rows, int width,
+    const double* src, double* dst, hipStream_t stream)
+{
+    if(nseg <= 0) return hipSuccess;
+    const long blocks = nseg < 65536 ? nseg : 65536;
+    hipLaunchKernelGGL(gatherRowsKernel, dim3((unsigned)blocks), dim3(256), 0, stream, nseg, srcRow, dstRow, rows,
+        width, src, dst);
+    return hipGetLastError();
+}
 
 hipError_t agxLaunchBuildSplines(const AgxSeries* dSeries, long ns, double Omega, double* nodes,
     double* pivots, int* count, int* reversed, hipStream_t stream)

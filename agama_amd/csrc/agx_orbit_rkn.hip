@@ -100,9 +100,11 @@ __device__ __forceinline__ void sampleStepRKN(const AgxOrbitArgs& A, LaneRKN<STR
             toRotating(A.Omega, A.Omega * L.tb, d, o);
             storeRow(A, base, L.tb, o);
         }
-        if(row < cap) {
+        const long r = row < cap ? base + row : overflowRowCall(A.chunkDir, A.poolCounter, A.poolBase,
+            A.poolChunk, A.maxChunks, A.poolChunks, L.oi, row - cap);
+        if(r >= 0) {
             toRotating(A.Omega, A.Omega * (L.h + L.tb), L.y, o);
-            storeRow(A, base + row, L.tb + L.h, o);
+            storeRow(A, r, L.tb + L.h, o);
         }
         L.actual = 1;
         return;

@@ -124,16 +124,31 @@ __device__ __forceinline__ double storeSolution(const AgxOrbitArgs& A, LaneVarT<
     double ca = 1, sa = 0;
     if(A.Omega != 0) sincosA(A.Omega * (tbegin + timeOffset), sa, ca);
     const bool write = NV == 6 && L.dev && ind >= 0;
+    // the row's place: the orbit's block ([vector][cap][6]) or, beyond it, the chunk of the
+    // overflow pool that the trajectory row of the same index took ([vector][chunk][6])
+    double* rowBase = nullptr;
+    long vstride = 0;
     if(write) {
         L.v(V_DEVSIZE) = (double)(ind + 1);
+        const long cap = rowCapacity(L.S);
+        if(ind < cap) {
+            rowBase = L.dev + (trajOffsetOf(A, L.oi) * 36 + ind * 6);
+            vstride = cap * 6;
+        } else if(L.S < 0 && A.poolCounter) {
+            const long k = (ind - cap) / A.poolChunk, r = ind - cap - k * A.poolChunk;
+            const int c = k < A.maxChunks ? A.chunkDir[(long)L.oi * A.maxChunks + k] : -1;
+            if(c >= 0) {
+                rowBase = L.dev + ((A.poolBase + (long)c * A.poolChunk) * 36 + r * 6);
+                vstride = (long)A.poolChunk * 6;
+            }
+        }
     }
     for(int vec = 0; vec < NV; vec++) {
         double q[6];
         for(int c = 0; c < 6; c++) q[c] = gl4Sol(L, timeOffset, vec * 6 + c);
         if(NV == 1) return logMagnitude6(q);
-        const long cap = rowCapacity(L.S);
-        if(write && ind < cap) {
-            double* o = L.dev + (trajOffsetOf(A, L.oi) * 36 + ((long)vec * cap + ind) * 6);
+        if(rowBase) {
+            double* o = rowBase + vec * vstride;
             o[0] = (q[0] * ca + q[1] * sa) * mul;
             o[1] = (q[1] * ca - q[0] * sa) * mul;
             o[2] = (q[2] * mul);

@@ -62,12 +62,13 @@ def parse():
     ap.add_argument("--cpu-orbits", type=int, default=20_000,
                     help="size of the CPU-baseline sample (0 disables)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["orbit", "fp64", "eval", "hess", "mlp", "copy"], default="orbit",
+    ap.add_argument("--mode", choices=["orbit", "fp64", "eval", "hess", "mlp", "copy", "orbitobj"], default="orbit",
                     help="orbit: surrogate and FP64 paths (default); fp64: the FP64 path alone; "
                          "eval: potential + force at 1M points drawn from the density; hess: the same "
                          "with the Hessian; mlp: the "
                          "surrogate's fused MFMA evaluation at 1M points; copy: one 1 GiB "
-                         "device-to-device copy, the reference for HBM byte counters")
+                         "device-to-device copy, the reference for HBM byte counters; orbitobj: "
+                         "orbit(dtype=object) - every step recorded, splines built and evaluated")
     ap.add_argument("--method", choices=["dop853", "dprkn8"], default="dop853",
                     help="integrator of the FP64 Multipole path (the surrogate path uses DOP853)")
     ap.add_argument("--train-steps", type=int, default=6000)
@@ -174,6 +175,8 @@ def main():
         return
     if args.mode == "mlp":
         return bench_mlp(args)
+    if args.mode == "orbitobj":
+        return bench_orbitobj(args)
     import numpy as np
     import torch
     import agama_amd
@@ -342,6 +345,95 @@ def main():
         out["surrogate"] = surrogate
         out["fp64_multipole"] = {"value": fp64_value, "ms_per_step": wall / args.steps * 1e3}
     print(json.dumps(out), flush=True)
+
+
+def bench_orbitobj(args):
+    """orbit(dtype=object) on the device: 100k orbits x 10 circular periods with every step
+    recorded (agx_orbit_record_batch), the quintic splines of the Orbit objects built, and 100
+    evaluations per orbit at random times; host arrays in and out as the Python call has them.
+    value = recorded rows (steps + 1 per orbit) per second of recording + spline build."""
+    import numpy as np
+    import agama_amd
+    from agama_amd import lib, workloads
+    from agama_amd._native import ptr
+    coef = os.path.join(ROOT, "tests", "golden", "dehnen_triax_lmax8.coef")
+    pot = agama_amd.Potential(file=coef)
+    n = 100_000 if args.orbits == 1_000_000 else args.orbits
+    ic, T = workloads.triaxial_dehnen_orbits(n, lambda p: pot.potential(p), pot.coefs(), seed=42, periods=10)
+    sz = np.zeros(n, dtype=np.int32)
+    rows = np.empty(n, dtype=np.int64)
+    ns = np.empty(n, dtype=np.int64)
+    na = np.empty(n, dtype=np.int64)
+    q = 100
+    rng = np.random.default_rng(3)
+    which = np.repeat(np.arange(n, dtype=np.int64), q)
+    tq = (rng.random(n * q) * np.repeat(T, q)).astype(np.float64)
+    out = np.empty((n * q, 6))
+    nodes = np.empty(n, dtype=np.int64)
+
+    def step():
+        t0 = time.perf_counter()
+        h = lib.agx_orbit_record_batch(pot.handle, n, ptr(ic), None, ptr(T), ptr(sz), 1e-8, 0.0, int(1e8), 0,
+                                       ptr(rows), None, ptr(ns), ptr(na), None, None, None, 0)
+        if not h:
+            raise RuntimeError(agama_amd._native.last_error())
+        t1 = time.perf_counter()
+        if lib.agx_orbit_record_series_info(h, ptr(nodes), None):
+            raise RuntimeError(agama_amd._native.last_error())
+        t2 = time.perf_counter()
+        if lib.agx_orbit_record_interp(h, n * q, ptr(which), ptr(tq), 0, ptr(out)):
+            raise RuntimeError(agama_amd._native.last_error())
+        t3 = time.perf_counter()
+        lib.agx_orbit_record_free(h)
+        return t1 - t0, t2 - t1, t3 - t2
+
+    for _ in range(args.warmup):
+        step()
+        print("orbitobj: warmup step done", file=sys.stderr, flush=True)
+    times = [step() for _ in range(max(args.steps, 1))]
+    rec = sum(t[0] for t in times) / len(times)
+    build = sum(t[1] for t in times) / len(times)
+    interp = sum(t[2] for t in times) / len(times)
+    total_rows = int(rows.sum())
+    attempts = int(na.sum())
+    flops = attempts * (EVALS_PER_ATTEMPT["dop853"] * FLOPS_PER_EVAL + FLOPS_PER_ATTEMPT_EXTRA) + \
+        n * FLOPS_PER_ORBIT_INIT
+    print(f"orbitobj: {n} orbits, {total_rows} rows (median {np.median(rows):.0f} per orbit); record "
+          f"{rec*1e3:.1f} ms, spline build {build*1e3:.1f} ms, {n*q} evaluations {interp*1e3:.1f} ms",
+          file=sys.stderr, flush=True)
+    cpu = None
+    if args.cpu_orbits > 0:
+        from oracle import pyoracle
+        threads = args.cpu_threads or os.cpu_count()
+        fast = pyoracle.OraclePotential(coef, library=pyoracle.fast())
+        m = min(40000, n)
+        t0 = time.perf_counter()
+        r = fast.orbits_steps(ic[:m], T[:m], nthreads=threads, capacity=1024)
+        for i, (ti, xi) in enumerate(r["series"]):
+            pyoracle.orbit_interp(ti, xi, 0.0, tq[i * q:(i + 1) * q])
+        dt = time.perf_counter() - t0
+        cpu = {"value": float(r["rows"].sum() / dt), "unit": "recorded rows/s", "cores": threads, "kind": "port",
+               "sample": f"first {m} of the {n} orbits: every-step integration (one pass, 1024-row blocks) "
+                         f"with the C++ OpenMP restatement (-O2, {threads} threads) plus one Orbit built and "
+                         f"evaluated {q} times per orbit (single thread), {dt:.1f} s"}
+        print(f"cpu baseline: {cpu['value']/1e6:.2f} M rows/s ({dt:.1f} s)", file=sys.stderr, flush=True)
+    achieved = flops / rec / 1e12
+    res = {"metric": "recorded orbit rows/s of orbit(dtype=object): every step + Orbit splines (triaxial "
+                     "Dehnen Multipole lmax=8)",
+           "value": total_rows / (rec + build), "unit": "rows/s", "n_gpus": 1, "steps": len(times),
+           "warmup": args.warmup, "ms_per_step": (rec + build + interp) * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{n} orbits x 10 circular periods, DOP853 accuracy 1e-8, every step "
+                                  f"recorded, quintic splines built, {q} evaluations per orbit",
+                      "rows": total_rows, "record_ms": rec * 1e3, "spline_build_ms": build * 1e3,
+                      "evaluations": n * q, "evaluate_ms": interp * 1e3,
+                      "evaluations_per_s": n * q / interp},
+           "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                        "kernel": "orbitKernel (FP64) with every-step recording; wall time of the record call",
+                        "kernel_ms": rec * 1e3},
+           "cpu_baseline": cpu}
+    print(json.dumps(res), flush=True)
 
 
 # algorithmic bf16 MFMA operations of one surrogate evaluation: 3 hidden 256x256 products in

@@ -184,15 +184,18 @@ typedef struct agx_orbit_record agx_orbit_record;
    in the rotating frame; der != 0 also records the six deviation vectors at the same times
    (DOP853, FP64 only, as agx_orbit_var_batch).  out_rows[n]: rows of orbit i (trajsize[i], or
    the recorded count when it is 0).  The other outputs are those of agx_orbit_batch; any may be
-   NULL.  Returns NULL on error (agx_last_error).  An every-step orbit that needs more rows than
-   the first pass reserved (a share of 2^24 rows) is integrated a second time. */
+   NULL.  Returns NULL on error (agx_last_error).  Every-step orbits share a budget of 2^28
+   rows (2^25 with der): a block per orbit, and an overflow pool of chunks that a lane takes on
+   the device when its orbit outgrows the block; an orbit that finds the pool exhausted is
+   integrated a second time with a block of its size.  The rows are then gathered into one
+   compact buffer. */
 agx_orbit_record* agx_orbit_record_batch(const agx_potential* pot, long n, const double* ic,
     const double* tstart, const double* ttotal, const int* trajsize, double accuracy, double Omega,
     long maxNumSteps, int der, long* out_rows, double* out_final, long* out_nsteps, long* out_nattempts,
     long* out_nevals, double* out_maxde, int* out_status, int flags);
 /* The recorded rows, orbit after orbit (orbit i at row offset sum_{j<i} out_rows[j]):
    out_time[rows], out_traj[rows*6], out_dev[rows*36] as [vector][row][6] per orbit (any may
-   be NULL; out_dev needs der). */
+   be NULL; out_dev needs der).  The rows are gathered on the device and copied once. */
 int agx_orbit_record_fetch(const agx_orbit_record* rec, double* out_time, double* out_traj,
     double* out_dev);
 /* Interpolated orbits (agama.Orbit, createOrbitObject, interface_python.cpp:6793-6880): each
@@ -204,8 +207,8 @@ int agx_orbit_record_fetch(const agx_orbit_record* rec, double* out_time, double
    recorded backward in time.  node_times: the node times of one series in increasing order. */
 long agx_orbit_record_num_series(const agx_orbit_record* rec);
 /* Rows the first pass of agx_orbit_record_batch shares among its every-step orbits (0: the
-   default 2^24, or 2^21 with deviation vectors); a small budget sends most orbits through the
-   second integration. */
+   default 2^28, or 2^25 with deviation vectors); a small budget sends orbits through the
+   overflow pool and, when that runs out, through the second integration. */
 int agx_set_record_rows(long rows);
 int agx_orbit_record_series_info(agx_orbit_record* rec, long* out_nodes, int* out_reversed);
 int agx_orbit_record_node_times(agx_orbit_record* rec, long series, double* out_t);

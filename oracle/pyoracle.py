@@ -234,44 +234,54 @@ OraclePotential.orbits_lyap = _orbits_lyap
 
 
 def _orbits_steps(self, ic, ttotal, tstart=None, accuracy=1e-8, Omega=0.0, maxNumSteps=int(1e8),
-                  method="dop853", der=False, nthreads=0):
+                  method="dop853", der=False, nthreads=0, capacity=0):
     """agama.orbit(trajsize=0) as the oracle restates it: every step recorded; returns per orbit
     (times, traj[rows, 6]) or, with der, (times, traj, dev[6, rows, 6]), and the step counts.
-    Two passes: the first counts the rows, the second records them."""
+    The first pass records up to `capacity` rows per orbit (0: it only counts them); orbits with
+    more rows are integrated again with blocks of their size."""
     ic = np.ascontiguousarray(ic, dtype=np.float64).reshape(-1, 6)
     n = len(ic)
     tt = np.ascontiguousarray(np.broadcast_to(np.asarray(ttotal, dtype=np.float64), (n,)))
     ts = None if tstart is None else np.ascontiguousarray(
         np.broadcast_to(np.asarray(tstart, dtype=np.float64), (n,)))
     m = {"dop853": 0, "dprkn8": 1}[method.lower()]
-    rows = np.empty(n, np.int64)
-    nsteps = np.empty(n, np.int64)
-    status = np.empty(n, np.int32)
-    zero = np.zeros(n, np.int64)
-    rc = self.lib.orc_orbit_steps(self.h, n, _p(ic), _p(ts), _p(tt), _p(zero), _p(zero), accuracy, Omega,
-                                  maxNumSteps, m, int(der), None, None, None, _p(rows), _p(nsteps), _p(status),
-                                  nthreads)
-    if rc:
-        raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
-    cap = rows.copy()
-    offs = np.concatenate([[0], np.cumsum(cap)[:-1]]).astype(np.int64)
-    total = int(cap.sum())
-    t = np.empty(total)
-    x = np.empty((total, 6))
-    d = np.empty(total * 36) if der else None
-    rc = self.lib.orc_orbit_steps(self.h, n, _p(ic), _p(ts), _p(tt), _p(cap), _p(offs), accuracy, Omega,
-                                  maxNumSteps, m, int(der), _p(t), _p(x), _p(d), _p(rows), _p(nsteps),
-                                  _p(status), nthreads)
-    if rc:
-        raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
-    out = []
-    for i in range(n):
-        a, r = offs[i], rows[i]
-        item = (t[a:a + r], x[a:a + r])
-        if der:
-            item += (d[a * 36:(a + r) * 36].reshape(6, r, 6),)
-        out.append(item)
-    return dict(series=out, rows=rows, nsteps=nsteps, status=status)
+
+    def run(sel, cap):
+        k = len(sel)
+        offs = np.concatenate([[0], np.cumsum(cap)[:-1]]).astype(np.int64)
+        total = int(cap.sum())
+        t = np.empty(total)
+        x = np.empty((total, 6))
+        d = np.empty(total * 36) if der else None
+        rows = np.empty(k, np.int64)
+        nsteps = np.empty(k, np.int64)
+        status = np.empty(k, np.int32)
+        ics = np.ascontiguousarray(ic[sel])
+        tts = np.ascontiguousarray(tt[sel])
+        tss = None if ts is None else np.ascontiguousarray(ts[sel])
+        rc = self.lib.orc_orbit_steps(self.h, k, _p(ics), _p(tss), _p(tts), _p(cap), _p(offs), accuracy, Omega,
+                                      maxNumSteps, m, int(der), _p(t), _p(x), _p(d), _p(rows), _p(nsteps),
+                                      _p(status), nthreads)
+        if rc:
+            raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
+        items = []
+        for i in range(k):
+            a, r = offs[i], min(rows[i], cap[i])
+            item = (t[a:a + r], x[a:a + r])
+            if der:
+                item += (d[a * 36:a * 36 + 36 * cap[i]].reshape(6, cap[i], 6)[:, :r],)
+            items.append(item)
+        return items, rows, nsteps, status
+
+    sel = np.arange(n)
+    series, rows, nsteps, status = run(sel, np.full(n, int(capacity), np.int64))
+    redo = np.nonzero(rows > capacity)[0]
+    if len(redo):
+        s2, r2, n2, st2 = run(redo, rows[redo].astype(np.int64))
+        for j, i in enumerate(redo):
+            series[i] = s2[j]
+        rows[redo], nsteps[redo], status[redo] = r2, n2, st2
+    return dict(series=series, rows=rows, nsteps=nsteps, status=status)
 
 
 OraclePotential.orbits_steps = _orbits_steps

@@ -86,13 +86,14 @@ def test_every_step_rows_match_oracle(setup, method):
 
 
 def test_second_pass_reproduces_rows(setup):
-    """a first-pass block of 16 rows sends nearly every orbit through the second integration"""
+    """a 512-row budget: 8-row blocks, a pool of 32 chunks of 8 rows that a few orbits finish in,
+    and a second integration for the rest"""
     import agama_amd
     from agama_amd._native import lib
     gpu, orc = setup
     ic, T = ics(setup, 48, seed=43, periods=3)
     sz = np.where(np.arange(48) % 3 == 0, 7, 0)   # mixed: regular sampling and every step
-    print("case: trajsize=0 with a 16-row first pass, mixed trajsize", flush=True)
+    print("case: trajsize=0 with a 512-row budget (pool + second pass), mixed trajsize", flush=True)
     t1, x1 = agama_amd.orbit(potential=gpu, ic=ic, time=T, trajsize=sz, dtype="float64", separateTime=True)
     assert lib.agx_set_record_rows(16 * 32) == 0
     try:
