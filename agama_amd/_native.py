@@ -72,6 +72,7 @@ def _load():
         "agx_orbit_record_interp": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, ctypes.c_int, _vp]),
         "agx_orbit_record_free": (None, [_vp]),
         "agx_set_record_rows": (ctypes.c_int, [ctypes.c_long]),
+        "agx_set_construction": (ctypes.c_int, [ctypes.c_int]),
         "agx_set_orbit_launch": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
         "agx_set_fetch": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
         "agx_surrogate_train": (_vp, [_vp, ctypes.c_char_p]),
@@ -121,7 +122,7 @@ EXPORTED = ["agx_last_error", "agx_potential_create", "agx_potential_from_coefs"
             "agx_eval_hess_many_device", "agx_orbit_var_batch", "agx_orbit_var_batch_device", "agx_lyapunov_fit", "agx_orbit_batch",
             "agx_orbit_batch_device", "agx_orbit_record_batch", "agx_orbit_record_fetch",
             "agx_orbit_record_num_series", "agx_orbit_record_series_info", "agx_orbit_record_node_times",
-            "agx_orbit_record_interp", "agx_orbit_record_free", "agx_set_record_rows", "agx_set_orbit_launch", "agx_set_fetch", "iniacceleration",
+            "agx_orbit_record_interp", "agx_orbit_record_free", "agx_set_record_rows", "agx_set_construction", "agx_set_orbit_launch", "agx_set_fetch", "iniacceleration",
             "agx_surrogate_train", "agx_surrogate_create", "agx_surrogate_destroy",
             "agx_surrogate_info", "agx_surrogate_loss_history", "agx_surrogate_eval_many",
             "agx_surrogate_eval_many_device", "agx_surrogate_network", "agx_surrogate_train_grad",

@@ -29,7 +29,7 @@ def build(verbose_resources=False):
         o = os.path.join(BUILD, src.replace(".cpp", ".o"))
         _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
-    for src in ("agx_kernels.hip", "agx_orbit_rkn.hip", "agx_hessian.hip", "agx_orbit_var.hip", "agx_orbit_interp.hip", "agx_schedule.hip",
+    for src in ("agx_kernels.hip", "agx_orbit_rkn.hip", "agx_hessian.hip", "agx_orbit_var.hip", "agx_orbit_interp.hip", "agx_construct.hip", "agx_schedule.hip",
                 "agx_surrogate.hip", "agx_orbit_surrogate.hip", "agx_capi.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
         extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources and src != "agx_capi.hip" else []

@@ -42,6 +42,8 @@ hipError_t agxLaunchEvalSplines(const AgxSeries* dSeries, const int* count, cons
     long m, const long* which, const double* t, int mode, double* out, hipStream_t stream);
 hipError_t agxLaunchGatherRows(long nseg, const long* srcRow, const long* dstRow, const long* rows, int width,
     const double* src, double* dst, hipStream_t stream);
+hipError_t agxPoissonDevice(const agx::PoissonSetup& S, const agx::Density& dens, std::vector<double>& Phi,
+    std::vector<double>& dPhi);
 int agxOrbitKernelBlockSize();
 size_t agxScheduleWorkspace(long n);
 hipError_t agxLaunchSchedule(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, long n,
@@ -67,6 +69,10 @@ int g_fetchEval = AGX_FETCH_VEC2, g_fetchOrbit = AGX_FETCH_VEC2;   // set by agx
 int g_orbitWaves = 1;                                                // set by agx_set_orbit_waves
 
 int fail(int code, const std::string& msg) { g_err = msg; return code; }
+
+// where Multipole coefficients of a density are computed (agx_set_construction); -1: not set,
+// the device for agx_potential_create and the host for agx_build_coef_file
+int g_construction = -1;
 
 struct HipError : std::runtime_error {
     explicit HipError(const std::string& s) : std::runtime_error(s) {}
@@ -130,6 +136,42 @@ void upload(agx_potential& p)
     put(oH, &D, sizeof(D));
 }
 
+// the Poisson solver for a density: the bulk on the device (default), or on the host in the
+// reference's order or in the device's order (agx_set_construction)
+agx::Coefs coefsFromDensity(const agx::Density& dens, int lmax, int mmax, int sym, int nr, double rmin, double rmax,
+    int mode)
+{
+    if(mode == AGX_CONSTRUCT_HOST)
+        return agx::multipoleFromDensity(dens, lmax, mmax, sym, nr, rmin, rmax);
+    // the grids and the transform matrix depend on the density only through its symmetry:
+    // kept from the previous construction with the same parameters (rebuilding a potential in a
+    // loop then costs the device part and the tables only)
+    static std::mutex setupLock;
+    static std::shared_ptr<const agx::PoissonSetup> lastSetup;
+    static std::vector<double> lastKey;
+    const std::vector<double> key = { (double)dens.symmetry(), (double)lmax, (double)mmax, (double)sym, (double)nr,
+        rmin, rmax };
+    std::shared_ptr<const agx::PoissonSetup> Sp;
+    {
+        std::lock_guard<std::mutex> g(setupLock);
+        if(!lastSetup || lastKey != key) {
+            lastSetup = std::make_shared<const agx::PoissonSetup>(agx::poissonSetup(dens, lmax, mmax, sym, nr, rmin, rmax));
+            lastKey = key;
+        }
+        Sp = lastSetup;
+    }
+    const agx::PoissonSetup& S = *Sp;
+    std::vector<double> Phi, dPhi;
+    if(mode == AGX_CONSTRUCT_HOST_STAGED)
+        agx::poissonStagesHost(S, dens, Phi, dPhi);
+    else {
+        const hipError_t e = agxPoissonDevice(S, dens, Phi, dPhi);
+        if(e != hipSuccess)
+            throw std::runtime_error(std::string("Multipole construction on the device: ") + hipGetErrorString(e));
+    }
+    return agx::poissonFinish(S, Phi, dPhi);
+}
+
 agx_potential* makeFromCoefs(agx::Coefs&& c, bool allowLowOrder = false, double nfwMass = 0,
     double nfwScale = 1)
 {
@@ -189,8 +231,8 @@ agx_potential* createFromParams(const std::string& text)
     const int mmax = (int)P.num("mmax", lmax);
     const int nr = (int)P.num("gridsizer", 25);
     const int sym = agx::symmetryFromName(P.str("symmetry"));
-    return makeFromCoefs(agx::multipoleFromDensity(*dens, lmax, mmax, sym, nr,
-        P.num("rmin", 0), P.num("rmax", 0)), false, nfwMass, nfwScale);
+    return makeFromCoefs(coefsFromDensity(*dens, lmax, mmax, sym, nr, P.num("rmin", 0), P.num("rmax", 0),
+        g_construction < 0 ? AGX_CONSTRUCT_DEVICE : g_construction), false, nfwMass, nfwScale);
 }
 
 int orbitGeometry(long n, int precision, bool nfw, int& blocks, int& bs, bool rkn = false)
@@ -273,12 +315,21 @@ int agx_build_coef_file(const char* params, const char* path)
             "0.*", "1.*", "2.*", "3.*", "4.*", "5.*"});
         auto dens = agx::makeDensity(P);
         const int lmax = (int)P.num("lmax", 6);
-        agx::Coefs c = agx::multipoleFromDensity(*dens, lmax, (int)P.num("mmax", lmax),
+        agx::Coefs c = coefsFromDensity(*dens, lmax, (int)P.num("mmax", lmax),
             agx::symmetryFromName(P.str("symmetry")), (int)P.num("gridsizer", 25),
-            P.num("rmin", 0), P.num("rmax", 0));
+            P.num("rmin", 0), P.num("rmax", 0), g_construction < 0 ? AGX_CONSTRUCT_HOST : g_construction);
         agx::writeCoefFile(path, c);
         return AGX_OK;
     } catch(std::exception& e) { return fail(AGX_EINVAL, e.what()); }
+}
+
+int agx_set_construction(int mode)
+{
+    if(mode != AGX_CONSTRUCT_DEVICE && mode != AGX_CONSTRUCT_HOST && mode != AGX_CONSTRUCT_HOST_STAGED &&
+        mode != AGX_CONSTRUCT_DEFAULT)
+        return fail(AGX_EINVAL, "unknown construction mode");
+    g_construction = mode;
+    return AGX_OK;
 }
 
 long agx_debug_tables(const char* coef_path, int which, int m, double* out)

@@ -321,6 +321,8 @@ struct DehnenDensity : Density {        // src/potential_dehnen.cpp:37-42
     }
     int symmetry() const override
     { return p == 1 && q == 1 ? ST_SPHERICAL : p == 1 ? ST_AXISYM : ST_TRIAXIAL; }
+    void describe(std::vector<DensityComp>& out) const override
+    { DensityComp d; d.type = DENS_DEHNEN; d.p[0] = M; d.p[1] = a; d.p[2] = gamma; d.p[3] = p; d.p[4] = q; out.push_back(d); }
 };
 struct FerrersDensity : Density {       // src/potential_ferrers.cpp:17-31
     double a, b, c, rho0;
@@ -331,12 +333,16 @@ struct FerrersDensity : Density {       // src/potential_ferrers.cpp:17-31
         return m2 > 1 ? 0 : rho0 * sq(1-m2);
     }
     int symmetry() const override { return ST_TRIAXIAL; }
+    void describe(std::vector<DensityComp>& out) const override
+    { DensityComp d; d.type = DENS_FERRERS; d.p[0] = a; d.p[1] = b; d.p[2] = c; d.p[3] = rho0; out.push_back(d); }
 };
 struct DiskExpDensity : Density {       // src/potential_disk.h:97-139 (exponential in R and |z|)
     double Sigma0, Rd, h;
     double rhoCyl(double R, double z, double) const override
     { return Sigma0 * exp(-R/Rd) * (0.5/h) * exp(-fabs(z)/h); }
     int symmetry() const override { return ST_AXISYM; }
+    void describe(std::vector<DensityComp>& out) const override
+    { DensityComp d; d.type = DENS_DISK; d.p[0] = Sigma0; d.p[1] = Rd; d.p[2] = h; out.push_back(d); }
 };
 struct NFWDensity : Density {           // Laplacian of src/potential_analytic.cpp:55-75 / (4 pi)
     double M, rs;
@@ -346,12 +352,16 @@ struct NFWDensity : Density {           // Laplacian of src/potential_analytic.c
         return M / (4*M_PI) / (r * sq(rs + r));
     }
     int symmetry() const override { return ST_SPHERICAL; }
+    void describe(std::vector<DensityComp>& out) const override
+    { DensityComp d; d.type = DENS_NFW; d.p[0] = M; d.p[1] = rs; out.push_back(d); }
 };
 struct PlummerDensity : Density {
     double M, a;
     double rhoCyl(double R, double z, double) const override
     { return 3*M / (4*M_PI*a*a*a) * pow(1 + (R*R + z*z) / (a*a), -2.5); }
     int symmetry() const override { return ST_SPHERICAL; }
+    void describe(std::vector<DensityComp>& out) const override
+    { DensityComp d; d.type = DENS_PLUMMER; d.p[0] = M; d.p[1] = a; out.push_back(d); }
 };
 struct SumDensity : Density {
     std::vector<std::unique_ptr<Density> > parts;
@@ -367,6 +377,7 @@ struct SumDensity : Density {
         for(auto& p : parts) s &= p->symmetry();
         return s;
     }
+    void describe(std::vector<DensityComp>& out) const override { for(auto& p : parts) p->describe(out); }
 };
 
 std::unique_ptr<Density> makeOne(const std::string& type, const Params& p, const std::string& pre)
@@ -693,6 +704,8 @@ std::unique_ptr<Density> makeDensity(const Params& p)
     return std::move(sum);
 }
 
+Coefs restrictCoefs(Coefs&& C, int lmax, int mmax, int symExp, int gridSizeR);
+
 Coefs multipoleFromDensity(const Density& dens, int lmax, int mmax, int symExp,
     int gridSizeR, double rmin, double rmax)
 {
@@ -777,7 +790,13 @@ Coefs multipoleFromDensity(const Density& dens, int lmax, int mmax, int symExp,
                 C.dPhi[c][k] = mul * (-(l+1)*Pint[k] + l*Pext[k]) / R[k];
             }
         }
-    // restrictSphHarmCoefs<2> (potential_multipole.cpp:150-185)
+    return restrictCoefs(std::move(C), lmax, mmax, symExp, gridSizeR);
+}
+
+// restrictSphHarmCoefs<2> (potential_multipole.cpp:150-185): terms beyond (lmax, mmax) or not
+// allowed by the expansion's symmetry set to zero, trailing empty orders dropped
+Coefs restrictCoefs(Coefs&& C, int lmax, int mmax, int symExp, int gridSizeR)
+{
     Ind ir(lmax, mmax, symExp);
     std::vector<bool> allowed(ir.size(), false);
     for(int m = ir.mmin(); m <= mmax; m++)
@@ -799,6 +818,139 @@ Coefs multipoleFromDensity(const Density& dens, int lmax, int mmax, int symExp,
         C.dPhi.resize(sq(lmaxActual+1));
     }
     return C;
+}
+
+PoissonSetup poissonSetup(const Density& dens, int lmax, int mmax, int symExp,
+    int gridSizeR, double rmin, double rmax)
+{
+    // the same grids, orders and quadrature as multipoleFromDensity
+    if(gridSizeR < 2 || !(rmin > 0) || !(rmax > rmin))
+        throw std::invalid_argument("Multipole: invalid grid parameters (rmin, rmax must be given, 0<rmin<rmax)");
+    if(lmax < 0 || mmax < 0 || mmax > lmax)
+        throw std::invalid_argument("Multipole: invalid choice of expansion order");
+    PoissonSetup S;
+    const int symSrc = dens.symmetry();
+    symExp = symExp == ST_UNKNOWN ? symSrc : (symSrc | symExp);
+    S.R.resize(gridSizeR);
+    {
+        double lmin = log(rmin), lmx = log(rmax);
+        S.R.front() = rmin; S.R.back() = rmax;
+        for(int k = 1; k < gridSizeR-1; k++) S.R[k] = exp(lmin + k*(lmx-lmin)/(gridSizeR-1));
+    }
+    int lmaxSrc = std::max(lmax + 6, 12), mmaxSrc = std::max(mmax + 6, 12);
+    if(symHas(symSrc, ST_SPHERICAL)) lmaxSrc = 0;
+    if(symHas(symSrc, ST_ZROT)) mmaxSrc = 0;
+    if(symHas(symExp, ST_SPHERICAL)) lmax = 0;
+    if(symHas(symExp, ST_ZROT)) mmax = 0;
+    S.gridSizeR = gridSizeR; S.lmax = lmax; S.mmax = mmax; S.symExp = symExp;
+    Ind ind(lmaxSrc, mmaxSrc, symSrc);
+    S.glo = 10;
+    S.gln.resize(S.glo); S.glw.resize(S.glo);
+    glRule(0, 1, S.glo, &S.gln[0], &S.glw[0]);
+    S.nint = (gridSizeR+1) * S.glo;
+    S.rint.resize(S.nint);
+    for(int k = 0; k <= gridSizeR; k++) {
+        double rkm1 = k > 0 ? S.R[k-1] : 0, dR = k < gridSizeR ? S.R[k] - rkm1 : S.R.back();
+        for(int s = 0; s < S.glo; s++)
+            S.rint[k*S.glo + s] = k < gridSizeR ? rkm1 + S.gln[s] * dR : S.R.back() / S.gln[s];
+    }
+    ForwardSH tr(ind);
+    S.nang = tr.size();
+    S.ncoef = (int)ind.size();
+    S.cth.resize(S.nang); S.phi.resize(S.nang);
+    for(int ia = 0; ia < S.nang; ia++) { S.cth[ia] = tr.costheta(ia); S.phi[ia] = tr.phi(ia); }
+    // the transform as a matrix: coefs[c] = sum_{j,k} P(j; l, m) F(m, k) values[j][k], with the
+    // Fourier weights F of ForwardSH::fourier and the Legendre weights of ForwardSH::transform
+    S.B.assign((size_t)S.nang * S.ncoef, 0.);
+    S.active.assign(S.ncoef, 0);
+    for(int m = ind.mmin(); m <= ind.mmax; m++)
+        for(int l = ind.lmin(m); l <= ind.lmax; l += ind.step) S.active[Ind::index(l, m)] = 1;
+    if(ind.size() == 1) {
+        S.B[0] = 1;
+        return S;
+    }
+    const int nfnc = tr.nfnc, mmx = ind.mmax, ngrid = ind.lmax+1, nsamp = ind.thetasize();
+    std::vector<double> F(nfnc);
+    for(int m = ind.mmin(); m <= mmx; m++) {
+        for(int k = 0; k < nfnc; k++) {
+            const int iphi = k <= mmx ? k : 2*mmx+1-k, ifn = m >= 0 ? m : mmx - m;
+            F[k] = tr.trigFnc[iphi*nfnc + ifn] * (m < 0 && k > mmx ? -1 : 1);
+        }
+        for(int l = ind.lmin(m); l <= ind.lmax; l += ind.step) {
+            const unsigned c = Ind::index(l, m);
+            for(int j = 0; j < ngrid; j++) {
+                const int js = j < nsamp ? j : ngrid-1-j;
+                double P = tr.legFnc[tr.indLeg(j, l, m)];
+                if((l+m) % 2 == 1 && j > ind.lmax/2) P *= -1;
+                for(int k = 0; k < nfnc; k++) S.B[(size_t)(js*nfnc + k) * S.ncoef + c] += P * F[k];
+            }
+        }
+    }
+    return S;
+}
+
+void poissonStagesHost(const PoissonSetup& S, const Density& dens, std::vector<double>& Phi,
+    std::vector<double>& dPhi)
+{
+    const int nint = S.nint, nang = S.nang, nc = S.ncoef, nr = S.gridSizeR, G = S.glo;
+    std::vector<double> ic((size_t)nc * nint), vals(nang), sh(nc);
+    for(int ir = 0; ir < nint; ir++) {
+        const double rad = S.rint[ir];
+        for(int ia = 0; ia < nang; ia++)
+            vals[ia] = dens.rhoCyl(rad * sqrt(1 - sq(S.cth[ia])), rad * S.cth[ia], S.phi[ia]);
+        double mag = 0;
+        for(int c = 0; c < nc; c++) {
+            double acc = 0;
+            for(int ia = 0; ia < nang; ia++) acc += vals[ia] * S.B[(size_t)ia * nc + c];
+            sh[c] = acc;
+            mag = fmax(mag, fabs(acc));
+        }
+        for(int c = 0; c < nc; c++) ic[(size_t)c * nint + ir] = fabs(sh[c]) <= mag * 1e-10 ? 0 : sh[c];
+    }
+    Phi.assign((size_t)nc * nr, 0.);
+    dPhi.assign((size_t)nc * nr, 0.);
+    std::vector<double> Pint(nr), Pext(nr);
+    for(int c = 0; c < nc; c++) {
+        if(!S.active[c]) continue;
+        const int l = Ind::indexL(c);
+        const double* q = &ic[(size_t)c * nint];
+        for(int k = 0; k < nr; k++) {
+            double integr = 0;
+            for(int s = 0; s < G; s++) integr += q[k*G + s] * S.glw[s] * ipow(S.rint[k*G + s] / S.R[k], l+2);
+            Pint[k] = integr * (S.R[k] - (k > 0 ? S.R[k-1] : 0)) * S.R[k] +
+                (k > 0 ? Pint[k-1] * ipow(S.R[k-1] / S.R[k], l+1) : 0);
+        }
+        {
+            double integr = 0;
+            for(int s = 0; s < G; s++)
+                integr += q[nr*G + s] * S.glw[s] * ipow(S.rint[nr*G + s] / S.R[nr-1], 1-l) / sq(S.gln[s]);
+            Pext[nr-1] = integr * S.R[nr-1] * S.R[nr-1];
+        }
+        for(int k = nr-2; k >= 0; k--) {
+            double integr = 0;
+            for(int s = 0; s < G; s++) integr += q[(k+1)*G + s] * S.glw[s] * ipow(S.rint[(k+1)*G + s] / S.R[k], 1-l);
+            Pext[k] = integr * (S.R[k+1] - S.R[k]) * S.R[k] + Pext[k+1] * ipow(S.R[k] / S.R[k+1], l);
+        }
+        const double mul = -4*M_PI / (2*l+1);
+        for(int k = 0; k < nr; k++) {
+            Phi[(size_t)c * nr + k] = mul * (Pint[k] + Pext[k]);
+            dPhi[(size_t)c * nr + k] = mul * (-(l+1)*Pint[k] + l*Pext[k]) / S.R[k];
+        }
+    }
+}
+
+Coefs poissonFinish(const PoissonSetup& S, const std::vector<double>& Phi, const std::vector<double>& dPhi)
+{
+    Coefs C;
+    C.radii = S.R;
+    C.Phi.assign(S.ncoef, std::vector<double>(S.gridSizeR, 0.));
+    C.dPhi.assign(S.ncoef, std::vector<double>(S.gridSizeR, 0.));
+    for(int c = 0; c < S.ncoef; c++)
+        for(int k = 0; k < S.gridSizeR; k++) {
+            C.Phi[c][k] = Phi[(size_t)c * S.gridSizeR + k];
+            C.dPhi[c][k] = dPhi[(size_t)c * S.gridSizeR + k];
+        }
+    return restrictCoefs(std::move(C), S.lmax, S.mmax, S.symExp, S.gridSizeR);
 }
 
 Coefs readCoefFile(const std::string& path)

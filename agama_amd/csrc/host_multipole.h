@@ -38,16 +38,47 @@ struct Coefs {
     int lmax() const;
 };
 
+// one density model as plain parameters, for evaluation on the device (agx_construct.hip)
+enum DensityType { DENS_DEHNEN = 1, DENS_FERRERS = 2, DENS_DISK = 3, DENS_NFW = 4, DENS_PLUMMER = 5 };
+struct DensityComp {
+    int type = 0;
+    double p[5] = { 0, 0, 0, 0, 0 };   // Dehnen M a gamma p q; Ferrers a b c rho0; Disk Sigma0 Rd h;
+                                       // NFW M rs; Plummer M a
+};
+
 struct Density {
     virtual ~Density() {}
     virtual double rhoCyl(double R, double z, double phi) const = 0;
     virtual int symmetry() const = 0;
+    virtual void describe(std::vector<DensityComp>& out) const = 0;   // components, summed in order
 };
 std::unique_ptr<Density> makeDensity(const Params& p);   // Dehnen, Ferrers, Disk, NFW, Plummer,
                                                          // or a sum given as "a+b" components
 
 Coefs multipoleFromDensity(const Density& dens, int lmax, int mmax, int symExp,
     int gridSizeR, double rmin, double rmax);
+
+// The Poisson solver of multipoleFromDensity in three parts, so that its bulk can run on the
+// device: the grids and the forward spherical-harmonic transform as a matrix (host), the
+// density at every (quadrature radius, angular node), its transform and the radial
+// integrals (device, agx_construct.hip), and the restriction to the requested order (host).
+struct PoissonSetup {
+    int gridSizeR = 0, lmax = 0, mmax = 0, symExp = 0, ncoef = 0, nang = 0, nint = 0, glo = 0;
+    std::vector<double> R;           // grid radii [gridSizeR]
+    std::vector<double> rint;        // quadrature radii [nint = (gridSizeR+1)*glo]
+    std::vector<double> gln, glw;    // Gauss-Legendre nodes and weights on [0, 1] [glo]
+    std::vector<double> cth, phi;    // angular nodes [nang]
+    std::vector<double> B;           // forward transform [nang][ncoef]: coefs = values . B
+    std::vector<int> active;         // [ncoef]: 1 for the (l, m) of the source's index set
+};
+PoissonSetup poissonSetup(const Density& dens, int lmax, int mmax, int symExp,
+    int gridSizeR, double rmin, double rmax);
+// Phi[c*gridSizeR + k], dPhi likewise, for all ncoef terms -> the coefficients of the expansion
+Coefs poissonFinish(const PoissonSetup& S, const std::vector<double>& Phi, const std::vector<double>& dPhi);
+// the device's middle part (density values, matrix transform with the near-zero cut, radial
+// integrals) run on the host in the device's order: the CPU check of agx_construct.hip
+void poissonStagesHost(const PoissonSetup& S, const Density& dens, std::vector<double>& Phi,
+    std::vector<double>& dPhi);
 Coefs readCoefFile(const std::string& path);
 void writeCoefFile(const std::string& path, const Coefs& c);
 

@@ -62,13 +62,15 @@ def parse():
     ap.add_argument("--cpu-orbits", type=int, default=20_000,
                     help="size of the CPU-baseline sample (0 disables)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["orbit", "fp64", "eval", "hess", "mlp", "copy", "orbitobj"], default="orbit",
+    ap.add_argument("--mode", choices=["orbit", "fp64", "eval", "hess", "mlp", "copy", "orbitobj", "construct"],
+                    default="orbit",
                     help="orbit: surrogate and FP64 paths (default); fp64: the FP64 path alone; "
                          "eval: potential + force at 1M points drawn from the density; hess: the same "
                          "with the Hessian; mlp: the "
                          "surrogate's fused MFMA evaluation at 1M points; copy: one 1 GiB "
                          "device-to-device copy, the reference for HBM byte counters; orbitobj: "
-                         "orbit(dtype=object) - every step recorded, splines built and evaluated")
+                         "orbit(dtype=object) - every step recorded, splines built and evaluated; "
+                         "construct: Multipole from a density (Poisson solver) on the device vs the host")
     ap.add_argument("--method", choices=["dop853", "dprkn8"], default="dop853",
                     help="integrator of the FP64 Multipole path (the surrogate path uses DOP853)")
     ap.add_argument("--train-steps", type=int, default=6000)
@@ -177,6 +179,8 @@ def main():
         return bench_mlp(args)
     if args.mode == "orbitobj":
         return bench_orbitobj(args)
+    if args.mode == "construct":
+        return bench_construct(args)
     import numpy as np
     import torch
     import agama_amd
@@ -434,6 +438,67 @@ def bench_orbitobj(args):
                         "kernel_ms": rec * 1e3},
            "cpu_baseline": cpu}
     print(json.dumps(res), flush=True)
+
+
+def bench_construct(args):
+    """Multipole(density=...) construction: the Poisson solver (density at the quadrature
+    points, forward spherical-harmonic transform, radial integrals) on the device against the
+    host in the reference's order, for the triaxial Dehnen lmax=8 (50 radii) and lmax=16 (80
+    radii) expansions; the timed unit is one agx_build_coef_file call (coefficients + file) and
+    one agx_potential_create (coefficients + spline tables + upload)."""
+    import agama_amd
+    from agama_amd import lib
+    configs = [("lmax=8, gridSizeR=50", 8, 50), ("lmax=16, gridSizeR=80", 16, 80)]
+    reps = max(args.steps, 1) * 10
+    rows = []
+    for label, lmax, nr in configs:
+        params = (f"type=Multipole, density=Dehnen, gamma=1, axisRatioY=0.8, axisRatioZ=0.6, lmax={lmax}, "
+                  f"gridSizeR={nr}, rmin=1e-3, rmax=1e3").encode()
+        res = {}
+        for mode, name in ((0, "device"), (1, "host")):
+            assert lib.agx_set_construction(mode) == 0
+            path = f"/tmp/agx_bench_{os.getpid()}.coef".encode()
+            for _ in range(max(args.warmup, 1)):
+                if lib.agx_build_coef_file(params, path):
+                    raise RuntimeError(agama_amd._native.last_error())
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                lib.agx_build_coef_file(params, path)
+            coef_s = (time.perf_counter() - t0) / reps
+            # a new grid every time: the device path also recomputes its grids and transform matrix
+            t0 = time.perf_counter()
+            for i in range(reps):
+                lib.agx_build_coef_file(params.replace(b"rmax=1e3", f"rmax={1000 + i + 1}".encode()), path)
+            cold_s = (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                h = lib.agx_potential_create(params)
+                if not h:
+                    raise RuntimeError(agama_amd._native.last_error())
+                lib.agx_potential_destroy(h)
+            create_s = (time.perf_counter() - t0) / reps
+            res[name] = (coef_s, create_s, cold_s)
+            os.remove(path)
+        lib.agx_set_construction(-1)
+        rows.append({"config": label, "device_coef_ms": res["device"][0] * 1e3, "host_coef_ms": res["host"][0] * 1e3,
+                     "device_coef_new_grid_ms": res["device"][2] * 1e3, "host_coef_new_grid_ms": res["host"][2] * 1e3,
+                     "device_create_ms": res["device"][1] * 1e3, "host_create_ms": res["host"][1] * 1e3})
+        print(f"construct {label}: coefficients device {res['device'][0]*1e3:.2f} ms (new grid "
+              f"{res['device'][2]*1e3:.2f}) / host {res['host'][0]*1e3:.2f} ms (new grid {res['host'][2]*1e3:.2f}); "
+              f"potential create device {res['device'][1]*1e3:.2f} ms / host {res['host'][1]*1e3:.2f} ms",
+              file=sys.stderr, flush=True)
+    big = rows[-1]
+    out = {"metric": "Multipole(density=Dehnen triaxial) constructions/s (Poisson solver + tables + upload)",
+           "value": 1e3 / big["device_create_ms"], "unit": "constructions/s", "n_gpus": 1, "steps": reps,
+           "warmup": args.warmup, "ms_per_step": big["device_create_ms"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "triaxial Dehnen gamma=1 (0.8, 0.6) expanded at lmax=16, 80 radii; lmax=8 / 50 "
+                                  "radii alongside", "runs": rows},
+           "roofline": None,
+           "cpu_baseline": {"value": 1e3 / big["host_create_ms"], "unit": "constructions/s", "cores": 1,
+                            "kind": "port", "sample": "the same construction with the Poisson solver on the "
+                                                      "host in the reference's order (single thread)"}}
+    print(json.dumps(out), flush=True)
 
 
 # algorithmic bf16 MFMA operations of one surrogate evaluation: 3 hidden 256x256 products in

@@ -55,6 +55,9 @@ const char* agx_last_error(void);
    ("type=Multipole, file=coefs.ini" or "type=Multipole, density=Dehnen, gamma=1, ...").
    Replaces agama_createPotential (src/interface_c.h:46, src/interface_c.cpp:31-54) and the
    Multipole part of potential::createPotential (src/potential_factory.cpp:1357-1441).
+   A density is expanded by the Poisson solver (computePotentialCoefsFromSource,
+   src/potential_multipole.cpp:1013-1114) on the device: density at the quadrature points,
+   forward spherical-harmonic transform and radial integrals (agx_set_construction).
    Returns NULL on error. */
 agx_potential* agx_potential_create(const char* params);
 
@@ -67,11 +70,23 @@ agx_potential* agx_potential_from_coefs(int nr, int lmax, const double* radii,
 /* Replaces agama_deletePotential (src/interface_c.cpp:56-60). */
 void agx_potential_destroy(agx_potential* pot);
 
-/* Host-only construction (no GPU needed): build a Multipole expansion from key=value
-   parameters as agx_potential_create does and write its coefficient file.  This is the
-   reference's Potential(type='Multipole', density=...).export(path) sequence
-   (src/potential_multipole.cpp:1400-1441, src/potential_factory.cpp:909-929). */
+/* Build a Multipole expansion from key=value parameters as agx_potential_create does and
+   write its coefficient file: the reference's Potential(type='Multipole', density=...)
+   .export(path) sequence (src/potential_multipole.cpp:1400-1441,
+   src/potential_factory.cpp:909-929).  Needs no GPU unless agx_set_construction chose the
+   device. */
 int agx_build_coef_file(const char* params, const char* path);
+
+/* Where the Poisson solver of a density runs: AGX_CONSTRUCT_DEVICE, AGX_CONSTRUCT_HOST (on the
+   CPU in the reference's operation order), AGX_CONSTRUCT_HOST_STAGED (on the CPU in the
+   device's order, the check of the device kernels), or AGX_CONSTRUCT_DEFAULT: the device for
+   agx_potential_create, the host for agx_build_coef_file.  Returns AGX_OK, or AGX_EINVAL for
+   another value. */
+#define AGX_CONSTRUCT_DEFAULT    -1
+#define AGX_CONSTRUCT_DEVICE      0
+#define AGX_CONSTRUCT_HOST        1
+#define AGX_CONSTRUCT_HOST_STAGED 2
+int agx_set_construction(int mode);
 
 /* Host-only diagnostic: the interpolation tables built from a coefficient file, in the
    reference's per-harmonic layout.  which: 0 = ln r grid, 1 = tau grid, 2 = the 9 node arrays
