@@ -7,6 +7,8 @@ construction, and `orbit(potential, ic, time, timestart, trajsize, accuracy, max
 Omega, dtype, separateTime)`.  All evaluation and integration runs in libagama_amd.so on
 the GPU; there is no CPU fallback.
 """
+import ctypes
+
 import numpy as np
 
 from . import _native
@@ -205,7 +207,8 @@ class Potential:
 
 def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, accuracy=1e-8,
           maxNumSteps=None, Omega=0.0, dtype="float32", separateTime=False, method="dop853",
-          verbose=False, info=None, precision=PRECISION_FP64, max_rel_de=1e-6, der=False, lyapunov=False):
+          verbose=False, info=None, precision=PRECISION_FP64, max_rel_de=1e-6, der=False, lyapunov=False,
+          targets=None):
     """Integrate orbits with DOP853 or DPRKN8 (method=) on the GPU - reference: agama.orbit
     (src/interface_python.cpp:7689-8281, output conventions :7941-7961).
 
@@ -232,7 +235,13 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
     and returns arrays of each orbit's own length.  dtype=object returns agama.Orbit objects
     (`Orbit`: quintic-spline interpolators of the trajectory, interface_python.cpp:6605-6880)
     instead of arrays, recorded at every step unless trajsize is given (:7864-7869); with der=True
-    the deviation vectors become an (N, 6) array of Orbit objects too."""
+    the deviation vectors become an (N, 6) array of Orbit objects too.
+
+    targets= (a Target or a sequence of up to 4) collects each target's data during the
+    integration (RuntimeFncTarget, galaxymodel_target.h:95-155) and puts one (N, numCoefs)
+    float32 array per target first in the output (interface_python.cpp:7991-7994, 8153-8162).
+    Targets are collected with DOP853 in FP64; trajectories, deviation vectors or Lyapunov
+    exponents requested with them come from a second integration of the same orbits."""
     surrogate = isinstance(potential, Surrogate)
     if not surrogate and not isinstance(potential, Potential):
         raise TypeError("Argument 'potential' must be a valid Potential object")
@@ -300,6 +309,29 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
     else:
         raise AgamaAmdError(f"dtype={dtype} is not provided by this version")
     maxsteps = int(1e8) if maxNumSteps is None else int(maxNumSteps)
+    if targets is not None:
+        tlist = list(targets) if isinstance(targets, (list, tuple)) else [targets]
+        if not all(isinstance(t, Target) for t in tlist) or not 1 <= len(tlist) <= 4:
+            raise TypeError("Argument 'targets' must be a Target or a sequence of up to 4 Targets")
+        if surrogate or method_flag or precision != PRECISION_FP64:
+            raise AgamaAmdError("targets are collected with DOP853 and the FP64 Multipole only")
+        mats = [np.empty((n, len(t)), dtype=np.float32) for t in tlist]
+        final_t, nsteps_t, status_t = np.empty((n, 6)), np.empty(n, dtype=np.int64), np.empty(n, dtype=np.int32)
+        tptr = (ctypes.c_void_p * len(tlist))(*[t._h for t in tlist])
+        optr = (ctypes.c_void_p * len(tlist))(*[m.ctypes.data for m in mats])
+        check(lib.agx_orbit_target_batch(potential.handle, n, ptr(ic2), ptr(ts), ptr(tt), float(accuracy),
+                                         float(Omega), maxsteps, len(tlist), tptr, optr, ptr(final_t), ptr(nsteps_t),
+                                         ptr(status_t), int(precision)), "orbit")
+        mats = [m[0] if single else m for m in mats]
+        if trajsize is None and not obj and not der and not lyapunov:
+            if isinstance(info, dict):
+                info.update(nsteps=nsteps_t, status=status_t, final=final_t)
+            return mats[0] if len(mats) == 1 else tuple(mats)
+        rest = orbit(potential=potential, ic=ic, time=time, timestart=timestart, trajsize=trajsize,
+                     accuracy=accuracy, maxNumSteps=maxNumSteps, Omega=Omega, dtype=dtype,
+                     separateTime=separateTime, method=method, info=info, precision=precision, der=der,
+                     lyapunov=lyapunov)
+        return tuple(mats) + (rest if isinstance(rest, tuple) else (rest,))
     if have_traj and (obj or np.any(sz == 0)):
         if surrogate:
             raise AgamaAmdError("trajsize=0 and dtype=object are integrated with the FP64 Multipole only")
@@ -435,6 +467,47 @@ def _trajectory_outputs(n, single, sz, times, traj, dev, out_dtype, separateTime
     if der:
         return (res[0], darr[0]) if single else (res, darr)
     return res[0] if single else res
+
+
+class Target:
+    """A target plug-in collecting data along orbits - reference: agama.Target
+    (src/interface_python.cpp:7005-7250).  Provided: type='DensitySphHarm' with gridr (radial
+    nodes), lmax, mmax (even orders of the spherical-harmonic expansion): the density
+    discretisation of TargetDensitySphHarm (src/galaxymodel_densitygrid.cpp:356-411).
+    len(target) is the number of values recorded per orbit."""
+
+    def __init__(self, **kwargs):
+        self._h = None
+        type_str = str(kwargs.pop("type", "") or "")
+        if not type_str:
+            raise TypeError("Must provide a type='...' argument")
+        if type_str.lower() != "densitysphharm":
+            raise AgamaAmdError(f"Target type {type_str} is not provided by this version (DensitySphHarm is)")
+        if "gridr" not in kwargs:
+            raise TypeError("Argument 'gridr' is required")
+        gridr = np.ascontiguousarray(kwargs.pop("gridr"), dtype=np.float64).reshape(-1)
+        if len(gridr) < 2:
+            raise ValueError("gridr must be an array with >=2 elements")
+        lmax = int(kwargs.pop("lmax", 0))
+        mmax = int(kwargs.pop("mmax", 0))
+        if kwargs:
+            raise TypeError("Unknown arguments: " + ", ".join(kwargs))
+        h = lib.agx_target_density_sphharm(lmax, mmax, ptr(gridr), len(gridr))
+        if not h:
+            raise ValueError(_native.last_error())
+        self._h = h
+        self.type, self.lmax, self.mmax, self.gridr = "DensitySphHarm", lmax, mmax, gridr
+
+    def __len__(self):
+        return lib.agx_target_num_values(self._h)
+
+    def __repr__(self):
+        return f"{self.type} target with {len(self)} values"
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h and lib is not None:   # not during interpreter shutdown
+            lib.agx_target_destroy(h)
 
 
 from .surrogate import Surrogate  # noqa: E402

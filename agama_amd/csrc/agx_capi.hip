@@ -17,12 +17,13 @@
 #include "agx_surrogate.h"
 #include "host_multipole.h"
 #include "host_lyapunov.h"
+#include "agx_target.h"
 
 hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
     long n, const double* pos, double* phi, double* grad, hipStream_t stream);
 hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
     int waves, const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream);
-int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves, bool nfw);
+int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves, bool nfw, bool targets = false);
 hipError_t agxLaunchOrbitRKN(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision,
     const AgxOrbitArgs& A, int numBlocks, hipStream_t stream);
 int agxOrbitKernelRKNBlocksPerCU(int precision, bool nfw);
@@ -235,7 +236,7 @@ agx_potential* createFromParams(const std::string& text)
         g_construction < 0 ? AGX_CONSTRUCT_DEVICE : g_construction), false, nfwMass, nfwScale);
 }
 
-int orbitGeometry(long n, int precision, bool nfw, int& blocks, int& bs, bool rkn = false)
+int orbitGeometry(long n, int precision, bool nfw, int& blocks, int& bs, bool rkn = false, bool targets = false)
 {
     bs = agxOrbitKernelBlockSize();   // the DOP853 and DPRKN8 kernels share the block size
     if(g_orbitBlocks > 0) blocks = g_orbitBlocks;
@@ -244,7 +245,7 @@ int orbitGeometry(long n, int precision, bool nfw, int& blocks, int& bs, bool rk
         HIPCHECK(hipGetDevice(&dev));
         HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         const int perCU = std::max(1, rkn ? agxOrbitKernelRKNBlocksPerCU(precision, nfw) :
-            agxOrbitKernelBlocksPerCU(precision, g_fetchOrbit, g_orbitWaves, nfw));
+            agxOrbitKernelBlocksPerCU(precision, g_fetchOrbit, g_orbitWaves, nfw, targets));
         blocks = cus * perCU;
     }
     const long need = (n + bs - 1) / bs;
@@ -525,7 +526,7 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
     const long* d_traj_offset, const int* d_order, double accuracy, double Omega,
     long maxNumSteps, double* d_time, double* d_traj, double* d_final, double* d_dev, long* d_nsteps,
     long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags, void* stream,
-    const LyapDev& ly = LyapDev(), const RecordPool& pool = RecordPool())
+    const LyapDev& ly = LyapDev(), const RecordPool& pool = RecordPool(), const AgxTargetList* d_targets = nullptr)
 {
     if(!pot) return fail(AGX_EINVAL, "NULL potential");
     if(n == 0) return AGX_OK;
@@ -543,6 +544,8 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
     if(variational && (precision != AGX_PRECISION_FP64 || (flags & AGX_METHOD_DPRKN8)))
         return fail(AGX_EUNSUPPORTED, "deviation vectors are integrated with DOP853 in FP64 only");
     if(d_dev && !d_traj_offset) return fail(AGX_EINVAL, "deviation vectors need trajectory offsets");
+    if(d_targets && (variational || precision != AGX_PRECISION_FP64 || (flags & AGX_METHOD_DPRKN8)))
+        return fail(AGX_EUNSUPPORTED, "targets are collected with DOP853 in FP64, without deviation vectors");
     try {
         hipStream_t st = (hipStream_t)stream;
         void* work = nullptr;
@@ -569,8 +572,9 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
             A.chunkDir = pool.chunkDir; A.poolCounter = pool.counter; A.poolBase = pool.base;
             A.poolChunk = pool.chunk; A.maxChunks = pool.maxChunks; A.poolChunks = pool.chunks;
         }
+        A.targets = d_targets;
         int blocks, bs;
-        orbitGeometry(n, precision, pot->nfwMass != 0, blocks, bs, flags & AGX_METHOD_DPRKN8);
+        orbitGeometry(n, precision, pot->nfwMass != 0, blocks, bs, flags & AGX_METHOD_DPRKN8, d_targets != nullptr);
         HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
         hipError_t e;
         if(variational) {
@@ -1242,6 +1246,105 @@ void agx_orbit_record_free(agx_orbit_record* R)
 {
     if(R) (void)hipSetDevice(R->device);
     delete R;
+}
+
+}  // extern "C"
+
+// ---------------- target plug-ins ----------------
+struct agx_target {
+    int type = 0, lmax = 0, mmax = 0;
+    std::vector<double> gridr;        // radial nodes without the origin
+    int numValues = 0;
+};
+
+extern "C" {
+
+agx_target* agx_target_density_sphharm(int lmax, int mmax, const double* gridr, int ngridr)
+{
+    g_err.clear();
+    // TargetDensitySphHarm ctor (galaxymodel_densitygrid.cpp:356-369): a grid starting at zero
+    // loses that node (the origin is always implied)
+    if(!gridr || ngridr < 1) { fail(AGX_EINVAL, "TargetDensitySphHarm: invalid grid parameters"); return nullptr; }
+    std::unique_ptr<agx_target> T(new agx_target);
+    T->type = AGX_TARGET_DENSITY_SPHHARM;
+    T->lmax = lmax; T->mmax = mmax;
+    T->gridr.assign(gridr[0] > 0 ? gridr : gridr + 1, gridr + ngridr);
+    bool ok = lmax >= 0 && mmax >= 0 && mmax <= lmax && T->gridr.size() >= 1 && T->gridr[0] > 0;
+    for(size_t i = 1; i < T->gridr.size(); i++) ok &= T->gridr[i] > T->gridr[i-1];
+    if(!ok) { fail(AGX_EINVAL, "TargetDensitySphHarm: invalid grid parameters"); return nullptr; }
+    const int angular = (lmax/2+1) * (mmax/2+1) - mmax/2 * (mmax/2+1) / 2;
+    T->numValues = angular * (int)T->gridr.size() + 1;
+    return T.release();
+}
+
+int agx_target_num_values(const agx_target* t) { return t ? t->numValues : -1; }
+
+void agx_target_destroy(agx_target* t) { delete t; }
+
+int agx_orbit_target_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
+    const double* ttotal, double accuracy, double Omega, long maxNumSteps, int ntargets,
+    const agx_target* const* targets, float* const* out_targets, double* out_final, long* out_nsteps,
+    int* out_status, int flags)
+{
+    g_err.clear();
+    if(!pot) return fail(AGX_EINVAL, "NULL potential");
+    if(n < 0 || (n > 0 && (!ic || !ttotal))) return fail(AGX_EINVAL, "invalid orbit arrays");
+    if(ntargets < 1 || ntargets > AGX_MAX_TARGETS || !targets || !out_targets)
+        return fail(AGX_EINVAL, "between 1 and 4 targets with their outputs are required");
+    for(int t = 0; t < ntargets; t++)
+        if(!targets[t] || !out_targets[t]) return fail(AGX_EINVAL, "NULL target or output");
+    if(n == 0) return AGX_OK;
+    try {
+        HIPCHECK(hipSetDevice(pot->device));
+        AgxTargetList TL;
+        TL.count = ntargets;
+        std::vector<std::unique_ptr<DevBuf<double> > > grids, cubes;
+        for(int t = 0; t < ntargets; t++) {
+            const agx_target& T = *targets[t];
+            grids.emplace_back(new DevBuf<double>(T.gridr.size()));
+            cubes.emplace_back(new DevBuf<double>((size_t)n * (T.numValues + 1)));
+            HIPCHECK(hipMemcpy(grids.back()->p, T.gridr.data(), T.gridr.size() * 8, hipMemcpyHostToDevice));
+            HIPCHECK(hipMemset(cubes.back()->p, 0, (size_t)n * (T.numValues + 1) * 8));
+            AgxTargetDev& D = TL.t[t];
+            D.type = T.type; D.lmax = T.lmax; D.mmax = T.mmax; D.nr = (int)T.gridr.size();
+            D.numValues = T.numValues; D.gridr = grids.back()->p; D.cube = cubes.back()->p;
+        }
+        DevBuf<AgxTargetList> dTL(1);
+        HIPCHECK(hipMemcpy(dTL.p, &TL, sizeof(TL), hipMemcpyHostToDevice));
+        DevBuf<double> dic(n * 6), dts(tstart ? n : 0), dtt(n), dfin(out_final ? n * 6 : 0);
+        DevBuf<int> dsz(n), dst(out_status ? n : 0);
+        DevBuf<long> dns(out_nsteps ? n : 0);
+        std::vector<int> ones(n, 1);
+        HIPCHECK(hipMemcpy(dic.p, ic, n * 48, hipMemcpyHostToDevice));
+        if(tstart) HIPCHECK(hipMemcpy(dts.p, tstart, n * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dtt.p, ttotal, n * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dsz.p, ones.data(), n * 4, hipMemcpyHostToDevice));
+        int rc = orbitBatchDevice(pot, n, dic.p, dts.p, dtt.p, dsz.p, nullptr, nullptr, accuracy, Omega, maxNumSteps,
+            nullptr, nullptr, dfin.p, nullptr, dns.p, nullptr, nullptr, nullptr, dst.p, flags, nullptr, LyapDev(),
+            RecordPool(), dTL.p);
+        if(rc) return rc;
+        HIPCHECK(hipDeviceSynchronize());
+        // ~RuntimeFncTarget (galaxymodel_target.h:129-137): the datacube in storage precision,
+        // times 1/time (itself rounded to storage precision); nothing when no time elapsed
+        for(int t = 0; t < ntargets; t++) {
+            const int nv = targets[t]->numValues;
+            std::vector<double> cube((size_t)n * (nv + 1));
+            HIPCHECK(hipMemcpy(cube.data(), cubes[t]->p, cube.size() * 8, hipMemcpyDeviceToHost));
+            for(long i = 0; i < n; i++) {
+                const double* row = &cube[(size_t)i * (nv + 1)];
+                const double time = row[nv];
+                float* out = out_targets[t] + (size_t)i * nv;
+                for(int j = 0; j < nv; j++) out[j] = static_cast<float>(row[j]);
+                if(time == 0) continue;
+                const float invtime = static_cast<float>(1. / time);
+                for(int j = 0; j < nv; j++) out[j] *= invtime;
+            }
+        }
+        if(out_final) HIPCHECK(hipMemcpy(out_final, dfin.p, n * 48, hipMemcpyDeviceToHost));
+        if(out_nsteps) HIPCHECK(hipMemcpy(out_nsteps, dns.p, n * 8, hipMemcpyDeviceToHost));
+        if(out_status) HIPCHECK(hipMemcpy(out_status, dst.p, n * 4, hipMemcpyDeviceToHost));
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
 }
 
 }  // extern "C"

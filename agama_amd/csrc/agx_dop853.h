@@ -9,6 +9,7 @@
 #include <type_traits>
 #include "agx_device.h"
 #include "agx_kernels.h"
+#include "agx_target.h"
 
 namespace agxd {
 
@@ -86,6 +87,7 @@ struct LaneT {
     typedef typename std::conditional<KLDS, LdsVec<STRIDE>, double[6]>::type Vec;
     static constexpr int LDS_FIELDS = KLDS ? F_COUNT_K : F_COUNT;
     static constexpr int EVALS_PER_ATTEMPT = 12;
+    static constexpr bool HAS_TARGET = false;
     double* sh;                // this lane's LDS column
     int oi, phase, nbad, S, nsteps, natt, actual;
     Vec x, k1, k2, k3, k4, k5;
@@ -105,6 +107,12 @@ struct LaneT {
     __device__ __forceinline__ double& k8(int i) const { return sh[(F_K8 + i) * STRIDE]; }
     __device__ __forceinline__ double& k9(int i) const { return sh[(F_K9 + i) * STRIDE]; }
     __device__ __forceinline__ double& k10(int i) const { return sh[(F_K10 + i) * STRIDE]; }
+};
+
+// the lane of the orbit kernel that also feeds target plug-ins
+template<int STRIDE>
+struct LaneTgtT : LaneT<STRIDE, false> {
+    static constexpr bool HAS_TARGET = true;
 };
 
 template<class V>
@@ -248,6 +256,35 @@ __device__ __forceinline__ void sampleStep(const AgxOrbitArgs& A, Lane& L)
     }
 }
 
+// RuntimeFncTarget::processTimestep (galaxymodel_target.h:141-154): the step's duration, and
+// ten equally spaced dense-output points (rotating frame) each weighted by a tenth of the step
+template<class Lane>
+__device__ __forceinline__ void targetStep(const AgxOrbitArgs& A, Lane& L)
+{
+    const AgxTargetList& TL = *A.targets;
+    const int NUM_SAMPLES_PER_STEP = 10;
+    const double timestep = L.h, substep = timestep / NUM_SAMPLES_PER_STEP;
+    for(int s = 0; s < NUM_SAMPLES_PER_STEP; s++) {
+        const double off = substep * (s + 0.5);
+        const double d0 = denseOut(L, 0, off), d1 = denseOut(L, 1, off), d2 = denseOut(L, 2, off);
+        double x = d0, y = d1;
+        if(A.Omega != 0) {
+            double ca = 1, sa = 0;
+            sincosA(A.Omega * (off + L.tb), sa, ca);
+            x = d0 * ca + d1 * sa;
+            y = d1 * ca - d0 * sa;
+        }
+        for(int t = 0; t < TL.count; t++) {
+            const AgxTargetDev& T = TL.t[t];
+            targetAdd(T, T.cube + (long)L.oi * (T.numValues + 1), x, y, d2, substep);
+        }
+    }
+    for(int t = 0; t < TL.count; t++) {
+        const AgxTargetDev& T = TL.t[t];
+        T.cube[(long)L.oi * (T.numValues + 1) + T.numValues] += timestep;
+    }
+}
+
 template<class Lane>
 __device__ __forceinline__ void finishOrbit(const AgxOrbitArgs& A, Lane& L, int status)
 {
@@ -345,6 +382,7 @@ __device__ __forceinline__ void acceptStep(const AgxOrbitArgs& A, Lane& L, doubl
     const double mde = L.f(F_MAXDE);
     if(de > mde || de != de) L.f(F_MAXDE) = de;
     sampleStep(A, L);
+    if constexpr(Lane::HAS_TARGET) targetStep(A, L);
     L.f(F_LASTTB) = L.tb;
     L.f(F_LASTH) = L.h;
     if(L.S == 1) L.actual = 1;

@@ -29,6 +29,8 @@ struct AgxOrbitArgs {
     unsigned int* poolCounter = nullptr;
     long poolBase = 0;
     int poolChunk = 0, maxChunks = 0, poolChunks = 0;
+    // target plug-ins (agx_target.h), device memory, or null
+    const struct AgxTargetList* targets = nullptr;
 };
 
 enum AgxPrecision { AGX_FP64 = 0, AGX_FP32_TABLES = 1 };

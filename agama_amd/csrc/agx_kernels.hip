@@ -16,6 +16,7 @@
 // The integrator restates src/math_ode.cpp:35-423 (initTimeStep, DOP853 init/doStep/getSol),
 // src/orbit.cpp:15-133, 182-208 (trajectory sampling, run loop, rotating-frame RHS).
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "agx_device.h"
 #include "agx_kernels.h"
 #include "agx_dop853.h"
@@ -101,13 +102,15 @@ __device__ __forceinline__ void rhsEval(const AgxMultipoleDev& P, vd2* stage, do
 
 // WAVES: wavefronts per SIMD the kernel is compiled for (register budget 512 / WAVES); both
 // variants exist so that the autotuner can choose at run time.
-template<int MODE, int WAVES, bool NFW>
+// TGT: the variant that also feeds target plug-ins (agx_target.h)
+template<int MODE, int WAVES, bool NFW, bool TGT = false>
 __global__ __launch_bounds__(OBS) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
 void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
 {
     // at one wavefront per SIMD the block has the CU's LDS to itself: the stage vectors k1..k5
     // and x live there too (the cooperative fetcher needs its own LDS staging area instead)
-    typedef LaneT<OBS, AGX_K_IN_LDS && WAVES == 1 && MODE != AGX_FETCH_COOP> Lane;
+    typedef typename std::conditional<TGT, LaneTgtT<OBS>,
+        LaneT<OBS, AGX_K_IN_LDS && WAVES == 1 && MODE != AGX_FETCH_COOP> >::type Lane;
     __shared__ double lds[Lane::LDS_FIELDS * OBS];
     __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? (OBS / 64) * FetchCoop::LDS_DOUBLE2_PER_WAVE : 1];
     __shared__ double gridLds[AGX_LDS_GRID];
@@ -176,8 +179,10 @@ int agxOrbitKernelBlockSize() { return OBS; }
 
 namespace {
 typedef void (*OrbitKernelFn)(const AgxMultipoleDev*, AgxOrbitArgs);
-OrbitKernelFn orbitKernelFor(int precision, int fetch, int waves, bool nfw)
+OrbitKernelFn orbitKernelFor(int precision, int fetch, int waves, bool nfw, bool targets = false)
 {
+    if(targets)   // target plug-ins: FP64, 16-byte per-lane fetch, 1 wave/SIMD
+        return nfw ? orbitKernel<AGX_FETCH_VEC2, 1, true, true> : orbitKernel<AGX_FETCH_VEC2, 1, false, true>;
     if(nfw)   // composite with an NFW halo: 16-byte per-lane fetch (or FP32 tables), 1 wave/SIMD
         return precision == AGX_FP32_TABLES ? orbitKernel<-1, 1, true> : orbitKernel<AGX_FETCH_VEC2, 1, true>;
     const bool two = waves == 2;
@@ -193,10 +198,11 @@ __global__ void fillU32Kernel(unsigned int* p, long n, unsigned int v)
 }
 }  // namespace
 
-int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves, bool nfw)
+int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves, bool nfw, bool targets)
 {
     int nb = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernelFor(precision, fetch, waves, nfw), OBS, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernelFor(precision, fetch, waves, nfw,
+        targets), OBS, 0);
     return e == hipSuccess ? nb : 1;
 }
 
@@ -206,8 +212,9 @@ hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, i
     (void)P;
     if(A.n <= 0) return hipSuccess;
     if(blockSize != OBS) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(orbitKernelFor(precision, fetch, waves, P.nfwMass != 0), dim3(numBlocks), dim3(OBS), 0,
-        stream, dP, A);
+    if(A.targets && precision == AGX_FP32_TABLES) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(orbitKernelFor(precision, fetch, waves, P.nfwMass != 0, A.targets != nullptr), dim3(numBlocks),
+        dim3(OBS), 0, stream, dP, A);
     return hipGetLastError();
 }
 

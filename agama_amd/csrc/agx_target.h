@@ -1,0 +1,95 @@
+// Target plug-ins collected during orbit integration on the device: RuntimeFncTarget
+// (src/galaxymodel_target.h:95-155: ten dense-output samples per accepted step, each weighted
+// by its share of the step) with the density discretisation TargetDensitySphHarm
+// (src/galaxymodel_densitygrid.cpp:356-411: linear B-splines in r times even spherical
+// harmonics).  One lane accumulates its orbit's datacube in FP64 in global memory (its own
+// row: no atomics); the host normalises by the integration time at the end.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "agx_device.h"
+
+enum : int { AGX_TARGET_DENSITY_SPHHARM = 1 };
+enum : int { AGX_MAX_TARGETS = 4 };
+
+struct AgxTargetDev {
+    int type;
+    int lmax, mmax;          // even orders of the angular expansion
+    int nr;                  // radial grid nodes (the origin implied)
+    int numValues;           // datacube length; the row holds one more value: the accumulated time
+    const double* gridr;     // nr nodes, increasing, > 0
+    double* cube;            // n x (numValues + 1) per orbit
+};
+
+struct AgxTargetList {
+    int count = 0;
+    AgxTargetDev t[AGX_MAX_TARGETS];
+};
+
+namespace agxd {
+
+// TargetDensitySphHarm::addPoint (galaxymodel_densitygrid.cpp:371-411): the point's share of
+// the two radial nodes around r of every even (l, m) term
+__device__ __noinline__ void sphHarmTargetAdd(const AgxTargetDev& T, double* values, double x, double y, double z,
+    double mult)
+{
+    const double R = sqrt(x * x + y * y), phi = atan2A(y, x);
+    const double r = sqrt(R * R + z * z), tau = z / (r + R);
+    if(r == 0) { values[0] += mult; return; }
+    const int nr = T.nr;
+    const int indr = binSearchD(r, T.gridr, nr) + 1;
+    if(indr >= nr || mult == 0) return;
+    const double offr = indr == 0 ? r / T.gridr[0] : (r - T.gridr[indr - 1]) / (T.gridr[indr] - T.gridr[indr - 1]);
+    // cos(k phi) by the recurrence of trigMultiAngle (math_sphharm.cpp:134-152), two steps per even m
+    double sphi, cphi, sh, ch;
+    sincosA(phi, sphi, cphi);
+    sincosA(phi / 2, sh, ch);
+    const double alpha = 2 * sh * sh, beta = sphi;
+    double c1 = 1, s1 = 0;   // cos, sin of k*phi, k = 0
+    const double ct = 2 * tau / (1 + tau * tau), st = (1 - tau * tau) / (1 + tau * tau);
+    int offset = indr + 1;
+    for(int m = 0; m <= T.mmax; m += 2) {
+        if(m > 0)
+            for(int k = 0; k < 2; k++) {
+                const double cn = c1 - (alpha * c1 + beta * s1), sn = s1 - (alpha * s1 - beta * c1);
+                c1 = cn; s1 = sn;
+            }
+        const double angular = m == 0 ? 1. : M_SQRT2 * c1;
+        // normalised associated Legendre functions P_l^m(theta), l = m..lmax (sphHarmArray,
+        // math_sphharm.cpp:22-132, values only)
+        double fact2m = 1, dfact = 1;
+        for(int i = 2; i <= 2 * m; i++) fact2m *= i;
+        for(int i = 2 * m - 1; i > 1; i -= 2) dfact *= i;
+        double prefact = 0.5 / 1.77245385090551602729816748334 * sqrt((2 * m + 1) / fact2m);
+        double Pl;
+        if(T.lmax == 0) Pl = prefact;
+        else if(m == 0) Pl = prefact;
+        else {
+            const double coef = prefact * dfact;   // (-1)^m = 1 for even m
+            double sinm2 = 1, xb = st;
+            for(int n = m - 2; n; n >>= 1) { if(n % 2) sinm2 *= xb; xb *= xb; }
+            Pl = coef * sinm2 * (st * st);
+        }
+        double Plm1 = Pl / prefact, Plm = ct * (2 * m + 1) * Plm1, Plm2 = 0;
+        for(int l = m; l <= T.lmax; l++) {
+            if(l > m) {
+                if(l > m + 1) Plm = (ct * (2 * l - 1) * Plm1 - (l + m - 1) * Plm2) / (l - m);
+                prefact *= sqrt((2 * l + 1.) / (2 * l - 1.) * (l - m) / (l + m));
+                Pl = Plm * prefact;
+                Plm2 = Plm1;
+                Plm1 = Plm;
+            }
+            if((l - m) % 2) continue;
+            const double val = mult * Pl * 2 * 1.77245385090551602729816748334 * angular;
+            values[offset] += val * offr;
+            if(indr > 0 || l == 0) values[offset - 1] += val * (1 - offr);
+            offset += nr;
+        }
+    }
+}
+
+__device__ __forceinline__ void targetAdd(const AgxTargetDev& T, double* row, double x, double y, double z, double mult)
+{
+    if(T.type == AGX_TARGET_DENSITY_SPHHARM) sphHarmTargetAdd(T, row, x, y, z, mult);
+}
+
+}  // namespace agxd

@@ -60,7 +60,7 @@ class _Record:
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
-        if h:
+        if h and lib is not None:   # not during interpreter shutdown
             lib.agx_orbit_record_free(h)
 
 

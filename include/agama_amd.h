@@ -236,6 +236,29 @@ int agx_orbit_record_interp(agx_orbit_record* rec, long m, const long* series, c
     double* out);
 void agx_orbit_record_free(agx_orbit_record* rec);
 
+/* ---- target plug-ins (agama.Target, src/interface_python.cpp:7005-7250, and orbit(targets=),
+   :8115-8120, 8153-8162) ---- */
+typedef struct agx_target agx_target;
+/* TargetDensitySphHarm (src/galaxymodel_densitygrid.cpp:356-411): density discretised as
+   linear B-splines on the radial grid (a grid starting at 0 loses that node; the origin is
+   always a node of the l=0 term) times the spherical harmonics of even l <= lmax and even
+   0 <= m <= mmax.  NULL on invalid parameters. */
+agx_target* agx_target_density_sphharm(int lmax, int mmax, const double* gridr, int ngridr);
+/* number of values a target records per orbit (numCoefs), -1 for NULL */
+int agx_target_num_values(const agx_target* t);
+void agx_target_destroy(agx_target* t);
+/* Integrate a batch (DOP853, FP64) feeding 1..4 targets: per accepted step ten equally spaced
+   dense-output points, each weighted by a tenth of the step (RuntimeFncTarget,
+   src/galaxymodel_target.h:95-155), accumulated per orbit in FP64 on the device;
+   out_targets[t][n * numValues] receives the datacube in float divided by the orbit's
+   integration time (unchanged when it is 0).  out_final/out_nsteps/out_status as in
+   agx_orbit_batch (may be NULL).  DPRKN8, FP32 tables and deviation vectors are not combined
+   with targets (AGX_EUNSUPPORTED). */
+int agx_orbit_target_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
+    const double* ttotal, double accuracy, double Omega, long maxNumSteps, int ntargets,
+    const agx_target* const* targets, float* const* out_targets, double* out_final, long* out_nsteps,
+    int* out_status, int flags);
+
 /* Launch geometry of the orbit kernel: number of workgroups (0 = one full wave of resident
    workgroups on every CU) and threads per workgroup (0 or 256, the only size it is built for).
    Set by the autotuner (agama_amd.tune). */

@@ -2470,6 +2470,42 @@ inline double energyOf(const double* xv, double Epot, double Omega)
     return E;
 }
 
+// TargetDensitySphHarm (galaxymodel_densitygrid.cpp:356-411) with the datacube of a
+// RuntimeFncTarget (galaxymodel_target.h:95-155): ten dense-output points per step
+struct OrcTargetSphHarm {
+    int lmax = 0, mmax = 0;
+    std::vector<double> gridr;   // without a node at the origin
+    int numValues() const
+    { return ((lmax/2+1) * (mmax/2+1) - mmax/2 * (mmax/2+1) / 2) * (int)gridr.size() + 1; }
+    void addPoint(const double point[3], double mult, double values[]) const
+    {
+        const double R = sqrt(pow_2(point[0]) + pow_2(point[1])), z = point[2];
+        const double phi = atan2_r(point[1], point[0]);
+        const double r = sqrt(pow_2(R) + pow_2(z)), tau = z / (r + R);
+        if(r == 0) { values[0] += mult; return; }
+        const int gridrsize = gridr.size();
+        const int indr = (int)binSearch(r, &gridr[0], gridrsize) + 1;
+        if(indr >= gridrsize || mult == 0) return;
+        const double offr = indr == 0 ? r / gridr[0] : (r - gridr[indr-1]) / (gridr[indr] - gridr[indr-1]);
+        std::vector<double> leg(lmax + 1), trig(mmax + 1);
+        trigMultiAngle(phi, mmax, false, &trig[0]);
+        for(int m = 0, offset = indr + 1; m <= mmax; m += 2) {
+            sphHarmArray(lmax, m, tau, &leg[0], NULL, NULL);
+            for(int l = m; l <= lmax; l += 2, offset += gridrsize) {
+                const double val = mult * leg[l-m] * 2*SQRTPI * (m == 0 ? 1. : M_SQRT2 * trig[m-1]);
+                values[offset] += val * offr;
+                if(indr > 0 || l == 0)
+                    values[offset-1] += val * (1 - offr);
+            }
+        }
+    }
+};
+struct TargetSink {
+    const OrcTargetSphHarm* target = nullptr;
+    std::vector<double> cube;
+    double time = 0;
+};
+
 // orbit.cpp:52-84 (run), 98-133 (init/getSol for Car), 15-47 (RuntimeTrajectory), and the
 // per-orbit body of interface_python.cpp:8107-8202 with trajectory stored as float64
 struct OrbitResult { long nsteps, nattempts, nevals; double maxde; int status; double fin[6]; long rows; };
@@ -2488,7 +2524,8 @@ struct VarOutput {
 template<class Stepper>
 void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart, double ttotal,
     int trajsize, double accuracy, double Omega, long maxNumSteps,
-    double* out_time, double* out_traj, OrbitResult& res, VarOutput* var = nullptr, long capacity = -1)
+    double* out_time, double* out_traj, OrbitResult& res, VarOutput* var = nullptr, long capacity = -1,
+    TargetSink* tgt = nullptr)
 {
     const double ROUNDOFF = 10*DBL_EPSILON;
     // rows available in the output block: trajsize, or `capacity` (trajsize 0 records every
@@ -2697,6 +2734,18 @@ void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart
         if(logDevVec > 10)
             renormalize();
     };
+    // RuntimeFncTarget::processTimestep (galaxymodel_target.h:141-154)
+    auto targetStep = [&](double tbegin, double timestep) {
+        (void)tbegin;
+        tgt->time += timestep;
+        const int NUM_SAMPLES_PER_STEP = 10;
+        const double substep = timestep / NUM_SAMPLES_PER_STEP;
+        for(int s = 0; s < NUM_SAMPLES_PER_STEP; s++) {
+            double point[6];
+            getSol(substep * (s + 0.5), point);
+            tgt->target->addPoint(point, substep, &tgt->cube[0]);
+        }
+    };
     // run (orbit.cpp:52-84)
     res.status = 0;
     long numSteps = 0;
@@ -2704,6 +2753,7 @@ void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart
     if(ttotal==0 || !isFinite(timeEnd)) {
         processTimestep(sys.timeBegin, timeStep);
         if(wantVar) varStep(sys.timeBegin, timeStep);
+        if(tgt) targetStep(sys.timeBegin, timeStep);
     } else {
         double sign = ttotal>0 ? +1 : -1;
         while(sys.timeBegin != timeEnd) {
@@ -2717,6 +2767,7 @@ void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart
             if(de > res.maxde || de != de) res.maxde = de;
             processTimestep(sys.timeBegin, timeStep);
             if(wantVar) varStep(sys.timeBegin, timeStep);
+            if(tgt) targetStep(sys.timeBegin, timeStep);
             sys.timeBegin = timeStep == timeRemaining ? timeEnd : sys.timeBegin + timeStep;
             if(++numSteps >= maxNumSteps) {
                 res.status = 2;
@@ -2747,14 +2798,14 @@ void integrateOneOrbitWith(const orc_pot& pot, const double ic[6], double tstart
 void integrateOneOrbit(const orc_pot& pot, const double ic[6], double tstart, double ttotal,
     int trajsize, double accuracy, double Omega, long maxNumSteps,
     double* out_time, double* out_traj, OrbitResult& res, int method=0, VarOutput* var=nullptr,
-    long capacity=-1)
+    long capacity=-1, TargetSink* tgt=nullptr)
 {
     if(method == 1)
         integrateOneOrbitWith<DPRKN8>(pot, ic, tstart, ttotal, trajsize, accuracy, Omega, maxNumSteps,
-            out_time, out_traj, res, var, capacity);
+            out_time, out_traj, res, var, capacity, tgt);
     else
         integrateOneOrbitWith<DOP853>(pot, ic, tstart, ttotal, trajsize, accuracy, Omega, maxNumSteps,
-            out_time, out_traj, res, var, capacity);
+            out_time, out_traj, res, var, capacity, tgt);
 }
 
 struct CallbackSys : public OdeSys {
@@ -3060,6 +3111,43 @@ int orc_orbit_steps(const orc_pot* p, long n, const double* ic, const double* ts
         if(out_rows) out_rows[orb] = r.rows;
         if(out_nsteps) out_nsteps[orb] = r.nsteps;
         if(out_status) out_status[orb] = r.status;
+    }
+    if(fail) g_err = "exception during orbit integration";
+    return fail;
+}
+
+int orc_orbit_target_sphharm(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, double accuracy, double Omega, long maxNumSteps, int lmax, int mmax,
+    const double* gridr, int ngridr, float* out, int nthreads)
+{
+    g_err.clear();
+    OrcTargetSphHarm T;
+    T.lmax = lmax; T.mmax = mmax;
+    T.gridr.assign(ngridr > 0 && gridr[0] > 0 ? gridr : gridr + 1, gridr + ngridr);
+    const int nv = T.numValues();
+    int fail = 0;
+#ifdef _OPENMP
+    if(nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for(long orb=0; orb<n; orb++) {
+        OrbitResult r;
+        TargetSink sink;
+        sink.target = &T;
+        sink.cube.assign(nv, 0.);
+        try {
+            integrateOneOrbit(*p, &ic[orb*6], tstart ? tstart[orb] : 0, ttotal[orb], 1, accuracy, Omega,
+                maxNumSteps, NULL, NULL, r, 0, nullptr, -1, &sink);
+        } catch(std::exception&) {
+            fail = 1;
+            continue;
+        }
+        // ~RuntimeFncTarget (galaxymodel_target.h:129-137)
+        float* o = out + orb * nv;
+        for(int j=0; j<nv; j++) o[j] = static_cast<float>(sink.cube[j]);
+        if(sink.time == 0) continue;
+        const float invtime = static_cast<float>(1. / sink.time);
+        for(int j=0; j<nv; j++) o[j] *= invtime;
     }
     if(fail) g_err = "exception during orbit integration";
     return fail;

@@ -96,6 +96,11 @@ int orc_orbit_steps(const orc_pot* p, long n, const double* ic, const double* ts
     const double* ttotal, const long* capacity, const long* traj_offset, double accuracy, double Omega,
     long maxNumSteps, int method, int der, double* out_time, double* out_traj, double* out_dev,
     long* out_rows, long* out_nsteps, int* out_status, int nthreads);
+/* agama.orbit(targets=[Target(type='DensitySphHarm', gridr, lmax, mmax)]) with DOP853:
+   out[n * numValues] the normalised datacube in float (RuntimeFncTarget + TargetDensitySphHarm) */
+int orc_orbit_target_sphharm(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, double accuracy, double Omega, long maxNumSteps, int lmax, int mmax,
+    const double* gridr, int ngridr, float* out, int nthreads);
 /* agama.Orbit on one recorded series (createOrbitObject + Orbit.__call__): out[m*6] rotating
    frame, NaN outside the span; out_spl[m*9] the inertial-frame splines x,y,z, their first and
    second derivatives (linear continuation outside) */

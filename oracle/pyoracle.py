@@ -55,6 +55,9 @@ def _load(name):
                                            _vp, _vp, _vp, _vp, ctypes.c_int]),
         "orc_orbit_interp": (ctypes.c_int, [ctypes.c_long, _vp, _vp, ctypes.c_double, ctypes.c_long, _vp, _vp,
                                             _vp, _vp, _vp]),
+        "orc_orbit_target_sphharm": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_long, ctypes.c_int, ctypes.c_int, _vp,
+                                                    ctypes.c_int, _vp, ctypes.c_int]),
         "orc_ode_gl4": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                        _vp, _vp]),
         "orc_ode_dop853_fixed": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
@@ -285,6 +288,32 @@ def _orbits_steps(self, ic, ttotal, tstart=None, accuracy=1e-8, Omega=0.0, maxNu
 
 
 OraclePotential.orbits_steps = _orbits_steps
+
+
+def sphharm_target_size(lmax, mmax, gridr):
+    ng = len(gridr) - (1 if gridr[0] == 0 else 0)
+    return ((lmax // 2 + 1) * (mmax // 2 + 1) - (mmax // 2) * (mmax // 2 + 1) // 2) * ng + 1
+
+
+def _orbits_target_sphharm(self, ic, ttotal, gridr, lmax, mmax, tstart=None, accuracy=1e-8, Omega=0.0,
+                           maxNumSteps=int(1e8), nthreads=0):
+    """agama.orbit(targets=[Target(type='DensitySphHarm', ...)]) as the oracle restates it:
+    (n, numCoefs) float32"""
+    ic = np.ascontiguousarray(ic, dtype=np.float64).reshape(-1, 6)
+    n = len(ic)
+    tt = np.ascontiguousarray(np.broadcast_to(np.asarray(ttotal, dtype=np.float64), (n,)))
+    ts = None if tstart is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(tstart, dtype=np.float64), (n,)))
+    gr = np.ascontiguousarray(gridr, dtype=np.float64)
+    out = np.empty((n, sphharm_target_size(lmax, mmax, gr)), dtype=np.float32)
+    rc = self.lib.orc_orbit_target_sphharm(self.h, n, _p(ic), _p(ts), _p(tt), accuracy, Omega, maxNumSteps,
+                                           int(lmax), int(mmax), _p(gr), len(gr), _p(out), nthreads)
+    if rc:
+        raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
+    return out
+
+
+OraclePotential.orbits_target_sphharm = _orbits_target_sphharm
 
 
 def orbit_interp(t, traj, Omega, tq):
