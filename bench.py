@@ -62,7 +62,8 @@ def parse():
     ap.add_argument("--cpu-orbits", type=int, default=20_000,
                     help="size of the CPU-baseline sample (0 disables)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["orbit", "fp64", "eval", "hess", "mlp", "copy", "orbitobj", "construct"],
+    ap.add_argument("--mode", choices=["orbit", "fp64", "eval", "hess", "mlp", "copy", "orbitobj", "construct",
+                                       "target"],
                     default="orbit",
                     help="orbit: surrogate and FP64 paths (default); fp64: the FP64 path alone; "
                          "eval: potential + force at 1M points drawn from the density; hess: the same "
@@ -70,7 +71,8 @@ def parse():
                          "surrogate's fused MFMA evaluation at 1M points; copy: one 1 GiB "
                          "device-to-device copy, the reference for HBM byte counters; orbitobj: "
                          "orbit(dtype=object) - every step recorded, splines built and evaluated; "
-                         "construct: Multipole from a density (Poisson solver) on the device vs the host")
+                         "construct: Multipole from a density (Poisson solver) on the device vs the host; "
+                         "target: orbits feeding a DensitySphHarm target")
     ap.add_argument("--method", choices=["dop853", "dprkn8"], default="dop853",
                     help="integrator of the FP64 Multipole path (the surrogate path uses DOP853)")
     ap.add_argument("--train-steps", type=int, default=6000)
@@ -181,6 +183,8 @@ def main():
         return bench_orbitobj(args)
     if args.mode == "construct":
         return bench_construct(args)
+    if args.mode == "target":
+        return bench_target(args)
     import numpy as np
     import torch
     import agama_amd
@@ -438,6 +442,59 @@ def bench_orbitobj(args):
                         "kernel_ms": rec * 1e3},
            "cpu_baseline": cpu}
     print(json.dumps(res), flush=True)
+
+
+def bench_target(args):
+    """orbit(targets=[Target(type='DensitySphHarm', gridr=16 nodes, lmax=6, mmax=4)]): 100k orbits
+    x 10 circular periods, DOP853 accuracy 1e-8, ten dense-output samples per step added into a
+    145-value datacube per orbit; value = accepted steps/s of the whole call (host arrays in,
+    float32 matrix out), beside the same orbits without the target"""
+    import numpy as np
+    import agama_amd
+    from agama_amd import workloads
+    coef = os.path.join(ROOT, "tests", "golden", "dehnen_triax_lmax8.coef")
+    pot = agama_amd.Potential(file=coef)
+    n = 100_000 if args.orbits == 1_000_000 else args.orbits
+    ic, T = workloads.triaxial_dehnen_orbits(n, lambda p: pot.potential(p), pot.coefs(), seed=42, periods=10)
+    gridr = np.concatenate([[0.0], np.logspace(-2, 3, 16)])
+    tgt = agama_amd.Target(type="DensitySphHarm", gridr=gridr, lmax=6, mmax=4)
+    info = {}
+    for _ in range(args.warmup):
+        agama_amd.orbit(potential=pot, ic=ic, time=T, targets=tgt)
+        print("target: warmup step done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(max(args.steps, 1)):
+        mat = agama_amd.orbit(potential=pot, ic=ic, time=T, targets=tgt, info=info)
+    wall = (time.perf_counter() - t0) / max(args.steps, 1)
+    steps = int(info["nsteps"].sum())
+    t0 = time.perf_counter()
+    agama_amd.orbit(potential=pot, ic=ic, time=T, info=info)
+    plain = time.perf_counter() - t0
+    print(f"target: {n} orbits, {steps} steps; with target {wall*1e3:.1f} ms, without {plain*1e3:.1f} ms; "
+          f"datacube {mat.shape}", file=sys.stderr, flush=True)
+    cpu = None
+    if args.cpu_orbits > 0:
+        from oracle import pyoracle
+        threads = args.cpu_threads or os.cpu_count()
+        fast = pyoracle.OraclePotential(coef, library=pyoracle.fast())
+        m = min(20000, n)
+        t0 = time.perf_counter()
+        fast.orbits_target_sphharm(ic[:m], T[:m], gridr, 6, 4, nthreads=threads)
+        dt = time.perf_counter() - t0
+        frac = m / n
+        cpu = {"value": steps * frac / dt, "unit": "orbit-steps/s", "cores": threads, "kind": "port",
+               "sample": f"first {m} of the {n} orbits with the same target, C++ OpenMP restatement (-O2, "
+                         f"{threads} threads), {dt:.1f} s (steps scaled by the sample's share of orbits)"}
+        print(f"cpu baseline: {cpu['value']/1e6:.2f} Msteps/s ({dt:.1f} s)", file=sys.stderr, flush=True)
+    out = {"metric": "orbit-steps/s with a DensitySphHarm target (triaxial Dehnen Multipole lmax=8)",
+           "value": steps / wall, "unit": "orbit-steps/s", "n_gpus": 1, "steps": max(args.steps, 1),
+           "warmup": args.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{n} orbits x 10 circular periods, DOP853 accuracy 1e-8, DensitySphHarm target "
+                                  "(16 radial nodes, lmax=6, mmax=4: 145 values per orbit)",
+                      "accepted_steps": steps, "without_target_ms": plain * 1e3},
+           "roofline": None, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
 
 
 def bench_construct(args):
