@@ -368,6 +368,9 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
         info.update(nsteps=nsteps, nattempts=natt, nevals=nev, maxde=maxde, status=status, final=final)
         if surrogate:
             info["n_fallback"] = int(nfb[0])
+            pil = np.zeros(2, dtype=np.int64)
+            lib.agx_surrogate_last_pilot(ptr(pil[:1]), ptr(pil[1:]))
+            info["pilot"] = (int(pil[0]), int(pil[1]))   # pilot orbits, of which fell back
     if not have_traj:
         if lyapunov:
             return lyap[0] if single else lyap

@@ -71,6 +71,10 @@ int g_orbitWaves = 1;                                                // set by a
 
 int fail(int code, const std::string& msg) { g_err = msg; return code; }
 
+// the surrogate orbit batch's pilot (agx_set_surrogate_pilot) and its last outcome
+int g_surrogatePilot = 1;
+thread_local long g_lastPilot[2] = { 0, 0 };   // pilot orbits, of which fell back
+
 // where Multipole coefficients of a density are computed (agx_set_construction); -1: not set,
 // the device for agx_potential_create and the host for agx_build_coef_file
 int g_construction = -1;
@@ -1725,13 +1729,49 @@ int agx_surrogate_orbit_batch_device(const agx_surrogate* s, long n, const doubl
         HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         const long need = (n + 63) / 64;
         const int blocks = (int)std::min<long>(need, (long)cus * agxOrbitSurrogateBlocksPerCU());
-        HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
-        HIPCHECK(agxLaunchFillU32(nfb, 1, 0u, st));
-        e = agxLaunchOrbitSurrogate(pot->ddev, s->base->ddev, s->dM, A, SA, blocks, st);
-        if(e != hipSuccess) { release(); return fail(AGX_EDEVICE, std::string("surrogate orbit launch: ") + hipGetErrorString(e)); }
+        // Pilot (batches of 8192 orbits or more): every 64th orbit for at most 200 steps on the
+        // surrogate, outputs discarded.  When half of them or more already exceed max_rel_de or
+        // leave the training radii, the surrogate cannot hold this batch to the tolerance and
+        // every orbit goes to the FP64 Multipole directly (all marked as fallback) instead of
+        // being integrated twice.
+        bool routeAll = false;
+        g_lastPilot[0] = g_lastPilot[1] = 0;
+        if(n >= 8192 && g_surrogatePilot) {
+            const long np = std::max(1024L, n / 64), stride = n / np;
+            std::vector<int> pl(np);
+            for(long j = 0; j < np; j++) pl[j] = (int)(j * stride);
+            void* pw = nullptr;
+            HIPCHECK(hipMallocAsync(&pw, (size_t)np * 4, st));
+            HIPCHECK(hipMemcpyAsync(pw, pl.data(), (size_t)np * 4, hipMemcpyHostToDevice, st));
+            AgxOrbitArgs PA = A;
+            PA.n = np; PA.order = static_cast<int*>(pw);
+            PA.maxNumSteps = std::min(maxNumSteps, 200L);
+            PA.outTime = nullptr; PA.outTraj = nullptr; PA.outFinal = nullptr; PA.outNsteps = nullptr;
+            PA.outNattempts = nullptr; PA.outNevals = nullptr; PA.outMaxde = nullptr; PA.outStatus = nullptr;
+            const int pblocks = (int)std::min<long>((np + 63) / 64, (long)cus * agxOrbitSurrogateBlocksPerCU());
+            HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
+            HIPCHECK(agxLaunchFillU32(nfb, 1, 0u, st));
+            e = agxLaunchOrbitSurrogate(pot->ddev, s->base->ddev, s->dM, PA, SA, pblocks, st);
+            unsigned int pf = 0;
+            if(e == hipSuccess) e = hipMemcpyAsync(&pf, nfb, sizeof(pf), hipMemcpyDeviceToHost, st);
+            if(e == hipSuccess) e = hipStreamSynchronize(st);
+            (void)hipFreeAsync(pw, st);
+            if(e != hipSuccess) { release(); return fail(AGX_EDEVICE, std::string("surrogate pilot: ") + hipGetErrorString(e)); }
+            g_lastPilot[0] = np; g_lastPilot[1] = pf;
+            routeAll = 2 * (long)pf >= np;
+        }
         unsigned int nf = 0;
-        HIPCHECK(hipMemcpyAsync(&nf, nfb, sizeof(nf), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
+        if(routeAll) {
+            nf = (unsigned int)n;
+            list = ord;   // every orbit, longest first
+        } else {
+            HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
+            HIPCHECK(agxLaunchFillU32(nfb, 1, 0u, st));
+            e = agxLaunchOrbitSurrogate(pot->ddev, s->base->ddev, s->dM, A, SA, blocks, st);
+            if(e != hipSuccess) { release(); return fail(AGX_EDEVICE, std::string("surrogate orbit launch: ") + hipGetErrorString(e)); }
+            HIPCHECK(hipMemcpyAsync(&nf, nfb, sizeof(nf), hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+        }
         if(nf > 0) {   // the fallback orbits, integrated again with the FP64 Multipole
             A.n = nf;
             A.order = list;
@@ -1746,6 +1786,19 @@ int agx_surrogate_orbit_batch_device(const agx_surrogate* s, long n, const doubl
         if(n_fallback) *n_fallback = nf;
         return AGX_OK;
     } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
+int agx_set_surrogate_pilot(int enable)
+{
+    g_surrogatePilot = enable != 0;
+    return AGX_OK;
+}
+
+int agx_surrogate_last_pilot(long* out_pilot_orbits, long* out_pilot_fallbacks)
+{
+    if(out_pilot_orbits) *out_pilot_orbits = g_lastPilot[0];
+    if(out_pilot_fallbacks) *out_pilot_fallbacks = g_lastPilot[1];
+    return AGX_OK;
 }
 
 int agx_surrogate_orbit_batch(const agx_surrogate* s, long n, const double* ic, const double* tstart,

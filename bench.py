@@ -10,8 +10,10 @@ One bench step = one integration of the whole batch; inputs and outputs stay res
 Default mode runs both forms on the same initial conditions:
   FP64 Multipole: one persistent-kernel launch per step (its time and HIP-event kernel time);
   MLP surrogate: the network is first trained on the GPU (not timed), then each step is one
-    surrogate-orbit call - the surrogate kernel with the FP64 Multipole as energy judge, and
-    the FP64 kernel for the orbits that exceed dE/E = 1e-6 or leave the training radii.
+    surrogate-orbit call - a pilot of every 64th orbit on the surrogate (200 steps), then either
+    the surrogate kernel with the FP64 Multipole as energy judge and the FP64 kernel for the
+    orbits that exceed dE/E = 1e-6 or leave the training radii, or, when the pilot shows that
+    the surrogate cannot hold the batch to the tolerance, the FP64 kernel for the whole batch.
 `value` = accepted orbit-steps / s of the surrogate path (the steps of the final results over
 the whole call); the FP64 path's figure and the ratio are reported next to it.  `--mode
 fp64` runs the FP64 path alone (its figure is then `value`).
@@ -277,6 +279,8 @@ def main():
                 raise RuntimeError(agama_amd._native.last_error())
 
         swall, _ = timed(launch_sur, "MLP surrogate")
+        pilot = np.zeros(2, dtype=np.int64)
+        lib.agx_surrogate_last_pilot(pilot[:1].ctypes.data, pilot[1:].ctypes.data)
         sns = d_ns.cpu().numpy()
         sde = d_de.cpu().numpy()
         sst = d_st.cpu().numpy()
@@ -293,6 +297,8 @@ def main():
         surrogate = {
             "value": value, "fp64_multipole_value": fp64_value, "surrogate_over_fp64": value / fp64_value,
             "max_rel_de": args.max_rel_de, "fallback_orbits": nf, "fallback_fraction": nf / n,
+            "pilot": {"orbits": int(pilot[0]), "fell_back": int(pilot[1]),
+                      "batch_routed_to_fp64": bool(2 * pilot[1] >= pilot[0] > 0)},
             "orbits_kept_on_surrogate": n - nf,
             "max_dE_over_E": float(np.max(sde)), "status_counts": np.bincount(sst & 7).tolist(),
             "force_err_median": float(np.median(ferr)), "force_err_p99": float(np.quantile(ferr, 0.99)),

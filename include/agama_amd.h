@@ -365,6 +365,14 @@ int agx_surrogate_orbit_batch_device(const agx_surrogate* s, long n, const doubl
                                      long maxNumSteps, double max_rel_de, double* d_time, double* d_traj,
                                      double* d_final, long* d_nsteps, long* d_nattempts, long* d_nevals,
                                      double* d_maxde, int* d_status, long* n_fallback, void* stream);
+/* Batch routing of the two calls above (no counterpart in the reference, whose path has no
+   surrogate): for n >= 8192 a pilot of every 64th orbit runs on the surrogate for at most 200
+   steps first; when half of the pilot or more already falls back, every orbit of the batch is
+   integrated with the FP64 Multipole directly (each counted and marked as a fallback) rather
+   than twice.  enable = 0 turns the pilot off (default on); last_pilot reports the last batch's
+   pilot size and fallbacks on this thread (0, 0 without a pilot). */
+int agx_set_surrogate_pilot(int enable);
+int agx_surrogate_last_pilot(long* out_pilot_orbits, long* out_pilot_fallbacks);
 /* the training GEMM on host arrays: C[M][N] = sum_k A(m,k) B(k,n), bf16 bit patterns,
    A row-major [M][K] if a_kcontig else [K][M], B [N][K] if b_kcontig else [K][N] */
 int agx_debug_gemm(int M, int N, int K, int splits, int a_kcontig, int b_kcontig,

@@ -221,3 +221,23 @@ def test_surrogate_orbits_loose_threshold(pot, trained):
                 f"dE/E median {np.median(mde):.2e} max {mde.max():.2e}")
         assert np.median(mde) < 1e-2
     print(msg)
+
+
+def test_pilot_routes_a_batch_the_surrogate_cannot_hold(pot, trained):
+    """8192 orbits at max_rel_de=1e-6: the pilot (every 64th orbit, 200 steps) finds that the
+    surrogate cannot keep the energy error, so every orbit is integrated with the FP64
+    Multipole directly - marked as a fallback and equal to the plain FP64 integration"""
+    import agama_amd
+    from agama_amd import workloads
+    from oracle.pyoracle import OraclePotential
+    orc = OraclePotential(COEF)
+    ic, T = workloads.triaxial_dehnen_orbits(8192, lambda p: orc.eval(p)[0], pot.coefs(), seed=33, periods=2)
+    info_s, info_p = {}, {}
+    agama_amd.orbit(potential=trained, ic=ic, time=T, accuracy=1e-8, info=info_s, max_rel_de=1e-6)
+    agama_amd.orbit(potential=pot, ic=ic, time=T, accuracy=1e-8, info=info_p)
+    np_, pf = info_s["pilot"]
+    print(f"pilot: {pf} of {np_} orbits fell back; batch fallbacks {info_s['n_fallback']}")
+    assert np_ == 1024 and 2 * pf >= np_
+    assert info_s["n_fallback"] == 8192 and np.all(info_s["status"] == 8)
+    assert np.array_equal(info_s["final"], info_p["final"])
+    assert np.array_equal(info_s["nsteps"], info_p["nsteps"])
