@@ -476,7 +476,9 @@ class Target:
     """A target plug-in collecting data along orbits - reference: agama.Target
     (src/interface_python.cpp:7005-7250).  Provided: type='DensitySphHarm' with gridr (radial
     nodes), lmax, mmax (even orders of the spherical-harmonic expansion): the density
-    discretisation of TargetDensitySphHarm (src/galaxymodel_densitygrid.cpp:356-411).
+    discretisation of TargetDensitySphHarm (src/galaxymodel_densitygrid.cpp:356-411); and
+    type='DensityCylindricalTopHat' / 'DensityCylindricalLinear' with gridr, gridz (nodes in
+    R and |z|) and mmax (TargetDensityCylindrical<0/1>, :486-545).
     len(target) is the number of values recorded per orbit."""
 
     def __init__(self, **kwargs):
@@ -484,22 +486,40 @@ class Target:
         type_str = str(kwargs.pop("type", "") or "")
         if not type_str:
             raise TypeError("Must provide a type='...' argument")
-        if type_str.lower() != "densitysphharm":
-            raise AgamaAmdError(f"Target type {type_str} is not provided by this version (DensitySphHarm is)")
+        kind = type_str.lower()
+        names = {"densitysphharm": "DensitySphHarm", "densitycylindricaltophat": "DensityCylindricalTopHat",
+                 "densitycylindricallinear": "DensityCylindricalLinear"}
+        if kind not in names:
+            raise AgamaAmdError(f"Target type {type_str} is not provided by this version "
+                                "(DensitySphHarm, DensityCylindricalTopHat, DensityCylindricalLinear are)")
         if "gridr" not in kwargs:
             raise TypeError("Argument 'gridr' is required")
         gridr = np.ascontiguousarray(kwargs.pop("gridr"), dtype=np.float64).reshape(-1)
         if len(gridr) < 2:
             raise ValueError("gridr must be an array with >=2 elements")
-        lmax = int(kwargs.pop("lmax", 0))
-        mmax = int(kwargs.pop("mmax", 0))
-        if kwargs:
-            raise TypeError("Unknown arguments: " + ", ".join(kwargs))
-        h = lib.agx_target_density_sphharm(lmax, mmax, ptr(gridr), len(gridr))
+        self.type, self.gridr = names[kind], gridr
+        if kind == "densitysphharm":
+            lmax = int(kwargs.pop("lmax", 0))
+            mmax = int(kwargs.pop("mmax", 0))
+            self.lmax, self.mmax = lmax, mmax
+            if kwargs:
+                raise TypeError("Unknown arguments: " + ", ".join(kwargs))
+            h = lib.agx_target_density_sphharm(lmax, mmax, ptr(gridr), len(gridr))
+        else:   # interface_python.cpp:7209-7221
+            if "gridz" not in kwargs:
+                raise TypeError("Argument 'gridz' is required")
+            gridz = np.ascontiguousarray(kwargs.pop("gridz"), dtype=np.float64).reshape(-1)
+            if len(gridz) < 2:
+                raise ValueError("gridz must be an array with >=2 elements")
+            mmax = int(kwargs.pop("mmax", 0))
+            self.gridz, self.mmax = gridz, mmax
+            if kwargs:
+                raise TypeError("Unknown arguments: " + ", ".join(kwargs))
+            h = lib.agx_target_density_cylindrical(0 if kind.endswith("tophat") else 1, mmax, ptr(gridr), len(gridr),
+                                                   ptr(gridz), len(gridz))
         if not h:
             raise ValueError(_native.last_error())
         self._h = h
-        self.type, self.lmax, self.mmax, self.gridr = "DensitySphHarm", lmax, mmax, gridr
 
     def __len__(self):
         return lib.agx_target_num_values(self._h)

@@ -1258,6 +1258,7 @@ void agx_orbit_record_free(agx_orbit_record* R)
 struct agx_target {
     int type = 0, lmax = 0, mmax = 0;
     std::vector<double> gridr;        // radial nodes without the origin
+    std::vector<double> gridz;        // vertical nodes without z = 0 (cylindrical targets)
     int numValues = 0;
 };
 
@@ -1278,6 +1279,29 @@ agx_target* agx_target_density_sphharm(int lmax, int mmax, const double* gridr, 
     if(!ok) { fail(AGX_EINVAL, "TargetDensitySphHarm: invalid grid parameters"); return nullptr; }
     const int angular = (lmax/2+1) * (mmax/2+1) - mmax/2 * (mmax/2+1) / 2;
     T->numValues = angular * (int)T->gridr.size() + 1;
+    return T.release();
+}
+
+agx_target* agx_target_density_cylindrical(int degree, int mmax, const double* gridR, int ngridR,
+    const double* gridz, int ngridz)
+{
+    g_err.clear();
+    // TargetDensityCylindrical<N> ctor (galaxymodel_densitygrid.cpp:486-503): grids starting at 0
+    // lose that node
+    if((degree != 0 && degree != 1) || !gridR || !gridz || ngridR < 1 || ngridz < 1) {
+        fail(AGX_EINVAL, "TargetDensityCylindrical: invalid grid parameters");
+        return nullptr;
+    }
+    std::unique_ptr<agx_target> T(new agx_target);
+    T->type = degree == 0 ? AGX_TARGET_DENSITY_CYL_TOPHAT : AGX_TARGET_DENSITY_CYL_LINEAR;
+    T->mmax = mmax;
+    T->gridr.assign(gridR[0] > 0 ? gridR : gridR + 1, gridR + ngridR);
+    T->gridz.assign(gridz[0] > 0 ? gridz : gridz + 1, gridz + ngridz);
+    bool ok = mmax >= 0 && T->gridr.size() >= 1 && T->gridz.size() >= 1 && T->gridr[0] > 0 && T->gridz[0] > 0;
+    for(size_t i = 1; i < T->gridr.size(); i++) ok &= T->gridr[i] > T->gridr[i-1];
+    for(size_t i = 1; i < T->gridz.size(); i++) ok &= T->gridz[i] > T->gridz[i-1];
+    if(!ok) { fail(AGX_EINVAL, "TargetDensityCylindrical: invalid grid parameters"); return nullptr; }
+    T->numValues = ((int)T->gridz.size() + degree) * ((int)T->gridr.size() * (mmax/2+1) + degree);
     return T.release();
 }
 
@@ -1305,13 +1329,18 @@ int agx_orbit_target_batch(const agx_potential* pot, long n, const double* ic, c
         std::vector<std::unique_ptr<DevBuf<double> > > grids, cubes;
         for(int t = 0; t < ntargets; t++) {
             const agx_target& T = *targets[t];
-            grids.emplace_back(new DevBuf<double>(T.gridr.size()));
+            grids.emplace_back(new DevBuf<double>(T.gridr.size() + T.gridz.size()));
             cubes.emplace_back(new DevBuf<double>((size_t)n * (T.numValues + 1)));
             HIPCHECK(hipMemcpy(grids.back()->p, T.gridr.data(), T.gridr.size() * 8, hipMemcpyHostToDevice));
+            if(!T.gridz.empty())
+                HIPCHECK(hipMemcpy(grids.back()->p + T.gridr.size(), T.gridz.data(), T.gridz.size() * 8,
+                    hipMemcpyHostToDevice));
             HIPCHECK(hipMemset(cubes.back()->p, 0, (size_t)n * (T.numValues + 1) * 8));
             AgxTargetDev& D = TL.t[t];
             D.type = T.type; D.lmax = T.lmax; D.mmax = T.mmax; D.nr = (int)T.gridr.size();
-            D.numValues = T.numValues; D.gridr = grids.back()->p; D.cube = cubes.back()->p;
+            D.nz = (int)T.gridz.size();
+            D.numValues = T.numValues; D.gridr = grids.back()->p; D.gridz = grids.back()->p + T.gridr.size();
+            D.cube = cubes.back()->p;
         }
         DevBuf<AgxTargetList> dTL(1);
         HIPCHECK(hipMemcpy(dTL.p, &TL, sizeof(TL), hipMemcpyHostToDevice));

@@ -1,22 +1,25 @@
 // Target plug-ins collected during orbit integration on the device: RuntimeFncTarget
 // (src/galaxymodel_target.h:95-155: ten dense-output samples per accepted step, each weighted
-// by its share of the step) with the density discretisation TargetDensitySphHarm
+// by its share of the step) with the density discretisations TargetDensitySphHarm
 // (src/galaxymodel_densitygrid.cpp:356-411: linear B-splines in r times even spherical
-// harmonics).  One lane accumulates its orbit's datacube in FP64 in global memory (its own
+// harmonics) and TargetDensityCylindrical<0/1> (:486-545, 92-118: top-hat cells or bilinear
+// B-splines in (R, |z|) times even azimuthal Fourier terms).  One lane accumulates its orbit's datacube in FP64 in global memory (its own
 // row: no atomics); the host normalises by the integration time at the end.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "agx_device.h"
 
-enum : int { AGX_TARGET_DENSITY_SPHHARM = 1 };
+enum : int { AGX_TARGET_DENSITY_SPHHARM = 1, AGX_TARGET_DENSITY_CYL_TOPHAT = 2, AGX_TARGET_DENSITY_CYL_LINEAR = 3 };
 enum : int { AGX_MAX_TARGETS = 4 };
 
 struct AgxTargetDev {
     int type;
     int lmax, mmax;          // even orders of the angular expansion
     int nr;                  // radial grid nodes (the origin implied)
+    int nz;                  // vertical grid nodes (cylindrical targets; z = 0 implied)
     int numValues;           // datacube length; the row holds one more value: the accumulated time
-    const double* gridr;     // nr nodes, increasing, > 0
+    const double* gridr;     // nr nodes, increasing, > 0 (spherical radius, or cylindrical R)
+    const double* gridz;     // nz nodes, increasing, > 0 (cylindrical targets)
     double* cube;            // n x (numValues + 1) per orbit
 };
 
@@ -87,9 +90,55 @@ __device__ __noinline__ void sphHarmTargetAdd(const AgxTargetDev& T, double* val
     }
 }
 
+// TargetDensityCylindrical<N>::addPoint (galaxymodel_densitygrid.cpp:505-545) with the
+// corner indexing of getCornerIndicesCylindrical (:92-118); N = 0 top-hat cells, 1 bilinear
+__device__ __noinline__ void cylTargetAdd(const AgxTargetDev& T, double* values, double x, double y, double z,
+    double mult)
+{
+    const int N = T.type == AGX_TARGET_DENSITY_CYL_LINEAR ? 1 : 0;
+    const double zz = fabs(z), R = sqrt(x * x + y * y), phi = atan2A(y, x);
+    const int nR = T.nr, nz = T.nz;
+    const int indR = binSearchD(R, T.gridr, nR) + 1, indz = binSearchD(zz, T.gridz, nz) + 1;
+    if(indR >= nR || indz >= nz || mult == 0) return;
+    const double prevR = indR > 0 ? T.gridr[indR - 1] : 0., prevz = indz > 0 ? T.gridz[indz - 1] : 0.;
+    const double offR = (R - prevR) / (T.gridr[indR] - prevR), offz = (zz - prevz) / (T.gridz[indz] - prevz);
+    // cos(k phi) by the recurrence of trigMultiAngle (math_sphharm.cpp:134-152), two steps per even m
+    double sphi, cphi, sh, ch;
+    sincosA(phi, sphi, cphi);
+    sincosA(phi / 2, sh, ch);
+    const double alpha = 2 * sh * sh, beta = sphi;
+    double c1 = 1, s1 = 0;
+    for(int m = 0; m <= T.mmax; m += 2) {
+        if(m > 0)
+            for(int k = 0; k < 2; k++) {
+                const double cn = c1 - (alpha * c1 + beta * s1), sn = s1 - (alpha * s1 - beta * c1);
+                c1 = cn; s1 = sn;
+            }
+        const double val = mult * (m == 0 ? 1. : 2 * c1);
+        if(N == 0) {
+            values[indR + (indz + m / 2 * nz) * nR] += val;
+            continue;
+        }
+        int offsetm = 0, gR = nR, iR = indR;
+        if(m > 0) {   // no basis functions at R = 0 for m > 0
+            offsetm = (1 + m / 2 * nR) * (nz + 1);
+            gR -= 1;
+            iR -= 1;
+        }
+        const int ll = offsetm + iR + indz * (gR + 1), ul = ll + 1, lu = ll + gR + 1, uu = lu + 1;
+        values[ul] += val * offR * (1 - offz);
+        values[uu] += val * offR * offz;
+        if(m == 0 || indR > 0) {
+            values[ll] += val * (1 - offR) * (1 - offz);
+            values[lu] += val * (1 - offR) * offz;
+        }
+    }
+}
+
 __device__ __forceinline__ void targetAdd(const AgxTargetDev& T, double* row, double x, double y, double z, double mult)
 {
     if(T.type == AGX_TARGET_DENSITY_SPHHARM) sphHarmTargetAdd(T, row, x, y, z, mult);
+    else cylTargetAdd(T, row, x, y, z, mult);
 }
 
 }  // namespace agxd

@@ -244,6 +244,13 @@ typedef struct agx_target agx_target;
    always a node of the l=0 term) times the spherical harmonics of even l <= lmax and even
    0 <= m <= mmax.  NULL on invalid parameters. */
 agx_target* agx_target_density_sphharm(int lmax, int mmax, const double* gridr, int ngridr);
+/* TargetDensityCylindrical<degree> (src/galaxymodel_densitygrid.cpp:486-545): degree 0
+   (DensityCylindricalTopHat) constant cells, degree 1 (DensityCylindricalLinear) bilinear
+   B-splines on the (R, |z|) grid (grids starting at 0 lose that node; R = 0 and z = 0 are
+   implied), times the azimuthal Fourier terms cos(m phi) of even 0 <= m <= mmax.  NULL on
+   invalid parameters. */
+agx_target* agx_target_density_cylindrical(int degree, int mmax, const double* gridR, int ngridR,
+    const double* gridz, int ngridz);
 /* number of values a target records per orbit (numCoefs), -1 for NULL */
 int agx_target_num_values(const agx_target* t);
 void agx_target_destroy(agx_target* t);

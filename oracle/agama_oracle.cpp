@@ -2470,14 +2470,21 @@ inline double energyOf(const double* xv, double Epot, double Omega)
     return E;
 }
 
-// TargetDensitySphHarm (galaxymodel_densitygrid.cpp:356-411) with the datacube of a
-// RuntimeFncTarget (galaxymodel_target.h:95-155): ten dense-output points per step
-struct OrcTargetSphHarm {
+// density targets (galaxymodel_densitygrid.h) with the datacube of a RuntimeFncTarget
+// (galaxymodel_target.h:95-155): ten dense-output points per step
+struct OrcTarget {
+    virtual ~OrcTarget() {}
+    virtual int numValues() const = 0;
+    virtual void addPoint(const double point[3], double mult, double values[]) const = 0;
+};
+
+// TargetDensitySphHarm (galaxymodel_densitygrid.cpp:356-411)
+struct OrcTargetSphHarm : OrcTarget {
     int lmax = 0, mmax = 0;
     std::vector<double> gridr;   // without a node at the origin
-    int numValues() const
+    int numValues() const override
     { return ((lmax/2+1) * (mmax/2+1) - mmax/2 * (mmax/2+1) / 2) * (int)gridr.size() + 1; }
-    void addPoint(const double point[3], double mult, double values[]) const
+    void addPoint(const double point[3], double mult, double values[]) const override
     {
         const double R = sqrt(pow_2(point[0]) + pow_2(point[1])), z = point[2];
         const double phi = atan2_r(point[1], point[0]);
@@ -2500,8 +2507,46 @@ struct OrcTargetSphHarm {
         }
     }
 };
+// TargetDensityCylindrical<N> (galaxymodel_densitygrid.cpp:486-545, corner indices :92-118)
+struct OrcTargetCyl : OrcTarget {
+    int N = 0, mmax = 0;
+    std::vector<double> gridR, gridz;   // without nodes at 0
+    int numValues() const override
+    { return ((int)gridz.size() + N) * ((int)gridR.size() * (mmax/2+1) + N); }
+    void addPoint(const double point[3], double mult, double values[]) const override
+    {
+        const double x = point[0], y = point[1], z = fabs(point[2]);
+        const double R = sqrt(pow_2(x) + pow_2(y)), phi = atan2_r(y, x);
+        const int gridRsize = gridR.size(), gridzsize = gridz.size();
+        const int indR = (int)binSearch(R, &gridR[0], gridRsize) + 1;
+        const int indz = (int)binSearch(z, &gridz[0], gridzsize) + 1;
+        if(indR >= gridRsize || indz >= gridzsize || mult == 0) return;
+        const double prevR = indR > 0 ? gridR[indR-1] : 0., prevz = indz > 0 ? gridz[indz-1] : 0.;
+        const double offR = (R - prevR) / (gridR[indR] - prevR), offz = (z - prevz) / (gridz[indz] - prevz);
+        std::vector<double> trig(mmax + 1);
+        trigMultiAngle(phi, mmax, false, &trig[0]);
+        for(int m = 0; m <= mmax; m += 2) {
+            const double val = mult * (m == 0 ? 1. : 2*trig[m-1]);
+            if(N == 0) {
+                values[indR + (indz + m/2 * gridzsize) * gridRsize] += val;
+                continue;
+            }
+            int offsetm = 0, gRs = gridRsize, iR = indR;
+            if(m > 0) { offsetm = (1 + m/2 * gridRsize) * (gridzsize + 1); gRs -= 1; iR -= 1; }
+            const int indll = offsetm + iR + indz * (gRs + 1), indul = indll + 1, indlu = indll + gRs + 1,
+                induu = indlu + 1;
+            values[indul] += val * offR * (1-offz);
+            values[induu] += val * offR * offz;
+            if(m == 0 || indR > 0) {
+                values[indll] += val * (1-offR) * (1-offz);
+                values[indlu] += val * (1-offR) * offz;
+            }
+        }
+    }
+};
+
 struct TargetSink {
-    const OrcTargetSphHarm* target = nullptr;
+    const OrcTarget* target = nullptr;
     std::vector<double> cube;
     double time = 0;
 };
@@ -3116,14 +3161,34 @@ int orc_orbit_steps(const orc_pot* p, long n, const double* ic, const double* ts
     return fail;
 }
 
+static int runTarget(const orc_pot* p, long n, const double* ic, const double* tstart, const double* ttotal,
+    double accuracy, double Omega, long maxNumSteps, const OrcTarget& T, float* out, int nthreads);
+
 int orc_orbit_target_sphharm(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int lmax, int mmax,
     const double* gridr, int ngridr, float* out, int nthreads)
 {
-    g_err.clear();
     OrcTargetSphHarm T;
     T.lmax = lmax; T.mmax = mmax;
     T.gridr.assign(ngridr > 0 && gridr[0] > 0 ? gridr : gridr + 1, gridr + ngridr);
+    return runTarget(p, n, ic, tstart, ttotal, accuracy, Omega, maxNumSteps, T, out, nthreads);
+}
+
+int orc_orbit_target_cyl(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree, int mmax,
+    const double* gridR, int ngridR, const double* gridz, int ngridz, float* out, int nthreads)
+{
+    OrcTargetCyl T;
+    T.N = degree; T.mmax = mmax;
+    T.gridR.assign(ngridR > 0 && gridR[0] > 0 ? gridR : gridR + 1, gridR + ngridR);
+    T.gridz.assign(ngridz > 0 && gridz[0] > 0 ? gridz : gridz + 1, gridz + ngridz);
+    return runTarget(p, n, ic, tstart, ttotal, accuracy, Omega, maxNumSteps, T, out, nthreads);
+}
+
+static int runTarget(const orc_pot* p, long n, const double* ic, const double* tstart, const double* ttotal,
+    double accuracy, double Omega, long maxNumSteps, const OrcTarget& T, float* out, int nthreads)
+{
+    g_err.clear();
     const int nv = T.numValues();
     int fail = 0;
 #ifdef _OPENMP

@@ -101,6 +101,10 @@ int orc_orbit_steps(const orc_pot* p, long n, const double* ic, const double* ts
 int orc_orbit_target_sphharm(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int lmax, int mmax,
     const double* gridr, int ngridr, float* out, int nthreads);
+/* the same with TargetDensityCylindrical<degree> (DensityCylindricalTopHat / Linear) */
+int orc_orbit_target_cyl(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree, int mmax,
+    const double* gridR, int ngridR, const double* gridz, int ngridz, float* out, int nthreads);
 /* agama.Orbit on one recorded series (createOrbitObject + Orbit.__call__): out[m*6] rotating
    frame, NaN outside the span; out_spl[m*9] the inertial-frame splines x,y,z, their first and
    second derivatives (linear continuation outside) */

@@ -58,6 +58,9 @@ def _load(name):
         "orc_orbit_target_sphharm": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, ctypes.c_double,
                                                     ctypes.c_double, ctypes.c_long, ctypes.c_int, ctypes.c_int, _vp,
                                                     ctypes.c_int, _vp, ctypes.c_int]),
+        "orc_orbit_target_cyl": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_long, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp,
+                                                ctypes.c_int, _vp, ctypes.c_int]),
         "orc_ode_gl4": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                        _vp, _vp]),
         "orc_ode_dop853_fixed": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
@@ -314,6 +317,34 @@ def _orbits_target_sphharm(self, ic, ttotal, gridr, lmax, mmax, tstart=None, acc
 
 
 OraclePotential.orbits_target_sphharm = _orbits_target_sphharm
+
+
+def cyl_target_size(degree, mmax, gridR, gridz):
+    nR = len(gridR) - (1 if gridR[0] == 0 else 0)
+    nz = len(gridz) - (1 if gridz[0] == 0 else 0)
+    return (nz + degree) * (nR * (mmax // 2 + 1) + degree)
+
+
+def _orbits_target_cyl(self, ic, ttotal, gridR, gridz, mmax, degree, tstart=None, accuracy=1e-8, Omega=0.0,
+                       maxNumSteps=int(1e8), nthreads=0):
+    """orbit(targets=[Target(type='DensityCylindricalTopHat' (degree 0) / 'DensityCylindricalLinear'
+    (degree 1), ...)]) as the oracle restates it: (n, numCoefs) float32"""
+    ic = np.ascontiguousarray(ic, dtype=np.float64).reshape(-1, 6)
+    n = len(ic)
+    tt = np.ascontiguousarray(np.broadcast_to(np.asarray(ttotal, dtype=np.float64), (n,)))
+    ts = None if tstart is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(tstart, dtype=np.float64), (n,)))
+    gR = np.ascontiguousarray(gridR, dtype=np.float64)
+    gz = np.ascontiguousarray(gridz, dtype=np.float64)
+    out = np.empty((n, cyl_target_size(degree, mmax, gR, gz)), dtype=np.float32)
+    rc = self.lib.orc_orbit_target_cyl(self.h, n, _p(ic), _p(ts), _p(tt), accuracy, Omega, maxNumSteps, int(degree),
+                                       int(mmax), _p(gR), len(gR), _p(gz), len(gz), _p(out), nthreads)
+    if rc:
+        raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
+    return out
+
+
+OraclePotential.orbits_target_cyl = _orbits_target_cyl
 
 
 def orbit_interp(t, traj, Omega, tq):

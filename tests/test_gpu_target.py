@@ -66,3 +66,24 @@ def test_target_outputs_and_edge_cases(setup):
     assert one.shape == (len(t1),) and np.array_equal(one, a[0])
     with pytest.raises(agama_amd.AgamaAmdError):
         agama_amd.orbit(potential=gpu, ic=ic, time=T, targets=t1, method="dprkn8")
+
+
+@pytest.mark.parametrize("degree", [0, 1])
+def test_cylindrical_target_matches_oracle(setup, degree):
+    import agama_amd
+    from agama_amd import workloads
+    gpu, orc = setup
+    ic, T = workloads.triaxial_dehnen_orbits(64, lambda p: orc.eval(p)[0], gpu.coefs(), seed=63, periods=5)
+    gridR = np.concatenate([[0.0], np.logspace(-2, 2, 12)])
+    gridz = np.concatenate([[0.0], np.logspace(-2, 2, 10)])
+    kind = "DensityCylindricalTopHat" if degree == 0 else "DensityCylindricalLinear"
+    print(f"case: {kind} target, 64 orbits x 5 periods", flush=True)
+    tgt = agama_amd.Target(type=kind, gridr=gridR, gridz=gridz, mmax=4)
+    out = agama_amd.orbit(potential=gpu, ic=ic, time=T, targets=tgt, Omega=0.2)
+    ref = orc.orbits_target_cyl(ic, T, gridR, gridz, 4, degree, Omega=0.2)
+    assert out.shape == ref.shape == (64, len(tgt))
+    empty = np.all(ref == 0, axis=1)           # orbits that never enter the grid
+    assert np.all(out[empty] == 0)
+    rel = np.max(np.abs(out - ref), axis=1)[~empty] / np.max(np.abs(ref), axis=1)[~empty]
+    print(f"per-orbit max rel diff: median {np.median(rel):.2e}, 90% {np.quantile(rel, 0.9):.2e}, max {rel.max():.2e}")
+    assert np.median(rel) <= 1e-6 and np.quantile(rel, 0.9) <= 1e-5

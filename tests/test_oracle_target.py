@@ -79,3 +79,54 @@ def test_radial_orbit_on_the_x_axis_fixes_every_angular_term():
             assert sel.sum() >= 5
             assert np.max(np.abs(vals[sel] / base[sel] - ratio)) < 2e-5 * max(1, abs(ratio)), (l, m)
             blk += 1
+
+
+@pytest.mark.parametrize("degree", [0, 1])
+def test_cylindrical_targets_closed_forms(degree):
+    """TargetDensityCylindrical<0/1> (galaxymodel_densitygrid.cpp:486-545): a circular orbit in
+    the equatorial plane puts its whole weight at z = 0 and R = r0 (one top-hat cell; the two
+    bilinear R-nodes around r0 at the z = 0 row, linearly), with the m > 0 terms averaging to
+    zero; a radial orbit on the long axis has cos(2 phi) = 1, so every m = 2 value is twice the
+    m = 0 value of the same cell or R > 0 node; the m = 0 basis partitions unity."""
+    from agama_amd import workloads
+    gridR = np.array([0.2, 0.5, 1.0, 2.0, 5.0])
+    gridz = np.array([0.1, 0.4, 1.0, 3.0])
+    nR, nz, mmax = 5, 4, 2
+    orc = OraclePotential(os.path.join(GOLDEN, "disk_bulge_halo_lmax12.coef"))
+    r0 = 0.7
+    _, g = orc.eval(np.array([[r0, 0.0, 0.0]]))
+    vc = math.sqrt(r0 * abs(g[0, 0]))
+    period = 2 * math.pi * r0 / vc
+    out = orc.orbits_target_cyl(np.array([[r0, 0, 0, 0, vc, 0]]), 5 * period, gridR, gridz, mmax, degree,
+                                accuracy=1e-10)[0]
+    assert len(out) == (nz + degree) * (nR * 2 + degree)
+    expected = np.zeros(len(out))
+    offR = (r0 - 0.5) / (1.0 - 0.5)
+    if degree == 0:
+        expected[2] = 1.0                      # indR = 2, indz = 0, m = 0
+    else:
+        expected[2] = 1 - offR                 # node R = 0.5 (index 2 with R = 0 at 0), z-row 0
+        expected[3] = offR
+    # the m = 2 terms carry 2 cos(2 phi): their time average over whole turns vanishes to the
+    # sampling of ten points per step (a few 1e-6)
+    assert np.max(np.abs(out - expected)) < 1e-5, (out, expected)
+    # radial orbit on the long axis of the triaxial model
+    coef = os.path.join(GOLDEN, "dehnen_triax_lmax8.coef")
+    tri = OraclePotential(coef)
+    rad = tri.orbits_target_cyl(np.array([[3.0, 0, 0, 0, 0, 0]]), 40.0, gridR, gridz, mmax, degree)[0]
+    if degree == 0:
+        m0, m2 = rad[:nR * nz], rad[nR * nz:]
+        assert np.allclose(m2, 2 * m0, rtol=2e-6, atol=1e-9)
+    else:
+        m0 = rad[:(nR + 1) * (nz + 1)].reshape(nz + 1, nR + 1)[:, 1:]      # R > 0 nodes
+        m2 = rad[(nR + 1) * (nz + 1):].reshape(nz + 1, nR)
+        assert np.allclose(m2, 2 * m0, rtol=2e-6, atol=1e-9)
+    # partition of unity of the m = 0 terms for orbits inside the grid
+    ic, T = workloads.triaxial_dehnen_orbits(12, lambda p: tri.eval(p)[0], workloads.read_coef_file(coef),
+                                             seed=52, periods=4)
+    r = np.linalg.norm(ic[:, :3], axis=1)
+    sel = r < 0.5
+    big = np.concatenate([[0.0], np.logspace(-3, 3, 30)])
+    res = tri.orbits_target_cyl(ic[sel], T[sel], big, big, mmax, degree)
+    m0 = res[:, :(30 + degree) * (30 + degree)]
+    assert np.max(np.abs(m0.sum(axis=1) - 1)) < 2e-6
