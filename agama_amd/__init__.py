@@ -478,7 +478,9 @@ class Target:
     nodes), lmax, mmax (even orders of the spherical-harmonic expansion): the density
     discretisation of TargetDensitySphHarm (src/galaxymodel_densitygrid.cpp:356-411); and
     type='DensityCylindricalTopHat' / 'DensityCylindricalLinear' with gridr, gridz (nodes in
-    R and |z|) and mmax (TargetDensityCylindrical<0/1>, :486-545).
+    R and |z|) and mmax (TargetDensityCylindrical<0/1>, :486-545); and type='KinemShell' with
+    gridr (nodes in r, from r = 0) and degree 0..3: v_r^2 and v_t^2 on B-splines of that degree
+    (TargetKinemShell<N>, src/galaxymodel_losvd.cpp:568-576; interface_python.cpp:7228-7243).
     len(target) is the number of values recorded per orbit."""
 
     def __init__(self, **kwargs):
@@ -488,10 +490,11 @@ class Target:
             raise TypeError("Must provide a type='...' argument")
         kind = type_str.lower()
         names = {"densitysphharm": "DensitySphHarm", "densitycylindricaltophat": "DensityCylindricalTopHat",
-                 "densitycylindricallinear": "DensityCylindricalLinear"}
+                 "densitycylindricallinear": "DensityCylindricalLinear", "kinemshell": "KinemShell"}
         if kind not in names:
             raise AgamaAmdError(f"Target type {type_str} is not provided by this version "
-                                "(DensitySphHarm, DensityCylindricalTopHat, DensityCylindricalLinear are)")
+                                "(DensitySphHarm, DensityCylindricalTopHat, DensityCylindricalLinear, "
+                                "KinemShell are)")
         if "gridr" not in kwargs:
             raise TypeError("Argument 'gridr' is required")
         gridr = np.ascontiguousarray(kwargs.pop("gridr"), dtype=np.float64).reshape(-1)
@@ -505,6 +508,14 @@ class Target:
             if kwargs:
                 raise TypeError("Unknown arguments: " + ", ".join(kwargs))
             h = lib.agx_target_density_sphharm(lmax, mmax, ptr(gridr), len(gridr))
+        elif kind == "kinemshell":
+            degree = int(kwargs.pop("degree", -1))
+            if not 0 <= degree <= 3:
+                raise ValueError("KinemShell: degree of interpolation should be between 0 and 3")
+            self.degree = degree
+            if kwargs:
+                raise TypeError("Unknown arguments: " + ", ".join(kwargs))
+            h = lib.agx_target_kinem_shell(degree, ptr(gridr), len(gridr))
         else:   # interface_python.cpp:7209-7221
             if "gridz" not in kwargs:
                 raise TypeError("Argument 'gridz' is required")

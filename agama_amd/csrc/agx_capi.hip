@@ -1305,6 +1305,27 @@ agx_target* agx_target_density_cylindrical(int degree, int mmax, const double* g
     return T.release();
 }
 
+agx_target* agx_target_kinem_shell(int degree, const double* gridr, int ngridr)
+{
+    g_err.clear();
+    // TargetKinemShell<N> (galaxymodel_losvd.h:121-136): B-splines of degree N on gridr as given
+    if(degree < 0 || degree > 3 || !gridr || ngridr < 2) {
+        fail(AGX_EINVAL, "KinemShell: degree must be 0..3 and gridr must have >= 2 nodes");
+        return nullptr;
+    }
+    std::unique_ptr<agx_target> T(new agx_target);
+    T->type = AGX_TARGET_KINEM_SHELL;
+    T->lmax = degree;
+    T->gridr.assign(gridr, gridr + ngridr);
+    for(int i = 0; i < ngridr; i++)
+        if(!(gridr[i] > -INFINITY && gridr[i] < INFINITY) || (i > 0 && !(gridr[i] > gridr[i-1]))) {
+            fail(AGX_EINVAL, "BsplineInterpolator1d: x coordinates must be finite and monotonically increasing");
+            return nullptr;
+        }
+    T->numValues = 2 * (ngridr + degree - 1);
+    return T.release();
+}
+
 int agx_target_num_values(const agx_target* t) { return t ? t->numValues : -1; }
 
 void agx_target_destroy(agx_target* t) { delete t; }
@@ -1326,6 +1347,7 @@ int agx_orbit_target_batch(const agx_potential* pot, long n, const double* ic, c
         HIPCHECK(hipSetDevice(pot->device));
         AgxTargetList TL;
         TL.count = ntargets;
+        for(int t = 0; t < ntargets; t++) TL.needVelocity |= targets[t]->type == AGX_TARGET_KINEM_SHELL;
         std::vector<std::unique_ptr<DevBuf<double> > > grids, cubes;
         for(int t = 0; t < ntargets; t++) {
             const agx_target& T = *targets[t];

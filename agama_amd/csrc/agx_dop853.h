@@ -266,17 +266,16 @@ __device__ __forceinline__ void targetStep(const AgxOrbitArgs& A, Lane& L)
     const double timestep = L.h, substep = timestep / NUM_SAMPLES_PER_STEP;
     for(int s = 0; s < NUM_SAMPLES_PER_STEP; s++) {
         const double off = substep * (s + 0.5);
-        const double d0 = denseOut(L, 0, off), d1 = denseOut(L, 1, off), d2 = denseOut(L, 2, off);
-        double x = d0, y = d1;
-        if(A.Omega != 0) {
-            double ca = 1, sa = 0;
-            sincosA(A.Omega * (off + L.tb), sa, ca);
-            x = d0 * ca + d1 * sa;
-            y = d1 * ca - d0 * sa;
-        }
+        double d[6], p[6];
+        for(int c = 0; c < 3; c++) d[c] = denseOut(L, c, off);
+        if(TL.needVelocity)
+            for(int c = 3; c < 6; c++) d[c] = denseOut(L, c, off);
+        else
+            d[3] = d[4] = d[5] = 0;
+        toRotating(A.Omega, A.Omega * (off + L.tb), d, p);
         for(int t = 0; t < TL.count; t++) {
             const AgxTargetDev& T = TL.t[t];
-            targetAdd(T, T.cube + (long)L.oi * (T.numValues + 1), x, y, d2, substep);
+            targetAdd(T, T.cube + (long)L.oi * (T.numValues + 1), p, substep);
         }
     }
     for(int t = 0; t < TL.count; t++) {

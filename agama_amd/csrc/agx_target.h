@@ -2,19 +2,22 @@
 // (src/galaxymodel_target.h:95-155: ten dense-output samples per accepted step, each weighted
 // by its share of the step) with the density discretisations TargetDensitySphHarm
 // (src/galaxymodel_densitygrid.cpp:356-411: linear B-splines in r times even spherical
-// harmonics) and TargetDensityCylindrical<0/1> (:486-545, 92-118: top-hat cells or bilinear
-// B-splines in (R, |z|) times even azimuthal Fourier terms).  One lane accumulates its orbit's datacube in FP64 in global memory (its own
+// harmonics), TargetDensityCylindrical<0/1> (:486-545, 92-118: top-hat cells or bilinear
+// B-splines in (R, |z|) times even azimuthal Fourier terms) and the kinematic
+// TargetKinemShell<N> (src/galaxymodel_losvd.cpp:568-576: v_r^2 and v_t^2 on B-splines of degree
+// N in r, src/math_spline.cpp:35-82, 1877-1886).  One lane accumulates its orbit's datacube in FP64 in global memory (its own
 // row: no atomics); the host normalises by the integration time at the end.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "agx_device.h"
 
-enum : int { AGX_TARGET_DENSITY_SPHHARM = 1, AGX_TARGET_DENSITY_CYL_TOPHAT = 2, AGX_TARGET_DENSITY_CYL_LINEAR = 3 };
+enum : int { AGX_TARGET_DENSITY_SPHHARM = 1, AGX_TARGET_DENSITY_CYL_TOPHAT = 2, AGX_TARGET_DENSITY_CYL_LINEAR = 3,
+    AGX_TARGET_KINEM_SHELL = 4 };
 enum : int { AGX_MAX_TARGETS = 4 };
 
 struct AgxTargetDev {
     int type;
-    int lmax, mmax;          // even orders of the angular expansion
+    int lmax, mmax;          // even orders of the angular expansion (KinemShell: lmax = B-spline degree)
     int nr;                  // radial grid nodes (the origin implied)
     int nz;                  // vertical grid nodes (cylindrical targets; z = 0 implied)
     int numValues;           // datacube length; the row holds one more value: the accumulated time
@@ -25,6 +28,7 @@ struct AgxTargetDev {
 
 struct AgxTargetList {
     int count = 0;
+    int needVelocity = 0;    // a kinematic target is present: samples carry velocities
     AgxTargetDev t[AGX_MAX_TARGETS];
 };
 
@@ -135,10 +139,49 @@ __device__ __noinline__ void cylTargetAdd(const AgxTargetDev& T, double* values,
     }
 }
 
-__device__ __forceinline__ void targetAdd(const AgxTargetDev& T, double* row, double x, double y, double z, double mult)
+// linInt of math_spline.cpp:35-45: node positions clamped to the grid
+__device__ __forceinline__ double linIntD(double x, const double* g, int size, int i1, int i2)
 {
-    if(T.type == AGX_TARGET_DENSITY_SPHHARM) sphHarmTargetAdd(T, row, x, y, z, mult);
-    else cylTargetAdd(T, row, x, y, z, mult);
+    const double x1 = g[i1 < 0 ? 0 : i1 >= size ? size - 1 : i1], x2 = g[i2 < 0 ? 0 : i2 >= size ? size - 1 : i2];
+    return x1 == x2 ? (x == x1 ? 1 : 0) : (x - x1) / (x2 - x1);
+}
+
+// BsplineInterpolator1d<N>::addPoint (math_spline.cpp:1877-1886) with de Boor's recurrence
+// (bsplineValues, :67-82)
+__device__ void bsplineAddD(int N, const double* grid, int size, double x, double mult, double* values)
+{
+    const int ind = binSearchD(x, grid, size);
+    if(ind < 0 || ind >= size - 1) return;
+    double B[4] = { 0, 0, 0, 0 };
+    B[N] = 1;
+    for(int l = 1; l <= N; l++) {
+        double Bip1 = 0;
+        for(int i = N, j = ind; j >= ind - l; i--, j--) {
+            const double Bi = B[i] * linIntD(x, grid, size, j, j + l) + Bip1 * linIntD(x, grid, size, j + l + 1, j + 1);
+            Bip1 = B[i];
+            B[i] = Bi;
+        }
+    }
+    for(int i = 0; i <= N; i++) values[i + ind] += mult * B[i];
+}
+
+// TargetKinemShell<N>::addPoint (galaxymodel_losvd.cpp:568-576)
+__device__ __noinline__ void kinemShellTargetAdd(const AgxTargetDev& T, double* values, const double* p, double mult)
+{
+    const double r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2], r = sqrt(r2);
+    const double vr = p[0] * p[3] + p[1] * p[4] + p[2] * p[5];
+    const double vr2 = vr * vr / r2, vt2 = p[3] * p[3] + p[4] * p[4] + p[5] * p[5] - vr2;
+    const int nb = T.nr + T.lmax - 1;
+    bsplineAddD(T.lmax, T.gridr, T.nr, r, mult * vr2, values);
+    bsplineAddD(T.lmax, T.gridr, T.nr, r, mult * vt2, values + nb);
+}
+
+// p: position, and velocity when the list needs it, in the rotating frame
+__device__ __forceinline__ void targetAdd(const AgxTargetDev& T, double* row, const double* p, double mult)
+{
+    if(T.type == AGX_TARGET_DENSITY_SPHHARM) sphHarmTargetAdd(T, row, p[0], p[1], p[2], mult);
+    else if(T.type == AGX_TARGET_KINEM_SHELL) kinemShellTargetAdd(T, row, p, mult);
+    else cylTargetAdd(T, row, p[0], p[1], p[2], mult);
 }
 
 }  // namespace agxd

@@ -251,6 +251,10 @@ agx_target* agx_target_density_sphharm(int lmax, int mmax, const double* gridr, 
    invalid parameters. */
 agx_target* agx_target_density_cylindrical(int degree, int mmax, const double* gridR, int ngridR,
     const double* gridz, int ngridz);
+/* TargetKinemShell<degree> (src/galaxymodel_losvd.h:121-136, .cpp:568-576): the radial and
+   tangential velocity squared, v_r^2 and v_t^2, each on the B-spline basis of degree 0..3 over
+   gridr (from r = 0), 2 * (ngridr + degree - 1) values.  NULL on invalid parameters. */
+agx_target* agx_target_kinem_shell(int degree, const double* gridr, int ngridr);
 /* number of values a target records per orbit (numCoefs), -1 for NULL */
 int agx_target_num_values(const agx_target* t);
 void agx_target_destroy(agx_target* t);

@@ -2470,12 +2470,13 @@ inline double energyOf(const double* xv, double Epot, double Omega)
     return E;
 }
 
-// density targets (galaxymodel_densitygrid.h) with the datacube of a RuntimeFncTarget
+// density targets (galaxymodel_densitygrid.h) and the kinematic shell target
+// (galaxymodel_losvd.h:121-136) with the datacube of a RuntimeFncTarget
 // (galaxymodel_target.h:95-155): ten dense-output points per step
 struct OrcTarget {
     virtual ~OrcTarget() {}
     virtual int numValues() const = 0;
-    virtual void addPoint(const double point[3], double mult, double values[]) const = 0;
+    virtual void addPoint(const double point[6], double mult, double values[]) const = 0;
 };
 
 // TargetDensitySphHarm (galaxymodel_densitygrid.cpp:356-411)
@@ -2484,7 +2485,7 @@ struct OrcTargetSphHarm : OrcTarget {
     std::vector<double> gridr;   // without a node at the origin
     int numValues() const override
     { return ((lmax/2+1) * (mmax/2+1) - mmax/2 * (mmax/2+1) / 2) * (int)gridr.size() + 1; }
-    void addPoint(const double point[3], double mult, double values[]) const override
+    void addPoint(const double point[6], double mult, double values[]) const override
     {
         const double R = sqrt(pow_2(point[0]) + pow_2(point[1])), z = point[2];
         const double phi = atan2_r(point[1], point[0]);
@@ -2513,7 +2514,7 @@ struct OrcTargetCyl : OrcTarget {
     std::vector<double> gridR, gridz;   // without nodes at 0
     int numValues() const override
     { return ((int)gridz.size() + N) * ((int)gridR.size() * (mmax/2+1) + N); }
-    void addPoint(const double point[3], double mult, double values[]) const override
+    void addPoint(const double point[6], double mult, double values[]) const override
     {
         const double x = point[0], y = point[1], z = fabs(point[2]);
         const double R = sqrt(pow_2(x) + pow_2(y)), phi = atan2_r(y, x);
@@ -2542,6 +2543,45 @@ struct OrcTargetCyl : OrcTarget {
                 values[indlu] += val * (1-offR) * offz;
             }
         }
+    }
+};
+
+// TargetKinemShell<N> (galaxymodel_losvd.cpp:568-576) on BsplineInterpolator1d<N>::addPoint
+// (math_spline.cpp:1877-1886) with de Boor's recurrence (bsplineValues, :35-82)
+struct OrcTargetKinemShell : OrcTarget {
+    int N = 0;
+    std::vector<double> gridr;
+    int numValues() const override { return 2 * ((int)gridr.size() + N - 1); }
+    static double linInt(double x, const double* g, int size, int i1, int i2)
+    {
+        const double x1 = g[i1 < 0 ? 0 : i1 >= size ? size-1 : i1], x2 = g[i2 < 0 ? 0 : i2 >= size ? size-1 : i2];
+        if(x1 == x2) return x == x1 ? 1 : 0;
+        return (x - x1) / (x2 - x1);
+    }
+    void bspline(double x, double mult, double values[]) const
+    {
+        const int size = gridr.size();
+        const int ind = (int)binSearch(x, &gridr[0], size);
+        if(ind < 0 || ind >= size-1) return;
+        double B[4] = {0, 0, 0, 0};
+        B[N] = 1;
+        for(int l = 1; l <= N; l++) {
+            double Bip1 = 0;
+            for(int i = N, j = ind; j >= ind-l; i--, j--) {
+                const double Bi = B[i] * linInt(x, &gridr[0], size, j, j+l) + Bip1 * linInt(x, &gridr[0], size, j+l+1, j+1);
+                Bip1 = B[i];
+                B[i] = Bi;
+            }
+        }
+        for(int i = 0; i <= N; i++) values[i + ind] += mult * B[i];
+    }
+    void addPoint(const double point[6], double mult, double values[]) const override
+    {
+        const double r2 = pow_2(point[0]) + pow_2(point[1]) + pow_2(point[2]), r = sqrt(r2);
+        const double vr2 = pow_2(point[0]*point[3] + point[1]*point[4] + point[2]*point[5]) / r2;
+        const double vt2 = pow_2(point[3]) + pow_2(point[4]) + pow_2(point[5]) - vr2;
+        bspline(r, mult * vr2, values);
+        bspline(r, mult * vt2, values + numValues() / 2);
     }
 };
 
@@ -3183,6 +3223,30 @@ int orc_orbit_target_cyl(const orc_pot* p, long n, const double* ic, const doubl
     T.gridR.assign(ngridR > 0 && gridR[0] > 0 ? gridR : gridR + 1, gridR + ngridR);
     T.gridz.assign(ngridz > 0 && gridz[0] > 0 ? gridz : gridz + 1, gridz + ngridz);
     return runTarget(p, n, ic, tstart, ttotal, accuracy, Omega, maxNumSteps, T, out, nthreads);
+}
+
+int orc_orbit_target_kinem_shell(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree,
+    const double* gridr, int ngridr, float* out, int nthreads)
+{
+    OrcTargetKinemShell T;
+    T.N = degree;
+    T.gridr.assign(gridr, gridr + ngridr);
+    return runTarget(p, n, ic, tstart, ttotal, accuracy, Omega, maxNumSteps, T, out, nthreads);
+}
+
+// the B-spline basis of TargetKinemShell<degree> at points x: out[i*(ngridr+degree-1) + k]
+void orc_bspline_values(int degree, const double* gridr, int ngridr, long n, const double* x, double* out)
+{
+    OrcTargetKinemShell T;
+    T.N = degree;
+    T.gridr.assign(gridr, gridr + ngridr);
+    const int nb = ngridr + degree - 1;
+    for(long i = 0; i < n; i++) {
+        double* o = out + i * nb;
+        for(int k = 0; k < nb; k++) o[k] = 0;
+        T.bspline(x[i], 1., o);
+    }
 }
 
 static int runTarget(const orc_pot* p, long n, const double* ic, const double* tstart, const double* ttotal,

@@ -105,6 +105,12 @@ int orc_orbit_target_sphharm(const orc_pot* p, long n, const double* ic, const d
 int orc_orbit_target_cyl(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree, int mmax,
     const double* gridR, int ngridR, const double* gridz, int ngridz, float* out, int nthreads);
+/* the same with TargetKinemShell<degree> (KinemShell: v_r^2 and v_t^2 on B-splines in r) */
+int orc_orbit_target_kinem_shell(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree,
+    const double* gridr, int ngridr, float* out, int nthreads);
+/* BsplineInterpolator1d<degree> basis at n points: out[n * (ngridr + degree - 1)] */
+void orc_bspline_values(int degree, const double* gridr, int ngridr, long n, const double* x, double* out);
 /* agama.Orbit on one recorded series (createOrbitObject + Orbit.__call__): out[m*6] rotating
    frame, NaN outside the span; out_spl[m*9] the inertial-frame splines x,y,z, their first and
    second derivatives (linear continuation outside) */

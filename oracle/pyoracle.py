@@ -61,6 +61,10 @@ def _load(name):
         "orc_orbit_target_cyl": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_long, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp,
                                                 ctypes.c_int, _vp, ctypes.c_int]),
+        "orc_orbit_target_kinem_shell": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, ctypes.c_double,
+                                                        ctypes.c_double, ctypes.c_long, ctypes.c_int, _vp,
+                                                        ctypes.c_int, _vp, ctypes.c_int]),
+        "orc_bspline_values": (None, [ctypes.c_int, _vp, ctypes.c_int, ctypes.c_long, _vp, _vp]),
         "orc_ode_gl4": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                        _vp, _vp]),
         "orc_ode_dop853_fixed": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
@@ -345,6 +349,36 @@ def _orbits_target_cyl(self, ic, ttotal, gridR, gridz, mmax, degree, tstart=None
 
 
 OraclePotential.orbits_target_cyl = _orbits_target_cyl
+
+
+def _orbits_target_kinem_shell(self, ic, ttotal, gridr, degree, tstart=None, accuracy=1e-8, Omega=0.0,
+                               maxNumSteps=int(1e8), nthreads=0):
+    """orbit(targets=[Target(type='KinemShell', gridr, degree)]) as the oracle restates it:
+    (n, 2 * (len(gridr) + degree - 1)) float32, v_r^2 coefficients then v_t^2"""
+    ic = np.ascontiguousarray(ic, dtype=np.float64).reshape(-1, 6)
+    n = len(ic)
+    tt = np.ascontiguousarray(np.broadcast_to(np.asarray(ttotal, dtype=np.float64), (n,)))
+    ts = None if tstart is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(tstart, dtype=np.float64), (n,)))
+    gr = np.ascontiguousarray(gridr, dtype=np.float64)
+    out = np.empty((n, 2 * (len(gr) + degree - 1)), dtype=np.float32)
+    rc = self.lib.orc_orbit_target_kinem_shell(self.h, n, _p(ic), _p(ts), _p(tt), accuracy, Omega, maxNumSteps,
+                                               int(degree), _p(gr), len(gr), _p(out), nthreads)
+    if rc:
+        raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
+    return out
+
+
+OraclePotential.orbits_target_kinem_shell = _orbits_target_kinem_shell
+
+
+def bspline_values(degree, gridr, x):
+    """the oracle's B-spline basis of degree 0..3 on gridr at points x: (len(x), len(gridr)+degree-1)"""
+    gr = np.ascontiguousarray(gridr, dtype=np.float64)
+    x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+    out = np.empty((len(x), len(gr) + degree - 1))
+    lib.orc_bspline_values(int(degree), _p(gr), len(gr), len(x), _p(x), _p(out))
+    return out
 
 
 def orbit_interp(t, traj, Omega, tq):

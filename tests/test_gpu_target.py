@@ -87,3 +87,29 @@ def test_cylindrical_target_matches_oracle(setup, degree):
     rel = np.max(np.abs(out - ref), axis=1)[~empty] / np.max(np.abs(ref), axis=1)[~empty]
     print(f"per-orbit max rel diff: median {np.median(rel):.2e}, 90% {np.quantile(rel, 0.9):.2e}, max {rel.max():.2e}")
     assert np.median(rel) <= 1e-6 and np.quantile(rel, 0.9) <= 1e-5
+
+
+@pytest.mark.parametrize("degree", [0, 1, 3])
+def test_kinem_shell_target_matches_oracle(setup, degree):
+    import agama_amd
+    from agama_amd import workloads
+    from oracle.pyoracle import bspline_values
+    gpu, orc = setup
+    ic, T = workloads.triaxial_dehnen_orbits(64, lambda p: orc.eval(p)[0], gpu.coefs(), seed=64, periods=5)
+    gridr = np.concatenate([[0.0], np.logspace(-2, 3, 14)])
+    print(f"case: KinemShell degree {degree} target, 64 orbits x 5 periods, with a density target", flush=True)
+    tgt = agama_amd.Target(type="KinemShell", gridr=gridr, degree=degree)
+    dens = agama_amd.Target(type="DensitySphHarm", gridr=GRIDR, lmax=4, mmax=2)
+    out, d = agama_amd.orbit(potential=gpu, ic=ic, time=T, targets=[tgt, dens], Omega=0.2)
+    ref = orc.orbits_target_kinem_shell(ic, T, gridr, degree, Omega=0.2)
+    assert out.shape == ref.shape == (64, 2 * (len(gridr) + degree - 1)) == (64, len(tgt))
+    rel = np.max(np.abs(out - ref), axis=1) / np.max(np.abs(ref), axis=1)
+    print(f"per-orbit max rel diff: median {np.median(rel):.2e}, 90% {np.quantile(rel, 0.9):.2e}, max {rel.max():.2e}")
+    assert np.median(rel) <= 1e-6 and np.quantile(rel, 0.9) <= 1e-5
+    # the density target computed beside it (samples with velocities) is the same as alone
+    assert np.array_equal(d, agama_amd.orbit(potential=gpu, ic=ic, time=T, targets=dens, Omega=0.2))
+    # the v_r^2 + v_t^2 values summed over the basis: the time average of v^2, positive
+    assert np.all(out.sum(axis=1) > 0)
+    with pytest.raises(ValueError):
+        agama_amd.Target(type="KinemShell", gridr=gridr, degree=4)
+    assert bspline_values(degree, gridr, [0.5]).shape == (1, len(gridr) + degree - 1)

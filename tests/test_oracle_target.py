@@ -130,3 +130,56 @@ def test_cylindrical_targets_closed_forms(degree):
     res = tri.orbits_target_cyl(ic[sel], T[sel], big, big, mmax, degree)
     m0 = res[:, :(30 + degree) * (30 + degree)]
     assert np.max(np.abs(m0.sum(axis=1) - 1)) < 2e-6
+
+
+def _scipy_bspline_basis(degree, grid, x):
+    """the clamped B-spline basis of scipy (end knots repeated degree times) at points x"""
+    from scipy.interpolate import BSpline
+    knots = np.concatenate([[grid[0]] * degree, grid, [grid[-1]] * degree])
+    return BSpline.design_matrix(x, knots, degree).toarray()
+
+
+@pytest.mark.parametrize("degree", [0, 1, 2, 3])
+def test_kinem_shell_bspline_basis_matches_scipy(degree):
+    """BsplineInterpolator1d<N> (math_spline.cpp:35-82, 1877-1886): de Boor's recurrence with
+    node positions clamped at the grid ends is the clamped B-spline basis; it partitions unity
+    inside the grid and is zero outside it"""
+    from oracle.pyoracle import bspline_values
+    grid = np.array([0.0, 0.1, 0.25, 0.3, 0.7, 1.5, 2.0, 4.0])
+    rng = np.random.default_rng(71)
+    x = np.concatenate([rng.uniform(0, 4.0, 300), grid[:-1]])
+    B = bspline_values(degree, grid, x)
+    assert B.shape == (len(x), len(grid) + degree - 1)
+    assert np.max(np.abs(B - _scipy_bspline_basis(degree, grid, x))) < 1e-14
+    assert np.max(np.abs(B.sum(axis=1) - 1)) < 1e-14
+    assert np.all(bspline_values(degree, grid, [-0.1, 4.1]) == 0)
+
+
+@pytest.mark.parametrize("degree", [0, 1, 2, 3])
+def test_kinem_shell_circular_and_radial_orbits(degree):
+    """TargetKinemShell<N> (galaxymodel_losvd.cpp:568-576): a circular orbit in the equatorial
+    plane of an axisymmetric potential has v_r = 0 and v_t = v_c at one radius, so the v_t^2
+    values are v_c^2 times the basis at r0 and the v_r^2 values vanish, in the inertial frame and
+    in a rotating one (the target sees velocities rotated, not corotating ones); an orbit released
+    from rest on the long axis of the triaxial model has v_t = 0, and its v_r^2 values are
+    positive where the orbit goes"""
+    orc = OraclePotential(os.path.join(GOLDEN, "disk_bulge_halo_lmax12.coef"))
+    r0 = 0.7
+    _, g = orc.eval(np.array([[r0, 0.0, 0.0]]))
+    vc2 = r0 * abs(g[0, 0])
+    period = 2 * math.pi * r0 / math.sqrt(vc2)
+    gridr = np.array([0.0, 0.2, 0.5, 1.0, 2.0, 5.0])
+    nb = len(gridr) + degree - 1
+    expected = np.concatenate([np.zeros(nb), vc2 * _scipy_bspline_basis(degree, gridr, np.array([r0]))[0]])
+    for Omega in (0.0, 0.4):
+        out = orc.orbits_target_kinem_shell(np.array([[r0, 0, 0, 0, math.sqrt(vc2), 0]]), 5 * period, gridr,
+                                            degree, accuracy=1e-10, Omega=Omega)[0]
+        assert len(out) == 2 * nb
+        assert np.max(np.abs(out - expected)) < 1e-5 * vc2, (Omega, out, expected)
+    tri = OraclePotential(os.path.join(GOLDEN, "dehnen_triax_lmax8.coef"))
+    rad = tri.orbits_target_kinem_shell(np.array([[3.0, 0, 0, 0, 0, 0]]), 40.0, gridr, degree)[0]
+    vr2, vt2 = rad[:nb], rad[nb:]
+    assert np.max(np.abs(vt2)) < 1e-8 * np.max(vr2)
+    assert np.all(vr2 >= 0) and np.sum(vr2 > 0) >= 4
+    # zero-length orbits record nothing
+    assert np.all(tri.orbits_target_kinem_shell(np.array([[3.0, 0, 0, 0, 0, 0]]), 0.0, gridr, degree) == 0)
