@@ -478,7 +478,9 @@ class Target:
     nodes), lmax, mmax (even orders of the spherical-harmonic expansion): the density
     discretisation of TargetDensitySphHarm (src/galaxymodel_densitygrid.cpp:356-411); and
     type='DensityCylindricalTopHat' / 'DensityCylindricalLinear' with gridr, gridz (nodes in
-    R and |z|) and mmax (TargetDensityCylindrical<0/1>, :486-545); and type='KinemShell' with
+    R and |z|) and mmax (TargetDensityCylindrical<0/1>, :486-545); type='DensityClassicTopHat' /
+    'DensityClassicLinear' with gridr (shell radii), stripsPerPane (default 2), axisRatioY and
+    axisRatioZ (default 1) (TargetDensityClassic<0/1>, :146-239); and type='KinemShell' with
     gridr (nodes in r, from r = 0) and degree 0..3: v_r^2 and v_t^2 on B-splines of that degree
     (TargetKinemShell<N>, src/galaxymodel_losvd.cpp:568-576; interface_python.cpp:7228-7243).
     len(target) is the number of values recorded per orbit."""
@@ -490,11 +492,12 @@ class Target:
             raise TypeError("Must provide a type='...' argument")
         kind = type_str.lower()
         names = {"densitysphharm": "DensitySphHarm", "densitycylindricaltophat": "DensityCylindricalTopHat",
-                 "densitycylindricallinear": "DensityCylindricalLinear", "kinemshell": "KinemShell"}
+                 "densitycylindricallinear": "DensityCylindricalLinear", "kinemshell": "KinemShell",
+                 "densityclassictophat": "DensityClassicTopHat", "densityclassiclinear": "DensityClassicLinear"}
         if kind not in names:
             raise AgamaAmdError(f"Target type {type_str} is not provided by this version "
                                 "(DensitySphHarm, DensityCylindricalTopHat, DensityCylindricalLinear, "
-                                "KinemShell are)")
+                                "DensityClassicTopHat, DensityClassicLinear, KinemShell are)")
         if "gridr" not in kwargs:
             raise TypeError("Argument 'gridr' is required")
         gridr = np.ascontiguousarray(kwargs.pop("gridr"), dtype=np.float64).reshape(-1)
@@ -508,6 +511,15 @@ class Target:
             if kwargs:
                 raise TypeError("Unknown arguments: " + ", ".join(kwargs))
             h = lib.agx_target_density_sphharm(lmax, mmax, ptr(gridr), len(gridr))
+        elif kind.startswith("densityclassic"):   # interface_python.cpp:7191-7203
+            axisRatioY = float(kwargs.pop("axisRatioY", 1.0))
+            axisRatioZ = float(kwargs.pop("axisRatioZ", 1.0))
+            strips = int(kwargs.pop("stripsPerPane", 2))
+            self.stripsPerPane, self.axisRatioY, self.axisRatioZ = strips, axisRatioY, axisRatioZ
+            if kwargs:
+                raise TypeError("Unknown arguments: " + ", ".join(kwargs))
+            h = lib.agx_target_density_classic(0 if kind.endswith("tophat") else 1, strips, ptr(gridr), len(gridr),
+                                               axisRatioY, axisRatioZ)
         elif kind == "kinemshell":
             degree = int(kwargs.pop("degree", -1))
             if not 0 <= degree <= 3:

@@ -1257,6 +1257,7 @@ void agx_orbit_record_free(agx_orbit_record* R)
 // ---------------- target plug-ins ----------------
 struct agx_target {
     int type = 0, lmax = 0, mmax = 0;
+    double axis[3] = {1, 1, 1};       // DensityClassic: shell axes
     std::vector<double> gridr;        // radial nodes without the origin
     std::vector<double> gridz;        // vertical nodes without z = 0 (cylindrical targets)
     int numValues = 0;
@@ -1302,6 +1303,31 @@ agx_target* agx_target_density_cylindrical(int degree, int mmax, const double* g
     for(size_t i = 1; i < T->gridz.size(); i++) ok &= T->gridz[i] > T->gridz[i-1];
     if(!ok) { fail(AGX_EINVAL, "TargetDensityCylindrical: invalid grid parameters"); return nullptr; }
     T->numValues = ((int)T->gridz.size() + degree) * ((int)T->gridr.size() * (mmax/2+1) + degree);
+    return T.release();
+}
+
+agx_target* agx_target_density_classic(int degree, int stripsPerPane, const double* gridr, int ngridr,
+    double axisRatioY, double axisRatioZ)
+{
+    g_err.clear();
+    // TargetDensityClassic<N> ctor (galaxymodel_densitygrid.cpp:148-167): a grid starting at 0
+    // loses that node; axes scaled to a unit product
+    if((degree != 0 && degree != 1) || !gridr || ngridr < 1) {
+        fail(AGX_EINVAL, "TargetDensityClassic: invalid grid parameters");
+        return nullptr;
+    }
+    std::unique_ptr<agx_target> T(new agx_target);
+    T->type = degree == 0 ? AGX_TARGET_DENSITY_CLASSIC_TOPHAT : AGX_TARGET_DENSITY_CLASSIC_LINEAR;
+    T->mmax = stripsPerPane;
+    T->gridr.assign(gridr[0] > 0 ? gridr : gridr + 1, gridr + ngridr);
+    bool ok = T->gridr.size() >= 1 && T->gridr[0] > 0 && stripsPerPane >= 1 && axisRatioY > 0 && axisRatioZ > 0;
+    for(size_t i = 1; i < T->gridr.size(); i++) ok &= T->gridr[i] > T->gridr[i-1];
+    if(!ok) { fail(AGX_EINVAL, "TargetDensityClassic: invalid grid parameters"); return nullptr; }
+    T->axis[0] = 1. / cbrt(axisRatioY * axisRatioZ);
+    T->axis[1] = axisRatioY * T->axis[0];
+    T->axis[2] = axisRatioZ * T->axis[0];
+    const int S = stripsPerPane;
+    T->numValues = (3 * S * (S + degree) + degree) * (int)T->gridr.size() + degree;
     return T.release();
 }
 
@@ -1363,6 +1389,7 @@ int agx_orbit_target_batch(const agx_potential* pot, long n, const double* ic, c
             D.nz = (int)T.gridz.size();
             D.numValues = T.numValues; D.gridr = grids.back()->p; D.gridz = grids.back()->p + T.gridr.size();
             D.cube = cubes.back()->p;
+            for(int c = 0; c < 3; c++) D.axis[c] = T.axis[c];
         }
         DevBuf<AgxTargetList> dTL(1);
         HIPCHECK(hipMemcpy(dTL.p, &TL, sizeof(TL), hipMemcpyHostToDevice));

@@ -3,7 +3,9 @@
 // by its share of the step) with the density discretisations TargetDensitySphHarm
 // (src/galaxymodel_densitygrid.cpp:356-411: linear B-splines in r times even spherical
 // harmonics), TargetDensityCylindrical<0/1> (:486-545, 92-118: top-hat cells or bilinear
-// B-splines in (R, |z|) times even azimuthal Fourier terms) and the kinematic
+// B-splines in (R, |z|) times even azimuthal Fourier terms), TargetDensityClassic<0/1> (:146-239,
+// 59-90: spheroidal shells, three panes of strips each, top-hat cells or trilinear nodes) and the
+// kinematic
 // TargetKinemShell<N> (src/galaxymodel_losvd.cpp:568-576: v_r^2 and v_t^2 on B-splines of degree
 // N in r, src/math_spline.cpp:35-82, 1877-1886).  One lane accumulates its orbit's datacube in FP64 in global memory (its own
 // row: no atomics); the host normalises by the integration time at the end.
@@ -12,18 +14,20 @@
 #include "agx_device.h"
 
 enum : int { AGX_TARGET_DENSITY_SPHHARM = 1, AGX_TARGET_DENSITY_CYL_TOPHAT = 2, AGX_TARGET_DENSITY_CYL_LINEAR = 3,
-    AGX_TARGET_KINEM_SHELL = 4 };
+    AGX_TARGET_KINEM_SHELL = 4, AGX_TARGET_DENSITY_CLASSIC_TOPHAT = 5, AGX_TARGET_DENSITY_CLASSIC_LINEAR = 6 };
 enum : int { AGX_MAX_TARGETS = 4 };
 
 struct AgxTargetDev {
     int type;
-    int lmax, mmax;          // even orders of the angular expansion (KinemShell: lmax = B-spline degree)
+    int lmax, mmax;          // even orders of the angular expansion (KinemShell: lmax = B-spline degree;
+                             // DensityClassic: mmax = strips per pane)
     int nr;                  // radial grid nodes (the origin implied)
     int nz;                  // vertical grid nodes (cylindrical targets; z = 0 implied)
     int numValues;           // datacube length; the row holds one more value: the accumulated time
     const double* gridr;     // nr nodes, increasing, > 0 (spherical radius, or cylindrical R)
     const double* gridz;     // nz nodes, increasing, > 0 (cylindrical targets)
     double* cube;            // n x (numValues + 1) per orbit
+    double axis[3];          // DensityClassic: axes of the spheroidal shells, product 1
 };
 
 struct AgxTargetList {
@@ -139,6 +143,62 @@ __device__ __noinline__ void cylTargetAdd(const AgxTargetDev& T, double* values,
     }
 }
 
+// TargetDensityClassic<N>::addPoint (galaxymodel_densitygrid.cpp:169-239) with the cell and
+// corner numbering of getCornerIndicesClassic (:59-90).  The reference throws for a point whose
+// pane cannot be determined (a NaN coordinate); such a sample adds nothing here.
+__device__ __noinline__ void classicTargetAdd(const AgxTargetDev& T, double* values, double px, double py, double pz,
+    double mult)
+{
+    const int N = T.type == AGX_TARGET_DENSITY_CLASSIC_LINEAR ? 1 : 0, S = T.mmax, numShells = T.nr;
+    const int valuesPerShell = 3 * S * (S + N) + N;
+    const double X = fabs(px / T.axis[0]), Y = fabs(py) / T.axis[1], Z = fabs(pz) / T.axis[2];
+    const double r = sqrt(X * X + Y * Y + Z * Z);
+    const int indShell = binSearchD(r, T.gridr, numShells) + 1;
+    if(indShell >= numShells || mult == 0) return;
+    int pane;
+    double c0, c1, c2;
+    if(X >= Y && X > Z)      { pane = 0; c0 = X; c1 = Y; c2 = Z; }
+    else if(Y >= Z && Y > X) { pane = 1; c0 = Y; c1 = Z; c2 = X; }
+    else if(Z >= X && Z >= Y){ pane = 2; c0 = Z; c1 = X; c2 = Y; }
+    else return;
+    double ratio1 = fmin(atan(c1 / c0) * (4 / M_PI), 1.) * S;
+    double ratio2 = fmin(atan(c2 / c0) * (4 / M_PI), 1.) * S;
+    const int ind1 = min((int)ratio1, S - 1), ind2 = min((int)ratio2, S - 1);
+    if(N == 0) {
+        values[ind1 + (ind2 + pane * S) * S + indShell * valuesPerShell] += mult;
+        return;
+    }
+    const int indll = ind1 + (ind2 + pane * S) * (S + 1), indul = indll + 1;
+    int indlu, induu;
+    if(ind2 < S - 1) {
+        indlu = indll + S + 1;
+        induu = indlu + 1;
+    } else {                       // the upper row lies on the adjacent pane
+        const int adjpane = (pane + 2) % 3;
+        indlu = (ind1 + adjpane * S + 1) * (S + 1) - 1;
+        induu = ind1 < S - 1 ? indlu + S + 1 : 3 * S * (S + 1);   // the node where three panes meet
+    }
+    ratio1 -= ind1;
+    ratio2 -= ind2;
+    double val = indShell == 0 ? mult * r / T.gridr[0]
+        : mult * (r - T.gridr[indShell - 1]) / (T.gridr[indShell] - T.gridr[indShell - 1]);
+    double* off = values + indShell * valuesPerShell + 1;
+    off[indll] += (1 - ratio1) * (1 - ratio2) * val;
+    off[indul] += ratio1 * (1 - ratio2) * val;
+    off[indlu] += (1 - ratio1) * ratio2 * val;
+    off[induu] += ratio1 * ratio2 * val;
+    if(indShell == 0) {
+        values[0] += mult - val;
+    } else {
+        off -= valuesPerShell;
+        val = mult - val;
+        off[indll] += (1 - ratio1) * (1 - ratio2) * val;
+        off[indul] += ratio1 * (1 - ratio2) * val;
+        off[indlu] += (1 - ratio1) * ratio2 * val;
+        off[induu] += ratio1 * ratio2 * val;
+    }
+}
+
 // linInt of math_spline.cpp:35-45: node positions clamped to the grid
 __device__ __forceinline__ double linIntD(double x, const double* g, int size, int i1, int i2)
 {
@@ -181,6 +241,7 @@ __device__ __forceinline__ void targetAdd(const AgxTargetDev& T, double* row, co
 {
     if(T.type == AGX_TARGET_DENSITY_SPHHARM) sphHarmTargetAdd(T, row, p[0], p[1], p[2], mult);
     else if(T.type == AGX_TARGET_KINEM_SHELL) kinemShellTargetAdd(T, row, p, mult);
+    else if(T.type >= AGX_TARGET_DENSITY_CLASSIC_TOPHAT) classicTargetAdd(T, row, p[0], p[1], p[2], mult);
     else cylTargetAdd(T, row, p[0], p[1], p[2], mult);
 }
 

@@ -251,6 +251,12 @@ agx_target* agx_target_density_sphharm(int lmax, int mmax, const double* gridr, 
    invalid parameters. */
 agx_target* agx_target_density_cylindrical(int degree, int mmax, const double* gridR, int ngridR,
     const double* gridz, int ngridz);
+/* TargetDensityClassic<degree> (DensityClassicTopHat = 0 / DensityClassicLinear = 1,
+   src/galaxymodel_densitygrid.cpp:146-239): spheroidal shells on gridr (a leading 0 is dropped)
+   with axis ratios y/x and z/x, each shell cut into three panes of stripsPerPane^2 cells; top-hat
+   cells or trilinear nodes.  NULL on invalid parameters. */
+agx_target* agx_target_density_classic(int degree, int stripsPerPane, const double* gridr, int ngridr,
+    double axisRatioY, double axisRatioZ);
 /* TargetKinemShell<degree> (src/galaxymodel_losvd.h:121-136, .cpp:568-576): the radial and
    tangential velocity squared, v_r^2 and v_t^2, each on the B-spline basis of degree 0..3 over
    gridr (from r = 0), 2 * (ngridr + degree - 1) values.  NULL on invalid parameters. */

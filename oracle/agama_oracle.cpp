@@ -2546,6 +2546,54 @@ struct OrcTargetCyl : OrcTarget {
     }
 };
 
+// TargetDensityClassic<N> (galaxymodel_densitygrid.cpp:146-239, corner indices :59-90)
+struct OrcTargetClassic : OrcTarget {
+    int N = 0, S = 2;
+    double axisX = 1, axisY = 1, axisZ = 1;
+    std::vector<double> gridr;   // without a node at the origin
+    int vps() const { return 3 * S * (S + N) + N; }
+    int numValues() const override { return vps() * (int)gridr.size() + N; }
+    void corners(int pane, int ind1, int ind2, int& ll, int& ul, int& lu, int& uu) const
+    {
+        if(N == 0) { ll = ind1 + (ind2 + pane * S) * S; return; }
+        ll = ind1 + (ind2 + pane * S) * (S + 1);
+        ul = ll + 1;
+        if(ind2 < S - 1) { lu = ll + S + 1; uu = lu + 1; return; }
+        const int adj = (pane + 2) % 3;
+        lu = (ind1 + adj * S + 1) * (S + 1) - 1;
+        uu = ind1 < S - 1 ? lu + S + 1 : 3 * S * (S + 1);
+    }
+    void addPoint(const double point[6], double mult, double values[]) const override
+    {
+        const int nsh = gridr.size();
+        const double X = fabs(point[0] / axisX), Y = fabs(point[1]) / axisY, Z = fabs(point[2]) / axisZ;
+        const double r = sqrt(X*X + Y*Y + Z*Z);
+        const int sh = (int)binSearch(r, &gridr[0], nsh) + 1;
+        if(sh >= nsh || mult == 0) return;
+        int pane; double c0, c1, c2;
+        if(X >= Y && X > Z)       { pane = 0; c0 = X; c1 = Y; c2 = Z; }
+        else if(Y >= Z && Y > X)  { pane = 1; c0 = Y; c1 = Z; c2 = X; }
+        else if(Z >= X && Z >= Y) { pane = 2; c0 = Z; c1 = X; c2 = Y; }
+        else throw std::runtime_error("TargetDensityClassic: cannot determine the cell index");
+        double q1 = fmin(atan(c1 / c0) * (4 / M_PI), 1.) * S, q2 = fmin(atan(c2 / c0) * (4 / M_PI), 1.) * S;
+        const int i1 = std::min<int>(q1, S - 1), i2 = std::min<int>(q2, S - 1);
+        int ll = 0, ul = 0, lu = 0, uu = 0;
+        corners(pane, i1, i2, ll, ul, lu, uu);
+        if(N == 0) { values[ll + sh * vps()] += mult; return; }
+        q1 -= i1; q2 -= i2;
+        const double w[4] = { (1-q1) * (1-q2), q1 * (1-q2), (1-q1) * q2, q1 * q2 };
+        const int idx[4] = { ll, ul, lu, uu };
+        const double up = sh == 0 ? mult * r / gridr[0] : mult * (r - gridr[sh-1]) / (gridr[sh] - gridr[sh-1]);
+        double* o = &values[sh * vps() + 1];
+        for(int k = 0; k < 4; k++) o[idx[k]] += w[k] * up;
+        if(sh == 0) values[0] += mult - up;
+        else {
+            o -= vps();
+            for(int k = 0; k < 4; k++) o[idx[k]] += w[k] * (mult - up);
+        }
+    }
+};
+
 // TargetKinemShell<N> (galaxymodel_losvd.cpp:568-576) on BsplineInterpolator1d<N>::addPoint
 // (math_spline.cpp:1877-1886) with de Boor's recurrence (bsplineValues, :35-82)
 struct OrcTargetKinemShell : OrcTarget {
@@ -3232,6 +3280,19 @@ int orc_orbit_target_kinem_shell(const orc_pot* p, long n, const double* ic, con
     OrcTargetKinemShell T;
     T.N = degree;
     T.gridr.assign(gridr, gridr + ngridr);
+    return runTarget(p, n, ic, tstart, ttotal, accuracy, Omega, maxNumSteps, T, out, nthreads);
+}
+
+int orc_orbit_target_classic(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree, int stripsPerPane,
+    const double* gridr, int ngridr, double axisRatioY, double axisRatioZ, float* out, int nthreads)
+{
+    OrcTargetClassic T;
+    T.N = degree; T.S = stripsPerPane;
+    T.gridr.assign(ngridr > 0 && gridr[0] > 0 ? gridr : gridr + 1, gridr + ngridr);
+    T.axisX = 1. / cbrt(axisRatioY * axisRatioZ);
+    T.axisY = axisRatioY * T.axisX;
+    T.axisZ = axisRatioZ * T.axisX;
     return runTarget(p, n, ic, tstart, ttotal, accuracy, Omega, maxNumSteps, T, out, nthreads);
 }
 

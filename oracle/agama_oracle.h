@@ -105,6 +105,10 @@ int orc_orbit_target_sphharm(const orc_pot* p, long n, const double* ic, const d
 int orc_orbit_target_cyl(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree, int mmax,
     const double* gridR, int ngridR, const double* gridz, int ngridz, float* out, int nthreads);
+/* the same with TargetDensityClassic<degree> (DensityClassicTopHat / Linear) */
+int orc_orbit_target_classic(const orc_pot* p, long n, const double* ic, const double* tstart,
+    const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree, int stripsPerPane,
+    const double* gridr, int ngridr, double axisRatioY, double axisRatioZ, float* out, int nthreads);
 /* the same with TargetKinemShell<degree> (KinemShell: v_r^2 and v_t^2 on B-splines in r) */
 int orc_orbit_target_kinem_shell(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree,

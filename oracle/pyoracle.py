@@ -64,6 +64,10 @@ def _load(name):
         "orc_orbit_target_kinem_shell": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, ctypes.c_double,
                                                         ctypes.c_double, ctypes.c_long, ctypes.c_int, _vp,
                                                         ctypes.c_int, _vp, ctypes.c_int]),
+        "orc_orbit_target_classic": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_long, ctypes.c_int, ctypes.c_int, _vp,
+                                                    ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp,
+                                                    ctypes.c_int]),
         "orc_bspline_values": (None, [ctypes.c_int, _vp, ctypes.c_int, ctypes.c_long, _vp, _vp]),
         "orc_ode_gl4": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                        _vp, _vp]),
@@ -370,6 +374,33 @@ def _orbits_target_kinem_shell(self, ic, ttotal, gridr, degree, tstart=None, acc
 
 
 OraclePotential.orbits_target_kinem_shell = _orbits_target_kinem_shell
+
+
+def classic_target_size(degree, strips, gridr):
+    nsh = len(gridr) - (1 if gridr[0] == 0 else 0)
+    return (3 * strips * (strips + degree) + degree) * nsh + degree
+
+
+def _orbits_target_classic(self, ic, ttotal, gridr, degree, stripsPerPane=2, axisRatioY=1.0, axisRatioZ=1.0,
+                           tstart=None, accuracy=1e-8, Omega=0.0, maxNumSteps=int(1e8), nthreads=0):
+    """orbit(targets=[Target(type='DensityClassicTopHat' (degree 0) / 'DensityClassicLinear' (degree 1),
+    ...)]) as the oracle restates it: (n, numCoefs) float32"""
+    ic = np.ascontiguousarray(ic, dtype=np.float64).reshape(-1, 6)
+    n = len(ic)
+    tt = np.ascontiguousarray(np.broadcast_to(np.asarray(ttotal, dtype=np.float64), (n,)))
+    ts = None if tstart is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(tstart, dtype=np.float64), (n,)))
+    gr = np.ascontiguousarray(gridr, dtype=np.float64)
+    out = np.empty((n, classic_target_size(degree, stripsPerPane, gr)), dtype=np.float32)
+    rc = self.lib.orc_orbit_target_classic(self.h, n, _p(ic), _p(ts), _p(tt), accuracy, Omega, maxNumSteps,
+                                           int(degree), int(stripsPerPane), _p(gr), len(gr), float(axisRatioY),
+                                           float(axisRatioZ), _p(out), nthreads)
+    if rc:
+        raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
+    return out
+
+
+OraclePotential.orbits_target_classic = _orbits_target_classic
 
 
 def bspline_values(degree, gridr, x):

@@ -113,3 +113,23 @@ def test_kinem_shell_target_matches_oracle(setup, degree):
     with pytest.raises(ValueError):
         agama_amd.Target(type="KinemShell", gridr=gridr, degree=4)
     assert bspline_values(degree, gridr, [0.5]).shape == (1, len(gridr) + degree - 1)
+
+
+@pytest.mark.parametrize("degree", [0, 1])
+def test_classic_target_matches_oracle(setup, degree):
+    import agama_amd
+    from agama_amd import workloads
+    gpu, orc = setup
+    ic, T = workloads.triaxial_dehnen_orbits(64, lambda p: orc.eval(p)[0], gpu.coefs(), seed=65, periods=5)
+    gridr = np.concatenate([[0.0], np.logspace(-2, 2, 12)])
+    kind = "DensityClassicTopHat" if degree == 0 else "DensityClassicLinear"
+    print(f"case: {kind} target, 64 orbits x 5 periods, 3 strips per pane, axis ratios 0.8, 0.6", flush=True)
+    tgt = agama_amd.Target(type=kind, gridr=gridr, stripsPerPane=3, axisRatioY=0.8, axisRatioZ=0.6)
+    out = agama_amd.orbit(potential=gpu, ic=ic, time=T, targets=tgt, Omega=0.2)
+    ref = orc.orbits_target_classic(ic, T, gridr, degree, 3, 0.8, 0.6, Omega=0.2)
+    assert out.shape == ref.shape == (64, len(tgt))
+    empty = np.all(ref == 0, axis=1)
+    assert np.all(out[empty] == 0)
+    rel = np.max(np.abs(out - ref), axis=1)[~empty] / np.max(np.abs(ref), axis=1)[~empty]
+    print(f"per-orbit max rel diff: median {np.median(rel):.2e}, 90% {np.quantile(rel, 0.9):.2e}, max {rel.max():.2e}")
+    assert np.median(rel) <= 1e-6 and np.quantile(rel, 0.9) <= 1e-5

@@ -183,3 +183,65 @@ def test_kinem_shell_circular_and_radial_orbits(degree):
     assert np.all(vr2 >= 0) and np.sum(vr2 > 0) >= 4
     # zero-length orbits record nothing
     assert np.all(tri.orbits_target_kinem_shell(np.array([[3.0, 0, 0, 0, 0, 0]]), 0.0, gridr, degree) == 0)
+
+
+@pytest.mark.parametrize("degree", [0, 1])
+def test_classic_targets_closed_forms(degree):
+    """TargetDensityClassic<0/1> (galaxymodel_densitygrid.cpp:146-239, corner numbering :59-90).
+    A circular orbit in the equatorial plane of an axisymmetric potential, folded into the first
+    quadrant, moves uniformly in the angle psi = atan(|y|/|x|) through pane 0 (psi <= pi/4, strips
+    of pi/(4S) in atan(Y/X)) and pane 1 (psi > pi/4, strips in atan(X/Y)), always in the first row
+    of the other direction: each of the 2S top-hat cells holds 1/(2S) of the time; the trilinear
+    nodes along that arc are hat functions in psi, 1/(2S) each and 1/(4S) at the two ends (the
+    node at the diagonal shared by both panes through the adjacent-pane numbering), split
+    linearly between the two shells around r0.  Squeezed axes: on a radial orbit along x (or z)
+    the shells sit at radius r * axisX (axisZ), so the result equals that of a spherical grid
+    scaled by that axis.  The basis partitions unity inside the grid."""
+    from agama_amd import workloads
+    from oracle.pyoracle import classic_target_size
+    S = 3
+    gridr = np.array([0.0, 0.2, 0.5, 1.0, 2.0])
+    orc = OraclePotential(os.path.join(GOLDEN, "disk_bulge_halo_lmax12.coef"))
+    r0 = 0.7
+    _, g = orc.eval(np.array([[r0, 0.0, 0.0]]))
+    vc = math.sqrt(r0 * abs(g[0, 0]))
+    period = 2 * math.pi * r0 / vc
+    out = orc.orbits_target_classic(np.array([[r0, 0, 0, 0, vc, 0]]), 5 * period, gridr, degree, S,
+                                    accuracy=1e-10)[0]
+    assert len(out) == classic_target_size(degree, S, gridr) == (3 * S * (S + degree) + degree) * 4 + degree
+    expected = np.zeros(len(out))
+    if degree == 0:
+        vps, sh = 3 * S * S, 2
+        for k in range(S):
+            expected[sh * vps + k] = 1 / (2 * S)                    # pane 0, strip k, first row
+            expected[sh * vps + (k + S) * S] = 1 / (2 * S)          # pane 1, first strip, row k
+    else:
+        vps, offr = 3 * S * (S + 1) + 1, (r0 - 0.5) / (1.0 - 0.5)
+        w = np.full(2 * S + 1, 1 / (2 * S))
+        w[0] = w[-1] = 1 / (4 * S)
+        nodes = [k for k in range(S + 1)] + [(j + S) * (S + 1) for j in range(S - 1, -1, -1)]
+        for sh, frac in ((2, offr), (1, 1 - offr)):
+            for node, wk in zip(nodes, w):
+                expected[sh * vps + 1 + node] += frac * wk
+    # the samples (ten per step) misplace at most half a sample at each of the 8 crossings of a
+    # strip boundary per period: the time fractions agree to 4 / (samples per period)
+    nsteps = orc.orbits(np.array([[r0, 0, 0, 0, vc, 0]]), 5 * period, trajsize=1, accuracy=1e-10)["nsteps"][0]
+    bound = 4 / (10 * nsteps / 5)
+    assert np.max(np.abs(out - expected)) < bound, (np.max(np.abs(out - expected)), bound)
+    assert abs(out.sum() - 1) < 2e-6
+    # squeezed grid on radial orbits along the x and z axes of the triaxial model
+    coef = os.path.join(GOLDEN, "dehnen_triax_lmax8.coef")
+    tri = OraclePotential(coef)
+    qy, qz = 0.6, 0.8
+    ax = 1 / np.cbrt(qy * qz)
+    for ic, scale in (([3.0, 0, 0, 0, 0, 0], ax), ([0, 0, 2.0, 0, 0, 0], qz * ax)):
+        sq = tri.orbits_target_classic(np.array([ic]), 30.0, gridr, degree, S, qy, qz)[0]
+        sph = tri.orbits_target_classic(np.array([ic]), 30.0, gridr * scale, degree, S)[0]
+        assert np.count_nonzero(sq) >= 4
+        assert np.allclose(sq, sph, rtol=1e-6, atol=1e-9)
+    # partition of unity
+    ic, T = workloads.triaxial_dehnen_orbits(12, lambda p: tri.eval(p)[0], workloads.read_coef_file(coef),
+                                             seed=53, periods=4)
+    big = np.concatenate([[0.0], np.logspace(-3, 4, 36)])
+    res = tri.orbits_target_classic(ic, T, big, degree, 4, 0.7, 0.5)
+    assert np.max(np.abs(res.sum(axis=1) - 1)) < 2e-6
