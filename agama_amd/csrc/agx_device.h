@@ -422,6 +422,87 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const F& 
     gphi = inGrid ? (double)trDphi : NAN;
 }
 
+// ---- group evaluation: G lanes (an aligned group of the wavefront) share one point ----
+// Used for the longest orbits of a batch, which would otherwise set the batch's duration on a
+// single lane: lane g of the group evaluates the harmonic slots s = g, g + G, ... and the
+// partial Fourier sums are added across the group.  Every lane of the group ends with the same
+// bits (each butterfly step adds the partner's partial sum, and addition commutes).  All lanes of
+// the wavefront must execute it together.
+constexpr int AGX_GROUP = 8;
+
+template<int G>
+__device__ __forceinline__ double groupSum(double v)
+{
+#pragma unroll
+    for(int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// interiorEval with the harmonics split over the group; g = lane index within the group.
+// The combination differs from the serial Fourier sum only in the order of additions.
+template<int G, class F, class GP>
+__device__ __forceinline__ void interiorEvalGroup(const AgxMultipoleDev& P, const F& fetch,
+    double R, double z, double phi, double& pot, double& gR, double& gz, double& gphi,
+    GP gridR, GP gridT, int g)
+{
+    const double r = sqrt(R * R + z * z), logr = log(r), rplusRinv = 1. / (r + fabs(R)),
+        tau = R == 0 ? (z > 0 ? 1. : z < 0 ? -1. : 0.) : z * rplusRinv;
+    const int xi = binSearchD(logr, gridR, P.nr), yi = binSearchD(tau, gridT, P.nt);
+    const bool inGrid = xi >= 0 && xi < P.nr - 1 && yi >= 0 && yi < P.nt - 1;
+    const int xc = inGrid ? xi : 0, yc = inGrid ? yi : 0;
+    const double xlow = gridR[xc], xupp = gridR[xc + 1], ylow = gridT[yc], yupp = gridT[yc + 1];
+    const int offsetCorner = logr == xupp ? (tau == yupp ? 3 : 2) : (tau == yupp ? 1 : 0);
+    QuinticWT<double> wy, wx;
+    wy.init(tau, ylow, yupp);
+    wx.init(logr, xlow, xupp);
+    const int ns = P.ns, nt = P.nt, rec0 = (xc * nt + yc) * ns, zs = P.zeroSlot;
+    // the m = 0 value (log-unscaled) and the partial sums over this lane's other harmonics
+    double P0 = 0, d0r = 0, d0t = 0, Am = 0, Ar = 0, At = 0, Ap = 0;
+    if(g < ns) {
+        TrigRecT<double> tr;
+        tr.init(phi);
+        for(int s = g; s < ns; s += G) {
+            double Pm, dmr, dmt;
+            splineSlot(fetch, rec0 + s, ns, nt, offsetCorner, wy, wx, Pm, dmr, dmt);
+            if(s == zs) {
+                if(P.logScaling) {
+                    const double expX = exp(Pm), val = 1. / (P.invPhi0 - expX), dPhidX = val * val * expX;
+                    Pm = val;
+                    dmr *= dPhidX;
+                    dmt *= dPhidX;
+                }
+                P0 = Pm; d0r = dmr; d0t = dmt;
+            } else {
+                const int m = P.slotM[s];
+                tr.restart();
+                tr.advanceTo(m > 0 ? m : -m);
+                const double trig = m > 0 ? tr.c : tr.s, dtrig = double(-m) * (m > 0 ? tr.s : tr.c);
+                Am += Pm * trig;
+                Ar += dmr * trig;
+                At += dmt * trig;
+                Ap += Pm * dtrig;
+            }
+        }
+    }
+    P0 = groupSum<G>(P0); d0r = groupSum<G>(d0r); d0t = groupSum<G>(d0t);
+    Am = groupSum<G>(Am); Ar = groupSum<G>(Ar); At = groupSum<G>(At); Ap = groupSum<G>(Ap);
+    double trPot, trDr, trDth, trDphi;
+    if(P.logScaling) {   // the other harmonics are stored as ratios to the m = 0 term
+        trPot = P0 + P0 * Am;
+        trDr = d0r + (P0 * Ar + d0r * Am);
+        trDth = d0t + (P0 * At + d0t * Am);
+        trDphi = P0 * Ap;
+    } else {
+        trPot = P0 + Am; trDr = d0r + Ar; trDth = d0t + At; trDphi = Ap;
+    }
+    const double rinv = 1 / r, r2inv = rinv * rinv;
+    const double drdR = R * r2inv, drdz = z * r2inv, dthdR = -tau * rinv, dthdz = rinv - rplusRinv;
+    pot = inGrid ? trPot : NAN;
+    gR = inGrid ? trDr * drdR + trDth * dthdR : NAN;
+    gz = inGrid ? trDr * drdz + trDth * dthdz : NAN;
+    gphi = inGrid ? trDphi : NAN;
+}
+
 // PowerLawMultipole::evalCyl: extrapolation inside rmin / outside rmax
 __device__ __forceinline__ void powerLawEval(const AgxPowerLawDev& L, double R, double z, double phi,
     double& pot, double& gR, double& gz, double& gphi)
@@ -627,6 +708,33 @@ __device__ __forceinline__ void multipoleCar(const AgxMultipoleDev& P, const F& 
         gy = yr * dn + gy;
         gz = zr * dn + gz;
     }
+}
+
+// multipoleCar (without the NFW halo) for a group of G lanes sharing one point
+template<int G, class F, class GP>
+__device__ __forceinline__ void multipoleCarGroup(const AgxMultipoleDev& P, const F& fetch,
+    double x, double y, double z, double& pot, double& gx, double& gy, double& gz, GP gridR, GP gridT, int g)
+{
+    const double R = sqrt(x * x + y * y);
+    double dRdx, dRdy, dphidx, dphidy, phi;
+    if(R == 0) {
+        dRdx = dRdy = dphidx = dphidy = 1.;
+        phi = 0;
+    } else {
+        const double cosphi = x / R, sinphi = y / R;
+        dRdx = cosphi; dRdy = sinphi; dphidx = -sinphi / R; dphidy = cosphi / R;
+        phi = atan2A(y, x);
+    }
+    double gR, gzc, gphi;
+    const double rsq = R * R + z * z;
+    interiorEvalGroup<G>(P, fetch, R, z, phi, pot, gR, gzc, gphi, gridR, gridT, g);
+    if(rsq < P.rsqInner)
+        powerLawEval(P.inner, R, z, phi, pot, gR, gzc, gphi);
+    else if(rsq > P.rsqOuter)
+        powerLawEval(P.outer, R, z, phi, pot, gR, gzc, gphi);
+    gx = gR * dRdx + gphi * dphidx;
+    gy = gR * dRdy + gphi * dphidy;
+    gz = gzc;
 }
 
 }  // namespace agxd

@@ -311,11 +311,23 @@ __device__ __forceinline__ void finishOrbit(const AgxOrbitArgs& A, Lane& L, int 
     L.phase = PH_IDLE;
 }
 
+// group: this lane belongs to a group of AGX_GROUP lanes integrating one orbit together (all
+// lanes of the group call this at the same time, their states being identical)
 template<class Lane>
-__device__ __forceinline__ void fetchOrbit(const AgxOrbitArgs& A, Lane& L)
+__device__ __forceinline__ void fetchOrbit(const AgxOrbitArgs& A, Lane& L, bool group = false)
 {
-    const unsigned int q = atomicAdd(A.counter, 1u);
-    if((long)q >= A.n) {
+    long q;
+    if(group) {
+        const int lane = threadIdx.x & 63, lead = lane & ~(AGX_GROUP - 1);
+        unsigned int qq = 0;
+        if(lane == lead) qq = atomicAdd(A.groupCounter, 1u);
+        qq = __shfl(qq, lead, 64);
+        q = (long)qq < A.groupOrbits ? (long)qq : -1;
+    } else {
+        q = (long)atomicAdd(A.counter, 1u) + A.groupOrbits;
+        if(q >= A.n) q = -1;
+    }
+    if(q < 0) {
         L.oi = -1;
         L.phase = PH_IDLE;
         L.y[0] = 1; L.y[1] = L.y[2] = L.y[3] = L.y[4] = L.y[5] = 0;   // placeholder point

@@ -31,6 +31,13 @@ struct AgxOrbitArgs {
     int poolChunk = 0, maxChunks = 0, poolChunks = 0;
     // target plug-ins (agx_target.h), device memory, or null
     const struct AgxTargetList* targets = nullptr;
+    // group mode for the longest orbits: the wavefronts whose index is a multiple of groupStride
+    // (0: none) integrate the orbits order[0 .. groupOrbits) with AGX_GROUP lanes per orbit,
+    // taking them from groupCounter (zero on entry); the other wavefronts take order[groupOrbits
+    // ..) from counter
+    long groupOrbits = 0;
+    int groupStride = 0;
+    unsigned int* groupCounter = nullptr;
 };
 
 enum AgxPrecision { AGX_FP64 = 0, AGX_FP32_TABLES = 1 };

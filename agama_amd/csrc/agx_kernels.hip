@@ -103,7 +103,9 @@ __device__ __forceinline__ void rhsEval(const AgxMultipoleDev& P, vd2* stage, do
 // WAVES: wavefronts per SIMD the kernel is compiled for (register budget 512 / WAVES); both
 // variants exist so that the autotuner can choose at run time.
 // TGT: the variant that also feeds target plug-ins (agx_target.h)
-template<int MODE, int WAVES, bool NFW, bool TGT = false>
+// GRP: the variant with group mode (A.groupStride): some wavefronts integrate the longest orbits
+// with AGX_GROUP lanes per orbit, each lane evaluating a share of the harmonics
+template<int MODE, int WAVES, bool NFW, bool TGT = false, bool GRP = false>
 __global__ __launch_bounds__(OBS) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
 void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
 {
@@ -119,7 +121,10 @@ void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
     stageGrids(P, gridLds, gR, gT);
     Lane L;
     L.attach(lds + threadIdx.x);
-    fetchOrbit(A, L);
+    // wave-uniform: whether this wavefront works in group mode
+    const bool group = GRP && A.groupStride > 0 &&
+        (int)(blockIdx.x * (OBS / 64) + (threadIdx.x >> 6)) % A.groupStride == 0;
+    fetchOrbit(A, L, group);
     const double accRel = A.accuracy;
     unsigned int it = 0;
     while(true) {
@@ -135,8 +140,17 @@ void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
         double ca = 1, sa = 0;
         if(A.Omega != 0 && active) sincosA(A.Omega * (L.tb + L.toff), sa, ca);
         double pot, gx, gy, gz;
-        rhsEval<MODE, NFW>(P, stage, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa, L.y[2],
-            pot, gx, gy, gz, gR, gT);
+        if constexpr(GRP) {
+            if(group) {
+                FetchVec2 fg{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
+                multipoleCarGroup<AGX_GROUP>(P, fg, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa, L.y[2],
+                    pot, gx, gy, gz, gR, gT, (int)(threadIdx.x & (AGX_GROUP - 1)));
+            } else
+                rhsEval<MODE, NFW>(P, stage, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa, L.y[2],
+                    pot, gx, gy, gz, gR, gT);
+        } else
+            rhsEval<MODE, NFW>(P, stage, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa, L.y[2],
+                pot, gx, gy, gz, gR, gT);
         if(!run) continue;
         double f[6];
         f[0] = L.y[3]; f[1] = L.y[4]; f[2] = L.y[5];
@@ -144,7 +158,7 @@ void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
         f[4] = -gy * ca - gx * sa;
         f[5] = -gz;
         advancePhase(A, L, f, pot, accRel);
-        if(L.phase == PH_IDLE) fetchOrbit(A, L);
+        if(L.phase == PH_IDLE) fetchOrbit(A, L, group);
     }
 }
 
@@ -179,8 +193,10 @@ int agxOrbitKernelBlockSize() { return OBS; }
 
 namespace {
 typedef void (*OrbitKernelFn)(const AgxMultipoleDev*, AgxOrbitArgs);
-OrbitKernelFn orbitKernelFor(int precision, int fetch, int waves, bool nfw, bool targets = false)
+OrbitKernelFn orbitKernelFor(int precision, int fetch, int waves, bool nfw, bool targets = false, bool group = false)
 {
+    if(group)     // group mode: FP64, 16-byte per-lane fetch, 1 wave/SIMD, no NFW, no targets
+        return orbitKernel<AGX_FETCH_VEC2, 1, false, false, true>;
     if(targets)   // target plug-ins: FP64, 16-byte per-lane fetch, 1 wave/SIMD
         return nfw ? orbitKernel<AGX_FETCH_VEC2, 1, true, true> : orbitKernel<AGX_FETCH_VEC2, 1, false, true>;
     if(nfw)   // composite with an NFW halo: 16-byte per-lane fetch (or FP32 tables), 1 wave/SIMD
@@ -213,8 +229,12 @@ hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, i
     if(A.n <= 0) return hipSuccess;
     if(blockSize != OBS) return hipErrorInvalidValue;
     if(A.targets && precision == AGX_FP32_TABLES) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(orbitKernelFor(precision, fetch, waves, P.nfwMass != 0, A.targets != nullptr), dim3(numBlocks),
-        dim3(OBS), 0, stream, dP, A);
+    const bool group = A.groupStride > 0;
+    if(group && (precision != AGX_FP64 || fetch != AGX_FETCH_VEC2 || waves != 1 || P.nfwMass != 0 || A.targets ||
+                 A.poolCounter || !A.groupCounter || A.groupOrbits > A.n))
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(orbitKernelFor(precision, fetch, waves, P.nfwMass != 0, A.targets != nullptr, group),
+        dim3(numBlocks), dim3(OBS), 0, stream, dP, A);
     return hipGetLastError();
 }
 

@@ -77,6 +77,10 @@ def parse():
                          "target: orbits feeding a DensitySphHarm target")
     ap.add_argument("--method", choices=["dop853", "dprkn8"], default="dop853",
                     help="integrator of the FP64 Multipole path (the surrogate path uses DOP853)")
+    ap.add_argument("--groups", default=None,
+                    help="group mode of the FP64 DOP853 kernel, STRIDE[,ORBITS] (STRIDE 0: off): one "
+                         "wavefront in STRIDE integrates the first ORBITS orbits of the order with 8 lanes "
+                         "per orbit; default: the library's automatic choice (off at 1M orbits)")
     ap.add_argument("--train-steps", type=int, default=6000)
     ap.add_argument("--train-samples", type=float, default=20e6)
     ap.add_argument("--max-rel-de", type=float, default=1e-6)
@@ -196,6 +200,9 @@ def main():
     coef = os.path.join(ROOT, "tests", "golden", "dehnen_triax_lmax8.coef")
     pot = agama_amd.Potential(file=coef)
     n = args.orbits
+    if args.groups:
+        g = [int(v) for v in args.groups.split(",")]
+        assert lib.agx_set_orbit_groups(g[0], g[1] if len(g) > 1 else -1, 0) == 0
     ic, T = workloads.triaxial_dehnen_orbits(n, lambda p: pot.potential(p), pot.coefs(), seed=42, uneven=True)
     prec = agama_amd.PRECISION_FP64 if args.precision == "fp64" else agama_amd.PRECISION_FP32
 
@@ -346,7 +353,8 @@ def main():
                                f"periods, {args.method.upper()} accuracy 1e-8, final state + max dE/E <= 1e-6",
                    "orbits": n, "accepted_steps_per_step": steps_per_launch,
                    "attempts_per_step": attempts, "precision": args.precision,
-                   "path": "MLP surrogate with FP64 fallback" if surrogate else "FP64 Multipole"},
+                   "path": "MLP surrogate with FP64 fallback" if surrogate else "FP64 Multipole",
+                   "group_mode": args.groups or "automatic"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": ORBIT_TRAFFIC_BYTES if (n == 1_000_000 and args.precision == "fp64"

@@ -312,6 +312,15 @@ double agama_evalPotential(const agama_Potential* pot, const double pos[3], doub
    per SIMD the persistent kernel is compiled for (1: 512 registers per lane, 2: 256), and the
    current settings of agx_set_orbit_launch / agx_set_fetch / agx_set_orbit_waves. */
 int agx_set_orbit_waves(int waves);
+/* Group mode of the DOP853 kernel (no counterpart in the reference: a scheduling knob of this
+   library).  In batches of at least min_batch orbits (FP64, 16-byte fetch, 1 wave/SIMD, no NFW
+   halo, no targets, deviation vectors or every-step recording), one wavefront in `stride`
+   integrates the first `orbits` orbits of the processing order - the longest expected - with 8
+   lanes per orbit, each lane evaluating a share of the azimuthal harmonics; orbits = -1: one per
+   8 lanes of those wavefronts; stride = 0: off; stride = -1: automatic (every orbit in groups when
+   they fit, 4 for up to 2 orbits per lane, 8 for up to 4, off above).  Results differ from the
+   one-lane form only by the order of the additions in the Fourier sum.  Defaults: -1, -1, 65536. */
+int agx_set_orbit_groups(int stride, long orbits, long min_batch);
 int agx_get_orbit_config(int* num_blocks, int* fetch_eval, int* fetch_orbit, int* waves);
 
 /* One integration of a fixed batch of device arrays (arguments as agx_orbit_batch_device)
