@@ -114,3 +114,14 @@ def test_library_exports_every_declared_symbol():
     missing = sorted(names - exported)
     assert not missing, missing
     assert set(_native.EXPORTED) <= exported
+
+
+def test_orbit_group_settings_validation():
+    """agx_set_orbit_groups: host-side validation only (no GPU call); the defaults are restored"""
+    lib = _native.lib
+    assert lib.agx_set_orbit_groups(-2, -1, 0) != 0
+    assert "group-mode" in _native.last_error()
+    assert lib.agx_set_orbit_groups(4, -2, 0) != 0
+    assert lib.agx_set_orbit_groups(4, 100, -1) != 0
+    assert lib.agx_set_orbit_groups(0, -1, 0) == 0
+    assert lib.agx_set_orbit_groups(-1, -1, 65536) == 0
