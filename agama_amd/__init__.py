@@ -313,15 +313,15 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
         tlist = list(targets) if isinstance(targets, (list, tuple)) else [targets]
         if not all(isinstance(t, Target) for t in tlist) or not 1 <= len(tlist) <= 4:
             raise TypeError("Argument 'targets' must be a Target or a sequence of up to 4 Targets")
-        if surrogate or method_flag or precision != PRECISION_FP64:
-            raise AgamaAmdError("targets are collected with DOP853 and the FP64 Multipole only")
+        if surrogate or precision != PRECISION_FP64:
+            raise AgamaAmdError("targets are collected with the FP64 Multipole only")
         mats = [np.empty((n, len(t)), dtype=np.float32) for t in tlist]
         final_t, nsteps_t, status_t = np.empty((n, 6)), np.empty(n, dtype=np.int64), np.empty(n, dtype=np.int32)
         tptr = (ctypes.c_void_p * len(tlist))(*[t._h for t in tlist])
         optr = (ctypes.c_void_p * len(tlist))(*[m.ctypes.data for m in mats])
         check(lib.agx_orbit_target_batch(potential.handle, n, ptr(ic2), ptr(ts), ptr(tt), float(accuracy),
                                          float(Omega), maxsteps, len(tlist), tptr, optr, ptr(final_t), ptr(nsteps_t),
-                                         ptr(status_t), int(precision)), "orbit")
+                                         ptr(status_t), int(precision) | method_flag), "orbit")
         mats = [m[0] if single else m for m in mats]
         if trajsize is None and not obj and not der and not lyapunov:
             if isinstance(info, dict):

@@ -562,8 +562,8 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
     if(variational && (precision != AGX_PRECISION_FP64 || (flags & AGX_METHOD_DPRKN8)))
         return fail(AGX_EUNSUPPORTED, "deviation vectors are integrated with DOP853 in FP64 only");
     if(d_dev && !d_traj_offset) return fail(AGX_EINVAL, "deviation vectors need trajectory offsets");
-    if(d_targets && (variational || precision != AGX_PRECISION_FP64 || (flags & AGX_METHOD_DPRKN8)))
-        return fail(AGX_EUNSUPPORTED, "targets are collected with DOP853 in FP64, without deviation vectors");
+    if(d_targets && (variational || precision != AGX_PRECISION_FP64))
+        return fail(AGX_EUNSUPPORTED, "targets are collected in FP64, without deviation vectors");
     try {
         hipStream_t st = (hipStream_t)stream;
         void* work = nullptr;

@@ -131,6 +131,34 @@ __device__ __forceinline__ void sampleStepRKN(const AgxOrbitArgs& A, LaneRKN<STR
     }
 }
 
+// RuntimeFncTarget::processTimestep (galaxymodel_target.h:141-154) for an accepted RKN step:
+// ten equally spaced dense-output points (rotating frame), each weighted by a tenth of the step
+template<int STRIDE>
+__device__ __forceinline__ void targetStepRKN(const AgxOrbitArgs& A, const LaneRKN<STRIDE>& L, const double* kn)
+{
+    const AgxTargetList& TL = *A.targets;
+    const int NUM_SAMPLES_PER_STEP = 10;
+    const double timestep = L.h, substep = timestep / NUM_SAMPLES_PER_STEP;
+    for(int s = 0; s < NUM_SAMPLES_PER_STEP; s++) {
+        const double off = substep * (s + 0.5);
+        double d[6], p[6];
+        for(int c = 0; c < 3; c++) d[c] = denseOutRKN(L, kn, c, off);
+        if(TL.needVelocity)
+            for(int c = 3; c < 6; c++) d[c] = denseOutRKN(L, kn, c, off);
+        else
+            d[3] = d[4] = d[5] = 0;
+        toRotating(A.Omega, A.Omega * (off + L.tb), d, p);
+        for(int t = 0; t < TL.count; t++) {
+            const AgxTargetDev& T = TL.t[t];
+            targetAdd(T, T.cube + (long)L.oi * (T.numValues + 1), p, substep);
+        }
+    }
+    for(int t = 0; t < TL.count; t++) {
+        const AgxTargetDev& T = TL.t[t];
+        T.cube[(long)L.oi * (T.numValues + 1) + T.numValues] += timestep;
+    }
+}
+
 template<int STRIDE>
 __device__ __forceinline__ bool beginAttemptRKN(const AgxOrbitArgs& A, LaneRKN<STRIDE>& L)
 {
@@ -158,7 +186,8 @@ __device__ __forceinline__ void beginStepRKN(const AgxOrbitArgs& A, LaneRKN<STRI
 
 // one evaluation consumed: acceleration a (3) at the lane's evaluation point, potential pot;
 // accRel is the stepper's relative tolerance 10 accuracy^0.9 (math_ode.cpp:428-431)
-template<int STRIDE>
+// TGT: also feed the target plug-ins of A.targets at every accepted step
+template<int STRIDE, bool TGT = false>
 __device__ __forceinline__ void advanceRKN(const AgxOrbitArgs& A, LaneRKN<STRIDE>& L, const double* a,
     double pot, double accRel)
 {
@@ -291,6 +320,7 @@ __device__ __forceinline__ void advanceRKN(const AgxOrbitArgs& A, LaneRKN<STRIDE
             const double mde = L.f(F_MAXDE);
             if(de > mde || de != de) L.f(F_MAXDE) = de;
             sampleStepRKN(A, L, a);
+            if constexpr(TGT) targetStepRKN(A, L, a);
             L.f(F_LASTTB) = L.tb;
             L.f(F_LASTH) = L.h;
             if(L.S == 1) L.actual = 1;
@@ -335,8 +365,8 @@ __device__ __forceinline__ void rhsRKN(const AgxMultipoleDev& P, double x, doubl
     }
 }
 
-// MODE: AGX_FETCH_VEC2 (FP64 tables) or -1 (FP32 form)
-template<int MODE, bool NFW>
+// MODE: AGX_FETCH_VEC2 (FP64 tables) or -1 (FP32 form); TGT: the variant feeding target plug-ins
+template<int MODE, bool NFW, bool TGT = false>
 __global__ __launch_bounds__(RBS) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void orbitKernelRKN(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
 {
@@ -366,7 +396,7 @@ void orbitKernelRKN(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
         a[0] = -gx * ca + gy * sa;
         a[1] = -gy * ca - gx * sa;
         a[2] = -gz;
-        advanceRKN(A, L, a, pot, accRel);
+        advanceRKN<RBS, TGT>(A, L, a, pot, accRel);
         if(L.phase == PH_IDLE) fetchOrbit(A, L);
     }
 }
@@ -392,6 +422,12 @@ hipError_t agxLaunchOrbitRKN(const AgxMultipoleDev& P, const AgxMultipoleDev* dP
     if(A.n <= 0) return hipSuccess;
     const bool nfw = P.nfwMass != 0;
     const dim3 g(numBlocks), b(RBS);
+    if(A.targets) {   // target plug-ins: FP64 tables only
+        if(precision == AGX_FP32_TABLES) return hipErrorInvalidValue;
+        if(nfw) hipLaunchKernelGGL((orbitKernelRKN<AGX_FETCH_VEC2, true, true>), g, b, 0, stream, dP, A);
+        else hipLaunchKernelGGL((orbitKernelRKN<AGX_FETCH_VEC2, false, true>), g, b, 0, stream, dP, A);
+        return hipGetLastError();
+    }
     if(precision == AGX_FP32_TABLES) {
         if(nfw) hipLaunchKernelGGL((orbitKernelRKN<-1, true>), g, b, 0, stream, dP, A);
         else hipLaunchKernelGGL((orbitKernelRKN<-1, false>), g, b, 0, stream, dP, A);

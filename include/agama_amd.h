@@ -269,8 +269,9 @@ void agx_target_destroy(agx_target* t);
    src/galaxymodel_target.h:95-155), accumulated per orbit in FP64 on the device;
    out_targets[t][n * numValues] receives the datacube in float divided by the orbit's
    integration time (unchanged when it is 0).  out_final/out_nsteps/out_status as in
-   agx_orbit_batch (may be NULL).  DPRKN8, FP32 tables and deviation vectors are not combined
-   with targets (AGX_EUNSUPPORTED). */
+   agx_orbit_batch (may be NULL).  flags: AGX_METHOD_DPRKN8 selects DPRKN8 (its dense output
+   feeds the samples).  FP32 tables and deviation vectors are not combined with targets
+   (AGX_EUNSUPPORTED). */
 int agx_orbit_target_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int ntargets,
     const agx_target* const* targets, float* const* out_targets, double* out_final, long* out_nsteps,

@@ -3250,42 +3250,42 @@ int orc_orbit_steps(const orc_pot* p, long n, const double* ic, const double* ts
 }
 
 static int runTarget(const orc_pot* p, long n, const double* ic, const double* tstart, const double* ttotal,
-    double accuracy, double Omega, long maxNumSteps, const OrcTarget& T, float* out, int nthreads);
+    double accuracy, double Omega, long maxNumSteps, const OrcTarget& T, float* out, int nthreads, int method);
 
 int orc_orbit_target_sphharm(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int lmax, int mmax,
-    const double* gridr, int ngridr, float* out, int nthreads)
+    const double* gridr, int ngridr, float* out, int nthreads, int method)
 {
     OrcTargetSphHarm T;
     T.lmax = lmax; T.mmax = mmax;
     T.gridr.assign(ngridr > 0 && gridr[0] > 0 ? gridr : gridr + 1, gridr + ngridr);
-    return runTarget(p, n, ic, tstart, ttotal, accuracy, Omega, maxNumSteps, T, out, nthreads);
+    return runTarget(p, n, ic, tstart, ttotal, accuracy, Omega, maxNumSteps, T, out, nthreads, method);
 }
 
 int orc_orbit_target_cyl(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree, int mmax,
-    const double* gridR, int ngridR, const double* gridz, int ngridz, float* out, int nthreads)
+    const double* gridR, int ngridR, const double* gridz, int ngridz, float* out, int nthreads, int method)
 {
     OrcTargetCyl T;
     T.N = degree; T.mmax = mmax;
     T.gridR.assign(ngridR > 0 && gridR[0] > 0 ? gridR : gridR + 1, gridR + ngridR);
     T.gridz.assign(ngridz > 0 && gridz[0] > 0 ? gridz : gridz + 1, gridz + ngridz);
-    return runTarget(p, n, ic, tstart, ttotal, accuracy, Omega, maxNumSteps, T, out, nthreads);
+    return runTarget(p, n, ic, tstart, ttotal, accuracy, Omega, maxNumSteps, T, out, nthreads, method);
 }
 
 int orc_orbit_target_kinem_shell(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree,
-    const double* gridr, int ngridr, float* out, int nthreads)
+    const double* gridr, int ngridr, float* out, int nthreads, int method)
 {
     OrcTargetKinemShell T;
     T.N = degree;
     T.gridr.assign(gridr, gridr + ngridr);
-    return runTarget(p, n, ic, tstart, ttotal, accuracy, Omega, maxNumSteps, T, out, nthreads);
+    return runTarget(p, n, ic, tstart, ttotal, accuracy, Omega, maxNumSteps, T, out, nthreads, method);
 }
 
 int orc_orbit_target_classic(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree, int stripsPerPane,
-    const double* gridr, int ngridr, double axisRatioY, double axisRatioZ, float* out, int nthreads)
+    const double* gridr, int ngridr, double axisRatioY, double axisRatioZ, float* out, int nthreads, int method)
 {
     OrcTargetClassic T;
     T.N = degree; T.S = stripsPerPane;
@@ -3293,7 +3293,7 @@ int orc_orbit_target_classic(const orc_pot* p, long n, const double* ic, const d
     T.axisX = 1. / cbrt(axisRatioY * axisRatioZ);
     T.axisY = axisRatioY * T.axisX;
     T.axisZ = axisRatioZ * T.axisX;
-    return runTarget(p, n, ic, tstart, ttotal, accuracy, Omega, maxNumSteps, T, out, nthreads);
+    return runTarget(p, n, ic, tstart, ttotal, accuracy, Omega, maxNumSteps, T, out, nthreads, method);
 }
 
 // the B-spline basis of TargetKinemShell<degree> at points x: out[i*(ngridr+degree-1) + k]
@@ -3311,7 +3311,7 @@ void orc_bspline_values(int degree, const double* gridr, int ngridr, long n, con
 }
 
 static int runTarget(const orc_pot* p, long n, const double* ic, const double* tstart, const double* ttotal,
-    double accuracy, double Omega, long maxNumSteps, const OrcTarget& T, float* out, int nthreads)
+    double accuracy, double Omega, long maxNumSteps, const OrcTarget& T, float* out, int nthreads, int method)
 {
     g_err.clear();
     const int nv = T.numValues();
@@ -3327,7 +3327,7 @@ static int runTarget(const orc_pot* p, long n, const double* ic, const double* t
         sink.cube.assign(nv, 0.);
         try {
             integrateOneOrbit(*p, &ic[orb*6], tstart ? tstart[orb] : 0, ttotal[orb], 1, accuracy, Omega,
-                maxNumSteps, NULL, NULL, r, 0, nullptr, -1, &sink);
+                maxNumSteps, NULL, NULL, r, method, nullptr, -1, &sink);
         } catch(std::exception&) {
             fail = 1;
             continue;

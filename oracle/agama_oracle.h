@@ -96,23 +96,24 @@ int orc_orbit_steps(const orc_pot* p, long n, const double* ic, const double* ts
     const double* ttotal, const long* capacity, const long* traj_offset, double accuracy, double Omega,
     long maxNumSteps, int method, int der, double* out_time, double* out_traj, double* out_dev,
     long* out_rows, long* out_nsteps, int* out_status, int nthreads);
-/* agama.orbit(targets=[Target(type='DensitySphHarm', gridr, lmax, mmax)]) with DOP853:
+/* agama.orbit(targets=[Target(type='DensitySphHarm', gridr, lmax, mmax)]) with DOP853 (method 0) or
+   DPRKN8 (method 1):
    out[n * numValues] the normalised datacube in float (RuntimeFncTarget + TargetDensitySphHarm) */
 int orc_orbit_target_sphharm(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int lmax, int mmax,
-    const double* gridr, int ngridr, float* out, int nthreads);
+    const double* gridr, int ngridr, float* out, int nthreads, int method);
 /* the same with TargetDensityCylindrical<degree> (DensityCylindricalTopHat / Linear) */
 int orc_orbit_target_cyl(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree, int mmax,
-    const double* gridR, int ngridR, const double* gridz, int ngridz, float* out, int nthreads);
+    const double* gridR, int ngridR, const double* gridz, int ngridz, float* out, int nthreads, int method);
 /* the same with TargetDensityClassic<degree> (DensityClassicTopHat / Linear) */
 int orc_orbit_target_classic(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree, int stripsPerPane,
-    const double* gridr, int ngridr, double axisRatioY, double axisRatioZ, float* out, int nthreads);
+    const double* gridr, int ngridr, double axisRatioY, double axisRatioZ, float* out, int nthreads, int method);
 /* the same with TargetKinemShell<degree> (KinemShell: v_r^2 and v_t^2 on B-splines in r) */
 int orc_orbit_target_kinem_shell(const orc_pot* p, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int degree,
-    const double* gridr, int ngridr, float* out, int nthreads);
+    const double* gridr, int ngridr, float* out, int nthreads, int method);
 /* BsplineInterpolator1d<degree> basis at n points: out[n * (ngridr + degree - 1)] */
 void orc_bspline_values(int degree, const double* gridr, int ngridr, long n, const double* x, double* out);
 /* agama.Orbit on one recorded series (createOrbitObject + Orbit.__call__): out[m*6] rotating

@@ -57,17 +57,17 @@ def _load(name):
                                             _vp, _vp, _vp]),
         "orc_orbit_target_sphharm": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, ctypes.c_double,
                                                     ctypes.c_double, ctypes.c_long, ctypes.c_int, ctypes.c_int, _vp,
-                                                    ctypes.c_int, _vp, ctypes.c_int]),
+                                                    ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int]),
         "orc_orbit_target_cyl": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_long, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp,
-                                                ctypes.c_int, _vp, ctypes.c_int]),
+                                                ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int]),
         "orc_orbit_target_kinem_shell": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, ctypes.c_double,
                                                         ctypes.c_double, ctypes.c_long, ctypes.c_int, _vp,
-                                                        ctypes.c_int, _vp, ctypes.c_int]),
+                                                        ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int]),
         "orc_orbit_target_classic": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, ctypes.c_double,
                                                     ctypes.c_double, ctypes.c_long, ctypes.c_int, ctypes.c_int, _vp,
                                                     ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp,
-                                                    ctypes.c_int]),
+                                                    ctypes.c_int, ctypes.c_int]),
         "orc_bspline_values": (None, [ctypes.c_int, _vp, ctypes.c_int, ctypes.c_long, _vp, _vp]),
         "orc_ode_gl4": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                        _vp, _vp]),
@@ -306,8 +306,12 @@ def sphharm_target_size(lmax, mmax, gridr):
     return ((lmax // 2 + 1) * (mmax // 2 + 1) - (mmax // 2) * (mmax // 2 + 1) // 2) * ng + 1
 
 
+# integrator names of agama.orbit(method=...) -> the oracle's method codes
+_METHODS = {"dop853": 0, "dprkn8": 1}
+
+
 def _orbits_target_sphharm(self, ic, ttotal, gridr, lmax, mmax, tstart=None, accuracy=1e-8, Omega=0.0,
-                           maxNumSteps=int(1e8), nthreads=0):
+                           maxNumSteps=int(1e8), nthreads=0, method="dop853"):
     """agama.orbit(targets=[Target(type='DensitySphHarm', ...)]) as the oracle restates it:
     (n, numCoefs) float32"""
     ic = np.ascontiguousarray(ic, dtype=np.float64).reshape(-1, 6)
@@ -318,7 +322,7 @@ def _orbits_target_sphharm(self, ic, ttotal, gridr, lmax, mmax, tstart=None, acc
     gr = np.ascontiguousarray(gridr, dtype=np.float64)
     out = np.empty((n, sphharm_target_size(lmax, mmax, gr)), dtype=np.float32)
     rc = self.lib.orc_orbit_target_sphharm(self.h, n, _p(ic), _p(ts), _p(tt), accuracy, Omega, maxNumSteps,
-                                           int(lmax), int(mmax), _p(gr), len(gr), _p(out), nthreads)
+                                           int(lmax), int(mmax), _p(gr), len(gr), _p(out), nthreads, _METHODS[method])
     if rc:
         raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
     return out
@@ -334,7 +338,7 @@ def cyl_target_size(degree, mmax, gridR, gridz):
 
 
 def _orbits_target_cyl(self, ic, ttotal, gridR, gridz, mmax, degree, tstart=None, accuracy=1e-8, Omega=0.0,
-                       maxNumSteps=int(1e8), nthreads=0):
+                       maxNumSteps=int(1e8), nthreads=0, method="dop853"):
     """orbit(targets=[Target(type='DensityCylindricalTopHat' (degree 0) / 'DensityCylindricalLinear'
     (degree 1), ...)]) as the oracle restates it: (n, numCoefs) float32"""
     ic = np.ascontiguousarray(ic, dtype=np.float64).reshape(-1, 6)
@@ -346,7 +350,7 @@ def _orbits_target_cyl(self, ic, ttotal, gridR, gridz, mmax, degree, tstart=None
     gz = np.ascontiguousarray(gridz, dtype=np.float64)
     out = np.empty((n, cyl_target_size(degree, mmax, gR, gz)), dtype=np.float32)
     rc = self.lib.orc_orbit_target_cyl(self.h, n, _p(ic), _p(ts), _p(tt), accuracy, Omega, maxNumSteps, int(degree),
-                                       int(mmax), _p(gR), len(gR), _p(gz), len(gz), _p(out), nthreads)
+                                       int(mmax), _p(gR), len(gR), _p(gz), len(gz), _p(out), nthreads, _METHODS[method])
     if rc:
         raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
     return out
@@ -356,7 +360,7 @@ OraclePotential.orbits_target_cyl = _orbits_target_cyl
 
 
 def _orbits_target_kinem_shell(self, ic, ttotal, gridr, degree, tstart=None, accuracy=1e-8, Omega=0.0,
-                               maxNumSteps=int(1e8), nthreads=0):
+                               maxNumSteps=int(1e8), nthreads=0, method="dop853"):
     """orbit(targets=[Target(type='KinemShell', gridr, degree)]) as the oracle restates it:
     (n, 2 * (len(gridr) + degree - 1)) float32, v_r^2 coefficients then v_t^2"""
     ic = np.ascontiguousarray(ic, dtype=np.float64).reshape(-1, 6)
@@ -367,7 +371,7 @@ def _orbits_target_kinem_shell(self, ic, ttotal, gridr, degree, tstart=None, acc
     gr = np.ascontiguousarray(gridr, dtype=np.float64)
     out = np.empty((n, 2 * (len(gr) + degree - 1)), dtype=np.float32)
     rc = self.lib.orc_orbit_target_kinem_shell(self.h, n, _p(ic), _p(ts), _p(tt), accuracy, Omega, maxNumSteps,
-                                               int(degree), _p(gr), len(gr), _p(out), nthreads)
+                                               int(degree), _p(gr), len(gr), _p(out), nthreads, _METHODS[method])
     if rc:
         raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
     return out
@@ -382,7 +386,7 @@ def classic_target_size(degree, strips, gridr):
 
 
 def _orbits_target_classic(self, ic, ttotal, gridr, degree, stripsPerPane=2, axisRatioY=1.0, axisRatioZ=1.0,
-                           tstart=None, accuracy=1e-8, Omega=0.0, maxNumSteps=int(1e8), nthreads=0):
+                           tstart=None, accuracy=1e-8, Omega=0.0, maxNumSteps=int(1e8), nthreads=0, method="dop853"):
     """orbit(targets=[Target(type='DensityClassicTopHat' (degree 0) / 'DensityClassicLinear' (degree 1),
     ...)]) as the oracle restates it: (n, numCoefs) float32"""
     ic = np.ascontiguousarray(ic, dtype=np.float64).reshape(-1, 6)
@@ -394,7 +398,7 @@ def _orbits_target_classic(self, ic, ttotal, gridr, degree, stripsPerPane=2, axi
     out = np.empty((n, classic_target_size(degree, stripsPerPane, gr)), dtype=np.float32)
     rc = self.lib.orc_orbit_target_classic(self.h, n, _p(ic), _p(ts), _p(tt), accuracy, Omega, maxNumSteps,
                                            int(degree), int(stripsPerPane), _p(gr), len(gr), float(axisRatioY),
-                                           float(axisRatioZ), _p(out), nthreads)
+                                           float(axisRatioZ), _p(out), nthreads, _METHODS[method])
     if rc:
         raise RuntimeError("oracle orbit: " + self.lib.orc_last_error().decode())
     return out

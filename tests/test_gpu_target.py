@@ -65,7 +65,32 @@ def test_target_outputs_and_edge_cases(setup):
     one = agama_amd.orbit(potential=gpu, ic=ic[0], time=T[0], targets=t1)
     assert one.shape == (len(t1),) and np.array_equal(one, a[0])
     with pytest.raises(agama_amd.AgamaAmdError):
-        agama_amd.orbit(potential=gpu, ic=ic, time=T, targets=t1, method="dprkn8")
+        agama_amd.orbit(potential=gpu, ic=ic, time=T, targets=t1, precision=agama_amd.PRECISION_FP32)
+
+
+@pytest.mark.parametrize("kind", ["DensitySphHarm", "KinemShell"])
+def test_targets_with_dprkn8_match_oracle(setup, kind):
+    import agama_amd
+    from agama_amd import workloads
+    gpu, orc = setup
+    ic, T = workloads.triaxial_dehnen_orbits(64, lambda p: orc.eval(p)[0], gpu.coefs(), seed=66, periods=5)
+    print(f"case: {kind} target with DPRKN8, 64 orbits x 5 periods, Omega=0.2", flush=True)
+    if kind == "DensitySphHarm":
+        tgt = agama_amd.Target(type=kind, gridr=GRIDR, lmax=6, mmax=4)
+        ref = orc.orbits_target_sphharm(ic, T, GRIDR, 6, 4, Omega=0.2, method="dprkn8")
+    else:
+        gridr = np.concatenate([[0.0], np.logspace(-2, 3, 14)])
+        tgt = agama_amd.Target(type=kind, gridr=gridr, degree=2)
+        ref = orc.orbits_target_kinem_shell(ic, T, gridr, 2, Omega=0.2, method="dprkn8")
+    out = agama_amd.orbit(potential=gpu, ic=ic, time=T, targets=tgt, Omega=0.2, method="dprkn8")
+    assert out.shape == ref.shape == (64, len(tgt))
+    rel = np.max(np.abs(out - ref), axis=1) / np.max(np.abs(ref), axis=1)
+    print(f"per-orbit max rel diff: median {np.median(rel):.2e}, 90% {np.quantile(rel, 0.9):.2e}, max {rel.max():.2e}")
+    assert np.median(rel) <= 1e-6 and np.quantile(rel, 0.9) <= 1e-5
+    # the two integrators sample the same orbits: their target values agree to the integration error
+    dop = agama_amd.orbit(potential=gpu, ic=ic, time=T, targets=tgt, Omega=0.2)
+    d = np.max(np.abs(out - dop), axis=1) / np.max(np.abs(dop), axis=1)
+    assert np.median(d) < 1e-2
 
 
 @pytest.mark.parametrize("degree", [0, 1])

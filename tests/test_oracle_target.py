@@ -245,3 +245,33 @@ def test_classic_targets_closed_forms(degree):
     big = np.concatenate([[0.0], np.logspace(-3, 4, 36)])
     res = tri.orbits_target_classic(ic, T, big, degree, 4, 0.7, 0.5)
     assert np.max(np.abs(res.sum(axis=1) - 1)) < 2e-6
+
+
+def test_targets_with_dprkn8_closed_forms():
+    """the same target restatements fed by DPRKN8's dense output (any integrator drives a
+    RuntimeFncTarget): the circular-orbit values and the partition of unity hold as with DOP853"""
+    from agama_amd import workloads
+    orc = OraclePotential(os.path.join(GOLDEN, "disk_bulge_halo_lmax12.coef"))
+    r0 = 0.7
+    _, g = orc.eval(np.array([[r0, 0.0, 0.0]]))
+    vc = math.sqrt(r0 * abs(g[0, 0]))
+    period = 2 * math.pi * r0 / vc
+    gridr = np.array([0.2, 0.5, 1.0, 2.0])
+    out = orc.orbits_target_sphharm(np.array([[r0, 0, 0, 0, vc, 0]]), 5 * period, gridr, 4, 2, accuracy=1e-10,
+                                    method="dprkn8")[0]
+    offr = (r0 - 0.5) / 0.5
+    expected = np.zeros(len(out))
+    for blk, val in enumerate([1.0, -math.sqrt(5) / 2, 9 / 8, 0.0, 0.0]):
+        expected[1 + blk * 4 + 1] = val * (1 - offr)
+        expected[1 + blk * 4 + 2] = val * offr
+    assert np.max(np.abs(out - expected)) < 2e-6
+    kin = orc.orbits_target_kinem_shell(np.array([[r0, 0, 0, 0, vc, 0]]), 5 * period, np.array([0.0, 0.5, 1.0, 2.0]),
+                                        1, accuracy=1e-10, method="dprkn8")[0]
+    assert np.max(np.abs(kin[:4])) < 1e-5 * vc ** 2 and abs(kin[4:].sum() - vc ** 2) < 1e-5 * vc ** 2
+    coef = os.path.join(GOLDEN, "dehnen_triax_lmax8.coef")
+    tri = OraclePotential(coef)
+    ic, T = workloads.triaxial_dehnen_orbits(8, lambda p: tri.eval(p)[0], workloads.read_coef_file(coef),
+                                             seed=54, periods=3)
+    big = np.concatenate([[0.0], np.logspace(-3, 4, 36)])
+    res = tri.orbits_target_classic(ic, T, big, 1, 3, method="dprkn8")
+    assert np.max(np.abs(res.sum(axis=1) - 1)) < 2e-6
