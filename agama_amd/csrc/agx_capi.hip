@@ -1874,13 +1874,15 @@ int agx_surrogate_orbit_batch_device(const agx_surrogate* s, long n, const doubl
             const int pblocks = (int)std::min<long>((np + 63) / 64, (long)cus * agxOrbitSurrogateBlocksPerCU());
             HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
             HIPCHECK(agxLaunchFillU32(nfb, 1, 0u, st));
-            e = agxLaunchOrbitSurrogate(pot->ddev, s->base->ddev, s->dM, PA, SA, pblocks, st);
+            AgxSurrogateOrbitArgs SP = SA;
+            SP.abortAt = (unsigned int)((np + 1) / 2);   // the routing decision below is then known
+            e = agxLaunchOrbitSurrogate(pot->ddev, s->base->ddev, s->dM, PA, SP, pblocks, st);
             unsigned int pf = 0;
             if(e == hipSuccess) e = hipMemcpyAsync(&pf, nfb, sizeof(pf), hipMemcpyDeviceToHost, st);
             if(e == hipSuccess) e = hipStreamSynchronize(st);
             (void)hipFreeAsync(pw, st);
             if(e != hipSuccess) { release(); return fail(AGX_EDEVICE, std::string("surrogate pilot: ") + hipGetErrorString(e)); }
-            g_lastPilot[0] = np; g_lastPilot[1] = pf;
+            g_lastPilot[0] = np; g_lastPilot[1] = pf;   // pf stops growing at the decision
             routeAll = 2 * (long)pf >= np;
         }
         unsigned int nf = 0;

@@ -50,6 +50,11 @@ __global__ __launch_bounds__(SBS) void orbitSurrogateKernel(const AgxMultipoleDe
     while(true) {
         const bool active = L.phase != PH_IDLE;
         if(!__any(active)) break;
+        // a pilot launch ends as soon as its decision is known (enough orbits have fallen back);
+        // the count is read with a device-scope atomic load, every 12 iterations
+        if(SA.abortAt && it % 12 == 0 &&
+           __any(__hip_atomic_load(SA.nfallback, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= SA.abortAt))
+            break;
         const bool run = active && !(L.phase == PH_S2 && it % 12 != 0);
         it++;
         if(!__any(run)) continue;

@@ -52,6 +52,7 @@ struct AgxSurrogateOrbitArgs {
     double rmin, rmax;        // ... or when an evaluation point leaves [rmin, rmax]
     int* fallback;            // orbit indices handed over to the FP64 kernel
     unsigned int* nfallback;  // their count (zero on entry)
+    unsigned int abortAt = 0; // > 0: stop the whole launch once nfallback reaches it (pilot runs)
 };
 int agxOrbitSurrogateBlocksPerCU();
 hipError_t agxLaunchOrbitSurrogate(const AgxMultipoleDev* dP, const AgxMultipoleDev* dBase, const AgxMlpDev* dM,

@@ -396,8 +396,10 @@ int agx_surrogate_orbit_batch_device(const agx_surrogate* s, long n, const doubl
    surrogate): for n >= 8192 a pilot of every 64th orbit runs on the surrogate for at most 200
    steps first; when half of the pilot or more already falls back, every orbit of the batch is
    integrated with the FP64 Multipole directly (each counted and marked as a fallback) rather
-   than twice.  enable = 0 turns the pilot off (default on); last_pilot reports the last batch's
-   pilot size and fallbacks on this thread (0, 0 without a pilot). */
+   than twice.  The pilot launch ends as soon as half of its orbits have fallen back (the
+   decision is then known).  enable = 0 turns the pilot off (default on); last_pilot reports the
+   last batch's pilot size and the fallbacks counted until the pilot ended, on this thread (0, 0
+   without a pilot). */
 int agx_set_surrogate_pilot(int enable);
 int agx_surrogate_last_pilot(long* out_pilot_orbits, long* out_pilot_fallbacks);
 /* the training GEMM on host arrays: C[M][N] = sum_k A(m,k) B(k,n), bf16 bit patterns,
