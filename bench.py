@@ -81,6 +81,8 @@ def parse():
                     help="group mode of the FP64 DOP853 kernel, STRIDE[,ORBITS] (STRIDE 0: off): one "
                          "wavefront in STRIDE integrates the first ORBITS orbits of the order with 8 lanes "
                          "per orbit; default: the library's automatic choice (off at 1M orbits)")
+    ap.add_argument("--fetch", choices=["scalar", "vec2", "coop"], default=None,
+                    help="record gather of the FP64 orbit kernel (default: the library's, vec2)")
     ap.add_argument("--train-steps", type=int, default=6000)
     ap.add_argument("--train-samples", type=float, default=20e6)
     ap.add_argument("--max-rel-de", type=float, default=1e-6)
@@ -200,6 +202,8 @@ def main():
     coef = os.path.join(ROOT, "tests", "golden", "dehnen_triax_lmax8.coef")
     pot = agama_amd.Potential(file=coef)
     n = args.orbits
+    if args.fetch:
+        assert lib.agx_set_fetch(1, {"scalar": 0, "vec2": 1, "coop": 2}[args.fetch]) == 0
     if args.groups:
         g = [int(v) for v in args.groups.split(",")]
         assert lib.agx_set_orbit_groups(g[0], g[1] if len(g) > 1 else -1, 0) == 0
