@@ -14,8 +14,8 @@ import numpy as np
 from . import _native
 from ._native import AgamaAmdError, check, lib, ptr
 
-__all__ = ["Potential", "Density", "orbit", "Surrogate", "AgamaAmdError", "PRECISION_FP64",
-           "PRECISION_FP32"]
+__all__ = ["Potential", "Density", "orbit", "Surrogate", "Target", "Orbit", "AgamaAmdError", "PRECISION_FP64",
+           "PRECISION_FP32", "set_orbit_groups"]
 
 PRECISION_FP64 = 0
 PRECISION_FP32 = 1
@@ -470,6 +470,17 @@ def _trajectory_outputs(n, single, sz, times, traj, dev, out_dtype, separateTime
     if der:
         return (res[0], darr[0]) if single else (res, darr)
     return res[0] if single else res
+
+
+def set_orbit_groups(stride=-1, orbits=-1, min_batch=65536):
+    """Group mode of the DOP853 orbit kernel (no counterpart in the reference: a scheduling knob
+    of this library).  In batches of at least min_batch orbits, one wavefront in `stride`
+    integrates the first `orbits` orbits of the processing order (the longest expected) with 8
+    lanes per orbit, each lane evaluating a share of the azimuthal harmonics.  stride=-1: chosen
+    from the orbits per lane of the launch; 0: off; orbits=-1: one per 8 lanes of those
+    wavefronts.  Results differ from the one-lane form only in the order of the additions of the
+    Fourier sum.  The setting is process-wide."""
+    check(lib.agx_set_orbit_groups(int(stride), int(orbits), int(min_batch)), "set_orbit_groups")
 
 
 class Target:

@@ -125,3 +125,7 @@ def test_orbit_group_settings_validation():
     assert lib.agx_set_orbit_groups(4, 100, -1) != 0
     assert lib.agx_set_orbit_groups(0, -1, 0) == 0
     assert lib.agx_set_orbit_groups(-1, -1, 65536) == 0
+    import agama_amd
+    with pytest.raises(agama_amd.AgamaAmdError):
+        agama_amd.set_orbit_groups(stride=-3)
+    agama_amd.set_orbit_groups()
