@@ -65,7 +65,7 @@ def parse():
                     help="size of the CPU-baseline sample (0 disables)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--mode", choices=["orbit", "fp64", "eval", "hess", "mlp", "copy", "orbitobj", "construct",
-                                       "target"],
+                                       "target", "var"],
                     default="orbit",
                     help="orbit: surrogate and FP64 paths (default); fp64: the FP64 path alone; "
                          "eval: potential + force at 1M points drawn from the density; hess: the same "
@@ -74,7 +74,8 @@ def parse():
                          "device-to-device copy, the reference for HBM byte counters; orbitobj: "
                          "orbit(dtype=object) - every step recorded, splines built and evaluated; "
                          "construct: Multipole from a density (Poisson solver) on the device vs the host; "
-                         "target: orbits feeding a DensitySphHarm target")
+                         "target: orbits feeding a DensitySphHarm target; var: orbits with the six "
+                         "deviation vectors (der=True) and with Lyapunov exponents")
     ap.add_argument("--method", choices=["dop853", "dprkn8"], default="dop853",
                     help="integrator of the FP64 Multipole path (the surrogate path uses DOP853)")
     ap.add_argument("--groups", default=None,
@@ -193,6 +194,8 @@ def main():
         return bench_construct(args)
     if args.mode == "target":
         return bench_target(args)
+    if args.mode == "var":
+        return bench_var(args)
     import numpy as np
     import torch
     import agama_amd
@@ -511,6 +514,62 @@ def bench_target(args):
            "config": {"workload": f"{n} orbits x 10 circular periods, DOP853 accuracy 1e-8, DensitySphHarm target "
                                   "(16 radial nodes, lmax=6, mmax=4: 145 values per orbit)",
                       "accepted_steps": steps, "without_target_ms": plain * 1e3},
+           "roofline": None, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+def bench_var(args):
+    """orbit(der=True) and orbit(lyapunov=True): 100k orbits x 10 circular periods, DOP853
+    accuracy 1e-8; der=True integrates the six deviation vectors with the GL4 variational stepper
+    (four Hessian evaluations per accepted step) and returns them at the end (trajsize=2);
+    value = accepted steps/s of the der=True call (host arrays in, results out), beside the
+    lyapunov=True call and the plain orbits"""
+    import numpy as np
+    import agama_amd
+    from agama_amd import workloads
+    coef = os.path.join(ROOT, "tests", "golden", "dehnen_triax_lmax8.coef")
+    pot = agama_amd.Potential(file=coef)
+    n = 100_000 if args.orbits == 1_000_000 else args.orbits
+    ic, T = workloads.triaxial_dehnen_orbits(n, lambda p: pot.potential(p), pot.coefs(), seed=42, periods=10)
+    info = {}
+    for _ in range(args.warmup):
+        agama_amd.orbit(potential=pot, ic=ic, time=T, trajsize=2, der=True)
+        print("var: warmup step done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(max(args.steps, 1)):
+        agama_amd.orbit(potential=pot, ic=ic, time=T, trajsize=2, der=True, info=info)
+    wall = (time.perf_counter() - t0) / max(args.steps, 1)
+    steps = int(info["nsteps"].sum())
+    t0 = time.perf_counter()
+    lyap = agama_amd.orbit(potential=pot, ic=ic, time=T, lyapunov=True)
+    lyap_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    agama_amd.orbit(potential=pot, ic=ic, time=T, info={})
+    plain = time.perf_counter() - t0
+    chaotic = float(np.mean(np.asarray(lyap)[:, 0] > 0))
+    print(f"var: {n} orbits, {steps} steps; der=True {wall*1e3:.1f} ms, lyapunov=True {lyap_s*1e3:.1f} ms, "
+          f"plain {plain*1e3:.1f} ms; chaotic fraction {chaotic:.3f}", file=sys.stderr, flush=True)
+    cpu = None
+    if args.cpu_orbits > 0:
+        from oracle import pyoracle
+        threads = args.cpu_threads or os.cpu_count()
+        fast = pyoracle.OraclePotential(coef, library=pyoracle.fast())
+        m = min(40000, n)
+        t0 = time.perf_counter()
+        ref = fast.orbits_var(ic[:m], T[:m], trajsize=2, nthreads=threads)
+        dt = time.perf_counter() - t0
+        cpu = {"value": int(np.sum(ref["nsteps"])) / dt, "unit": "orbit-steps/s", "cores": threads, "kind": "port",
+               "sample": f"first {m} of the {n} orbits with der=True, C++ OpenMP restatement (-O2, {threads} "
+                         f"threads), {dt:.1f} s"}
+        print(f"cpu baseline: {cpu['value']/1e6:.3f} Msteps/s ({dt:.1f} s)", file=sys.stderr, flush=True)
+    out = {"metric": "orbit-steps/s with the six deviation vectors (der=True, triaxial Dehnen Multipole lmax=8)",
+           "value": steps / wall, "unit": "orbit-steps/s", "n_gpus": 1, "steps": max(args.steps, 1),
+           "warmup": args.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{n} orbits x 10 circular periods, DOP853 accuracy 1e-8, der=True with "
+                                  "trajsize=2; lyapunov=True on the same orbits",
+                      "accepted_steps": steps, "lyapunov_ms": lyap_s * 1e3, "plain_ms": plain * 1e3,
+                      "chaotic_fraction": chaotic},
            "roofline": None, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
