@@ -319,6 +319,19 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
         final_t, nsteps_t, status_t = np.empty((n, 6)), np.empty(n, dtype=np.int64), np.empty(n, dtype=np.int32)
         tptr = (ctypes.c_void_p * len(tlist))(*[t._h for t in tlist])
         optr = (ctypes.c_void_p * len(tlist))(*[m.ctypes.data for m in mats])
+        if trajsize is not None and not obj and not der and not lyapunov and np.all(sz > 0):
+            # regular trajectory samples recorded in the same integration as the targets
+            rows = int(sz.sum())
+            times, traj = np.empty(rows), np.empty((rows, 6))
+            check(lib.agx_orbit_target_traj_batch(potential.handle, n, ptr(ic2), ptr(ts), ptr(tt), ptr(sz),
+                                                  float(accuracy), float(Omega), maxsteps, len(tlist), tptr, optr,
+                                                  ptr(times), ptr(traj), ptr(final_t), ptr(nsteps_t),
+                                                  ptr(status_t), int(precision) | method_flag), "orbit")
+            mats = [m[0] if single else m for m in mats]
+            if isinstance(info, dict):
+                info.update(nsteps=nsteps_t, status=status_t, final=final_t)
+            out = _trajectory_outputs(n, single, sz, times, traj, None, out_dtype, separateTime)
+            return tuple(mats) + (out if isinstance(out, tuple) else (out,))
         check(lib.agx_orbit_target_batch(potential.handle, n, ptr(ic2), ptr(ts), ptr(tt), float(accuracy),
                                          float(Omega), maxsteps, len(tlist), tptr, optr, ptr(final_t), ptr(nsteps_t),
                                          ptr(status_t), int(precision) | method_flag), "orbit")

@@ -1401,10 +1401,24 @@ int agx_target_num_values(const agx_target* t) { return t ? t->numValues : -1; }
 
 void agx_target_destroy(agx_target* t) { delete t; }
 
+int agx_orbit_target_traj_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, double accuracy, double Omega, long maxNumSteps, int ntargets,
+    const agx_target* const* targets, float* const* out_targets, double* out_time, double* out_traj,
+    double* out_final, long* out_nsteps, int* out_status, int flags);
+
 int agx_orbit_target_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int ntargets,
     const agx_target* const* targets, float* const* out_targets, double* out_final, long* out_nsteps,
     int* out_status, int flags)
+{
+    return agx_orbit_target_traj_batch(pot, n, ic, tstart, ttotal, nullptr, accuracy, Omega, maxNumSteps, ntargets,
+        targets, out_targets, nullptr, nullptr, out_final, out_nsteps, out_status, flags);
+}
+
+int agx_orbit_target_traj_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, double accuracy, double Omega, long maxNumSteps, int ntargets,
+    const agx_target* const* targets, float* const* out_targets, double* out_time, double* out_traj,
+    double* out_final, long* out_nsteps, int* out_status, int flags)
 {
     g_err.clear();
     if(!pot) return fail(AGX_EINVAL, "NULL potential");
@@ -1413,6 +1427,9 @@ int agx_orbit_target_batch(const agx_potential* pot, long n, const double* ic, c
         return fail(AGX_EINVAL, "between 1 and 4 targets with their outputs are required");
     for(int t = 0; t < ntargets; t++)
         if(!targets[t] || !out_targets[t]) return fail(AGX_EINVAL, "NULL target or output");
+    if(trajsize)
+        for(long i = 0; i < n; i++)
+            if(trajsize[i] < 1) return fail(AGX_EINVAL, "trajsize must be >= 1 with targets");
     if(n == 0) return AGX_OK;
     try {
         HIPCHECK(hipSetDevice(pot->device));
@@ -1441,14 +1458,23 @@ int agx_orbit_target_batch(const agx_potential* pot, long n, const double* ic, c
         DevBuf<double> dic(n * 6), dts(tstart ? n : 0), dtt(n), dfin(out_final ? n * 6 : 0);
         DevBuf<int> dsz(n), dst(out_status ? n : 0);
         DevBuf<long> dns(out_nsteps ? n : 0);
-        std::vector<int> ones(n, 1);
+        // trajectory rows as in agx_orbit_batch: trajsize[i] rows per orbit, in orbit order
+        std::vector<int> sz(trajsize ? trajsize : nullptr, trajsize ? trajsize + n : nullptr);
+        if(!trajsize) sz.assign(n, 1);
+        std::vector<long> off(n);
+        long rows = 0;
+        for(long i = 0; i < n; i++) { off[i] = rows; rows += sz[i]; }
+        const bool traj = trajsize && (out_time || out_traj);
+        DevBuf<long> doff(traj ? n : 0);
+        DevBuf<double> dtime(traj && out_time ? rows : 0), dtraj(traj && out_traj ? rows * 6 : 0);
         HIPCHECK(hipMemcpy(dic.p, ic, n * 48, hipMemcpyHostToDevice));
         if(tstart) HIPCHECK(hipMemcpy(dts.p, tstart, n * 8, hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(dtt.p, ttotal, n * 8, hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(dsz.p, ones.data(), n * 4, hipMemcpyHostToDevice));
-        int rc = orbitBatchDevice(pot, n, dic.p, dts.p, dtt.p, dsz.p, nullptr, nullptr, accuracy, Omega, maxNumSteps,
-            nullptr, nullptr, dfin.p, nullptr, dns.p, nullptr, nullptr, nullptr, dst.p, flags, nullptr, LyapDev(),
-            RecordPool(), dTL.p);
+        HIPCHECK(hipMemcpy(dsz.p, sz.data(), n * 4, hipMemcpyHostToDevice));
+        if(traj) HIPCHECK(hipMemcpy(doff.p, off.data(), n * 8, hipMemcpyHostToDevice));
+        int rc = orbitBatchDevice(pot, n, dic.p, dts.p, dtt.p, dsz.p, traj ? doff.p : nullptr, nullptr, accuracy,
+            Omega, maxNumSteps, dtime.p, dtraj.p, dfin.p, nullptr, dns.p, nullptr, nullptr, nullptr, dst.p, flags,
+            nullptr, LyapDev(), RecordPool(), dTL.p);
         if(rc) return rc;
         HIPCHECK(hipDeviceSynchronize());
         // ~RuntimeFncTarget (galaxymodel_target.h:129-137): the datacube in storage precision,
@@ -1470,6 +1496,8 @@ int agx_orbit_target_batch(const agx_potential* pot, long n, const double* ic, c
         if(out_final) HIPCHECK(hipMemcpy(out_final, dfin.p, n * 48, hipMemcpyDeviceToHost));
         if(out_nsteps) HIPCHECK(hipMemcpy(out_nsteps, dns.p, n * 8, hipMemcpyDeviceToHost));
         if(out_status) HIPCHECK(hipMemcpy(out_status, dst.p, n * 4, hipMemcpyDeviceToHost));
+        if(traj && out_time) HIPCHECK(hipMemcpy(out_time, dtime.p, rows * 8, hipMemcpyDeviceToHost));
+        if(traj && out_traj) HIPCHECK(hipMemcpy(out_traj, dtraj.p, rows * 48, hipMemcpyDeviceToHost));
         return AGX_OK;
     } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
 }

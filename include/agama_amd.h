@@ -276,6 +276,14 @@ int agx_orbit_target_batch(const agx_potential* pot, long n, const double* ic, c
     const double* ttotal, double accuracy, double Omega, long maxNumSteps, int ntargets,
     const agx_target* const* targets, float* const* out_targets, double* out_final, long* out_nsteps,
     int* out_status, int flags);
+/* The same with trajectories recorded in the same integration (interface_python.cpp:7991-7994:
+   the target matrices, then the trajectory outputs): trajsize[n] >= 1 rows per orbit at regular
+   intervals, stored as in agx_orbit_batch (out_time[rows], out_traj[rows * 6], orbit after orbit;
+   either may be NULL). */
+int agx_orbit_target_traj_batch(const agx_potential* pot, long n, const double* ic, const double* tstart,
+    const double* ttotal, const int* trajsize, double accuracy, double Omega, long maxNumSteps, int ntargets,
+    const agx_target* const* targets, float* const* out_targets, double* out_time, double* out_traj,
+    double* out_final, long* out_nsteps, int* out_status, int flags);
 
 /* Launch geometry of the orbit kernel: number of workgroups (0 = one full wave of resident
    workgroups on every CU) and threads per workgroup (0 or 256, the only size it is built for).

@@ -58,6 +58,10 @@ def test_target_outputs_and_edge_cases(setup):
     m, times, traj = agama_amd.orbit(potential=gpu, ic=ic, time=T, targets=t1, trajsize=5, separateTime=True,
                                      dtype="float64")
     assert np.array_equal(m, a) and times.shape == (8, 5) and traj.shape == (8, 5, 6)
+    # ... recorded in the same integration: the samples of a plain trajsize=5 call
+    t_plain, x_plain = agama_amd.orbit(potential=gpu, ic=ic, time=T, trajsize=5, separateTime=True, dtype="float64")
+    assert np.array_equal(times, t_plain)
+    assert np.max(np.abs(traj - x_plain)) <= 1e-12 * np.max(np.abs(x_plain))
     z = agama_amd.orbit(potential=gpu, ic=ic, time=0.0, targets=t1)
     assert np.all(z == 0)
     back = agama_amd.orbit(potential=gpu, ic=ic, time=-T, targets=t1)
