@@ -275,3 +275,30 @@ def test_targets_with_dprkn8_closed_forms():
     big = np.concatenate([[0.0], np.logspace(-3, 4, 36)])
     res = tri.orbits_target_classic(ic, T, big, 1, 3, method="dprkn8")
     assert np.max(np.abs(res.sum(axis=1) - 1)) < 2e-6
+
+
+def test_target_objects_host_side():
+    """agama_amd.Target (no GPU call): len() equals the oracle's value counts for every type, and
+    invalid parameters raise as in the reference's constructors"""
+    import agama_amd
+    from oracle.pyoracle import classic_target_size, cyl_target_size, sphharm_target_size
+    gr = np.concatenate([[0.0], np.logspace(-2, 2, 9)])
+    gz = np.concatenate([[0.0], np.logspace(-2, 1, 6)])
+    assert len(agama_amd.Target(type="DensitySphHarm", gridr=gr, lmax=6, mmax=4)) == sphharm_target_size(6, 4, gr)
+    for deg, kind in ((0, "DensityCylindricalTopHat"), (1, "DensityCylindricalLinear")):
+        assert len(agama_amd.Target(type=kind, gridr=gr, gridz=gz, mmax=4)) == cyl_target_size(deg, 4, gr, gz)
+    for deg, kind in ((0, "DensityClassicTopHat"), (1, "DensityClassicLinear")):
+        t = agama_amd.Target(type=kind, gridr=gr, stripsPerPane=3, axisRatioY=0.8, axisRatioZ=0.6)
+        assert len(t) == classic_target_size(deg, 3, gr)
+    for deg in range(4):
+        assert len(agama_amd.Target(type="kinemshell", gridr=gr, degree=deg)) == 2 * (len(gr) + deg - 1)
+    with pytest.raises(ValueError):
+        agama_amd.Target(type="DensityClassicLinear", gridr=gr, stripsPerPane=0)
+    with pytest.raises(ValueError):
+        agama_amd.Target(type="DensityClassicTopHat", gridr=gr, axisRatioY=-1)
+    with pytest.raises(ValueError):
+        agama_amd.Target(type="KinemShell", gridr=gr[::-1], degree=1)
+    with pytest.raises(TypeError):
+        agama_amd.Target(type="DensitySphHarm", gridr=gr, lmax=2, bogus=1)
+    with pytest.raises(agama_amd.AgamaAmdError):
+        agama_amd.Target(type="LOSVD", gridr=gr)
