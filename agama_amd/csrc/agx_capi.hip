@@ -544,7 +544,8 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
     const long* d_traj_offset, const int* d_order, double accuracy, double Omega,
     long maxNumSteps, double* d_time, double* d_traj, double* d_final, double* d_dev, long* d_nsteps,
     long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags, void* stream,
-    const LyapDev& ly = LyapDev(), const RecordPool& pool = RecordPool(), const AgxTargetList* d_targets = nullptr)
+    const LyapDev& ly = LyapDev(), const RecordPool& pool = RecordPool(), const AgxTargetList* d_targets = nullptr,
+    bool allowGroups = true)
 {
     if(!pot) return fail(AGX_EINVAL, "NULL potential");
     if(n == 0) return AGX_OK;
@@ -602,13 +603,16 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
             const long lanes = (long)blocks * bs;
             groupStride = n * 8 <= lanes ? 1 : n <= 2 * lanes ? 4 : n <= 4 * lanes ? 8 : 0;
         }
-        if(groupStride > 0 && n >= g_groupMinBatch && !variational && !d_targets && !pool.counter &&
+        if(allowGroups && groupStride > 0 && n >= g_groupMinBatch && !variational && !d_targets && !pool.counter &&
            !(flags & AGX_METHOD_DPRKN8) && precision == AGX_PRECISION_FP64 && g_fetchOrbit == AGX_FETCH_VEC2 &&
            g_orbitWaves == 1 && pot->nfwMass == 0) {
             const long waves = (long)blocks * (bs / 64);
             const long stride = std::min<long>(groupStride, waves);   // wavefront 0 is always a group one
             const long groupWaves = (waves + stride - 1) / stride;
             A.groupOrbits = std::min<long>(n, g_groupOrbits >= 0 ? g_groupOrbits : groupWaves * (64 / 8));
+            // with every wavefront in group mode no lane reads the single-lane queue: all the
+            // orbits then go to the groups
+            if(stride == 1 && A.groupOrbits > 0) A.groupOrbits = n;
             A.groupStride = A.groupOrbits > 0 ? (int)stride : 0;
             A.groupCounter = pot->dcounter + 1;
             if(A.groupStride == 0) A.groupOrbits = 0;
@@ -1051,7 +1055,7 @@ agx_orbit_record* agx_orbit_record_batch(const agx_potential* pot, long n, const
             HIPCHECK(hipMemcpy(doff.p, hoff.data(), m * 8, hipMemcpyHostToDevice));
             int rc = orbitBatchDevice(pot, m, dic.p, dts.p, dtt.p, dsz.p, doff.p, nullptr, accuracy, Omega,
                 maxNumSteps, dtime, dtraj, dfin.p, ddev, dns.p, dna.p, dne.p, dmde.p, dst.p, flags, nullptr,
-                LyapDev(), pool);
+                LyapDev(), pool, nullptr, false);   // one-lane form in both passes: the same steps
             if(rc) return rc;
             HIPCHECK(hipDeviceSynchronize());
             std::vector<double> f2(m * 6), d2(m);

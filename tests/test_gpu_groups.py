@@ -95,3 +95,46 @@ def test_mixed_batch_and_sampling(setup, groups):
     with pytest.raises(Exception):
         from agama_amd._native import check
         check(groups.agx_set_orbit_groups(-2, -1, 0), "groups")
+
+
+def test_stride_one_runs_every_orbit(setup, groups):
+    """stride 1 makes every wavefront a group wavefront; a group-orbit count below the batch
+    size must not leave the other orbits unintegrated (they go to the groups as well)"""
+    import agama_amd
+    gpu, orc = setup
+    ic, T = ics(setup, 4096, seed=27, periods=2)
+    print("case: agx_set_orbit_groups(1, 64, 0) on 4096 orbits x 2 periods", flush=True)
+    assert groups.agx_set_orbit_groups(1, 64, 0) == 0
+    info = {}
+    agama_amd.orbit(potential=gpu, ic=ic, time=T, accuracy=1e-8, info=info)
+    ref = orc.orbits(ic, T, trajsize=1, accuracy=1e-8)
+    assert np.all(info["status"] == 0) and np.all(info["nsteps"] > 0)
+    assert np.all(np.isfinite(info["final"]))
+    err = np.linalg.norm(info["final"][:, :3] - ref["final"][:, :3], axis=1) / np.linalg.norm(ref["final"][:, :3], axis=1)
+    print(f"position rel err median {np.median(err):.2e}, max {np.max(err):.2e}; "
+          f"identical step counts {np.mean(info['nsteps'] == ref['nsteps']):.3f}")
+    assert np.median(err) <= 1e-9 and np.quantile(err, 0.99) <= 1e-7
+
+
+def test_record_second_pass_with_group_defaults(setup, groups):
+    """every-step recording whose second pass integrates orbits again: with group mode allowed
+    for every batch size (min_batch 0) both passes still run the one-lane kernel, so the orbits
+    integrated again take the same steps and their rows equal a one-pass recording bit for bit"""
+    import agama_amd
+    from agama_amd._native import lib
+    gpu, orc = setup
+    ic, T = ics(setup, 48, seed=43, periods=3)
+    print("case: trajsize=0, 48 orbits, 512-row budget, agx_set_orbit_groups(-1, -1, 0)", flush=True)
+    t1, x1 = agama_amd.orbit(potential=gpu, ic=ic, time=T, trajsize=0, dtype="float64", separateTime=True)
+    assert groups.agx_set_orbit_groups(-1, -1, 0) == 0
+    assert lib.agx_set_record_rows(16 * 32) == 0
+    try:
+        t2, x2 = agama_amd.orbit(potential=gpu, ic=ic, time=T, trajsize=0, dtype="float64", separateTime=True)
+    finally:
+        lib.agx_set_record_rows(0)
+    for i in range(48):
+        assert np.array_equal(t1[i], t2[i]) and np.array_equal(x1[i], x2[i]), i
+    ref = orc.orbits_steps(ic, T)
+    rows = np.array([len(a) for a in t2])
+    print(f"identical step counts with the oracle {np.mean(rows == ref['rows']):.3f}")
+    assert np.mean(rows == ref["rows"]) >= 0.9
