@@ -62,6 +62,10 @@ def _load():
         "agx_orbit_batch_device": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp,
                                                   ctypes.c_double, ctypes.c_double, ctypes.c_long,
                                                   _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+        "agx_orbit_batch_device_noalloc": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                          ctypes.c_double, ctypes.c_double, ctypes.c_long,
+                                                          _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
+                                                          _vp]),
         "agx_orbit_record_batch": (_vp, [_vp, ctypes.c_long, _vp, _vp, _vp, _vp, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_long, ctypes.c_int, _vp, _vp, _vp, _vp,
                                          _vp, _vp, _vp, ctypes.c_int]),
@@ -135,7 +139,7 @@ EXPORTED = ["agx_last_error", "agx_potential_create", "agx_potential_from_coefs"
             "agx_debug_tables", "agx_potential_coefs",
             "agx_eval_potential", "agx_eval_many", "agx_eval_many_device", "agx_eval_hess_many",
             "agx_eval_hess_many_device", "agx_orbit_var_batch", "agx_orbit_var_batch_device", "agx_lyapunov_fit", "agx_orbit_batch",
-            "agx_orbit_batch_device", "agx_orbit_record_batch", "agx_orbit_record_fetch",
+            "agx_orbit_batch_device", "agx_orbit_batch_device_noalloc", "agx_orbit_record_batch", "agx_orbit_record_fetch",
             "agx_orbit_record_num_series", "agx_orbit_record_series_info", "agx_orbit_record_node_times",
             "agx_orbit_record_interp", "agx_orbit_record_free", "agx_set_record_rows", "agx_set_construction", "agx_set_surrogate_pilot", "agx_surrogate_last_pilot", "agx_target_density_sphharm", "agx_target_density_cylindrical", "agx_target_kinem_shell", "agx_target_density_classic", "agx_target_num_values",
             "agx_target_destroy", "agx_orbit_target_batch", "agx_orbit_target_traj_batch", "agx_set_orbit_launch", "agx_set_fetch", "agx_set_orbit_groups", "iniacceleration",

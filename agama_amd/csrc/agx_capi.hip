@@ -537,6 +537,13 @@ struct RecordPool {
     int chunk = 0, maxChunks = 0, chunks = 0;
 };
 
+// bytes of the schedule's workspace for n orbits: agxLaunchSchedule's keys and sort storage,
+// then the order itself (n ints)
+static size_t scheduleWorkspaceBytes(long n)
+{
+    return agxScheduleWorkspace(n) + ((size_t)n * 4 + 255) / 256 * 256;
+}
+
 // the batch entry points' common body; d_dev non-null or ly.bins non-null: the variational
 // kernel (der=True: six vectors; lyapunov only: one)
 static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic,
@@ -545,7 +552,7 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
     long maxNumSteps, double* d_time, double* d_traj, double* d_final, double* d_dev, long* d_nsteps,
     long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags, void* stream,
     const LyapDev& ly = LyapDev(), const RecordPool& pool = RecordPool(), const AgxTargetList* d_targets = nullptr,
-    bool allowGroups = true)
+    bool allowGroups = true, void* scheduleWork = nullptr)
 {
     if(!pot) return fail(AGX_EINVAL, "NULL potential");
     if(n == 0) return AGX_OK;
@@ -569,14 +576,15 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
         hipStream_t st = (hipStream_t)stream;
         void* work = nullptr;
         if(!d_order && !(flags & AGX_ORDER_AS_GIVEN)) {
-            // longest expected integrations first (agx_schedule.hip); stream-ordered workspace
-            const size_t wb = agxScheduleWorkspace(n) + ((size_t)n * 4 + 255) / 256 * 256;
-            HIPCHECK(hipMallocAsync(&work, wb, st));
+            // longest expected integrations first (agx_schedule.hip); stream-ordered workspace,
+            // or the caller's (scheduleWork: at least scheduleWorkspaceBytes(n) bytes)
+            if(scheduleWork) work = scheduleWork;
+            else HIPCHECK(hipMallocAsync(&work, scheduleWorkspaceBytes(n), st));
             int* ord = reinterpret_cast<int*>(static_cast<char*>(work) + agxScheduleWorkspace(n));
             hipError_t e = agxLaunchSchedule(pot->dev, pot->ddev, n, d_ic, d_ttotal, ord,
                 work, st);
             if(e != hipSuccess) {
-                (void)hipFreeAsync(work, st);
+                if(!scheduleWork) (void)hipFreeAsync(work, st);
                 return fail(AGX_EDEVICE, std::string("schedule: ") + hipGetErrorString(e));
             }
             d_order = ord;
@@ -646,7 +654,7 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
             HIPCHECK(hipFreeAsync(vstate, st));
         } else
             e = launchOrbitMethod(pot, precision, flags, A, blocks, bs, st);
-        if(work) HIPCHECK(hipFreeAsync(work, st));
+        if(work && !scheduleWork) HIPCHECK(hipFreeAsync(work, st));
         if(e != hipSuccess) return fail(AGX_EDEVICE, std::string("orbit launch: ") + hipGetErrorString(e));
         return AGX_OK;
     } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
@@ -675,6 +683,72 @@ int agx_orbit_var_batch_device(const agx_potential* pot, long n, const double* d
     return orbitBatchDevice(pot, n, d_ic, d_tstart, d_ttotal, d_trajsize, d_traj_offset, d_order, accuracy,
         Omega, maxNumSteps, d_time, d_traj, d_final, d_dev, d_nsteps, d_nattempts, d_nevals, d_maxde,
         d_status, flags, stream);
+}
+
+}  // extern "C"
+
+// ---- orbit batches without allocation (agx_orbit_batch_device_noalloc) ----
+// The schedule's workspace (sort keys, the radix sort's storage and the work order) of at most
+// NOALLOC_MAX_ORBITS orbits lives in this array: one copy per device, held until the process
+// ends.  Host code never passes `scratch` itself to a kernel (on the host it names a stand-in,
+// not device memory): it uses the device address taken once per device by scratchAddress().
+// Nothing in it needs clearing between calls (the cost kernel writes every key and index it
+// sorts).  Calls that overlap on one device race on it.
+static constexpr long NOALLOC_MAX_ORBITS = 1L << 20;
+static constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)48 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratchMutex;
+void* g_scratchPtr[MAX_DEVICES] = {};
+
+// device address of `scratch` on the current device (hipGetSymbolAddress on the first call)
+int scratchAddress(void** out)
+{
+    int dev = 0;
+    if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES)
+        return fail(AGX_EDEVICE, "no usable current HIP device");
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if(!g_scratchPtr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if(e != hipSuccess || !p)
+            return fail(AGX_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratchPtr[dev] = p;
+    }
+    *out = g_scratchPtr[dev];
+    return AGX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int agx_orbit_batch_device_noalloc(const agx_potential* pot, long n, const double* d_ic,
+    const double* d_tstart, const double* d_ttotal, const int* d_trajsize,
+    const long* d_traj_offset, const int* d_order, double accuracy, double Omega,
+    long maxNumSteps, double* d_time, double* d_traj, double* d_final, long* d_nsteps,
+    long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags, void* stream)
+{
+    g_err.clear();
+    if(!pot) return fail(AGX_EINVAL, "NULL potential");
+    if(n > NOALLOC_MAX_ORBITS)
+        return fail(AGX_EINVAL, "agx_orbit_batch_device_noalloc: at most 1048576 orbits per call (got " +
+            std::to_string(n) + ")");
+    int dev = -1;
+    if(hipGetDevice(&dev) != hipSuccess || dev != pot->device)
+        return fail(AGX_EINVAL, "agx_orbit_batch_device_noalloc: the potential's device must be current");
+    void* work = nullptr;
+    if(n > 0 && !d_order && !(flags & AGX_ORDER_AS_GIVEN)) {
+        if(scheduleWorkspaceBytes(n) > NOALLOC_SCRATCH_BYTES)
+            return fail(AGX_EINVAL, "agx_orbit_batch_device_noalloc: the schedule of " + std::to_string(n) +
+                " orbits needs more than the library's scratch memory");
+        const int rc = scratchAddress(&work);
+        if(rc) return rc;
+    }
+    return orbitBatchDevice(pot, n, d_ic, d_tstart, d_ttotal, d_trajsize, d_traj_offset, d_order, accuracy,
+        Omega, maxNumSteps, d_time, d_traj, d_final, nullptr, d_nsteps, d_nattempts, d_nevals, d_maxde,
+        d_status, flags, stream, LyapDev(), RecordPool(), nullptr, true, work);
 }
 
 }  // extern "C"

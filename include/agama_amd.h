@@ -157,6 +157,23 @@ int agx_orbit_batch_device(const agx_potential* pot, long n, const double* d_ic,
     long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags,
     void* stream);
 
+/* agx_orbit_batch_device without any allocation: no counterpart in the reference; for callers
+   that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc).  Same arguments, results and stream semantics as
+   agx_orbit_batch_device (the same kernels, bit for bit); the longest-first schedule's workspace
+   (sort keys, radix-sort storage, work order) lives in a static device array of 48 MiB that the
+   library holds on each device it runs on until the process ends.  At most 1,048,576 orbits
+   per call (AGX_EINVAL beyond); the potential's device must be current; plain DOP853/DPRKN8
+   integration only (no deviation vectors, targets or every-step recording).  Calls that overlap
+   on one device race on that array (and, as for every orbit entry point, on the potential's
+   work-queue counter). */
+int agx_orbit_batch_device_noalloc(const agx_potential* pot, long n, const double* d_ic,
+    const double* d_tstart, const double* d_ttotal, const int* d_trajsize,
+    const long* d_traj_offset, const int* d_order, double accuracy, double Omega,
+    long maxNumSteps, double* d_time, double* d_traj, double* d_final, long* d_nsteps,
+    long* d_nattempts, long* d_nevals, double* d_maxde, int* d_status, int flags,
+    void* stream);
+
 /* agx_orbit_batch with the variational equation (src/orbit_variational.cpp RuntimeVariational,
    deviation vectors advanced with Ode2StepperGL4<3>, src/math_ode.cpp:1017-1250):
    out_dev (agama.orbit(der=True)): the six deviation vectors started from the identity, sampled
