@@ -12,8 +12,10 @@
       GL nodes ................................ Legendre roots / weights sum, exact polynomials
       sphHarmArray / trigMultiAngle ........... scipy.special.lpmv closed forms, libm cos/sin
       QuinticSpline2d ......................... node exactness, exact bicubic reproduction
-      MultipoleInterp2d / PowerLawMultipole ... node-lattice exactness, continuity across
-                                                rmin/rmax, symmetry, Poisson closed forms
+      MultipoleInterp2d / PowerLawMultipole ... node lattice = direct SH synthesis of the
+                                                coefficient file, gradient = finite differences,
+                                                continuity across rmin/rmax, far-field -M/r
+                                                (test_oracle_multipole)
       DOP853 .................................. closed-form linear ODEs (order 8 / dense 7),
                                                 bitwise vs the reference's own math_ode.cpp
                                                 compiled into oracle/_ref
