@@ -56,3 +56,42 @@ def test_rotating_bar_orbits_match_oracle(pots):
           f"(oracle {np.median(ref['maxde']):.2e})")
     assert np.median(err) <= 1e-7
     assert np.median(info["maxde"]) <= 1.5 * np.median(ref["maxde"]) + 1e-13
+
+
+def test_bar_halo_surrogate_path(pots):
+    """the rotating bar + NFW workload on its named path: an MLP surrogate trained on the
+    composite (l <= 2 part of the bar plus the NFW halo as its base), integrated in the frame
+    rotating at Omega = 40 under the persistent schedule, the FP64 Multipole + NFW judging the
+    Jacobi integral.  Every orbit either keeps |E_J/E_J0 - 1| <= 1e-6 with the surrogate force or
+    is integrated again with the FP64 kernel and then equals the plain FP64 integration bit for
+    bit.  Run with the pilot (8192 orbits: the pilot decides whether the batch stays on the
+    surrogate) and without it (1024 orbits, each on the surrogate first)."""
+    import agama_amd
+    from agama_amd import lib, workloads
+    gpu, orc = pots
+    print("case: surrogate training on the bar + NFW composite (4M samples, 3000 steps)", flush=True)
+    sur = agama_amd.Surrogate.train(gpu, samples=4e6, steps=3000, batch=16384, seed=5)
+    ic, T = workloads.bar_halo_orbits(8192, lambda p: -orc.eval(p)[1], seed=44)
+    try:
+        for pilot, n in ((1, 8192), (0, 1024)):
+            print(f"case: surrogate orbits at Omega=40, {n} orbits, pilot {'on' if pilot else 'off'}", flush=True)
+            assert lib.agx_set_surrogate_pilot(pilot) == 0
+            info_s, info_p = {}, {}
+            agama_amd.orbit(potential=sur, ic=ic[:n], time=T[:n], accuracy=1e-8, Omega=workloads.BAR_OMEGA,
+                            info=info_s, max_rel_de=1e-6)
+            agama_amd.orbit(potential=gpu, ic=ic[:n], time=T[:n], accuracy=1e-8, Omega=workloads.BAR_OMEGA,
+                            info=info_p)
+            st = info_s["status"]
+            fb = (st & 8) != 0
+            kept = ~fb
+            print(f"pilot {'on' if pilot else 'off'}: {fb.sum()} of {n} orbits fell back to the FP64 kernel, "
+                  f"{kept.sum()} kept the surrogate" + (f" (pilot: {info_s['pilot'][1]} of {info_s['pilot'][0]} fell "
+                                                        f"back)" if pilot else "") +
+                  (f"; their max dE_J/E_J {info_s['maxde'][kept].max():.2e}" if kept.any() else ""))
+            assert info_s["n_fallback"] == fb.sum()
+            assert np.all((st & ~8) == 0) and np.all(info_p["status"] == 0)
+            assert np.array_equal(info_s["final"][fb], info_p["final"][fb])
+            assert np.array_equal(info_s["nsteps"][fb], info_p["nsteps"][fb])
+            assert np.all(info_s["maxde"][kept] <= 1e-6)
+    finally:
+        lib.agx_set_surrogate_pilot(1)
