@@ -59,7 +59,20 @@ struct agx_potential {
     unsigned int* dcounter = nullptr;
     int device = 0;
     double nfwMass = 0, nfwScale = 1;  // optional analytic NFW component
-    ~agx_potential() { if(dmem) (void)hipFree(dmem); }
+    // staging of the host-buffer evaluation entry points (agx_eval_many, agx_eval_hess_many):
+    // the handle's own stream and a device buffer kept between calls (grown to the largest
+    // call), so that a call neither allocates nor waits for other streams; calls on one handle
+    // take the mutex in turn
+    mutable std::mutex evalMutex;
+    mutable hipStream_t evalStream = nullptr;
+    mutable double* evalBuf = nullptr;
+    mutable size_t evalCap = 0;
+    ~agx_potential()
+    {
+        if(evalBuf) (void)hipFree(evalBuf);
+        if(evalStream) (void)hipStreamDestroy(evalStream);
+        if(dmem) (void)hipFree(dmem);
+    }
 };
 
 namespace {
@@ -417,6 +430,37 @@ int agx_eval_hess_many_device(const agx_potential* pot, long n, const double* d_
     return AGX_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// makes the potential's device current for the scope of a call and restores the caller's
+struct DeviceScope {
+    int prev = -1;
+    explicit DeviceScope(int dev)
+    {
+        if(hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if(prev != dev) HIPCHECK(hipSetDevice(dev));
+    }
+    ~DeviceScope() { if(prev >= 0) (void)hipSetDevice(prev); }
+};
+
+// the handle's staging buffer with room for `doubles` values (pot->evalMutex held)
+double* evalStaging(const agx_potential* pot, size_t doubles)
+{
+    if(!pot->evalStream) HIPCHECK(hipStreamCreateWithFlags(&pot->evalStream, hipStreamNonBlocking));
+    if(pot->evalCap < doubles) {
+        if(pot->evalBuf) HIPCHECK(hipFree(pot->evalBuf));
+        pot->evalBuf = nullptr;
+        pot->evalCap = 0;
+        HIPCHECK(hipMalloc(&pot->evalBuf, doubles * sizeof(double)));
+        pot->evalCap = doubles;
+    }
+    return pot->evalBuf;
+}
+}  // namespace
+
+extern "C" {
+
 int agx_eval_hess_many(const agx_potential* pot, long n, const double* pos, double t,
     double* phi, double* grad, double* hess, int flags)
 {
@@ -425,15 +469,23 @@ int agx_eval_hess_many(const agx_potential* pot, long n, const double* pos, doub
     if(n == 0) return AGX_OK;
     if(n < 0 || !pos) return fail(AGX_EINVAL, "invalid point array");
     try {
-        HIPCHECK(hipSetDevice(pot->device));
-        DevBuf<double> dp(n * 3), dphi(phi ? n : 0), dg(grad ? n * 3 : 0), dh(hess ? n * 6 : 0);
-        HIPCHECK(hipMemcpy(dp.p, pos, n * 3 * 8, hipMemcpyHostToDevice));
-        int rc = agx_eval_hess_many_device(pot, n, dp.p, t, dphi.p, dg.p, dh.p, flags, nullptr);
+        std::lock_guard<std::mutex> lock(pot->evalMutex);
+        DeviceScope scope(pot->device);
+        double* w = evalStaging(pot, (size_t)n * (3 + (phi ? 1 : 0) + (grad ? 3 : 0) + (hess ? 6 : 0)));
+        size_t off = 0;
+        auto take = [&](bool want, long count) { double* q = want ? w + off : nullptr; if(want) off += count; return q; };
+        double* dp = take(true, n * 3);
+        double* dphi = take(phi != nullptr, n);
+        double* dg = take(grad != nullptr, n * 3);
+        double* dh = take(hess != nullptr, n * 6);
+        hipStream_t st = pot->evalStream;
+        HIPCHECK(hipMemcpyAsync(dp, pos, n * 3 * 8, hipMemcpyHostToDevice, st));
+        int rc = agx_eval_hess_many_device(pot, n, dp, t, dphi, dg, dh, flags, st);
         if(rc) return rc;
-        HIPCHECK(hipDeviceSynchronize());
-        if(phi) HIPCHECK(hipMemcpy(phi, dphi.p, n * 8, hipMemcpyDeviceToHost));
-        if(grad) HIPCHECK(hipMemcpy(grad, dg.p, n * 3 * 8, hipMemcpyDeviceToHost));
-        if(hess) HIPCHECK(hipMemcpy(hess, dh.p, n * 6 * 8, hipMemcpyDeviceToHost));
+        if(phi) HIPCHECK(hipMemcpyAsync(phi, dphi, n * 8, hipMemcpyDeviceToHost, st));
+        if(grad) HIPCHECK(hipMemcpyAsync(grad, dg, n * 3 * 8, hipMemcpyDeviceToHost, st));
+        if(hess) HIPCHECK(hipMemcpyAsync(hess, dh, n * 6 * 8, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
         return AGX_OK;
     } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
 }
@@ -446,14 +498,21 @@ int agx_eval_many(const agx_potential* pot, long n, const double* pos, double t,
     if(n == 0) return AGX_OK;
     if(n < 0 || !pos) return fail(AGX_EINVAL, "invalid point array");
     try {
-        HIPCHECK(hipSetDevice(pot->device));
-        DevBuf<double> dp(n * 3), dphi(phi ? n : 0), dg(grad ? n * 3 : 0);
-        HIPCHECK(hipMemcpy(dp.p, pos, n * 3 * 8, hipMemcpyHostToDevice));
-        int rc = agx_eval_many_device(pot, n, dp.p, t, dphi.p, dg.p, flags, nullptr);
+        std::lock_guard<std::mutex> lock(pot->evalMutex);
+        DeviceScope scope(pot->device);
+        double* w = evalStaging(pot, (size_t)n * (3 + (phi ? 1 : 0) + (grad ? 3 : 0)));
+        size_t off = 0;
+        auto take = [&](bool want, long count) { double* q = want ? w + off : nullptr; if(want) off += count; return q; };
+        double* dp = take(true, n * 3);
+        double* dphi = take(phi != nullptr, n);
+        double* dg = take(grad != nullptr, n * 3);
+        hipStream_t st = pot->evalStream;
+        HIPCHECK(hipMemcpyAsync(dp, pos, n * 3 * 8, hipMemcpyHostToDevice, st));
+        int rc = agx_eval_many_device(pot, n, dp, t, dphi, dg, flags, st);
         if(rc) return rc;
-        HIPCHECK(hipDeviceSynchronize());
-        if(phi) HIPCHECK(hipMemcpy(phi, dphi.p, n * 8, hipMemcpyDeviceToHost));
-        if(grad) HIPCHECK(hipMemcpy(grad, dg.p, n * 3 * 8, hipMemcpyDeviceToHost));
+        if(phi) HIPCHECK(hipMemcpyAsync(phi, dphi, n * 8, hipMemcpyDeviceToHost, st));
+        if(grad) HIPCHECK(hipMemcpyAsync(grad, dg, n * 3 * 8, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
         return AGX_OK;
     } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
 }
@@ -2106,11 +2165,15 @@ double agama_evalPotential(const agama_Potential* pot, const double pos[3], doub
 
 }  // extern "C"
 
-// ---- NEMO acceleration plugin (src/interface_nemo.cpp:74-220) ----
+// ---- NEMO acceleration plugin (src/interface_nemo.cpp:47-220) ----
 namespace {
 std::unique_ptr<agx_potential> g_nemoPot;
 double g_nemoOmega = 0;
 
+// getAcc (interface_nemo.cpp:74-97): the active bodies' positions rotated into the frame of the
+// potential at angle Omega*time (the reference's math::sincos), one batched evaluation on the
+// device, accelerations rotated back; potential (flag bit 0) and acceleration (bit 1) added
+// or assigned
 template<typename NumT>
 void nemoAcc(int ndim, double time, int nbody, const void* pos_, const int* active, void* pot_,
     void* acc_, int flag)
@@ -2119,9 +2182,11 @@ void nemoAcc(int ndim, double time, int nbody, const void* pos_, const int* acti
     NumT* pot = static_cast<NumT*>(pot_);
     NumT* acc = static_cast<NumT*>(acc_);
     double ca = 1, sa = 0;
-    if(g_nemoOmega != 0) { sa = sin(g_nemoOmega * time); ca = cos(g_nemoOmega * time); }
+    agx::sincosAgama(g_nemoOmega * time, sa, ca);
     std::vector<long> idx;
     std::vector<double> pts;
+    idx.reserve(nbody);
+    pts.reserve((size_t)nbody * 3);
     for(int i = 0; i < nbody; i++) {
         if(active && (active[i] & 1) != 1) continue;
         idx.push_back(i);
@@ -2130,8 +2195,8 @@ void nemoAcc(int ndim, double time, int nbody, const void* pos_, const int* acti
         pts.push_back(y * ca - x * sa);
         pts.push_back(z);
     }
-    std::vector<double> phi(idx.size()), g(idx.size() * 3);
     if(idx.empty()) return;
+    std::vector<double> phi(idx.size()), g(idx.size() * 3);
     if(agx_eval_many(g_nemoPot.get(), (long)idx.size(), pts.data(), time, phi.data(), g.data(),
         AGX_PRECISION_FP64) != AGX_OK) {
         fprintf(stderr, "agama_amd NEMO plugin: %s\n", agx_last_error());
@@ -2140,14 +2205,15 @@ void nemoAcc(int ndim, double time, int nbody, const void* pos_, const int* acti
     const bool addPot = flag & 1, addAcc = flag & 2;
     for(size_t j = 0; j < idx.size(); j++) {
         const long i = idx[j];
-        const NumT ax = (NumT)-(g[j * 3] * ca - g[j * 3 + 1] * sa),
-                   ay = (NumT)-(g[j * 3 + 1] * ca + g[j * 3] * sa), az = (NumT)-g[j * 3 + 2];
+        const NumT ax = -(NumT)(g[j * 3] * ca - g[j * 3 + 1] * sa),
+                   ay = -(NumT)(g[j * 3 + 1] * ca + g[j * 3] * sa), az = -(NumT)g[j * 3 + 2];
         if(pot) { if(addPot) pot[i] += (NumT)phi[j]; else pot[i] = (NumT)phi[j]; }
         if(addAcc) { acc[i * ndim] += ax; acc[i * ndim + 1] += ay; if(ndim == 3) acc[i * ndim + 2] += az; }
         else { acc[i * ndim] = ax; acc[i * ndim + 1] = ay; if(ndim == 3) acc[i * ndim + 2] = az; }
     }
 }
 
+// myacc (interface_nemo.cpp:114-147)
 void nemoAccel(int ndim, double t, int n, const void*, const void* pos, const void*,
     const int* flags, void* pot, void* acc, int flag, char numtype)
 {
@@ -2157,15 +2223,18 @@ void nemoAccel(int ndim, double t, int n, const void*, const void* pos, const vo
 }
 }  // namespace
 
-extern "C" void iniacceleration(const double* pars, int npar, const char* file,
-    agx_acc_pter* accel, bool* need_m, bool* need_v)
+// inipotential (interface_nemo.cpp:180-206): pars = {Omega, G}; the potential comes from a
+// Multipole coefficient file.  G != 1 rescales masses in the reference's unit conversion, which
+// this library does not provide: it stops with a message, as for a missing file.
+extern "C" void inipotential(const int* npar, const double* pars, const char* file)
 {
     if(!file || !file[0]) {
         fprintf(stderr, "agama_amd NEMO plugin: should provide the coefficient file in accfile=...\n");
         exit(1);
     }
-    g_nemoOmega = npar >= 1 ? pars[0] : 0;
-    const double G = npar >= 2 ? pars[1] : 1;
+    const int np = npar ? *npar : 0;
+    g_nemoOmega = np >= 1 ? pars[0] : 0;
+    const double G = np >= 2 ? pars[1] : 1;
     if(G != 1) {
         fprintf(stderr, "agama_amd NEMO plugin: G != 1 (unit conversion) is not supported\n");
         exit(1);
@@ -2176,7 +2245,28 @@ extern "C" void iniacceleration(const double* pars, int npar, const char* file,
         exit(1);
     }
     g_nemoPot.reset(p);
+    fprintf(stderr, "agama_amd NEMO plugin: created a Multipole potential from %s", file);
+    if(g_nemoOmega != 0) fprintf(stderr, " with pattern speed %g", g_nemoOmega);
+    fprintf(stderr, "\n");
+}
+
+// iniacceleration (interface_nemo.cpp:209-220)
+extern "C" void iniacceleration(const double* pars, int npar, const char* file,
+    agx_acc_pter* accel, bool* need_m, bool* need_v)
+{
+    inipotential(&npar, pars, file);
     *accel = nemoAccel;
     if(need_m) *need_m = false;
     if(need_v) *need_v = false;
+}
+
+// potential_double / potential_float (interface_nemo.cpp:150-177): one body, assigned
+extern "C" void potential_double(const int* ndim, const double* pos, double* acc, double* pot, const double* time)
+{
+    if(g_nemoPot && (*ndim == 2 || *ndim == 3)) nemoAcc<double>(*ndim, *time, 1, pos, nullptr, pot, acc, 0);
+}
+
+extern "C" void potential_float(const int* ndim, const float* pos, float* acc, float* pot, const float* time)
+{
+    if(g_nemoPot && (*ndim == 2 || *ndim == 3)) nemoAcc<float>(*ndim, *time, 1, pos, nullptr, pot, acc, 0);
 }

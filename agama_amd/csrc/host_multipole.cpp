@@ -606,6 +606,8 @@ std::vector<double> gridInTau(int lmax)
 
 }  // namespace
 
+void sincosAgama(double x, double& s, double& c) { sincosA(x, s, c); }
+
 // ---------------------------------------------------------------------------------------
 Params Params::parse(const std::string& text)
 {

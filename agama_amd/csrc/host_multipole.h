@@ -104,4 +104,7 @@ HostTables buildTables(const Coefs& c, bool allowLowOrder = false);
 // the expansion with every term of degree l > lmax set to zero
 Coefs truncateCoefs(const Coefs& c, int lmax);
 
+// the reference's sin/cos (math::sincos, src/math_core.cpp:171-206: multiples of pi/2 exact)
+void sincosAgama(double x, double& s, double& c);
+
 }  // namespace agx

@@ -312,17 +312,25 @@ int agx_set_orbit_launch(int num_blocks, int block_size);
    wavefront-cooperative gather through LDS.  Results are identical; set by the autotuner. */
 int agx_set_fetch(int eval_mode, int orbit_mode);
 
-/* ---- NEMO acceleration plugin (src/interface_nemo.cpp:47-60, 114-147, 180-220) ----
-   iniacceleration(pars, npar, file, &accel, &need_m, &need_v): pars[0] = Omega (pattern
-   speed), pars[1] = G is ignored unless 1; file names a Multipole coefficient file.  The
+/* ---- NEMO acceleration plugin (src/interface_nemo.cpp:47-60, 114-220) ----
+   iniacceleration(pars, npar, file, &accel, &need_m, &need_v) (replaces interface_nemo.cpp:
+   209-220): pars[0] = Omega (pattern speed), pars[1] = G (only 1: other values stop the process
+   with a message, as a missing file does); file names a Multipole coefficient file.  The
    returned accel has the acc_pter signature:
      accel(ndim, t, n, m, pos, vel, flags, pot, acc, flag, numtype)
    with ndim = 2 or 3, numtype 'f' or 'd', flag bit 0: add to pot, bit 1: add to acc (else assign),
-   flags[i] & 1 selects active bodies (NULL: all); positions are in the rotating frame.  */
+   flags[i] & 1 selects active bodies (NULL: all); positions and accelerations are in the inertial
+   frame, the potential rotating at Omega (its frame at angle Omega*t).  The active bodies of a
+   call are evaluated as one batch on the device.  */
 typedef void (*agx_acc_pter)(int ndim, double t, int n, const void* m, const void* pos,
     const void* vel, const int* flags, void* pot, void* acc, int flag, char numtype);
 void iniacceleration(const double* pars, int npar, const char* file, agx_acc_pter* accel,
     bool* need_m, bool* need_v);
+/* the older interface (replaces interface_nemo.cpp:180-206): installs the potential as above */
+void inipotential(const int* npar, const double* pars, const char* file);
+/* potential and acceleration of one body, assigned (replace interface_nemo.cpp:150-177) */
+void potential_double(const int* ndim, const double* pos, double* acc, double* pot, const double* time);
+void potential_float(const int* ndim, const float* pos, float* acc, float* pot, const float* time);
 
 /* ---- drop-in aliases of the reference's plain C interface (src/interface_c.h:15-85) ---- */
 typedef agx_potential agama_Potential;

@@ -106,3 +106,58 @@ def test_public_api_shapes_and_c_abi(pots):
     assert abs(val / ophi[0] - 1) < 1e-12 and np.max(np.abs(d - og[0])) < 1e-12 * np.linalg.norm(og)
     assert not lib.agama_createPotential(b"type=Multipole, file=/nonexistent")
     assert b"cannot open" in lib.agama_getError()
+
+
+def test_host_entry_points_share_a_handle_across_threads(pots):
+    """agx_eval_many / agx_eval_hess_many with host buffers stage through the handle's own stream
+    and a buffer kept between calls: calls of growing and shrinking sizes from four threads at
+    once on one handle give the device entry points' results bit for bit, and the calling
+    thread's current device is left as it was"""
+    import threading
+    import torch
+    from agama_amd import lib, workloads
+    gpu, _ = pots["dehnen_triax_lmax8.coef"]
+    pts = workloads.density_points(300_000, seed=14)
+    dev = torch.device("cuda:0")
+    d_pts = torch.from_numpy(pts).to(dev)
+    d_phi = torch.empty(len(pts), dtype=torch.float64, device=dev)
+    d_g = torch.empty((len(pts), 3), dtype=torch.float64, device=dev)
+    d_h = torch.empty((len(pts), 6), dtype=torch.float64, device=dev)
+    assert lib.agx_eval_hess_many_device(gpu.handle, len(pts), d_pts.data_ptr(), 0.0, d_phi.data_ptr(),
+                                         d_g.data_ptr(), d_h.data_ptr(), 0, None) == 0
+    torch.cuda.synchronize()
+    ref_phi, ref_g, ref_h = d_phi.cpu().numpy(), d_g.cpu().numpy(), d_h.cpu().numpy()
+    assert lib.agx_eval_many_device(gpu.handle, len(pts), d_pts.data_ptr(), 0.0, d_phi.data_ptr(), d_g.data_ptr(),
+                                    0, None) == 0
+    torch.cuda.synchronize()
+    ref_phi1, ref_g1 = d_phi.cpu().numpy(), d_g.cpu().numpy()
+    sizes = [1, 300_000, 17, 120_000, 5_000, 299_999, 64]
+    errors = []
+
+    def worker(k):
+        try:
+            for j, n in enumerate(sizes[k:] + sizes[:k]):
+                lo = (k * 7919 + j * 104729) % (len(pts) - n + 1)
+                phi, g, h = np.empty(n), np.empty((n, 3)), np.empty((n, 6))
+                sl = slice(lo, lo + n)
+                p = np.ascontiguousarray(pts[sl])
+                if (k + j) % 2:
+                    rc = lib.agx_eval_hess_many(gpu.handle, n, p.ctypes.data, 0.0, phi.ctypes.data, g.ctypes.data,
+                                                h.ctypes.data, 0)
+                    ok = np.array_equal(h, ref_h[sl]) and np.array_equal(phi, ref_phi[sl]) and \
+                        np.array_equal(g, ref_g[sl])
+                else:
+                    rc = lib.agx_eval_many(gpu.handle, n, p.ctypes.data, 0.0, phi.ctypes.data, g.ctypes.data, 0)
+                    ok = np.array_equal(phi, ref_phi1[sl]) and np.array_equal(g, ref_g1[sl])
+                if rc or not ok:
+                    errors.append((k, j, n, rc))
+        except Exception as e:   # pragma: no cover - reported below
+            errors.append(repr(e))
+    print("case: host-buffer evaluations from 4 threads on one handle", flush=True)
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    assert torch.cuda.current_device() == 0

@@ -106,9 +106,10 @@ def test_library_exports_every_declared_symbol():
     header = open(os.path.join(ROOT, "include", "agama_amd.h")).read()
     header = re.sub(r"/\*.*?\*/", "", header, flags=re.S)
     names = set(re.findall(r"\b([A-Za-z_]\w*)\s*\(", header)) - {"if", "defined", "sizeof"}
-    names = {n for n in names if n.startswith(("agx_", "agama_")) or n == "iniacceleration"}
+    names = {n for n in names if n.startswith(("agx_", "agama_")) or
+             n in ("iniacceleration", "inipotential", "potential_double", "potential_float")}
     names.discard("agx_acc_pter")
-    assert "agx_orbit_batch" in names and "agama_evalPotential" in names
+    assert "agx_orbit_batch" in names and "agama_evalPotential" in names and "potential_float" in names
     out = subprocess.check_output(["nm", "-D", "--defined-only", _native.LIB_PATH]).decode()
     exported = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
     missing = sorted(names - exported)
