@@ -116,7 +116,7 @@ void upload(agx_potential& p)
         oN = take(T.nodes.size() * 8), oF = take(T.nodes.size() * 4),
         oIS = take(nc * 8), oIU = take(nc * 8), oIW = take(nc * 8),
         oOS = take(nc * 8), oOU = take(nc * 8), oOW = take(nc * 8), oC = take(64),
-        oH = take(sizeof(AgxMultipoleDev));
+        oN1 = take(T.nodes1d.size() * 8), oH = take(sizeof(AgxMultipoleDev));
     HIPCHECK(hipGetDevice(&p.device));
     HIPCHECK(hipMalloc(&p.dmem, off));
     char* base = static_cast<char*>(p.dmem);
@@ -130,6 +130,7 @@ void upload(agx_potential& p)
     put(oF, nf.data(), nf.size() * 4);
     put(oIS, T.inner.S.data(), nc * 8); put(oIU, T.inner.U.data(), nc * 8); put(oIW, T.inner.W.data(), nc * 8);
     put(oOS, T.outer.S.data(), nc * 8); put(oOU, T.outer.U.data(), nc * 8); put(oOW, T.outer.W.data(), nc * 8);
+    put(oN1, T.nodes1d.data(), T.nodes1d.size() * 8);
     HIPCHECK(hipMemset(base + oC, 0, 64));
     AgxMultipoleDev& D = p.dev;
     memset(&D, 0, sizeof(D));
@@ -154,6 +155,11 @@ void upload(agx_potential& p)
     };
     fillPL(D.inner, T.inner, oIS, oIU, oIW);
     fillPL(D.outer, T.outer, oOS, oOU, oOW);
+    D.interp1d = T.interp1d;
+    D.nc1d = T.nc1d;
+    D.nodes1d = reinterpret_cast<const double*>(base + oN1);
+    D.ind1d.lmax = T.lmax1d; D.ind1d.mmax = T.mmax1d; D.ind1d.mmin = T.mmin1d; D.ind1d.step = T.step1d;
+    for(int i = 0; i < AGX_MAX_SLOTS; i++) D.ind1d.lmin[i] = i < (int)T.lmin1d.size() ? T.lmin1d[i] : 1 << 20;
     p.dcounter = reinterpret_cast<unsigned int*>(base + oC);
     p.ddev = reinterpret_cast<AgxMultipoleDev*>(base + oH);
     put(oH, &D, sizeof(D));
@@ -374,7 +380,12 @@ long agx_debug_tables(const char* coef_path, int which, int m, double* out)
                     v[q * n2 + i] = T.nodes[(i * T.ns + s) * AGX_NODE_STRIDE + q];
         } else if(which == 3)
             v = { (double)T.logScaling, T.invPhi0, (double)T.lmax, (double)T.mmax, (double)T.mmin, (double)T.step };
-        else return -1;
+        else if(which == 4) {   // 1-d variant: term c = m of the call: f, f_x, f_xx at every node
+            if(!T.interp1d || m < 0 || m >= T.nc1d || !T.term1d[m]) return 0;
+            v.resize(3 * (size_t)T.nr);
+            for(int q = 0; q < 3; q++)
+                for(int k = 0; k < T.nr; k++) v[q * T.nr + k] = T.nodes1d[((size_t)k * T.nc1d + m) * 4 + q];
+        } else return -1;
         if(out) std::copy(v.begin(), v.end(), out);
         return (long)v.size();
     } catch(std::exception& e) { fail(AGX_EINVAL, e.what()); return -1; }
@@ -672,7 +683,7 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
         }
         if(allowGroups && groupStride > 0 && n >= g_groupMinBatch && !variational && !d_targets && !pool.counter &&
            !(flags & AGX_METHOD_DPRKN8) && precision == AGX_PRECISION_FP64 && g_fetchOrbit == AGX_FETCH_VEC2 &&
-           g_orbitWaves == 1 && pot->nfwMass == 0) {
+           g_orbitWaves == 1 && pot->nfwMass == 0 && !pot->dev.interp1d) {
             const long waves = (long)blocks * (bs / 64);
             const long stride = std::min<long>(groupStride, waves);   // wavefront 0 is always a group one
             const long groupWaves = (waves + stride - 1) / stride;

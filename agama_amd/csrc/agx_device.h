@@ -503,6 +503,112 @@ __device__ __forceinline__ void interiorEvalGroup(const AgxMultipoleDev& P, cons
     gphi = inGrid ? trDphi : NAN;
 }
 
+// sphHarmTransformInverseDeriv[2] (potential_multipole.cpp:722-821) followed by
+// transformDerivsSphToCyl (:824-848): potential and (R, z, phi) gradient from the (l, m) terms
+// of the index set I (fields lmax, mmax, mmin, step, lmin[m + mmax]), where
+// term(l, m, C_lm, dC_lm / d ln r) supplies each term once; lmax >= 1
+template<class Ind, class Term>
+__device__ __forceinline__ void sphHarmSynthesis(const Ind& I, double R, double z, double phi, const Term& term,
+    double& pot, double& gR, double& gz, double& gphi)
+{
+    const double tau = z == 0 ? 0 : z / (sqrt(R * R + z * z) + R);
+    double vr, gr, gth, gph = 0;
+    if(I.lmax == 2 && I.mmin == 0 && I.step == 2) {
+        // sphHarmTransformInverseDeriv2 (potential_multipole.cpp:722-772)
+        const double C2 = sqrt(1.25), D2 = sqrt(3.75);
+        const double ct = 2 * tau / (1 + tau * tau), st = (1 - tau * tau) / (1 + tau * tau),
+            cc = ct * ct, cs = ct * st, ss = st * st, Y20 = (3 * C2 * cc - C2), dY20 = -6 * C2 * cs;
+        double P00, d00, P20, d20;
+        term(0, 0, P00, d00);
+        term(2, 0, P20, d20);
+        vr = Y20 * P20 + P00;
+        gr = Y20 * d20 + d00;
+        gth = dY20 * P20;
+        if(I.mmax == 2) {
+            double sp, cp, P22, d22;
+            term(2, 2, P22, d22);
+            const double Y22 = D2 * ss, dY22 = 2 * D2 * cs;
+            sincosA(2 * phi, sp, cp);
+            vr  += Y22 * P22 * cp;
+            gr  += Y22 * d22 * cp;
+            gth += dY22 * P22 * cp;
+            gph += Y22 * P22 * -2 * sp;
+        }
+    } else {
+        // sphHarmTransformInverseDeriv (potential_multipole.cpp:778-821): per harmonic m the
+        // Legendre recurrence of sphHarmArray (math_sphharm.cpp:63-132) streamed in l, then the
+        // Fourier synthesis in the reference's order (m = 0 first, then ascending m)
+        const double ct = 2 * tau / (1 + tau * tau), st = (1 - tau * tau) / (1 + tau * tau);
+        TrigRecT<double> pos, neg;
+        pos.init(phi);
+        neg.alpha = pos.alpha; neg.beta = pos.beta;
+        vr = gr = gth = 0;
+        for(int pass = 0; pass < 2; pass++)
+        for(int m = (pass == 0 ? 0 : I.mmin); m <= (pass == 0 ? 0 : I.mmax); m++) {
+            if(pass == 1 && m == 0) continue;
+            const int lmin = I.lmin[m + I.mmax];
+            if(lmin > I.lmax) continue;
+            const double mul = m == 0 ? 2 * 1.77245385090551602729816748334 :
+                2 * 1.77245385090551602729816748334 * M_SQRT2;
+            double Cv = 0, Cr = 0, Ct = 0;
+            const int am = m < 0 ? -m : m;
+            // legendrePmm
+            double fact2m = 1, dfact = 1;
+            for(int i = 2; i <= 2 * am; i++) fact2m *= i;
+            for(int i = 2 * am - 1; i > 1; i -= 2) dfact *= i;
+            double prefact = 0.5 / 1.77245385090551602729816748334 * sqrt((2 * am + 1) / fact2m);
+            double Pl, dPl;
+            if(am == 0) { Pl = prefact; dPl = 0; }
+            else if(am == 1) { Pl = -st * prefact; dPl = -ct * prefact; }
+            else {
+                const double coef = prefact * dfact * (am % 2 == 1 ? -1 : 1);
+                double sinm2 = 1, xb = st;
+                for(int n = am - 2; n; n >>= 1) { if(n % 2) sinm2 *= xb; xb *= xb; }
+                dPl = am * coef * sinm2 * st * ct;
+                Pl = coef * sinm2 * (st * st);
+            }
+            double Plm1 = Pl / prefact, Plm = ct * (2 * am + 1) * Plm1, Plm2 = 0;
+            for(int l = am; l <= I.lmax; l++) {
+                if(l > am) {
+                    if(l > am + 1) Plm = (ct * (2 * l - 1) * Plm1 - (l + am - 1) * Plm2) / (l - am);
+                    prefact *= sqrt((2 * l + 1.) / (2 * l - 1.) * (l - am) / (l + am));
+                    Pl = Plm * prefact;
+                    double d = 0;
+                    if(st >= 1e-8 || (am > 2 && st > 0)) d = (l * ct * Plm - (l + am) * Plm1) / st;
+                    else if(am == 0) d = -l * (l + 1) / 2 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
+                    else if(am == 1) d = -l * (l + 1) / 2 * (ct > 0 || l % 2 == 0 ? 1 : -1);
+                    else if(am == 2) d = l * (l + 1) * (l + 2) * (l - 1) / 4 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
+                    dPl = prefact * d;
+                    Plm2 = Plm1;
+                    Plm1 = Plm;
+                }
+                if(l >= lmin && (l - lmin) % I.step == 0) {
+                    double Pv, dPv;
+                    term(l, m, Pv, dPv);
+                    Cv += Pl * Pv * mul;
+                    Cr += Pl * dPv * mul;
+                    Ct += dPl * Pv * mul;
+                }
+            }
+            if(m == 0) { vr = Cv; gr = Cr; gth = Ct; continue; }
+            double trig, dtrig;
+            if(m > 0) { pos.advanceTo(m); trig = pos.c; dtrig = -m * pos.s; }
+            else { neg.restart(); neg.advanceTo(-m); trig = neg.s; dtrig = -m * neg.c; }
+            vr  += Cv * trig;
+            gr  += Cr * trig;
+            gth += Ct * trig;
+            gph += Cv * dtrig;
+        }
+    }
+    pot = vr;
+    // transformDerivsSphToCyl
+    const double r2inv = 1 / (R * R + z * z), drdR = R * r2inv, drdz = z * r2inv;
+    gR = gr * drdR + gth * drdz;
+    gz = gr * drdz + gth * -drdR;
+    gphi = gph;
+}
+
+
 // PowerLawMultipole::evalCyl: extrapolation inside rmin / outside rmax
 __device__ __forceinline__ void powerLawEval(const AgxPowerLawDev& L, double R, double z, double phi,
     double& pot, double& gR, double& gz, double& gphi)
@@ -530,101 +636,55 @@ __device__ __forceinline__ void powerLawEval(const AgxPowerLawDev& L, double R, 
         gR = dP0 * Rr2; gz = dP0 * zr2; gphi = 0;
         return;
     }
-    const double tau = z == 0 ? 0 : z / (sqrt(R * R + z * z) + R);
-    double vr, gr, gth, gph = 0;
-    if(L.lmax == 2 && L.mmin == 0 && L.step == 2) {
-        // sphHarmTransformInverseDeriv2 (potential_multipole.cpp:722-772)
-        const double C2 = sqrt(1.25), D2 = sqrt(3.75);
-        const double ct = 2 * tau / (1 + tau * tau), st = (1 - tau * tau) / (1 + tau * tau),
-            cc = ct * ct, cs = ct * st, ss = st * st, Y20 = (3 * C2 * cc - C2), dY20 = -6 * C2 * cs;
-        double P00, d00, P20, d20;
-        term(0, 0, P00, d00);
-        term(2, 0, P20, d20);
-        vr = Y20 * P20 + P00;
-        gr = Y20 * d20 + d00;
-        gth = dY20 * P20;
-        if(L.mmax == 2) {
-            double sp, cp, P22, d22;
-            term(2, 2, P22, d22);
-            const double Y22 = D2 * ss, dY22 = 2 * D2 * cs;
-            sincosA(2 * phi, sp, cp);
-            vr  += Y22 * P22 * cp;
-            gr  += Y22 * d22 * cp;
-            gth += dY22 * P22 * cp;
-            gph += Y22 * P22 * -2 * sp;
-        }
-    } else {
-        // sphHarmTransformInverseDeriv (potential_multipole.cpp:778-821): per harmonic m the
-        // Legendre recurrence of sphHarmArray (math_sphharm.cpp:63-132) streamed in l, then the
-        // Fourier synthesis in the reference's order (m = 0 first, then ascending m)
-        const double ct = 2 * tau / (1 + tau * tau), st = (1 - tau * tau) / (1 + tau * tau);
-        TrigRecT<double> pos, neg;
-        pos.init(phi);
-        neg.alpha = pos.alpha; neg.beta = pos.beta;
-        vr = gr = gth = 0;
-        for(int pass = 0; pass < 2; pass++)
-        for(int m = (pass == 0 ? 0 : L.mmin); m <= (pass == 0 ? 0 : L.mmax); m++) {
-            if(pass == 1 && m == 0) continue;
-            const int lmin = L.lmin[m + L.mmax];
-            if(lmin > L.lmax) continue;
-            const double mul = m == 0 ? 2 * 1.77245385090551602729816748334 :
-                2 * 1.77245385090551602729816748334 * M_SQRT2;
-            double Cv = 0, Cr = 0, Ct = 0;
-            const int am = m < 0 ? -m : m;
-            // legendrePmm
-            double fact2m = 1, dfact = 1;
-            for(int i = 2; i <= 2 * am; i++) fact2m *= i;
-            for(int i = 2 * am - 1; i > 1; i -= 2) dfact *= i;
-            double prefact = 0.5 / 1.77245385090551602729816748334 * sqrt((2 * am + 1) / fact2m);
-            double Pl, dPl;
-            if(am == 0) { Pl = prefact; dPl = 0; }
-            else if(am == 1) { Pl = -st * prefact; dPl = -ct * prefact; }
-            else {
-                const double coef = prefact * dfact * (am % 2 == 1 ? -1 : 1);
-                double sinm2 = 1, xb = st;
-                for(int n = am - 2; n; n >>= 1) { if(n % 2) sinm2 *= xb; xb *= xb; }
-                dPl = am * coef * sinm2 * st * ct;
-                Pl = coef * sinm2 * (st * st);
-            }
-            double Plm1 = Pl / prefact, Plm = ct * (2 * am + 1) * Plm1, Plm2 = 0;
-            for(int l = am; l <= L.lmax; l++) {
-                if(l > am) {
-                    if(l > am + 1) Plm = (ct * (2 * l - 1) * Plm1 - (l + am - 1) * Plm2) / (l - am);
-                    prefact *= sqrt((2 * l + 1.) / (2 * l - 1.) * (l - am) / (l + am));
-                    Pl = Plm * prefact;
-                    double d = 0;
-                    if(st >= 1e-8 || (am > 2 && st > 0)) d = (l * ct * Plm - (l + am) * Plm1) / st;
-                    else if(am == 0) d = -l * (l + 1) / 2 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
-                    else if(am == 1) d = -l * (l + 1) / 2 * (ct > 0 || l % 2 == 0 ? 1 : -1);
-                    else if(am == 2) d = l * (l + 1) * (l + 2) * (l - 1) / 4 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
-                    dPl = prefact * d;
-                    Plm2 = Plm1;
-                    Plm1 = Plm;
-                }
-                if(l >= lmin && (l - lmin) % L.step == 0 && l <= lmax) {
-                    double Pv, dPv;
-                    term(l, m, Pv, dPv);
-                    Cv += Pl * Pv * mul;
-                    Cr += Pl * dPv * mul;
-                    Ct += dPl * Pv * mul;
-                }
-            }
-            if(m == 0) { vr = Cv; gr = Cr; gth = Ct; continue; }
-            double trig, dtrig;
-            if(m > 0) { pos.advanceTo(m); trig = pos.c; dtrig = -m * pos.s; }
-            else { neg.restart(); neg.advanceTo(-m); trig = neg.s; dtrig = -m * neg.c; }
-            vr  += Cv * trig;
-            gr  += Cr * trig;
-            gth += Ct * trig;
-            gph += Cv * dtrig;
-        }
+    sphHarmSynthesis(L, R, z, phi, term, pot, gR, gz, gphi);
+}
+
+// MultipoleInterp1d::evalCyl (potential_multipole.cpp:1753-1815), the interior of expansions
+// with lmax <= 2: a quintic spline in ln r per (l, m) term (value and ln r derivative,
+// QuinticSpline::evalDeriv, math_spline.cpp:1405-1429), the l=0 term log-unscaled and the others
+// multiplied by it, then the spherical-harmonic synthesis.  Lanes outside the grid evaluate the
+// end cell (their result is replaced by the extrapolation).
+template<class GP>
+__device__ __forceinline__ void interior1dEval(const AgxMultipoleDev& P, double R, double z, double phi,
+    double& pot, double& gR, double& gz, double& gphi, GP gridR)
+{
+    const double r = sqrt(R * R + z * z), logr = log(r);
+    int k = binSearchD(logr, gridR, P.nr);
+    k = k < 0 ? 0 : k > P.nr - 2 ? P.nr - 2 : k;
+    QuinticWT<double> w;
+    w.init(logr, gridR[k], gridR[k + 1]);
+    const vd2 AGX_GLOBAL* nodes = gmem(reinterpret_cast<const vd2*>(P.nodes1d));
+    const int nc = P.nc1d;
+    auto spline = [&](int c, double& f, double& df) {
+        const int lo = (k * nc + c) * 2, hi = ((k + 1) * nc + c) * 2;
+        const vd2 a0 = nodes[lo], a1 = nodes[lo + 1], b0 = nodes[hi], b1 = nodes[hi + 1];
+        w.eval2(a0.x, b0.x, a0.y, b0.y, a1.x, b1.x, f, df);
+    };
+    double P0, dP0;
+    spline(0, P0, dP0);
+    if(P.logScaling) {
+        const double expX = exp(P0), Phi = 1 / (P.invPhi0 - expX), dPhidX = Phi * Phi * expX;
+        P0 = Phi;
+        dP0 *= dPhidX;
     }
-    pot = vr;
-    // transformDerivsSphToCyl
-    const double r2inv = 1 / (R * R + z * z), drdR = R * r2inv, drdz = z * r2inv;
-    gR = gr * drdR + gth * drdz;
-    gz = gr * drdz + gth * -drdR;
-    gphi = gph;
+    if(P.ind1d.lmax == 0) {   // spherical: d/d ln r only, then transformDerivsSphToCyl
+        const double r2inv = 1 / (R * R + z * z);
+        pot = P0;
+        gR = dP0 * (R * r2inv);
+        gz = dP0 * (z * r2inv);
+        gphi = 0;
+        return;
+    }
+    auto term = [&](int l, int m, double& Pv, double& dPv) {
+        const int c = l * (l + 1) + m;
+        if(c == 0) { Pv = P0; dPv = dP0; return; }
+        spline(c, Pv, dPv);
+        if(P.logScaling) {
+            dPv = dPv * P0 + Pv * dP0;
+            Pv *= P0;
+        }
+    };
+    sphHarmSynthesis(P.ind1d, R, z, phi, term, pot, gR, gz, gphi);
 }
 
 // Multipole::evalCyl dispatch (potential_multipole.cpp:1538-1548).  The interior spline is
@@ -638,8 +698,11 @@ __device__ __forceinline__ void multipoleCyl(const AgxMultipoleDev& P, const F& 
     GP gridR = nullptr, GP gridT = nullptr)
 {
     const double rsq = R * R + z * z;
-    interiorEval<F, GP, PF>(P, fetch, R, z, phi, pot, gR, gz, gphi, gridR ? gridR : GP(gmem(P.gridR)),
-        gridT ? gridT : GP(gmem(P.gridT)));
+    if(P.interp1d)
+        interior1dEval(P, R, z, phi, pot, gR, gz, gphi, gridR ? gridR : GP(gmem(P.gridR)));
+    else
+        interiorEval<F, GP, PF>(P, fetch, R, z, phi, pot, gR, gz, gphi, gridR ? gridR : GP(gmem(P.gridR)),
+            gridT ? gridT : GP(gmem(P.gridT)));
     if(rsq < P.rsqInner)
         powerLawEval(P.inner, R, z, phi, pot, gR, gz, gphi);
     else if(rsq > P.rsqOuter)

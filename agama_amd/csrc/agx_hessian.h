@@ -181,40 +181,31 @@ __device__ __forceinline__ void interiorEvalHess(const AgxMultipoleDev& P, const
 }
 
 // PowerLawMultipole::evalCyl with the Hessian
-__device__ __forceinline__ void powerLawEvalHess(const AgxPowerLawDev& L, double R, double z, double phi,
+// transformDerivsSphToCyl with the Hessian (potential_multipole.cpp:824-848): (ln r, theta, phi)
+// gradient g and Hessian h {dr2, dth2, dph2, drdth, drdph, dthdph} -> Cyl layout
+__device__ __forceinline__ void sphToCylDerivs(double R, double z, const double (&g)[3], const double (&h)[6],
+    double (&gc)[3], double (&hc)[6])
+{
+    // transformDerivsSphToCyl with the Hessian
+    const double r2inv = 1 / (R * R + z * z), drdR = R * r2inv, drdz = z * r2inv, dthdR = drdz, dthdz = -drdR;
+    gc[0] = g[0] * drdR + g[1] * dthdR;
+    gc[1] = g[0] * drdz + g[1] * dthdz;
+    gc[2] = g[2];
+    const double d2rdR2 = drdz * drdz - drdR * drdR, d2rdRdz = -2 * drdR * drdz, d2rdz2 = -d2rdR2;
+    sphToCylHess(g, h, drdR, drdz, dthdR, dthdz, d2rdR2, d2rdRdz, d2rdz2, d2rdRdz, -d2rdR2, -d2rdRdz, hc);
+}
+
+
+// sphHarmTransformInverseDeriv[2] with the Hessian (potential_multipole.cpp:722-821) and
+// transformDerivsSphToCyl (:824-848) for the (l, m) terms of the index set I; term(l, m, C, dC, d2C)
+// gives each term and its first two ln r derivatives; lmax >= 1
+template<class Ind, class Term>
+__device__ __forceinline__ void sphHarmSynthesisHess(const Ind& I, double R, double z, double phi, const Term& term,
     double& pot, double (&gc)[3], double (&hc)[6])
 {
-    const double rsq = R * R + z * z, dlogr = log(rsq / L.r0sq) * 0.5;
-    const int lmax = (L.inner && rsq < L.r0sq * 1e-16) || (!L.inner && rsq > L.r0sq * 1e16) ? 0 : L.lmax;
-    auto term = [&](int l, int m, double& Pv, double& dPv, double& d2Pv) {
-        const int c = l * (l + 1) + m;
-        const double s = gmem(L.S)[c], u = gmem(L.U)[c], w = gmem(L.W)[c], v = L.inner ? l : -l - 1;
-        const double rv  = v != 0 ? exp(dlogr * v) : 1;
-        const double rs  = s != v ? (s != 0 ? exp(dlogr * s) : 1) : rv;
-        const double urs = u * rs * (s != v || u == 0 ? 1 : dlogr);
-        const double wrv = w * rv;
-        const double qr2 = v == 0 ? L.Q * rsq / L.r0sq : 0;
-        Pv   = urs + wrv + qr2;
-        dPv  = urs * s + wrv * v + (s != v ? 0 : u * rs) + qr2 * 2;
-        d2Pv = urs * s * s + wrv * v * v + (s != v ? 0 : 2 * s * u * rs) + qr2 * 4;
-    };
-    if(lmax == 0) {
-        double P0, dP0, d2P0;
-        term(0, 0, P0, dP0, d2P0);
-        pot = P0;
-        const double rsqinv = rsq > 0 ? 1 / rsq : 0, Rr2 = rsq < INFINITY ? R * rsqinv : 0,
-            zr2 = rsq < INFINITY ? z * rsqinv : 0;
-        gc[0] = dP0 * Rr2; gc[1] = dP0 * zr2; gc[2] = 0;
-        const double d2 = d2P0 - 2 * dP0;
-        hc[0] = d2 * (Rr2 * Rr2) + dP0 * rsqinv;
-        hc[1] = d2 * (zr2 * zr2) + dP0 * rsqinv;
-        hc[3] = d2 * Rr2 * zr2;
-        hc[2] = hc[4] = hc[5] = 0;
-        return;
-    }
     const double tau = z == 0 ? 0 : z / (sqrt(R * R + z * z) + R);
     double vr, g[3] = { 0, 0, 0 }, h[6] = { 0, 0, 0, 0, 0, 0 };   // Sph: {dr2, dth2, dph2, drdth, drdph, dthdph}
-    if(L.lmax == 2 && L.mmin == 0 && L.step == 2) {
+    if(I.lmax == 2 && I.mmin == 0 && I.step == 2) {
         const double C2 = sqrt(1.25), D2 = sqrt(3.75);
         const double ct = 2 * tau / (1 + tau * tau), st = (1 - tau * tau) / (1 + tau * tau),
             cc = ct * ct, cs = ct * st, ss = st * st, Y20 = (3 * C2 * cc - C2), dY20 = -6 * C2 * cs,
@@ -228,7 +219,7 @@ __device__ __forceinline__ void powerLawEvalHess(const AgxPowerLawDev& L, double
         h[0] = Y20 * dd20 + dd00;
         h[3] = dY20 * d20;
         h[1] = d2Y20 * P20;
-        if(L.mmax == 2) {
+        if(I.mmax == 2) {
             double sp, cp, P22, d22, dd22;
             term(2, 2, P22, d22, dd22);
             const double Y22 = D2 * ss, dY22 = 2 * D2 * cs, d2Y22 = 4 * D2 * cc - 2 * D2;
@@ -252,10 +243,10 @@ __device__ __forceinline__ void powerLawEvalHess(const AgxPowerLawDev& L, double
         neg.alpha = pos.alpha; neg.beta = pos.beta;
         vr = 0;
         for(int pass = 0; pass < 2; pass++)
-        for(int m = (pass == 0 ? 0 : L.mmin); m <= (pass == 0 ? 0 : L.mmax); m++) {
+        for(int m = (pass == 0 ? 0 : I.mmin); m <= (pass == 0 ? 0 : I.mmax); m++) {
             if(pass == 1 && m == 0) continue;
-            const int lmin = L.lmin[m + L.mmax];
-            if(lmin > L.lmax) continue;
+            const int lmin = I.lmin[m + I.mmax];
+            if(lmin > I.lmax) continue;
             const double mul = m == 0 ? 2 * 1.77245385090551602729816748334 :
                 2 * 1.77245385090551602729816748334 * M_SQRT2;
             double Cv = 0, Cr = 0, Ct = 0, Crr = 0, Crt = 0, Ctt = 0;
@@ -278,7 +269,7 @@ __device__ __forceinline__ void powerLawEvalHess(const AgxPowerLawDev& L, double
             }
             double Plm1 = Pl / prefact, Plm = ct * (2 * am + 1) * Plm1, Plm2 = 0;
             double d2Plm1 = st, d2Plm2 = 0, d2Plm = 12 * ct * st;   // m = 1 pole recurrence
-            for(int l = am; l <= L.lmax; l++) {
+            for(int l = am; l <= I.lmax; l++) {
                 if(l > am) {
                     if(l > am + 1) Plm = (ct * (2 * l - 1) * Plm1 - (l + am - 1) * Plm2) / (l - am);
                     prefact *= sqrt((2 * l + 1.) / (2 * l - 1.) * (l - am) / (l + am));
@@ -309,7 +300,7 @@ __device__ __forceinline__ void powerLawEvalHess(const AgxPowerLawDev& L, double
                     Plm2 = Plm1;
                     Plm1 = Plm;
                 }
-                if(l >= lmin && (l - lmin) % L.step == 0 && l <= lmax) {
+                if(l >= lmin && (l - lmin) % I.step == 0) {
                     double Pv, dPv, d2Pv;
                     term(l, m, Pv, dPv, d2Pv);
                     Cv  += Pl * Pv * mul;
@@ -342,13 +333,86 @@ __device__ __forceinline__ void powerLawEvalHess(const AgxPowerLawDev& L, double
         }
     }
     pot = vr;
-    // transformDerivsSphToCyl with the Hessian
-    const double r2inv = 1 / (R * R + z * z), drdR = R * r2inv, drdz = z * r2inv, dthdR = drdz, dthdz = -drdR;
-    gc[0] = g[0] * drdR + g[1] * dthdR;
-    gc[1] = g[0] * drdz + g[1] * dthdz;
-    gc[2] = g[2];
-    const double d2rdR2 = drdz * drdz - drdR * drdR, d2rdRdz = -2 * drdR * drdz, d2rdz2 = -d2rdR2;
-    sphToCylHess(g, h, drdR, drdz, dthdR, dthdz, d2rdR2, d2rdRdz, d2rdz2, d2rdRdz, -d2rdR2, -d2rdRdz, hc);
+    sphToCylDerivs(R, z, g, h, gc, hc);
+}
+
+__device__ __forceinline__ void powerLawEvalHess(const AgxPowerLawDev& L, double R, double z, double phi,
+    double& pot, double (&gc)[3], double (&hc)[6])
+{
+    const double rsq = R * R + z * z, dlogr = log(rsq / L.r0sq) * 0.5;
+    const int lmax = (L.inner && rsq < L.r0sq * 1e-16) || (!L.inner && rsq > L.r0sq * 1e16) ? 0 : L.lmax;
+    auto term = [&](int l, int m, double& Pv, double& dPv, double& d2Pv) {
+        const int c = l * (l + 1) + m;
+        const double s = gmem(L.S)[c], u = gmem(L.U)[c], w = gmem(L.W)[c], v = L.inner ? l : -l - 1;
+        const double rv  = v != 0 ? exp(dlogr * v) : 1;
+        const double rs  = s != v ? (s != 0 ? exp(dlogr * s) : 1) : rv;
+        const double urs = u * rs * (s != v || u == 0 ? 1 : dlogr);
+        const double wrv = w * rv;
+        const double qr2 = v == 0 ? L.Q * rsq / L.r0sq : 0;
+        Pv   = urs + wrv + qr2;
+        dPv  = urs * s + wrv * v + (s != v ? 0 : u * rs) + qr2 * 2;
+        d2Pv = urs * s * s + wrv * v * v + (s != v ? 0 : 2 * s * u * rs) + qr2 * 4;
+    };
+    if(lmax == 0) {
+        double P0, dP0, d2P0;
+        term(0, 0, P0, dP0, d2P0);
+        pot = P0;
+        const double rsqinv = rsq > 0 ? 1 / rsq : 0, Rr2 = rsq < INFINITY ? R * rsqinv : 0,
+            zr2 = rsq < INFINITY ? z * rsqinv : 0;
+        gc[0] = dP0 * Rr2; gc[1] = dP0 * zr2; gc[2] = 0;
+        const double d2 = d2P0 - 2 * dP0;
+        hc[0] = d2 * (Rr2 * Rr2) + dP0 * rsqinv;
+        hc[1] = d2 * (zr2 * zr2) + dP0 * rsqinv;
+        hc[3] = d2 * Rr2 * zr2;
+        hc[2] = hc[4] = hc[5] = 0;
+        return;
+    }
+    sphHarmSynthesisHess(L, R, z, phi, term, pot, gc, hc);
+}
+
+// MultipoleInterp1d::evalCyl with the Hessian (potential_multipole.cpp:1753-1815): 1-d quintic
+// splines in ln r with second derivatives, the l=0 term log-unscaled, the others multiplied by it
+__device__ __forceinline__ void interior1dEvalHess(const AgxMultipoleDev& P, double R, double z, double phi,
+    double& pot, double (&gc)[3], double (&hc)[6])
+{
+    const double AGX_GLOBAL* gridR = gmem(P.gridR);
+    const double r = sqrt(R * R + z * z), logr = log(r);
+    int k = binSearchD(logr, gridR, P.nr);
+    k = k < 0 ? 0 : k > P.nr - 2 ? P.nr - 2 : k;
+    QuinticW2 w;
+    w.init(logr, gridR[k], gridR[k + 1]);
+    const vd2 AGX_GLOBAL* nodes = gmem(reinterpret_cast<const vd2*>(P.nodes1d));
+    const int nc = P.nc1d;
+    auto spline = [&](int c, double& f, double& df, double& d2f) {
+        const int lo = (k * nc + c) * 2, hi = ((k + 1) * nc + c) * 2;
+        const vd2 a0 = nodes[lo], a1 = nodes[lo + 1], b0 = nodes[hi], b1 = nodes[hi + 1];
+        w.eval3(a0.x, b0.x, a0.y, b0.y, a1.x, b1.x, f, df, d2f);
+    };
+    double P0, dP0, d2P0;
+    spline(0, P0, dP0, d2P0);
+    if(P.logScaling) {
+        const double expX = exp(P0), Phi = 1 / (P.invPhi0 - expX), dPhidX = Phi * Phi * expX;
+        P0 = Phi;
+        d2P0 = dPhidX * (d2P0 + dP0 * dP0 * Phi * (P.invPhi0 + expX));
+        dP0 *= dPhidX;
+    }
+    if(P.ind1d.lmax == 0) {
+        const double g[3] = { dP0, 0, 0 }, h[6] = { d2P0, 0, 0, 0, 0, 0 };
+        pot = P0;
+        sphToCylDerivs(R, z, g, h, gc, hc);
+        return;
+    }
+    auto term = [&](int l, int m, double& Pv, double& dPv, double& d2Pv) {
+        const int c = l * (l + 1) + m;
+        if(c == 0) { Pv = P0; dPv = dP0; d2Pv = d2P0; return; }
+        spline(c, Pv, dPv, d2Pv);
+        if(P.logScaling) {
+            d2Pv = d2Pv * P0 + 2 * dPv * dP0 + Pv * d2P0;
+            dPv = dPv * P0 + Pv * dP0;
+            Pv *= P0;
+        }
+    };
+    sphHarmSynthesisHess(P.ind1d, R, z, phi, term, pot, gc, hc);
 }
 
 // NFW: potential, radial first and second derivatives
@@ -372,6 +436,7 @@ __device__ __forceinline__ void multipoleCylHess(const AgxMultipoleDev& P, const
     const double rsq = R * R + z * z;
     if(rsq < P.rsqInner) powerLawEvalHess(P.inner, R, z, phi, pot, gc, hc);
     else if(rsq > P.rsqOuter) powerLawEvalHess(P.outer, R, z, phi, pot, gc, hc);
+    else if(P.interp1d) interior1dEvalHess(P, R, z, phi, pot, gc, hc);
     else interiorEvalHess(P, fetch, R, z, phi, pot, gc, hc);
     if(P.nfwMass != 0) {
         const double r = sqrt(R * R + z * z);
@@ -417,6 +482,7 @@ __device__ __forceinline__ void multipoleCarHess(const AgxMultipoleDev& P, const
     double gc[3], hc[6];
     if(rsq < P.rsqInner) powerLawEvalHess(P.inner, R, z, phi, pot, gc, hc);
     else if(rsq > P.rsqOuter) powerLawEvalHess(P.outer, R, z, phi, pot, gc, hc);
+    else if(P.interp1d) interior1dEvalHess(P, R, z, phi, pot, gc, hc);
     else interiorEvalHess(P, fetch, R, z, phi, pot, gc, hc);
     // toGrad<Cyl,Car>, toHess<Cyl,Car>
     const double dR2 = hc[0], dz2 = hc[1], dphi2 = hc[2], dRdz = hc[3], dzdphi = hc[4], dRdphi = hc[5];

@@ -7,6 +7,10 @@
 //     nodes[((i*nt + j)*ns + s)*AGX_NODE_STRIDE + q],  i < nr (ln r), j < nt (tau), s < ns,
 //     q = 0..8 : f, f_x, f_y, f_xx, f_xy, f_yy, f_xxy, f_xyy, f_xxyy   (q = 9: padding)
 //   so a point's 4 cell corners for all harmonics are two contiguous runs of 2*ns*80 bytes.
+// Interior for lmax <= 2 (MultipoleInterp1d, src/potential_multipole.cpp:1691-1815): one 1-D
+//   quintic spline in ln r per (l,m) term, node-major:
+//     nodes1d[(i*nc1d + c)*4 + q],  i < nr, c = l*(l+1)+m < nc1d = (lmax+1)^2,
+//     q = 0..2 : f, f_x, f_xx  (q = 3: padding)
 // Extrapolation (PowerLawMultipole, src/potential_multipole.cpp:1585-1658): per (l,m) term
 //   S, U, W indexed by c = l*(l+1)+m, plus Q for the inner monopole.
 #pragma once
@@ -27,6 +31,12 @@ struct AgxPowerLawDev {
     const double* W;
 };
 
+// index set of a spherical-harmonic expansion (math::SphHarmIndices, src/math_sphharm.h:116-157)
+struct AgxSphIndDev {
+    int lmax, mmax, mmin, step;
+    int lmin[AGX_MAX_SLOTS];   // lmin(m) at [m + mmax]; > lmax means the harmonic is empty
+};
+
 struct AgxMultipoleDev {
     int nr, nt, ns;             // radial nodes, tau nodes, stored harmonic slots
     int mmax, mmin;             // azimuthal index range of the expansion
@@ -41,6 +51,12 @@ struct AgxMultipoleDev {
     const double* nodes;        // node-major spline data (see above)
     const float* nodesF;        // the same data rounded to float (precision-split form), or null
     AgxPowerLawDev inner, outer;
+    // lmax <= 2: the 1-D variant replaces the 2-D tables (nt = ns = 0); logScaling and invPhi0
+    // then refer to the l=0 coefficient Phi_00 (MultipoleInterp1d's convention)
+    int interp1d;
+    int nc1d;
+    AgxSphIndDev ind1d;
+    const double* nodes1d;
     // optional analytic NFW halo added to the expansion (a two-component Composite, as in
     // src/potential_composite.cpp:51-163): Phi = -M ln(1 + r/a) / r; nfwMass == 0: none
     double nfwMass, nfwScale;

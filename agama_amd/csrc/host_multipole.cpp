@@ -1041,6 +1041,69 @@ Coefs truncateCoefs(const Coefs& C, int lmax)
     return t;
 }
 
+// MultipoleInterp1d ctor (potential_multipole.cpp:1691-1738): the extrapolation's 1/Phi_00(0)
+// from the two innermost nodes, log-scaling when every Phi_00(r_k) < 0, and a quintic spline in
+// ln r (values and ln r derivatives at the nodes, second derivatives by constructQuinticSpline,
+// math_spline.cpp:1376-1389) for each (l,m) term of the index set
+static HostTables buildTables1d(const Coefs& C, const Ind& ind)
+{
+    const int nr = (int)C.radii.size();
+    HostTables T;
+    T.interp1d = 1;
+    T.nr = nr; T.nt = 0; T.ns = 0;
+    T.lmax = ind.lmax; T.mmax = ind.mmax; T.mmin = ind.mmin(); T.step = ind.step; T.sym = ind.sym;
+    double s, U, W, Q;
+    extrapolationCoefs(C.Phi[0][0], C.Phi[0][1], C.dPhi[0][0], C.dPhi[0][1], C.radii[0], C.radii[1],
+        0, NAN, s, U, W, Q);
+    T.invPhi0 = s > 0 ? 1./W : 0;
+    bool logScaling = true;
+    T.gridR.resize(nr);
+    for(int k = 0; k < nr; k++) {
+        T.gridR[k] = log(C.radii[k]);
+        logScaling &= C.Phi[0][k] < 0;
+        if(C.Phi[0][k] * T.invPhi0 >= 1) T.invPhi0 = 0;
+    }
+    T.logScaling = logScaling;
+    T.nc1d = (ind.lmax + 1) * (ind.lmax + 1);
+    T.nodes1d.assign((size_t)nr * T.nc1d * 4, 0.);
+    T.term1d.assign(T.nc1d, 0);
+    std::vector<double> f(nr), df(nr);
+    for(int m = ind.mmin(); m <= ind.mmax; m++)
+        for(int l = ind.lmin(m); l <= ind.lmax; l += ind.step) {
+            const unsigned c = Ind::index(l, m);
+            T.term1d[c] = 1;
+            for(int k = 0; k < nr; k++) {
+                if(logScaling) {
+                    if(c == 0) {
+                        f[k]  = log(T.invPhi0 - 1 / C.Phi[c][k]);
+                        df[k] = C.radii[k] * C.dPhi[c][k] / (C.Phi[c][k] * (T.invPhi0 * C.Phi[c][k] - 1));
+                    } else {
+                        f[k]  = C.Phi[c][k] / C.Phi[0][k];
+                        df[k] = (C.dPhi[c][k] - f[k] * C.dPhi[0][k]) * C.radii[k] / C.Phi[0][k];
+                    }
+                } else {
+                    f[k] = C.Phi[c][k];
+                    df[k] = C.radii[k] * C.dPhi[c][k];
+                }
+                if(!finite(f[k]) || !finite(df[k]))
+                    throw std::invalid_argument("QuinticSpline: non-finite input");
+            }
+            const std::vector<double> d2 = quinticSecondDerivs(T.gridR, f, df);
+            for(int k = 0; k < nr; k++) {
+                double* dst = &T.nodes1d[((size_t)k * T.nc1d + c) * 4];
+                dst[0] = f[k]; dst[1] = df[k]; dst[2] = d2[k];
+            }
+        }
+    T.lmax1d = ind.lmax; T.mmax1d = ind.mmax; T.mmin1d = ind.mmin(); T.step1d = ind.step;
+    T.lmin1d.resize(2 * ind.mmax + 1);
+    for(int m = -ind.mmax; m <= ind.mmax; m++) T.lmin1d[m + ind.mmax] = ind.lmin(m);
+    T.rsqInner = sq(C.radii.front() * (1 + ROOT3_DBL_EPS));
+    T.rsqOuter = sq(C.radii.back() * (1 - ROOT3_DBL_EPS));
+    T.inner = powerLaw(C, true);
+    T.outer = powerLaw(C, false);
+    return T;
+}
+
 HostTables buildTables(const Coefs& C, bool allowLowOrder)
 {
     const int nr = (int)C.radii.size();
@@ -1050,10 +1113,8 @@ HostTables buildTables(const Coefs& C, bool allowLowOrder)
         correct &= (int)C.Phi[c].size() == nr && (int)C.dPhi[c].size() == nr;
     for(int k = 1; correct && k < nr; k++) correct &= C.radii[k] > C.radii[k-1];
     if(!correct) throw std::invalid_argument("Multipole: invalid radial grid");
-    if(ind.lmax <= 2 && !allowLowOrder)
-        throw std::invalid_argument("Multipole: expansions with lmax<=2 (the reference's 1-d spline "
-            "variant) are not supported by this build");
     if(ind.mmax > AGX_MAX_M) throw std::invalid_argument("Multipole: mmax > 16 is not supported");
+    if(ind.lmax <= 2 && !allowLowOrder) return buildTables1d(C, ind);
 
     HostTables T;
     T.nr = nr; T.lmax = ind.lmax; T.mmax = ind.mmax; T.mmin = ind.mmin(); T.step = ind.step; T.sym = ind.sym;

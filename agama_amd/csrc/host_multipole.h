@@ -97,9 +97,16 @@ struct HostTables {
     int zeroSlot = 0;
     std::vector<double> gridR, gridT, nodes;   // nodes: see agx_tables.h
     HostPowerLaw inner, outer;
+    // lmax <= 2 (the reference's MultipoleInterp1d): 1-D splines per (l,m) term, no 2-D tables
+    int interp1d = 0, nc1d = 0;
+    int lmax1d = 0, mmax1d = 0, mmin1d = 0, step1d = 1;
+    std::vector<int> lmin1d;        // [m + mmax1d]
+    std::vector<double> nodes1d;    // see agx_tables.h
+    std::vector<char> term1d;       // [c]: 1 for the (l,m) terms of the index set
 };
-// allowLowOrder: also build the 2-d tables for expansions with lmax <= 2 (the reference uses
-// 1-d splines there; the 2-d form is used internally, e.g. for the surrogate's l<=2 baseline)
+// Expansions with lmax <= 2 get the reference's 1-d splines per (l,m) term (MultipoleInterp1d);
+// allowLowOrder: build the 2-d tables for them instead (used internally, e.g. for the
+// surrogate's l<=2 baseline)
 HostTables buildTables(const Coefs& c, bool allowLowOrder = false);
 // the expansion with every term of degree l > lmax set to zero
 Coefs truncateCoefs(const Coefs& c, int lmax);

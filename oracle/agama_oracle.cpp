@@ -417,6 +417,43 @@ inline void evalQuinticSplines(const double x, const double xl, const double xh,
     }
 }
 
+// math_spline.cpp:1376-1389 (QuinticSpline from values and first derivatives, second
+// derivatives by constructQuinticSpline) and :1405-1429 (evalDeriv: linear continuation beyond the
+// grid, NaN for a NaN argument)
+struct QuinticSpline1d {
+    std::vector<double> xval, fval, fder, fder2;
+    QuinticSpline1d() {}
+    QuinticSpline1d(const std::vector<double>& x, const std::vector<double>& f, const std::vector<double>& d) :
+        xval(x), fval(f), fder(d)
+    {
+        for(size_t i=0; i<f.size(); i++)
+            if(!isFinite(f[i]) || !isFinite(d[i]))
+                throw std::invalid_argument("QuinticSpline: non-finite input");
+        fder2 = constructQuinticSpline(xval, fval, fder);
+    }
+    void evalDeriv(const double x, double* value, double* deriv, double* deriv2) const
+    {
+        const long size = xval.size();
+        auto fill = [&](double a, double b, double c) {
+            if(value) *value = a;
+            if(deriv) *deriv = b;
+            if(deriv2) *deriv2 = c;
+        };
+        if(size == 0 || x!=x) { fill(NAN, NAN, NAN); return; }
+        const long index = binSearch(x, &xval[0], size);
+        if(index < 0) {
+            fill(fval[0] + (fder[0]==0 ? 0 : fder[0] * (x-xval[0])), fder[0], 0);
+            return;
+        }
+        if(index >= size-1) {
+            fill(fval[size-1] + (fder[size-1]==0 ? 0 : fder[size-1] * (x-xval[size-1])), fder[size-1], 0);
+            return;
+        }
+        evalQuinticSplines<1>(x, xval[index], xval[index+1], &fval[index], &fval[index+1], &fder[index],
+            &fder[index+1], &fder2[index], &fder2[index+1], value, deriv, deriv2);
+    }
+};
+
 // math_spline.cpp:2321-2359, 2478-2533, 2535-2597  2-D quintic spline with mixed derivative
 struct QuinticSpline2d {
     std::vector<double> xval, yval, fval, fx, fy, fxx, fxy, fyy, fxxy, fxyy, fxxyy;
@@ -1307,9 +1344,117 @@ inline void cylToCar(const CarCylDeriv& d, const GradCyl& g, const HessCyl& h, d
 }
 
 // potential_multipole.cpp:1491-1516, 1538-1548 ; potential_base.h:240-243 ; coord.h:831-854
+// potential_multipole.cpp:1354-1375, 1691-1815  MultipoleInterp1d: a 1-d quintic spline in
+// ln r for each (l, m) term, the l=0 term log-scaled as ln(invPhi0 - 1/Phi_00) and the others
+// divided by Phi_00 when every Phi_00(r_k) < 0; the reference uses it for lmax <= 2 (:40-43,
+// 1509-1511)
+struct MultipoleInterp1d {
+    SphHarmIndices ind;
+    std::vector<QuinticSpline1d> spl;
+    bool logScaling;
+    double invPhi0;
+
+    MultipoleInterp1d(const std::vector<double>& radii,
+        const std::vector<std::vector<double> >& Phi,
+        const std::vector<std::vector<double> >& dPhi) :
+        ind(getIndicesFromCoefs2(Phi, dPhi)), logScaling(true)
+    {
+        unsigned int gridSizeR = radii.size();
+        double s, U, W, Q;
+        computeExtrapolationCoefs(Phi[0][0], Phi[0][1], dPhi[0][0], dPhi[0][1], radii[0], radii[1],
+            0, NAN, s, U, W, Q);
+        invPhi0 = s>0 ? 1./W : 0;
+        std::vector<double> gridR(gridSizeR);
+        for(unsigned int k=0; k<gridSizeR; k++) {
+            gridR[k] = log(radii[k]);
+            logScaling &= Phi[0][k] < 0;
+            if(Phi[0][k]*invPhi0 >= 1)
+                invPhi0 = 0;
+        }
+        std::vector<double> Phi_lm(gridSizeR), dPhi_lm(gridSizeR);
+        spl.resize(ind.size());
+        for(int m=ind.mmin(); m<=ind.mmax; m++)
+            for(int l=ind.lmin(m); l<=ind.lmax; l+=ind.step) {
+                unsigned int c = SphHarmIndices::index(l, m);
+                for(unsigned int k=0; k<gridSizeR; k++) {
+                    if(logScaling) {
+                        if(c==0) {
+                            Phi_lm [k] = log(invPhi0 - 1 / Phi[c][k]);
+                            dPhi_lm[k] = radii[k] * dPhi[c][k] / (Phi[c][k] * (invPhi0 * Phi[c][k] - 1));
+                        } else {
+                            Phi_lm [k] = Phi[c][k] / Phi[0][k];
+                            dPhi_lm[k] = (dPhi[c][k] - Phi_lm[k] * dPhi[0][k]) * radii[k] / Phi[0][k];
+                        }
+                    } else {
+                        Phi_lm [k] = Phi[c][k];
+                        dPhi_lm[k] = radii[k] * dPhi[c][k];
+                    }
+                }
+                spl[c] = QuinticSpline1d(gridR, Phi_lm, dPhi_lm);
+            }
+    }
+
+    // potential_multipole.cpp:1753-1815; a Hessian request needs grad as well
+    void evalCyl(const PosCyl& pos, double* potential, GradCyl* grad, HessCyl* hess = NULL) const
+    {
+        bool needGrad = grad!=NULL || hess!=NULL;
+        bool needHess = hess!=NULL;
+        double r = sqrt(pow_2(pos.R) + pow_2(pos.z)), logr = log(r);
+        int ncoefs = pow_2(ind.lmax + 1);
+        std::vector<double> Phi_lm(ncoefs), dPhi_lm(ncoefs), d2Phi_lm(ncoefs);
+        GradSph gradSph;
+        HessSph hessSph;
+        spl[0].evalDeriv(logr, &Phi_lm[0], needGrad ? &dPhi_lm[0] : NULL, needHess ? &d2Phi_lm[0] : NULL);
+        if(logScaling) {
+            double expX = exp(Phi_lm[0]), Phi = 1 / (invPhi0 - expX);
+            Phi_lm[0] = Phi;
+            if(needGrad) {
+                double dPhidX = pow_2(Phi) * expX;
+                if(needHess)
+                    d2Phi_lm[0] = dPhidX * (d2Phi_lm[0] + pow_2(dPhi_lm[0]) * Phi * (invPhi0 + expX));
+                dPhi_lm[0] *= dPhidX;
+            }
+        }
+        if(ind.lmax == 0) {
+            if(potential)
+                *potential = Phi_lm[0];
+            if(needGrad) {
+                gradSph.dr = dPhi_lm[0];
+                gradSph.dtheta = gradSph.dphi = 0;
+            }
+            if(needHess) {
+                hessSph.dr2 = d2Phi_lm[0];
+                hessSph.dtheta2 = hessSph.dphi2 = hessSph.drdtheta = hessSph.drdphi = hessSph.dthetadphi = 0;
+            }
+        } else {
+            for(int m=ind.mmin(); m<=ind.mmax; m++)
+                for(int l=ind.lmin(m); l<=ind.lmax; l+=ind.step) {
+                    unsigned int c = SphHarmIndices::index(l, m);
+                    if(c==0)
+                        continue;
+                    spl[c].evalDeriv(logr, &Phi_lm[c], needGrad ? &dPhi_lm[c] : NULL,
+                        needHess ? &d2Phi_lm[c] : NULL);
+                    if(logScaling) {
+                        if(needHess)
+                            d2Phi_lm[c] = d2Phi_lm[c] * Phi_lm[0] + 2 * dPhi_lm[c] * dPhi_lm[0] +
+                                Phi_lm[c] * d2Phi_lm[0];
+                        if(needGrad)
+                            dPhi_lm[c] = dPhi_lm[c] * Phi_lm[0] + Phi_lm[c] * dPhi_lm[0];
+                        Phi_lm[c] *= Phi_lm[0];
+                    }
+                }
+            sphHarmTransformInverseDeriv(ind, pos, &Phi_lm[0], &dPhi_lm[0], &d2Phi_lm[0], potential,
+                needGrad ? &gradSph : NULL, needHess ? &hessSph : NULL);
+        }
+        if(needGrad)
+            transformDerivsSphToCyl(pos, gradSph, grad, needHess ? &hessSph : NULL, hess);
+    }
+};
+
 struct OrcMultipole : public orc_pot {
     std::vector<double> gridRadii;
-    std::unique_ptr<MultipoleInterp2d> impl;
+    std::unique_ptr<MultipoleInterp2d> impl;     // lmax > 2
+    std::unique_ptr<MultipoleInterp1d> impl1d;   // lmax <= 2 (potential_multipole.cpp:1509-1511)
     PowerLawMultipole asymptInner, asymptOuter;
 
     OrcMultipole(const std::vector<double>& radii, const std::vector<std::vector<double> >& Phi,
@@ -1325,8 +1470,9 @@ struct OrcMultipole : public orc_pot {
         if(!correct)
             throw std::invalid_argument("Multipole: invalid radial grid");
         if(ind.lmax <= 2)
-            throw std::invalid_argument("Multipole: the 1-d spline variant (lmax<=2) is not restated");
-        impl.reset(new MultipoleInterp2d(gridRadii, Phi, dPhi));
+            impl1d.reset(new MultipoleInterp1d(gridRadii, Phi, dPhi));
+        else
+            impl.reset(new MultipoleInterp2d(gridRadii, Phi, dPhi));
         asymptInner = initAsympt(gridRadii, Phi, dPhi, true);
         asymptOuter = initAsympt(gridRadii, Phi, dPhi, false);
     }
@@ -1338,11 +1484,13 @@ struct OrcMultipole : public orc_pot {
             asymptInner.evalCyl(pos, potential, deriv, deriv2);
         else if(rsq > pow_2(gridRadii.back() * (1-ROOT3_DBL_EPS)))
             asymptOuter.evalCyl(pos, potential, deriv, deriv2);
+        else if(impl1d)
+            impl1d->evalCyl(pos, potential, deriv, deriv2);
         else
             impl->evalCyl(pos, potential, deriv, deriv2);
     }
 
-    virtual int symmetry() const { return impl->ind.sym; }
+    virtual int symmetry() const { return impl1d ? impl1d->ind.sym : impl->ind.sym; }
     // grad {dR, dz, dphi}, hess {dR2, dz2, dphi2, dRdz, dzdphi, dRdphi}
     virtual void evalCylHess(double R, double z, double phi, double* pot, double grad[3], double hess[6]) const
     {
@@ -3015,6 +3163,21 @@ long orc_multipole_tables(const orc_pot* p, int which, int m, double* out)
 {
     const OrcMultipole* mp = dynamic_cast<const OrcMultipole*>(p);
     if(!mp) return -1;
+    if(mp->impl1d) {   // the 1-d variant: grid, scalars, and per term c (= m) f, f_x, f_xx
+        const MultipoleInterp1d& J = *mp->impl1d;
+        std::vector<double> v;
+        if(which == 0) v = J.spl[0].xval;
+        else if(which == 3)
+            v = { J.logScaling ? 1.0 : 0.0, J.invPhi0, (double)J.ind.lmax, (double)J.ind.mmax,
+                  (double)J.ind.mmin(), (double)J.ind.step };
+        else if(which == 4) {
+            if(m < 0 || m >= (int)J.spl.size() || J.spl[m].fval.empty()) return 0;
+            const QuinticSpline1d& S = J.spl[m];
+            v = S.fval; v.insert(v.end(), S.fder.begin(), S.fder.end()); v.insert(v.end(), S.fder2.begin(), S.fder2.end());
+        } else return -1;
+        if(out) std::copy(v.begin(), v.end(), out);
+        return (long)v.size();
+    }
     const MultipoleInterp2d& I = *mp->impl;
     std::vector<double> v;
     if(which == 0) v = I.gridR;

@@ -16,6 +16,10 @@ FIXTURES = {
     # triaxial Dehnen gamma=1, axis ratios 1:0.8:0.6, lmax=8, 50 radial nodes in [1e-3, 1e3]
     "dehnen_triax_lmax8.coef": "type=Multipole, density=Dehnen, gamma=1, mass=1, scaleRadius=1, "
                                "axisRatioY=0.8, axisRatioZ=0.6, lmax=8, gridSizeR=50, rmin=1e-3, rmax=1e3",
+    # the same density expanded with lmax=2, 30 nodes: the reference evaluates it with 1-d splines
+    # per (l, m) term and the lmax=2 fast path of the spherical-harmonic synthesis
+    "dehnen_triax_lmax2.coef": "type=Multipole, density=Dehnen, gamma=1, mass=1, scaleRadius=1, "
+                               "axisRatioY=0.8, axisRatioZ=0.6, lmax=2, gridSizeR=30, rmin=1e-3, rmax=1e3",
     # spherical Plummer (closed-form potential -M/sqrt(r^2+a^2)) expanded with lmax=4
     "plummer_lmax4.coef": "type=Multipole, density=Plummer, mass=1, scaleRadius=1, lmax=4, "
                           "gridSizeR=40, rmin=1e-2, rmax=1e2, symmetry=Triaxial",

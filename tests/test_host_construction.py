@@ -130,3 +130,42 @@ def test_orbit_group_settings_validation():
     with pytest.raises(agama_amd.AgamaAmdError):
         agama_amd.set_orbit_groups(stride=-3)
     agama_amd.set_orbit_groups()
+
+
+def write_coef_file(path, radii, Phi, dPhi, symmetry="None"):
+    """a Multipole coefficient file in the reference's text format (potential_factory.cpp:834-852,
+    909-929): header, the #Phi and #dPhi/dr blocks of (l+1)^2 columns, values as repr()"""
+    nc = Phi.shape[0]
+    lmax = int(round(np.sqrt(nc))) - 1
+    cols = "\t".join(f"l={l},m={m}" for l in range(lmax + 1) for m in range(-l, l + 1))
+    lines = ["[Potential]", "type=Multipole", f"gridSizeR={len(radii)}", f"lmax={lmax}", f"symmetry={symmetry}",
+             "Coefficients"]
+    for tag, block in (("#Phi", Phi), ("#dPhi/dr", dPhi)):
+        lines += [tag, "#radius\t" + cols]
+        lines += ["\t".join(repr(float(v)) for v in [radii[k]] + list(block[:, k])) for k in range(len(radii))]
+        lines.append("")
+    open(path, "w").write("\n".join(lines) + "\n")
+
+
+@pytest.mark.parametrize("case", ["plummer_lmax4.coef", "dehnen_triax_lmax2.coef", "general_lmax2"])
+def test_interp1d_tables_bitwise_equal_to_oracle(case, tmp_path):
+    """expansions with lmax <= 2 are built as the reference's MultipoleInterp1d
+    (potential_multipole.cpp:1691-1738): grid, log-scaling flag and 1/Phi_00(0), and each (l, m)
+    term's spline values, ln r derivatives and second derivatives agree bit for bit"""
+    from tests.test_oracle_multipole import synthetic_general_coefs
+    if case == "general_lmax2":
+        path = str(tmp_path / "general.coef")
+        write_coef_file(path, *synthetic_general_coefs())
+    else:
+        path = os.path.join(GOLDEN, case)
+    orc = OraclePotential(path)
+    assert np.array_equal(product_tables(path, 0), orc.tables(0))
+    assert np.array_equal(product_tables(path, 3), orc.tables(3))
+    lmax = int(orc.tables(3)[2])
+    terms = 0
+    for c in range((lmax + 1) ** 2):
+        a, b = product_tables(path, 4, c), orc.tables(4, c)
+        assert np.array_equal(a, b), c
+        terms += a.size > 0
+    assert terms == {"plummer_lmax4.coef": 1, "dehnen_triax_lmax2.coef": 3, "general_lmax2": 9}[case]
+    assert lib.agx_debug_tables(path.encode(), 2, 0, None) == 0     # no 2-d tables

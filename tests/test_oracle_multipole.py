@@ -222,3 +222,71 @@ def test_poisson_route_equals_independent_quadrature(tmp_path):
               f"difference {e:.1e} of Phi_00")
         worst = max(worst, e)
     assert worst <= 1e-8
+
+
+def synthetic_general_coefs():
+    """coefficients with every (l, m) of lmax = 2 non-zero (odd l, sine terms: no symmetry, so
+    the general synthesis path rather than the lmax = 2 fast path): Phi_00 = 3 f, the others
+    a_lm g, f = -1/sqrt(r^2+1), g = -r^2/(r^2+1)^(5/2), derivatives in closed form"""
+    r = np.logspace(-2, 2, 30)
+    f, df = -1 / np.sqrt(r * r + 1), r / (r * r + 1) ** 1.5
+    g = -r * r / (r * r + 1) ** 2.5
+    dg = (-2 * r * (r * r + 1) + 5 * r ** 3) / (r * r + 1) ** 3.5
+    a = np.random.default_rng(3).uniform(-0.2, 0.2, 9)
+    a[0] = 3.0
+    P = np.array([a[c] * (f if c == 0 else g) for c in range(9)])
+    dP = np.array([a[c] * (df if c == 0 else dg) for c in range(9)])
+    return r, P, dP
+
+
+@pytest.mark.parametrize("case", ["plummer_lmax4.coef", "dehnen_triax_lmax2.coef", "general_lmax2"])
+def test_interp1d_pins(case):
+    """the reference's 1-d variant for lmax <= 2 (MultipoleInterp1d, src/potential_multipole.cpp:
+    1691-1815): one quintic spline in ln r per (l, m) term, so at every grid radius the potential
+    in ANY direction equals the direct synthesis of the file's coefficients (and dPhi/dr that of
+    the #dPhi/dr block); gradient = finite differences; continuity across the branch radii; the
+    Plummer expansion (l = 0 only) equals -M/sqrt(r^2+a^2) between the nodes"""
+    from oracle.pyoracle import OraclePotential
+    from agama_amd.workloads import read_coef_file
+    radii, P, dP = synthetic_general_coefs() if case == "general_lmax2" else read_coef_file(os.path.join(GOLDEN, case))
+    orc = OraclePotential(coefs=(radii, P, dP))
+    d = unit_vectors(64, 5)
+    costh, phi = d[:, 2], np.arctan2(d[:, 1], d[:, 0])
+    worst_p = worst_d = 0.0
+    for k in range(1, len(radii) - 1):
+        p, g = orc.eval(d * radii[k])
+        s, ds = synthesis(P, k, costh, phi), synthesis(dP, k, costh, phi)
+        worst_p = max(worst_p, np.max(np.abs(p - s) / np.abs(s)))
+        worst_d = max(worst_d, np.max(np.abs(np.sum(d * g, axis=1) - ds)) / np.max(np.abs(ds)))
+    # finite differences over and beyond the grid
+    gen = np.random.default_rng(11)
+    r = np.exp(gen.uniform(np.log(radii[0] * 1e-2), np.log(radii[-1] * 1e2), 2000))
+    far = (np.abs(np.log(r / (radii[0] * (1 + EPS_BRANCH)))) > 0.01) & \
+        (np.abs(np.log(r / (radii[-1] * (1 - EPS_BRANCH)))) > 0.01)
+    pts = (unit_vectors(2000, 12) * r[:, None])[far]
+    r = r[far]
+    pot, grad = orc.eval(pts)
+    h = 1e-3 * r
+    fd = np.empty_like(grad)
+    for c in range(3):
+        def f(s):
+            q = pts.copy()
+            q[:, c] += s * h
+            return orc.eval(q)[0]
+        fd[:, c] = (f(-2) - 8 * f(-1) + 8 * f(1) - f(2)) / (12 * h)
+    gnorm = np.linalg.norm(grad, axis=1)
+    err = np.linalg.norm(fd - grad, axis=1) / gnorm
+    noise = 1.5 * 2.2e-16 * np.abs(pot) / h / gnorm
+    inside = (r > radii[0]) & (r < radii[-1])
+    print(f"{case}: nodes Phi {worst_p:.1e}, dPhi/dr {worst_d:.1e}; FD inside median {np.median(err[inside]):.1e}")
+    assert worst_p <= 1e-14 and worst_d <= 1e-11
+    assert np.median(err[inside]) <= 1e-10 and np.all(err <= 1e-7 + 10 * noise)
+    for rb in (radii[0] * (1 + EPS_BRANCH), radii[-1] * (1 - EPS_BRANCH)):
+        pa, ga = orc.eval(d * rb * (1 - 1e-12))
+        pb, gb = orc.eval(d * rb * (1 + 1e-12))
+        assert np.max(np.abs(pb - pa) / np.abs(pa)) <= 1e-9
+        assert np.max(np.linalg.norm(gb - ga, axis=1) / np.linalg.norm(ga, axis=1)) <= 1e-5
+    if case.startswith("plummer"):
+        r = np.exp(np.random.default_rng(4).uniform(np.log(radii[0] * 1.01), np.log(radii[-1] * 0.99), 2000))
+        p, _ = orc.eval(unit_vectors(2000, 6) * r[:, None])
+        assert np.max(np.abs(p / (-1 / np.sqrt(r * r + 1)) - 1)) <= 1e-7
