@@ -22,7 +22,7 @@
 hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
     long n, const double* pos, double* phi, double* grad, hipStream_t stream);
 hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
-    int waves, const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream);
+    int waves, const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream, AgxOrbitArgs* dArgs);
 int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves, bool nfw, bool targets = false);
 hipError_t agxLaunchOrbitRKN(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision,
     const AgxOrbitArgs& A, int numBlocks, hipStream_t stream);
@@ -57,6 +57,7 @@ struct agx_potential {
     AgxMultipoleDev* ddev = nullptr;   // the same header in device memory
     void* dmem = nullptr;
     unsigned int* dcounter = nullptr;
+    AgxOrbitArgs* dargs = nullptr;     // device copy of the orbit kernel's arguments (per launch)
     int device = 0;
     double nfwMass = 0, nfwScale = 1;  // optional analytic NFW component
     // staging of the host-buffer evaluation entry points (agx_eval_many, agx_eval_hess_many):
@@ -116,7 +117,7 @@ void upload(agx_potential& p)
         oN = take(T.nodes.size() * 8), oF = take(T.nodes.size() * 4),
         oIS = take(nc * 8), oIU = take(nc * 8), oIW = take(nc * 8),
         oOS = take(nc * 8), oOU = take(nc * 8), oOW = take(nc * 8), oC = take(64),
-        oN1 = take(T.nodes1d.size() * 8), oH = take(sizeof(AgxMultipoleDev));
+        oN1 = take(T.nodes1d.size() * 8), oA = take(sizeof(AgxOrbitArgs)), oH = take(sizeof(AgxMultipoleDev));
     HIPCHECK(hipGetDevice(&p.device));
     HIPCHECK(hipMalloc(&p.dmem, off));
     char* base = static_cast<char*>(p.dmem);
@@ -161,6 +162,7 @@ void upload(agx_potential& p)
     D.ind1d.lmax = T.lmax1d; D.ind1d.mmax = T.mmax1d; D.ind1d.mmin = T.mmin1d; D.ind1d.step = T.step1d;
     for(int i = 0; i < AGX_MAX_SLOTS; i++) D.ind1d.lmin[i] = i < (int)T.lmin1d.size() ? T.lmin1d[i] : 1 << 20;
     p.dcounter = reinterpret_cast<unsigned int*>(base + oC);
+    p.dargs = reinterpret_cast<AgxOrbitArgs*>(base + oA);
     p.ddev = reinterpret_cast<AgxMultipoleDev*>(base + oH);
     put(oH, &D, sizeof(D));
 }
@@ -282,11 +284,13 @@ int orbitGeometry(long n, int precision, bool nfw, int& blocks, int& bs, bool rk
 }
 
 // the persistent orbit kernel of the method named in flags (DOP853 by default)
+// dArgs: where the DOP853 kernel's arguments are placed (default: the potential's slot)
 hipError_t launchOrbitMethod(const agx_potential* pot, int precision, int flags, const AgxOrbitArgs& A,
-    int blocks, int bs, hipStream_t st)
+    int blocks, int bs, hipStream_t st, AgxOrbitArgs* dArgs = nullptr)
 {
     if(flags & AGX_METHOD_DPRKN8) return agxLaunchOrbitRKN(pot->dev, pot->ddev, precision, A, blocks, st);
-    return agxLaunchOrbit(pot->dev, pot->ddev, precision, g_fetchOrbit, g_orbitWaves, A, blocks, bs, st);
+    return agxLaunchOrbit(pot->dev, pot->ddev, precision, g_fetchOrbit, g_orbitWaves, A, blocks, bs, st,
+        dArgs ? dArgs : pot->dargs);
 }
 
 template<typename T>
@@ -858,14 +862,18 @@ agx_orbit_graph* agx_orbit_graph_create(const agx_potential* pot, long n, const 
         std::unique_ptr<agx_orbit_graph> G(new agx_orbit_graph);
         HIPCHECK(hipSetDevice(pot->device));
         G->device = pot->device;
-        const size_t ordBytes = ((size_t)n * 4 + 255) / 256 * 256, schedBytes = agxScheduleWorkspace(n);
-        HIPCHECK(hipMalloc(&G->work, ordBytes + 256 + schedBytes));
+        // work order, queue counter, the kernel's argument block, the schedule's workspace
+        const size_t ordBytes = ((size_t)n * 4 + 255) / 256 * 256, schedBytes = agxScheduleWorkspace(n),
+            argBytes = align256(sizeof(AgxOrbitArgs));
+        HIPCHECK(hipMalloc(&G->work, ordBytes + 256 + argBytes + schedBytes));
         char* w = static_cast<char*>(G->work);
         int* ord = reinterpret_cast<int*>(w);
         unsigned int* counter = reinterpret_cast<unsigned int*>(w + ordBytes);
+        AgxOrbitArgs* dArgs = reinterpret_cast<AgxOrbitArgs*>(w + ordBytes + 256);
         const int* order = nullptr;
         if(!(flags & AGX_ORDER_AS_GIVEN)) {   // the work order, computed once from the initial conditions
-            HIPCHECK(agxLaunchSchedule(pot->dev, pot->ddev, n, d_ic, d_ttotal, ord, w + ordBytes + 256, nullptr));
+            HIPCHECK(agxLaunchSchedule(pot->dev, pot->ddev, n, d_ic, d_ttotal, ord, w + ordBytes + 256 + argBytes,
+                nullptr));
             HIPCHECK(hipDeviceSynchronize());
             order = ord;
         }
@@ -882,7 +890,7 @@ agx_orbit_graph* agx_orbit_graph_create(const agx_potential* pot, long n, const 
         std::unique_ptr<void, void(*)(void*)> sg((void*)st, [](void* q) { (void)hipStreamDestroy((hipStream_t)q); });
         HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipError_t e = agxLaunchFillU32(counter, 1, 0u, st);
-        if(e == hipSuccess) e = launchOrbitMethod(pot, precision, flags, A, blocks, bs, st);
+        if(e == hipSuccess) e = launchOrbitMethod(pot, precision, flags, A, blocks, bs, st, dArgs);
         HIPCHECK(hipStreamEndCapture(st, &G->graph));
         HIPCHECK(e);
         HIPCHECK(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
@@ -2079,7 +2087,7 @@ int agx_surrogate_orbit_batch_device(const agx_surrogate* s, long n, const doubl
             int fb, bs;
             orbitGeometry(nf, AGX_FP64, pot->nfwMass != 0, fb, bs);
             HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
-            e = agxLaunchOrbit(pot->dev, pot->ddev, AGX_FP64, g_fetchOrbit, g_orbitWaves, A, fb, bs, st);
+            e = agxLaunchOrbit(pot->dev, pot->ddev, AGX_FP64, g_fetchOrbit, g_orbitWaves, A, fb, bs, st, pot->dargs);
             if(e == hipSuccess) e = agxLaunchMarkFallback(list, nf, d_status, st);
             if(e != hipSuccess) { release(); return fail(AGX_EDEVICE, std::string("fallback launch: ") + hipGetErrorString(e)); }
         }

@@ -78,25 +78,35 @@ constexpr int OBS = 256;
 #define AGX_ALIGN_PHASES 1
 #endif
 
+// whether the orbit kernel requests the next harmonic's records before evaluating the current
+// one (more registers in flight, one L2 round trip fewer on the critical path per harmonic)
+#ifndef AGX_ORBIT_PREFETCH
+#define AGX_ORBIT_PREFETCH 1
+#endif
+
+// the ln r and tau grids of the orbit kernel, staged in LDS (at most this many values together)
+constexpr int AGX_LDS_GRID_ORBIT = 512;
+typedef const double AGX_LDS* LdsGrid;
+
 // one right-hand-side potential gradient in the inertial frame (orbit.cpp:182-208)
 template<int MODE, bool NFW>
 __device__ __forceinline__ void rhsEval(const AgxMultipoleDev& P, vd2* stage, double x, double y,
-    double z, double& pot, double& gx, double& gy, double& gz, const double* gR, const double* gT)
+    double z, double& pot, double& gx, double& gy, double& gz, LdsGrid gR, LdsGrid gT)
 {
     if constexpr(MODE == AGX_FETCH_COOP) {
         FetchCoop f{ gmem(reinterpret_cast<const vd2*>(P.nodes)),
             (vd2 AGX_LDS*)(stage + (threadIdx.x >> 6) * FetchCoop::LDS_DOUBLE2_PER_WAVE),
             (int)(threadIdx.x & 63) };
-        multipoleCar<decltype(f), const double*, NFW, true>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
+        multipoleCar<decltype(f), LdsGrid, NFW, AGX_ORBIT_PREFETCH != 0>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     } else if constexpr(MODE == AGX_FETCH_VEC2) {
         FetchVec2 f{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
-        multipoleCar<decltype(f), const double*, NFW, true>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
+        multipoleCar<decltype(f), LdsGrid, NFW, AGX_ORBIT_PREFETCH != 0>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     } else if constexpr(MODE == AGX_FETCH_SCALAR) {
         FetchScalar f{ gmem(P.nodes) };
-        multipoleCar<decltype(f), const double*, NFW, true>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
+        multipoleCar<decltype(f), LdsGrid, NFW, AGX_ORBIT_PREFETCH != 0>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     } else {
         FetchFloat f{ gmem(P.nodesF) };
-        multipoleCar<decltype(f), const double*, NFW, true>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
+        multipoleCar<decltype(f), LdsGrid, NFW, AGX_ORBIT_PREFETCH != 0>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     }
 }
 
@@ -105,20 +115,27 @@ __device__ __forceinline__ void rhsEval(const AgxMultipoleDev& P, vd2* stage, do
 // TGT: the variant that also feeds target plug-ins (agx_target.h)
 // GRP: the variant with group mode (A.groupStride): some wavefronts integrate the longest orbits
 // with AGX_GROUP lanes per orbit, each lane evaluating a share of the harmonics
+// The arguments are read through a pointer to device memory (written by storeArgsKernel just
+// before): passed by value, the whole block would be loaded into scalar registers at entry and
+// kept live for the whole kernel, which spilled scalar registers into vector-register lanes.
 template<int MODE, int WAVES, bool NFW, bool TGT = false, bool GRP = false>
 __global__ __launch_bounds__(OBS) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
-void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
+void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, const AgxOrbitArgs* __restrict__ Ap)
 {
+    const AgxOrbitArgs& A = *Ap;
     // at one wavefront per SIMD the block has the CU's LDS to itself: the stage vectors k1..k5
     // and x live there too (the cooperative fetcher needs its own LDS staging area instead)
     typedef typename std::conditional<TGT, LaneTgtT<OBS>,
         LaneT<OBS, AGX_K_IN_LDS && WAVES == 1 && MODE != AGX_FETCH_COOP> >::type Lane;
     __shared__ double lds[Lane::LDS_FIELDS * OBS];
     __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? (OBS / 64) * FetchCoop::LDS_DOUBLE2_PER_WAVE : 1];
-    __shared__ double gridLds[AGX_LDS_GRID];
+    __shared__ double gridLds[AGX_LDS_GRID_ORBIT];
     const AgxMultipoleDev& P = *Pp;
-    const double *gR, *gT;
-    stageGrids(P, gridLds, gR, gT);
+    // the grids in LDS (the launcher refuses larger ones): binary searches read LDS directly
+    for(int i = threadIdx.x; i < P.nr + P.nt; i += blockDim.x)
+        gridLds[i] = i < P.nr ? P.gridR[i] : P.gridT[i - P.nr];
+    __syncthreads();
+    const LdsGrid gR = (LdsGrid)gridLds, gT = gR + P.nr;
     Lane L;
     L.attach(lds + threadIdx.x);
     // wave-uniform: whether this wavefront works in group mode
@@ -192,7 +209,7 @@ hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, in
 int agxOrbitKernelBlockSize() { return OBS; }
 
 namespace {
-typedef void (*OrbitKernelFn)(const AgxMultipoleDev*, AgxOrbitArgs);
+typedef void (*OrbitKernelFn)(const AgxMultipoleDev*, const AgxOrbitArgs*);
 OrbitKernelFn orbitKernelFor(int precision, int fetch, int waves, bool nfw, bool targets = false, bool group = false)
 {
     if(group)     // group mode: FP64, 16-byte per-lane fetch, 1 wave/SIMD, no NFW, no targets
@@ -206,6 +223,11 @@ OrbitKernelFn orbitKernelFor(int precision, int fetch, int waves, bool nfw, bool
     if(fetch == AGX_FETCH_COOP) return two ? orbitKernel<AGX_FETCH_COOP, 2, false> : orbitKernel<AGX_FETCH_COOP, 1, false>;
     if(fetch == AGX_FETCH_VEC2) return two ? orbitKernel<AGX_FETCH_VEC2, 2, false> : orbitKernel<AGX_FETCH_VEC2, 1, false>;
     return two ? orbitKernel<AGX_FETCH_SCALAR, 2, false> : orbitKernel<AGX_FETCH_SCALAR, 1, false>;
+}
+
+__global__ void storeArgsKernel(AgxOrbitArgs A, AgxOrbitArgs* dst)
+{
+    if(threadIdx.x == 0) *dst = A;
 }
 
 __global__ void fillU32Kernel(unsigned int* p, long n, unsigned int v)
@@ -223,7 +245,7 @@ int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves, bool nfw, boo
 }
 
 hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
-    int waves, const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream)
+    int waves, const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream, AgxOrbitArgs* dArgs)
 {
     (void)P;
     if(A.n <= 0) return hipSuccess;
@@ -233,8 +255,12 @@ hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, i
     if(group && (precision != AGX_FP64 || fetch != AGX_FETCH_VEC2 || waves != 1 || P.nfwMass != 0 || A.targets ||
                  A.poolCounter || !A.groupCounter || A.groupOrbits > A.n))
         return hipErrorInvalidValue;
+    if(!dArgs || P.nr + P.nt > AGX_LDS_GRID_ORBIT) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(storeArgsKernel, dim3(1), dim3(64), 0, stream, A, dArgs);
+    hipError_t e = hipGetLastError();
+    if(e != hipSuccess) return e;
     hipLaunchKernelGGL(orbitKernelFor(precision, fetch, waves, P.nfwMass != 0, A.targets != nullptr, group),
-        dim3(numBlocks), dim3(OBS), 0, stream, dP, A);
+        dim3(numBlocks), dim3(OBS), 0, stream, dP, static_cast<const AgxOrbitArgs*>(dArgs));
     return hipGetLastError();
 }
 
