@@ -240,8 +240,10 @@ def orbit(potential=None, ic=None, time=None, timestart=None, trajsize=None, acc
     targets= (a Target or a sequence of up to 4) collects each target's data during the
     integration (RuntimeFncTarget, galaxymodel_target.h:95-155) and puts one (N, numCoefs)
     float32 array per target first in the output (interface_python.cpp:7991-7994, 8153-8162).
-    Targets are collected with DOP853 in FP64; trajectories, deviation vectors or Lyapunov
-    exponents requested with them come from a second integration of the same orbits."""
+    Targets are collected with DOP853 or DPRKN8 in FP64.  Regular trajectory samples
+    (trajsize > 0) are recorded in the same integration as the targets; every-step output
+    (trajsize=0), Orbit objects (dtype=object), deviation vectors (der=True) and Lyapunov
+    exponents requested with targets come from a second integration of the same orbits."""
     surrogate = isinstance(potential, Surrogate)
     if not surrogate and not isinstance(potential, Potential):
         raise TypeError("Argument 'potential' must be a valid Potential object")

@@ -118,6 +118,9 @@ void upload(agx_potential& p)
         oIS = take(nc * 8), oIU = take(nc * 8), oIW = take(nc * 8),
         oOS = take(nc * 8), oOU = take(nc * 8), oOW = take(nc * 8), oC = take(64),
         oN1 = take(T.nodes1d.size() * 8), oA = take(sizeof(AgxOrbitArgs)), oH = take(sizeof(AgxMultipoleDev));
+    const std::vector<double> cells = agx::buildCellPolys(T);
+    if(cells.size() * 8 >= (1ull << 31)) throw std::runtime_error("Multipole tables too large for 32-bit offsets");
+    const size_t oCell = take(cells.size() * 8);
     HIPCHECK(hipGetDevice(&p.device));
     HIPCHECK(hipMalloc(&p.dmem, off));
     char* base = static_cast<char*>(p.dmem);
@@ -132,6 +135,7 @@ void upload(agx_potential& p)
     put(oIS, T.inner.S.data(), nc * 8); put(oIU, T.inner.U.data(), nc * 8); put(oIW, T.inner.W.data(), nc * 8);
     put(oOS, T.outer.S.data(), nc * 8); put(oOU, T.outer.U.data(), nc * 8); put(oOW, T.outer.W.data(), nc * 8);
     put(oN1, T.nodes1d.data(), T.nodes1d.size() * 8);
+    put(oCell, cells.data(), cells.size() * 8);
     HIPCHECK(hipMemset(base + oC, 0, 64));
     AgxMultipoleDev& D = p.dev;
     memset(&D, 0, sizeof(D));
@@ -146,6 +150,8 @@ void upload(agx_potential& p)
     D.gridT = reinterpret_cast<const double*>(base + oT);
     D.nodes = reinterpret_cast<const double*>(base + oN);
     D.nodesF = reinterpret_cast<const float*>(base + oF);
+    D.cells = cells.empty() ? nullptr : reinterpret_cast<const double*>(base + oCell);
+    D.cellBytes = (unsigned int)(cells.size() * 8);
     auto fillPL = [&](AgxPowerLawDev& L, const agx::HostPowerLaw& H, size_t s, size_t u, size_t w) {
         L.r0sq = H.r0sq; L.Q = H.Q; L.inner = H.inner; L.lmax = H.lmax; L.mmax = H.mmax;
         L.mmin = H.mmin; L.step = H.step;
@@ -384,6 +390,16 @@ long agx_debug_tables(const char* coef_path, int which, int m, double* out)
                     v[q * n2 + i] = T.nodes[(i * T.ns + s) * AGX_NODE_STRIDE + q];
         } else if(which == 3)
             v = { (double)T.logScaling, T.invPhi0, (double)T.lmax, (double)T.mmax, (double)T.mmin, (double)T.step };
+        else if(which == 5) {   // per-cell polynomial form of harmonic m: 36 coefficients per cell
+            auto it = std::find(T.slotM.begin(), T.slotM.end(), m);
+            if(T.interp1d || it == T.slotM.end()) return 0;
+            const int s = (int)(it - T.slotM.begin());
+            const std::vector<double> cells = agx::buildCellPolys(T);
+            const size_t nc = (size_t)(T.nr - 1) * (T.nt - 1);
+            v.resize(nc * AGX_CELL_COEFS);
+            for(size_t c = 0; c < nc; c++)
+                std::copy_n(&cells[(c * T.ns + s) * AGX_CELL_COEFS], AGX_CELL_COEFS, &v[c * AGX_CELL_COEFS]);
+        }
         else if(which == 4) {   // 1-d variant: term c = m of the call: f, f_x, f_xx at every node
             if(!T.interp1d || m < 0 || m >= T.nc1d || !T.term1d[m]) return 0;
             v.resize(3 * (size_t)T.nr);
@@ -559,8 +575,9 @@ int agx_set_orbit_launch(int num_blocks, int block_size)
 
 int agx_set_fetch(int eval_mode, int orbit_mode)
 {
-    auto ok = [](int m) { return m == AGX_FETCH_SCALAR || m == AGX_FETCH_VEC2 || m == AGX_FETCH_COOP; };
-    if(!ok(eval_mode) || !ok(orbit_mode)) return fail(AGX_EINVAL, "fetch mode must be 0, 1 or 2");
+    auto ok = [](int m) { return m == AGX_FETCH_SCALAR || m == AGX_FETCH_VEC2 || m == AGX_FETCH_COOP ||
+                                 m == AGX_FETCH_CELL; };
+    if(!ok(eval_mode) || !ok(orbit_mode)) return fail(AGX_EINVAL, "fetch mode must be 0, 1, 2 or 3");
     g_fetchEval = eval_mode;
     g_fetchOrbit = orbit_mode;
     return AGX_OK;

@@ -19,6 +19,7 @@
 //     private arrays that keeps the reference's summation order.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "agx_tables.h"
 
 namespace agxd {
@@ -422,6 +423,109 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const F& 
     gphi = inGrid ? (double)trDphi : NAN;
 }
 
+// ---- per-cell polynomial evaluation (FP64 kernels) ----
+// The tensor-product quintic Hermite interpolant of one cell and harmonic is a bi-quintic
+// polynomial; its 36 coefficients in the cell's local coordinates (u, v) = (ln r - lnr_i,
+// tau - tau_j) are computed on the host from the corner records (buildCellPolys), so a lane
+// evaluates p, dp/du, dp/dv by Horner's rule: 6 rows in v (value and derivative, 9 FMAs each),
+// then 14 FMAs in u.  The same function as the two-pass Hermite form of
+// QuinticSpline2d::evalDeriv (src/math_spline.cpp:2535-2597), in about half the arithmetic and
+// without the per-point Hermite weights; all harmonics of a cell are one contiguous run read
+// with buffer loads at a single 32-bit offset (the slot and row offsets are immediates).
+struct FetchCell {
+    typedef double real;
+    static constexpr bool uniform = false;
+    __amdgpu_buffer_rsrc_t rsrc;
+    __device__ __forceinline__ FetchCell(const double* cells, unsigned int bytes)
+        : rsrc(__builtin_amdgcn_make_buffer_rsrc((void*)cells, 0, (int)bytes, 0x00020000)) {}
+    // 16 bytes at byte offset off
+    __device__ __forceinline__ vd2 ld(int off) const
+    {
+        return __builtin_bit_cast(vd2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
+    }
+};
+
+// value and (u, v) derivatives of one slot whose coefficients start at byte offset off
+__device__ __forceinline__ void cellSlot(const FetchCell& f, int off, double u, double v,
+    double& p, double& pu, double& pv)
+{
+    double P = 0, Pu = 0, Pv = 0;
+#pragma unroll
+    for(int a = 5; a >= 0; a--) {
+        const vd2 c01 = f.ld(off + a * 48), c23 = f.ld(off + a * 48 + 16), c45 = f.ld(off + a * 48 + 32);
+        // row a: A = sum_b c[a][b] v^b and B = dA/dv
+        double A = c45.y * v + c45.x, B = c45.y;
+        B = B * v + A; A = A * v + c23.y;
+        B = B * v + A; A = A * v + c23.x;
+        B = B * v + A; A = A * v + c01.y;
+        B = B * v + A; A = A * v + c01.x;
+        if(a == 5) { P = A; Pv = B; }
+        else { Pu = Pu * u + P; P = P * u + A; Pv = Pv * u + B; }
+    }
+    p = P; pu = Pu; pv = Pv;
+}
+
+// interiorEval on the per-cell polynomial tables (same control flow and Fourier synthesis)
+template<class GP>
+__device__ __forceinline__ void interiorEvalCell(const AgxMultipoleDev& P, const FetchCell& fetch,
+    double R, double z, double phi, double& pot, double& gR, double& gz, double& gphi,
+    GP gridR, GP gridT)
+{
+    const double r = sqrt(R * R + z * z), logr = log(r), rplusRinv = 1. / (r + fabs(R)),
+        tau = R == 0 ? (z > 0 ? 1. : z < 0 ? -1. : 0.) : z * rplusRinv;
+    const int xi = binSearchD(logr, gridR, P.nr), yi = binSearchD(tau, gridT, P.nt);
+    const bool inGrid = xi >= 0 && xi < P.nr - 1 && yi >= 0 && yi < P.nt - 1;
+    const int xc = inGrid ? xi : 0, yc = inGrid ? yi : 0;
+    const double u = logr - gridR[xc], v = tau - gridT[yc];
+    const int ns = P.ns, SLOT = AGX_CELL_COEFS * 8;
+    const int off0 = (xc * (P.nt - 1) + yc) * ns * SLOT;
+    const int zs = P.zeroSlot;
+    double P0, d0r, d0t;
+    cellSlot(fetch, off0 + zs * SLOT, u, v, P0, d0r, d0t);
+    if(P.logScaling) {
+        const double expX = exp(P0), val = 1. / (P.invPhi0 - expX), dPhidX = val * val * expX;
+        P0 = val;
+        d0r *= dPhidX;
+        d0t *= dPhidX;
+    }
+    double trPot = P0, trDr = d0r, trDth = d0t, trDphi = 0;
+    if(ns > 1) {
+        TrigRecT<double> pos, neg;
+        pos.init(phi);
+        neg.alpha = pos.alpha; neg.beta = pos.beta;
+        for(int s = 0; s < ns; s++) {
+            if(s == zs) continue;
+            const int m = P.slotM[s];
+            double trig, dtrig;
+            if(m > 0) {
+                pos.advanceTo(m);
+                trig = pos.c; dtrig = double(-m) * pos.s;
+            } else {
+                neg.restart();
+                neg.advanceTo(-m);
+                trig = neg.s; dtrig = double(-m) * neg.c;
+            }
+            double Pm, dmr, dmt;
+            cellSlot(fetch, off0 + s * SLOT, u, v, Pm, dmr, dmt);
+            if(P.logScaling) {
+                dmr = dmr * P0 + Pm * d0r;
+                dmt = dmt * P0 + Pm * d0t;
+                Pm *= P0;
+            }
+            trPot  += Pm * trig;
+            trDr   += dmr * trig;
+            trDth  += dmt * trig;
+            trDphi += Pm * dtrig;
+        }
+    }
+    const double rinv = 1 / r, r2inv = rinv * rinv;
+    const double drdR = R * r2inv, drdz = z * r2inv, dthdR = -tau * rinv, dthdz = rinv - rplusRinv;
+    pot = inGrid ? trPot : NAN;
+    gR = inGrid ? trDr * drdR + trDth * dthdR : NAN;
+    gz = inGrid ? trDr * drdz + trDth * dthdz : NAN;
+    gphi = inGrid ? trDphi : NAN;
+}
+
 // ---- group evaluation: G lanes (an aligned group of the wavefront) share one point ----
 // Used for the longest orbits of a batch, which would otherwise set the batch's duration on a
 // single lane: lane g of the group evaluates the harmonic slots s = g, g + G, ... and the
@@ -700,6 +804,9 @@ __device__ __forceinline__ void multipoleCyl(const AgxMultipoleDev& P, const F& 
     const double rsq = R * R + z * z;
     if(P.interp1d)
         interior1dEval(P, R, z, phi, pot, gR, gz, gphi, gridR ? gridR : GP(gmem(P.gridR)));
+    else if constexpr(std::is_same<F, FetchCell>::value)
+        interiorEvalCell(P, fetch, R, z, phi, pot, gR, gz, gphi, gridR ? gridR : GP(gmem(P.gridR)),
+            gridT ? gridT : GP(gmem(P.gridT)));
     else
         interiorEval<F, GP, PF>(P, fetch, R, z, phi, pot, gR, gz, gphi, gridR ? gridR : GP(gmem(P.gridR)),
             gridT ? gridT : GP(gmem(P.gridT)));

@@ -54,6 +54,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AGX_EVAL_WA
             (vd2 AGX_LDS*)(stage + (threadIdx.x >> 6) * FetchCoop::LDS_DOUBLE2_PER_WAVE),
             (int)(threadIdx.x & 63) };
         multipoleCar<decltype(f), const double AGX_GLOBAL*, NFW>(P, f, x, y, z, p, gx, gy, gz, gR, gT);
+    } else if constexpr(MODE == AGX_FETCH_CELL) {
+        FetchCell f(P.cells, P.cellBytes);
+        multipoleCar<decltype(f), const double AGX_GLOBAL*, NFW>(P, f, x, y, z, p, gx, gy, gz, gR, gT);
     } else if constexpr(MODE == AGX_FETCH_VEC2) {
         FetchVec2 f{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
         multipoleCar<decltype(f), const double AGX_GLOBAL*, NFW>(P, f, x, y, z, p, gx, gy, gz, gR, gT);
@@ -98,6 +101,9 @@ __device__ __forceinline__ void rhsEval(const AgxMultipoleDev& P, vd2* stage, do
             (vd2 AGX_LDS*)(stage + (threadIdx.x >> 6) * FetchCoop::LDS_DOUBLE2_PER_WAVE),
             (int)(threadIdx.x & 63) };
         multipoleCar<decltype(f), LdsGrid, NFW, AGX_ORBIT_PREFETCH != 0>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
+    } else if constexpr(MODE == AGX_FETCH_CELL) {
+        FetchCell f(P.cells, P.cellBytes);
+        multipoleCar<decltype(f), LdsGrid, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     } else if constexpr(MODE == AGX_FETCH_VEC2) {
         FetchVec2 f{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
         multipoleCar<decltype(f), LdsGrid, NFW, AGX_ORBIT_PREFETCH != 0>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
@@ -197,6 +203,8 @@ hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, in
             hipLaunchKernelGGL((evalKernel<AGX_FETCH_VEC2, true>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     } else if(precision == AGX_FP32_TABLES)
         hipLaunchKernelGGL((evalKernel<-1, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
+    else if(fetch == AGX_FETCH_CELL && !P.interp1d)
+        hipLaunchKernelGGL((evalKernel<AGX_FETCH_CELL, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     else if(fetch == AGX_FETCH_COOP)
         hipLaunchKernelGGL((evalKernel<AGX_FETCH_COOP, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     else if(fetch == AGX_FETCH_VEC2)
@@ -220,6 +228,7 @@ OrbitKernelFn orbitKernelFor(int precision, int fetch, int waves, bool nfw, bool
         return precision == AGX_FP32_TABLES ? orbitKernel<-1, 1, true> : orbitKernel<AGX_FETCH_VEC2, 1, true>;
     const bool two = waves == 2;
     if(precision == AGX_FP32_TABLES) return two ? orbitKernel<-1, 2, false> : orbitKernel<-1, 1, false>;
+    if(fetch == AGX_FETCH_CELL) return two ? orbitKernel<AGX_FETCH_CELL, 2, false> : orbitKernel<AGX_FETCH_CELL, 1, false>;
     if(fetch == AGX_FETCH_COOP) return two ? orbitKernel<AGX_FETCH_COOP, 2, false> : orbitKernel<AGX_FETCH_COOP, 1, false>;
     if(fetch == AGX_FETCH_VEC2) return two ? orbitKernel<AGX_FETCH_VEC2, 2, false> : orbitKernel<AGX_FETCH_VEC2, 1, false>;
     return two ? orbitKernel<AGX_FETCH_SCALAR, 2, false> : orbitKernel<AGX_FETCH_SCALAR, 1, false>;

@@ -7,6 +7,11 @@
 //     nodes[((i*nt + j)*ns + s)*AGX_NODE_STRIDE + q],  i < nr (ln r), j < nt (tau), s < ns,
 //     q = 0..8 : f, f_x, f_y, f_xx, f_xy, f_yy, f_xxy, f_xyy, f_xxyy   (q = 9: padding)
 //   so a point's 4 cell corners for all harmonics are two contiguous runs of 2*ns*80 bytes.
+// Per-cell polynomial form of the same splines (FP64 kernels): for cell (i, j) of the grid and
+//   slot s, the 36 coefficients of the bi-quintic p(u, v) = sum_{a,b<6} c[a][b] u^a v^b,
+//   u = ln r - lnr_i, v = tau - tau_j, that the tensor-product quintic Hermite interpolation of
+//   the cell's corner records is:  cells[((i*(nt-1) + j)*ns + s)*AGX_CELL_COEFS + 6a + b];
+//   a point's whole evaluation reads one contiguous run of ns*288 bytes.
 // Interior for lmax <= 2 (MultipoleInterp1d, src/potential_multipole.cpp:1691-1815): one 1-D
 //   quintic spline in ln r per (l,m) term, node-major:
 //     nodes1d[(i*nc1d + c)*4 + q],  i < nr, c = l*(l+1)+m < nc1d = (lmax+1)^2,
@@ -16,6 +21,7 @@
 #pragma once
 
 #define AGX_NODE_STRIDE 10
+#define AGX_CELL_COEFS 36
 #define AGX_MAX_M 16
 #define AGX_MAX_SLOTS (2 * AGX_MAX_M + 1)
 
@@ -50,6 +56,8 @@ struct AgxMultipoleDev {
     const double* gridT;        // tau nodes [nt]
     const double* nodes;        // node-major spline data (see above)
     const float* nodesF;        // the same data rounded to float (precision-split form), or null
+    const double* cells;        // per-cell polynomial form (see above), or null (1-D variant)
+    unsigned int cellBytes;     // its size in bytes
     AgxPowerLawDev inner, outer;
     // lmax <= 2: the 1-D variant replaces the 2-D tables (nt = ns = 0); logScaling and invPhi0
     // then refer to the l=0 coefficient Phi_00 (MultipoleInterp1d's convention)

@@ -1041,6 +1041,62 @@ Coefs truncateCoefs(const Coefs& C, int lmax)
     return t;
 }
 
+// 1-D quintic Hermite interpolation on [0, h] as a polynomial in u: a[0..5] from
+// d = (f(0), f(h), f'(0), f'(h), f''(0), f''(h)).  Expanding the weights of the Hermite form
+// (P = 10t^3 - 15t^4 + 6t^5, Q/h = t^3 - t^4/2, R/h^2 = t^3 - 1.25t^4 + t^5/2 with t = u/h, applied
+// to fd = f(h) - f(0) - h/2 (f'(0) + f'(h)), f1d = f'(h) - f'(0) - h/2 (f''(0) + f''(h)) and
+// f2d = f''(h) - f''(0), besides f(0) + f'(0) u + f''(0) u^2/2) gives the coefficients below.
+// The differences are formed first, as in the Hermite form: values of one magnitude that vary
+// little across a cell cancel exactly instead of after scaling by 1/h^3.
+static void hermiteToPoly(long double h, const long double d[6], long double a[6])
+{
+    const long double fd = (d[1] - d[0]) - h / 2 * (d[2] + d[3]),
+        f1d = (d[3] - d[2]) - h / 2 * (d[4] + d[5]), f2d = d[5] - d[4];
+    const long double h2 = h * h, h3 = h2 * h;
+    a[0] = d[0];
+    a[1] = d[2];
+    a[2] = d[4] / 2;
+    a[3] = (10 * fd + h * f1d + h2 * f2d) / h3;
+    a[4] = (-15 * fd - 0.5L * h * f1d - 1.25L * h2 * f2d) / (h3 * h);
+    a[5] = (6 * fd + 0.5L * h2 * f2d) / (h3 * h2);
+}
+
+std::vector<double> buildCellPolys(const HostTables& T)
+{
+    if(T.interp1d || T.nr < 2 || T.nt < 2) return {};
+    const int nr = T.nr, nt = T.nt, ns = T.ns;
+    std::vector<double> cells((size_t)(nr - 1) * (nt - 1) * ns * AGX_CELL_COEFS);
+    // record index q of the derivative d^px/dx^px d^py/dy^py (agx_tables.h)
+    static const int Q[3][3] = { { 0, 2, 5 }, { 1, 4, 7 }, { 3, 6, 8 } };
+    for(int i = 0; i < nr - 1; i++) {
+        const long double hx = (long double)T.gridR[i + 1] - T.gridR[i];
+        for(int j = 0; j < nt - 1; j++) {
+            const long double hy = (long double)T.gridT[j + 1] - T.gridT[j];
+            for(int s = 0; s < ns; s++) {
+                // the tau pass for each x-datum alpha = 2 px + ex (f, f_x, f_xx at x_i, x_i+1),
+                // as QuinticSpline2d's first pass; then the ln r pass on each tau coefficient
+                long double Y[6][6], d[6], a[6];
+                for(int al = 0; al < 6; al++) {
+                    const int px = al / 2, ex = al % 2;
+                    for(int be = 0; be < 6; be++) {
+                        const int py = be / 2, ey = be % 2;
+                        const size_t node = (size_t)(i + ex) * nt + (j + ey);
+                        d[be] = T.nodes[(node * ns + s) * AGX_NODE_STRIDE + Q[px][py]];
+                    }
+                    hermiteToPoly(hy, d, Y[al]);
+                }
+                double* dst = &cells[(((size_t)i * (nt - 1) + j) * ns + s) * AGX_CELL_COEFS];
+                for(int b = 0; b < 6; b++) {
+                    for(int al = 0; al < 6; al++) d[al] = Y[al][b];
+                    hermiteToPoly(hx, d, a);
+                    for(int k = 0; k < 6; k++) dst[k * 6 + b] = (double)a[k];
+                }
+            }
+        }
+    }
+    return cells;
+}
+
 // MultipoleInterp1d ctor (potential_multipole.cpp:1691-1738): the extrapolation's 1/Phi_00(0)
 // from the two innermost nodes, log-scaling when every Phi_00(r_k) < 0, and a quintic spline in
 // ln r (values and ln r derivatives at the nodes, second derivatives by constructQuinticSpline,

@@ -110,6 +110,10 @@ struct HostTables {
 HostTables buildTables(const Coefs& c, bool allowLowOrder = false);
 // the expansion with every term of degree l > lmax set to zero
 Coefs truncateCoefs(const Coefs& c, int lmax);
+// Per-cell polynomial form of the 2-D tables (agx_tables.h, AGX_CELL_COEFS): for every cell of
+// the (ln r, tau) grid and harmonic slot, the 36 coefficients of the bi-quintic that the
+// tensor-product quintic Hermite interpolation of the cell's four corner records equals
+std::vector<double> buildCellPolys(const HostTables& T);
 
 // the reference's sin/cos (math::sincos, src/math_core.cpp:171-206: multiples of pi/2 exact)
 void sincosAgama(double x, double& s, double& c);

@@ -52,6 +52,8 @@ EVALS_PER_ATTEMPT = {"dop853": 12, "dprkn8": 9}
 FLOPS_PER_ATTEMPT_EXTRA_BY_METHOD = {"dop853": FLOPS_PER_ATTEMPT_EXTRA, "dprkn8": 620}
 METHOD_FLAG = {"dop853": 0, "dprkn8": 0x200}
 FLOPS_PER_ORBIT_INIT = 3 * FLOPS_PER_EVAL + 80
+# record fetch modes of the Multipole evaluation (agx_set_fetch)
+FETCH_MODES = {"scalar": 0, "vec2": 1, "coop": 2, "cell": 3}
 # launches timed for a leg that is reported beside the headline figure (not the headline)
 COMPARISON_STEPS = 2
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix), AMD published figure
@@ -88,7 +90,7 @@ def parse():
                     help="group mode of the FP64 DOP853 kernel, STRIDE[,ORBITS] (STRIDE 0: off): one "
                          "wavefront in STRIDE integrates the first ORBITS orbits of the order with 8 lanes "
                          "per orbit; default: the library's automatic choice (off at 1M orbits)")
-    ap.add_argument("--fetch", choices=["scalar", "vec2", "coop"], default=None,
+    ap.add_argument("--fetch", choices=["scalar", "vec2", "coop", "cell"], default=None,
                     help="record gather of the FP64 orbit kernel (default: the library's, vec2)")
     ap.add_argument("--train-steps", type=int, default=6000)
     ap.add_argument("--train-samples", type=float, default=20e6)
@@ -125,6 +127,8 @@ def bench_eval(args, hess=False):
         raise SystemExit("bench.py: the Hessian path is FP64 only")
     stream = torch.cuda.Stream()
     reps = max(args.steps, 1) * 20
+    if args.fetch:
+        assert lib.agx_set_fetch(FETCH_MODES[args.fetch], 1) == 0
 
     def launch():
         if hess:
@@ -212,7 +216,7 @@ def main():
     pot = agama_amd.Potential(file=coef)
     n = args.orbits
     if args.fetch:
-        assert lib.agx_set_fetch(1, {"scalar": 0, "vec2": 1, "coop": 2}[args.fetch]) == 0
+        assert lib.agx_set_fetch(1, FETCH_MODES[args.fetch]) == 0
     if args.groups:
         g = [int(v) for v in args.groups.split(",")]
         assert lib.agx_set_orbit_groups(g[0], g[1] if len(g) > 1 else -1, 0) == 0

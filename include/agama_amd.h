@@ -91,8 +91,10 @@ int agx_set_construction(int mode);
 /* Host-only diagnostic: the interpolation tables built from a coefficient file, in the
    reference's per-harmonic layout.  which: 0 = ln r grid, 1 = tau grid, 2 = the 9 node arrays
    (f, f_x, f_y, f_xx, f_xy, f_yy, f_xxy, f_xyy, f_xxyy, each nr*nt row-major) of harmonic m,
-   3 = {logScaling, invPhi0, lmax, mmax, mmin, step}.  Returns the number of doubles written
-   (or needed when out is NULL), or -1. */
+   3 = {logScaling, invPhi0, lmax, mmax, mmin, step}, 4 = the 1-d variant's f, f_x, f_xx of
+   term c = m, 5 = the per-cell polynomial form of harmonic m used by the FP64 kernels (36
+   coefficients c[a][b] of u^a v^b per cell, cells (i, j) row-major over (nr-1) x (nt-1)).
+   Returns the number of doubles written (or needed when out is NULL), or -1. */
 long agx_debug_tables(const char* coef_path, int which, int m, double* out);
 
 /* Write the coefficients in the reference's text format

@@ -1,6 +1,7 @@
 """GPU parity of the batched Multipole evaluation against the CPU oracle.
 Tolerance: 1e-12 relative for potential and gradient (FP64 kernel); the FP32-table form is
 checked against its own stated error bound."""
+import ctypes
 import os
 
 import numpy as np
@@ -161,3 +162,36 @@ def test_host_entry_points_share_a_handle_across_threads(pots):
         th.join()
     assert not errors, errors
     assert torch.cuda.current_device() == 0
+
+
+@pytest.mark.parametrize("fetch", [0, 1, 2, 3])
+@pytest.mark.parametrize("name", ["dehnen_triax_lmax8.coef", "disk_bulge_halo_lmax12.coef",
+                                  "ferrers_bar_lmax16.coef"])
+def test_eval_every_fetch_mode(name, fetch):
+    """every evaluation form of the FP64 kernel (8- and 16-byte record gathers, the cooperative
+    gather, the per-cell polynomial tables) against the oracle at 200k density points and at
+    the grid nodes, cell edges and both extrapolation branches; bar 1e-12 relative"""
+    import agama_amd
+    from agama_amd import lib, workloads
+    from oracle.pyoracle import OraclePotential
+    path = os.path.join(GOLDEN, name)
+    gpu, orc = agama_amd.Potential(file=path), OraclePotential(path)
+    pts = workloads.density_points(200_000, seed=21) * (3.0 if "ferrers" in name else 1.0)
+    rg, tg = np.exp(orc.tables(0)), orc.tables(1)
+    # points on the (ln r, tau) node lattice: R = r (1 - tau^2) / (1 + tau^2), z = 2 r tau / (1 + tau^2)
+    rr, tt = np.meshgrid(rg[::3], tg[::2], indexing="ij")
+    R, z = rr * (1 - tt ** 2) / (1 + tt ** 2), 2 * rr * tt / (1 + tt ** 2)
+    phi0 = 0.7
+    lat = np.stack([R.ravel() * np.cos(phi0), R.ravel() * np.sin(phi0), z.ravel()], axis=1)
+    extra = np.array([[0, 0, 1.0], [1e-4, 2e-4, -3e-4], [2e3, 10, 5], [0.9999e-3, 0, 0], [1.0001e3, 0, 0]])
+    pts = np.concatenate([pts, lat, extra])
+    old = (ctypes.c_int(), ctypes.c_int())
+    lib.agx_get_orbit_config(None, ctypes.byref(old[0]), ctypes.byref(old[1]), None)
+    try:
+        assert lib.agx_set_fetch(fetch, old[1].value) == 0
+        ephi, egrad, phi, _ = rel_errors(gpu, orc, pts)
+    finally:
+        lib.agx_set_fetch(old[0].value, old[1].value)
+    print(f"{name} fetch {fetch}: max rel err phi {ephi.max():.3e} grad {egrad.max():.3e}")
+    assert np.all(np.isfinite(phi))
+    assert ephi.max() <= 1e-12 and egrad.max() <= 1e-12

@@ -81,6 +81,30 @@ def test_orbits_100_periods_match_oracle(setup):
     assert np.max(info["maxde"]) < 1e-5
 
 
+@pytest.mark.parametrize("fetch", [1, 3])
+def test_orbit_fetch_modes_match_oracle(setup, fetch):
+    """the orbit kernel with the 16-byte record gather (1) and with the per-cell polynomial
+    tables (3): the same parity bars as above on 128 orbits x 100 periods"""
+    import ctypes
+    import agama_amd
+    from agama_amd import lib
+    gpu, orc = setup
+    ic, T = ics(setup, 128, seed=24)
+    print(f"case: 128 orbits x 100 periods, orbit fetch mode {fetch}", flush=True)
+    old = (ctypes.c_int(), ctypes.c_int())
+    lib.agx_get_orbit_config(None, ctypes.byref(old[0]), ctypes.byref(old[1]), None)
+    info = {}
+    try:
+        assert lib.agx_set_fetch(old[0].value, fetch) == 0
+        agama_amd.orbit(potential=gpu, ic=ic, time=T, accuracy=1e-8, info=info)
+    finally:
+        lib.agx_set_fetch(old[0].value, old[1].value)
+    lyap = np.asarray(agama_amd.orbit(potential=gpu, ic=ic, time=T, accuracy=1e-8, lyapunov=True))
+    assert np.all(info["status"] == 0)
+    ref = regular_orbit_parity(info["final"], ic, T, orc, lyap[:, 0] == 0, f"fetch {fetch}")
+    assert np.median(info["maxde"]) <= 1.5 * np.median(ref["maxde"]) + 1e-13
+
+
 def test_trajectory_sampling_matches_oracle(setup):
     import agama_amd
     gpu, orc = setup

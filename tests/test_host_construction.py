@@ -169,3 +169,46 @@ def test_interp1d_tables_bitwise_equal_to_oracle(case, tmp_path):
         terms += a.size > 0
     assert terms == {"plummer_lmax4.coef": 1, "dehnen_triax_lmax2.coef": 3, "general_lmax2": 9}[case]
     assert lib.agx_debug_tables(path.encode(), 2, 0, None) == 0     # no 2-d tables
+
+
+@pytest.mark.parametrize("name", FIXTURES + ["ferrers_bar_lmax16.coef"])
+def test_cell_polynomials_interpolate_the_node_records(name):
+    """The per-cell bi-quintic form used by the FP64 kernels is the tensor-product quintic
+    Hermite interpolant of QuinticSpline2d (src/math_spline.cpp:2535-2597): at each of a cell's
+    four corners its derivatives d^a/du^a d^b/dv^b (a, b <= 2) equal the corner's node record.
+    These 36 conditions determine a bi-quintic uniquely, so they pin the conversion.  Bar: the
+    rounding bound of the check itself, 64 eps (sum of the absolute terms + |record|); a wrong
+    coefficient misses it by orders of magnitude."""
+    path = os.path.join(GOLDEN, name)
+    gR, gT = product_tables(path, 0), product_tables(path, 1)
+    nr, nt = gR.size, gT.size
+    hx, hy = np.diff(gR), np.diff(gT)
+    mmax = int(product_tables(path, 3)[3])
+    q = {(0, 0): 0, (1, 0): 1, (0, 1): 2, (2, 0): 3, (1, 1): 4, (0, 2): 5, (2, 1): 6, (1, 2): 7, (2, 2): 8}
+    eps = np.finfo(float).eps
+    tested = 0
+    for m in range(-mmax, mmax + 1):
+        nodes = product_tables(path, 2, m)
+        if nodes.size == 0:
+            continue
+        rec = nodes.reshape(9, nr, nt)
+        c = product_tables(path, 5, m).reshape(nr - 1, nt - 1, 6, 6)
+        for ex in (0, 1):
+            for ey in (0, 1):
+                U = (hx * ex)[:, None]
+                V = (hy * ey)[None, :]
+                for (da, db), qi in q.items():
+                    # d^da/du^da d^db/dv^db of sum c[a][b] u^a v^b at (U, V)
+                    val = np.zeros((nr - 1, nt - 1))
+                    mag = np.zeros((nr - 1, nt - 1))
+                    for a in range(da, 6):
+                        for b in range(db, 6):
+                            k = np.prod(np.arange(a - da + 1, a + 1)) * np.prod(np.arange(b - db + 1, b + 1))
+                            term = k * c[:, :, a, b] * U ** (a - da) * V ** (b - db)
+                            val += term
+                            mag += np.abs(term)
+                    ref = rec[qi, ex:nr - 1 + ex, ey:nt - 1 + ey]
+                    bound = 64 * eps * (mag + np.abs(ref))
+                    assert np.all(np.abs(val - ref) <= bound), (m, ex, ey, da, db, np.max(np.abs(val - ref) / bound), np.unravel_index(np.argmax(np.abs(val - ref) / bound), val.shape))
+                    tested += 1
+    assert tested >= 36
