@@ -121,6 +121,8 @@ void upload(agx_potential& p)
     const std::vector<double> cells = agx::buildCellPolys(T);
     if(cells.size() * 8 >= (1ull << 31)) throw std::runtime_error("Multipole tables too large for 32-bit offsets");
     const size_t oCell = take(cells.size() * 8);
+    const std::vector<int> tauLut = agx::buildTauLookup(T);
+    const size_t oLut = take(tauLut.size() * 4);
     HIPCHECK(hipGetDevice(&p.device));
     HIPCHECK(hipMalloc(&p.dmem, off));
     char* base = static_cast<char*>(p.dmem);
@@ -136,6 +138,7 @@ void upload(agx_potential& p)
     put(oOS, T.outer.S.data(), nc * 8); put(oOU, T.outer.U.data(), nc * 8); put(oOW, T.outer.W.data(), nc * 8);
     put(oN1, T.nodes1d.data(), T.nodes1d.size() * 8);
     put(oCell, cells.data(), cells.size() * 8);
+    put(oLut, tauLut.data(), tauLut.size() * 4);
     HIPCHECK(hipMemset(base + oC, 0, 64));
     AgxMultipoleDev& D = p.dev;
     memset(&D, 0, sizeof(D));
@@ -152,6 +155,9 @@ void upload(agx_potential& p)
     D.nodesF = reinterpret_cast<const float*>(base + oF);
     D.cells = cells.empty() ? nullptr : reinterpret_cast<const double*>(base + oCell);
     D.cellBytes = (unsigned int)(cells.size() * 8);
+    D.lnrScale = T.nr > 1 ? (T.nr - 1) / (T.gridR.back() - T.gridR.front()) : 0;
+    D.nTauLut = (int)tauLut.size();
+    D.tauLut = tauLut.empty() ? nullptr : reinterpret_cast<const int*>(base + oLut);
     auto fillPL = [&](AgxPowerLawDev& L, const agx::HostPowerLaw& H, size_t s, size_t u, size_t w) {
         L.r0sq = H.r0sq; L.Q = H.Q; L.inner = H.inner; L.lmax = H.lmax; L.mmax = H.mmax;
         L.mmin = H.mmin; L.step = H.step;
@@ -576,8 +582,8 @@ int agx_set_orbit_launch(int num_blocks, int block_size)
 int agx_set_fetch(int eval_mode, int orbit_mode)
 {
     auto ok = [](int m) { return m == AGX_FETCH_SCALAR || m == AGX_FETCH_VEC2 || m == AGX_FETCH_COOP ||
-                                 m == AGX_FETCH_CELL; };
-    if(!ok(eval_mode) || !ok(orbit_mode)) return fail(AGX_EINVAL, "fetch mode must be 0, 1, 2 or 3");
+                                 m == AGX_FETCH_CELL || m == AGX_FETCH_CELL_COOP; };
+    if(!ok(eval_mode) || !ok(orbit_mode)) return fail(AGX_EINVAL, "fetch mode must be 0 to 4");
     g_fetchEval = eval_mode;
     g_fetchOrbit = orbit_mode;
     return AGX_OK;

@@ -431,28 +431,18 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const F& 
 // then 14 FMAs in u.  The same function as the two-pass Hermite form of
 // QuinticSpline2d::evalDeriv (src/math_spline.cpp:2535-2597), in about half the arithmetic and
 // without the per-point Hermite weights; all harmonics of a cell are one contiguous run read
-// with buffer loads at a single 32-bit offset (the slot and row offsets are immediates).
-struct FetchCell {
-    typedef double real;
-    static constexpr bool uniform = false;
-    __amdgpu_buffer_rsrc_t rsrc;
-    __device__ __forceinline__ FetchCell(const double* cells, unsigned int bytes)
-        : rsrc(__builtin_amdgcn_make_buffer_rsrc((void*)cells, 0, (int)bytes, 0x00020000)) {}
-    // 16 bytes at byte offset off
-    __device__ __forceinline__ vd2 ld(int off) const
-    {
-        return __builtin_bit_cast(vd2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
-    }
-};
+// with buffer loads at a single 32-bit offset.
+constexpr int AGX_CELL_SLOT_BYTES = AGX_CELL_COEFS * 8;
+constexpr int AGX_CELL_CHUNKS = AGX_CELL_SLOT_BYTES / 16;   // 16-byte loads per slot
 
-// value and (u, v) derivatives of one slot whose coefficients start at byte offset off
-__device__ __forceinline__ void cellSlot(const FetchCell& f, int off, double u, double v,
+// Horner evaluation of one slot from its 18 coefficient pairs (row a: c[3a .. 3a+2])
+__device__ __forceinline__ void cellHorner(const vd2 (&c)[AGX_CELL_CHUNKS], double u, double v,
     double& p, double& pu, double& pv)
 {
     double P = 0, Pu = 0, Pv = 0;
 #pragma unroll
     for(int a = 5; a >= 0; a--) {
-        const vd2 c01 = f.ld(off + a * 48), c23 = f.ld(off + a * 48 + 16), c45 = f.ld(off + a * 48 + 32);
+        const vd2 c01 = c[3 * a], c23 = c[3 * a + 1], c45 = c[3 * a + 2];
         // row a: A = sum_b c[a][b] v^b and B = dA/dv
         double A = c45.y * v + c45.x, B = c45.y;
         B = B * v + A; A = A * v + c23.y;
@@ -465,9 +455,90 @@ __device__ __forceinline__ void cellSlot(const FetchCell& f, int off, double u, 
     p = P; pu = Pu; pv = Pv;
 }
 
-// interiorEval on the per-cell polynomial tables (same control flow and Fourier synthesis)
-template<class GP>
-__device__ __forceinline__ void interiorEvalCell(const AgxMultipoleDev& P, const FetchCell& fetch,
+// every lane reads its own cell
+struct FetchCell {
+    typedef double real;
+    static constexpr bool uniform = false;
+    __amdgpu_buffer_rsrc_t rsrc;
+    __device__ __forceinline__ FetchCell(const double* cells, unsigned int bytes)
+        : rsrc(__builtin_amdgcn_make_buffer_rsrc((void*)cells, 0, (int)bytes, 0x00020000)) {}
+    // 16 bytes at byte offset off
+    __device__ __forceinline__ vd2 ld(int off) const
+    {
+        return __builtin_bit_cast(vd2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
+    }
+    // the coefficients of slot s of the cell at byte offset cellOff
+    __device__ __forceinline__ void load(int cellOff, int s, vd2 (&c)[AGX_CELL_CHUNKS]) const
+    {
+        const int off = cellOff + s * AGX_CELL_SLOT_BYTES;
+#pragma unroll
+        for(int t = 0; t < AGX_CELL_CHUNKS; t++) c[t] = ld(off + 16 * t);
+    }
+    // value and (u, v) derivatives of slot s of the cell at byte offset cellOff
+    __device__ __forceinline__ void slot(int cellOff, int s, double u, double v, double& p, double& pu,
+        double& pv) const
+    {
+        vd2 c[AGX_CELL_CHUNKS];
+        load(cellOff, s, c);
+        cellHorner(c, u, v, p, pu, pv);
+    }
+};
+
+// Cooperative gather of one slot for the whole wavefront: the 64 lanes' 288-byte slot blocks
+// (64 x 18 chunks) are read by 18 load instructions in which lane l fetches chunk k = l + 64t of
+// the wavefront's list (owner lane k / 18, chunk k % 18, the owner's cell offset obtained with
+// ds_bpermute), so that one instruction touches about four blocks' cache lines instead of 64;
+// the chunks are passed to their owners through LDS (written lane-consecutive, read with a
+// 288-byte lane stride: conflict-free in groups of 8 lanes).  All 64 lanes must call slot()
+// together (wave-uniform control flow).
+struct FetchCellCoop {
+    typedef double real;
+    static constexpr bool uniform = true;
+    static constexpr int LDS_VD2_PER_WAVE = 64 * AGX_CELL_CHUNKS;
+    __amdgpu_buffer_rsrc_t rsrc;
+    vd2 AGX_LDS* lds;              // this wavefront's staging area
+    int lane;
+    int perm[AGX_CELL_CHUNKS];     // 4 * owner lane of chunk l + 64t
+    int coff[AGX_CELL_CHUNKS];     // byte offset of that chunk in its slot block
+    __device__ __forceinline__ FetchCellCoop(const double* cells, unsigned int bytes, vd2 AGX_LDS* stage, int l)
+        : rsrc(__builtin_amdgcn_make_buffer_rsrc((void*)cells, 0, (int)bytes, 0x00020000)), lds(stage), lane(l)
+    {
+#pragma unroll
+        for(int t = 0; t < AGX_CELL_CHUNKS; t++) {
+            const int k = l + 64 * t, r = k / AGX_CELL_CHUNKS;
+            perm[t] = r * 4;
+            coff[t] = (k - r * AGX_CELL_CHUNKS) * 16;
+        }
+    }
+    __device__ __forceinline__ void slot(int cellOff, int s, double u, double v, double& p, double& pu,
+        double& pv) const
+    {
+        vd2 t4[AGX_CELL_CHUNKS], c[AGX_CELL_CHUNKS];
+        const int soff = s * AGX_CELL_SLOT_BYTES;
+#pragma unroll
+        for(int t = 0; t < AGX_CELL_CHUNKS; t++) {
+            const int owner = __builtin_amdgcn_ds_bpermute(perm[t], cellOff);
+            t4[t] = __builtin_bit_cast(vd2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, owner + coff[t], soff, 0));
+        }
+#pragma unroll
+        for(int t = 0; t < AGX_CELL_CHUNKS; t++) lds[lane + 64 * t] = t4[t];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for(int t = 0; t < AGX_CELL_CHUNKS; t++) c[t] = lds[lane * AGX_CELL_CHUNKS + t];
+        __builtin_amdgcn_wave_barrier();
+        cellHorner(c, u, v, p, pu, pv);
+    }
+};
+
+template<class F> struct IsCellFetch : std::false_type {};
+template<> struct IsCellFetch<FetchCell> : std::true_type {};
+template<> struct IsCellFetch<FetchCellCoop> : std::true_type {};
+
+// interiorEval on the per-cell polynomial tables (same control flow and Fourier synthesis).
+// With the per-lane fetcher the next harmonic's coefficients are requested before the current
+// one is evaluated (two register buffers), so the memory round trips of the harmonics overlap.
+template<class F, class GP>
+__device__ __forceinline__ void interiorEvalCell(const AgxMultipoleDev& P, const F& fetch,
     double R, double z, double phi, double& pot, double& gR, double& gz, double& gphi,
     GP gridR, GP gridT)
 {
@@ -477,11 +548,19 @@ __device__ __forceinline__ void interiorEvalCell(const AgxMultipoleDev& P, const
     const bool inGrid = xi >= 0 && xi < P.nr - 1 && yi >= 0 && yi < P.nt - 1;
     const int xc = inGrid ? xi : 0, yc = inGrid ? yi : 0;
     const double u = logr - gridR[xc], v = tau - gridT[yc];
-    const int ns = P.ns, SLOT = AGX_CELL_COEFS * 8;
-    const int off0 = (xc * (P.nt - 1) + yc) * ns * SLOT;
+    const int ns = P.ns;
+    const int off0 = (xc * (P.nt - 1) + yc) * ns * AGX_CELL_SLOT_BYTES;
     const int zs = P.zeroSlot;
+    constexpr bool PF = std::is_same<F, FetchCell>::value;
+    vd2 ca[AGX_CELL_CHUNKS], cb[AGX_CELL_CHUNKS];
+    const int s0 = zs == 0 ? 1 : 0;   // the first slot after the m = 0 one
     double P0, d0r, d0t;
-    cellSlot(fetch, off0 + zs * SLOT, u, v, P0, d0r, d0t);
+    if constexpr(PF) {
+        fetch.load(off0, zs, ca);
+        if(s0 < ns) fetch.load(off0, s0, cb);
+        cellHorner(ca, u, v, P0, d0r, d0t);
+    } else
+        fetch.slot(off0, zs, u, v, P0, d0r, d0t);
     if(P.logScaling) {
         const double expX = exp(P0), val = 1. / (P.invPhi0 - expX), dPhidX = val * val * expX;
         P0 = val;
@@ -493,8 +572,8 @@ __device__ __forceinline__ void interiorEvalCell(const AgxMultipoleDev& P, const
         TrigRecT<double> pos, neg;
         pos.init(phi);
         neg.alpha = pos.alpha; neg.beta = pos.beta;
-        for(int s = 0; s < ns; s++) {
-            if(s == zs) continue;
+        // harmonic slot s (not the m = 0 slot) from its value and derivatives
+        auto addSlot = [&](int s, double Pm, double dmr, double dmt) {
             const int m = P.slotM[s];
             double trig, dtrig;
             if(m > 0) {
@@ -505,8 +584,6 @@ __device__ __forceinline__ void interiorEvalCell(const AgxMultipoleDev& P, const
                 neg.advanceTo(-m);
                 trig = neg.s; dtrig = double(-m) * neg.c;
             }
-            double Pm, dmr, dmt;
-            cellSlot(fetch, off0 + s * SLOT, u, v, Pm, dmr, dmt);
             if(P.logScaling) {
                 dmr = dmr * P0 + Pm * d0r;
                 dmt = dmt * P0 + Pm * d0t;
@@ -516,6 +593,31 @@ __device__ __forceinline__ void interiorEvalCell(const AgxMultipoleDev& P, const
             trDr   += dmr * trig;
             trDth  += dmt * trig;
             trDphi += Pm * dtrig;
+        };
+        if constexpr(PF) {
+            // cb holds slot s on entry; the two buffers alternate
+            for(int s = s0; s < ns; ) {
+                int s2 = s + 1;
+                if(s2 == zs) s2++;
+                if(s2 < ns) fetch.load(off0, s2, ca);
+                double Pm, dmr, dmt;
+                cellHorner(cb, u, v, Pm, dmr, dmt);
+                addSlot(s, Pm, dmr, dmt);
+                if(s2 >= ns) break;
+                int s3 = s2 + 1;
+                if(s3 == zs) s3++;
+                if(s3 < ns) fetch.load(off0, s3, cb);
+                cellHorner(ca, u, v, Pm, dmr, dmt);
+                addSlot(s2, Pm, dmr, dmt);
+                s = s3;
+            }
+        } else {
+            for(int s = 0; s < ns; s++) {
+                if(s == zs) continue;
+                double Pm, dmr, dmt;
+                fetch.slot(off0, s, u, v, Pm, dmr, dmt);
+                addSlot(s, Pm, dmr, dmt);
+            }
         }
     }
     const double rinv = 1 / r, r2inv = rinv * rinv;
@@ -804,7 +906,7 @@ __device__ __forceinline__ void multipoleCyl(const AgxMultipoleDev& P, const F& 
     const double rsq = R * R + z * z;
     if(P.interp1d)
         interior1dEval(P, R, z, phi, pot, gR, gz, gphi, gridR ? gridR : GP(gmem(P.gridR)));
-    else if constexpr(std::is_same<F, FetchCell>::value)
+    else if constexpr(IsCellFetch<F>::value)
         interiorEvalCell(P, fetch, R, z, phi, pot, gR, gz, gphi, gridR ? gridR : GP(gmem(P.gridR)),
             gridT ? gridT : GP(gmem(P.gridT)));
     else
@@ -843,6 +945,174 @@ __device__ __forceinline__ void nfwDeriv(double mass, double a, double r, double
         (0.5 + 17. / 96 * q) / (1 + q * (27. / 16 + q * (0.75 + 11. / 160 * q))) / (a * a));
 }
 
+// ---- the FP64 kernels' evaluation on the per-cell tables ----
+// reciprocal within an ulp of 1/x (x finite, non-zero and normal): the hardware estimate refined
+// by two Newton steps (fewer instructions than the correctly rounded division)
+__device__ __forceinline__ double rcpD(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// cell of the (near-uniform) ln r grid holding x: i with arr[i] <= x < arr[i+1] (n-2 at the last
+// node), -1 outside [arr[0], arr[n-1]]; the same result as binSearchD, from a first guess
+// (x - arr[0]) * scale corrected by one node, or the full search when the grid is not uniform
+// enough for that
+template<class GP>
+__device__ __forceinline__ int cellIndexR(double x, GP arr, int n, double scale)
+{
+    const double a0 = arr[0];
+    if(!(x >= a0) || x > arr[n - 1]) return -1;
+    int i = (int)((x - a0) * scale);
+    i = i < 0 ? 0 : i > n - 2 ? n - 2 : i;
+    if(x < arr[i]) i--;
+    else if(i < n - 2 && x >= arr[i + 1]) i++;
+    if(i < 0 || x < arr[i] || (i < n - 2 && x >= arr[i + 1])) i = binSearchD(x, arr, n);
+    return i;
+}
+
+// cell of the tau grid holding t in [-1, 1], from the bucket table (each bucket narrower than a
+// cell, so at most one node lies inside it)
+template<class GP>
+__device__ __forceinline__ int cellIndexT(double t, GP arr, int n, const int AGX_GLOBAL* lut, int nlut)
+{
+    if(!(t >= arr[0]) || t > arr[n - 1]) return -1;
+    int b = (int)((t + 1) * (0.5 * nlut));
+    b = b < 0 ? 0 : b > nlut - 1 ? nlut - 1 : b;
+    int i = lut[b];
+    while(i < n - 2 && t >= arr[i + 1]) i++;
+    return i;
+}
+
+// multipoleCar on the per-cell tables (fetch: FetchCell or FetchCellCoop).  The same function as
+// Multipole::evalCyl behind BasePotentialCyl::evalCar (src/potential_multipole.cpp:1538-1548,
+// 1969-2080), arranged for fewer instructions: cos and sin of the azimuth come from x/R and
+// y/R instead of sincos(atan2(y, x)) (the angle itself is formed only for the power-law
+// branches), the trigonometric recurrence starts from them (2 sin^2(phi/2) = sin^2 / (1 + cos)),
+// reciprocals replace divisions, and the cell search uses the uniform ln r grid and the tau
+// bucket table.  Results agree with the reference's sequence to rounding.
+template<class F, class GP, bool NFW>
+__device__ __forceinline__ void multipoleCarCell(const AgxMultipoleDev& P, const F& fetch,
+    double x, double y, double z, double& pot, double& gx, double& gy, double& gz, GP gridR, GP gridT)
+{
+    const double R2 = x * x + y * y, R = sqrt(R2), rsq = R2 + z * z, r = sqrt(rsq);
+    const bool axis = R == 0;
+    const double Rinv = axis ? 0. : rcpD(R);
+    const double cphi = axis ? 1. : x * Rinv, sphi = axis ? 0. : y * Rinv;
+    const double logr = log(r), rplusRinv = rcpD(r + R),
+        tau = axis ? (z > 0 ? 1. : z < 0 ? -1. : 0.) : z * rplusRinv;
+    const int xi = cellIndexR(logr, gridR, P.nr, P.lnrScale),
+        yi = cellIndexT(tau, gridT, P.nt, gmem(P.tauLut), P.nTauLut);
+    const bool inGrid = xi >= 0 && yi >= 0;
+    const int xc = inGrid ? xi : 0, yc = inGrid ? yi : 0;
+    const double u = logr - gridR[xc], v = tau - gridT[yc];
+    const int ns = P.ns, zs = P.zeroSlot;
+    const int off0 = (xc * (P.nt - 1) + yc) * ns * AGX_CELL_SLOT_BYTES;
+    constexpr bool PF = std::is_same<F, FetchCell>::value;
+    vd2 ca[AGX_CELL_CHUNKS], cb[AGX_CELL_CHUNKS];
+    const int s0 = zs == 0 ? 1 : 0;   // the first slot after the m = 0 one
+    double P0, d0r, d0t;
+    if constexpr(PF) {
+        fetch.load(off0, zs, ca);
+        if(s0 < ns) fetch.load(off0, s0, cb);
+        cellHorner(ca, u, v, P0, d0r, d0t);
+    } else
+        fetch.slot(off0, zs, u, v, P0, d0r, d0t);
+    if(P.logScaling) {
+        const double expX = exp(P0), val = rcpD(P.invPhi0 - expX), dPhidX = val * val * expX;
+        P0 = val;
+        d0r *= dPhidX;
+        d0t *= dPhidX;
+    }
+    double trPot = P0, trDr = d0r, trDth = d0t, trDphi = 0;
+    if(ns > 1) {
+        // cos/sin(k phi) by the recurrence of trigMultiAngle (math_sphharm.cpp:134-152)
+        TrigRecT<double> pos, neg;
+        pos.alpha = cphi >= 0 ? sphi * sphi * rcpD(1 + cphi) : 1 - cphi;
+        pos.beta = sphi;
+        pos.restart();
+        neg.alpha = pos.alpha; neg.beta = pos.beta;
+        auto addSlot = [&](int s, double Pm, double dmr, double dmt) {
+            const int m = P.slotM[s];
+            double trig, dtrig;
+            if(m > 0) {
+                pos.advanceTo(m);
+                trig = pos.c; dtrig = double(-m) * pos.s;
+            } else {
+                neg.restart();
+                neg.advanceTo(-m);
+                trig = neg.s; dtrig = double(-m) * neg.c;
+            }
+            if(P.logScaling) {
+                dmr = dmr * P0 + Pm * d0r;
+                dmt = dmt * P0 + Pm * d0t;
+                Pm *= P0;
+            }
+            trPot  += Pm * trig;
+            trDr   += dmr * trig;
+            trDth  += dmt * trig;
+            trDphi += Pm * dtrig;
+        };
+        if constexpr(PF) {
+            // cb holds slot s on entry; the two buffers alternate
+            for(int s = s0; s < ns; ) {
+                int s2 = s + 1;
+                if(s2 == zs) s2++;
+                if(s2 < ns) fetch.load(off0, s2, ca);
+                double Pm, dmr, dmt;
+                cellHorner(cb, u, v, Pm, dmr, dmt);
+                addSlot(s, Pm, dmr, dmt);
+                if(s2 >= ns) break;
+                int s3 = s2 + 1;
+                if(s3 == zs) s3++;
+                if(s3 < ns) fetch.load(off0, s3, cb);
+                cellHorner(ca, u, v, Pm, dmr, dmt);
+                addSlot(s2, Pm, dmr, dmt);
+                s = s3;
+            }
+        } else {
+            for(int s = 0; s < ns; s++) {
+                if(s == zs) continue;
+                double Pm, dmr, dmt;
+                fetch.slot(off0, s, u, v, Pm, dmr, dmt);
+                addSlot(s, Pm, dmr, dmt);
+            }
+        }
+    }
+    const double rinv = rcpD(r), r2inv = rinv * rinv;
+    const double drdR = R * r2inv, drdz = z * r2inv, dthdR = -tau * rinv, dthdz = rinv - rplusRinv;
+    double gR = inGrid ? trDr * drdR + trDth * dthdR : NAN;
+    double gzc = inGrid ? trDr * drdz + trDth * dthdz : NAN;
+    double gphi = inGrid ? trDphi : NAN;
+    pot = inGrid ? trPot : NAN;
+    if(rsq < P.rsqInner || rsq > P.rsqOuter) {   // power-law extrapolation (lanes beyond the grid)
+        const double phi = axis ? 0. : atan2A(y, x);
+        if(rsq < P.rsqInner)
+            powerLawEval(P.inner, R, z, phi, pot, gR, gzc, gphi);
+        else
+            powerLawEval(P.outer, R, z, phi, pot, gR, gzc, gphi);
+    }
+    // BasePotentialCyl::evalCar: gradient back to Cartesian (on the axis the reference takes
+    // dR/dx = dR/dy = dphi/dx = dphi/dy = 1)
+    const double dRdx = axis ? 1. : cphi, dRdy = axis ? 1. : sphi,
+        dphidx = axis ? 1. : -sphi * Rinv, dphidy = axis ? 1. : cphi * Rinv;
+    gx = gR * dRdx + gphi * dphidx;
+    gy = gR * dRdy + gphi * dphidy;
+    gz = gzc;
+    if(NFW && P.nfwMass != 0) {
+        double pn, dn;
+        nfwDeriv(P.nfwMass, P.nfwScale, r, pn, dn);
+        const double xr = r == 0 ? 0. : x * rinv, yr = r == 0 ? 0. : y * rinv, zr = r == 0 ? 0. : z * rinv;
+        pot = pot + pn;
+        gx = xr * dn + gx;
+        gy = yr * dn + gy;
+        gz = zr * dn + gz;
+    }
+}
+
 // BasePotentialCyl::evalCar: Car -> Cyl, evaluate, gradient back to Car (+dPhi/dx)
 // NFW: whether the optional analytic halo is handled (kernels for potentials without one are
 // instantiated with false: keeping x, y, z live to the end costs a wavefront of occupancy)
@@ -851,6 +1121,13 @@ __device__ __forceinline__ void multipoleCar(const AgxMultipoleDev& P, const F& 
     double x, double y, double z, double& pot, double& gx, double& gy, double& gz,
     GP gridR = nullptr, GP gridT = nullptr)
 {
+    if constexpr(IsCellFetch<F>::value) {
+        if(!P.interp1d) {
+            multipoleCarCell<F, GP, NFW>(P, fetch, x, y, z, pot, gx, gy, gz, gridR ? gridR : GP(gmem(P.gridR)),
+                gridT ? gridT : GP(gmem(P.gridT)));
+            return;
+        }
+    }
     const double R = sqrt(x * x + y * y);
     double dRdx, dRdy, dphidx, dphidy, phi;
     if(R == 0) {

@@ -44,7 +44,8 @@ enum AgxPrecision { AGX_FP64 = 0, AGX_FP32_TABLES = 1 };
 
 // how FP64 node records are fetched: per-lane 8-byte loads, per-lane 16-byte loads, or the
 // wavefront-cooperative gather through LDS (agx_device.h)
-enum AgxFetch { AGX_FETCH_SCALAR = 0, AGX_FETCH_VEC2 = 1, AGX_FETCH_COOP = 2, AGX_FETCH_CELL = 3 };
+enum AgxFetch { AGX_FETCH_SCALAR = 0, AGX_FETCH_VEC2 = 1, AGX_FETCH_COOP = 2, AGX_FETCH_CELL = 3,
+    AGX_FETCH_CELL_COOP = 4 };
 
 // one recorded series - an orbit's trajectory or one of its deviation vectors - behind the
 // quintic-spline interpolators of Orbit objects (agx_orbit_interp.hip)

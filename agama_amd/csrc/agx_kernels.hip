@@ -39,7 +39,8 @@ template<int MODE, bool NFW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AGX_EVAL_WAVES))) void evalKernel(const AgxMultipoleDev* __restrict__ Pp,
     long n, const double* __restrict__ pos, double* __restrict__ phi, double* __restrict__ grad)
 {
-    __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? 4 * FetchCoop::LDS_DOUBLE2_PER_WAVE : 1];
+    __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? 4 * FetchCoop::LDS_DOUBLE2_PER_WAVE :
+                         MODE == AGX_FETCH_CELL_COOP ? 4 * FetchCellCoop::LDS_VD2_PER_WAVE : 1];
     const AgxMultipoleDev& P = *Pp;
     // short-lived blocks: the grids are read from global memory (staging them costs more)
     const double AGX_GLOBAL* gR = gmem(P.gridR);
@@ -56,6 +57,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AGX_EVAL_WA
         multipoleCar<decltype(f), const double AGX_GLOBAL*, NFW>(P, f, x, y, z, p, gx, gy, gz, gR, gT);
     } else if constexpr(MODE == AGX_FETCH_CELL) {
         FetchCell f(P.cells, P.cellBytes);
+        multipoleCar<decltype(f), const double AGX_GLOBAL*, NFW>(P, f, x, y, z, p, gx, gy, gz, gR, gT);
+    } else if constexpr(MODE == AGX_FETCH_CELL_COOP) {
+        FetchCellCoop f(P.cells, P.cellBytes, (vd2 AGX_LDS*)(stage + (threadIdx.x >> 6) * FetchCellCoop::LDS_VD2_PER_WAVE),
+            (int)(threadIdx.x & 63));
         multipoleCar<decltype(f), const double AGX_GLOBAL*, NFW>(P, f, x, y, z, p, gx, gy, gz, gR, gT);
     } else if constexpr(MODE == AGX_FETCH_VEC2) {
         FetchVec2 f{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
@@ -104,6 +109,10 @@ __device__ __forceinline__ void rhsEval(const AgxMultipoleDev& P, vd2* stage, do
     } else if constexpr(MODE == AGX_FETCH_CELL) {
         FetchCell f(P.cells, P.cellBytes);
         multipoleCar<decltype(f), LdsGrid, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
+    } else if constexpr(MODE == AGX_FETCH_CELL_COOP) {
+        FetchCellCoop f(P.cells, P.cellBytes, (vd2 AGX_LDS*)(stage + (threadIdx.x >> 6) * FetchCellCoop::LDS_VD2_PER_WAVE),
+            (int)(threadIdx.x & 63));
+        multipoleCar<decltype(f), LdsGrid, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     } else if constexpr(MODE == AGX_FETCH_VEC2) {
         FetchVec2 f{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
         multipoleCar<decltype(f), LdsGrid, NFW, AGX_ORBIT_PREFETCH != 0>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
@@ -132,9 +141,10 @@ void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, const AgxOrbitArgs* __r
     // at one wavefront per SIMD the block has the CU's LDS to itself: the stage vectors k1..k5
     // and x live there too (the cooperative fetcher needs its own LDS staging area instead)
     typedef typename std::conditional<TGT, LaneTgtT<OBS>,
-        LaneT<OBS, AGX_K_IN_LDS && WAVES == 1 && MODE != AGX_FETCH_COOP> >::type Lane;
+        LaneT<OBS, AGX_K_IN_LDS && WAVES == 1 && MODE != AGX_FETCH_COOP && MODE != AGX_FETCH_CELL_COOP> >::type Lane;
     __shared__ double lds[Lane::LDS_FIELDS * OBS];
-    __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? (OBS / 64) * FetchCoop::LDS_DOUBLE2_PER_WAVE : 1];
+    __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? (OBS / 64) * FetchCoop::LDS_DOUBLE2_PER_WAVE :
+                         MODE == AGX_FETCH_CELL_COOP ? (OBS / 64) * FetchCellCoop::LDS_VD2_PER_WAVE : 1];
     __shared__ double gridLds[AGX_LDS_GRID_ORBIT];
     const AgxMultipoleDev& P = *Pp;
     // the grids in LDS (the launcher refuses larger ones): binary searches read LDS directly
@@ -205,6 +215,8 @@ hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, in
         hipLaunchKernelGGL((evalKernel<-1, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     else if(fetch == AGX_FETCH_CELL && !P.interp1d)
         hipLaunchKernelGGL((evalKernel<AGX_FETCH_CELL, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
+    else if(fetch == AGX_FETCH_CELL_COOP && !P.interp1d)
+        hipLaunchKernelGGL((evalKernel<AGX_FETCH_CELL_COOP, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     else if(fetch == AGX_FETCH_COOP)
         hipLaunchKernelGGL((evalKernel<AGX_FETCH_COOP, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     else if(fetch == AGX_FETCH_VEC2)
@@ -229,6 +241,7 @@ OrbitKernelFn orbitKernelFor(int precision, int fetch, int waves, bool nfw, bool
     const bool two = waves == 2;
     if(precision == AGX_FP32_TABLES) return two ? orbitKernel<-1, 2, false> : orbitKernel<-1, 1, false>;
     if(fetch == AGX_FETCH_CELL) return two ? orbitKernel<AGX_FETCH_CELL, 2, false> : orbitKernel<AGX_FETCH_CELL, 1, false>;
+    if(fetch == AGX_FETCH_CELL_COOP) return orbitKernel<AGX_FETCH_CELL_COOP, 1, false>;
     if(fetch == AGX_FETCH_COOP) return two ? orbitKernel<AGX_FETCH_COOP, 2, false> : orbitKernel<AGX_FETCH_COOP, 1, false>;
     if(fetch == AGX_FETCH_VEC2) return two ? orbitKernel<AGX_FETCH_VEC2, 2, false> : orbitKernel<AGX_FETCH_VEC2, 1, false>;
     return two ? orbitKernel<AGX_FETCH_SCALAR, 2, false> : orbitKernel<AGX_FETCH_SCALAR, 1, false>;

@@ -58,6 +58,12 @@ struct AgxMultipoleDev {
     const float* nodesF;        // the same data rounded to float (precision-split form), or null
     const double* cells;        // per-cell polynomial form (see above), or null (1-D variant)
     unsigned int cellBytes;     // its size in bytes
+    // cell search of the FP64 kernels: (nr-1)/(lnr_last - lnr_0) for the first guess on the
+    // (near-uniform) ln r grid, and a table of nTauLut uniform buckets of [-1, 1]: tauLut[b] = the
+    // tau cell holding the bucket's left edge
+    double lnrScale;
+    int nTauLut;
+    const int* tauLut;
     AgxPowerLawDev inner, outer;
     // lmax <= 2: the 1-D variant replaces the 2-D tables (nt = ns = 0); logScaling and invPhi0
     // then refer to the l=0 coefficient Phi_00 (MultipoleInterp1d's convention)

@@ -1097,6 +1097,23 @@ std::vector<double> buildCellPolys(const HostTables& T)
     return cells;
 }
 
+std::vector<int> buildTauLookup(const HostTables& T)
+{
+    if(T.interp1d || T.nt < 2) return {};
+    double minw = INFINITY;
+    for(int j = 0; j + 1 < T.nt; j++) minw = std::min(minw, T.gridT[j + 1] - T.gridT[j]);
+    int nb = 64;
+    while(nb < 4096 && 2.0 / nb >= minw) nb *= 2;
+    std::vector<int> lut(nb);
+    for(int b = 0; b < nb; b++) {
+        const double t = -1 + 2.0 * b / nb;
+        int j = 0;
+        while(j < T.nt - 2 && t >= T.gridT[j + 1]) j++;
+        lut[b] = j;
+    }
+    return lut;
+}
+
 // MultipoleInterp1d ctor (potential_multipole.cpp:1691-1738): the extrapolation's 1/Phi_00(0)
 // from the two innermost nodes, log-scaling when every Phi_00(r_k) < 0, and a quintic spline in
 // ln r (values and ln r derivatives at the nodes, second derivatives by constructQuinticSpline,

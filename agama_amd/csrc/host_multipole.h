@@ -114,6 +114,10 @@ Coefs truncateCoefs(const Coefs& c, int lmax);
 // the (ln r, tau) grid and harmonic slot, the 36 coefficients of the bi-quintic that the
 // tensor-product quintic Hermite interpolation of the cell's four corner records equals
 std::vector<double> buildCellPolys(const HostTables& T);
+// Bucket table of the tau grid for the FP64 kernels' cell search (agx_tables.h): a power of two
+// of uniform buckets of [-1, 1], each narrower than the narrowest tau cell (at most 4096), and
+// for each the index of the cell holding its left edge
+std::vector<int> buildTauLookup(const HostTables& T);
 
 // the reference's sin/cos (math::sincos, src/math_core.cpp:171-206: multiples of pi/2 exact)
 void sincosAgama(double x, double& s, double& c);

@@ -53,7 +53,7 @@ FLOPS_PER_ATTEMPT_EXTRA_BY_METHOD = {"dop853": FLOPS_PER_ATTEMPT_EXTRA, "dprkn8"
 METHOD_FLAG = {"dop853": 0, "dprkn8": 0x200}
 FLOPS_PER_ORBIT_INIT = 3 * FLOPS_PER_EVAL + 80
 # record fetch modes of the Multipole evaluation (agx_set_fetch)
-FETCH_MODES = {"scalar": 0, "vec2": 1, "coop": 2, "cell": 3}
+FETCH_MODES = {"scalar": 0, "vec2": 1, "coop": 2, "cell": 3, "cellcoop": 4}
 # launches timed for a leg that is reported beside the headline figure (not the headline)
 COMPARISON_STEPS = 2
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix), AMD published figure
@@ -90,8 +90,11 @@ def parse():
                     help="group mode of the FP64 DOP853 kernel, STRIDE[,ORBITS] (STRIDE 0: off): one "
                          "wavefront in STRIDE integrates the first ORBITS orbits of the order with 8 lanes "
                          "per orbit; default: the library's automatic choice (off at 1M orbits)")
-    ap.add_argument("--fetch", choices=["scalar", "vec2", "coop", "cell"], default=None,
+    ap.add_argument("--fetch", choices=list(FETCH_MODES), default=None,
                     help="record gather of the FP64 orbit kernel (default: the library's, vec2)")
+    ap.add_argument("--periods", type=float, default=0,
+                    help="FP64 path only: integrate every orbit for this many circular periods instead "
+                         "of the log-uniform 1-300 (a uniform batch, for profiling the bulk)")
     ap.add_argument("--train-steps", type=int, default=6000)
     ap.add_argument("--train-samples", type=float, default=20e6)
     ap.add_argument("--max-rel-de", type=float, default=1e-6)
@@ -220,7 +223,10 @@ def main():
     if args.groups:
         g = [int(v) for v in args.groups.split(",")]
         assert lib.agx_set_orbit_groups(g[0], g[1] if len(g) > 1 else -1, 0) == 0
-    ic, T = workloads.triaxial_dehnen_orbits(n, lambda p: pot.potential(p), pot.coefs(), seed=42, uneven=True)
+    if args.periods > 0 and args.mode != "fp64":
+        raise SystemExit("bench.py: --periods applies to --mode fp64 only")
+    ic, T = workloads.triaxial_dehnen_orbits(n, lambda p: pot.potential(p), pot.coefs(), seed=42,
+                                             uneven=args.periods <= 0, periods=args.periods)
     prec = agama_amd.PRECISION_FP64 if args.precision == "fp64" else agama_amd.PRECISION_FP32
 
     d_ic = torch.from_numpy(ic).to(dev)
@@ -381,8 +387,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f64 (fp32 spline tables)",
         "data": "synthetic",
-        "config": {"workload": "triaxial Dehnen Multipole lmax=8, 1M orbits, log-uniform 1-300 circular "
-                               f"periods, {args.method.upper()} accuracy 1e-8, final state + max dE/E <= 1e-6",
+        "config": {"workload": f"triaxial Dehnen Multipole lmax=8, {n} orbits, " +
+                               (f"{args.periods:g}" if args.periods > 0 else "log-uniform 1-300") +
+                               f" circular periods, {args.method.upper()} accuracy 1e-8, final state + max "
+                               "dE/E <= 1e-6",
                    "orbits": n, "accepted_steps_per_step": steps_per_launch,
                    "attempts_per_step": attempts, "precision": args.precision,
                    "path": "MLP surrogate with FP64 fallback" if surrogate else "FP64 Multipole",

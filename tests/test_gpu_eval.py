@@ -164,12 +164,12 @@ def test_host_entry_points_share_a_handle_across_threads(pots):
     assert torch.cuda.current_device() == 0
 
 
-@pytest.mark.parametrize("fetch", [0, 1, 2, 3])
+@pytest.mark.parametrize("fetch", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("name", ["dehnen_triax_lmax8.coef", "disk_bulge_halo_lmax12.coef",
                                   "ferrers_bar_lmax16.coef"])
 def test_eval_every_fetch_mode(name, fetch):
     """every evaluation form of the FP64 kernel (8- and 16-byte record gathers, the cooperative
-    gather, the per-cell polynomial tables) against the oracle at 200k density points and at
+    gather, the per-cell polynomial tables read per lane or cooperatively) against the oracle at 200k density points and at
     the grid nodes, cell edges and both extrapolation branches; bar 1e-12 relative"""
     import agama_amd
     from agama_amd import lib, workloads
