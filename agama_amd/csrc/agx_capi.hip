@@ -47,6 +47,9 @@ hipError_t agxPoissonDevice(const agx::PoissonSetup& S, const agx::Density& dens
     std::vector<double>& dPhi);
 int agxOrbitKernelBlockSize();
 size_t agxScheduleWorkspace(long n);
+double* agxScheduleProbeTimes(void* work, long n);
+hipError_t agxLaunchScheduleFromProbe(long n, const double* tstart, const double* ttotal, int probeSteps,
+    int* order, void* work, int** finalOrder, hipStream_t stream);
 hipError_t agxLaunchSchedule(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, long n,
     const double* ic, const double* ttotal, int* order, void* work, hipStream_t stream);
 
@@ -80,7 +83,7 @@ namespace {
 
 thread_local std::string g_err;
 int g_orbitBlocks = 0;
-int g_fetchEval = AGX_FETCH_VEC2, g_fetchOrbit = AGX_FETCH_VEC2;   // set by agx_set_fetch
+int g_fetchEval = AGX_FETCH_CELL, g_fetchOrbit = AGX_FETCH_CELL;   // set by agx_set_fetch
 int g_orbitWaves = 1;                                                // set by agx_set_orbit_waves
 // group mode of the DOP853 kernel (agx_set_orbit_groups): one wavefront in g_groupStride
 // integrates the longest orbits with 8 lanes each, for batches of at least g_groupMinBatch;
@@ -91,6 +94,10 @@ long g_groupOrbits = -1, g_groupMinBatch = 65536;
 int fail(int code, const std::string& msg) { g_err = msg; return code; }
 
 // the surrogate orbit batch's pilot (agx_set_surrogate_pilot) and its last outcome
+// work-order probe (agx_set_orbit_probe): steps per orbit (0: off) and the smallest batch probed
+int g_probeSteps = 64;
+long g_probeMinBatch = 65536;
+
 int g_surrogatePilot = 1;
 thread_local long g_lastPilot[2] = { 0, 0 };   // pilot orbits, of which fell back
 
@@ -123,6 +130,9 @@ void upload(agx_potential& p)
     const size_t oCell = take(cells.size() * 8);
     const std::vector<int> tauLut = agx::buildTauLookup(T);
     const size_t oLut = take(tauLut.size() * 4);
+    const int legLmax = std::max(T.inner.lmax, T.outer.lmax);
+    const std::vector<double> legTab = agx::buildLegendreTable(legLmax);
+    const size_t oLeg = take(legTab.size() * 8);
     HIPCHECK(hipGetDevice(&p.device));
     HIPCHECK(hipMalloc(&p.dmem, off));
     char* base = static_cast<char*>(p.dmem);
@@ -139,6 +149,7 @@ void upload(agx_potential& p)
     put(oN1, T.nodes1d.data(), T.nodes1d.size() * 8);
     put(oCell, cells.data(), cells.size() * 8);
     put(oLut, tauLut.data(), tauLut.size() * 4);
+    put(oLeg, legTab.data(), legTab.size() * 8);
     HIPCHECK(hipMemset(base + oC, 0, 64));
     AgxMultipoleDev& D = p.dev;
     memset(&D, 0, sizeof(D));
@@ -158,6 +169,9 @@ void upload(agx_potential& p)
     D.lnrScale = T.nr > 1 ? (T.nr - 1) / (T.gridR.back() - T.gridR.front()) : 0;
     D.nTauLut = (int)tauLut.size();
     D.tauLut = tauLut.empty() ? nullptr : reinterpret_cast<const int*>(base + oLut);
+    D.legLmax = legLmax;
+    D.legTab = reinterpret_cast<const double*>(base + oLeg);
+    D.legStart = D.legTab + (size_t)(legLmax + 1) * (legLmax + 1) * 3;
     auto fillPL = [&](AgxPowerLawDev& L, const agx::HostPowerLaw& H, size_t s, size_t u, size_t w) {
         L.r0sq = H.r0sq; L.Q = H.Q; L.inner = H.inner; L.lmax = H.lmax; L.mmax = H.mmax;
         L.mmin = H.mmin; L.step = H.step;
@@ -582,8 +596,8 @@ int agx_set_orbit_launch(int num_blocks, int block_size)
 int agx_set_fetch(int eval_mode, int orbit_mode)
 {
     auto ok = [](int m) { return m == AGX_FETCH_SCALAR || m == AGX_FETCH_VEC2 || m == AGX_FETCH_COOP ||
-                                 m == AGX_FETCH_CELL || m == AGX_FETCH_CELL_COOP; };
-    if(!ok(eval_mode) || !ok(orbit_mode)) return fail(AGX_EINVAL, "fetch mode must be 0 to 4");
+                                 m == AGX_FETCH_CELL; };
+    if(!ok(eval_mode) || !ok(orbit_mode)) return fail(AGX_EINVAL, "fetch mode must be 0 to 3");
     g_fetchEval = eval_mode;
     g_fetchOrbit = orbit_mode;
     return AGX_OK;
@@ -602,6 +616,14 @@ int agx_set_orbit_groups(int stride, long orbits, long min_batch)
     g_groupStride = stride;
     g_groupOrbits = orbits;
     g_groupMinBatch = min_batch;
+    return AGX_OK;
+}
+
+int agx_set_orbit_probe(int steps, long min_batch)
+{
+    if(steps < 0 || min_batch < 0) return fail(AGX_EINVAL, "invalid probe settings");
+    g_probeSteps = steps;
+    g_probeMinBatch = min_batch;
     return AGX_OK;
 }
 
@@ -641,6 +663,34 @@ static size_t scheduleWorkspaceBytes(long n)
     return agxScheduleWorkspace(n) + ((size_t)n * 4 + 255) / 256 * 256;
 }
 
+// group stride of large probed batches: the probe ranks the truly longest orbits first, and one
+// wavefront in this many integrates them with a group of lanes per orbit
+constexpr int PROBED_GROUP_STRIDE = 32;
+
+// Re-rank the work order `ord` (agxLaunchSchedule's, inside `work`) from a probe: every orbit is
+// integrated for at most g_probeSteps steps, without output but the time it reached, and the
+// orbits are sorted by the steps they would take at that rate.  Returns the new order (inside
+// `work`).
+static int* probeOrder(const agx_potential* pot, long n, const double* d_ic, const double* d_tstart,
+    const double* d_ttotal, double accuracy, double Omega, long maxNumSteps, int precision, int* ord, void* work,
+    hipStream_t st, AgxOrbitArgs* dArgs)
+{
+    AgxOrbitArgs P{};
+    P.n = n; P.ic = d_ic; P.tstart = d_tstart; P.ttotal = d_ttotal; P.trajsize = nullptr; P.trajOffset = nullptr;
+    P.order = ord; P.accuracy = accuracy; P.Omega = Omega;
+    P.maxNumSteps = std::min<long>(maxNumSteps, g_probeSteps);
+    P.counter = pot->dcounter;
+    P.outTend = agxScheduleProbeTimes(work, n);
+    int blocks, bs;
+    orbitGeometry(n, precision, pot->nfwMass != 0, blocks, bs);
+    HIPCHECK(agxLaunchFillU32(pot->dcounter, 2, 0u, st));
+    HIPCHECK(agxLaunchOrbit(pot->dev, pot->ddev, precision, g_fetchOrbit, g_orbitWaves, P, blocks, bs, st,
+        dArgs ? dArgs : pot->dargs));
+    int* fin = nullptr;
+    HIPCHECK(agxLaunchScheduleFromProbe(n, d_tstart, d_ttotal, g_probeSteps, ord, work, &fin, st));
+    return fin;
+}
+
 // the batch entry points' common body; d_dev non-null or ly.bins non-null: the variational
 // kernel (der=True: six vectors; lyapunov only: one)
 static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic,
@@ -672,6 +722,7 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
     try {
         hipStream_t st = (hipStream_t)stream;
         void* work = nullptr;
+        bool probed = false;
         if(!d_order && !(flags & AGX_ORDER_AS_GIVEN)) {
             // longest expected integrations first (agx_schedule.hip); stream-ordered workspace,
             // or the caller's (scheduleWork: at least scheduleWorkspaceBytes(n) bytes)
@@ -685,6 +736,10 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
                 return fail(AGX_EDEVICE, std::string("schedule: ") + hipGetErrorString(e));
             }
             d_order = ord;
+            if(g_probeSteps > 0 && n >= g_probeMinBatch && !variational && !d_targets && !pool.counter)
+                d_order = probeOrder(pot, n, d_ic, d_tstart, d_ttotal, accuracy, Omega, maxNumSteps, precision, ord,
+                    work, st, nullptr);
+            probed = d_order != ord;
         }
         AgxOrbitArgs A;
         A.n = n; A.ic = d_ic; A.tstart = d_tstart; A.ttotal = d_ttotal; A.trajsize = d_trajsize;
@@ -706,10 +761,11 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
             // total work) set the batch's duration (measured on the uneven 1-300-period
             // workload: 131k orbits -11% at one wavefront in 4, 262k -4% at one in 8, 1M neutral)
             const long lanes = (long)blocks * bs;
-            groupStride = n * 8 <= lanes ? 1 : n <= 2 * lanes ? 4 : n <= 4 * lanes ? 8 : 0;
+            groupStride = n * 8 <= lanes ? 1 : n <= 2 * lanes ? 4 : n <= 4 * lanes ? 8 : probed ? PROBED_GROUP_STRIDE : 0;
         }
         if(allowGroups && groupStride > 0 && n >= g_groupMinBatch && !variational && !d_targets && !pool.counter &&
-           !(flags & AGX_METHOD_DPRKN8) && precision == AGX_PRECISION_FP64 && g_fetchOrbit == AGX_FETCH_VEC2 &&
+           !(flags & AGX_METHOD_DPRKN8) && precision == AGX_PRECISION_FP64 &&
+           (g_fetchOrbit == AGX_FETCH_VEC2 || g_fetchOrbit == AGX_FETCH_CELL) &&
            g_orbitWaves == 1 && pot->nfwMass == 0 && !pot->dev.interp1d) {
             const long waves = (long)blocks * (bs / 64);
             const long stride = std::min<long>(groupStride, waves);   // wavefront 0 is always a group one

@@ -435,20 +435,23 @@ __device__ __forceinline__ void interiorEval(const AgxMultipoleDev& P, const F& 
 constexpr int AGX_CELL_SLOT_BYTES = AGX_CELL_COEFS * 8;
 constexpr int AGX_CELL_CHUNKS = AGX_CELL_SLOT_BYTES / 16;   // 16-byte loads per slot
 
-// Horner evaluation of one slot from its 18 coefficient pairs (row a: c[3a .. 3a+2])
-__device__ __forceinline__ void cellHorner(const vd2 (&c)[AGX_CELL_CHUNKS], double u, double v,
+// Horner evaluation of one slot from its 36 coefficients (row a: c[6a .. 6a+5]).  Each
+// coefficient sits in a register of its own (8-byte loads): with halves of 16-byte loads the
+// compiler formed most multiply-adds as a copy of the addend plus v_fmac_f64, since the tuple's
+// other half was still live.
+__device__ __forceinline__ void cellHorner(const double (&c)[AGX_CELL_COEFS], double u, double v,
     double& p, double& pu, double& pv)
 {
     double P = 0, Pu = 0, Pv = 0;
 #pragma unroll
     for(int a = 5; a >= 0; a--) {
-        const vd2 c01 = c[3 * a], c23 = c[3 * a + 1], c45 = c[3 * a + 2];
+        const double* r = c + 6 * a;
         // row a: A = sum_b c[a][b] v^b and B = dA/dv
-        double A = c45.y * v + c45.x, B = c45.y;
-        B = B * v + A; A = A * v + c23.y;
-        B = B * v + A; A = A * v + c23.x;
-        B = B * v + A; A = A * v + c01.y;
-        B = B * v + A; A = A * v + c01.x;
+        double A = r[5] * v + r[4], B = r[5];
+        B = B * v + A; A = A * v + r[3];
+        B = B * v + A; A = A * v + r[2];
+        B = B * v + A; A = A * v + r[1];
+        B = B * v + A; A = A * v + r[0];
         if(a == 5) { P = A; Pv = B; }
         else { Pu = Pu * u + P; P = P * u + A; Pv = Pv * u + B; }
     }
@@ -462,77 +465,30 @@ struct FetchCell {
     __amdgpu_buffer_rsrc_t rsrc;
     __device__ __forceinline__ FetchCell(const double* cells, unsigned int bytes)
         : rsrc(__builtin_amdgcn_make_buffer_rsrc((void*)cells, 0, (int)bytes, 0x00020000)) {}
-    // 16 bytes at byte offset off
-    __device__ __forceinline__ vd2 ld(int off) const
+    // 8 bytes at byte offset off
+    __device__ __forceinline__ double ld(int off) const
     {
-        return __builtin_bit_cast(vd2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
     }
     // the coefficients of slot s of the cell at byte offset cellOff
-    __device__ __forceinline__ void load(int cellOff, int s, vd2 (&c)[AGX_CELL_CHUNKS]) const
+    __device__ __forceinline__ void load(int cellOff, int s, double (&c)[AGX_CELL_COEFS]) const
     {
         const int off = cellOff + s * AGX_CELL_SLOT_BYTES;
 #pragma unroll
-        for(int t = 0; t < AGX_CELL_CHUNKS; t++) c[t] = ld(off + 16 * t);
+        for(int t = 0; t < AGX_CELL_COEFS; t++) c[t] = ld(off + 8 * t);
     }
     // value and (u, v) derivatives of slot s of the cell at byte offset cellOff
     __device__ __forceinline__ void slot(int cellOff, int s, double u, double v, double& p, double& pu,
         double& pv) const
     {
-        vd2 c[AGX_CELL_CHUNKS];
+        double c[AGX_CELL_COEFS];
         load(cellOff, s, c);
-        cellHorner(c, u, v, p, pu, pv);
-    }
-};
-
-// Cooperative gather of one slot for the whole wavefront: the 64 lanes' 288-byte slot blocks
-// (64 x 18 chunks) are read by 18 load instructions in which lane l fetches chunk k = l + 64t of
-// the wavefront's list (owner lane k / 18, chunk k % 18, the owner's cell offset obtained with
-// ds_bpermute), so that one instruction touches about four blocks' cache lines instead of 64;
-// the chunks are passed to their owners through LDS (written lane-consecutive, read with a
-// 288-byte lane stride: conflict-free in groups of 8 lanes).  All 64 lanes must call slot()
-// together (wave-uniform control flow).
-struct FetchCellCoop {
-    typedef double real;
-    static constexpr bool uniform = true;
-    static constexpr int LDS_VD2_PER_WAVE = 64 * AGX_CELL_CHUNKS;
-    __amdgpu_buffer_rsrc_t rsrc;
-    vd2 AGX_LDS* lds;              // this wavefront's staging area
-    int lane;
-    int perm[AGX_CELL_CHUNKS];     // 4 * owner lane of chunk l + 64t
-    int coff[AGX_CELL_CHUNKS];     // byte offset of that chunk in its slot block
-    __device__ __forceinline__ FetchCellCoop(const double* cells, unsigned int bytes, vd2 AGX_LDS* stage, int l)
-        : rsrc(__builtin_amdgcn_make_buffer_rsrc((void*)cells, 0, (int)bytes, 0x00020000)), lds(stage), lane(l)
-    {
-#pragma unroll
-        for(int t = 0; t < AGX_CELL_CHUNKS; t++) {
-            const int k = l + 64 * t, r = k / AGX_CELL_CHUNKS;
-            perm[t] = r * 4;
-            coff[t] = (k - r * AGX_CELL_CHUNKS) * 16;
-        }
-    }
-    __device__ __forceinline__ void slot(int cellOff, int s, double u, double v, double& p, double& pu,
-        double& pv) const
-    {
-        vd2 t4[AGX_CELL_CHUNKS], c[AGX_CELL_CHUNKS];
-        const int soff = s * AGX_CELL_SLOT_BYTES;
-#pragma unroll
-        for(int t = 0; t < AGX_CELL_CHUNKS; t++) {
-            const int owner = __builtin_amdgcn_ds_bpermute(perm[t], cellOff);
-            t4[t] = __builtin_bit_cast(vd2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, owner + coff[t], soff, 0));
-        }
-#pragma unroll
-        for(int t = 0; t < AGX_CELL_CHUNKS; t++) lds[lane + 64 * t] = t4[t];
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for(int t = 0; t < AGX_CELL_CHUNKS; t++) c[t] = lds[lane * AGX_CELL_CHUNKS + t];
-        __builtin_amdgcn_wave_barrier();
         cellHorner(c, u, v, p, pu, pv);
     }
 };
 
 template<class F> struct IsCellFetch : std::false_type {};
 template<> struct IsCellFetch<FetchCell> : std::true_type {};
-template<> struct IsCellFetch<FetchCellCoop> : std::true_type {};
 
 // interiorEval on the per-cell polynomial tables (same control flow and Fourier synthesis).
 // With the per-lane fetcher the next harmonic's coefficients are requested before the current
@@ -551,8 +507,8 @@ __device__ __forceinline__ void interiorEvalCell(const AgxMultipoleDev& P, const
     const int ns = P.ns;
     const int off0 = (xc * (P.nt - 1) + yc) * ns * AGX_CELL_SLOT_BYTES;
     const int zs = P.zeroSlot;
-    constexpr bool PF = std::is_same<F, FetchCell>::value;
-    vd2 ca[AGX_CELL_CHUNKS], cb[AGX_CELL_CHUNKS];
+    constexpr bool PF = true;
+    double ca[AGX_CELL_COEFS], cb[AGX_CELL_COEFS];
     const int s0 = zs == 0 ? 1 : 0;   // the first slot after the m = 0 one
     double P0, d0r, d0t;
     if constexpr(PF) {
@@ -987,7 +943,146 @@ __device__ __forceinline__ int cellIndexT(double t, GP arr, int n, const int AGX
     return i;
 }
 
-// multipoleCar on the per-cell tables (fetch: FetchCell or FetchCellCoop).  The same function as
+// PowerLawMultipole::evalCyl (src/potential_multipole.cpp:1600-1658) with the spherical-harmonic
+// synthesis of sphHarmTransformInverseDeriv (:778-821) for the FP64 kernels' per-cell path: the
+// same terms and sums as powerLawEval, with the Legendre normalisations and recurrence factors
+// taken from a table (no square roots or divisions in the recurrence), (r/r0)^v for the integer
+// powers v formed by multiplication, cos/sin(theta) = z/r, R/r and cos/sin(phi) given.  Agrees
+// with powerLawEval to rounding.
+// the arrays that powerLawEvalFast reads, in global memory or staged in LDS (plStage): S, U, W of
+// the inner [0] and outer [1] extrapolation for c < (legLmax+1)^2, the Legendre table and legStart
+template<class GP>
+struct PLTabs {
+    GP S[2], U[2], W[2], leg, legStart;
+};
+__device__ __forceinline__ PLTabs<const double AGX_GLOBAL*> plTabsGlobal(const AgxMultipoleDev& P)
+{
+    PLTabs<const double AGX_GLOBAL*> t;
+    t.S[0] = gmem(P.inner.S); t.U[0] = gmem(P.inner.U); t.W[0] = gmem(P.inner.W);
+    t.S[1] = gmem(P.outer.S); t.U[1] = gmem(P.outer.U); t.W[1] = gmem(P.outer.W);
+    t.leg = gmem(P.legTab); t.legStart = gmem(P.legStart);
+    return t;
+}
+// doubles of the LDS copy made by plStage
+__device__ __forceinline__ int plStageDoubles(const AgxMultipoleDev& P)
+{
+    const int L1 = P.legLmax + 1;
+    return 9 * L1 * L1 + L1;
+}
+// copy the power-law arrays into `lds` (plStageDoubles(P) doubles) with the whole block; the
+// caller synchronises the block afterwards
+__device__ __forceinline__ PLTabs<const double AGX_LDS*> plStage(const AgxMultipoleDev& P, double AGX_LDS* lds)
+{
+    const int L1 = P.legLmax + 1, nc = L1 * L1;
+    const double* src[8] = { P.inner.S, P.inner.U, P.inner.W, P.outer.S, P.outer.U, P.outer.W, P.legTab, P.legStart };
+    const int len[8] = { nc, nc, nc, nc, nc, nc, 3 * nc, L1 };
+    PLTabs<const double AGX_LDS*> t;
+    const double AGX_LDS* dst[8];
+    int off = 0;
+    for(int a = 0; a < 8; a++) {
+        dst[a] = lds + off;
+        for(int i = threadIdx.x; i < len[a]; i += blockDim.x) lds[off + i] = src[a][i];
+        off += len[a];
+    }
+    t.S[0] = dst[0]; t.U[0] = dst[1]; t.W[0] = dst[2]; t.S[1] = dst[3]; t.U[1] = dst[4]; t.W[1] = dst[5];
+    t.leg = dst[6]; t.legStart = dst[7];
+    return t;
+}
+
+template<class GP>
+__device__ __forceinline__ void powerLawEvalFast(const AgxMultipoleDev& P, const AgxPowerLawDev& L,
+    const PLTabs<GP>& T, double R, double z, double r, double cphi, double sphi, double& pot, double& gR,
+    double& gz, double& gphi)
+{
+    const double rsq = r * r;
+    const bool mono = (L.inner && rsq < L.r0sq * 1e-16) || (!L.inner && rsq > L.r0sq * 1e16) || L.lmax == 0;
+    if(mono || (L.lmax == 2 && L.mmin == 0 && L.step == 2)) {   // the cheap branches as they are
+        const double phi = R == 0 ? 0. : atan2A(R * sphi, R * cphi);
+        powerLawEval(L, R, z, phi, pot, gR, gz, gphi);
+        return;
+    }
+    const double dlogr = log(rsq / L.r0sq) * 0.5, q = exp(dlogr), qstep = L.inner ? q : rcpD(q);
+    const double rinv = rcpD(r), ct = z * rinv, st = R * rinv, stinv = st > 0 ? rcpD(st) : 0.;
+    const int side = L.inner ? 0 : 1;
+    const GP S = T.S[side], U = T.U[side], W = T.W[side];
+    const int L1 = P.legLmax + 1;
+    TrigRecT<double> pos, neg;
+    pos.alpha = cphi >= 0 ? sphi * sphi * rcpD(1 + cphi) : 1 - cphi;
+    pos.beta = sphi;
+    pos.restart();
+    neg.alpha = pos.alpha; neg.beta = pos.beta;
+    double vr = 0, gr = 0, gth = 0, gph = 0;
+    for(int pass = 0; pass < 2; pass++)
+    for(int m = (pass == 0 ? 0 : L.mmin); m <= (pass == 0 ? 0 : L.mmax); m++) {
+        if(pass == 1 && m == 0) continue;
+        const int lmin = L.lmin[m + L.mmax];
+        if(lmin > L.lmax) continue;
+        const int am = m < 0 ? -m : m;
+        const GP tab = T.leg + am * L1 * 3;
+        // P_am^am and its theta derivative before normalisation (legendrePmm)
+        double Plm1, dP0;
+        if(am == 0) { Plm1 = 1; dP0 = 0; }
+        else if(am == 1) { Plm1 = -st; dP0 = -ct; }
+        else {
+            double sinm2 = 1, xb = st;
+            for(int k = am - 2; k; k >>= 1) { if(k % 2) sinm2 *= xb; xb *= xb; }
+            const double coef = T.legStart[am];
+            dP0 = am * coef * sinm2 * st * ct;
+            Plm1 = coef * sinm2 * (st * st);
+        }
+        double Plm = ct * (2 * am + 1) * Plm1, Plm2 = 0;
+        // (r/r0)^v at l = am: v = l inside, -l-1 outside
+        double qv = L.inner ? 1. : qstep;
+        for(int k = 0; k < am; k++) qv *= qstep;
+        double Cv = 0, Cr = 0, Ct = 0;
+        for(int l = am; l <= L.lmax; l++, qv *= qstep) {
+            const double nrm = tab[l * 3];
+            double Pl, dPl;
+            if(l == am) { Pl = Plm1 * nrm; dPl = dP0 * nrm; }
+            else {
+                if(l > am + 1) Plm = ct * tab[l * 3 + 1] * Plm1 - tab[l * 3 + 2] * Plm2;
+                Pl = Plm * nrm;
+                double d = 0;
+                if(st >= 1e-8 || (am > 2 && st > 0)) d = (l * ct * Plm - (l + am) * Plm1) * stinv;
+                else if(am == 0) d = -l * (l + 1) / 2 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
+                else if(am == 1) d = -l * (l + 1) / 2 * (ct > 0 || l % 2 == 0 ? 1 : -1);
+                else if(am == 2) d = l * (l + 1) * (l + 2) * (l - 1) / 4 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
+                dPl = nrm * d;
+                Plm2 = Plm1;
+                Plm1 = Plm;
+            }
+            if(l >= lmin && (l - lmin) % L.step == 0) {
+                const int c = l * (l + 1) + m;
+                const double sc = S[c], u = U[c], w = W[c], v = L.inner ? l : -l - 1;
+                const double rv = qv;
+                const double rs = sc != v ? (sc != 0 ? exp(dlogr * sc) : 1) : rv;
+                const double urs = u * rs * (sc != v || u == 0 ? 1 : dlogr);
+                const double wrv = w * rv;
+                const double qr2 = v == 0 ? L.Q * rsq / L.r0sq : 0;
+                const double Pv = urs + wrv + qr2, dPv = urs * sc + wrv * v + (sc != v ? 0 : u * rs) + qr2 * 2;
+                Cv += Pl * Pv;
+                Cr += Pl * dPv;
+                Ct += dPl * Pv;
+            }
+        }
+        if(m == 0) { vr = Cv; gr = Cr; gth = Ct; continue; }
+        double trig, dtrig;
+        if(m > 0) { pos.advanceTo(m); trig = pos.c; dtrig = -m * pos.s; }
+        else { neg.restart(); neg.advanceTo(-m); trig = neg.s; dtrig = -m * neg.c; }
+        vr  += Cv * trig;
+        gr  += Cr * trig;
+        gth += Ct * trig;
+        gph += Cv * dtrig;
+    }
+    pot = vr;
+    // transformDerivsSphToCyl
+    const double r2inv = rinv * rinv, drdR = R * r2inv, drdz = z * r2inv;
+    gR = gr * drdR + gth * drdz;
+    gz = gr * drdz + gth * -drdR;
+    gphi = gph;
+}
+
+// multipoleCar on the per-cell tables (fetch: FetchCell).  The same function as
 // Multipole::evalCyl behind BasePotentialCyl::evalCar (src/potential_multipole.cpp:1538-1548,
 // 1969-2080), arranged for fewer instructions: cos and sin of the azimuth come from x/R and
 // y/R instead of sincos(atan2(y, x)) (the angle itself is formed only for the power-law
@@ -996,7 +1091,8 @@ __device__ __forceinline__ int cellIndexT(double t, GP arr, int n, const int AGX
 // bucket table.  Results agree with the reference's sequence to rounding.
 template<class F, class GP, bool NFW>
 __device__ __forceinline__ void multipoleCarCell(const AgxMultipoleDev& P, const F& fetch,
-    double x, double y, double z, double& pot, double& gx, double& gy, double& gz, GP gridR, GP gridT)
+    double x, double y, double z, double& pot, double& gx, double& gy, double& gz, GP gridR, GP gridT,
+    const PLTabs<GP>& plt)
 {
     const double R2 = x * x + y * y, R = sqrt(R2), rsq = R2 + z * z, r = sqrt(rsq);
     const bool axis = R == 0;
@@ -1011,8 +1107,8 @@ __device__ __forceinline__ void multipoleCarCell(const AgxMultipoleDev& P, const
     const double u = logr - gridR[xc], v = tau - gridT[yc];
     const int ns = P.ns, zs = P.zeroSlot;
     const int off0 = (xc * (P.nt - 1) + yc) * ns * AGX_CELL_SLOT_BYTES;
-    constexpr bool PF = std::is_same<F, FetchCell>::value;
-    vd2 ca[AGX_CELL_CHUNKS], cb[AGX_CELL_CHUNKS];
+    constexpr bool PF = true;
+    double ca[AGX_CELL_COEFS], cb[AGX_CELL_COEFS];
     const int s0 = zs == 0 ? 1 : 0;   // the first slot after the m = 0 one
     double P0, d0r, d0t;
     if constexpr(PF) {
@@ -1088,13 +1184,10 @@ __device__ __forceinline__ void multipoleCarCell(const AgxMultipoleDev& P, const
     double gzc = inGrid ? trDr * drdz + trDth * dthdz : NAN;
     double gphi = inGrid ? trDphi : NAN;
     pot = inGrid ? trPot : NAN;
-    if(rsq < P.rsqInner || rsq > P.rsqOuter) {   // power-law extrapolation (lanes beyond the grid)
-        const double phi = axis ? 0. : atan2A(y, x);
-        if(rsq < P.rsqInner)
-            powerLawEval(P.inner, R, z, phi, pot, gR, gzc, gphi);
-        else
-            powerLawEval(P.outer, R, z, phi, pot, gR, gzc, gphi);
-    }
+    if(rsq < P.rsqInner)   // power-law extrapolation (lanes beyond the grid)
+        powerLawEvalFast(P, P.inner, plt, R, z, r, cphi, sphi, pot, gR, gzc, gphi);
+    else if(rsq > P.rsqOuter)
+        powerLawEvalFast(P, P.outer, plt, R, z, r, cphi, sphi, pot, gR, gzc, gphi);
     // BasePotentialCyl::evalCar: gradient back to Cartesian (on the axis the reference takes
     // dR/dx = dR/dy = dphi/dx = dphi/dy = 1)
     const double dRdx = axis ? 1. : cphi, dRdy = axis ? 1. : sphi,
@@ -1113,6 +1206,85 @@ __device__ __forceinline__ void multipoleCarCell(const AgxMultipoleDev& P, const
     }
 }
 
+// multipoleCarCell for an aligned group of G lanes sharing one point (the longest orbits of a
+// batch): lane g evaluates the harmonic slots s = g, g + G, ... and the partial sums (the m = 0
+// value and derivatives, the m != 0 Fourier sums) are added across the group by a butterfly,
+// which leaves the same bits on every lane of the group.  Only the order of the additions
+// differs from the one-lane form.  All lanes of the wavefront must execute it together.
+template<int G, class GP>
+__device__ __forceinline__ void multipoleCarCellGroup(const AgxMultipoleDev& P, const FetchCell& fetch,
+    double x, double y, double z, double& pot, double& gx, double& gy, double& gz, GP gridR, GP gridT,
+    const PLTabs<GP>& plt, int g)
+{
+    const double R2 = x * x + y * y, R = sqrt(R2), rsq = R2 + z * z, r = sqrt(rsq);
+    const bool axis = R == 0;
+    const double Rinv = axis ? 0. : rcpD(R);
+    const double cphi = axis ? 1. : x * Rinv, sphi = axis ? 0. : y * Rinv;
+    const double logr = log(r), rplusRinv = rcpD(r + R),
+        tau = axis ? (z > 0 ? 1. : z < 0 ? -1. : 0.) : z * rplusRinv;
+    const int xi = cellIndexR(logr, gridR, P.nr, P.lnrScale),
+        yi = cellIndexT(tau, gridT, P.nt, gmem(P.tauLut), P.nTauLut);
+    const bool inGrid = xi >= 0 && yi >= 0;
+    const int xc = inGrid ? xi : 0, yc = inGrid ? yi : 0;
+    const double u = logr - gridR[xc], v = tau - gridT[yc];
+    const int ns = P.ns, zs = P.zeroSlot;
+    const int off0 = (xc * (P.nt - 1) + yc) * ns * AGX_CELL_SLOT_BYTES;
+    double P0 = 0, d0r = 0, d0t = 0, Am = 0, Ar = 0, At = 0, Ap = 0;
+    if(g < ns) {
+        TrigRecT<double> tr;
+        tr.alpha = cphi >= 0 ? sphi * sphi * rcpD(1 + cphi) : 1 - cphi;
+        tr.beta = sphi;
+        for(int s = g; s < ns; s += G) {
+            double Pm, dmr, dmt;
+            fetch.slot(off0, s, u, v, Pm, dmr, dmt);
+            if(s == zs) {
+                if(P.logScaling) {
+                    const double expX = exp(Pm), val = rcpD(P.invPhi0 - expX), dPhidX = val * val * expX;
+                    Pm = val;
+                    dmr *= dPhidX;
+                    dmt *= dPhidX;
+                }
+                P0 = Pm; d0r = dmr; d0t = dmt;
+            } else {
+                const int m = P.slotM[s];
+                tr.restart();
+                tr.advanceTo(m > 0 ? m : -m);
+                const double trig = m > 0 ? tr.c : tr.s, dtrig = double(-m) * (m > 0 ? tr.s : tr.c);
+                Am += Pm * trig;
+                Ar += dmr * trig;
+                At += dmt * trig;
+                Ap += Pm * dtrig;
+            }
+        }
+    }
+    P0 = groupSum<G>(P0); d0r = groupSum<G>(d0r); d0t = groupSum<G>(d0t);
+    Am = groupSum<G>(Am); Ar = groupSum<G>(Ar); At = groupSum<G>(At); Ap = groupSum<G>(Ap);
+    double trPot, trDr, trDth, trDphi;
+    if(P.logScaling) {   // the other harmonics are stored as ratios to the m = 0 term
+        trPot = P0 + P0 * Am;
+        trDr = d0r + (P0 * Ar + d0r * Am);
+        trDth = d0t + (P0 * At + d0t * Am);
+        trDphi = P0 * Ap;
+    } else {
+        trPot = P0 + Am; trDr = d0r + Ar; trDth = d0t + At; trDphi = Ap;
+    }
+    const double rinv = rcpD(r), r2inv = rinv * rinv;
+    const double drdR = R * r2inv, drdz = z * r2inv, dthdR = -tau * rinv, dthdz = rinv - rplusRinv;
+    double gR = inGrid ? trDr * drdR + trDth * dthdR : NAN;
+    double gzc = inGrid ? trDr * drdz + trDth * dthdz : NAN;
+    double gphi = inGrid ? trDphi : NAN;
+    pot = inGrid ? trPot : NAN;
+    if(rsq < P.rsqInner)
+        powerLawEvalFast(P, P.inner, plt, R, z, r, cphi, sphi, pot, gR, gzc, gphi);
+    else if(rsq > P.rsqOuter)
+        powerLawEvalFast(P, P.outer, plt, R, z, r, cphi, sphi, pot, gR, gzc, gphi);
+    const double dRdx = axis ? 1. : cphi, dRdy = axis ? 1. : sphi,
+        dphidx = axis ? 1. : -sphi * Rinv, dphidy = axis ? 1. : cphi * Rinv;
+    gx = gR * dRdx + gphi * dphidx;
+    gy = gR * dRdy + gphi * dphidy;
+    gz = gzc;
+}
+
 // BasePotentialCyl::evalCar: Car -> Cyl, evaluate, gradient back to Car (+dPhi/dx)
 // NFW: whether the optional analytic halo is handled (kernels for potentials without one are
 // instantiated with false: keeping x, y, z live to the end costs a wavefront of occupancy)
@@ -1121,10 +1293,11 @@ __device__ __forceinline__ void multipoleCar(const AgxMultipoleDev& P, const F& 
     double x, double y, double z, double& pot, double& gx, double& gy, double& gz,
     GP gridR = nullptr, GP gridT = nullptr)
 {
-    if constexpr(IsCellFetch<F>::value) {
+    // (kernels with grids in LDS call multipoleCarCell themselves, with the power-law arrays in LDS)
+    if constexpr(IsCellFetch<F>::value && std::is_same<GP, const double AGX_GLOBAL*>::value) {
         if(!P.interp1d) {
             multipoleCarCell<F, GP, NFW>(P, fetch, x, y, z, pot, gx, gy, gz, gridR ? gridR : GP(gmem(P.gridR)),
-                gridT ? gridT : GP(gmem(P.gridT)));
+                gridT ? gridT : GP(gmem(P.gridT)), plTabsGlobal(P));
             return;
         }
     }

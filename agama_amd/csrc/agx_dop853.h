@@ -308,6 +308,7 @@ __device__ __forceinline__ void finishOrbit(const AgxOrbitArgs& A, Lane& L, int 
     if(A.outNevals) A.outNevals[oi] = 3 + Lane::EVALS_PER_ATTEMPT * (long)L.natt;   // initialisation + attempts
     if(A.outMaxde) A.outMaxde[oi] = L.f(F_MAXDE);
     if(A.outStatus) A.outStatus[oi] = status;
+    if(A.outTend) A.outTend[oi] = L.tb;
     L.phase = PH_IDLE;
 }
 
@@ -338,7 +339,7 @@ __device__ __forceinline__ void fetchOrbit(const AgxOrbitArgs& A, Lane& L, bool 
     L.tb = A.tstart ? A.tstart[oi] : 0.0;
     const double T = A.ttotal[oi];
     L.tend = L.tb + T;
-    L.S = A.trajsize[oi];
+    L.S = A.trajsize ? A.trajsize[oi] : 1;
     L.f(F_DTS) = L.S > 1 ? fabs(T) / (L.S - 1) : INFINITY;
     L.f(F_T0) = NAN;
     L.f(F_MAXDE) = 0;

@@ -7,7 +7,8 @@ struct AgxOrbitArgs {
     const double* ic;        // n x 6 (x, y, z, vx, vy, vz), rotating-frame input as in the reference
     const double* tstart;    // n or null (0)
     const double* ttotal;    // n
-    const int* trajsize;     // n: >= 1 rows at regular intervals; -c: every step, c rows reserved
+    const int* trajsize;     // n: >= 1 rows at regular intervals; -c: every step, c rows reserved;
+                             // null: 1 (no rows are written without outTime/outTraj)
     const long* trajOffset;  // n: first output row of each orbit
     const int* order;        // n or null: processing order (a permutation of 0..n-1)
     double accuracy, Omega;
@@ -38,14 +39,16 @@ struct AgxOrbitArgs {
     long groupOrbits = 0;
     int groupStride = 0;
     unsigned int* groupCounter = nullptr;
+    // the time each orbit reached (n, indexed by orbit), or null: the work-order probe's output
+    double* outTend = nullptr;
 };
 
 enum AgxPrecision { AGX_FP64 = 0, AGX_FP32_TABLES = 1 };
 
-// how FP64 node records are fetched: per-lane 8-byte loads, per-lane 16-byte loads, or the
-// wavefront-cooperative gather through LDS (agx_device.h)
-enum AgxFetch { AGX_FETCH_SCALAR = 0, AGX_FETCH_VEC2 = 1, AGX_FETCH_COOP = 2, AGX_FETCH_CELL = 3,
-    AGX_FETCH_CELL_COOP = 4 };
+// how the FP64 kernels read the 2-D splines: node records with per-lane 8-byte loads, per-lane
+// 16-byte loads or the wavefront-cooperative gather through LDS, or the per-cell polynomial
+// tables (agx_device.h)
+enum AgxFetch { AGX_FETCH_SCALAR = 0, AGX_FETCH_VEC2 = 1, AGX_FETCH_COOP = 2, AGX_FETCH_CELL = 3 };
 
 // one recorded series - an orbit's trajectory or one of its deviation vectors - behind the
 // quintic-spline interpolators of Orbit objects (agx_orbit_interp.hip)

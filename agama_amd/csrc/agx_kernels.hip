@@ -39,8 +39,7 @@ template<int MODE, bool NFW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AGX_EVAL_WAVES))) void evalKernel(const AgxMultipoleDev* __restrict__ Pp,
     long n, const double* __restrict__ pos, double* __restrict__ phi, double* __restrict__ grad)
 {
-    __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? 4 * FetchCoop::LDS_DOUBLE2_PER_WAVE :
-                         MODE == AGX_FETCH_CELL_COOP ? 4 * FetchCellCoop::LDS_VD2_PER_WAVE : 1];
+    __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? 4 * FetchCoop::LDS_DOUBLE2_PER_WAVE : 1];
     const AgxMultipoleDev& P = *Pp;
     // short-lived blocks: the grids are read from global memory (staging them costs more)
     const double AGX_GLOBAL* gR = gmem(P.gridR);
@@ -58,10 +57,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AGX_EVAL_WA
     } else if constexpr(MODE == AGX_FETCH_CELL) {
         FetchCell f(P.cells, P.cellBytes);
         multipoleCar<decltype(f), const double AGX_GLOBAL*, NFW>(P, f, x, y, z, p, gx, gy, gz, gR, gT);
-    } else if constexpr(MODE == AGX_FETCH_CELL_COOP) {
-        FetchCellCoop f(P.cells, P.cellBytes, (vd2 AGX_LDS*)(stage + (threadIdx.x >> 6) * FetchCellCoop::LDS_VD2_PER_WAVE),
-            (int)(threadIdx.x & 63));
-        multipoleCar<decltype(f), const double AGX_GLOBAL*, NFW>(P, f, x, y, z, p, gx, gy, gz, gR, gT);
+
     } else if constexpr(MODE == AGX_FETCH_VEC2) {
         FetchVec2 f{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
         multipoleCar<decltype(f), const double AGX_GLOBAL*, NFW>(P, f, x, y, z, p, gx, gy, gz, gR, gT);
@@ -94,12 +90,14 @@ constexpr int OBS = 256;
 
 // the ln r and tau grids of the orbit kernel, staged in LDS (at most this many values together)
 constexpr int AGX_LDS_GRID_ORBIT = 512;
+// LDS for the per-cell path's power-law arrays (9 (legLmax+1)^2 + legLmax + 1 doubles: legLmax <= 17)
+constexpr int AGX_LDS_PL_ORBIT = 2944;
 typedef const double AGX_LDS* LdsGrid;
 
 // one right-hand-side potential gradient in the inertial frame (orbit.cpp:182-208)
 template<int MODE, bool NFW>
 __device__ __forceinline__ void rhsEval(const AgxMultipoleDev& P, vd2* stage, double x, double y,
-    double z, double& pot, double& gx, double& gy, double& gz, LdsGrid gR, LdsGrid gT)
+    double z, double& pot, double& gx, double& gy, double& gz, LdsGrid gR, LdsGrid gT, const PLTabs<LdsGrid>& plt)
 {
     if constexpr(MODE == AGX_FETCH_COOP) {
         FetchCoop f{ gmem(reinterpret_cast<const vd2*>(P.nodes)),
@@ -108,11 +106,9 @@ __device__ __forceinline__ void rhsEval(const AgxMultipoleDev& P, vd2* stage, do
         multipoleCar<decltype(f), LdsGrid, NFW, AGX_ORBIT_PREFETCH != 0>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     } else if constexpr(MODE == AGX_FETCH_CELL) {
         FetchCell f(P.cells, P.cellBytes);
-        multipoleCar<decltype(f), LdsGrid, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
-    } else if constexpr(MODE == AGX_FETCH_CELL_COOP) {
-        FetchCellCoop f(P.cells, P.cellBytes, (vd2 AGX_LDS*)(stage + (threadIdx.x >> 6) * FetchCellCoop::LDS_VD2_PER_WAVE),
-            (int)(threadIdx.x & 63));
-        multipoleCar<decltype(f), LdsGrid, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
+        if(P.interp1d) multipoleCar<decltype(f), LdsGrid, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
+        else multipoleCarCell<decltype(f), LdsGrid, NFW>(P, f, x, y, z, pot, gx, gy, gz, gR, gT, plt);
+
     } else if constexpr(MODE == AGX_FETCH_VEC2) {
         FetchVec2 f{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
         multipoleCar<decltype(f), LdsGrid, NFW, AGX_ORBIT_PREFETCH != 0>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
@@ -141,15 +137,19 @@ void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, const AgxOrbitArgs* __r
     // at one wavefront per SIMD the block has the CU's LDS to itself: the stage vectors k1..k5
     // and x live there too (the cooperative fetcher needs its own LDS staging area instead)
     typedef typename std::conditional<TGT, LaneTgtT<OBS>,
-        LaneT<OBS, AGX_K_IN_LDS && WAVES == 1 && MODE != AGX_FETCH_COOP && MODE != AGX_FETCH_CELL_COOP> >::type Lane;
+        LaneT<OBS, AGX_K_IN_LDS && WAVES == 1 && MODE != AGX_FETCH_COOP> >::type Lane;
     __shared__ double lds[Lane::LDS_FIELDS * OBS];
-    __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? (OBS / 64) * FetchCoop::LDS_DOUBLE2_PER_WAVE :
-                         MODE == AGX_FETCH_CELL_COOP ? (OBS / 64) * FetchCellCoop::LDS_VD2_PER_WAVE : 1];
+    __shared__ vd2 stage[MODE == AGX_FETCH_COOP ? (OBS / 64) * FetchCoop::LDS_DOUBLE2_PER_WAVE : 1];
     __shared__ double gridLds[AGX_LDS_GRID_ORBIT];
+    // the per-cell path's power-law extrapolation arrays (the launcher checks that they fit)
+    constexpr bool CELLS = MODE == AGX_FETCH_CELL;
+    __shared__ double plLds[CELLS ? AGX_LDS_PL_ORBIT : 1];
     const AgxMultipoleDev& P = *Pp;
     // the grids in LDS (the launcher refuses larger ones): binary searches read LDS directly
     for(int i = threadIdx.x; i < P.nr + P.nt; i += blockDim.x)
         gridLds[i] = i < P.nr ? P.gridR[i] : P.gridT[i - P.nr];
+    PLTabs<LdsGrid> plt{};
+    if constexpr(CELLS) { if(!P.interp1d) plt = plStage(P, (double AGX_LDS*)plLds); }
     __syncthreads();
     const LdsGrid gR = (LdsGrid)gridLds, gT = gR + P.nr;
     Lane L;
@@ -175,15 +175,21 @@ void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, const AgxOrbitArgs* __r
         double pot, gx, gy, gz;
         if constexpr(GRP) {
             if(group) {
-                FetchVec2 fg{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
-                multipoleCarGroup<AGX_GROUP>(P, fg, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa, L.y[2],
-                    pot, gx, gy, gz, gR, gT, (int)(threadIdx.x & (AGX_GROUP - 1)));
+                if constexpr(MODE == AGX_FETCH_CELL) {
+                    FetchCell fg(P.cells, P.cellBytes);
+                    multipoleCarCellGroup<AGX_GROUP>(P, fg, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa,
+                        L.y[2], pot, gx, gy, gz, gR, gT, plt, (int)(threadIdx.x & (AGX_GROUP - 1)));
+                } else {
+                    FetchVec2 fg{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
+                    multipoleCarGroup<AGX_GROUP>(P, fg, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa,
+                        L.y[2], pot, gx, gy, gz, gR, gT, (int)(threadIdx.x & (AGX_GROUP - 1)));
+                }
             } else
                 rhsEval<MODE, NFW>(P, stage, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa, L.y[2],
-                    pot, gx, gy, gz, gR, gT);
+                    pot, gx, gy, gz, gR, gT, plt);
         } else
             rhsEval<MODE, NFW>(P, stage, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa, L.y[2],
-                pot, gx, gy, gz, gR, gT);
+                pot, gx, gy, gz, gR, gT, plt);
         if(!run) continue;
         double f[6];
         f[0] = L.y[3]; f[1] = L.y[4]; f[2] = L.y[5];
@@ -206,17 +212,17 @@ hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, in
     if(n <= 0) return hipSuccess;
     const int block = 256;
     const dim3 grid((unsigned)((n + block - 1) / block));
-    if(P.nfwMass != 0) {   // composite with an NFW halo: 16-byte per-lane fetch (or FP32 tables)
+    if(P.nfwMass != 0) {   // composite with an NFW halo: per-cell tables, 16-byte per-lane fetch or FP32 tables
         if(precision == AGX_FP32_TABLES)
             hipLaunchKernelGGL((evalKernel<-1, true>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
+        else if(fetch == AGX_FETCH_CELL && !P.interp1d)
+            hipLaunchKernelGGL((evalKernel<AGX_FETCH_CELL, true>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
         else
             hipLaunchKernelGGL((evalKernel<AGX_FETCH_VEC2, true>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     } else if(precision == AGX_FP32_TABLES)
         hipLaunchKernelGGL((evalKernel<-1, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     else if(fetch == AGX_FETCH_CELL && !P.interp1d)
         hipLaunchKernelGGL((evalKernel<AGX_FETCH_CELL, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
-    else if(fetch == AGX_FETCH_CELL_COOP && !P.interp1d)
-        hipLaunchKernelGGL((evalKernel<AGX_FETCH_CELL_COOP, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     else if(fetch == AGX_FETCH_COOP)
         hipLaunchKernelGGL((evalKernel<AGX_FETCH_COOP, false>), grid, dim3(block), 0, stream, dP, n, pos, phi, grad);
     else if(fetch == AGX_FETCH_VEC2)
@@ -232,16 +238,17 @@ namespace {
 typedef void (*OrbitKernelFn)(const AgxMultipoleDev*, const AgxOrbitArgs*);
 OrbitKernelFn orbitKernelFor(int precision, int fetch, int waves, bool nfw, bool targets = false, bool group = false)
 {
-    if(group)     // group mode: FP64, 16-byte per-lane fetch, 1 wave/SIMD, no NFW, no targets
-        return orbitKernel<AGX_FETCH_VEC2, 1, false, false, true>;
+    if(group)     // group mode: FP64 (node records or per-cell tables), 1 wave/SIMD, no NFW, no targets
+        return fetch == AGX_FETCH_CELL ? orbitKernel<AGX_FETCH_CELL, 1, false, false, true>
+                                       : orbitKernel<AGX_FETCH_VEC2, 1, false, false, true>;
     if(targets)   // target plug-ins: FP64, 16-byte per-lane fetch, 1 wave/SIMD
         return nfw ? orbitKernel<AGX_FETCH_VEC2, 1, true, true> : orbitKernel<AGX_FETCH_VEC2, 1, false, true>;
-    if(nfw)   // composite with an NFW halo: 16-byte per-lane fetch (or FP32 tables), 1 wave/SIMD
-        return precision == AGX_FP32_TABLES ? orbitKernel<-1, 1, true> : orbitKernel<AGX_FETCH_VEC2, 1, true>;
+    if(nfw)   // composite with an NFW halo: per-cell tables or 16-byte per-lane fetch (or FP32 tables), 1 wave/SIMD
+        return precision == AGX_FP32_TABLES ? orbitKernel<-1, 1, true> :
+               fetch == AGX_FETCH_CELL ? orbitKernel<AGX_FETCH_CELL, 1, true> : orbitKernel<AGX_FETCH_VEC2, 1, true>;
     const bool two = waves == 2;
     if(precision == AGX_FP32_TABLES) return two ? orbitKernel<-1, 2, false> : orbitKernel<-1, 1, false>;
     if(fetch == AGX_FETCH_CELL) return two ? orbitKernel<AGX_FETCH_CELL, 2, false> : orbitKernel<AGX_FETCH_CELL, 1, false>;
-    if(fetch == AGX_FETCH_CELL_COOP) return orbitKernel<AGX_FETCH_CELL_COOP, 1, false>;
     if(fetch == AGX_FETCH_COOP) return two ? orbitKernel<AGX_FETCH_COOP, 2, false> : orbitKernel<AGX_FETCH_COOP, 1, false>;
     if(fetch == AGX_FETCH_VEC2) return two ? orbitKernel<AGX_FETCH_VEC2, 2, false> : orbitKernel<AGX_FETCH_VEC2, 1, false>;
     return two ? orbitKernel<AGX_FETCH_SCALAR, 2, false> : orbitKernel<AGX_FETCH_SCALAR, 1, false>;
@@ -274,10 +281,14 @@ hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, i
     if(blockSize != OBS) return hipErrorInvalidValue;
     if(A.targets && precision == AGX_FP32_TABLES) return hipErrorInvalidValue;
     const bool group = A.groupStride > 0;
-    if(group && (precision != AGX_FP64 || fetch != AGX_FETCH_VEC2 || waves != 1 || P.nfwMass != 0 || A.targets ||
+    if(group && (precision != AGX_FP64 || (fetch != AGX_FETCH_VEC2 && fetch != AGX_FETCH_CELL) || waves != 1 ||
+                 P.nfwMass != 0 || P.interp1d || A.targets ||
                  A.poolCounter || !A.groupCounter || A.groupOrbits > A.n))
         return hipErrorInvalidValue;
     if(!dArgs || P.nr + P.nt > AGX_LDS_GRID_ORBIT) return hipErrorInvalidValue;
+    if(fetch == AGX_FETCH_CELL && !P.interp1d &&
+       9 * (P.legLmax + 1) * (P.legLmax + 1) + P.legLmax + 1 > AGX_LDS_PL_ORBIT)
+        fetch = AGX_FETCH_VEC2;   // power-law arrays too large for the kernel's LDS copy
     hipLaunchKernelGGL(storeArgsKernel, dim3(1), dim3(64), 0, stream, A, dArgs);
     hipError_t e = hipGetLastError();
     if(e != hipSuccess) return e;

@@ -64,6 +64,14 @@ struct AgxMultipoleDev {
     double lnrScale;
     int nTauLut;
     const int* tauLut;
+    // Legendre factors of the FP64 kernels' power-law extrapolation, for 0 <= m <= l <= legLmax:
+    // legTab[(m*(legLmax+1) + l)*3 + {0,1,2}] = the normalisation of P_l^m times the synthesis
+    // factor (2 sqrt(pi), times sqrt(2) for m != 0), (2l-1)/(l-m) and (l+m-1)/(l-m) of the
+    // recurrence in l; legStart[m] = the coefficient of sin^m(theta) in P_m^m before
+    // normalisation, (-1)^m (2m-1)!!
+    int legLmax;
+    const double* legTab;
+    const double* legStart;
     AgxPowerLawDev inner, outer;
     // lmax <= 2: the 1-D variant replaces the 2-D tables (nt = ns = 0); logScaling and invPhi0
     // then refer to the l=0 coefficient Phi_00 (MultipoleInterp1d's convention)

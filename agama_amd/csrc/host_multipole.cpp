@@ -1097,6 +1097,30 @@ std::vector<double> buildCellPolys(const HostTables& T)
     return cells;
 }
 
+std::vector<double> buildLegendreTable(int lmax)
+{
+    const int L1 = lmax + 1;
+    std::vector<double> t((size_t)L1 * L1 * 3 + L1, 0.);
+    const long double pi = 3.14159265358979323846264338327950288L;
+    for(int m = 0; m <= lmax; m++) {
+        // sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!), by its closed form at l = m and the ratio in l
+        long double fact2m = 1, dfact = 1;
+        for(int i = 2; i <= 2 * m; i++) fact2m *= i;
+        for(int i = 2 * m - 1; i > 1; i -= 2) dfact *= i;
+        long double pf = 0.5L / sqrtl(pi) * sqrtl((2 * m + 1) / fact2m);
+        const long double mul = m == 0 ? 2 * sqrtl(pi) : 2 * sqrtl(pi) * sqrtl(2.0L);
+        for(int l = m; l <= lmax; l++) {
+            if(l > m) pf *= sqrtl((2 * l + 1.0L) / (2 * l - 1.0L) * (l - m) / (l + m));
+            double* e = &t[((size_t)m * L1 + l) * 3];
+            e[0] = (double)(pf * mul);
+            e[1] = l > m ? (double)((2 * l - 1.0L) / (l - m)) : 0.;
+            e[2] = l > m ? (double)((l + m - 1.0L) / (l - m)) : 0.;
+        }
+        t[(size_t)L1 * L1 * 3 + m] = (double)(dfact * (m % 2 ? -1 : 1));
+    }
+    return t;
+}
+
 std::vector<int> buildTauLookup(const HostTables& T)
 {
     if(T.interp1d || T.nt < 2) return {};

@@ -118,6 +118,9 @@ std::vector<double> buildCellPolys(const HostTables& T);
 // of uniform buckets of [-1, 1], each narrower than the narrowest tau cell (at most 4096), and
 // for each the index of the cell holding its left edge
 std::vector<int> buildTauLookup(const HostTables& T);
+// Legendre factors of the FP64 kernels' power-law extrapolation (agx_tables.h: legTab, then
+// legStart appended), for degrees up to lmax
+std::vector<double> buildLegendreTable(int lmax);
 
 // the reference's sin/cos (math::sincos, src/math_core.cpp:171-206: multiples of pi/2 exact)
 void sincosAgama(double x, double& s, double& c);

@@ -53,7 +53,7 @@ FLOPS_PER_ATTEMPT_EXTRA_BY_METHOD = {"dop853": FLOPS_PER_ATTEMPT_EXTRA, "dprkn8"
 METHOD_FLAG = {"dop853": 0, "dprkn8": 0x200}
 FLOPS_PER_ORBIT_INIT = 3 * FLOPS_PER_EVAL + 80
 # record fetch modes of the Multipole evaluation (agx_set_fetch)
-FETCH_MODES = {"scalar": 0, "vec2": 1, "coop": 2, "cell": 3, "cellcoop": 4}
+FETCH_MODES = {"scalar": 0, "vec2": 1, "coop": 2, "cell": 3}
 # launches timed for a leg that is reported beside the headline figure (not the headline)
 COMPARISON_STEPS = 2
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix), AMD published figure

@@ -309,9 +309,13 @@ int agx_orbit_target_traj_batch(const agx_potential* pot, long n, const double* 
    Set by the autotuner (agama_amd.tune). */
 int agx_set_orbit_launch(int num_blocks, int block_size);
 
-/* How the FP64 spline node records are gathered by the evaluation kernel and by the orbit
-   kernel: 0 = one 8-byte load per value and lane, 1 = 16-byte loads per lane, 2 = a
-   wavefront-cooperative gather through LDS.  Results are identical; set by the autotuner. */
+/* How the FP64 kernels read the Multipole's 2-D splines, for the evaluation kernel and for the
+   orbit kernel: from the node records, 0 = one 8-byte load per value and lane, 1 = 16-byte
+   loads per lane, 2 = a wavefront-cooperative gather through LDS; 3 = from the per-cell
+   polynomial form of the same splines.  Modes 0-2 give identical results; mode 3 agrees with
+   them to rounding (it also takes the azimuth's cos/sin from x/R, y/R, uses reciprocals instead
+   of divisions and a tabulated Legendre recurrence in the power-law regions).  Default: 3, 3
+   (the autotuner, agama_amd.tune, may choose another). */
 int agx_set_fetch(int eval_mode, int orbit_mode);
 
 /* ---- NEMO acceleration plugin (src/interface_nemo.cpp:47-60, 114-220) ----
@@ -349,14 +353,24 @@ double agama_evalPotential(const agama_Potential* pot, const double pos[3], doub
    current settings of agx_set_orbit_launch / agx_set_fetch / agx_set_orbit_waves. */
 int agx_set_orbit_waves(int waves);
 /* Group mode of the DOP853 kernel (no counterpart in the reference: a scheduling knob of this
-   library).  In batches of at least min_batch orbits (FP64, 16-byte fetch, 1 wave/SIMD, no NFW
+   library).  In batches of at least min_batch orbits (FP64, node records or per-cell tables (orbit
+   fetch mode 1 or 3), 1 wave/SIMD, no NFW
    halo, no targets, deviation vectors or every-step recording), one wavefront in `stride`
    integrates the first `orbits` orbits of the processing order - the longest expected - with 8
    lanes per orbit, each lane evaluating a share of the azimuthal harmonics; orbits = -1: one per
    8 lanes of those wavefronts; stride = 0: off; stride = -1: automatic (every orbit in groups when
-   they fit, 4 for up to 2 orbits per lane, 8 for up to 4, off above).  Results differ from the
+   they fit, 4 for up to 2 orbits per lane, 8 for up to 4, above that 32 after a work-order probe
+   (agx_set_orbit_probe) and off without one).  Results differ from the
    one-lane form only by the order of the additions in the Fourier sum.  Defaults: -1, -1, 65536. */
 int agx_set_orbit_groups(int stride, long orbits, long min_batch);
+/* Work-order probe of the DOP853/DPRKN8 batches (no counterpart in the reference: a scheduling
+   knob of this library).  In batches of at least min_batch orbits (without deviation vectors,
+   targets or every-step recording) every orbit is first integrated for at most `steps` steps
+   with no output but the time it reached, and the orbits are processed in the order of the
+   steps they would take at that rate, longest first (otherwise: |T| / t_dyn at the start);
+   with the probe, group mode's automatic stride above 4 orbits per lane is 32.  Results do not
+   depend on the order except through group mode.  steps = 0: off.  Defaults: 64, 65536. */
+int agx_set_orbit_probe(int steps, long min_batch);
 int agx_get_orbit_config(int* num_blocks, int* fetch_eval, int* fetch_orbit, int* waves);
 
 /* One integration of a fixed batch of device arrays (arguments as agx_orbit_batch_device)

@@ -71,7 +71,7 @@ def test_autotuner_settings_agree_and_restore(pot):
     before = tune.current()
     cfg = tune.tune(pot, n=4096, repeats=1, verbose=True)
     assert tune.current() == before
-    assert len(cfg["candidates"]) == 6
+    assert len(cfg["candidates"]) == 8
     assert cfg["fetch_orbit"] in tune.FETCH and cfg["waves"] in (1, 2)
     print("tuned:", {k: cfg[k] for k in ("fetch_orbit", "waves")},
           "identical:", all(c["identical_to_first"] for c in cfg["candidates"]))

@@ -26,11 +26,18 @@ def setup():
     return agama_amd.Potential(file=COEF), OraclePotential(COEF)
 
 
-@pytest.fixture
-def groups():
+@pytest.fixture(params=[1, 3], ids=["records", "cells"])
+def groups(request):
+    """group mode over the node records (orbit fetch mode 1) and over the per-cell tables (3)"""
+    import ctypes
     from agama_amd._native import lib
+    old = (ctypes.c_int(), ctypes.c_int())
+    lib.agx_get_orbit_config(None, ctypes.byref(old[0]), ctypes.byref(old[1]), None)
+    assert lib.agx_set_fetch(old[0].value, request.param) == 0
+    print(f"orbit fetch mode {request.param}", flush=True)
     yield lib
     assert lib.agx_set_orbit_groups(-1, -1, 65536) == 0   # the library's defaults
+    assert lib.agx_set_fetch(old[0].value, old[1].value) == 0
 
 
 def ics(setup, n, seed, periods=100.0, uneven=False):

@@ -49,10 +49,11 @@ def test_hessian_matches_oracle(case):
           f"max {eh.max():.2e}")
     assert ep.max() <= 1e-12 and eg.max() <= 1e-12
     assert eh.max() <= 1e-11
-    # the gradient of the Hessian path is that of the first-derivative path
+    # the gradient of the Hessian path (node records) is that of the first-derivative path (the
+    # per-cell polynomial form of the same splines), to rounding
     phi1, grad1 = gpu._eval(pts, True, True)
-    assert np.max(np.abs(phi1 / phi - 1)) <= 1e-13
-    assert np.max(np.linalg.norm(grad1 - grad, axis=1) / np.linalg.norm(grad, axis=1)) <= 1e-13
+    assert np.max(np.abs(phi1 / phi - 1)) <= 1e-12
+    assert np.max(np.linalg.norm(grad1 - grad, axis=1) / np.linalg.norm(grad, axis=1)) <= 1e-12
 
 
 def test_python_and_c_surfaces():
