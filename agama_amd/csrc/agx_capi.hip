@@ -2161,13 +2161,13 @@ int agx_surrogate_orbit_batch_device(const agx_surrogate* s, long n, const doubl
             HIPCHECK(hipStreamSynchronize(st));
         }
         if(nf > 0) {   // the fallback orbits, integrated again with the FP64 Multipole
-            A.n = nf;
-            A.order = list;
-            int fb, bs;
-            orbitGeometry(nf, AGX_FP64, pot->nfwMass != 0, fb, bs);
-            HIPCHECK(agxLaunchFillU32(pot->dcounter, 1, 0u, st));
-            e = agxLaunchOrbit(pot->dev, pot->ddev, AGX_FP64, g_fetchOrbit, g_orbitWaves, A, fb, bs, st, pot->dargs);
-            if(e == hipSuccess) e = agxLaunchMarkFallback(list, nf, d_status, st);
+            // the FP64 batch path (work order, probe and group mode as for agx_orbit_batch_device):
+            // the whole batch in its own order, or the fallback list in the order it was filled
+            const int rc = orbitBatchDevice(pot, routeAll ? n : (long)nf, d_ic, d_tstart, d_ttotal, d_trajsize,
+                d_traj_offset, routeAll ? nullptr : list, accuracy, Omega, maxNumSteps, d_time, d_traj, d_final,
+                nullptr, d_nsteps, d_nattempts, d_nevals, d_maxde, d_status, AGX_PRECISION_FP64, st);
+            if(rc != AGX_OK) { const std::string msg = g_err; release(); return fail(rc, "fallback: " + msg); }
+            e = agxLaunchMarkFallback(list, nf, d_status, st);
             if(e != hipSuccess) { release(); return fail(AGX_EDEVICE, std::string("fallback launch: ") + hipGetErrorString(e)); }
         }
         release();

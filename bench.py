@@ -57,9 +57,11 @@ FETCH_MODES = {"scalar": 0, "vec2": 1, "coop": 2, "cell": 3}
 # launches timed for a leg that is reported beside the headline figure (not the headline)
 COMPARISON_STEPS = 2
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix), AMD published figure
-# HBM bytes per launch of the orbit kernel on this workload, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE runs scaled by a 1 GiB copy (profiles/orbitKernel_1M_uneven_triaxial_dehnen_lmax8.md)
-ORBIT_TRAFFIC_BYTES = 566e6 + 289e6
+# HBM bytes per launch of the orbit path on this workload (the probe and the main persistent
+# kernel), from rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy
+# (profiles/orbitKernel_1M_uneven_triaxial_dehnen_lmax8.md): FETCH 465,962 KiB x 2.000,
+# WRITE 354,396 KiB x 1.000
+ORBIT_TRAFFIC_BYTES = 465962 * 1024 * 2.0 + 354396 * 1024
 
 
 def parse():
