@@ -171,11 +171,19 @@ void upload(agx_potential& p)
     D.tauLut = tauLut.empty() ? nullptr : reinterpret_cast<const int*>(base + oLut);
     D.legLmax = legLmax;
     D.legTab = reinterpret_cast<const double*>(base + oLeg);
-    D.legStart = D.legTab + (size_t)(legLmax + 1) * (legLmax + 1) * 3;
+    D.legStart = D.legTab + (size_t)(legLmax + 1) * (legLmax + 1) * AGX_LEG_STRIDE;
     auto fillPL = [&](AgxPowerLawDev& L, const agx::HostPowerLaw& H, size_t s, size_t u, size_t w) {
         L.r0sq = H.r0sq; L.Q = H.Q; L.inner = H.inner; L.lmax = H.lmax; L.mmax = H.mmax;
         L.mmin = H.mmin; L.step = H.step;
         for(int i = 0; i < AGX_MAX_SLOTS; i++) L.lmin[i] = i < (int)H.lmin.size() ? H.lmin[i] : 1 << 20;
+        L.uniformS = 1;   // over the terms of the index set
+        for(int m = H.mmin; m <= H.mmax; m++) {
+            const int lm = H.lmin[m + H.mmax];
+            for(int l = lm; l <= H.lmax; l += H.step) {
+                const int c = l * (l + 1) + m;
+                if(c < (int)H.S.size() && H.S[c] != (H.inner ? l : -l - 1) && H.S[c] != H.S[0]) L.uniformS = 0;
+            }
+        }
         L.S = reinterpret_cast<const double*>(base + s);
         L.U = reinterpret_cast<const double*>(base + u);
         L.W = reinterpret_cast<const double*>(base + w);

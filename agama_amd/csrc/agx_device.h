@@ -967,7 +967,7 @@ __device__ __forceinline__ PLTabs<const double AGX_GLOBAL*> plTabsGlobal(const A
 __device__ __forceinline__ int plStageDoubles(const AgxMultipoleDev& P)
 {
     const int L1 = P.legLmax + 1;
-    return 9 * L1 * L1 + L1;
+    return (6 + AGX_LEG_STRIDE) * L1 * L1 + L1;
 }
 // copy the power-law arrays into `lds` (plStageDoubles(P) doubles) with the whole block; the
 // caller synchronises the block afterwards
@@ -975,7 +975,7 @@ __device__ __forceinline__ PLTabs<const double AGX_LDS*> plStage(const AgxMultip
 {
     const int L1 = P.legLmax + 1, nc = L1 * L1;
     const double* src[8] = { P.inner.S, P.inner.U, P.inner.W, P.outer.S, P.outer.U, P.outer.W, P.legTab, P.legStart };
-    const int len[8] = { nc, nc, nc, nc, nc, nc, 3 * nc, L1 };
+    const int len[8] = { nc, nc, nc, nc, nc, nc, AGX_LEG_STRIDE * nc, L1 };
     PLTabs<const double AGX_LDS*> t;
     const double AGX_LDS* dst[8];
     int off = 0;
@@ -989,6 +989,120 @@ __device__ __forceinline__ PLTabs<const double AGX_LDS*> plStage(const AgxMultip
     return t;
 }
 
+// powerLawEvalFast for expansions with legLmax <= LM, with every loop over (l, m) unrolled: the
+// Legendre recurrence of each |m| is shared by m and -m (the same values), the table and
+// coefficient offsets are constants, the recurrences of different |m| are independent chains,
+// and exp(dlogr * S) is formed once: for extrapolations whose every S equals v or S[0]
+// (AgxPowerLawDev::uniformS; the outer one of a finite-mass model).  The harmonics are added
+// to the Fourier sum in the order 0, +-1, +-2, ... instead of 0, mmin, ..., mmax (rounding only).
+template<int LM, class GP>
+__device__ __forceinline__ void powerLawEvalUnrolled(const AgxMultipoleDev& P, const AgxPowerLawDev& L,
+    const PLTabs<GP>& T, double R, double z, double r, double cphi, double sphi, double& pot, double& gR,
+    double& gz, double& gphi)
+{
+    const double rsq = r * r;
+    const double dlogr = log(rsq / L.r0sq) * 0.5, qq = exp(dlogr), qstep = L.inner ? qq : rcpD(qq);
+    const double rinv = rcpD(r), ct = z * rinv, st = R * rinv, stinv = st > 0 ? rcpD(st) : 0.;
+    const int side = L.inner ? 0 : 1;
+    const GP S = T.S[side], U = T.U[side], W = T.W[side];
+    constexpr int L1 = LM + 1;
+    const int Lmax = L.lmax, mmin = L.mmin, mmax = L.mmax, stepMask = L.step - 1;
+    const int stride = P.legLmax + 1;
+    // (r/r0)^v for every l: v = l inside, -l-1 outside
+    double qv[L1];
+    qv[0] = L.inner ? 1. : qstep;
+#pragma unroll
+    for(int l = 1; l < L1; l++) qv[l] = qv[l - 1] * qstep;
+    const double alpha = cphi >= 0 ? sphi * sphi * rcpD(1 + cphi) : 1 - cphi;
+    double cm = 1, sm = 0;          // cos, sin(am phi) by the recurrence of trigMultiAngle
+    // every term of the index set has S equal to v or to S[0] (the caller checks): one exp()
+    const double S0 = S[0], eS0 = S0 != 0 ? exp(dlogr * S0) : 1;
+    double vr = 0, gr = 0, gth = 0, gph = 0;
+#pragma unroll
+    for(int am = 0; am < L1; am++) {
+        if(am > 0) {
+            const double cn = cm - (alpha * cm + sphi * sm), sn = sm - (alpha * sm - sphi * cm);
+            cm = cn; sm = sn;
+        }
+        if(am > Lmax || (am > mmax && -am < mmin)) continue;
+        const bool usePos = am <= mmax && L.lmin[am + mmax] <= Lmax;
+        const bool useNeg = am > 0 && -am >= mmin && L.lmin[-am + mmax] <= Lmax;
+        if(!usePos && !useNeg) continue;
+        const int lminP = usePos ? L.lmin[am + mmax] : 1 << 20, lminN = useNeg ? L.lmin[-am + mmax] : 1 << 20;
+        const GP tab = T.leg + am * stride * AGX_LEG_STRIDE;
+        double Plm1, dP0;
+        if(am == 0) { Plm1 = 1; dP0 = 0; }
+        else if(am == 1) { Plm1 = -st; dP0 = -ct; }
+        else {
+            double sinm2 = 1, xb = st;
+#pragma unroll
+            for(int k = am - 2; k; k >>= 1) { if(k % 2) sinm2 *= xb; xb *= xb; }
+            const double coef = T.legStart[am];
+            dP0 = am * coef * sinm2 * st * ct;
+            Plm1 = coef * sinm2 * (st * st);
+        }
+        double Plm = ct * (2 * am + 1) * Plm1, Plm2 = 0;
+        double CvP = 0, CrP = 0, CtP = 0, CvN = 0, CrN = 0, CtN = 0;
+        const bool pole = !(st >= 1e-8 || (am > 2 && st > 0));
+#pragma unroll
+        for(int l = am; l < L1; l++) {
+            if(l > Lmax) break;
+            const GP e = tab + l * AGX_LEG_STRIDE;
+            const double nrm = e[0];
+            double Pl, dPl;
+            if(l == am) { Pl = Plm1 * nrm; dPl = dP0 * nrm; }
+            else {
+                if(l > am + 1) Plm = ct * e[1] * Plm1 - e[2] * Plm2;
+                Pl = Plm * nrm;
+                double d = (l * ct * Plm - (l + am) * Plm1) * stinv;
+                if(pole) {   // the reference's limits at the poles
+                    if(am == 0) d = -l * (l + 1) / 2 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
+                    else if(am == 1) d = -l * (l + 1) / 2 * (ct > 0 || l % 2 == 0 ? 1 : -1);
+                    else if(am == 2) d = l * (l + 1) * (l + 2) * (l - 1) / 4 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
+                    else d = 0;
+                }
+                dPl = nrm * d;
+                Plm2 = Plm1;
+                Plm1 = Plm;
+            }
+            const double v = L.inner ? l : -l - 1, rv = qv[l];
+            auto term = [&](int m, double& Cv, double& Cr, double& Ct) {
+                const int c = l * (l + 1) + m;
+                const double sc = S[c], u = U[c], w = W[c];
+                const double rs = sc != v ? eS0 : rv;
+                const double urs = u * rs * (sc != v || u == 0 ? 1 : dlogr);
+                const double wrv = w * rv;
+                const double qr2 = v == 0 ? L.Q * rsq / L.r0sq : 0;
+                const double Pv = urs + wrv + qr2, dPv = urs * sc + wrv * v + (sc != v ? 0 : u * rs) + qr2 * 2;
+                Cv += Pl * Pv;
+                Cr += Pl * dPv;
+                Ct += dPl * Pv;
+            };
+            if(l >= lminP && ((l - lminP) & stepMask) == 0) term(am, CvP, CrP, CtP);
+            if(l >= lminN && ((l - lminN) & stepMask) == 0) term(-am, CvN, CrN, CtN);
+        }
+        if(am == 0) { vr = CvP; gr = CrP; gth = CtP; continue; }
+        if(useNeg) {   // sine terms: sin(am phi), d/dphi = am cos(am phi)
+            vr  += CvN * sm;
+            gr  += CrN * sm;
+            gth += CtN * sm;
+            gph += CvN * (am * cm);
+        }
+        if(usePos) {
+            vr  += CvP * cm;
+            gr  += CrP * cm;
+            gth += CtP * cm;
+            gph += CvP * (-am * sm);
+        }
+    }
+    pot = vr;
+    // transformDerivsSphToCyl
+    const double r2inv = rinv * rinv, drdR = R * r2inv, drdz = z * r2inv;
+    gR = gr * drdR + gth * drdz;
+    gz = gr * drdz + gth * -drdR;
+    gphi = gph;
+}
+
 template<class GP>
 __device__ __forceinline__ void powerLawEvalFast(const AgxMultipoleDev& P, const AgxPowerLawDev& L,
     const PLTabs<GP>& T, double R, double z, double r, double cphi, double sphi, double& pot, double& gR,
@@ -999,6 +1113,10 @@ __device__ __forceinline__ void powerLawEvalFast(const AgxMultipoleDev& P, const
     if(mono || (L.lmax == 2 && L.mmin == 0 && L.step == 2)) {   // the cheap branches as they are
         const double phi = R == 0 ? 0. : atan2A(R * sphi, R * cphi);
         powerLawEval(L, R, z, phi, pot, gR, gz, gphi);
+        return;
+    }
+    if(P.legLmax <= 8 && L.uniformS) {
+        powerLawEvalUnrolled<8>(P, L, T, R, z, r, cphi, sphi, pot, gR, gz, gphi);
         return;
     }
     const double dlogr = log(rsq / L.r0sq) * 0.5, q = exp(dlogr), qstep = L.inner ? q : rcpD(q);
@@ -1018,7 +1136,7 @@ __device__ __forceinline__ void powerLawEvalFast(const AgxMultipoleDev& P, const
         const int lmin = L.lmin[m + L.mmax];
         if(lmin > L.lmax) continue;
         const int am = m < 0 ? -m : m;
-        const GP tab = T.leg + am * L1 * 3;
+        const GP tab = T.leg + am * L1 * AGX_LEG_STRIDE;
         // P_am^am and its theta derivative before normalisation (legendrePmm)
         double Plm1, dP0;
         if(am == 0) { Plm1 = 1; dP0 = 0; }
@@ -1031,29 +1149,37 @@ __device__ __forceinline__ void powerLawEvalFast(const AgxMultipoleDev& P, const
             Plm1 = coef * sinm2 * (st * st);
         }
         double Plm = ct * (2 * am + 1) * Plm1, Plm2 = 0;
+        const double dam = am;
         // (r/r0)^v at l = am: v = l inside, -l-1 outside
         double qv = L.inner ? 1. : qstep;
         for(int k = 0; k < am; k++) qv *= qstep;
         double Cv = 0, Cr = 0, Ct = 0;
+        // the table entries and coefficients of every l are read unconditionally (all indices
+        // are in range), so that the loads of the next l can be issued early
+#pragma unroll 2
         for(int l = am; l <= L.lmax; l++, qv *= qstep) {
-            const double nrm = tab[l * 3];
+            const GP e = tab + l * AGX_LEG_STRIDE;
+            const double nrm = e[0], A = e[1], B = e[2], dl = e[3];
+            const int c = l * (l + 1) + m;
+            const double sc = S[c], u = U[c], w = W[c];
             double Pl, dPl;
             if(l == am) { Pl = Plm1 * nrm; dPl = dP0 * nrm; }
             else {
-                if(l > am + 1) Plm = ct * tab[l * 3 + 1] * Plm1 - tab[l * 3 + 2] * Plm2;
+                if(l > am + 1) Plm = ct * A * Plm1 - B * Plm2;
                 Pl = Plm * nrm;
-                double d = 0;
-                if(st >= 1e-8 || (am > 2 && st > 0)) d = (l * ct * Plm - (l + am) * Plm1) * stinv;
-                else if(am == 0) d = -l * (l + 1) / 2 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
-                else if(am == 1) d = -l * (l + 1) / 2 * (ct > 0 || l % 2 == 0 ? 1 : -1);
-                else if(am == 2) d = l * (l + 1) * (l + 2) * (l - 1) / 4 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
+                double d = (dl * ct * Plm - (dl + dam) * Plm1) * stinv;
+                if(!(st >= 1e-8 || (am > 2 && st > 0))) {   // at the poles (the reference's limits)
+                    if(am == 0) d = -l * (l + 1) / 2 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
+                    else if(am == 1) d = -l * (l + 1) / 2 * (ct > 0 || l % 2 == 0 ? 1 : -1);
+                    else if(am == 2) d = l * (l + 1) * (l + 2) * (l - 1) / 4 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
+                    else d = 0;
+                }
                 dPl = nrm * d;
                 Plm2 = Plm1;
                 Plm1 = Plm;
             }
-            if(l >= lmin && (l - lmin) % L.step == 0) {
-                const int c = l * (l + 1) + m;
-                const double sc = S[c], u = U[c], w = W[c], v = L.inner ? l : -l - 1;
+            if(l >= lmin && ((l - lmin) & (L.step - 1)) == 0) {   // step is 1 or 2
+                const double v = L.inner ? dl : -dl - 1;
                 const double rv = qv;
                 const double rs = sc != v ? (sc != 0 ? exp(dlogr * sc) : 1) : rv;
                 const double urs = u * rs * (sc != v || u == 0 ? 1 : dlogr);

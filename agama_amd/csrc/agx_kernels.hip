@@ -90,7 +90,7 @@ constexpr int OBS = 256;
 
 // the ln r and tau grids of the orbit kernel, staged in LDS (at most this many values together)
 constexpr int AGX_LDS_GRID_ORBIT = 512;
-// LDS for the per-cell path's power-law arrays (9 (legLmax+1)^2 + legLmax + 1 doubles: legLmax <= 17)
+// LDS for the per-cell path's power-law arrays (10 (legLmax+1)^2 + legLmax + 1 doubles: legLmax <= 16)
 constexpr int AGX_LDS_PL_ORBIT = 2944;
 typedef const double AGX_LDS* LdsGrid;
 
@@ -287,7 +287,7 @@ hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, i
         return hipErrorInvalidValue;
     if(!dArgs || P.nr + P.nt > AGX_LDS_GRID_ORBIT) return hipErrorInvalidValue;
     if(fetch == AGX_FETCH_CELL && !P.interp1d &&
-       9 * (P.legLmax + 1) * (P.legLmax + 1) + P.legLmax + 1 > AGX_LDS_PL_ORBIT)
+       (6 + AGX_LEG_STRIDE) * (P.legLmax + 1) * (P.legLmax + 1) + P.legLmax + 1 > AGX_LDS_PL_ORBIT)
         fetch = AGX_FETCH_VEC2;   // power-law arrays too large for the kernel's LDS copy
     hipLaunchKernelGGL(storeArgsKernel, dim3(1), dim3(64), 0, stream, A, dArgs);
     hipError_t e = hipGetLastError();

@@ -22,6 +22,7 @@
 
 #define AGX_NODE_STRIDE 10
 #define AGX_CELL_COEFS 36
+#define AGX_LEG_STRIDE 4
 #define AGX_MAX_M 16
 #define AGX_MAX_SLOTS (2 * AGX_MAX_M + 1)
 
@@ -32,6 +33,7 @@ struct AgxPowerLawDev {
     int lmax;         // of the index set deduced from the non-zero U terms
     int mmax, mmin, step;
     int lmin[AGX_MAX_SLOTS];   // lmin(m) at [m + mmax]; > lmax means the harmonic is empty
+    int uniformS;     // 1: every S of the index set equals its term's v or S[0]
     const double* S;
     const double* U;
     const double* W;
@@ -65,10 +67,10 @@ struct AgxMultipoleDev {
     int nTauLut;
     const int* tauLut;
     // Legendre factors of the FP64 kernels' power-law extrapolation, for 0 <= m <= l <= legLmax:
-    // legTab[(m*(legLmax+1) + l)*3 + {0,1,2}] = the normalisation of P_l^m times the synthesis
-    // factor (2 sqrt(pi), times sqrt(2) for m != 0), (2l-1)/(l-m) and (l+m-1)/(l-m) of the
-    // recurrence in l; legStart[m] = the coefficient of sin^m(theta) in P_m^m before
-    // normalisation, (-1)^m (2m-1)!!
+    // legTab[(m*(legLmax+1) + l)*AGX_LEG_STRIDE + {0,1,2,3}] = the normalisation of P_l^m times
+    // the synthesis factor (2 sqrt(pi), times sqrt(2) for m != 0), (2l-1)/(l-m) and
+    // (l+m-1)/(l-m) of the recurrence in l, and l; legStart[m] = the coefficient of
+    // sin^m(theta) in P_m^m before normalisation, (-1)^m (2m-1)!!
     int legLmax;
     const double* legTab;
     const double* legStart;

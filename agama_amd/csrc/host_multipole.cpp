@@ -1100,7 +1100,7 @@ std::vector<double> buildCellPolys(const HostTables& T)
 std::vector<double> buildLegendreTable(int lmax)
 {
     const int L1 = lmax + 1;
-    std::vector<double> t((size_t)L1 * L1 * 3 + L1, 0.);
+    std::vector<double> t((size_t)L1 * L1 * AGX_LEG_STRIDE + L1, 0.);
     const long double pi = 3.14159265358979323846264338327950288L;
     for(int m = 0; m <= lmax; m++) {
         // sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!), by its closed form at l = m and the ratio in l
@@ -1111,12 +1111,13 @@ std::vector<double> buildLegendreTable(int lmax)
         const long double mul = m == 0 ? 2 * sqrtl(pi) : 2 * sqrtl(pi) * sqrtl(2.0L);
         for(int l = m; l <= lmax; l++) {
             if(l > m) pf *= sqrtl((2 * l + 1.0L) / (2 * l - 1.0L) * (l - m) / (l + m));
-            double* e = &t[((size_t)m * L1 + l) * 3];
+            double* e = &t[((size_t)m * L1 + l) * AGX_LEG_STRIDE];
             e[0] = (double)(pf * mul);
             e[1] = l > m ? (double)((2 * l - 1.0L) / (l - m)) : 0.;
             e[2] = l > m ? (double)((l + m - 1.0L) / (l - m)) : 0.;
+            e[3] = l;
         }
-        t[(size_t)L1 * L1 * 3 + m] = (double)(dfact * (m % 2 ? -1 : 1));
+        t[(size_t)L1 * L1 * AGX_LEG_STRIDE + m] = (double)(dfact * (m % 2 ? -1 : 1));
     }
     return t;
 }
