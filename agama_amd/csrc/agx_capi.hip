@@ -52,6 +52,7 @@ hipError_t agxLaunchScheduleFromProbe(long n, const double* tstart, const double
     int* order, void* work, int** finalOrder, hipStream_t stream);
 hipError_t agxLaunchSchedule(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, long n,
     const double* ic, const double* ttotal, int* order, void* work, hipStream_t stream);
+hipError_t agxLaunchDealGroups(int* order, long groupOrbits, hipStream_t stream);
 
 struct agx_potential {
     agx::Coefs coefs;
@@ -673,7 +674,7 @@ static size_t scheduleWorkspaceBytes(long n)
 
 // group stride of large probed batches: the probe ranks the truly longest orbits first, and one
 // wavefront in this many integrates them with a group of lanes per orbit
-constexpr int PROBED_GROUP_STRIDE = 32;
+constexpr int PROBED_GROUP_STRIDE = 8;
 
 // Re-rank the work order `ord` (agxLaunchSchedule's, inside `work`) from a probe: every orbit is
 // integrated for at most g_probeSteps steps, without output but the time it reached, and the
@@ -731,6 +732,7 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
         hipStream_t st = (hipStream_t)stream;
         void* work = nullptr;
         bool probed = false;
+        int* probedOrder = nullptr;   // the probe's order, inside the workspace (ours to permute)
         if(!d_order && !(flags & AGX_ORDER_AS_GIVEN)) {
             // longest expected integrations first (agx_schedule.hip); stream-ordered workspace,
             // or the caller's (scheduleWork: at least scheduleWorkspaceBytes(n) bytes)
@@ -745,8 +747,8 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
             }
             d_order = ord;
             if(g_probeSteps > 0 && n >= g_probeMinBatch && !variational && !d_targets && !pool.counter)
-                d_order = probeOrder(pot, n, d_ic, d_tstart, d_ttotal, accuracy, Omega, maxNumSteps, precision, ord,
-                    work, st, nullptr);
+                d_order = probedOrder = probeOrder(pot, n, d_ic, d_tstart, d_ttotal, accuracy, Omega, maxNumSteps,
+                    precision, ord, work, st, nullptr);
             probed = d_order != ord;
         }
         AgxOrbitArgs A;
@@ -785,7 +787,10 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
             A.groupStride = A.groupOrbits > 0 ? (int)stride : 0;
             A.groupCounter = pot->dcounter + 1;
             if(A.groupStride == 0) A.groupOrbits = 0;
-            else {   // enough blocks for the group orbits (8 lanes each) and the rest (one lane each)
+            else {
+                // one of the longest orbits per group wavefront (each takes eight at the start)
+                if(probed && stride > 1 && A.groupOrbits == groupWaves * 8 && A.groupOrbits <= 8192)
+                    HIPCHECK(agxLaunchDealGroups(probedOrder, A.groupOrbits, st));   // enough blocks for the group orbits (8 lanes each) and the rest (one lane each)
                 int dev = 0, cus = 0;
                 HIPCHECK(hipGetDevice(&dev));
                 HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));

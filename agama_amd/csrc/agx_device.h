@@ -592,12 +592,32 @@ __device__ __forceinline__ void interiorEvalCell(const AgxMultipoleDev& P, const
 // the wavefront must execute it together.
 constexpr int AGX_GROUP = 8;
 
+// a double moved between lanes by a DPP control (two 32-bit halves); every source lane the
+// control names must be active
+template<int CTRL>
+__device__ __forceinline__ double dppMove(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
 template<int G>
 __device__ __forceinline__ double groupSum(double v)
 {
+    if constexpr(G == 8) {
+        // the same butterfly as below with DPP moves instead of LDS permutes: quad_perm
+        // [1,0,3,2] (lane ^ 1), quad_perm [2,3,0,1] (lane ^ 2), then row_half_mirror (lane i of
+        // the 8 takes lane 7 - i, which lies in the other quad, holding the other quad's sum)
+        v += dppMove<0xB1>(v);
+        v += dppMove<0x4E>(v);
+        v += dppMove<0x141>(v);
+        return v;
+    } else {
 #pragma unroll
-    for(int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o, 64);
-    return v;
+        for(int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o, 64);
+        return v;
+    }
 }
 
 // interiorEval with the harmonics split over the group; g = lane index within the group.
@@ -932,8 +952,8 @@ __device__ __forceinline__ int cellIndexR(double x, GP arr, int n, double scale)
 
 // cell of the tau grid holding t in [-1, 1], from the bucket table (each bucket narrower than a
 // cell, so at most one node lies inside it)
-template<class GP>
-__device__ __forceinline__ int cellIndexT(double t, GP arr, int n, const int AGX_GLOBAL* lut, int nlut)
+template<class GP, class IP>
+__device__ __forceinline__ int cellIndexT(double t, GP arr, int n, IP lut, int nlut)
 {
     if(!(t >= arr[0]) || t > arr[n - 1]) return -1;
     int b = (int)((t + 1) * (0.5 * nlut));
@@ -950,10 +970,14 @@ __device__ __forceinline__ int cellIndexT(double t, GP arr, int n, const int AGX
 // powers v formed by multiplication, cos/sin(theta) = z/r, R/r and cos/sin(phi) given.  Agrees
 // with powerLawEval to rounding.
 // the arrays that powerLawEvalFast reads, in global memory or staged in LDS (plStage): S, U, W of
-// the inner [0] and outer [1] extrapolation for c < (legLmax+1)^2, the Legendre table and legStart
+// the inner [0] and outer [1] extrapolation for c < (legLmax+1)^2, the Legendre table and legStart;
+// and the tau bucket table of the cell search (lut), in the same address space
 template<class GP>
 struct PLTabs {
+    typedef typename std::conditional<std::is_same<GP, const double AGX_LDS*>::value,
+        const int AGX_LDS*, const int AGX_GLOBAL*>::type IP;
     GP S[2], U[2], W[2], leg, legStart;
+    IP lut;
 };
 __device__ __forceinline__ PLTabs<const double AGX_GLOBAL*> plTabsGlobal(const AgxMultipoleDev& P)
 {
@@ -961,6 +985,7 @@ __device__ __forceinline__ PLTabs<const double AGX_GLOBAL*> plTabsGlobal(const A
     t.S[0] = gmem(P.inner.S); t.U[0] = gmem(P.inner.U); t.W[0] = gmem(P.inner.W);
     t.S[1] = gmem(P.outer.S); t.U[1] = gmem(P.outer.U); t.W[1] = gmem(P.outer.W);
     t.leg = gmem(P.legTab); t.legStart = gmem(P.legStart);
+    t.lut = gmem(P.tauLut);
     return t;
 }
 // doubles of the LDS copy made by plStage
@@ -969,10 +994,12 @@ __device__ __forceinline__ int plStageDoubles(const AgxMultipoleDev& P)
     const int L1 = P.legLmax + 1;
     return (6 + AGX_LEG_STRIDE) * L1 * L1 + L1;
 }
-// copy the power-law arrays into `lds` (plStageDoubles(P) doubles) with the whole block; the
-// caller synchronises the block afterwards
-__device__ __forceinline__ PLTabs<const double AGX_LDS*> plStage(const AgxMultipoleDev& P, double AGX_LDS* lds)
+// copy the power-law arrays into `lds` (plStageDoubles(P) doubles) and the tau bucket table into
+// `lut` (P.nTauLut ints) with the whole block; the caller synchronises the block afterwards
+__device__ __forceinline__ PLTabs<const double AGX_LDS*> plStage(const AgxMultipoleDev& P, double AGX_LDS* lds,
+    int AGX_LDS* lut)
 {
+    for(int i = threadIdx.x; i < P.nTauLut; i += blockDim.x) lut[i] = P.tauLut[i];
     const int L1 = P.legLmax + 1, nc = L1 * L1;
     const double* src[8] = { P.inner.S, P.inner.U, P.inner.W, P.outer.S, P.outer.U, P.outer.W, P.legTab, P.legStart };
     const int len[8] = { nc, nc, nc, nc, nc, nc, AGX_LEG_STRIDE * nc, L1 };
@@ -986,6 +1013,7 @@ __device__ __forceinline__ PLTabs<const double AGX_LDS*> plStage(const AgxMultip
     }
     t.S[0] = dst[0]; t.U[0] = dst[1]; t.W[0] = dst[2]; t.S[1] = dst[3]; t.U[1] = dst[4]; t.W[1] = dst[5];
     t.leg = dst[6]; t.legStart = dst[7];
+    t.lut = lut;
     return t;
 }
 
@@ -1208,6 +1236,119 @@ __device__ __forceinline__ void powerLawEvalFast(const AgxMultipoleDev& P, const
     gphi = gph;
 }
 
+// powerLawEvalFast for a group of G lanes sharing one point (g = lane index within the group):
+// lane g forms the Legendre chains of |m| = g, g + G, ... (each shared by m and -m, as in
+// powerLawEvalUnrolled) and its part of the Fourier sum; the parts are added across the group.
+// The chains have the same operations as in powerLawEvalUnrolled, so that one lane's terms carry
+// the same bits; only the order of the final additions differs (rounding).  The whole group
+// must take the call together (its lanes hold the same point, so they take the same branch).
+template<int G, class GP>
+__device__ __forceinline__ void powerLawEvalGroup(const AgxMultipoleDev& P, const AgxPowerLawDev& L,
+    const PLTabs<GP>& T, double R, double z, double r, double cphi, double sphi, double& pot, double& gR,
+    double& gz, double& gphi, int g)
+{
+    const double rsq = r * r;
+    const bool mono = (L.inner && rsq < L.r0sq * 1e-16) || (!L.inner && rsq > L.r0sq * 1e16) || L.lmax == 0;
+    if(mono || (L.lmax == 2 && L.mmin == 0 && L.step == 2)) {   // the cheap branches, on every lane
+        const double phi = R == 0 ? 0. : atan2A(R * sphi, R * cphi);
+        powerLawEval(L, R, z, phi, pot, gR, gz, gphi);
+        return;
+    }
+    const double dlogr = log(rsq / L.r0sq) * 0.5, qq = exp(dlogr), qstep = L.inner ? qq : rcpD(qq);
+    const double rinv = rcpD(r), ct = z * rinv, st = R * rinv, stinv = st > 0 ? rcpD(st) : 0.;
+    const int side = L.inner ? 0 : 1;
+    const GP S = T.S[side], U = T.U[side], W = T.W[side];
+    const int Lmax = L.lmax, mmin = L.mmin, mmax = L.mmax, stepMask = L.step - 1;
+    const int stride = P.legLmax + 1;
+    const bool uni = L.uniformS != 0;
+    const double S0 = S[0], eS0 = uni && S0 != 0 ? exp(dlogr * S0) : 1;
+    const double alpha = cphi >= 0 ? sphi * sphi * rcpD(1 + cphi) : 1 - cphi;
+    double vr = 0, gr = 0, gth = 0, gph = 0;
+    for(int am = g; am <= Lmax; am += G) {
+        if(am > mmax && -am < mmin) continue;
+        const bool usePos = am <= mmax && L.lmin[am + mmax] <= Lmax;
+        const bool useNeg = am > 0 && -am >= mmin && L.lmin[-am + mmax] <= Lmax;
+        if(!usePos && !useNeg) continue;
+        const int lminP = usePos ? L.lmin[am + mmax] : 1 << 20, lminN = useNeg ? L.lmin[-am + mmax] : 1 << 20;
+        // cos, sin(am phi) by the recurrence of trigMultiAngle, and (r/r0)^v at l = am
+        double cm = 1, sm = 0, rv = L.inner ? 1. : qstep;
+        for(int k = 0; k < am; k++) {
+            const double cn = cm - (alpha * cm + sphi * sm), sn = sm - (alpha * sm - sphi * cm);
+            cm = cn; sm = sn;
+            rv *= qstep;
+        }
+        const GP tab = T.leg + am * stride * AGX_LEG_STRIDE;
+        double Plm1, dP0;
+        if(am == 0) { Plm1 = 1; dP0 = 0; }
+        else if(am == 1) { Plm1 = -st; dP0 = -ct; }
+        else {
+            double sinm2 = 1, xb = st;
+            for(int k = am - 2; k; k >>= 1) { if(k % 2) sinm2 *= xb; xb *= xb; }
+            const double coef = T.legStart[am];
+            dP0 = am * coef * sinm2 * st * ct;
+            Plm1 = coef * sinm2 * (st * st);
+        }
+        double Plm = ct * (2 * am + 1) * Plm1, Plm2 = 0;
+        double CvP = 0, CrP = 0, CtP = 0, CvN = 0, CrN = 0, CtN = 0;
+        const bool pole = !(st >= 1e-8 || (am > 2 && st > 0));
+        for(int l = am; l <= Lmax; l++, rv *= qstep) {
+            const GP e = tab + l * AGX_LEG_STRIDE;
+            const double nrm = e[0];
+            double Pl, dPl;
+            if(l == am) { Pl = Plm1 * nrm; dPl = dP0 * nrm; }
+            else {
+                if(l > am + 1) Plm = ct * e[1] * Plm1 - e[2] * Plm2;
+                Pl = Plm * nrm;
+                double d = (l * ct * Plm - (l + am) * Plm1) * stinv;
+                if(pole) {   // the reference's limits at the poles
+                    if(am == 0) d = -l * (l + 1) / 2 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
+                    else if(am == 1) d = -l * (l + 1) / 2 * (ct > 0 || l % 2 == 0 ? 1 : -1);
+                    else if(am == 2) d = l * (l + 1) * (l + 2) * (l - 1) / 4 * st * (ct > 0 || l % 2 == 1 ? 1 : -1);
+                    else d = 0;
+                }
+                dPl = nrm * d;
+                Plm2 = Plm1;
+                Plm1 = Plm;
+            }
+            const double v = L.inner ? l : -l - 1;
+            auto term = [&](int m, double& Cv, double& Cr, double& Ct) {
+                const int c = l * (l + 1) + m;
+                const double sc = S[c], u = U[c], w = W[c];
+                const double rs = sc != v ? (uni ? eS0 : sc != 0 ? exp(dlogr * sc) : 1) : rv;
+                const double urs = u * rs * (sc != v || u == 0 ? 1 : dlogr);
+                const double wrv = w * rv;
+                const double qr2 = v == 0 ? L.Q * rsq / L.r0sq : 0;
+                const double Pv = urs + wrv + qr2, dPv = urs * sc + wrv * v + (sc != v ? 0 : u * rs) + qr2 * 2;
+                Cv += Pl * Pv;
+                Cr += Pl * dPv;
+                Ct += dPl * Pv;
+            };
+            if(l >= lminP && ((l - lminP) & stepMask) == 0) term(am, CvP, CrP, CtP);
+            if(l >= lminN && ((l - lminN) & stepMask) == 0) term(-am, CvN, CrN, CtN);
+        }
+        if(am == 0) { vr += CvP; gr += CrP; gth += CtP; continue; }
+        if(useNeg) {   // sine terms: sin(am phi), d/dphi = am cos(am phi)
+            vr  += CvN * sm;
+            gr  += CrN * sm;
+            gth += CtN * sm;
+            gph += CvN * (am * cm);
+        }
+        if(usePos) {
+            vr  += CvP * cm;
+            gr  += CrP * cm;
+            gth += CtP * cm;
+            gph += CvP * (-am * sm);
+        }
+    }
+    vr = groupSum<G>(vr); gr = groupSum<G>(gr); gth = groupSum<G>(gth); gph = groupSum<G>(gph);
+    pot = vr;
+    // transformDerivsSphToCyl
+    const double r2inv = rinv * rinv, drdR = R * r2inv, drdz = z * r2inv;
+    gR = gr * drdR + gth * drdz;
+    gz = gr * drdz + gth * -drdR;
+    gphi = gph;
+}
+
 // multipoleCar on the per-cell tables (fetch: FetchCell).  The same function as
 // Multipole::evalCyl behind BasePotentialCyl::evalCar (src/potential_multipole.cpp:1538-1548,
 // 1969-2080), arranged for fewer instructions: cos and sin of the azimuth come from x/R and
@@ -1227,7 +1368,7 @@ __device__ __forceinline__ void multipoleCarCell(const AgxMultipoleDev& P, const
     const double logr = log(r), rplusRinv = rcpD(r + R),
         tau = axis ? (z > 0 ? 1. : z < 0 ? -1. : 0.) : z * rplusRinv;
     const int xi = cellIndexR(logr, gridR, P.nr, P.lnrScale),
-        yi = cellIndexT(tau, gridT, P.nt, gmem(P.tauLut), P.nTauLut);
+        yi = cellIndexT(tau, gridT, P.nt, plt.lut, P.nTauLut);
     const bool inGrid = xi >= 0 && yi >= 0;
     const int xc = inGrid ? xi : 0, yc = inGrid ? yi : 0;
     const double u = logr - gridR[xc], v = tau - gridT[yc];
@@ -1310,7 +1451,8 @@ __device__ __forceinline__ void multipoleCarCell(const AgxMultipoleDev& P, const
     double gzc = inGrid ? trDr * drdz + trDth * dthdz : NAN;
     double gphi = inGrid ? trDphi : NAN;
     pot = inGrid ? trPot : NAN;
-    if(rsq < P.rsqInner)   // power-law extrapolation (lanes beyond the grid)
+    // power-law extrapolation (lanes beyond the grid)
+    if(rsq < P.rsqInner)
         powerLawEvalFast(P, P.inner, plt, R, z, r, cphi, sphi, pot, gR, gzc, gphi);
     else if(rsq > P.rsqOuter)
         powerLawEvalFast(P, P.outer, plt, R, z, r, cphi, sphi, pot, gR, gzc, gphi);
@@ -1349,7 +1491,7 @@ __device__ __forceinline__ void multipoleCarCellGroup(const AgxMultipoleDev& P, 
     const double logr = log(r), rplusRinv = rcpD(r + R),
         tau = axis ? (z > 0 ? 1. : z < 0 ? -1. : 0.) : z * rplusRinv;
     const int xi = cellIndexR(logr, gridR, P.nr, P.lnrScale),
-        yi = cellIndexT(tau, gridT, P.nt, gmem(P.tauLut), P.nTauLut);
+        yi = cellIndexT(tau, gridT, P.nt, plt.lut, P.nTauLut);
     const bool inGrid = xi >= 0 && yi >= 0;
     const int xc = inGrid ? xi : 0, yc = inGrid ? yi : 0;
     const double u = logr - gridR[xc], v = tau - gridT[yc];
@@ -1401,9 +1543,9 @@ __device__ __forceinline__ void multipoleCarCellGroup(const AgxMultipoleDev& P, 
     double gphi = inGrid ? trDphi : NAN;
     pot = inGrid ? trPot : NAN;
     if(rsq < P.rsqInner)
-        powerLawEvalFast(P, P.inner, plt, R, z, r, cphi, sphi, pot, gR, gzc, gphi);
+        powerLawEvalGroup<G>(P, P.inner, plt, R, z, r, cphi, sphi, pot, gR, gzc, gphi, g);
     else if(rsq > P.rsqOuter)
-        powerLawEvalFast(P, P.outer, plt, R, z, r, cphi, sphi, pot, gR, gzc, gphi);
+        powerLawEvalGroup<G>(P, P.outer, plt, R, z, r, cphi, sphi, pot, gR, gzc, gphi, g);
     const double dRdx = axis ? 1. : cphi, dRdy = axis ? 1. : sphi,
         dphidx = axis ? 1. : -sphi * Rinv, dphidy = axis ? 1. : cphi * Rinv;
     gx = gR * dRdx + gphi * dphidx;

@@ -92,6 +92,8 @@ constexpr int OBS = 256;
 constexpr int AGX_LDS_GRID_ORBIT = 512;
 // LDS for the per-cell path's power-law arrays (10 (legLmax+1)^2 + legLmax + 1 doubles: legLmax <= 16)
 constexpr int AGX_LDS_PL_ORBIT = 2944;
+// LDS for the tau bucket table of the per-cell path (the host builds at most 4096 buckets)
+constexpr int AGX_LDS_TAULUT_ORBIT = 4096;
 typedef const double AGX_LDS* LdsGrid;
 
 // one right-hand-side potential gradient in the inertial frame (orbit.cpp:182-208)
@@ -144,12 +146,13 @@ void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, const AgxOrbitArgs* __r
     // the per-cell path's power-law extrapolation arrays (the launcher checks that they fit)
     constexpr bool CELLS = MODE == AGX_FETCH_CELL;
     __shared__ double plLds[CELLS ? AGX_LDS_PL_ORBIT : 1];
+    __shared__ int lutLds[CELLS ? AGX_LDS_TAULUT_ORBIT : 1];
     const AgxMultipoleDev& P = *Pp;
     // the grids in LDS (the launcher refuses larger ones): binary searches read LDS directly
     for(int i = threadIdx.x; i < P.nr + P.nt; i += blockDim.x)
         gridLds[i] = i < P.nr ? P.gridR[i] : P.gridT[i - P.nr];
     PLTabs<LdsGrid> plt{};
-    if constexpr(CELLS) { if(!P.interp1d) plt = plStage(P, (double AGX_LDS*)plLds); }
+    if constexpr(CELLS) { if(!P.interp1d) plt = plStage(P, (double AGX_LDS*)plLds, (int AGX_LDS*)lutLds); }
     __syncthreads();
     const LdsGrid gR = (LdsGrid)gridLds, gT = gR + P.nr;
     Lane L;
@@ -287,8 +290,9 @@ hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, i
         return hipErrorInvalidValue;
     if(!dArgs || P.nr + P.nt > AGX_LDS_GRID_ORBIT) return hipErrorInvalidValue;
     if(fetch == AGX_FETCH_CELL && !P.interp1d &&
-       (6 + AGX_LEG_STRIDE) * (P.legLmax + 1) * (P.legLmax + 1) + P.legLmax + 1 > AGX_LDS_PL_ORBIT)
-        fetch = AGX_FETCH_VEC2;   // power-law arrays too large for the kernel's LDS copy
+       ((6 + AGX_LEG_STRIDE) * (P.legLmax + 1) * (P.legLmax + 1) + P.legLmax + 1 > AGX_LDS_PL_ORBIT ||
+        P.nTauLut > AGX_LDS_TAULUT_ORBIT))
+        fetch = AGX_FETCH_VEC2;   // power-law arrays or bucket table too large for the kernel's LDS copy
     hipLaunchKernelGGL(storeArgsKernel, dim3(1), dim3(64), 0, stream, A, dArgs);
     hipError_t e = hipGetLastError();
     if(e != hipSuccess) return e;

@@ -51,7 +51,32 @@ __global__ __launch_bounds__(256) void probeKeyKernel(long n, const double* __re
     idx[i] = (int)i;
 }
 
+// the group wavefronts' share of a probed order, dealt round-robin: entry 8k + j of the first
+// 8 * groupWaves entries takes rank j * groupWaves + k.  The groups of a wavefront take eight
+// consecutive entries at the start, so each group wavefront receives one of the groupWaves
+// longest orbits and seven shorter ones, instead of one wavefront receiving the eight longest
+// (it would run all eight groups, in turn where their branches differ, for the longest orbit's
+// whole duration).  One block; 8 * groupWaves <= AGX_DEAL_MAX.
+constexpr int AGX_DEAL_MAX = 8192;
+__global__ __launch_bounds__(1024) void dealGroupsKernel(int* __restrict__ order, int groupWaves)
+{
+    __shared__ int s[AGX_DEAL_MAX];
+    const int m = 8 * groupWaves;
+    for(int i = threadIdx.x; i < m; i += blockDim.x) s[i] = order[i];
+    __syncthreads();
+    for(int i = threadIdx.x; i < m; i += blockDim.x) order[i] = s[(i & 7) * groupWaves + (i >> 3)];
+}
+
 }  // namespace
+
+// deal the first groupOrbits entries of `order` over the group wavefronts (dealGroupsKernel);
+// groupOrbits: a multiple of 8, at most 8192
+hipError_t agxLaunchDealGroups(int* order, long groupOrbits, hipStream_t stream)
+{
+    if(groupOrbits <= 0 || groupOrbits % 8 != 0 || groupOrbits > AGX_DEAL_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(dealGroupsKernel, dim3(1), dim3(1024), 0, stream, order, (int)(groupOrbits / 8));
+    return hipGetLastError();
+}
 
 // bytes of workspace needed by agxLaunchSchedule for n orbits
 size_t agxScheduleWorkspace(long n)

@@ -35,23 +35,22 @@ sys.path.insert(0, ROOT)
 
 METRIC = "orbit-steps/s at 1M orbits: MLP surrogate vs FP64 Multipole at ΔE/E≤1e-6"
 
-# Algorithmic FP64 operations (+ - * / sqrt log exp each count 1) of the GPU path, counted
-# from the kernel's arithmetic for the triaxial lmax=8 expansion (5 harmonics), Omega = 0:
-#   per right-hand-side evaluation: Cartesian->cylindrical incl. atan2 38, branch test 3,
-#     interior setup 18, quintic weights 2x38, 5 harmonics x 229 (2-D quintic spline),
-#     4 x 16 (rescaling + Fourier sums), cos/sin recurrence 146, m=0 unscaling 7,
-#     chain rule 19, frame rotation 15                                     -> 1531
-#   per step attempt besides the 12 evaluations: stage sums 666, b-sum 102, error norm and
-#     step control 200, energy bookkeeping 10                             -> 978
-#   per orbit: 3 initial evaluations + first-step estimate ~80
-FLOPS_PER_EVAL = 1531
-FLOPS_PER_ATTEMPT_EXTRA = 978
-# DPRKN8 (--method dprkn8): 9 evaluations per attempt; besides them the position-only stage
-#   sums 324, x/v b-sums 96, error norm and PI control 190, energy bookkeeping 10     -> 620
-EVALS_PER_ATTEMPT = {"dop853": 12, "dprkn8": 9}
-FLOPS_PER_ATTEMPT_EXTRA_BY_METHOD = {"dop853": FLOPS_PER_ATTEMPT_EXTRA, "dprkn8": 620}
+# Algorithmic FP64 operations per unit of work: the reference algorithm's own counts, from the
+# oracle (its C++ restatement) compiled with an operation-counting double (+ - * / sqrt and each
+# libm call count 1; oracle/flopcount.py on 10^4 orbits of the 1-300-period workload and 2x10^4
+# density points of the triaxial Dehnen lmax=8 expansion):
+#   per DOP853 step attempt, everything included (12 evaluations, stage sums, error norm and
+#     step control, energy bookkeeping, the orbits' starts amortised)           27,795
+#   per DPRKN8 step attempt (9 evaluations)                                      20,508
+#   per potential + gradient evaluation at density points                        2,194
+#   per potential + gradient + Hessian evaluation                                3,044
+FLOPS_PER_ATTEMPT = {"dop853": 27795, "dprkn8": 20508}
+# The GPU kernels execute less arithmetic for the same work (per-cell polynomial tables instead of
+# per-point Hermite weights): counted by hand from the kernel code, 1,531 per right-hand-side
+# evaluation and 978 (DOP853) / 620 (DPRKN8) per attempt besides the evaluations; reported beside
+# the roofline figure as executed_frac.
+GPU_FLOPS_PER_ATTEMPT = {"dop853": 12 * 1531 + 978, "dprkn8": 9 * 1531 + 620}
 METHOD_FLAG = {"dop853": 0, "dprkn8": 0x200}
-FLOPS_PER_ORBIT_INIT = 3 * FLOPS_PER_EVAL + 80
 # record fetch modes of the Multipole evaluation (agx_set_fetch)
 FETCH_MODES = {"scalar": 0, "vec2": 1, "coop": 2, "cell": 3}
 # launches timed for a leg that is reported beside the headline figure (not the headline)
@@ -59,9 +58,9 @@ COMPARISON_STEPS = 2
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix), AMD published figure
 # HBM bytes per launch of the orbit path on this workload (the probe and the main persistent
 # kernel), from rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy
-# (profiles/orbitKernel_1M_uneven_triaxial_dehnen_lmax8.md): FETCH 465,962 KiB x 2.000,
-# WRITE 354,396 KiB x 1.000
-ORBIT_TRAFFIC_BYTES = 465962 * 1024 * 2.0 + 354396 * 1024
+# (profiles/orbitKernel_1M_uneven_triaxial_dehnen_lmax8.md): FETCH 467,243 KiB x 2.000,
+# WRITE 354,277 KiB x 1.000
+ORBIT_TRAFFIC_BYTES = 467243 * 1024 * 2.0 + 354277 * 1024
 
 
 def parse():
@@ -103,14 +102,10 @@ def parse():
     return ap.parse_args()
 
 
-# algorithmic FP64 operations of one potential+gradient evaluation (eval mode): the
-# right-hand-side count above without the frame rotation
-FLOPS_PER_POINT = FLOPS_PER_EVAL - 15
-# ... and of one potential+gradient+Hessian evaluation (hess mode): per harmonic 9 quintic passes
-#   with second derivatives (35 each) + corner offset 5 -> 320, log-scaling products 26 and
-#   Fourier sums 22 for m != 0; 5 harmonics -> 1,840; weights 80, cylindrical and Cartesian
-#   gradient/Hessian transforms incl. atan2 and the (ln r, tau) second derivatives 280  -> 2,200
-FLOPS_PER_POINT_HESS = 2200
+# algorithmic FP64 operations of one potential+gradient evaluation (eval mode) and of one
+# potential+gradient+Hessian evaluation (hess mode): the oracle's counts (see FLOPS_PER_ATTEMPT)
+FLOPS_PER_POINT = 2194
+FLOPS_PER_POINT_HESS = 3044
 
 
 def bench_eval(args, hess=False):
@@ -294,8 +289,8 @@ def main():
     st = d_st.cpu().numpy()
     steps_per_launch = int(ns.sum())
     attempts = int(na.sum())
-    flops = attempts * (EVALS_PER_ATTEMPT[args.method] * FLOPS_PER_EVAL +
-                        FLOPS_PER_ATTEMPT_EXTRA_BY_METHOD[args.method]) + n * FLOPS_PER_ORBIT_INIT
+    flops = attempts * FLOPS_PER_ATTEMPT[args.method]
+    gpu_flops = attempts * GPU_FLOPS_PER_ATTEMPT[args.method]
     per_launch = kernel_s / fp64_steps
     fp64_value = steps_per_launch * fp64_steps / wall
     achieved = flops / per_launch / 1e12
@@ -405,7 +400,9 @@ def main():
                      "traffic": ORBIT_TRAFFIC_BYTES if (n == 1_000_000 and args.precision == "fp64"
                                                         and args.method == "dop853") else None,
                      "kernel": "orbitKernel (FP64)" if args.method == "dop853" else "orbitKernelRKN (FP64)",
-                     "kernel_ms": per_launch * 1e3},
+                     "kernel_ms": per_launch * 1e3,
+                     "flops_per_attempt": FLOPS_PER_ATTEMPT[args.method],
+                     "executed_frac": gpu_flops / per_launch / 1e12 / PEAK_FP64_TFLOPS},
         "cpu_baseline": cpu,
     }
     if surrogate:
@@ -464,8 +461,7 @@ def bench_orbitobj(args):
     interp = sum(t[2] for t in times) / len(times)
     total_rows = int(rows.sum())
     attempts = int(na.sum())
-    flops = attempts * (EVALS_PER_ATTEMPT["dop853"] * FLOPS_PER_EVAL + FLOPS_PER_ATTEMPT_EXTRA) + \
-        n * FLOPS_PER_ORBIT_INIT
+    flops = attempts * FLOPS_PER_ATTEMPT["dop853"]
     print(f"orbitobj: {n} orbits, {total_rows} rows (median {np.median(rows):.0f} per orbit); record "
           f"{rec*1e3:.1f} ms, spline build {build*1e3:.1f} ms, {n*q} evaluations {interp*1e3:.1f} ms",
           file=sys.stderr, flush=True)

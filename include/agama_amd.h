@@ -359,16 +359,18 @@ int agx_set_orbit_waves(int waves);
    integrates the first `orbits` orbits of the processing order - the longest expected - with 8
    lanes per orbit, each lane evaluating a share of the azimuthal harmonics; orbits = -1: one per
    8 lanes of those wavefronts; stride = 0: off; stride = -1: automatic (every orbit in groups when
-   they fit, 4 for up to 2 orbits per lane, 8 for up to 4, above that 32 after a work-order probe
-   (agx_set_orbit_probe) and off without one).  Results differ from the
-   one-lane form only by the order of the additions in the Fourier sum.  Defaults: -1, -1, 65536. */
+   they fit, 4 for up to 2 orbits per lane, 8 for up to 4, above that 8 after a work-order probe
+   (agx_set_orbit_probe) and off without one; after a probe, with the default `orbits`, the
+   group wavefronts' orbits are dealt round-robin so that each holds one of the longest).  Results
+   differ from the one-lane form only by the order of the additions in the Fourier sum and in the
+   power-law extrapolation's sum over harmonics.  Defaults: -1, -1, 65536. */
 int agx_set_orbit_groups(int stride, long orbits, long min_batch);
 /* Work-order probe of the DOP853/DPRKN8 batches (no counterpart in the reference: a scheduling
    knob of this library).  In batches of at least min_batch orbits (without deviation vectors,
    targets or every-step recording) every orbit is first integrated for at most `steps` steps
    with no output but the time it reached, and the orbits are processed in the order of the
    steps they would take at that rate, longest first (otherwise: |T| / t_dyn at the start);
-   with the probe, group mode's automatic stride above 4 orbits per lane is 32.  Results do not
+   with the probe, group mode's automatic stride above 4 orbits per lane is 8.  Results do not
    depend on the order except through group mode.  steps = 0: off.  Defaults: 64, 65536. */
 int agx_set_orbit_probe(int steps, long min_batch);
 int agx_get_orbit_config(int* num_blocks, int* fetch_eval, int* fetch_orbit, int* waves);

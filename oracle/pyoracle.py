@@ -104,6 +104,22 @@ def fast():
     return lib_fast
 
 
+lib_flops = None                     # strict build with an operation-counting double
+
+
+def flops():
+    """the oracle compiled with every double an operation-counting scalar (flopcount.cpp): the
+    same results as the strict build, plus flc_ops(nthreads) / flc_reset(nthreads)"""
+    global lib_flops
+    if lib_flops is None:
+        lib_flops = _load("liboracle_flops.so")
+        lib_flops.flc_ops.restype = ctypes.c_longlong
+        lib_flops.flc_ops.argtypes = [ctypes.c_int]
+        lib_flops.flc_reset.restype = None
+        lib_flops.flc_reset.argtypes = [ctypes.c_int]
+    return lib_flops
+
+
 RHS = ctypes.CFUNCTYPE(None, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                        ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
 MAT = ctypes.CFUNCTYPE(None, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
