@@ -636,6 +636,17 @@ int agx_set_orbit_probe(int steps, long min_batch)
     return AGX_OK;
 }
 
+int agx_get_orbit_schedule(int* group_stride, long* group_orbits, long* group_min_batch, int* probe_steps,
+    long* probe_min_batch)
+{
+    if(group_stride) *group_stride = g_groupStride;
+    if(group_orbits) *group_orbits = g_groupOrbits;
+    if(group_min_batch) *group_min_batch = g_groupMinBatch;
+    if(probe_steps) *probe_steps = g_probeSteps;
+    if(probe_min_batch) *probe_min_batch = g_probeMinBatch;
+    return AGX_OK;
+}
+
 int agx_get_orbit_config(int* num_blocks, int* fetch_eval, int* fetch_orbit, int* waves)
 {
     if(num_blocks) *num_blocks = g_orbitBlocks;
@@ -700,6 +711,54 @@ static int* probeOrder(const agx_potential* pot, long n, const double* d_ic, con
     return fin;
 }
 
+// Group mode of a DOP853 batch (agx_set_orbit_groups): sets A.groupOrbits, A.groupStride and
+// A.groupCounter (the word after the batch's queue counter, cleared with it) and widens `blocks`
+// so that the groups and the one-lane rest both fit.  probedOrder: the probe's work order (inside
+// the batch's workspace, ours to permute) or null; after a probe the group wavefronts' orbits are
+// dealt round-robin (stream st).  eligible: the batch's own conditions (no deviation vectors,
+// targets or every-step recording).
+static void setupGroups(const agx_potential* pot, long n, int precision, int flags, bool eligible,
+    int* probedOrder, unsigned int* groupCounter, AgxOrbitArgs& A, int& blocks, int bs, hipStream_t st)
+{
+    const bool probed = probedOrder != nullptr;
+    int groupStride = g_groupStride;
+    if(groupStride < 0) {
+        // automatic: the fewer orbits per lane, the more the longest orbits (rather than the
+        // total work) set the batch's duration (measured on the uneven 1-300-period workload:
+        // 131k orbits -11% at one wavefront in 4, 262k -4% at one in 8; 1M orbits after the
+        // probe 7.5 -> 6.5 s at one in 8)
+        const long lanes = (long)blocks * bs;
+        groupStride = n * 8 <= lanes ? 1 : n <= 2 * lanes ? 4 : n <= 4 * lanes ? 8 : probed ? PROBED_GROUP_STRIDE : 0;
+    }
+    if(!(eligible && groupStride > 0 && n >= g_groupMinBatch && !(flags & AGX_METHOD_DPRKN8) &&
+         precision == AGX_PRECISION_FP64 && (g_fetchOrbit == AGX_FETCH_VEC2 || g_fetchOrbit == AGX_FETCH_CELL) &&
+         g_orbitWaves == 1 && pot->nfwMass == 0 && !pot->dev.interp1d))
+        return;
+    const long waves = (long)blocks * (bs / 64);
+    const long stride = std::min<long>(groupStride, waves);   // wavefront 0 is always a group one
+    const long groupWaves = (waves + stride - 1) / stride;
+    A.groupOrbits = std::min<long>(n, g_groupOrbits >= 0 ? g_groupOrbits : groupWaves * (64 / 8));
+    // with every wavefront in group mode no lane reads the single-lane queue: all the orbits then
+    // go to the groups
+    if(stride == 1 && A.groupOrbits > 0) A.groupOrbits = n;
+    A.groupStride = A.groupOrbits > 0 ? (int)stride : 0;
+    A.groupCounter = groupCounter;
+    if(A.groupStride == 0) { A.groupOrbits = 0; return; }
+    // one of the longest orbits per group wavefront (each takes eight at the start)
+    if(probed && stride > 1 && A.groupOrbits == groupWaves * 8 && A.groupOrbits <= 8192)
+        HIPCHECK(agxLaunchDealGroups(probedOrder, A.groupOrbits, st));
+    // enough blocks for the group orbits (8 lanes each) and the rest (one lane each)
+    int dev = 0, cus = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const long maxBlocks = g_orbitBlocks > 0 ? g_orbitBlocks :
+        (long)cus * std::max(1, agxOrbitKernelBlocksPerCU(precision, g_fetchOrbit, g_orbitWaves, false, false));
+    const long gw = A.groupStride * ((A.groupOrbits + 7) / 8);
+    const long lw = A.groupStride > 1 ? ((n - A.groupOrbits + 63) / 64) * A.groupStride / (A.groupStride - 1) : 0;
+    const long needWaves = std::max(gw, lw);
+    blocks = (int)std::min(maxBlocks, std::max<long>(blocks, (needWaves + bs / 64 - 1) / (bs / 64)));
+}
+
 // the batch entry points' common body; d_dev non-null or ly.bins non-null: the variational
 // kernel (der=True: six vectors; lyapunov only: one)
 static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic,
@@ -731,7 +790,6 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
     try {
         hipStream_t st = (hipStream_t)stream;
         void* work = nullptr;
-        bool probed = false;
         int* probedOrder = nullptr;   // the probe's order, inside the workspace (ours to permute)
         if(!d_order && !(flags & AGX_ORDER_AS_GIVEN)) {
             // longest expected integrations first (agx_schedule.hip); stream-ordered workspace,
@@ -749,7 +807,6 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
             if(g_probeSteps > 0 && n >= g_probeMinBatch && !variational && !d_targets && !pool.counter)
                 d_order = probedOrder = probeOrder(pot, n, d_ic, d_tstart, d_ttotal, accuracy, Omega, maxNumSteps,
                     precision, ord, work, st, nullptr);
-            probed = d_order != ord;
         }
         AgxOrbitArgs A;
         A.n = n; A.ic = d_ic; A.tstart = d_tstart; A.ttotal = d_ttotal; A.trajsize = d_trajsize;
@@ -765,43 +822,9 @@ static int orbitBatchDevice(const agx_potential* pot, long n, const double* d_ic
         int blocks, bs;
         orbitGeometry(n, precision, pot->nfwMass != 0, blocks, bs, flags & AGX_METHOD_DPRKN8, d_targets != nullptr);
         // group mode: the first orbits of the order (the longest expected) on groups of 8 lanes
-        int groupStride = g_groupStride;
-        if(groupStride < 0) {
-            // automatic: the fewer orbits per lane, the more the longest orbits (rather than the
-            // total work) set the batch's duration (measured on the uneven 1-300-period
-            // workload: 131k orbits -11% at one wavefront in 4, 262k -4% at one in 8, 1M neutral)
-            const long lanes = (long)blocks * bs;
-            groupStride = n * 8 <= lanes ? 1 : n <= 2 * lanes ? 4 : n <= 4 * lanes ? 8 : probed ? PROBED_GROUP_STRIDE : 0;
-        }
-        if(allowGroups && groupStride > 0 && n >= g_groupMinBatch && !variational && !d_targets && !pool.counter &&
-           !(flags & AGX_METHOD_DPRKN8) && precision == AGX_PRECISION_FP64 &&
-           (g_fetchOrbit == AGX_FETCH_VEC2 || g_fetchOrbit == AGX_FETCH_CELL) &&
-           g_orbitWaves == 1 && pot->nfwMass == 0 && !pot->dev.interp1d) {
-            const long waves = (long)blocks * (bs / 64);
-            const long stride = std::min<long>(groupStride, waves);   // wavefront 0 is always a group one
-            const long groupWaves = (waves + stride - 1) / stride;
-            A.groupOrbits = std::min<long>(n, g_groupOrbits >= 0 ? g_groupOrbits : groupWaves * (64 / 8));
-            // with every wavefront in group mode no lane reads the single-lane queue: all the
-            // orbits then go to the groups
-            if(stride == 1 && A.groupOrbits > 0) A.groupOrbits = n;
-            A.groupStride = A.groupOrbits > 0 ? (int)stride : 0;
-            A.groupCounter = pot->dcounter + 1;
-            if(A.groupStride == 0) A.groupOrbits = 0;
-            else {
-                // one of the longest orbits per group wavefront (each takes eight at the start)
-                if(probed && stride > 1 && A.groupOrbits == groupWaves * 8 && A.groupOrbits <= 8192)
-                    HIPCHECK(agxLaunchDealGroups(probedOrder, A.groupOrbits, st));   // enough blocks for the group orbits (8 lanes each) and the rest (one lane each)
-                int dev = 0, cus = 0;
-                HIPCHECK(hipGetDevice(&dev));
-                HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-                const long maxBlocks = g_orbitBlocks > 0 ? g_orbitBlocks :
-                    (long)cus * std::max(1, agxOrbitKernelBlocksPerCU(precision, g_fetchOrbit, g_orbitWaves, false, false));
-                const long gw = A.groupStride * ((A.groupOrbits + 7) / 8);
-                const long lw = A.groupStride > 1 ? ((n - A.groupOrbits + 63) / 64) * A.groupStride / (A.groupStride - 1) : 0;
-                const long needWaves = std::max(gw, lw);
-                blocks = (int)std::min(maxBlocks, std::max<long>(blocks, (needWaves + bs / 64 - 1) / (bs / 64)));
-            }
-        }
+        if(allowGroups)
+            setupGroups(pot, n, precision, flags, !variational && !d_targets && !pool.counter, probedOrder,
+                pot->dcounter + 1, A, blocks, bs, st);
         HIPCHECK(agxLaunchFillU32(pot->dcounter, 2, 0u, st));
         hipError_t e;
         if(variational) {
@@ -963,11 +986,17 @@ agx_orbit_graph* agx_orbit_graph_create(const agx_potential* pot, long n, const 
         unsigned int* counter = reinterpret_cast<unsigned int*>(w + ordBytes);
         AgxOrbitArgs* dArgs = reinterpret_cast<AgxOrbitArgs*>(w + ordBytes + 256);
         const int* order = nullptr;
+        int* probedOrder = nullptr;
+        void* schedWork = w + ordBytes + 256 + argBytes;
         if(!(flags & AGX_ORDER_AS_GIVEN)) {   // the work order, computed once from the initial conditions
-            HIPCHECK(agxLaunchSchedule(pot->dev, pot->ddev, n, d_ic, d_ttotal, ord, w + ordBytes + 256 + argBytes,
-                nullptr));
-            HIPCHECK(hipDeviceSynchronize());
+            HIPCHECK(agxLaunchSchedule(pot->dev, pot->ddev, n, d_ic, d_ttotal, ord, schedWork, nullptr));
             order = ord;
+            // large batches: the order from the probe, as agx_orbit_batch_device takes it (the
+            // probe's order stays in the graph's workspace)
+            if(g_probeSteps > 0 && n >= g_probeMinBatch)
+                order = probedOrder = probeOrder(pot, n, d_ic, d_tstart, d_ttotal, accuracy, Omega, maxNumSteps,
+                    precision, ord, schedWork, nullptr, dArgs);
+            HIPCHECK(hipDeviceSynchronize());
         }
         AgxOrbitArgs A;
         A.n = n; A.ic = d_ic; A.tstart = d_tstart; A.ttotal = d_ttotal; A.trajsize = d_trajsize;
@@ -977,11 +1006,14 @@ agx_orbit_graph* agx_orbit_graph_create(const agx_potential* pot, long n, const 
         A.outMaxde = d_maxde; A.outStatus = d_status; A.counter = counter;
         int blocks, bs;
         orbitGeometry(n, precision, pot->nfwMass != 0, blocks, bs, flags & AGX_METHOD_DPRKN8);
+        // group mode as in agx_orbit_batch_device (any dealing of the probe's order happens now)
+        setupGroups(pot, n, precision, flags, true, probedOrder, counter + 1, A, blocks, bs, nullptr);
+        HIPCHECK(hipDeviceSynchronize());
         hipStream_t st;
         HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         std::unique_ptr<void, void(*)(void*)> sg((void*)st, [](void* q) { (void)hipStreamDestroy((hipStream_t)q); });
         HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        hipError_t e = agxLaunchFillU32(counter, 1, 0u, st);
+        hipError_t e = agxLaunchFillU32(counter, 2, 0u, st);   // queue counter and group counter
         if(e == hipSuccess) e = launchOrbitMethod(pot, precision, flags, A, blocks, bs, st, dArgs);
         HIPCHECK(hipStreamEndCapture(st, &G->graph));
         HIPCHECK(e);

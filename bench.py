@@ -96,6 +96,9 @@ def parse():
     ap.add_argument("--periods", type=float, default=0,
                     help="FP64 path only: integrate every orbit for this many circular periods instead "
                          "of the log-uniform 1-300 (a uniform batch, for profiling the bulk)")
+    ap.add_argument("--graph", action="store_true",
+                    help="fp64 mode: time replays of the batch captured as a HIP graph (agx_orbit_graph_*; "
+                         "the work order and probe are computed once, when the graph is created)")
     ap.add_argument("--train-steps", type=int, default=6000)
     ap.add_argument("--train-samples", type=float, default=20e6)
     ap.add_argument("--max-rel-de", type=float, default=1e-6)
@@ -248,6 +251,21 @@ def main():
         if rc:
             raise RuntimeError(agama_amd._native.last_error())
 
+    graph = None
+    if args.graph:
+        if args.mode != "fp64":
+            raise SystemExit("--graph applies to --mode fp64")
+        graph = lib.agx_orbit_graph_create(pot.handle, n, d_ic.data_ptr(), None, d_T.data_ptr(), d_sz.data_ptr(),
+                                           d_off.data_ptr(), 1e-8, 0.0, int(1e8), d_time.data_ptr(), None,
+                                           d_fin.data_ptr(), d_ns.data_ptr(), d_na.data_ptr(), d_ne.data_ptr(),
+                                           d_de.data_ptr(), d_st.data_ptr(), prec | METHOD_FLAG[args.method])
+        if not graph:
+            raise RuntimeError(agama_amd._native.last_error())
+
+        def launch_fp64():   # noqa: F811 (the graph replaces the eager launch)
+            if lib.agx_orbit_graph_launch(graph, stream.cuda_stream):
+                raise RuntimeError(agama_amd._native.last_error())
+
     # One bench step is one integration of the whole batch: a persistent launch of several
     # seconds.  After the first launch of a kernel nothing is left to warm up (code object
     # loaded, tables in L2 after the first evaluations), so each leg runs at most one warm-up
@@ -287,6 +305,8 @@ def main():
     na = d_na.cpu().numpy()
     de = d_de.cpu().numpy()
     st = d_st.cpu().numpy()
+    if graph:
+        lib.agx_orbit_graph_destroy(graph)
     steps_per_launch = int(ns.sum())
     attempts = int(na.sum())
     flops = attempts * FLOPS_PER_ATTEMPT[args.method]
@@ -392,6 +412,7 @@ def main():
                    "attempts_per_step": attempts, "precision": args.precision,
                    "path": "MLP surrogate with FP64 fallback" if surrogate else "FP64 Multipole",
                    "group_mode": args.groups or "automatic",
+                   "hip_graph": bool(args.graph),
                    "warmup_launches_per_leg": warmup_used,
                    "headline_timed_launches": args.steps,
                    "fp64_leg_timed_launches": fp64_steps},

@@ -373,6 +373,10 @@ int agx_set_orbit_groups(int stride, long orbits, long min_batch);
    with the probe, group mode's automatic stride above 4 orbits per lane is 8.  Results do not
    depend on the order except through group mode.  steps = 0: off.  Defaults: 64, 65536. */
 int agx_set_orbit_probe(int steps, long min_batch);
+/* The current settings of agx_set_orbit_groups and agx_set_orbit_probe (any pointer may be
+   NULL; no counterpart in the reference), for the autotuner to restore them. */
+int agx_get_orbit_schedule(int* group_stride, long* group_orbits, long* group_min_batch, int* probe_steps,
+    long* probe_min_batch);
 int agx_get_orbit_config(int* num_blocks, int* fetch_eval, int* fetch_orbit, int* waves);
 
 /* One integration of a fixed batch of device arrays (arguments as agx_orbit_batch_device)

@@ -20,10 +20,24 @@ def pot():
     return agama_amd.Potential(file=COEF)
 
 
-def test_orbit_graph_replays_match_eager(pot):
+@pytest.fixture(params=["static", "probe+groups"])
+def schedule(request):
+    """the batch's work order: the static cost key, or the probe with group mode at one wavefront
+    in 8 (the thresholds lowered to this batch; the probe's order is dealt over the groups)"""
+    from agama_amd import lib, tune
+    saved = tune.current_schedule()
+    if request.param == "probe+groups":
+        assert lib.agx_set_orbit_probe(64, 2048) == 0
+        assert lib.agx_set_orbit_groups(8, -1, 2048) == 0
+    yield request.param
+    tune.apply(saved)
+
+
+def test_orbit_graph_replays_match_eager(pot, schedule):
     import torch
     import agama_amd
     from agama_amd import lib, workloads
+    print("schedule:", schedule, flush=True)
     n = 4096
     ic, T = workloads.triaxial_dehnen_orbits(n, lambda p: pot.potential(p), pot.coefs(), seed=51, uneven=True)
     T = T / 20
@@ -68,13 +82,18 @@ def test_orbit_graph_replays_match_eager(pot):
 
 def test_autotuner_settings_agree_and_restore(pot):
     from agama_amd import tune
-    before = tune.current()
-    cfg = tune.tune(pot, n=4096, repeats=1, verbose=True)
-    assert tune.current() == before
-    assert len(cfg["candidates"]) == 8
+    before, before_sched = tune.current(), tune.current_schedule()
+    cfg = tune.tune(pot, n=4096, repeats=1, groups=(-1, 0, 8), probes=(64, 0), verbose=True)
+    assert tune.current() == before and tune.current_schedule() == before_sched
+    # stage 1: 4 fetch modes x 2 wave counts; stage 2: 3 x 2 schedules but the current one
+    assert len(cfg["candidates"]) == 8 + 5
     assert cfg["fetch_orbit"] in tune.FETCH and cfg["waves"] in (1, 2)
-    print("tuned:", {k: cfg[k] for k in ("fetch_orbit", "waves")},
+    assert cfg["group_stride"] in (-1, 0, 8) and cfg["probe_steps"] in (64, 0)
+    print("tuned:", {k: cfg[k] for k in ("fetch_orbit", "waves", "group_stride", "probe_steps")},
           "identical:", all(c["identical_to_first"] for c in cfg["candidates"]))
     applied = tune.apply(cfg)
     assert applied["fetch_orbit"] == cfg["fetch_orbit"] and applied["waves"] == cfg["waves"]
-    tune.apply(before)
+    assert applied["group_stride"] == cfg["group_stride"] and applied["probe_steps"] == cfg["probe_steps"]
+    assert applied["group_min_batch"] == applied["probe_min_batch"] == 4096
+    tune.apply(dict(before, **before_sched))
+    assert tune.current_schedule() == before_sched
