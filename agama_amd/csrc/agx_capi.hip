@@ -1868,7 +1868,7 @@ void packInference(agx_surrogate& S)
 agx_surrogate* makeSurrogate(const agx_potential* pot, int baseLmax, float lnScale, float outScale,
     double rmin, double rmax)
 {
-    if(baseLmax < 0 || baseLmax > 2) throw std::invalid_argument("surrogate: base_lmax must be 0, 1 or 2");
+    if(baseLmax < 0 || baseLmax > 8) throw std::invalid_argument("surrogate: base_lmax must be 0 to 8");
     if(!(rmin > 0 && rmax > rmin)) throw std::invalid_argument("surrogate: invalid radius range");
     HIPCHECK(hipSetDevice(pot->device));
     std::unique_ptr<agx_surrogate> S(new agx_surrogate);

@@ -408,7 +408,8 @@ typedef struct agx_surrogate agx_surrogate;
    baseLmax=2, rmin=1e-2, rmax=1e2, warmup=200" (all optional).  Samples have ln r uniform in
    [ln rmin, ln rmax] and isotropic directions; targets are the FP64 Multipole's relative
    residual over the base; Adam with cosine decay; one training step is a replayed HIP graph.
-   The potential must outlive the surrogate. */
+   baseLmax: 0 to 8, below the potential's own order (a higher base leaves a smaller residual
+   and costs more FP64 work per evaluation).  The potential must outlive the surrogate. */
 agx_surrogate* agx_surrogate_train(const agx_potential* pot, const char* params);
 /* from given parameters (e.g. saved by a caller from agx_surrogate_info) */
 agx_surrogate* agx_surrogate_create(const agx_potential* pot, const float* theta, int base_lmax,

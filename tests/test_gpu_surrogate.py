@@ -159,6 +159,26 @@ def test_short_training_run(pot, tmp_path):
     assert np.array_equal(phi2, phi[:1000]) and np.array_equal(g2, g[:1000])
 
 
+def test_higher_base_order(pot):
+    """a base expansion of higher order (l <= 4 instead of 2) leaves the network a smaller residual
+    to learn: the same short training gives a smaller potential and force error, and the saved
+    parameters carry the base order"""
+    import agama_amd
+    from agama_amd import workloads
+    pts = workloads.density_points(100_000, seed=16)
+    r = np.linalg.norm(pts, axis=1)
+    pts = pts[(r > 0.02) & (r < 50)]
+    ophi, og = pot._eval(pts, True, True)
+    err = {}
+    for b in (2, 4):
+        sur = agama_amd.Surrogate.train(pot, samples=2e6, steps=1500, batch=16384, seed=2, baselmax=b)
+        phi, g = sur._eval(pts, True, True)
+        err[b] = (np.median(np.abs(phi / ophi - 1)), np.median(np.linalg.norm(g - og, axis=1) / np.linalg.norm(og, axis=1)))
+        assert int(sur.params()[1][0]) == b
+    print("median potential / force error by base order:", err)
+    assert err[4][0] < 0.5 * err[2][0] and err[4][1] < 0.5 * err[2][1]
+
+
 @pytest.fixture(scope="module")
 def trained(pot):
     import agama_amd
