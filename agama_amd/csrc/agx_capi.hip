@@ -736,7 +736,9 @@ static void setupGroups(const agx_potential* pot, long n, int precision, int fla
         return;
     const long waves = (long)blocks * (bs / 64);
     const long stride = std::min<long>(groupStride, waves);   // wavefront 0 is always a group one
-    const long groupWaves = (waves + stride - 1) / stride;
+    long groupWaves = 0;
+    for(int b = 0; b < blocks; b++)
+        for(int w = 0; w < bs / 64; w++) groupWaves += agxIsGroupWave(b, w, bs / 64, (int)stride);
     A.groupOrbits = std::min<long>(n, g_groupOrbits >= 0 ? g_groupOrbits : groupWaves * (64 / 8));
     // with every wavefront in group mode no lane reads the single-lane queue: all the orbits then
     // go to the groups

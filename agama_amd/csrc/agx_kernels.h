@@ -43,6 +43,20 @@ struct AgxOrbitArgs {
     double* outTend = nullptr;
 };
 
+// Placement of group mode's wavefronts (AgxOrbitArgs::groupStride): one wavefront in `stride`,
+// or with AGX_GROUP_BLOCKS every wavefront of the workgroups b with (b / 8) % stride == 0 (whole
+// workgroups, eight consecutive ones at a time, which the eight XCDs take in turn), so that the
+// group wavefronts share their CUs' memory pipelines with each other only.
+#ifndef AGX_GROUP_BLOCKS
+#define AGX_GROUP_BLOCKS 0
+#endif
+__host__ __device__ inline bool agxIsGroupWave(int block, int waveInBlock, int wavesPerBlock, int stride)
+{
+    if(stride <= 0) return false;
+    if(AGX_GROUP_BLOCKS) return (block / 8) % stride == 0;
+    return (block * wavesPerBlock + waveInBlock) % stride == 0;
+}
+
 enum AgxPrecision { AGX_FP64 = 0, AGX_FP32_TABLES = 1 };
 
 // how the FP64 kernels read the 2-D splines: node records with per-lane 8-byte loads, per-lane

@@ -158,8 +158,7 @@ void orbitKernel(const AgxMultipoleDev* __restrict__ Pp, const AgxOrbitArgs* __r
     Lane L;
     L.attach(lds + threadIdx.x);
     // wave-uniform: whether this wavefront works in group mode
-    const bool group = GRP && A.groupStride > 0 &&
-        (int)(blockIdx.x * (OBS / 64) + (threadIdx.x >> 6)) % A.groupStride == 0;
+    const bool group = GRP && agxIsGroupWave((int)blockIdx.x, (int)(threadIdx.x >> 6), OBS / 64, A.groupStride);
     fetchOrbit(A, L, group);
     const double accRel = A.accuracy;
     unsigned int it = 0;
