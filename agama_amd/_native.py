@@ -73,6 +73,7 @@ def _load():
         "agx_orbit_record_num_series": (ctypes.c_long, [_vp]),
         "agx_orbit_record_series_info": (ctypes.c_int, [_vp, _vp, _vp]),
         "agx_orbit_record_node_times": (ctypes.c_int, [_vp, ctypes.c_long, _vp]),
+        "agx_orbit_record_nodes": (ctypes.c_int, [_vp, ctypes.c_long, _vp]),
         "agx_orbit_record_interp": (ctypes.c_int, [_vp, ctypes.c_long, _vp, _vp, ctypes.c_int, _vp]),
         "agx_orbit_record_free": (None, [_vp]),
         "agx_set_record_rows": (ctypes.c_int, [ctypes.c_long]),
@@ -144,7 +145,7 @@ EXPORTED = ["agx_last_error", "agx_potential_create", "agx_potential_from_coefs"
             "agx_orbit_batch_device", "agx_orbit_batch_device_noalloc", "agx_orbit_record_batch", "agx_orbit_record_fetch",
             "agx_orbit_record_num_series", "agx_orbit_record_series_info", "agx_orbit_record_node_times",
             "agx_orbit_record_interp", "agx_orbit_record_free", "agx_set_record_rows", "agx_set_construction", "agx_set_surrogate_pilot", "agx_surrogate_last_pilot", "agx_target_density_sphharm", "agx_target_density_cylindrical", "agx_target_kinem_shell", "agx_target_density_classic", "agx_target_num_values",
-            "agx_target_destroy", "agx_orbit_target_batch", "agx_orbit_target_traj_batch", "agx_set_orbit_launch", "agx_set_fetch", "agx_set_orbit_groups", "agx_set_orbit_probe", "agx_get_orbit_schedule", "iniacceleration",
+            "agx_target_destroy", "agx_orbit_target_batch", "agx_orbit_target_traj_batch", "agx_set_orbit_launch", "agx_set_fetch", "agx_set_orbit_groups", "agx_set_orbit_probe", "agx_get_orbit_schedule", "agx_orbit_record_nodes", "iniacceleration",
             "inipotential", "potential_double", "potential_float",
             "agx_surrogate_train", "agx_surrogate_create", "agx_surrogate_destroy",
             "agx_surrogate_info", "agx_surrogate_loss_history", "agx_surrogate_eval_many",

@@ -1547,6 +1547,21 @@ int agx_orbit_record_node_times(agx_orbit_record* R, long series, double* out_t)
     } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
 }
 
+int agx_orbit_record_nodes(agx_orbit_record* R, long series, double* out)
+{
+    g_err.clear();
+    if(!R || !out) return fail(AGX_EINVAL, "NULL argument");
+    if(series < 0 || series >= R->numSeries()) return fail(AGX_EINVAL, "series index out of range");
+    try {
+        HIPCHECK(hipSetDevice(R->device));
+        buildSplines(*R);
+        const long k = R->hcount[series];
+        if(k) HIPCHECK(hipMemcpy(out, R->nodes->p + R->nodeOff[series] * AGX_NODE_DOUBLES,
+            (size_t)k * AGX_NODE_DOUBLES * 8, hipMemcpyDeviceToHost));
+        return AGX_OK;
+    } catch(std::exception& e) { return fail(AGX_EDEVICE, e.what()); }
+}
+
 int agx_orbit_record_interp(agx_orbit_record* R, long m, const long* series, const double* t, int mode,
     double* out)
 {

@@ -58,6 +58,14 @@ class _Record:
             check(lib.agx_orbit_record_node_times(self._h, int(series), ptr(out)), "orbit interpolation")
         return out
 
+    def nodes(self, series):
+        """the series' spline nodes [k, 10]: t, x, y, z, x', y', z', x'', y'', z''"""
+        k = int(self.info()[0][series])
+        out = np.empty((k, 10))
+        if k:
+            check(lib.agx_orbit_record_nodes(self._h, int(series), ptr(out)), "orbit interpolation")
+        return out
+
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
         if h and lib is not None:   # not during interpreter shutdown
@@ -74,21 +82,202 @@ def _time_argument(t):
     return arr, False
 
 
+class QuinticPieces:
+    """A quintic spline given by its nodes t[k] (increasing), values f, first and second
+    derivatives d, s2 (the QuinticSpline of src/math_spline.cpp:1391-1429), held as one
+    polynomial in u = (t - t_i) / h_i per segment, and continued linearly beyond the end nodes.
+    Host-side operations of the reference's Spline objects on it (interface_python.cpp:6159-6330):
+    evaluation up to the third derivative, integrals with a monomial or another spline as weight,
+    extrema and roots."""
+
+    def __init__(self, t, f, d, s2):
+        self.t = np.ascontiguousarray(t, dtype=np.float64)
+        self.f = np.asarray(f, dtype=np.float64)
+        self.d = np.asarray(d, dtype=np.float64)
+        h = np.diff(self.t)
+        df = np.diff(self.f)
+        d0, d1 = h * self.d[:-1], h * self.d[1:]
+        s0, s1 = h * h * np.asarray(s2)[:-1], h * h * np.asarray(s2)[1:]
+        # the quintic with value, derivative and second derivative given at u = 0 and u = 1
+        c = np.empty((len(h), 6))
+        c[:, 0] = self.f[:-1]
+        c[:, 1] = d0
+        c[:, 2] = 0.5 * s0
+        c[:, 3] = 10 * df - 6 * d0 - 4 * d1 - 1.5 * s0 + 0.5 * s1
+        c[:, 4] = -15 * df + 8 * d0 + 7 * d1 + 1.5 * s0 - s1
+        c[:, 5] = 6 * df - 3 * d0 - 3 * d1 - 0.5 * s0 + 0.5 * s1
+        self.c, self.h = c, h
+
+    def _segment(self, x):
+        # binSearch's convention: the last node belongs to the last segment, beyond it is outside
+        return np.clip(np.searchsorted(self.t, x, side="right") - 1, 0, len(self.h) - 1)
+
+    def __call__(self, x, der=0):
+        x = np.asarray(x, dtype=np.float64)
+        out = np.full(x.shape, np.nan)
+        if len(self.t) == 0:
+            return out
+        if len(self.t) == 1:
+            inside = np.zeros(x.shape, bool)
+        else:
+            inside = (x >= self.t[0]) & (x <= self.t[-1])
+        if inside.any():
+            xi = x[inside]
+            i = self._segment(xi)
+            u = (xi - self.t[i]) / self.h[i]
+            c = self.c[i]
+            # Horner for the der-th derivative in u (d^der u^k / du^der = k!/(k-der)! u^(k-der)),
+            # then d/dt = (d/du) / h
+            v = np.zeros(len(xi))
+            for k in range(5, der - 1, -1):
+                v = v * u + c[:, k] * float(np.prod(np.arange(k - der + 1, k + 1)))
+            out[inside] = v / self.h[i] ** der
+        below, above = x < self.t[0], x > self.t[-1]
+        for m, j in ((below, 0), (above, -1)):
+            if m.any():
+                dj = self.d[j]
+                out[m] = (self.f[j] + (0 if dj == 0 else dj * (x[m] - self.t[j])) if der == 0 else
+                          dj if der == 1 else 0.)
+        return out
+
+    def integrate(self, x1, x2, n=0):
+        """integral of the spline times x^n, or times another QuinticPieces, over x1..x2; the
+        spline is zero outside its grid (integratePiecewise, src/math_spline.cpp:741-843)"""
+        sign = 1.
+        if x1 > x2:
+            x1, x2, sign = x2, x1, -1.
+        if len(self.t) < 2:
+            return 0.
+        other = n if isinstance(n, QuinticPieces) else None
+        lo, hi = max(x1, self.t[0]), min(x2, self.t[-1])
+        if other is not None:
+            if len(other.t) < 2:
+                return 0.
+            lo, hi = max(lo, other.t[0]), min(hi, other.t[-1])
+            order = 6             # a product of two quintics: degree 10
+        else:
+            order = (5 + int(n)) // 2 + 1
+        if not lo < hi:
+            return 0.
+        grid = self.t if other is None else np.union1d(self.t, other.t)
+        edges = np.concatenate([[lo], grid[(grid > lo) & (grid < hi)], [hi]])
+        gx, gw = np.polynomial.legendre.leggauss(order)
+        a, b = edges[:-1, None], edges[1:, None]
+        xs = (0.5 * (b - a) * (gx + 1) + a).ravel()
+        w = (0.5 * (b - a) * gw).ravel()
+        vals = self(xs) * (other(xs) if other is not None else xs ** int(n))
+        return sign * float(np.sum(vals * w))
+
+    def extrema(self, x1=np.nan, x2=np.nan):
+        """points of local minimum or maximum on x1..x2 (default: the grid) in increasing order,
+        with both ends of the interval (findRoots<true>, src/math_spline.cpp:915-975)"""
+        if len(self.t) == 0:
+            return np.empty(0)
+        if x1 > x2:
+            x1, x2 = x2, x1
+        x1 = self.t[0] if not x1 >= self.t[0] else x1
+        x2 = self.t[-1] if not x2 <= self.t[-1] else x2
+        res, prev, sign = [x1], float(self(np.array([x1]))[0]), 0.
+        if x1 == x2 or len(self.t) < 2:
+            return np.array(res)
+        i1, i2 = int(self._segment(np.array([x1]))[0]), int(self._segment(np.array([x2]))[0])
+        for i in range(i1, i2 + 1):
+            t1, t2 = max(x1, self.t[i]), min(x2, self.t[i + 1])
+            c = self.c[i]
+            # zeros of dp/du = c1 + 2 c2 u + 3 c3 u^2 + 4 c4 u^3 + 5 c5 u^4
+            dp = np.array([5 * c[5], 4 * c[4], 3 * c[3], 2 * c[2], c[1]])
+            nz = np.flatnonzero(dp)
+            cand = []
+            if len(nz) and nz[0] < 4:
+                r = np.roots(dp[nz[0]:])
+                cand = sorted(self.t[i] + self.h[i] * r.real[np.abs(r.imag) <= 1e-12 * np.maximum(1, np.abs(r.real))])
+            if i == i2:
+                cand.append(t2)
+            for xr in cand:
+                if t1 <= xr <= t2 and xr > res[-1] + 10 * np.finfo(float).eps * max(abs(xr), abs(res[-1])):
+                    v = float(self(np.array([xr]))[0])
+                    if (sign > 0 and v > prev) or (sign < 0 and v < prev):
+                        res[-1], prev = xr, v      # the trend continues: an inflection, not an extremum
+                    elif v != prev:
+                        sign = 1. if v > prev else -1.
+                        res.append(xr)
+                        prev = v
+        return np.array(res)
+
+    def roots(self, y=0., x1=np.nan, x2=np.nan):
+        """all points of x1..x2 where the spline equals y, in increasing order: one root on each
+        interval between consecutive extrema where the sign of spline - y changes
+        (QuinticSpline::roots, src/math_spline.cpp:1446-1461)"""
+        from scipy.optimize import brentq
+        ex = self.extrema(x1, x2)
+        res = []
+        for a, b in zip(ex[:-1], ex[1:]):
+            fa, fb = float(self(np.array([a]))[0]) - y, float(self(np.array([b]))[0]) - y
+            if fa == 0:
+                r = a
+            elif fb == 0:
+                r = b
+            elif fa * fb < 0:
+                r = brentq(lambda q: float(self(np.array([q]))[0]) - y, a, b, xtol=1e-15 * max(abs(a), abs(b), 1.),
+                           rtol=4 * np.finfo(float).eps)
+            else:
+                continue
+            if not res or r > res[-1] + 10 * np.finfo(float).eps * max(abs(r), abs(res[-1])):
+                res.append(r)
+        return np.array(res)
+
+
 class OrbitSpline:
     """one coordinate (x, y or z, inertial frame) of an interpolated orbit as a quintic spline
     through the recorded positions with the velocities as derivatives: spline(t, der=0) with
-    der = 0, 1, 2; continued linearly beyond the end nodes (QuinticSpline::evalDeriv,
-    src/math_spline.cpp:1405-1429)"""
+    der = 0..3, ext (a value for points outside the recorded span instead of the linear
+    continuation beyond the end nodes), and the reference's Spline methods integrate, roots and
+    extrema (QuinticSpline, src/math_spline.cpp:1391-1463; Spline_value / Spline_integrate /
+    Spline_roots / Spline_extrema, src/interface_python.cpp:6159-6330).  der 0-2 are evaluated on
+    the device, the third derivative and the methods on the host from the spline's nodes."""
 
     def __init__(self, rec, series, comp):
         self._rec, self._series, self._comp = rec, series, comp
+        self._pieces = None
 
-    def __call__(self, x, der=0):
-        if der not in (0, 1, 2):
-            raise ValueError("der must be 0, 1 or 2")
+    def _host(self):
+        if self._pieces is None:
+            nd = self._rec.nodes(self._series)
+            k = self._comp
+            self._pieces = QuinticPieces(nd[:, 0], nd[:, 1 + k], nd[:, 4 + k], nd[:, 7 + k])
+        return self._pieces
+
+    def __call__(self, x, der=0, ext=None):
+        if der not in (0, 1, 2, 3):
+            raise ValueError("Can only compute derivatives up to 3rd")
         t, scalar = _time_argument(x)
-        out = self._rec.interp(self._series, t, 1)[:, der * 3 + self._comp]
+        if der == 3:
+            out = self._host()(t, 3)
+        else:
+            out = self._rec.interp(self._series, t, 1)[:, der * 3 + self._comp]
+        if ext is not None:
+            nodes = self._host().t
+            if len(nodes):
+                out = np.where((t < nodes[0]) | (t > nodes[-1]), float(ext), out)
         return float(out[0]) if scalar else out
+
+    def integrate(self, x1, x2, n=0):
+        """integral over x1..x2 of the spline times x^n (n a non-negative integer) or times
+        another spline of an Orbit; the spline is zero outside its grid"""
+        if isinstance(n, OrbitSpline):
+            return self._host().integrate(float(x1), float(x2), n._host())
+        if isinstance(n, (int, np.integer)) and 0 <= n < 0x7fff:
+            return self._host().integrate(float(x1), float(x2), int(n))
+        raise RuntimeError("Argument 'n', if present, should be a nonnegative integer or another "
+                           "instance of Spline")
+
+    def roots(self, y=0., x1=np.nan, x2=np.nan):
+        """all points on x1..x2 (default: the grid) where the spline equals y, in increasing order"""
+        return self._host().roots(float(y), float(x1), float(x2))
+
+    def extrema(self, x1=np.nan, x2=np.nan):
+        """all local minima and maxima on x1..x2 (default: the grid) with both ends, increasing"""
+        return self._host().extrema(float(x1), float(x2))
 
     def __len__(self):
         return int(self._rec.info()[0][self._series])

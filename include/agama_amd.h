@@ -246,6 +246,11 @@ long agx_orbit_record_num_series(const agx_orbit_record* rec);
 int agx_set_record_rows(long rows);
 int agx_orbit_record_series_info(agx_orbit_record* rec, long* out_nodes, int* out_reversed);
 int agx_orbit_record_node_times(agx_orbit_record* rec, long series, double* out_t);
+/* The spline nodes of one series (no counterpart in the reference's C interface; the data of
+   its QuinticSpline x(t), y(t), z(t), src/math_spline.cpp:1391-1429): out[nodes*10], per node
+   t, x, y, z, dx/dt, dy/dt, dz/dt, d2x/dt2, d2y/dt2, d2z/dt2 in the inertial frame, in order of
+   increasing t (nodes from agx_orbit_record_series_info). */
+int agx_orbit_record_nodes(agx_orbit_record* rec, long series, double* out);
 /* Evaluate m queries (series[q], t[q]) on the device.  mode 0 (Orbit.__call__,
    interface_python.cpp:6643-6714): out[m*6] position and velocity rotated back into the
    rotating frame at angle Omega*t, NaN outside the recorded time span; mode 1 (the x, y, z

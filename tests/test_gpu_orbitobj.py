@@ -204,3 +204,62 @@ def test_orbit_objects_backward_zero_length_der(setup):
                                  separateTime=True)
     ly0 = agama_amd.orbit(potential=gpu, ic=ic, time=T, lyapunov=True)
     assert np.array_equal(ly, ly0, equal_nan=True)
+
+
+def test_orbit_spline_methods_match_oracle(setup):
+    """the reference's Spline methods on Orbit.x/y/z (third derivative, ext, integrate with x^n or
+    another spline, roots, extrema) against the oracle's quintic spline through the same rows"""
+    import agama_amd
+    from oracle.pyoracle import orbit_interp
+    gpu, orc = setup
+    ic, T = ics(setup, 6, seed=45, periods=3)
+    print("case: Orbit spline methods, 6 orbits x 3 periods", flush=True)
+    orbs = agama_amd.orbit(potential=gpu, ic=ic, time=T, dtype=object)
+    t, x = agama_amd.orbit(potential=gpu, ic=ic, time=T, trajsize=0, dtype="float64", separateTime=True)
+    gx, gw = np.polynomial.legendre.leggauss(6)
+    for i in range(6):
+        o, ti = orbs[i], t[i]
+        a, b = ti[:-1, None], ti[1:, None]
+        tg = (0.5 * (b - a) * (gx + 1) + a).ravel()            # 6 Gauss points per segment
+        wg = (0.5 * (b - a) * gw).ravel()
+        spl = orbit_interp(ti, x[i], 0.0, tg)[1]               # oracle x, y, z and derivatives
+        scale = np.max(np.abs(x[i][:, :3]))
+        for k, s in enumerate((o.x, o.y, o.z)):
+            # host evaluation from the nodes equals the oracle's spline
+            for der in range(3):
+                ref = spl[:, der * 3 + k]
+                assert np.max(np.abs(s._host()(tg, der) - ref)) <= 1e-10 * max(np.max(np.abs(ref)), 1e-300)
+            # the third derivative: central differences of the oracle's second derivative inside
+            # the segments
+            mid = 0.5 * (ti[:-1] + ti[1:])
+            dl = 1e-3 * np.diff(ti)
+            d2p = orbit_interp(ti, x[i], 0.0, mid + dl)[1][:, 6 + k]
+            d2m = orbit_interp(ti, x[i], 0.0, mid - dl)[1][:, 6 + k]
+            fd = (d2p - d2m) / (2 * dl)
+            d3 = s(mid, der=3)
+            assert np.max(np.abs(d3 - fd)) <= 1e-4 * np.max(np.abs(d3))
+            # integrals: with x^n, and with another coordinate's spline (exact Gauss rules)
+            for n in (0, 2):
+                assert s.integrate(ti[0], ti[-1], n) == pytest.approx(np.sum(wg * spl[:, k] * tg ** n), rel=1e-12)
+            other = (k + 1) % 3
+            assert s.integrate(ti[0], ti[-1], (o.x, o.y, o.z)[other]) == \
+                pytest.approx(np.sum(wg * spl[:, k] * spl[:, other]), rel=1e-11, abs=1e-12 * scale ** 2 * (ti[-1] - ti[0]))
+            assert s.integrate(ti[-1], ti[0]) == -s.integrate(ti[0], ti[-1])
+            # extrema: both ends and points of zero derivative; roots: the spline takes the value
+            ex = s.extrema()
+            assert ex[0] == ti[0] and ex[-1] == ti[-1]
+            vscale = np.max(np.abs(x[i][:, 3:]))
+            if len(ex) > 2:
+                dv = orbit_interp(ti, x[i], 0.0, ex[1:-1])[1][:, 3 + k]
+                assert np.max(np.abs(dv)) <= 1e-8 * vscale
+            y0 = float(np.mean(x[i][:, k]))
+            r = s.roots(y0)
+            dense = np.linspace(ti[0], ti[-1], 20 * len(ti))
+            vals = orbit_interp(ti, x[i], 0.0, dense)[1][:, k] - y0
+            assert len(r) == np.count_nonzero(np.sign(vals[1:]) != np.sign(vals[:-1]))
+            if len(r):
+                assert np.max(np.abs(orbit_interp(ti, x[i], 0.0, r)[1][:, k] - y0)) <= 1e-10 * scale
+            # ext: a value outside the recorded span
+            assert s(ti[-1] + 1.0, ext=0.0) == 0.0 and s(ti[0], ext=0.0) == s(ti[0])
+    with pytest.raises(ValueError):
+        orbs[0].x(1.0, der=4)
