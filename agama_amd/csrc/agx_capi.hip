@@ -24,7 +24,7 @@ hipError_t agxLaunchEval(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, in
 hipError_t agxLaunchOrbit(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
     int waves, const AgxOrbitArgs& A, int numBlocks, int blockSize, hipStream_t stream, AgxOrbitArgs* dArgs);
 int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves, bool nfw, bool targets = false);
-hipError_t agxLaunchOrbitRKN(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision,
+hipError_t agxLaunchOrbitRKN(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
     const AgxOrbitArgs& A, int numBlocks, hipStream_t stream);
 int agxOrbitKernelRKNBlocksPerCU(int precision, bool nfw);
 int agxOrbitVarKernelBlocksPerCU(bool nfw);
@@ -323,7 +323,7 @@ int orbitGeometry(long n, int precision, bool nfw, int& blocks, int& bs, bool rk
 hipError_t launchOrbitMethod(const agx_potential* pot, int precision, int flags, const AgxOrbitArgs& A,
     int blocks, int bs, hipStream_t st, AgxOrbitArgs* dArgs = nullptr)
 {
-    if(flags & AGX_METHOD_DPRKN8) return agxLaunchOrbitRKN(pot->dev, pot->ddev, precision, A, blocks, st);
+    if(flags & AGX_METHOD_DPRKN8) return agxLaunchOrbitRKN(pot->dev, pot->ddev, precision, g_fetchOrbit, A, blocks, st);
     return agxLaunchOrbit(pot->dev, pot->ddev, precision, g_fetchOrbit, g_orbitWaves, A, blocks, bs, st,
         dArgs ? dArgs : pot->dargs);
 }

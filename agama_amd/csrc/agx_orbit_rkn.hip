@@ -351,12 +351,22 @@ __device__ __forceinline__ void advanceRKN(const AgxOrbitArgs& A, LaneRKN<STRIDE
 namespace {
 
 constexpr int RBS = 256;
+// LDS of the per-cell variant: the power-law arrays (10 (legLmax+1)^2 + legLmax + 1 doubles,
+// legLmax <= 16) and the tau bucket table (at most 4096 ints); the grids must fit AGX_LDS_GRID
+constexpr int RKN_LDS_PL = 2944;
+constexpr int RKN_LDS_TAULUT = 4096;
+typedef const double AGX_LDS* RknLdsGrid;
 
 template<int MODE, bool NFW>
 __device__ __forceinline__ void rhsRKN(const AgxMultipoleDev& P, double x, double y, double z, double& pot,
-    double& gx, double& gy, double& gz, const double* gR, const double* gT)
+    double& gx, double& gy, double& gz, const double* gR, const double* gT, RknLdsGrid lR, RknLdsGrid lT,
+    const PLTabs<RknLdsGrid>& plt)
 {
-    if constexpr(MODE == AGX_FETCH_VEC2) {
+    if constexpr(MODE == AGX_FETCH_CELL) {   // per-cell tables, grids and power-law arrays in LDS
+        FetchCell f(P.cells, P.cellBytes);
+        if(P.interp1d) multipoleCar<FetchCell, RknLdsGrid, NFW>(P, f, x, y, z, pot, gx, gy, gz, lR, lT);
+        else multipoleCarCell<FetchCell, RknLdsGrid, NFW>(P, f, x, y, z, pot, gx, gy, gz, lR, lT, plt);
+    } else if constexpr(MODE == AGX_FETCH_VEC2) {
         FetchVec2 f{ gmem(reinterpret_cast<const vd2*>(P.nodes)) };
         multipoleCar<FetchVec2, const double*, NFW, true>(P, f, x, y, z, pot, gx, gy, gz, gR, gT);
     } else {
@@ -365,7 +375,8 @@ __device__ __forceinline__ void rhsRKN(const AgxMultipoleDev& P, double x, doubl
     }
 }
 
-// MODE: AGX_FETCH_VEC2 (FP64 tables) or -1 (FP32 form); TGT: the variant feeding target plug-ins
+// MODE: AGX_FETCH_CELL (per-cell FP64 tables), AGX_FETCH_VEC2 (FP64 node records) or -1 (FP32
+// form); TGT: the variant feeding target plug-ins
 template<int MODE, bool NFW, bool TGT = false>
 __global__ __launch_bounds__(RBS) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void orbitKernelRKN(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
@@ -373,9 +384,15 @@ void orbitKernelRKN(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
     typedef LaneRKN<RBS> Lane;
     __shared__ double lds[Lane::LDS_FIELDS * RBS];
     __shared__ double gridLds[AGX_LDS_GRID];
+    constexpr bool CELLS = MODE == AGX_FETCH_CELL;
+    __shared__ double plLds[CELLS ? RKN_LDS_PL : 1];
+    __shared__ int lutLds[CELLS ? RKN_LDS_TAULUT : 1];
     const AgxMultipoleDev& P = *Pp;
     const double *gR, *gT;
-    stageGrids(P, gridLds, gR, gT);
+    PLTabs<RknLdsGrid> plt{};
+    if constexpr(CELLS) { if(!P.interp1d) plt = plStage(P, (double AGX_LDS*)plLds, (int AGX_LDS*)lutLds); }
+    stageGrids(P, gridLds, gR, gT);   // (synchronises the block)
+    const RknLdsGrid lR = (RknLdsGrid)gridLds, lT = lR + P.nr;   // the per-cell variant: grids in LDS
     Lane L;
     L.attach(lds + threadIdx.x);
     fetchOrbit(A, L);
@@ -390,7 +407,8 @@ void orbitKernelRKN(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
         double ca = 1, sa = 0;
         if(A.Omega != 0 && active) sincosA(A.Omega * (L.tb + L.toff), sa, ca);
         double pot, gx, gy, gz;
-        rhsRKN<MODE, NFW>(P, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa, L.y[2], pot, gx, gy, gz, gR, gT);
+        rhsRKN<MODE, NFW>(P, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa, L.y[2], pot, gx, gy, gz, gR, gT,
+            lR, lT, plt);
         if(!run) continue;
         double a[3];
         a[0] = -gx * ca + gy * sa;
@@ -411,17 +429,22 @@ int agxOrbitKernelRKNBlocksPerCU(int precision, bool nfw)
         e = nfw ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernelRKN<-1, true>, RBS, 0)
                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernelRKN<-1, false>, RBS, 0);
     else
-        e = nfw ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernelRKN<AGX_FETCH_VEC2, true>, RBS, 0)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernelRKN<AGX_FETCH_VEC2, false>, RBS, 0);
+        e = nfw ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernelRKN<AGX_FETCH_CELL, true>, RBS, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, orbitKernelRKN<AGX_FETCH_CELL, false>, RBS, 0);
     return e == hipSuccess && nb > 0 ? nb : 1;
 }
 
-hipError_t agxLaunchOrbitRKN(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision,
+// fetch: the orbit fetch setting (agx_set_fetch); the per-cell variant when it is AGX_FETCH_CELL
+// and the grids, power-law arrays and bucket table fit its LDS, else the node records
+hipError_t agxLaunchOrbitRKN(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
     const AgxOrbitArgs& A, int numBlocks, hipStream_t stream)
 {
     if(A.n <= 0) return hipSuccess;
     const bool nfw = P.nfwMass != 0;
     const dim3 g(numBlocks), b(RBS);
+    const bool cells = fetch == AGX_FETCH_CELL && P.nr + P.nt <= AGX_LDS_GRID &&
+        (P.interp1d || (P.cells && P.nTauLut <= RKN_LDS_TAULUT &&
+                        (6 + AGX_LEG_STRIDE) * (P.legLmax + 1) * (P.legLmax + 1) + P.legLmax + 1 <= RKN_LDS_PL));
     if(A.targets) {   // target plug-ins: FP64 tables only
         if(precision == AGX_FP32_TABLES) return hipErrorInvalidValue;
         if(nfw) hipLaunchKernelGGL((orbitKernelRKN<AGX_FETCH_VEC2, true, true>), g, b, 0, stream, dP, A);
@@ -431,6 +454,9 @@ hipError_t agxLaunchOrbitRKN(const AgxMultipoleDev& P, const AgxMultipoleDev* dP
     if(precision == AGX_FP32_TABLES) {
         if(nfw) hipLaunchKernelGGL((orbitKernelRKN<-1, true>), g, b, 0, stream, dP, A);
         else hipLaunchKernelGGL((orbitKernelRKN<-1, false>), g, b, 0, stream, dP, A);
+    } else if(cells) {
+        if(nfw) hipLaunchKernelGGL((orbitKernelRKN<AGX_FETCH_CELL, true>), g, b, 0, stream, dP, A);
+        else hipLaunchKernelGGL((orbitKernelRKN<AGX_FETCH_CELL, false>), g, b, 0, stream, dP, A);
     } else {
         if(nfw) hipLaunchKernelGGL((orbitKernelRKN<AGX_FETCH_VEC2, true>), g, b, 0, stream, dP, A);
         else hipLaunchKernelGGL((orbitKernelRKN<AGX_FETCH_VEC2, false>), g, b, 0, stream, dP, A);
