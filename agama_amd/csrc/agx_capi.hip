@@ -27,6 +27,7 @@ int agxOrbitKernelBlocksPerCU(int precision, int fetch, int waves, bool nfw, boo
 hipError_t agxLaunchOrbitRKN(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
     const AgxOrbitArgs& A, int numBlocks, hipStream_t stream);
 int agxOrbitKernelRKNBlocksPerCU(int precision, bool nfw);
+bool agxOrbitRKNCellsFit(const AgxMultipoleDev& P);
 int agxOrbitVarKernelBlocksPerCU(bool nfw);
 size_t agxOrbitVarStateDoubles(int numBlocks);
 hipError_t agxLaunchOrbitVar(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, const AgxOrbitArgs& A,
@@ -730,9 +731,12 @@ static void setupGroups(const agx_potential* pot, long n, int precision, int fla
         const long lanes = (long)blocks * bs;
         groupStride = n * 8 <= lanes ? 1 : n <= 2 * lanes ? 4 : n <= 4 * lanes ? 8 : probed ? PROBED_GROUP_STRIDE : 0;
     }
-    if(!(eligible && groupStride > 0 && n >= g_groupMinBatch && !(flags & AGX_METHOD_DPRKN8) &&
-         precision == AGX_PRECISION_FP64 && (g_fetchOrbit == AGX_FETCH_VEC2 || g_fetchOrbit == AGX_FETCH_CELL) &&
-         g_orbitWaves == 1 && pot->nfwMass == 0 && !pot->dev.interp1d))
+    // DOP853: node records or per-cell tables at one wavefront per SIMD; DPRKN8: per-cell tables
+    const bool rkn = (flags & AGX_METHOD_DPRKN8) != 0;
+    const bool kernelOk = rkn ? g_fetchOrbit == AGX_FETCH_CELL && agxOrbitRKNCellsFit(pot->dev) :
+        (g_fetchOrbit == AGX_FETCH_VEC2 || g_fetchOrbit == AGX_FETCH_CELL) && g_orbitWaves == 1;
+    if(!(eligible && groupStride > 0 && n >= g_groupMinBatch && precision == AGX_PRECISION_FP64 && kernelOk &&
+         pot->nfwMass == 0 && !pot->dev.interp1d))
         return;
     const long waves = (long)blocks * (bs / 64);
     const long stride = std::min<long>(groupStride, waves);   // wavefront 0 is always a group one
@@ -753,8 +757,9 @@ static void setupGroups(const agx_potential* pot, long n, int precision, int fla
     int dev = 0, cus = 0;
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const long maxBlocks = g_orbitBlocks > 0 ? g_orbitBlocks :
-        (long)cus * std::max(1, agxOrbitKernelBlocksPerCU(precision, g_fetchOrbit, g_orbitWaves, false, false));
+    const long maxBlocks = g_orbitBlocks > 0 ? g_orbitBlocks : (long)cus * std::max(1, rkn ?
+        agxOrbitKernelRKNBlocksPerCU(precision, false) :
+        agxOrbitKernelBlocksPerCU(precision, g_fetchOrbit, g_orbitWaves, false, false));
     const long gw = A.groupStride * ((A.groupOrbits + 7) / 8);
     const long lw = A.groupStride > 1 ? ((n - A.groupOrbits + 63) / 64) * A.groupStride / (A.groupStride - 1) : 0;
     const long needWaves = std::max(gw, lw);

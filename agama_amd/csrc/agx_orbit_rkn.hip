@@ -376,8 +376,11 @@ __device__ __forceinline__ void rhsRKN(const AgxMultipoleDev& P, double x, doubl
 }
 
 // MODE: AGX_FETCH_CELL (per-cell FP64 tables), AGX_FETCH_VEC2 (FP64 node records) or -1 (FP32
-// form); TGT: the variant feeding target plug-ins
-template<int MODE, bool NFW, bool TGT = false>
+// form); TGT: the variant feeding target plug-ins; GRP: the variant with group mode (per-cell
+// tables, no NFW halo): the group wavefronts (agxIsGroupWave) integrate the longest orbits with
+// AGX_GROUP lanes per orbit, every lane of a group holding the same state and evaluating a share
+// of the harmonics (multipoleCarCellGroup), as in the DOP853 kernel
+template<int MODE, bool NFW, bool TGT = false, bool GRP = false>
 __global__ __launch_bounds__(RBS) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void orbitKernelRKN(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
 {
@@ -395,7 +398,9 @@ void orbitKernelRKN(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
     const RknLdsGrid lR = (RknLdsGrid)gridLds, lT = lR + P.nr;   // the per-cell variant: grids in LDS
     Lane L;
     L.attach(lds + threadIdx.x);
-    fetchOrbit(A, L);
+    // wave-uniform: whether this wavefront works in group mode
+    const bool group = GRP && agxIsGroupWave((int)blockIdx.x, (int)(threadIdx.x >> 6), RBS / 64, A.groupStride);
+    fetchOrbit(A, L, group);
     const double accRel = 10 * pow(A.accuracy, 0.9);
     unsigned int it = 0;
     while(true) {
@@ -407,15 +412,23 @@ void orbitKernelRKN(const AgxMultipoleDev* __restrict__ Pp, AgxOrbitArgs A)
         double ca = 1, sa = 0;
         if(A.Omega != 0 && active) sincosA(A.Omega * (L.tb + L.toff), sa, ca);
         double pot, gx, gy, gz;
-        rhsRKN<MODE, NFW>(P, L.y[0] * ca + L.y[1] * sa, L.y[1] * ca - L.y[0] * sa, L.y[2], pot, gx, gy, gz, gR, gT,
-            lR, lT, plt);
+        const double ex = L.y[0] * ca + L.y[1] * sa, ey = L.y[1] * ca - L.y[0] * sa, ez = L.y[2];
+        if constexpr(GRP) {
+            if(group) {
+                FetchCell fg(P.cells, P.cellBytes);
+                multipoleCarCellGroup<AGX_GROUP>(P, fg, ex, ey, ez, pot, gx, gy, gz, lR, lT, plt,
+                    (int)(threadIdx.x & (AGX_GROUP - 1)));
+            } else
+                rhsRKN<MODE, NFW>(P, ex, ey, ez, pot, gx, gy, gz, gR, gT, lR, lT, plt);
+        } else
+            rhsRKN<MODE, NFW>(P, ex, ey, ez, pot, gx, gy, gz, gR, gT, lR, lT, plt);
         if(!run) continue;
         double a[3];
         a[0] = -gx * ca + gy * sa;
         a[1] = -gy * ca - gx * sa;
         a[2] = -gz;
         advanceRKN<RBS, TGT>(A, L, a, pot, accRel);
-        if(L.phase == PH_IDLE) fetchOrbit(A, L);
+        if(L.phase == PH_IDLE) fetchOrbit(A, L, group);
     }
 }
 
@@ -434,17 +447,32 @@ int agxOrbitKernelRKNBlocksPerCU(int precision, bool nfw)
     return e == hipSuccess && nb > 0 ? nb : 1;
 }
 
+// whether the per-cell variant can take this expansion: its grids, power-law arrays and bucket
+// table fit the kernel's LDS
+bool agxOrbitRKNCellsFit(const AgxMultipoleDev& P)
+{
+    return P.nr + P.nt <= AGX_LDS_GRID &&
+        (P.interp1d || (P.cells && P.nTauLut <= RKN_LDS_TAULUT &&
+                        (6 + AGX_LEG_STRIDE) * (P.legLmax + 1) * (P.legLmax + 1) + P.legLmax + 1 <= RKN_LDS_PL));
+}
+
 // fetch: the orbit fetch setting (agx_set_fetch); the per-cell variant when it is AGX_FETCH_CELL
-// and the grids, power-law arrays and bucket table fit its LDS, else the node records
+// and the expansion fits its LDS, else the node records.  Group mode (A.groupStride > 0) needs
+// the per-cell variant, FP64, no NFW halo and no targets.
 hipError_t agxLaunchOrbitRKN(const AgxMultipoleDev& P, const AgxMultipoleDev* dP, int precision, int fetch,
     const AgxOrbitArgs& A, int numBlocks, hipStream_t stream)
 {
     if(A.n <= 0) return hipSuccess;
     const bool nfw = P.nfwMass != 0;
     const dim3 g(numBlocks), b(RBS);
-    const bool cells = fetch == AGX_FETCH_CELL && P.nr + P.nt <= AGX_LDS_GRID &&
-        (P.interp1d || (P.cells && P.nTauLut <= RKN_LDS_TAULUT &&
-                        (6 + AGX_LEG_STRIDE) * (P.legLmax + 1) * (P.legLmax + 1) + P.legLmax + 1 <= RKN_LDS_PL));
+    const bool cells = fetch == AGX_FETCH_CELL && agxOrbitRKNCellsFit(P);
+    if(A.groupStride > 0) {
+        if(!cells || precision == AGX_FP32_TABLES || nfw || P.interp1d || A.targets || A.poolCounter ||
+           !A.groupCounter || A.groupOrbits > A.n)
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL((orbitKernelRKN<AGX_FETCH_CELL, false, false, true>), g, b, 0, stream, dP, A);
+        return hipGetLastError();
+    }
     if(A.targets) {   // target plug-ins: FP64 tables only
         if(precision == AGX_FP32_TABLES) return hipErrorInvalidValue;
         if(nfw) hipLaunchKernelGGL((orbitKernelRKN<AGX_FETCH_VEC2, true, true>), g, b, 0, stream, dP, A);

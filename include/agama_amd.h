@@ -358,8 +358,8 @@ double agama_evalPotential(const agama_Potential* pot, const double pos[3], doub
    current settings of agx_set_orbit_launch / agx_set_fetch / agx_set_orbit_waves. */
 int agx_set_orbit_waves(int waves);
 /* Group mode of the DOP853 kernel (no counterpart in the reference: a scheduling knob of this
-   library).  In batches of at least min_batch orbits (FP64, node records or per-cell tables (orbit
-   fetch mode 1 or 3), 1 wave/SIMD, no NFW
+   library).  In batches of at least min_batch orbits (FP64; DOP853 on node records or per-cell
+   tables (orbit fetch mode 1 or 3) at 1 wave/SIMD, or DPRKN8 on per-cell tables; no NFW
    halo, no targets, deviation vectors or every-step recording), one wavefront in `stride`
    integrates the first `orbits` orbits of the processing order - the longest expected - with 8
    lanes per orbit, each lane evaluating a share of the azimuthal harmonics; orbits = -1: one per

@@ -145,3 +145,51 @@ def test_record_second_pass_with_group_defaults(setup, groups):
     rows = np.array([len(a) for a in t2])
     print(f"identical step counts with the oracle {np.mean(rows == ref['rows']):.3f}")
     assert np.mean(rows == ref["rows"]) >= 0.9
+
+
+def test_dprkn8_group_mode(setup):
+    """DPRKN8 with group mode (per-cell tables): every orbit in groups agrees with the oracle's
+    DPRKN8 at the one-lane kernel's bars and differs from the one-lane results only by rounding;
+    a mixed batch (one wavefront in 8 in group mode) completes with the oracle's step counts"""
+    import ctypes
+    import agama_amd
+    from agama_amd._native import lib
+    gpu, orc = setup
+    old = (ctypes.c_int(), ctypes.c_int())
+    lib.agx_get_orbit_config(None, ctypes.byref(old[0]), ctypes.byref(old[1]), None)
+    assert lib.agx_set_fetch(old[0].value, 3) == 0
+    try:
+        ic, T = ics(setup, 256, seed=29, periods=20)
+        print("case: DPRKN8, group mode for every orbit, 256 orbits x 20 periods", flush=True)
+        assert lib.agx_set_orbit_groups(1, 256, 0) == 0
+        info = {}
+        agama_amd.orbit(potential=gpu, ic=ic, time=T, accuracy=1e-8, method="dprkn8", info=info)
+        ref = orc.orbits(ic, T, trajsize=1, accuracy=1e-8, method="dprkn8")
+        assert np.all(info["status"] == 0)
+        err = np.linalg.norm(info["final"][:, :3] - ref["final"][:, :3], axis=1) / \
+            np.linalg.norm(ref["final"][:, :3], axis=1)
+        same = info["nsteps"] == ref["nsteps"]
+        print(f"position rel err median {np.median(err):.2e}; identical step counts {same.mean():.3f}")
+        assert np.median(err) <= 1e-8 and np.quantile(err[same], 0.9) <= 1e-7
+        assert np.max(info["maxde"]) < 1e-5
+        assert lib.agx_set_orbit_groups(0, -1, 0) == 0
+        one = {}
+        agama_amd.orbit(potential=gpu, ic=ic, time=T, accuracy=1e-8, method="dprkn8", info=one)
+        d = np.linalg.norm(info["final"][:, :3] - one["final"][:, :3], axis=1) / np.linalg.norm(one["final"][:, :3], axis=1)
+        print(f"group vs one-lane: median {np.median(d):.2e}")
+        assert not np.array_equal(info["final"], one["final"]) and np.median(d) <= 1e-8
+        ic2, T2 = ics(setup, 4096, seed=30, periods=3, uneven=True)
+        T2 = T2 / 10
+        print("case: DPRKN8, one wavefront in 8 in group mode, 4096 uneven orbits", flush=True)
+        assert lib.agx_set_orbit_groups(8, -1, 0) == 0
+        info2 = {}
+        agama_amd.orbit(potential=gpu, ic=ic2, time=T2, method="dprkn8", info=info2)
+        ref2 = orc.orbits(ic2, T2, method="dprkn8")
+        assert np.all(info2["status"] == 0) and np.all(info2["nsteps"] > 0)
+        err2 = np.linalg.norm(info2["final"][:, :3] - ref2["final"][:, :3], axis=1) / \
+            np.linalg.norm(ref2["final"][:, :3], axis=1)
+        print(f"steps identical {np.mean(info2['nsteps'] == ref2['nsteps']):.3f}, pos err median {np.median(err2):.2e}")
+        assert np.mean(info2["nsteps"] == ref2["nsteps"]) > 0.9 and np.median(err2) <= 1e-9
+    finally:
+        assert lib.agx_set_orbit_groups(-1, -1, 65536) == 0
+        assert lib.agx_set_fetch(old[0].value, old[1].value) == 0
