@@ -54,7 +54,7 @@ METHOD_FLAG = {"dop853": 0, "dprkn8": 0x200}
 # record fetch modes of the Multipole evaluation (agx_set_fetch)
 FETCH_MODES = {"scalar": 0, "vec2": 1, "coop": 2, "cell": 3}
 # launches timed for a leg that is reported beside the headline figure (not the headline)
-COMPARISON_STEPS = 2
+COMPARISON_STEPS = 4
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix), AMD published figure
 # HBM bytes per launch of the orbit path on this workload (the probe and the main persistent
 # kernel), from rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy
