@@ -193,3 +193,50 @@ def test_dprkn8_group_mode(setup):
     finally:
         assert lib.agx_set_orbit_groups(-1, -1, 65536) == 0
         assert lib.agx_set_fetch(old[0].value, old[1].value) == 0
+
+
+@pytest.mark.parametrize("method", ["dop853", "dprkn8"])
+def test_group_mode_in_the_power_law_regions(setup, method):
+    """group mode on orbits that stay beyond r_max (outer power-law extrapolation, split over the
+    group's lanes) or inside r_min (inner extrapolation): the same orbits as the oracle's"""
+    import ctypes
+    import agama_amd
+    from agama_amd._native import lib
+    gpu, orc = setup
+    rng = np.random.default_rng(33)
+    cases = []
+    for lo, hi in ((1500.0, 6000.0), (1e-4, 6e-4)):
+        n = 64
+        r = np.exp(rng.uniform(np.log(lo), np.log(hi), n))
+        u = rng.normal(size=(n, 3))
+        u /= np.linalg.norm(u, axis=1)[:, None]
+        pos = u * r[:, None]
+        grad = orc.eval(pos)[1]
+        vc = np.sqrt(np.abs(np.sum(grad * pos, axis=1)))       # sqrt(r |dPhi/dr|)
+        w = np.cross(u, rng.normal(size=(n, 3)))
+        w /= np.linalg.norm(w, axis=1)[:, None]
+        vel = w * (vc * rng.uniform(0.9, 1.0, n))[:, None]      # near-circular: stays in its region
+        cases.append((np.hstack([pos, vel]), 3 * 2 * np.pi * r / vc))
+    old = (ctypes.c_int(), ctypes.c_int())
+    lib.agx_get_orbit_config(None, ctypes.byref(old[0]), ctypes.byref(old[1]), None)
+    assert lib.agx_set_fetch(old[0].value, 3) == 0
+    try:
+        assert lib.agx_set_orbit_groups(1, -1, 0) == 0          # every orbit in groups
+        for (ic, T), region in zip(cases, ("beyond r_max", "inside r_min")):
+            print(f"case: {method}, group mode, 64 orbits {region}", flush=True)
+            info = {}
+            agama_amd.orbit(potential=gpu, ic=ic, time=T, accuracy=1e-8, method=method, info=info)
+            ref = orc.orbits(ic, T, trajsize=1, accuracy=1e-8, method=method)
+            assert np.all(info["status"] == 0)
+            rf = np.linalg.norm(ref["final"][:, :3], axis=1)
+            err = np.linalg.norm(info["final"][:, :3] - ref["final"][:, :3], axis=1) / rf
+            print(f"  final radii {rf.min():.3g}..{rf.max():.3g}; position rel err median {np.median(err):.2e}, "
+                  f"max {np.max(err):.2e}; identical step counts {np.mean(info['nsteps'] == ref['nsteps']):.3f}")
+            if region == "beyond r_max":
+                assert rf.min() > 1000
+            else:
+                assert rf.max() < 1e-3
+            assert np.median(err) <= 1e-9 and np.max(err) <= 1e-6
+    finally:
+        assert lib.agx_set_orbit_groups(-1, -1, 65536) == 0
+        assert lib.agx_set_fetch(old[0].value, old[1].value) == 0
