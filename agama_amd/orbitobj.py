@@ -168,6 +168,48 @@ class QuinticPieces:
         vals = self(xs) * (other(xs) if other is not None else xs ** int(n))
         return sign * float(np.sum(vals * w))
 
+    def convolve(self, x, kernel):
+        """the spline convolved with a Gaussian of width `kernel` (a number > 0) or with another
+        QuinticPieces, at the points x: the integral over the spline's grid of S(y) K(x - y) dy,
+        the spline being zero outside its grid (integratePiecewise<N, true>,
+        src/math_spline.cpp:733-805).  Gaussian: Gauss-Legendre rules of 16 points on pieces no
+        wider than sigma/2 within 12 sigma of x (the rest weighs below 1e-31); another spline:
+        6-point rules between the breakpoints of both, exact for the degree-10 products."""
+        x = np.atleast_1d(np.asarray(x, dtype=np.float64))
+        out = np.zeros(x.shape)
+        if len(self.t) < 2:
+            return out
+        lo, hi = self.t[0], self.t[-1]
+        if isinstance(kernel, QuinticPieces):
+            gx, gw = np.polynomial.legendre.leggauss(6)
+            for j, xj in enumerate(x):
+                if len(kernel.t) < 2:
+                    continue
+                a, b = max(lo, xj - kernel.t[-1]), min(hi, xj - kernel.t[0])
+                if not a < b:
+                    continue
+                br = np.union1d(self.t, xj - kernel.t)
+                e = np.concatenate([[a], br[(br > a) & (br < b)], [b]])
+                y = (0.5 * (e[1:, None] - e[:-1, None]) * (gx + 1) + e[:-1, None]).ravel()
+                w = (0.5 * (e[1:, None] - e[:-1, None]) * gw).ravel()
+                out[j] = np.sum(w * self(y) * kernel(xj - y))
+            return out
+        sigma = float(kernel)
+        if not sigma > 0:
+            raise ValueError("the Gaussian width must be positive")
+        gx, gw = np.polynomial.legendre.leggauss(16)
+        norm = 1 / (np.sqrt(2 * np.pi) * sigma)
+        for j, xj in enumerate(x):
+            a, b = max(lo, xj - 12 * sigma), min(hi, xj + 12 * sigma)
+            if not a < b:
+                continue
+            npieces = int(np.ceil((b - a) / (0.5 * sigma)))
+            br = np.union1d(np.linspace(a, b, npieces + 1), self.t[(self.t > a) & (self.t < b)])
+            y = (0.5 * (br[1:, None] - br[:-1, None]) * (gx + 1) + br[:-1, None]).ravel()
+            w = (0.5 * (br[1:, None] - br[:-1, None]) * gw).ravel()
+            out[j] = norm * np.sum(w * self(y) * np.exp(-0.5 * ((xj - y) / sigma) ** 2))
+        return out
+
     def extrema(self, x1=np.nan, x2=np.nan):
         """points of local minimum or maximum on x1..x2 (default: the grid) in increasing order,
         with both ends of the interval (findRoots<true>, src/math_spline.cpp:915-975)"""
@@ -231,10 +273,11 @@ class OrbitSpline:
     """one coordinate (x, y or z, inertial frame) of an interpolated orbit as a quintic spline
     through the recorded positions with the velocities as derivatives: spline(t, der=0) with
     der = 0..3, ext (a value for points outside the recorded span instead of the linear
-    continuation beyond the end nodes), and the reference's Spline methods integrate, roots and
-    extrema (QuinticSpline, src/math_spline.cpp:1391-1463; Spline_value / Spline_integrate /
+    continuation beyond the end nodes), conv (a Gaussian width or another spline to convolve
+    with), and the reference's Spline methods integrate, roots and extrema (QuinticSpline, src/math_spline.cpp:1391-1463; Spline_value / Spline_integrate /
     Spline_roots / Spline_extrema, src/interface_python.cpp:6159-6330).  der 0-2 are evaluated on
-    the device, the third derivative and the methods on the host from the spline's nodes."""
+    the device, the third derivative, convolutions and the methods on the host from the spline's
+    nodes."""
 
     def __init__(self, rec, series, comp):
         self._rec, self._series, self._comp = rec, series, comp
@@ -247,10 +290,15 @@ class OrbitSpline:
             self._pieces = QuinticPieces(nd[:, 0], nd[:, 1 + k], nd[:, 4 + k], nd[:, 7 + k])
         return self._pieces
 
-    def __call__(self, x, der=0, ext=None):
+    def __call__(self, x, der=0, ext=None, conv=None):
         if der not in (0, 1, 2, 3):
             raise ValueError("Can only compute derivatives up to 3rd")
+        if conv is not None and (der != 0 or ext is not None):
+            raise ValueError("`conv` cannot be used together with `der` or `ext`")
         t, scalar = _time_argument(x)
+        if conv is not None:
+            out = self._convolve(t, conv)
+            return float(out[0]) if scalar else out
         if der == 3:
             out = self._host()(t, 3)
         else:
@@ -260,6 +308,27 @@ class OrbitSpline:
             if len(nodes):
                 out = np.where((t < nodes[0]) | (t > nodes[-1]), float(ext), out)
         return float(out[0]) if scalar else out
+
+    def _convolve(self, t, conv):
+        # conv: another Orbit spline, or Gaussian widths (a number, or one per point); a width
+        # of zero gives the spline's value inside its grid and 0 outside (Spline_value,
+        # src/interface_python.cpp:6257-6270)
+        host = self._host()
+        if isinstance(conv, OrbitSpline):
+            return host.convolve(t, conv._host())
+        sig = np.asarray(conv, dtype=np.float64)
+        if sig.size != 1 and sig.shape != t.shape:
+            raise TypeError("Argument `sigma`, if provided, must be a single number or an array with the "
+                            "same shape as `x`")
+        sig = np.broadcast_to(sig.reshape(-1) if sig.size != 1 else sig.reshape(()), t.shape)
+        out = np.empty(t.shape)
+        for j in range(len(t)):
+            if sig[j] == 0:
+                inside = len(host.t) and host.t[0] <= t[j] <= host.t[-1]
+                out[j] = float(host(np.array([t[j]]))[0]) if inside else 0.
+            else:
+                out[j] = host.convolve(t[j:j + 1], float(sig[j]))[0]
+        return out
 
     def integrate(self, x1, x2, n=0):
         """integral over x1..x2 of the spline times x^n (n a non-negative integer) or times

@@ -261,5 +261,25 @@ def test_orbit_spline_methods_match_oracle(setup):
                 assert np.max(np.abs(orbit_interp(ti, x[i], 0.0, r)[1][:, k] - y0)) <= 1e-10 * scale
             # ext: a value outside the recorded span
             assert s(ti[-1] + 1.0, ext=0.0) == 0.0 and s(ti[0], ext=0.0) == s(ti[0])
+            # conv: a Gaussian of width sigma against a fine Gauss rule on the oracle's spline; a
+            # zero width gives the value inside the span and 0 outside; another spline as kernel
+            if k == 0:
+                sig = 0.02 * (ti[-1] - ti[0])
+                tq = np.array([ti[0] + 0.3 * (ti[-1] - ti[0]), ti[-1] + 2 * sig])
+                got = s(tq, conv=sig)
+                gx2, gw2 = np.polynomial.legendre.leggauss(64)
+                for tj, gj in zip(tq, got):
+                    a2, b2 = max(ti[0], tj - 12 * sig), min(ti[-1], tj + 12 * sig)
+                    e2 = np.linspace(a2, b2, 41)
+                    yy = (0.5 * np.diff(e2)[:, None] * (gx2 + 1) + e2[:-1, None]).ravel()
+                    ww = (0.5 * np.diff(e2)[:, None] * gw2).ravel()
+                    sv = orbit_interp(ti, x[i], 0.0, yy)[1][:, 0]
+                    ref = np.sum(ww * sv * np.exp(-0.5 * ((tj - yy) / sig) ** 2)) / (np.sqrt(2 * np.pi) * sig)
+                    assert gj == pytest.approx(ref, rel=1e-9, abs=1e-12 * scale)
+                z = s(np.array([ti[1], ti[-1] + 1.0]), conv=np.array([0.0, 0.0]))
+                assert z[0] == pytest.approx(s(ti[1]), rel=1e-13) and z[1] == 0.0
+                assert np.isfinite(s(ti[2], conv=o.y))
+                with pytest.raises(ValueError):
+                    s(ti[1], der=1, conv=sig)
     with pytest.raises(ValueError):
         orbs[0].x(1.0, der=4)

@@ -78,3 +78,26 @@ def test_stationary_inflection_is_not_an_extremum():
     q = spline(C, np.linspace(0, 5, 6))
     assert np.allclose(q.extrema(), [0.0, 5.0])
     assert np.allclose(q.roots(0.0), [2.0], atol=1e-5)         # a triple root: flat
+
+
+@pytest.mark.parametrize("sigma", [0.05, 0.4, 2.0])
+def test_gaussian_convolution(sigma):
+    from scipy.integrate import quad
+    q = spline()
+    xs = np.array([-1.0, 0.3, 2.5, 4.9, 5.6])
+    got = q.convolve(xs, sigma)
+    for x, g in zip(xs, got):
+        f = lambda y: P(y) * np.exp(-0.5 * ((x - y) / sigma) ** 2) / (np.sqrt(2 * np.pi) * sigma)  # noqa: E731
+        ref = quad(f, 0.0, 5.0, points=[x] if 0 < x < 5 else None, epsabs=1e-13, epsrel=1e-12, limit=200)[0]
+        assert g == pytest.approx(ref, rel=1e-10, abs=1e-13)
+
+
+def test_convolution_with_another_spline():
+    from scipy.integrate import quad
+    Q = np.polynomial.Polynomial([1.0, -0.5, 0.25])
+    tq = np.linspace(-1.0, 1.5, 6)
+    q, k = spline(), spline(Q, tq)
+    for x in (-0.5, 0.7, 3.0, 6.0, 7.0):
+        lo, hi = max(0.0, x - 1.5), min(5.0, x + 1.0)
+        ref = quad(lambda y: P(y) * Q(x - y), lo, hi, epsabs=1e-14, epsrel=1e-13)[0] if lo < hi else 0.0
+        assert q.convolve([x], k)[0] == pytest.approx(ref, rel=1e-11, abs=1e-13)
