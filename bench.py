@@ -46,10 +46,10 @@ METRIC = "orbit-steps/s at 1M orbits: MLP surrogate vs FP64 Multipole at Î”E/Eâ‰
 #   per potential + gradient + Hessian evaluation                                3,044
 FLOPS_PER_ATTEMPT = {"dop853": 27795, "dprkn8": 20508}
 # The GPU kernels execute less arithmetic for the same work (per-cell polynomial tables instead of
-# per-point Hermite weights): counted by hand from the kernel code, 1,531 per right-hand-side
-# evaluation and 978 (DOP853) / 620 (DPRKN8) per attempt besides the evaluations; reported beside
-# the roofline figure as executed_frac.
-GPU_FLOPS_PER_ATTEMPT = {"dop853": 12 * 1531 + 978, "dprkn8": 9 * 1531 + 620}
+# per-point Hermite weights): FP64 operations per attempt from the hardware counters
+# SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 on the 1M-orbit launch (an FMA counts 2; every lane of each
+# instruction, idle lanes included); reported beside the roofline figure as executed_frac.
+GPU_FLOPS_PER_ATTEMPT = {"dop853": 16290, "dprkn8": 12342}
 METHOD_FLAG = {"dop853": 0, "dprkn8": 0x200}
 # record fetch modes of the Multipole evaluation (agx_set_fetch)
 FETCH_MODES = {"scalar": 0, "vec2": 1, "coop": 2, "cell": 3}
